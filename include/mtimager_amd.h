@@ -1,0 +1,201 @@
+/*
+ * mtimager_amd.h -- C-ABI of the MI355X (gfx950) W-projection imaging path.
+ *
+ * Plain C types only: device pointers, element counts, strides, POD parameter
+ * structs and a hipStream_t passed as void*.  No C++ exceptions cross this
+ * boundary; every entry point returns an mti_status and the message of the
+ * last failure is available from mti_last_error().
+ *
+ * The caller owns every input and output device buffer (as the GAFW engine
+ * does for operator I/O, gafw/Implimentation/GPUImplimentation/
+ * CUDADeviceManager.cpp:1361-1530); scratch space is held by the context and
+ * grows on demand.  All work is enqueued on the stream given to each call;
+ * calls on one context must not overlap (one context per host thread, as the
+ * reference runs one submit thread per device, CUDADeviceManager.cpp:410-489).
+ *
+ * Reference citations are relative to code/mt-imager/lib/ unless noted
+ * ("gafw/" = code/gafw/Implimentation/).
+ */
+#ifndef MTIMAGER_AMD_H
+#define MTIMAGER_AMD_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    MTI_OK = 0,
+    MTI_ERR_INVALID = 1,     /* validation failure (ValidationException)     */
+    MTI_ERR_HIP = 2,         /* HIP runtime failure (CudaException)          */
+    MTI_ERR_UNSUPPORTED = 3, /* shape/feature not supported by this build    */
+    MTI_ERR_NOMEM = 4
+} mti_status;
+
+typedef enum { MTI_MODE_NORMAL = 0, MTI_MODE_WPROJ = 1 } mti_mode;
+typedef enum { MTI_TABLE_F32 = 0, MTI_TABLE_F16 = 1, MTI_TABLE_BF16 = 2 } mti_table_precision;
+typedef enum { MTI_WEIGHT_NATURAL = 0, MTI_WEIGHT_UNIFORM = 1, MTI_WEIGHT_BRIGGS = 2 } mti_weighting;
+typedef enum { MTI_POL_LINEAR = 0, MTI_POL_CIRCULAR = 1 } mti_poltype;
+
+/* Imaging parameters; the fields carry the mt-imager property keys
+ * (comp/ConfigurationManager.cpp:33-62, conf/mtimager.properties:1-78). */
+typedef struct {
+    int32_t nx;            /* image.nx                                       */
+    int32_t ny;            /* image.ny                                       */
+    double  cell_l;        /* image.lintervalsize, converted to radians      */
+    double  cell_m;        /* image.mintervalsize, converted to radians      */
+    int32_t mode;          /* mode: MTI_MODE_NORMAL | MTI_MODE_WPROJ         */
+    int32_t wplanes;       /* wproj.wplanes                                  */
+    int32_t sampling;      /* wproj.convFunction.sampling (wproj) or
+                              taper.sampling (normal)                        */
+    int32_t taper_support; /* taper.support (normal mode; casagridsf)        */
+} mti_image_params;
+
+/* Host-side description of a built convolution-function set. */
+typedef struct {
+    int32_t nplanes;
+    int32_t sampling;
+    int32_t T;             /* samples per cell axis: s (odd s) or s+1        */
+    int32_t trial_support; /* S_t                                            */
+    int32_t max_support;
+    int32_t precision;     /* mti_table_precision                            */
+    double  w_increment;   /* ConvolutionFunction.wsquareincrement (w_k = k^2 dw) */
+    double  divisor;       /* w=0 origin value the tables are divided by     */
+    int64_t total;         /* complex samples in the table                   */
+} mti_kernel_info;
+
+typedef struct mti_ctx mti_ctx;
+typedef struct mti_kernels mti_kernels;
+
+/* Visibility set in HBM.  Defaults describe MeasurementSet order
+ * (row, channel, correlation); strides let the reference's chunk layout
+ * (vis/StorageManager.h:147-150: uvw f32 [3][N], vis [N][2P], flags int [N][P],
+ * weights f32 [N][P], per-record f32 1/lambda) be passed without a copy. */
+typedef struct {
+    int64_t nrows;            /* rows in the arrays                           */
+    int32_t nchan;            /* channels in the arrays                       */
+    int32_t npol;             /* correlations: 1, 2 or 4                      */
+    /* uvw in metres */
+    const void *uvw;
+    int32_t uvw_f32;          /* 0: f64, 1: f32                               */
+    int64_t uvw_row_stride;   /* elements between rows (3 for [N][3])         */
+    int64_t uvw_comp_stride;  /* elements between u,v,w (1 for [N][3])        */
+    /* 1/lambda (= f/c, 1/m) */
+    const void *inv_lambda;
+    int32_t inv_lambda_f32;   /* 0: f64, 1: f32                               */
+    int32_t inv_lambda_per_row; /* 0: indexed [chan]; 1: indexed [row] (one channel) */
+    /* complex64 visibilities */
+    const float *vis;         /* interleaved re,im                            */
+    int64_t vis_row_stride;   /* complex elements between rows                */
+    int64_t vis_chan_stride;  /* complex elements between channels            */
+    /* flags (non-zero = flagged) */
+    const void *flags;
+    int32_t flags_i32;        /* 0: u8, 1: int32                              */
+    int64_t flags_row_stride; /* elements between rows                        */
+    int64_t flags_chan_stride;/* elements between channels                    */
+    /* weights f32, one per (row, correlation) */
+    const float *weight;
+    int64_t weight_row_stride;
+    /* optional row order (baseline-major from mti_sort_rows); NULL = 0..nrows-1 */
+    const int32_t *row_index;
+    int64_t nindex;           /* entries in row_index (ignored if NULL)       */
+} mti_vis;
+
+/* Gridding options for one call (WImager::calculate for one chunk,
+ * comp/WImager.cpp:271-290). */
+typedef struct {
+    int32_t chan0;            /* first channel gridded by this call           */
+    int32_t nchan;            /* channels gridded; one grid per channel       */
+    int32_t accumulate;       /* 0: grids/normalisers zeroed first
+                                 (initialize_output=true); 1: add in place    */
+    int32_t psf;              /* psf: every visibility replaced by 1+0i       */
+    int32_t weighting;        /* mti_weighting                                */
+    double  robust;           /* Briggs robustness R                          */
+} mti_grid_opts;
+
+typedef struct {
+    int64_t gridded_records;  /* Statistic:TotalGriddedRecords                */
+    int64_t gridded_points;   /* Statistic:TotalGriddedPoints (sum S^2 * P)   */
+    int64_t work_items;       /* tile work items processed                    */
+    int32_t tile;             /* interior tile edge (cells) chosen            */
+    int32_t path;             /* 0: LDS tile gridder, 1: global-atomic path   */
+} mti_grid_stats;
+
+/* ---- context ------------------------------------------------------------ */
+const char *mti_version(void);
+mti_status  mti_create(int device, mti_ctx **out);
+void        mti_destroy(mti_ctx *ctx);
+const char *mti_last_error(const mti_ctx *ctx);
+
+/* ---- a8-a14: convolution functions --------------------------------------
+ * Replaces WProjectionConvFunction::calculate (cfg/WProjectionConvFunction.cpp:186-291)
+ * with its operators casagridsf (op/CasaGridSF.cu:151-173), WProjPreFFTConvFunc
+ * (op/WProjPreFFTconvFunc.cu:34-105), multiFFT2D + ZeroToCentre2DShift +
+ * SingleMultiply (gafw/StandardOperators/MultiFFT2D.cpp:86-111,
+ * ZeroToCentre2DShift.cu:66-91, SingleMultiply.cu:69-100),
+ * ConvFunctionSupportFind (op/ConvFunctionSupportFind.cu:28-77),
+ * ConvFunctionReOrganize (op/ConvFunctionReOrganize.cu:30-71), ArraysToVector
+ * and SumConvolutionFunctions (op/SumConvolutionFunctions.cu:28-56); for
+ * mode=normal OnlyTaperConvFunction (cfg/OnlyTaperConvFunction.cpp:29-99).
+ * Blocks once (support read-back) like the reference's host loop. */
+mti_status mti_kernels_build(mti_ctx *ctx, const mti_image_params *p,
+                             int32_t precision, mti_kernels **out, void *stream);
+mti_status mti_kernels_info(const mti_kernels *k, mti_kernel_info *info);
+/* Host copies: support[nplanes], base[nplanes] (complex offsets). */
+mti_status mti_kernels_supports(const mti_kernels *k, int32_t *support, int64_t *base);
+/* Device pointers: table (complex, precision-dependent element size),
+ * sums f32 [nplanes][T*T]. */
+const void  *mti_kernels_table(const mti_kernels *k);
+const float *mti_kernels_sums(const mti_kernels *k);
+void         mti_kernels_free(mti_kernels *k);
+
+/* ---- a1: baseline-major row order ----------------------------------------
+ * Replaces SortIndex::sortFuncThread (vis/SortIndex.cpp:46-87): stable order by
+ * key Nant^2 - a1*Nant - a2 - 1, autocorrelations dropped.
+ * index_out must hold nrows entries; *kept_out receives the count (blocks). */
+mti_status mti_sort_rows(mti_ctx *ctx, const int32_t *ant1, const int32_t *ant2,
+                         int64_t nrows, int32_t nant, int32_t *index_out,
+                         int64_t *kept_out, void *stream);
+
+/* ---- a4: binning (PreProcessUVW) ------------------------------------------
+ * Replaces WProjPreProcessUVW (op/WProjPreProcessUVW.cu:111-324) for one
+ * channel: per record (row order of vis->row_index) writes
+ *   bins[r] = {to_grid, plane, support, corner_x, corner_y, off_x, off_y, conj}
+ * (int32 x8) computed in FP64.  Exposed for parity checks and for callers
+ * that keep the reference's operator split. */
+mti_status mti_preprocess(mti_ctx *ctx, const mti_kernels *k,
+                          const mti_image_params *p, const mti_vis *v,
+                          int32_t chan, int32_t *bins, void *stream);
+
+/* ---- a4-a7, a15, a21: grid channels ----------------------------------------
+ * Replaces the WImager DAG (comp/WImager.cpp:167-234: PreProcessUVW, the
+ * AccumulatedSum/CreateIndexAndReorder/CreateBlockDataIndex/
+ * NotSameSupportManipulate work plan, CreateCompressionPlan/
+ * CompressVisibilities, ConvolutionGridder op/ConvolutionGridder.cu:183-550,
+ * and the NormalizerValue Sum) for opts->nchan channels at once.
+ *   grids : complex64 [nchan][ny][nx][npol] (Grid, comp/WImager.cpp:231-232)
+ *   norm  : f64 [nchan][npol]               (NormalizerValue)
+ * Weighting (uniform/Briggs, not in the reference) is applied per channel. */
+mti_status mti_grid(mti_ctx *ctx, const mti_kernels *k, const mti_image_params *p,
+                    const mti_vis *v, const mti_grid_opts *opts, float *grids,
+                    double *norm, mti_grid_stats *stats, void *stream);
+
+/* ---- a17: 2-D inverse FFT (IFFT2D, gafw/StandardOperators/IFFT2D.cpp:86-111) --
+ * Unnormalised C2C inverse of nbatch planes complex64 [nbatch][ny][nx], in
+ * place.  nx, ny powers of two in [2, 8192]. */
+mti_status mti_fft2d_inverse(mti_ctx *ctx, float *data, int32_t nx, int32_t ny,
+                             int32_t nbatch, void *stream);
+
+/* ---- a16-a19: finalize (ImageFinalizer, comp/ImageFinalizer.cpp:31-44) -----
+ * PreIFFTUVCorrection + IFFT2D + ZeroToCentre2DShift + StokesConverter
+ * (normalise, Stokes, m-flip) + casagridsf taper correction + SincCorrector.
+ *   grids : complex64 [nchan][ny][nx][npol]  (consumed: used as scratch)
+ *   norm  : f64 [nchan][npol]
+ *   image : f32 [nchan][npol][ny][nx]  (Stokes I,Q,U,V / I,Q / I,V / I) */
+mti_status mti_finalize(mti_ctx *ctx, const mti_image_params *p, int32_t npol,
+                        int32_t poltype, int32_t nchan, float *grids,
+                        const double *norm, float *image, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

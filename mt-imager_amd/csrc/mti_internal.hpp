@@ -1,0 +1,159 @@
+// mti_internal.hpp -- shared definitions of the gfx950 imaging library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/mtimager_amd.h"
+
+namespace mti {
+
+// Scratch buffer owned by a context; grows on demand, never shrinks.
+struct Scratch {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct Status {
+    mti_status code = MTI_OK;
+    std::string msg;
+};
+
+}  // namespace mti
+
+struct mti_ctx {
+    int device = 0;
+    int num_cus = 0;
+    std::string last_error;
+    mti::Scratch scratch[12];
+    // FFT twiddle tables keyed by length (float2, exp(+2 pi i k / n)).
+    std::vector<std::pair<int, float2 *>> twiddles;
+};
+
+struct mti_kernels {
+    int device = 0;
+    mti_kernel_info info{};
+    std::vector<int32_t> support;    // host copy
+    std::vector<int64_t> base;       // host copy
+    int2 *d_convinfo = nullptr;      // {support, base} per plane (base < 2^31)
+    void *d_table = nullptr;         // complex samples, precision-dependent
+    float *d_sums = nullptr;         // [nplanes][T*T]
+};
+
+namespace mti {
+
+// Scratch slots.
+enum ScratchSlot {
+    SLOT_KEYS = 0,
+    SLOT_DESC = 1,
+    SLOT_SORTED = 2,
+    SLOT_BUCKETS = 3,
+    SLOT_ITEMS = 4,
+    SLOT_DENSITY = 5,
+    SLOT_MISC = 6,
+    SLOT_FFT = 7,
+    SLOT_RADIX = 8,
+    SLOT_NORM = 9,
+    SLOT_COUNT = 12
+};
+
+inline mti_status set_error(mti_ctx *ctx, mti_status code, const std::string &m)
+{
+    if (ctx) ctx->last_error = m;
+    return code;
+}
+
+#define MTI_HIP(ctx, call)                                                     \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess)                                                  \
+            return ::mti::set_error((ctx), MTI_ERR_HIP,                        \
+                                    std::string(#call ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+#define MTI_CHECK(ctx, cond, msg)                                              \
+    do {                                                                       \
+        if (!(cond)) return ::mti::set_error((ctx), MTI_ERR_INVALID, (msg));   \
+    } while (0)
+
+inline mti_status scratch_get(mti_ctx *ctx, int slot, size_t bytes, void **out)
+{
+    Scratch &s = ctx->scratch[slot];
+    if (s.bytes < bytes) {
+        if (s.ptr) {
+            hipError_t e = hipFree(s.ptr);
+            (void)e;
+            s.ptr = nullptr;
+            s.bytes = 0;
+        }
+        size_t want = bytes + bytes / 8 + 256;
+        hipError_t e = hipMalloc(&s.ptr, want);
+        if (e != hipSuccess)
+            return set_error(ctx, MTI_ERR_NOMEM,
+                             std::string("scratch hipMalloc failed: ") + hipGetErrorString(e));
+        s.bytes = want;
+    }
+    *out = s.ptr;
+    return MTI_OK;
+}
+
+inline hipStream_t to_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// dw of cfg/WProjectionConvFunction.cpp:105-112 in FP64.
+inline double w_increment(const mti_image_params *p)
+{
+    if (p->mode != MTI_MODE_WPROJ || p->wplanes <= 1) return 1.0;
+    double wmax = 0.25 / p->cell_l;
+    double wm1 = (double)(p->wplanes - 1);
+    return wmax / (wm1 * wm1);
+}
+
+mti_status validate_params(mti_ctx *ctx, const mti_image_params *p);
+mti_status validate_vis(mti_ctx *ctx, const mti_vis *v);
+
+// Exclusive scan of n uint32 counts (in place) on the stream; returns total
+// in *d_total (device).  Uses scratch SLOT_MISC.
+mti_status exclusive_scan_u32(mti_ctx *ctx, uint32_t *d_data, int64_t n,
+                              uint32_t *d_total, hipStream_t st);
+
+}  // namespace mti
+
+// Device helpers --------------------------------------------------------------
+namespace mti_dev {
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// psi(nu) of op/CasaGridSF.cu:41-85 (Schwab M=6, alpha=1), FP64.
+__device__ __forceinline__ double gridsf(double nu)
+{
+    const double P0[5] = {8.203343e-2, -3.644705e-1, 6.278660e-1, -5.335581e-1, 2.312756e-1};
+    const double P1[5] = {4.028559e-3, -3.697768e-2, 1.021332e-1, -1.201436e-1, 6.412774e-2};
+    const double Q0[3] = {1.0, 8.212018e-1, 2.078043e-1};
+    const double Q1[3] = {1.0, 9.599102e-1, 2.918724e-1};
+    bool hi = !(nu < 0.75);
+    const double *P = hi ? P1 : P0;
+    const double *Q = hi ? Q1 : Q0;
+    double d = nu * nu - (hi ? 1.0 : 0.5625);
+    double top = P[0], bot = Q[0], dk = 1.0;
+#pragma unroll
+    for (int k = 1; k < 5; k++) {
+        dk *= d;
+        top += P[k] * dk;
+    }
+    dk = 1.0;
+#pragma unroll
+    for (int k = 1; k < 3; k++) {
+        dk *= d;
+        bot += Q[k] * dk;
+    }
+    return bot != 0.0 ? top / bot : 0.0;
+}
+
+}  // namespace mti_dev

@@ -1,0 +1,12 @@
+// mti_sort.hpp -- stable radix sort shared by the row order and the tile bucketing.
+#pragma once
+#include "mti_internal.hpp"
+
+namespace mti {
+// Sorts (keys, vals) by the low key_bits bits of keys, stably.  Ping-pongs
+// between (keys, vals) and (keys_tmp, vals_tmp); *result_in_tmp tells which
+// pair holds the result.  Scratch slot SLOT_RADIX holds the digit counts.
+mti_status radix_sort_pairs(mti_ctx *ctx, uint32_t *keys, int32_t *vals, uint32_t *keys_tmp,
+                            int32_t *vals_tmp, int64_t n, int key_bits, hipStream_t st,
+                            bool *result_in_tmp);
+}  // namespace mti

@@ -1,0 +1,508 @@
+// mti_wkernel.hip -- convolution-function tables on the device (rows a8-a14).
+//
+// W-projection kernels (cfg/WProjectionConvFunction.cpp:186-291,
+// cfg/WPlaneConvFunctionCalculator.cpp:31-148):
+//   screen_k(c) = psi(|cy|/h) psi(|cx|/h) * exp(i pi 2 w_k (sqrt(1-l^2-m^2)-1))
+//     on the S_t x S_t centre of an (S_t s)^2 zero-padded plane
+//     (op/WProjPreFFTconvFunc.cu:34-105, op/CasaGridSF.cu:151-173);
+//   C_k(q) = FFT(screen_k) fftshifted, divided by the w=0 DC value
+//     (MultiFFT2D + ZeroToCentre2DShift + SingleMultiply);
+//   support from the outermost centre-row/column sample with |C|^2 > 1e-6
+//     (op/ConvFunctionSupportFind.cu:28-77), clamped to S_t-2;
+//   table = S*T x S*T window reordered to [oy][ox][sy][sx]
+//     (op/ConvFunctionReOrganize.cu:30-71), sums of Re per sub-kernel
+//     (op/SumConvolutionFunctions.cu:28-56).
+//
+// MI355X design: the FFT of a zero-padded plane followed by a crop is
+// evaluated as a separable DFT restricted to the window actually kept (the
+// same numbers; the padded FFT is never materialised), in FP64 with an LDS-free
+// incremental twiddle walk; the support search uses the identity "centre row
+// of the 2-D DFT = 1-D DFT of the column sums".  Only one host sync (support
+// read-back) per build.
+#include "mti_internal.hpp"
+
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+#include <algorithm>
+#include <cmath>
+
+namespace {
+
+struct ScreenGeom {
+    int St, h, s, L;
+    double linc, minc;
+};
+
+// twiddle table exp(-2 pi i m / L), FP64
+__global__ void k_twiddle(double2 *tw, int L)
+{
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= L) return;
+    double sn, cs;
+    sincospi(2.0 * (double)m / (double)L, &sn, &cs);
+    tw[m] = make_double2(cs, -sn);
+}
+
+// screens for planes [k0, k0+nb): scr[b][iy][ix], c = i - h
+__global__ void k_screen(double2 *scr, ScreenGeom g, double dw, int k0)
+{
+    const int b = blockIdx.z;
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+    if (ix >= g.St || iy >= g.St) return;
+    const int k = k0 + b;
+    const double w = (double)k * (double)k * dw;
+    const int cx = ix - g.h, cy = iy - g.h;
+    double taper = mti_dev::gridsf(fabs((double)cy) / (double)g.h) *
+                   mti_dev::gridsf(fabs((double)cx) / (double)g.h);
+    double msq = g.minc * (double)cy;
+    msq *= msq;
+    double lsq = g.linc * (double)cx;
+    lsq *= lsq;
+    double rsq = lsq + msq;
+    double2 v;
+    if (rsq < 1.0) {
+        double ph = 2.0 * w * (sqrt(1.0 - rsq) - 1.0);
+        double sn, cs;
+        sincospi(ph, &sn, &cs);
+        v = make_double2(taper * cs, taper * sn);
+    } else {
+        v = make_double2(taper, 0.0);
+    }
+    scr[((size_t)b * g.St + iy) * g.St + ix] = v;
+}
+
+__device__ __forceinline__ int mod_pos(long long a, int m)
+{
+    long long r = a % m;
+    return (int)(r < 0 ? r + m : r);
+}
+
+// One block per plane: support find on the centre row / column.
+constexpr int SUP_THREADS = 256;
+__global__ __launch_bounds__(SUP_THREADS) void k_support(const double2 *scr, ScreenGeom g,
+                                                         const double2 *tw, double inv_div,
+                                                         int k0, int *support_out,
+                                                         double2 *proj /* [nb][2][St] */)
+{
+    const int b = blockIdx.x;
+    const double2 *S = scr + (size_t)b * g.St * g.St;
+    double2 *rowp = proj + (size_t)b * 2 * g.St;
+    double2 *colp = rowp + g.St;
+    for (int i = threadIdx.x; i < g.St; i += blockDim.x) {
+        double2 r = make_double2(0, 0), c = make_double2(0, 0);
+        for (int j = 0; j < g.St; j++) {
+            double2 a = S[(size_t)j * g.St + i];  // column sum over rows j
+            r.x += a.x;
+            r.y += a.y;
+            double2 d = S[(size_t)i * g.St + j];  // row sum
+            c.x += d.x;
+            c.y += d.y;
+        }
+        rowp[i] = r;
+        colp[i] = c;
+    }
+    __syncthreads();
+    __shared__ int found;
+    if (threadIdx.x == 0) found = -1;
+    __syncthreads();
+    const int qmax = g.L - 1 - g.L / 2;
+    // Uniform trip count for every thread (the barriers below are block-wide).
+    for (int qb = qmax; qb >= 0; qb -= (int)blockDim.x) {
+        const int q = qb - (int)threadIdx.x;
+        for (int pass = 0; pass < 2 && q >= 0; pass++) {
+            const double2 *pr = pass ? colp : rowp;
+            int m = mod_pos(-(long long)g.h * q, g.L);
+            double re = 0, im = 0;
+            for (int c = 0; c < g.St; c++) {
+                double2 t = tw[m];
+                double2 v = pr[c];
+                re += v.x * t.x - v.y * t.y;
+                im += v.x * t.y + v.y * t.x;
+                m += q;
+                if (m >= g.L) m -= g.L;
+            }
+            re *= inv_div;
+            im *= inv_div;
+            if (re * re + im * im > 1e-6) {
+                atomicMax(&found, q);
+                break;
+            }
+        }
+        __syncthreads();
+        if (found >= 0) break;
+        __syncthreads();
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int S_ = 0;
+        if (found >= 0) {
+            S_ = ((found + (g.s + 1) / 2) / g.s) * 2 + 1;
+            if (S_ > g.St - 2) S_ = g.St - 2;
+        }
+        support_out[k0 + b] = S_;
+    }
+}
+
+// Stage 1 of the windowed DFT along x for one plane:
+// A[iy][j] = sum_cx scr[iy][cx] * tw[(cx*(qmin+j)) mod L], j < W
+__global__ void k_stage1(const double2 *scr, ScreenGeom g, const double2 *tw, int qmin, int W,
+                         double2 *A)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y;
+    if (j >= W) return;
+    const int q = qmin + j;
+    const int qm = mod_pos(q, g.L);
+    int m = mod_pos(-(long long)g.h * q, g.L);
+    const double2 *row = scr + (size_t)iy * g.St;
+    double re = 0, im = 0;
+    for (int c = 0; c < g.St; c++) {
+        double2 t = tw[m];
+        double2 v = row[c];
+        re += v.x * t.x - v.y * t.y;
+        im += v.x * t.y + v.y * t.x;
+        m += qm;
+        if (m >= g.L) m -= g.L;
+    }
+    A[(size_t)iy * W + j] = make_double2(re, im);
+}
+
+template <int PREC>
+__device__ __forceinline__ void store_sample(void *table, int64_t idx, double re, double im)
+{
+    if (PREC == MTI_TABLE_F32) {
+        ((float2 *)table)[idx] = make_float2((float)re, (float)im);
+    } else if (PREC == MTI_TABLE_F16) {
+        // direct FP64 -> FP16 rounding (no intermediate FP32 rounding)
+        _Float16 *h = (_Float16 *)table;
+        h[2 * idx] = (_Float16)re;
+        h[2 * idx + 1] = (_Float16)im;
+    } else {
+        __hip_bfloat16 a = __double2bfloat16(re), b = __double2bfloat16(im);
+        ((__hip_bfloat162 *)table)[idx] = __hip_bfloat162{a, b};
+    }
+}
+
+template <int PREC>
+__device__ __forceinline__ float2 load_sample(const void *table, int64_t idx)
+{
+    if (PREC == MTI_TABLE_F32) return ((const float2 *)table)[idx];
+    if (PREC == MTI_TABLE_F16) return __half22float2(((const __half2 *)table)[idx]);
+    __hip_bfloat162 v = ((const __hip_bfloat162 *)table)[idx];
+    return make_float2(__bfloat162float(v.x), __bfloat162float(v.y));
+}
+
+// Stage 2 (along y) written straight into the [oy][ox][sy][sx] table layout.
+template <int PREC>
+__global__ void k_stage2(const double2 *A, ScreenGeom g, const double2 *tw, int qmin, int W,
+                         int S, int T, double inv_div, void *table, int64_t base)
+{
+    const int64_t S2 = (int64_t)S * S;
+    const int64_t n = (int64_t)T * T * S2;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int o = (int)(e / S2);
+    const int t = (int)(e - (int64_t)o * S2);
+    const int oy = o / T, ox = o - oy * T;
+    const int sy = t / S, sx = t - sy * S;
+    const int jy = sy * g.s + oy, jx = sx * g.s + ox;
+    const int qy = qmin + jy;
+    const int qm = mod_pos(qy, g.L);
+    int m = mod_pos(-(long long)g.h * qy, g.L);
+    double re = 0, im = 0;
+    for (int c = 0; c < g.St; c++) {
+        double2 tt = tw[m];
+        double2 v = A[(size_t)c * W + jx];
+        re += v.x * tt.x - v.y * tt.y;
+        im += v.x * tt.y + v.y * tt.x;
+        m += qm;
+        if (m >= g.L) m -= g.L;
+    }
+    store_sample<PREC>(table, base + e, re * inv_div, im * inv_div);
+}
+
+// taper-only table (mode=normal): op/CasaGridSF.cu:107-149 + reorganize
+template <int PREC>
+__global__ void k_taper_table(int S, int s, int T, void *table)
+{
+    const int64_t S2 = (int64_t)S * S;
+    const int64_t n = (int64_t)T * T * S2;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int o = (int)(e / S2);
+    const int t = (int)(e - (int64_t)o * S2);
+    const int oy = o / T, ox = o - oy * T;
+    const int sy = t / S, sx = t - sy * S;
+    const int dim = (s % 2) ? S * s : S * s + 1;
+    const int c = dim / 2, fz = 3 * s;
+    auto g1 = [&](int i) -> double {
+        int a = abs(i - c);
+        if (a >= fz) return 0.0;
+        double nu = (double)a / (double)fz;
+        return (1.0 - nu * nu) * mti_dev::gridsf(nu);
+    };
+    store_sample<PREC>(table, e, g1(sy * s + oy) * g1(sx * s + ox), 0.0);
+}
+
+template <int PREC>
+__global__ void k_sums(const void *table, const int2 *convinfo, int nplanes, int T2, float *sums)
+{
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= nplanes * T2) return;
+    const int k = id / T2, o = id - k * T2;
+    const int S = convinfo[k].x;
+    const int64_t S2 = (int64_t)S * S;
+    const int64_t b = (int64_t)convinfo[k].y + (int64_t)o * S2;
+    double acc = 0;
+    for (int64_t i = 0; i < S2; i++) acc += (double)load_sample<PREC>(table, b + i).x;
+    sums[id] = (float)acc;
+}
+
+double host_gridsf(double nu)
+{
+    static const double P[2][5] = {
+        {8.203343e-2, -3.644705e-1, 6.278660e-1, -5.335581e-1, 2.312756e-1},
+        {4.028559e-3, -3.697768e-2, 1.021332e-1, -1.201436e-1, 6.412774e-2}};
+    static const double Q[2][3] = {{1.0, 8.212018e-1, 2.078043e-1},
+                                   {1.0, 9.599102e-1, 2.918724e-1}};
+    int part = (nu < 0.75) ? 0 : 1;
+    double d = nu * nu - (part ? 1.0 : 0.5625);
+    double top = P[part][0], bot = Q[part][0], dk = 1.0;
+    for (int k = 1; k < 5; k++) {
+        dk *= d;
+        top += P[part][k] * dk;
+    }
+    dk = 1.0;
+    for (int k = 1; k < 3; k++) {
+        dk *= d;
+        bot += Q[part][k] * dk;
+    }
+    return bot != 0.0 ? top / bot : 0.0;
+}
+
+size_t elem_bytes(int prec) { return prec == MTI_TABLE_F32 ? 8 : 4; }
+
+template <int PREC>
+void launch_stage2(dim3 gr, dim3 bl, hipStream_t st, const double2 *A, ScreenGeom g,
+                   const double2 *tw, int qmin, int W, int S, int T, double inv_div, void *table,
+                   int64_t base)
+{
+    hipLaunchKernelGGL(k_stage2<PREC>, gr, bl, 0, st, A, g, tw, qmin, W, S, T, inv_div, table, base);
+}
+
+}  // namespace
+
+using namespace mti;
+
+static mti_status build_impl(mti_ctx *ctx, const mti_image_params *p, int precision,
+                             mti_kernels *K, hipStream_t st)
+{
+    const int s = p->sampling;
+    const int T = (s % 2) ? s : s + 1;
+    K->info.sampling = s;
+    K->info.T = T;
+    K->info.precision = precision;
+    K->info.w_increment = w_increment(p);
+    const size_t eb = elem_bytes(precision);
+
+    if (p->mode == MTI_MODE_NORMAL) {
+        const int S = p->taper_support;
+        K->info.nplanes = 1;
+        K->info.trial_support = S;
+        K->info.max_support = S;
+        K->info.divisor = 1.0;
+        K->support = {S};
+        K->base = {0};
+        K->info.total = (int64_t)T * T * S * S;
+    } else {
+        const int W = p->wplanes;
+        int mn = std::min(p->nx, p->ny);
+        int St = mn / s;
+        if (St % 2 == 0) St++;
+        ScreenGeom g;
+        g.St = St;
+        g.h = St / 2;
+        g.s = s;
+        g.L = St * s;
+        g.linc = (p->cell_l * (double)p->nx) / (double)St;
+        g.minc = (p->cell_m * (double)p->ny) / (double)St;
+        K->info.nplanes = W;
+        K->info.trial_support = St;
+        // divisor: DC term of the w=0 screen = sum of the lm taper
+        double div = 0;
+        for (int cy = -g.h; cy <= g.h; cy++) {
+            double ty = host_gridsf(std::fabs((double)cy) / (double)g.h);
+            for (int cx = -g.h; cx <= g.h; cx++)
+                div += ty * host_gridsf(std::fabs((double)cx) / (double)g.h);
+        }
+        K->info.divisor = div;
+        const double inv = 1.0 / div;
+
+        // planes per batch bounded by ~512 MB of FP64 screens
+        const size_t plane_bytes = (size_t)St * St * sizeof(double2);
+        int nb = (int)std::max<size_t>(1, std::min<size_t>((size_t)W, (512u << 20) / plane_bytes));
+        const size_t tw_bytes = (size_t)g.L * sizeof(double2);
+        const size_t proj_bytes = (size_t)nb * 2 * St * sizeof(double2);
+        void *w0 = nullptr;
+        mti_status r = scratch_get(ctx, SLOT_SORTED,
+                                   tw_bytes + (size_t)nb * plane_bytes + proj_bytes +
+                                       (size_t)W * sizeof(int) + 256,
+                                   &w0);
+        if (r != MTI_OK) return r;
+        char *wp = (char *)w0;
+        double2 *d_tw = (double2 *)wp;
+        wp += (tw_bytes + 255) / 256 * 256;
+        double2 *d_scr = (double2 *)wp;
+        wp += (size_t)nb * plane_bytes;
+        double2 *d_proj = (double2 *)wp;
+        wp += proj_bytes;
+        int *d_sup = (int *)wp;
+
+        hipLaunchKernelGGL(k_twiddle, dim3((unsigned)ceil_div(g.L, 256)), dim3(256), 0, st, d_tw, g.L);
+        MTI_HIP(ctx, hipGetLastError());
+        dim3 sb(16, 16);
+        for (int k0 = 0; k0 < W; k0 += nb) {
+            int n = std::min(nb, W - k0);
+            dim3 sg((unsigned)ceil_div(St, 16), (unsigned)ceil_div(St, 16), (unsigned)n);
+            hipLaunchKernelGGL(k_screen, sg, sb, 0, st, d_scr, g, K->info.w_increment, k0);
+            MTI_HIP(ctx, hipGetLastError());
+            hipLaunchKernelGGL(k_support, dim3((unsigned)n), dim3(SUP_THREADS), 0, st, d_scr, g,
+                               d_tw, inv, k0, d_sup, d_proj);
+            MTI_HIP(ctx, hipGetLastError());
+        }
+        K->support.assign(W, 0);
+        MTI_HIP(ctx, hipMemcpyAsync(K->support.data(), d_sup, W * sizeof(int),
+                                    hipMemcpyDeviceToHost, st));
+        MTI_HIP(ctx, hipStreamSynchronize(st));
+        K->base.assign(W, 0);
+        int64_t tot = 0;
+        int smax = 0;
+        for (int k = 0; k < W; k++) {
+            K->base[k] = tot;
+            tot += (int64_t)T * T * K->support[k] * K->support[k];
+            smax = std::max(smax, K->support[k]);
+        }
+        K->info.total = tot;
+        K->info.max_support = smax;
+        if (tot >= (int64_t(1) << 31))
+            return set_error(ctx, MTI_ERR_UNSUPPORTED, "convolution table above 2^31 samples");
+        MTI_HIP(ctx, hipMalloc(&K->d_table, std::max<size_t>(1, (size_t)tot) * eb));
+
+        // stage-1 buffer sized for the widest window
+        int Wmax = (std::max(smax, 1) - 1) * s + T;
+        size_t a_bytes = (size_t)St * Wmax * sizeof(double2);
+        void *aw = nullptr;
+        r = scratch_get(ctx, SLOT_ITEMS, a_bytes, &aw);
+        if (r != MTI_OK) return r;
+        double2 *d_A = (double2 *)aw;
+        for (int k0 = 0; k0 < W; k0 += nb) {
+            int n = std::min(nb, W - k0);
+            dim3 sg((unsigned)ceil_div(St, 16), (unsigned)ceil_div(St, 16), (unsigned)n);
+            hipLaunchKernelGGL(k_screen, sg, sb, 0, st, d_scr, g, K->info.w_increment, k0);
+            MTI_HIP(ctx, hipGetLastError());
+            for (int b = 0; b < n; b++) {
+                int k = k0 + b, S = K->support[k];
+                if (S == 0) continue;
+                int qmin = -(S / 2) * s - s / 2;
+                int Wk = (S - 1) * s + T;
+                hipLaunchKernelGGL(k_stage1, dim3((unsigned)ceil_div(Wk, 64), (unsigned)St), dim3(64),
+                                   0, st, d_scr + (size_t)b * St * St, g, d_tw, qmin, Wk, d_A);
+                MTI_HIP(ctx, hipGetLastError());
+                int64_t ne = (int64_t)T * T * S * S;
+                dim3 gr((unsigned)ceil_div(ne, 256)), bl(256);
+                if (precision == MTI_TABLE_F32)
+                    launch_stage2<MTI_TABLE_F32>(gr, bl, st, d_A, g, d_tw, qmin, Wk, S, T, inv,
+                                                 K->d_table, K->base[k]);
+                else if (precision == MTI_TABLE_F16)
+                    launch_stage2<MTI_TABLE_F16>(gr, bl, st, d_A, g, d_tw, qmin, Wk, S, T, inv,
+                                                 K->d_table, K->base[k]);
+                else
+                    launch_stage2<MTI_TABLE_BF16>(gr, bl, st, d_A, g, d_tw, qmin, Wk, S, T, inv,
+                                                  K->d_table, K->base[k]);
+                MTI_HIP(ctx, hipGetLastError());
+            }
+        }
+    }
+
+    const int W = K->info.nplanes;
+    std::vector<int2> ci(W);
+    for (int k = 0; k < W; k++) ci[k] = make_int2(K->support[k], (int)K->base[k]);
+    MTI_HIP(ctx, hipMalloc(&K->d_convinfo, W * sizeof(int2)));
+    MTI_HIP(ctx, hipMemcpyAsync(K->d_convinfo, ci.data(), W * sizeof(int2), hipMemcpyHostToDevice, st));
+    if (p->mode == MTI_MODE_NORMAL) {
+        const int S = p->taper_support;
+        MTI_HIP(ctx, hipMalloc(&K->d_table, (size_t)K->info.total * eb));
+        dim3 gr((unsigned)ceil_div(K->info.total, 256)), bl(256);
+        if (precision == MTI_TABLE_F32)
+            hipLaunchKernelGGL(k_taper_table<MTI_TABLE_F32>, gr, bl, 0, st, S, s, T, K->d_table);
+        else if (precision == MTI_TABLE_F16)
+            hipLaunchKernelGGL(k_taper_table<MTI_TABLE_F16>, gr, bl, 0, st, S, s, T, K->d_table);
+        else
+            hipLaunchKernelGGL(k_taper_table<MTI_TABLE_BF16>, gr, bl, 0, st, S, s, T, K->d_table);
+        MTI_HIP(ctx, hipGetLastError());
+    }
+    const int T2 = T * T;
+    MTI_HIP(ctx, hipMalloc(&K->d_sums, (size_t)W * T2 * sizeof(float)));
+    dim3 gr((unsigned)ceil_div((int64_t)W * T2, 256)), bl(256);
+    if (precision == MTI_TABLE_F32)
+        hipLaunchKernelGGL(k_sums<MTI_TABLE_F32>, gr, bl, 0, st, K->d_table, K->d_convinfo, W, T2, K->d_sums);
+    else if (precision == MTI_TABLE_F16)
+        hipLaunchKernelGGL(k_sums<MTI_TABLE_F16>, gr, bl, 0, st, K->d_table, K->d_convinfo, W, T2, K->d_sums);
+    else
+        hipLaunchKernelGGL(k_sums<MTI_TABLE_BF16>, gr, bl, 0, st, K->d_table, K->d_convinfo, W, T2, K->d_sums);
+    MTI_HIP(ctx, hipGetLastError());
+    MTI_HIP(ctx, hipStreamSynchronize(st));
+    return MTI_OK;
+}
+
+extern "C" mti_status mti_kernels_build(mti_ctx *ctx, const mti_image_params *p, int32_t precision,
+                                        mti_kernels **out, void *stream)
+{
+    if (!ctx || !out) return MTI_ERR_INVALID;
+    *out = nullptr;
+    mti_status r = validate_params(ctx, p);
+    if (r != MTI_OK) return r;
+    MTI_CHECK(ctx, precision >= MTI_TABLE_F32 && precision <= MTI_TABLE_BF16, "bad table precision");
+    MTI_HIP(ctx, hipSetDevice(ctx->device));
+    mti_kernels *K = new mti_kernels();
+    K->device = ctx->device;
+    r = build_impl(ctx, p, precision, K, to_stream(stream));
+    if (r != MTI_OK) {
+        mti_kernels_free(K);
+        return r;
+    }
+    *out = K;
+    return MTI_OK;
+}
+
+extern "C" mti_status mti_kernels_info(const mti_kernels *k, mti_kernel_info *info)
+{
+    if (!k || !info) return MTI_ERR_INVALID;
+    *info = k->info;
+    return MTI_OK;
+}
+
+extern "C" mti_status mti_kernels_supports(const mti_kernels *k, int32_t *support, int64_t *base)
+{
+    if (!k) return MTI_ERR_INVALID;
+    for (int i = 0; i < k->info.nplanes; i++) {
+        if (support) support[i] = k->support[i];
+        if (base) base[i] = k->base[i];
+    }
+    return MTI_OK;
+}
+
+extern "C" const void *mti_kernels_table(const mti_kernels *k) { return k ? k->d_table : nullptr; }
+extern "C" const float *mti_kernels_sums(const mti_kernels *k) { return k ? k->d_sums : nullptr; }
+
+extern "C" void mti_kernels_free(mti_kernels *k)
+{
+    if (!k) return;
+    (void)hipSetDevice(k->device);
+    if (k->d_convinfo) (void)hipFree(k->d_convinfo);
+    if (k->d_table) (void)hipFree(k->d_table);
+    if (k->d_sums) (void)hipFree(k->d_sums);
+    delete k;
+}

@@ -1,0 +1,315 @@
+"""ctypes binding of the gfx950 imaging library's C-ABI (include/mtimager_amd.h).
+
+Device memory and streams come from PyTorch; every compute call goes through
+libmtimager_amd.so.  There is no CPU fallback: if the library or a GPU is
+missing, the calls raise.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libmtimager_amd.so")
+
+MTI_OK = 0
+MODE = {"normal": 0, "wproj": 1}
+PRECISION = {"f32": 0, "f16": 1, "bf16": 2}
+WEIGHTING = {"natural": 0, "uniform": 1, "briggs": 2}
+POLTYPE = {"linear": 0, "circular": 1}
+ARCSEC = 3.141592653589793 / 180.0 / 3600.0
+
+
+class MTIError(RuntimeError):
+    pass
+
+
+class ImageParams(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32),
+                ("cell_l", ctypes.c_double), ("cell_m", ctypes.c_double),
+                ("mode", ctypes.c_int32), ("wplanes", ctypes.c_int32),
+                ("sampling", ctypes.c_int32), ("taper_support", ctypes.c_int32)]
+
+
+class KernelInfo(ctypes.Structure):
+    _fields_ = [("nplanes", ctypes.c_int32), ("sampling", ctypes.c_int32),
+                ("T", ctypes.c_int32), ("trial_support", ctypes.c_int32),
+                ("max_support", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("w_increment", ctypes.c_double), ("divisor", ctypes.c_double),
+                ("total", ctypes.c_int64)]
+
+
+class Vis(ctypes.Structure):
+    _fields_ = [("nrows", ctypes.c_int64), ("nchan", ctypes.c_int32),
+                ("npol", ctypes.c_int32),
+                ("uvw", ctypes.c_void_p), ("uvw_f32", ctypes.c_int32),
+                ("uvw_row_stride", ctypes.c_int64), ("uvw_comp_stride", ctypes.c_int64),
+                ("inv_lambda", ctypes.c_void_p), ("inv_lambda_f32", ctypes.c_int32),
+                ("inv_lambda_per_row", ctypes.c_int32),
+                ("vis", ctypes.c_void_p), ("vis_row_stride", ctypes.c_int64),
+                ("vis_chan_stride", ctypes.c_int64),
+                ("flags", ctypes.c_void_p), ("flags_i32", ctypes.c_int32),
+                ("flags_row_stride", ctypes.c_int64), ("flags_chan_stride", ctypes.c_int64),
+                ("weight", ctypes.c_void_p), ("weight_row_stride", ctypes.c_int64),
+                ("row_index", ctypes.c_void_p), ("nindex", ctypes.c_int64)]
+
+
+class GridOpts(ctypes.Structure):
+    _fields_ = [("chan0", ctypes.c_int32), ("nchan", ctypes.c_int32),
+                ("accumulate", ctypes.c_int32), ("psf", ctypes.c_int32),
+                ("weighting", ctypes.c_int32), ("robust", ctypes.c_double)]
+
+
+class GridStats(ctypes.Structure):
+    _fields_ = [("gridded_records", ctypes.c_int64), ("gridded_points", ctypes.c_int64),
+                ("work_items", ctypes.c_int64), ("tile", ctypes.c_int32),
+                ("path", ctypes.c_int32)]
+
+
+# Every symbol include/mtimager_amd.h declares.
+EXPORTS = ["mti_version", "mti_create", "mti_destroy", "mti_last_error",
+           "mti_kernels_build", "mti_kernels_info", "mti_kernels_supports",
+           "mti_kernels_table", "mti_kernels_sums", "mti_kernels_free",
+           "mti_sort_rows", "mti_preprocess", "mti_grid", "mti_fft2d_inverse",
+           "mti_finalize"]
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MTIError(f"{LIB_PATH} not built: run __graft_entry__.build()")
+    # torch first: its bundled HIP runtime (soname libamdhip64.so.7) then
+    # satisfies the library's dependency, so both share one runtime.
+    import torch  # noqa: F401
+    L = ctypes.CDLL(LIB_PATH)
+    vp, P = ctypes.c_void_p, ctypes.POINTER
+    L.mti_version.restype = ctypes.c_char_p
+    L.mti_create.restype = ctypes.c_int
+    L.mti_create.argtypes = [ctypes.c_int, P(vp)]
+    L.mti_destroy.argtypes = [vp]
+    L.mti_last_error.restype = ctypes.c_char_p
+    L.mti_last_error.argtypes = [vp]
+    L.mti_kernels_build.restype = ctypes.c_int
+    L.mti_kernels_build.argtypes = [vp, P(ImageParams), ctypes.c_int32, P(vp), vp]
+    L.mti_kernels_info.restype = ctypes.c_int
+    L.mti_kernels_info.argtypes = [vp, P(KernelInfo)]
+    L.mti_kernels_supports.restype = ctypes.c_int
+    L.mti_kernels_supports.argtypes = [vp, vp, vp]
+    L.mti_kernels_table.restype = vp
+    L.mti_kernels_table.argtypes = [vp]
+    L.mti_kernels_sums.restype = vp
+    L.mti_kernels_sums.argtypes = [vp]
+    L.mti_kernels_free.argtypes = [vp]
+    L.mti_sort_rows.restype = ctypes.c_int
+    L.mti_sort_rows.argtypes = [vp, vp, vp, ctypes.c_int64, ctypes.c_int32, vp,
+                                P(ctypes.c_int64), vp]
+    L.mti_preprocess.restype = ctypes.c_int
+    L.mti_preprocess.argtypes = [vp, vp, P(ImageParams), P(Vis), ctypes.c_int32, vp, vp]
+    L.mti_grid.restype = ctypes.c_int
+    L.mti_grid.argtypes = [vp, vp, P(ImageParams), P(Vis), P(GridOpts), vp, vp,
+                           P(GridStats), vp]
+    L.mti_fft2d_inverse.restype = ctypes.c_int
+    L.mti_fft2d_inverse.argtypes = [vp, vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, vp]
+    L.mti_finalize.restype = ctypes.c_int
+    L.mti_finalize.argtypes = [vp, P(ImageParams), ctypes.c_int32, ctypes.c_int32,
+                               ctypes.c_int32, vp, vp, vp, vp]
+    _lib = L
+    return L
+
+
+def make_params(nx, ny, cell_l, cell_m=None, mode="wproj", wplanes=1, sampling=4,
+                taper_support=7, units="arcsec"):
+    """Parameters from mt-imager property values: image.nx, image.ny,
+    image.lintervalsize / mintervalsize (arcsec, or radians with
+    units="rad"), mode, wproj.wplanes, wproj.convFunction.sampling (or
+    taper.sampling), taper.support."""
+    if cell_m is None:
+        cell_m = cell_l
+    scale = ARCSEC if units == "arcsec" else 1.0
+    return ImageParams(int(nx), int(ny), float(cell_l) * scale, float(cell_m) * scale,
+                       MODE[mode], int(wplanes), int(sampling), int(taper_support))
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    return ctypes.c_void_p(stream)
+
+
+class Context:
+    def __init__(self, device=0):
+        self.h = ctypes.c_void_p()
+        rc = lib().mti_create(int(device), ctypes.byref(self.h))
+        if rc != MTI_OK:
+            raise MTIError(f"mti_create({device}) failed with status {rc}")
+
+    def check(self, rc, what):
+        if rc != MTI_OK:
+            msg = lib().mti_last_error(self.h).decode()
+            raise MTIError(f"{what} failed (status {rc}): {msg}")
+
+    def close(self):
+        if self.h:
+            lib().mti_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- a8-a14 ---------------------------------------------------------
+    def build_kernels(self, params, precision="f32", stream=None):
+        return Kernels(self, params, precision, stream)
+
+    # ---- a1 -------------------------------------------------------------
+    def sort_rows(self, ant1, ant2, nant, index_out, stream=None):
+        kept = ctypes.c_int64(0)
+        rc = lib().mti_sort_rows(self.h, ant1.data_ptr(), ant2.data_ptr(), ant1.numel(),
+                                 int(nant), index_out.data_ptr(), ctypes.byref(kept),
+                                 _stream(stream))
+        self.check(rc, "mti_sort_rows")
+        return int(kept.value)
+
+    # ---- a4 -------------------------------------------------------------
+    def preprocess(self, kern, params, vis, chan, bins_out, stream=None):
+        rc = lib().mti_preprocess(self.h, kern.h, ctypes.byref(params), ctypes.byref(vis),
+                                  int(chan), bins_out.data_ptr(), _stream(stream))
+        self.check(rc, "mti_preprocess")
+
+    # ---- a4-a7, a15, a21 --------------------------------------------------
+    def grid(self, kern, params, vis, grids, norm, chan0=0, nchan=1, accumulate=False,
+             psf=False, weighting="natural", robust=0.0, stats=True, stream=None):
+        o = GridOpts(int(chan0), int(nchan), 1 if accumulate else 0, 1 if psf else 0,
+                     WEIGHTING[weighting], float(robust))
+        st = GridStats()
+        rc = lib().mti_grid(self.h, kern.h, ctypes.byref(params), ctypes.byref(vis),
+                            ctypes.byref(o), grids.data_ptr(), norm.data_ptr(),
+                            ctypes.byref(st) if stats else None, _stream(stream))
+        self.check(rc, "mti_grid")
+        return st
+
+    # ---- a17 ------------------------------------------------------------
+    def fft2d_inverse(self, data, nx, ny, nbatch=1, stream=None):
+        rc = lib().mti_fft2d_inverse(self.h, data.data_ptr(), int(nx), int(ny), int(nbatch),
+                                     _stream(stream))
+        self.check(rc, "mti_fft2d_inverse")
+
+    # ---- a16-a19 ----------------------------------------------------------
+    def finalize(self, params, npol, grids, norm, image, nchan=1, poltype="linear",
+                 stream=None):
+        rc = lib().mti_finalize(self.h, ctypes.byref(params), int(npol), POLTYPE[poltype],
+                                int(nchan), grids.data_ptr(), norm.data_ptr(),
+                                image.data_ptr(), _stream(stream))
+        self.check(rc, "mti_finalize")
+
+
+class Kernels:
+    def __init__(self, ctx, params, precision="f32", stream=None):
+        self.ctx = ctx
+        self.h = ctypes.c_void_p()
+        rc = lib().mti_kernels_build(ctx.h, ctypes.byref(params), PRECISION[precision],
+                                     ctypes.byref(self.h), _stream(stream))
+        ctx.check(rc, "mti_kernels_build")
+        self.info = KernelInfo()
+        lib().mti_kernels_info(self.h, ctypes.byref(self.info))
+
+    def supports(self):
+        import numpy as np
+        n = self.info.nplanes
+        s = np.zeros(n, dtype=np.int32)
+        b = np.zeros(n, dtype=np.int64)
+        lib().mti_kernels_supports(self.h, s.ctypes.data, b.ctypes.data)
+        return s, b
+
+    def table(self):
+        """Device table copied to a torch tensor (complex64 [total])."""
+        import torch
+        n = int(self.info.total)
+        prec = self.info.precision
+        ptr = lib().mti_kernels_table(self.h)
+        esz = 8 if prec == 0 else 4
+        raw = torch.empty(n * esz, dtype=torch.uint8, device="cuda")
+        if n:
+            _memcpy_d2d(raw.data_ptr(), ptr, n * esz)
+        if prec == 0:
+            return raw.view(torch.complex64)
+        if prec == 1:
+            h = raw.view(torch.float16).float().view(-1, 2)
+        else:
+            h = raw.view(torch.bfloat16).float().view(-1, 2)
+        return torch.complex(h[:, 0], h[:, 1])
+
+    def sums(self):
+        import torch
+        n = self.info.nplanes * self.info.T * self.info.T
+        out = torch.empty(n, dtype=torch.float32, device="cuda")
+        _memcpy_d2d(out.data_ptr(), lib().mti_kernels_sums(self.h), n * 4)
+        return out.view(self.info.nplanes, self.info.T * self.info.T)
+
+    def free(self):
+        if self.h:
+            lib().mti_kernels_free(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _memcpy_d2d(dst, src, nbytes):
+    """Device-to-device copy through the HIP runtime the library is bound to."""
+    import torch
+    hip = lib()
+    hip.hipMemcpy.restype = ctypes.c_int
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    torch.cuda.synchronize()
+    rc = hip.hipMemcpy(ctypes.c_void_p(dst), ctypes.c_void_p(src), ctypes.c_size_t(nbytes), 3)
+    if rc != 0:
+        raise MTIError(f"hipMemcpy failed ({rc})")
+
+
+def vis_ms(uvw, inv_lambda, vis, flags, weight, row_index=None):
+    """mti_vis for MeasurementSet-ordered torch tensors on the GPU:
+    uvw f64 [nrows][3], inv_lambda f64 [nchan], vis complex64 [nrows][nchan][npol],
+    flags u8 [nrows][nchan][npol], weight f32 [nrows][npol], row_index int32 or None.
+    The tensors must stay alive while the descriptor is used."""
+    nrows, nchan, npol = flags.shape
+    for t in (uvw, inv_lambda, vis, flags, weight):
+        assert t.is_contiguous() and t.is_cuda
+    v = Vis()
+    v.nrows, v.nchan, v.npol = nrows, nchan, npol
+    v.uvw, v.uvw_f32, v.uvw_row_stride, v.uvw_comp_stride = uvw.data_ptr(), 0, 3, 1
+    v.inv_lambda, v.inv_lambda_f32, v.inv_lambda_per_row = inv_lambda.data_ptr(), 0, 0
+    v.vis, v.vis_row_stride, v.vis_chan_stride = vis.data_ptr(), nchan * npol, npol
+    v.flags, v.flags_i32 = flags.data_ptr(), 0
+    v.flags_row_stride, v.flags_chan_stride = nchan * npol, npol
+    v.weight, v.weight_row_stride = weight.data_ptr(), npol
+    if row_index is not None:
+        v.row_index, v.nindex = row_index.data_ptr(), row_index.numel()
+    else:
+        v.row_index, v.nindex = None, 0
+    return v
+
+
+def vis_reference_chunk(uvw_t, inv_lambda_rows, vis, flags_i32, weight):
+    """mti_vis for the reference's per-channel chunk layout
+    (vis/StorageManager.h:147-150): uvw f32 [3][N], 1/lambda f32 [N],
+    vis f32 [N][2P], flags int32 [N][P], weights f32 [N][P]."""
+    n, npol = flags_i32.shape
+    v = Vis()
+    v.nrows, v.nchan, v.npol = n, 1, npol
+    v.uvw, v.uvw_f32, v.uvw_row_stride, v.uvw_comp_stride = uvw_t.data_ptr(), 1, 1, n
+    v.inv_lambda, v.inv_lambda_f32, v.inv_lambda_per_row = inv_lambda_rows.data_ptr(), 1, 1
+    v.vis, v.vis_row_stride, v.vis_chan_stride = vis.data_ptr(), npol, 0
+    v.flags, v.flags_i32 = flags_i32.data_ptr(), 1
+    v.flags_row_stride, v.flags_chan_stride = npol, 0
+    v.weight, v.weight_row_stride = weight.data_ptr(), npol
+    v.row_index, v.nindex = None, 0
+    return v
