@@ -120,11 +120,33 @@ typedef struct {
     int32_t path;             /* 0: LDS tile gridder, 1: global-atomic path   */
 } mti_grid_stats;
 
+/* Pipeline stages timed with HIP events when profiling is on (the
+ * equivalent of the reference's per-operator start/end events,
+ * gafw/GPUImplimentation/CUDADeviceManager.cpp:435-560 -> engine.csv). */
+typedef enum {
+    MTI_STAGE_SORT_ROWS = 0,  /* mti_sort_rows                               */
+    MTI_STAGE_KERNELS = 1,    /* mti_kernels_build                           */
+    MTI_STAGE_BIN = 2,        /* binning pass (and weight density)           */
+    MTI_STAGE_SORT_RECORDS = 3,/* radix sort of records by (channel, tile)   */
+    MTI_STAGE_PLAN = 4,       /* bucket scans                                */
+    MTI_STAGE_GRID = 5,       /* tile gridder                                */
+    MTI_STAGE_FFT_ROWS = 6,   /* row transforms (+ uv reorder)               */
+    MTI_STAGE_FFT_COLS = 7,   /* first column pass                           */
+    MTI_STAGE_FINALIZE = 8,   /* second column pass + Stokes + corrections   */
+    MTI_STAGE_COUNT = 9
+} mti_stage;
+
 /* ---- context ------------------------------------------------------------ */
 const char *mti_version(void);
 mti_status  mti_create(int device, mti_ctx **out);
 void        mti_destroy(mti_ctx *ctx);
 const char *mti_last_error(const mti_ctx *ctx);
+/* Profiling: when enabled every stage launch is bracketed by HIP events on
+ * the caller's stream.  mti_stage_times waits for the recorded events, adds
+ * each stage's elapsed milliseconds and launch count to ms[MTI_STAGE_COUNT] /
+ * launches[MTI_STAGE_COUNT] (either may be NULL) and clears the record. */
+mti_status  mti_set_profiling(mti_ctx *ctx, int32_t enable);
+mti_status  mti_stage_times(mti_ctx *ctx, double *ms, int64_t *launches);
 
 /* ---- a8-a14: convolution functions --------------------------------------
  * Replaces WProjectionConvFunction::calculate (cfg/WProjectionConvFunction.cpp:186-291)

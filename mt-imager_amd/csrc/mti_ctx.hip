@@ -26,6 +26,11 @@ extern "C" void mti_destroy(mti_ctx *ctx)
 {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
+    for (auto &e : ctx->stage_events) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     for (auto &s : ctx->scratch)
         if (s.ptr) (void)hipFree(s.ptr);
     for (auto &t : ctx->twiddles)
@@ -38,7 +43,60 @@ extern "C" const char *mti_last_error(const mti_ctx *ctx)
     return ctx ? ctx->last_error.c_str() : "null context";
 }
 
+extern "C" mti_status mti_set_profiling(mti_ctx *ctx, int32_t enable)
+{
+    if (!ctx) return MTI_ERR_INVALID;
+    ctx->profiling = enable != 0;
+    return MTI_OK;
+}
+
+extern "C" mti_status mti_stage_times(mti_ctx *ctx, double *ms, int64_t *launches)
+{
+    if (!ctx) return MTI_ERR_INVALID;
+    MTI_HIP(ctx, hipSetDevice(ctx->device));
+    for (auto &e : ctx->stage_events) {
+        MTI_HIP(ctx, hipEventSynchronize(e.b));
+        float t = 0.0f;
+        MTI_HIP(ctx, hipEventElapsedTime(&t, e.a, e.b));
+        if (e.stage >= 0 && e.stage < MTI_STAGE_COUNT) {
+            if (ms) ms[e.stage] += t;
+            if (launches) launches[e.stage] += 1;
+        }
+        ctx->event_pool.push_back(e.a);
+        ctx->event_pool.push_back(e.b);
+    }
+    ctx->stage_events.clear();
+    return MTI_OK;
+}
+
 namespace mti {
+
+static hipEvent_t pool_event(mti_ctx *ctx)
+{
+    if (!ctx->event_pool.empty()) {
+        hipEvent_t e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+StageTimer::StageTimer(mti_ctx *c, int s, hipStream_t stream) : ctx(c), st(stream), stage(s)
+{
+    if (!ctx || !ctx->profiling) return;
+    a = pool_event(ctx);
+    b = pool_event(ctx);
+    if (a) (void)hipEventRecord(a, st);
+}
+
+StageTimer::~StageTimer()
+{
+    if (!a || !b) return;
+    (void)hipEventRecord(b, st);
+    ctx->stage_events.push_back({stage, a, b});
+}
 
 // Validation mirrors the reference's validate() checks: odd taper support
 // (op/CasaGridSF.cpp:65-66), known mode (comp/ConfigurationManager.cpp:106-107),

@@ -481,15 +481,17 @@ extern "C" mti_status mti_grid(mti_ctx *ctx, const mti_kernels *k, const mti_ima
     ba.counts = counts;
     ba.density = density;
     ba.stats = dstats;
-    r = launch_bin_pass(ctx, ba, nrec, st);
-    if (r != MTI_OK) return r;
-
-    if (density) {
-        dim3 gr(64, (unsigned)(nb * P));
-        hipLaunchKernelGGL(k_density_stats, gr, dim3(256), 0, st, density, gsz, dacc);
-        hipLaunchKernelGGL(k_briggs_f2, dim3((unsigned)ceil_div(nb * P, 64)), dim3(64), 0, st, dacc, nb * P,
-                           o->robust, f2);
-        MTI_HIP(ctx, hipGetLastError());
+    {
+        StageTimer tm(ctx, MTI_STAGE_BIN, st);
+        r = launch_bin_pass(ctx, ba, nrec, st);
+        if (r != MTI_OK) return r;
+        if (density) {
+            dim3 gr(64, (unsigned)(nb * P));
+            hipLaunchKernelGGL(k_density_stats, gr, dim3(256), 0, st, density, gsz, dacc);
+            hipLaunchKernelGGL(k_briggs_f2, dim3((unsigned)ceil_div(nb * P, 64)), dim3(64), 0, st, dacc,
+                               nb * P, o->robust, f2);
+            MTI_HIP(ctx, hipGetLastError());
+        }
     }
 
     TileArgs ta;
@@ -517,6 +519,7 @@ extern "C" mti_status mti_grid(mti_ctx *ctx, const mti_kernels *k, const mti_ima
 
     const int prec = k->info.precision;
     if (global_path) {
+        StageTimer tm(ctx, MTI_STAGE_GRID, st);
         ta.sorted_keys = keys;
         ta.sorted_vals = vals;
         switch (P) {
@@ -527,22 +530,29 @@ extern "C" mti_status mti_grid(mti_ctx *ctx, const mti_kernels *k, const mti_ima
         if (r != MTI_OK) return r;
     } else {
         bool in_tmp = false;
-        r = radix_sort_pairs(ctx, keys, vals, keys_t, vals_t, nrec, key_bits, st, &in_tmp);
-        if (r != MTI_OK) return r;
+        {
+            StageTimer tm(ctx, MTI_STAGE_SORT_RECORDS, st);
+            r = radix_sort_pairs(ctx, keys, vals, keys_t, vals_t, nrec, key_bits, st, &in_tmp);
+            if (r != MTI_OK) return r;
+        }
         ta.sorted_keys = in_tmp ? keys_t : keys;
         ta.sorted_vals = in_tmp ? vals_t : vals;
-        MTI_HIP(ctx, hipMemcpyAsync(starts, counts, nbuckets64 * 4, hipMemcpyDeviceToDevice, st));
-        r = exclusive_scan_u32(ctx, starts, nbuckets64, tot_rec, st);
-        if (r != MTI_OK) return r;
-        hipLaunchKernelGGL(k_items, dim3((unsigned)ceil_div(nbuckets64, 256)), dim3(256), 0, st, counts,
-                           nbuckets, items);
-        MTI_HIP(ctx, hipGetLastError());
-        r = exclusive_scan_u32(ctx, items, nbuckets64, tot_items, st);
-        if (r != MTI_OK) return r;
+        {
+            StageTimer tm(ctx, MTI_STAGE_PLAN, st);
+            MTI_HIP(ctx, hipMemcpyAsync(starts, counts, nbuckets64 * 4, hipMemcpyDeviceToDevice, st));
+            r = exclusive_scan_u32(ctx, starts, nbuckets64, tot_rec, st);
+            if (r != MTI_OK) return r;
+            hipLaunchKernelGGL(k_items, dim3((unsigned)ceil_div(nbuckets64, 256)), dim3(256), 0, st, counts,
+                               nbuckets, items);
+            MTI_HIP(ctx, hipGetLastError());
+            r = exclusive_scan_u32(ctx, items, nbuckets64, tot_items, st);
+            if (r != MTI_OK) return r;
+        }
         ta.bucket_start = starts;
         ta.bucket_count = counts;
         ta.item_prefix = items;
         ta.total_items = tot_items;
+        StageTimer tm(ctx, MTI_STAGE_GRID, st);
         switch (P) {
         case 1: r = dispatch_tiles<1>(ctx, prec, ta, lds, st); break;
         case 2: r = dispatch_tiles<2>(ctx, prec, ta, lds, st); break;

@@ -27,9 +27,17 @@ struct mti_ctx {
     int device = 0;
     int num_cus = 0;
     std::string last_error;
+    // profiling: (stage, start, end) events recorded on the caller's stream
+    bool profiling = false;
+    struct StageEv {
+        int stage;
+        hipEvent_t a, b;
+    };
+    std::vector<StageEv> stage_events;
+    std::vector<hipEvent_t> event_pool;
     mti::Scratch scratch[12];
-    // FFT twiddle tables keyed by length (float2, exp(+2 pi i k / n)).
-    std::vector<std::pair<int, float2 *>> twiddles;
+    // FFT twiddle tables keyed by length (FP64, exp(+2 pi i k / n)).
+    std::vector<std::pair<int, double2 *>> twiddles;
 };
 
 struct mti_kernels {
@@ -111,6 +119,16 @@ inline double w_increment(const mti_image_params *p)
     double wm1 = (double)(p->wplanes - 1);
     return wmax / (wm1 * wm1);
 }
+
+// RAII bracket of one stage with HIP events (no-op unless profiling).
+struct StageTimer {
+    mti_ctx *ctx;
+    hipStream_t st;
+    int stage;
+    hipEvent_t a = nullptr, b = nullptr;
+    StageTimer(mti_ctx *c, int s, hipStream_t stream);
+    ~StageTimer();
+};
 
 mti_status validate_params(mti_ctx *ctx, const mti_image_params *p);
 mti_status validate_vis(mti_ctx *ctx, const mti_vis *v);

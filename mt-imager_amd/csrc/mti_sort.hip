@@ -152,6 +152,7 @@ extern "C" mti_status mti_sort_rows(mti_ctx *ctx, const int32_t *ant1, const int
     MTI_CHECK(ctx, kept_out && (nrows == 0 || (ant1 && ant2 && index_out)), "null pointer");
     MTI_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = to_stream(stream);
+    StageTimer tm(ctx, MTI_STAGE_SORT_ROWS, st);
     if (nrows == 0) {
         *kept_out = 0;
         return MTI_OK;

@@ -468,7 +468,10 @@ extern "C" mti_status mti_kernels_build(mti_ctx *ctx, const mti_image_params *p,
     MTI_HIP(ctx, hipSetDevice(ctx->device));
     mti_kernels *K = new mti_kernels();
     K->device = ctx->device;
-    r = build_impl(ctx, p, precision, K, to_stream(stream));
+    {
+        StageTimer tm(ctx, MTI_STAGE_KERNELS, to_stream(stream));
+        r = build_impl(ctx, p, precision, K, to_stream(stream));
+    }
     if (r != MTI_OK) {
         mti_kernels_free(K);
         return r;

@@ -65,7 +65,11 @@ class GridStats(ctypes.Structure):
 
 
 # Every symbol include/mtimager_amd.h declares.
+STAGES = ["sort_rows", "kernels", "bin", "sort_records", "plan", "grid", "fft_rows",
+          "fft_cols", "finalize"]
+
 EXPORTS = ["mti_version", "mti_create", "mti_destroy", "mti_last_error",
+           "mti_set_profiling", "mti_stage_times",
            "mti_kernels_build", "mti_kernels_info", "mti_kernels_supports",
            "mti_kernels_table", "mti_kernels_sums", "mti_kernels_free",
            "mti_sort_rows", "mti_preprocess", "mti_grid", "mti_fft2d_inverse",
@@ -91,6 +95,10 @@ def lib():
     L.mti_destroy.argtypes = [vp]
     L.mti_last_error.restype = ctypes.c_char_p
     L.mti_last_error.argtypes = [vp]
+    L.mti_set_profiling.restype = ctypes.c_int
+    L.mti_set_profiling.argtypes = [vp, ctypes.c_int32]
+    L.mti_stage_times.restype = ctypes.c_int
+    L.mti_stage_times.argtypes = [vp, vp, vp]
     L.mti_kernels_build.restype = ctypes.c_int
     L.mti_kernels_build.argtypes = [vp, P(ImageParams), ctypes.c_int32, P(vp), vp]
     L.mti_kernels_info.restype = ctypes.c_int
@@ -161,6 +169,18 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    # ---- profiling ------------------------------------------------------
+    def set_profiling(self, enable=True):
+        self.check(lib().mti_set_profiling(self.h, 1 if enable else 0), "mti_set_profiling")
+
+    def stage_times(self):
+        """{stage: (total ms, launches)} since the last call (waits for events)."""
+        import numpy as np
+        ms = np.zeros(len(STAGES), np.float64)
+        n = np.zeros(len(STAGES), np.int64)
+        self.check(lib().mti_stage_times(self.h, ms.ctypes.data, n.ctypes.data), "mti_stage_times")
+        return {s: (float(ms[i]), int(n[i])) for i, s in enumerate(STAGES)}
 
     # ---- a8-a14 ---------------------------------------------------------
     def build_kernels(self, params, precision="f32", stream=None):
@@ -279,7 +299,7 @@ def vis_ms(uvw, inv_lambda, vis, flags, weight, row_index=None):
     """mti_vis for MeasurementSet-ordered torch tensors on the GPU:
     uvw f64 [nrows][3], inv_lambda f64 [nchan], vis complex64 [nrows][nchan][npol],
     flags u8 [nrows][nchan][npol], weight f32 [nrows][npol], row_index int32 or None.
-    The tensors must stay alive while the descriptor is used."""
+    The descriptor keeps references to the tensors it points into."""
     nrows, nchan, npol = flags.shape
     for t in (uvw, inv_lambda, vis, flags, weight):
         assert t.is_contiguous() and t.is_cuda
@@ -295,6 +315,7 @@ def vis_ms(uvw, inv_lambda, vis, flags, weight, row_index=None):
         v.row_index, v.nindex = row_index.data_ptr(), row_index.numel()
     else:
         v.row_index, v.nindex = None, 0
+    v._keep = (uvw, inv_lambda, vis, flags, weight, row_index)
     return v
 
 
@@ -312,4 +333,5 @@ def vis_reference_chunk(uvw_t, inv_lambda_rows, vis, flags_i32, weight):
     v.flags_row_stride, v.flags_chan_stride = npol, 0
     v.weight, v.weight_row_stride = weight.data_ptr(), npol
     v.row_index, v.nindex = None, 0
+    v._keep = (uvw_t, inv_lambda_rows, vis, flags_i32, weight)
     return v

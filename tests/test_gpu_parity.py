@@ -298,10 +298,10 @@ def test_all_flagged_and_empty(mti, ctx, c1_params):
     st = ctx.grid(K, p, v, grids, norm)
     assert st.gridded_records == 0
     assert float(grids.abs().max()) == 0.0 and float(norm.abs().max()) == 0.0
-    e = torch.empty(0, dtype=torch.int32, device="cuda")
-    v.row_index, v.nindex = e.data_ptr(), 0
+    v.nrows = 0
+    grids.fill_(3.0)
     st = ctx.grid(K, p, v, grids, norm)
-    assert st.gridded_records == 0
+    assert st.gridded_records == 0 and float(grids.abs().max()) == 0.0
 
 
 # ---- a17 -------------------------------------------------------------------
@@ -312,8 +312,9 @@ def test_fft2d_inverse_matches_numpy(mti, ctx, nx, ny, nb):
     d = torch.tensor(x, device="cuda")
     ctx.fft2d_inverse(d, nx, ny, nb)
     ref = np.fft.ifft2(x.astype(np.complex128)) * (nx * ny)
-    got = d.cpu().numpy()
-    assert H.rel_rms(np.abs(got - ref), np.zeros_like(np.abs(ref)) + np.sqrt(np.mean(np.abs(ref) ** 2))) < 1e-5
+    got = d.cpu().numpy().astype(np.complex128)
+    err = np.sqrt(np.mean(np.abs(got - ref) ** 2)) / np.sqrt(np.mean(np.abs(ref) ** 2))
+    assert err < 1e-6, err
 
 
 @pytest.mark.parametrize("npol,poltype", [(1, "linear"), (2, "circular"), (4, "linear"), (4, "circular")])
