@@ -35,6 +35,13 @@ typedef enum { MTI_MODE_NORMAL = 0, MTI_MODE_WPROJ = 1 } mti_mode;
 typedef enum { MTI_TABLE_F32 = 0, MTI_TABLE_F16 = 1, MTI_TABLE_BF16 = 2 } mti_table_precision;
 typedef enum { MTI_WEIGHT_NATURAL = 0, MTI_WEIGHT_UNIFORM = 1, MTI_WEIGHT_BRIGGS = 2 } mti_weighting;
 typedef enum { MTI_POL_LINEAR = 0, MTI_POL_CIRCULAR = 1 } mti_poltype;
+/* Grid layouts: the reference's WImager Grid [v][u][P] (comp/WImager.cpp:231-232)
+ * or planar [P][v][u] (what the reference's PreIFFTUVCorrection reorders it
+ * to before the transform, op/PreIFFTUVCorrection.cu:28-90). */
+typedef enum { MTI_GRID_VUP = 0, MTI_GRID_PVU = 1 } mti_grid_layout;
+/* Grid element type.  Gridding always accumulates in FP64; an FP32 grid (the
+ * reference's complex<float> Grid) is rounded once at the end of mti_grid. */
+typedef enum { MTI_GRID_C64 = 0, MTI_GRID_C128 = 1 } mti_grid_precision;
 
 /* Imaging parameters; the fields carry the mt-imager property keys
  * (comp/ConfigurationManager.cpp:33-62, conf/mtimager.properties:1-78). */
@@ -110,14 +117,15 @@ typedef struct {
     int32_t psf;              /* psf: every visibility replaced by 1+0i       */
     int32_t weighting;        /* mti_weighting                                */
     double  robust;           /* Briggs robustness R                          */
+    int32_t grid_layout;      /* mti_grid_layout of `grids`                   */
+    int32_t grid_precision;   /* mti_grid_precision of `grids`                */
 } mti_grid_opts;
 
 typedef struct {
     int64_t gridded_records;  /* Statistic:TotalGriddedRecords                */
     int64_t gridded_points;   /* Statistic:TotalGriddedPoints (sum S^2 * P)   */
-    int64_t work_items;       /* tile work items processed                    */
-    int32_t tile;             /* interior tile edge (cells) chosen            */
-    int32_t path;             /* 0: LDS tile gridder, 1: global-atomic path   */
+    int64_t work_items;       /* equal-support record runs (<= 512 records)   */
+    int64_t flushes;          /* reserved (0)                                 */
 } mti_grid_stats;
 
 /* Pipeline stages timed with HIP events when profiling is on (the
@@ -194,11 +202,15 @@ mti_status mti_preprocess(mti_ctx *ctx, const mti_kernels *k,
  * NotSameSupportManipulate work plan, CreateCompressionPlan/
  * CompressVisibilities, ConvolutionGridder op/ConvolutionGridder.cu:183-550,
  * and the NormalizerValue Sum) for opts->nchan channels at once.
- *   grids : complex64 [nchan][ny][nx][npol] (Grid, comp/WImager.cpp:231-232)
+ *   grids : per channel [ny][nx][npol] (MTI_GRID_VUP, the reference's Grid,
+ *           comp/WImager.cpp:231-232) or [npol][ny][nx] (MTI_GRID_PVU), of
+ *           complex64 or complex128 elements (opts->grid_precision)
  *   norm  : f64 [nchan][npol]               (NormalizerValue)
- * Weighting (uniform/Briggs, not in the reference) is applied per channel. */
+ * Weighting (uniform/Briggs, not in the reference) is applied per channel.
+ * Accumulation is FP64 (registers and grid); complex64 grids are rounded
+ * once from an FP64 work grid held by the context. */
 mti_status mti_grid(mti_ctx *ctx, const mti_kernels *k, const mti_image_params *p,
-                    const mti_vis *v, const mti_grid_opts *opts, float *grids,
+                    const mti_vis *v, const mti_grid_opts *opts, void *grids,
                     double *norm, mti_grid_stats *stats, void *stream);
 
 /* ---- a17: 2-D inverse FFT (IFFT2D, gafw/StandardOperators/IFFT2D.cpp:86-111) --
@@ -210,11 +222,15 @@ mti_status mti_fft2d_inverse(mti_ctx *ctx, float *data, int32_t nx, int32_t ny,
 /* ---- a16-a19: finalize (ImageFinalizer, comp/ImageFinalizer.cpp:31-44) -----
  * PreIFFTUVCorrection + IFFT2D + ZeroToCentre2DShift + StokesConverter
  * (normalise, Stokes, m-flip) + casagridsf taper correction + SincCorrector.
- *   grids : complex64 [nchan][ny][nx][npol]  (consumed: used as scratch)
+ * The transform runs in FP64; the image is FP32.
+ *   grids : per channel in grid_layout (mti_grid_layout), complex64 or
+ *           complex128 (grid_precision); read only
  *   norm  : f64 [nchan][npol]
- *   image : f32 [nchan][npol][ny][nx]  (Stokes I,Q,U,V / I,Q / I,V / I) */
+ *   image : f32 [nchan][npol][ny][nx]  (Stokes I,Q,U,V / I,Q / I,V / I)
+ * nx: power of two in [2,4096]; ny: power of two in [2,8192]. */
 mti_status mti_finalize(mti_ctx *ctx, const mti_image_params *p, int32_t npol,
-                        int32_t poltype, int32_t nchan, float *grids,
+                        int32_t poltype, int32_t nchan, int32_t grid_layout,
+                        int32_t grid_precision, const void *grids,
                         const double *norm, float *image, void *stream);
 
 #ifdef __cplusplus

@@ -102,54 +102,72 @@ __global__ void k_bin_only(VisView v, BinGeom g, int chan, int32_t *out)
     o[7] = b.conj;
 }
 
+// Wave-level segmented sum of v over runs of equal key in lane order; the
+// last lane of every run holding a valid key adds the run's sum with one
+// atomic.  Consecutive records of one baseline usually share a cell, so a wave
+// issues a few atomics instead of 64.
+__device__ __forceinline__ void wave_run_atomic_add(float *base, long long key, float v)
+{
+    const int lane = threadIdx.x & 63;
+    const long long prev = __shfl_up(key, 1, 64);
+    const long long next = __shfl_down(key, 1, 64);
+    int head = (lane == 0) || (prev != key);
+    const bool tail = (lane == 63) || (next != key);
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const float vu = __shfl_up(v, d, 64);
+        const int hu = __shfl_up(head, d, 64);
+        if (lane >= d && !head) {
+            v += vu;
+            head |= hu;
+        }
+    }
+    if (tail && key >= 0 && v != 0.0f) unsafeAtomicAdd(base + key, v);
+}
+
 // Record e = c_local * nidx + i (channel-major, so a wave sees consecutive
 // records of one channel in baseline-major order).
 __global__ __launch_bounds__(256) void k_bin_pass(BinPassArgs a, int64_t nrec)
 {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool in = e < nrec;
-    uint32_t key = a.nbuckets;
+    bool grid = false;
     unsigned long long pts = 0;
+    long long dcell = -1;
+    int64_t row = 0;
+    int chan = 0, c_local = 0;
     if (in) {
-        const int c_local = (int)(e / a.v.nidx);
+        c_local = (int)(e / a.v.nidx);
         const int64_t i = e - (int64_t)c_local * a.v.nidx;
-        const int64_t row = a.v.row_index ? (int64_t)a.v.row_index[i] : i;
-        const int chan = a.chan0 + c_local;
+        row = a.v.row_index ? (int64_t)a.v.row_index[i] : i;
+        chan = a.chan0 + c_local;
         BinOut b = bin_record(a.v, a.g, row, chan);
         int4 d = make_int4((int)row, 0, 0, 0);
         if (b.to_grid) {
-            const int tile = (b.cy / a.TS) * a.ntx + (b.cx / a.TS);
-            key = (uint32_t)c_local * (uint32_t)a.ntiles + (uint32_t)tile;
+            grid = true;
             d.y = (b.cy << 16) | b.cx;
             d.z = b.conv_idx;
             d.w = (b.plane << 16) | (b.conj << 15) | b.S;
             pts = (unsigned long long)b.S * b.S * a.v.npol;
-            if (a.density) {
-                const int64_t cell = (int64_t)(b.cy + b.S / 2) * a.g.nx + (b.cx + b.S / 2);
-                const int64_t plane_sz = (int64_t)a.g.nx * a.g.ny;
-                for (int p = 0; p < a.v.npol; p++) {
-                    if (vis_flag(a.v, row, chan, p)) continue;
-                    float w = a.v.weight[row * a.v.w_rs + p];
-                    unsafeAtomicAdd(&a.density[((int64_t)c_local * a.v.npol + p) * plane_sz + cell], w);
-                }
-            }
+            dcell = (long long)(b.cy + b.S / 2) * a.g.nx + (b.cx + b.S / 2);
         }
-        a.keys[e] = key;
-        a.vals[e] = (int32_t)e;
+        a.valid[e] = grid ? 1u : 0u;
         a.desc[e] = d;
     }
-    // wave-aggregated bucket histogram
-    bool pending = in && key < a.nbuckets;
-    while (__ballot(pending)) {
-        const uint64_t act = __ballot(pending);
-        const int leader = __ffsll((long long)act) - 1;
-        const uint32_t k0 = __shfl(key, leader, 64);
-        const uint64_t same = __ballot(pending && key == k0);
-        if ((int)(threadIdx.x & 63) == leader) atomicAdd(&a.counts[k0], (uint32_t)__popcll(same));
-        if (pending && key == k0) pending = false;
+    if (a.density) {
+        // weight density per (channel, correlation) at the record's nearest cell
+        const long long plane_sz = (long long)a.g.nx * a.g.ny;
+        for (int p = 0; p < a.v.npol; p++) {
+            long long key = -1;
+            float w = 0.0f;
+            if (grid && !vis_flag(a.v, row, chan, p)) {
+                key = ((long long)c_local * a.v.npol + p) * plane_sz + dcell;
+                w = a.v.weight[row * a.v.w_rs + p];
+            }
+            wave_run_atomic_add(a.density, key, w);
+        }
     }
-    // statistics
-    const uint64_t g = __ballot(in && key < a.nbuckets);
+    const uint64_t g = __ballot(grid);
     unsigned long long s = pts;
     for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
     if ((threadIdx.x & 63) == 0 && g) {

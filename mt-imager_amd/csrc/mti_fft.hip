@@ -107,25 +107,27 @@ __device__ C *lds_ifft(C *a, C *b, int n, int logn, int nb, const double2 *tw, i
 // the PreIFFTUVCorrection reorder (ix = (gx - nx/2) mod nx,
 // iy = (gy - ny/2) mod ny) into X[c][p][iy][kx] (element type C).
 // MODE 1: plain in-place rows of [c][y][x] (C = float2 only).
-template <int MODE, typename C>
-__global__ __launch_bounds__(FFT_THREADS) void k_fft_rows(const float2 *in, C *out, int nx, int lognx,
-                                                         int ny, int P, const double2 *tw, int tw_mul)
+template <int MODE, typename C, typename TI>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_rows(const TI *in, C *out, int nx, int lognx,
+                                                         int ny, int P, const double2 *tw, int tw_mul,
+                                                         int64_t cs, int64_t ps)
 {
     extern __shared__ char smem[];
     C *a = (C *)smem, *b = a + nx;
     const int y = blockIdx.x, p = blockIdx.y, c = blockIdx.z;
     const int hx = nx >> 1;
     if (MODE == 0) {
-        const float2 *src = in + (((size_t)c * ny + y) * nx) * P + p;
+        // grid element (c, y, gx, p) at c*nx*ny*P + (y*nx + gx)*cs + p*ps
+        const TI *src = in + (size_t)c * nx * ny * P + (size_t)y * nx * cs + (size_t)p * ps;
         for (int ix = threadIdx.x; ix < nx; ix += blockDim.x) {
             const int gx = (ix + hx) & (nx - 1);
-            const float2 v = src[(size_t)gx * P];
+            const TI v = src[(size_t)gx * cs];
             a[ix] = cmk<C>(v.x, v.y);
         }
     } else {
-        const float2 *src = in + ((size_t)c * ny + y) * nx;
+        const TI *src = in + ((size_t)c * ny + y) * nx;
         for (int ix = threadIdx.x; ix < nx; ix += blockDim.x) {
-            const float2 v = src[ix];
+            const TI v = src[ix];
             a[ix] = cmk<C>(v.x, v.y);
         }
     }
@@ -315,16 +317,16 @@ static FftGeom geom(int nx, int ny)
     return g;
 }
 
-template <int MODE, typename C>
-static mti_status rows_pass(mti_ctx *ctx, const FftGeom &g, const float2 *in, C *out, int P, int nch,
-                            const double2 *tw, hipStream_t st)
+template <int MODE, typename C, typename TI>
+static mti_status rows_pass(mti_ctx *ctx, const FftGeom &g, const TI *in, C *out, int P, int nch,
+                            const double2 *tw, hipStream_t st, int64_t cs = 1, int64_t ps = 1)
 {
-    auto kern = k_fft_rows<MODE, C>;
+    auto kern = k_fft_rows<MODE, C, TI>;
     size_t lds = (size_t)2 * g.nx * sizeof(C);
     mti_status r = set_lds(ctx, kern, lds);
     if (r != MTI_OK) return r;
     hipLaunchKernelGGL(kern, dim3((unsigned)g.ny, (unsigned)P, (unsigned)nch), dim3(FFT_THREADS), lds, st, in,
-                       out, g.nx, g.lognx, g.ny, P, tw, g.NT / g.nx);
+                       out, g.nx, g.lognx, g.ny, P, tw, g.NT / g.nx, cs, ps);
     MTI_HIP(ctx, hipGetLastError());
     return MTI_OK;
 }
@@ -358,15 +360,17 @@ static mti_status cols2(mti_ctx *ctx, const FftGeom &g, const C *X, int nch, con
     return MTI_OK;
 }
 
-template <int P>
-static mti_status finalize_batch(mti_ctx *ctx, const FftGeom &g, const float2 *G, double2 *X, int nb,
+template <int P, typename TI>
+static mti_status finalize_batch(mti_ctx *ctx, const FftGeom &g, const TI *G, double2 *X, int nb,
                                  const double2 *tw, const double *nr, int poltype, const float *cx,
-                                 const float *cy, float *img, hipStream_t st)
+                                 const float *cy, float *img, int layout, hipStream_t st)
 {
     mti_status r;
+    const int64_t cs = layout == MTI_GRID_VUP ? P : 1;
+    const int64_t ps = layout == MTI_GRID_VUP ? 1 : (int64_t)g.nx * g.ny;
     {
         StageTimer tm(ctx, MTI_STAGE_FFT_ROWS, st);
-        r = rows_pass<0, double2>(ctx, g, G, X, P, nb, tw, st);
+        r = rows_pass<0, double2, TI>(ctx, g, G, X, P, nb, tw, st, cs, ps);
         if (r != MTI_OK) return r;
     }
     {
@@ -396,7 +400,7 @@ extern "C" mti_status mti_fft2d_inverse(mti_ctx *ctx, float *data, int32_t nx, i
     mti_status r = twiddle_table(ctx, g.NT, &tw, st);
     if (r != MTI_OK) return r;
     float2 *X = (float2 *)data;
-    r = rows_pass<1, float2>(ctx, g, X, X, 1, nbatch, tw, st);
+    r = rows_pass<1, float2, float2>(ctx, g, X, X, 1, nbatch, tw, st);
     if (r != MTI_OK) return r;
     r = cols1<float2>(ctx, g, X, nbatch, tw, st);
     if (r != MTI_OK) return r;
@@ -411,8 +415,8 @@ extern "C" mti_status mti_fft2d_inverse(mti_ctx *ctx, float *data, int32_t nx, i
 }
 
 extern "C" mti_status mti_finalize(mti_ctx *ctx, const mti_image_params *p, int32_t npol, int32_t poltype,
-                                   int32_t nchan, float *grids, const double *norm, float *image,
-                                   void *stream)
+                                   int32_t nchan, int32_t grid_layout, int32_t grid_precision,
+                                   const void *grids, const double *norm, float *image, void *stream)
 {
     if (!ctx) return MTI_ERR_INVALID;
     mti_status r = validate_params(ctx, p);
@@ -420,6 +424,8 @@ extern "C" mti_status mti_finalize(mti_ctx *ctx, const mti_image_params *p, int3
     MTI_CHECK(ctx, npol == 1 || npol == 2 || npol == 4, "npol must be 1, 2 or 4");
     MTI_CHECK(ctx, poltype == MTI_POL_LINEAR || poltype == MTI_POL_CIRCULAR, "bad PolarizationType");
     MTI_CHECK(ctx, nchan >= 1 && grids && norm && image, "null pointer or nchan < 1");
+    MTI_CHECK(ctx, grid_layout == MTI_GRID_VUP || grid_layout == MTI_GRID_PVU, "bad grid_layout");
+    MTI_CHECK(ctx, grid_precision == MTI_GRID_C64 || grid_precision == MTI_GRID_C128, "bad grid_precision");
     // FP64 transform: the row pass holds 2 rows of nx complex128 in LDS.
     if (!is_pow2(p->nx) || !is_pow2(p->ny) || p->nx > 4096 || p->ny > 8192)
         return set_error(ctx, MTI_ERR_UNSUPPORTED,
@@ -444,15 +450,26 @@ extern "C" mti_status mti_finalize(mti_ctx *ctx, const mti_image_params *p, int3
     MTI_HIP(ctx, hipGetLastError());
     for (int c0 = 0; c0 < nchan; c0 += batch) {
         const int nb = std::min(batch, nchan - c0);
-        const float2 *G = (const float2 *)grids + (size_t)c0 * p->nx * p->ny * npol;
+        const size_t off = (size_t)c0 * p->nx * p->ny * npol;
         float *img = image + (size_t)c0 * npol * p->nx * p->ny;
         const double *nr = norm + (size_t)c0 * npol;
-        if (npol == 1)
-            r = finalize_batch<1>(ctx, g, G, X, nb, tw, nr, poltype, corr_x, corr_y, img, st);
-        else if (npol == 2)
-            r = finalize_batch<2>(ctx, g, G, X, nb, tw, nr, poltype, corr_x, corr_y, img, st);
-        else
-            r = finalize_batch<4>(ctx, g, G, X, nb, tw, nr, poltype, corr_x, corr_y, img, st);
+        if (grid_precision == MTI_GRID_C64) {
+            const float2 *G = (const float2 *)grids + off;
+            if (npol == 1)
+                r = finalize_batch<1>(ctx, g, G, X, nb, tw, nr, poltype, corr_x, corr_y, img, grid_layout, st);
+            else if (npol == 2)
+                r = finalize_batch<2>(ctx, g, G, X, nb, tw, nr, poltype, corr_x, corr_y, img, grid_layout, st);
+            else
+                r = finalize_batch<4>(ctx, g, G, X, nb, tw, nr, poltype, corr_x, corr_y, img, grid_layout, st);
+        } else {
+            const double2 *G = (const double2 *)grids + off;
+            if (npol == 1)
+                r = finalize_batch<1>(ctx, g, G, X, nb, tw, nr, poltype, corr_x, corr_y, img, grid_layout, st);
+            else if (npol == 2)
+                r = finalize_batch<2>(ctx, g, G, X, nb, tw, nr, poltype, corr_x, corr_y, img, grid_layout, st);
+            else
+                r = finalize_batch<4>(ctx, g, G, X, nb, tw, nr, poltype, corr_x, corr_y, img, grid_layout, st);
+        }
         if (r != MTI_OK) return r;
     }
     return MTI_OK;

@@ -1,5 +1,5 @@
 // mti_grid.hpp -- structures shared by the binning (mti_bin.hip, built with
-// -ffp-contract=off) and the tile gridder (mti_grid.hip).
+// -ffp-contract=off) and the gridder (mti_grid.hip).
 #pragma once
 #include "mti_internal.hpp"
 
@@ -37,14 +37,10 @@ struct BinGeom {
 struct BinPassArgs {
     VisView v;
     BinGeom g;
-    int chan0, nb;           // channels in this call
-    int TS, ntx, ntiles;     // tiling
-    uint32_t nbuckets;       // nb * ntiles; key nbuckets = not gridded
-    uint32_t *keys;
-    int32_t *vals;
-    int4 *desc;
-    uint32_t *counts;        // [nbuckets]
-    float *density;          // [nb][npol][ny][nx] or null
+    int chan0, nb;
+    uint32_t *valid;            // [nrec] 1 if the record is gridded
+    int4 *desc;                 // [nrec]
+    float *density;             // [nb][npol][ny][nx] or null (uniform/Briggs)
     unsigned long long *stats;  // [0] gridded records, [1] gridded points
 };
 

@@ -64,6 +64,8 @@ enum ScratchSlot {
     SLOT_FFT = 7,
     SLOT_RADIX = 8,
     SLOT_NORM = 9,
+    SLOT_GRID64 = 10,
+    SLOT_MISC2 = 11,
     SLOT_COUNT = 12
 };
 

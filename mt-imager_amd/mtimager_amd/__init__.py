@@ -15,6 +15,8 @@ MODE = {"normal": 0, "wproj": 1}
 PRECISION = {"f32": 0, "f16": 1, "bf16": 2}
 WEIGHTING = {"natural": 0, "uniform": 1, "briggs": 2}
 POLTYPE = {"linear": 0, "circular": 1}
+LAYOUT = {"vup": 0, "pvu": 1}
+GRID_PRECISION = {"c64": 0, "c128": 1}
 ARCSEC = 3.141592653589793 / 180.0 / 3600.0
 
 
@@ -55,13 +57,13 @@ class Vis(ctypes.Structure):
 class GridOpts(ctypes.Structure):
     _fields_ = [("chan0", ctypes.c_int32), ("nchan", ctypes.c_int32),
                 ("accumulate", ctypes.c_int32), ("psf", ctypes.c_int32),
-                ("weighting", ctypes.c_int32), ("robust", ctypes.c_double)]
+                ("weighting", ctypes.c_int32), ("robust", ctypes.c_double),
+                ("grid_layout", ctypes.c_int32), ("grid_precision", ctypes.c_int32)]
 
 
 class GridStats(ctypes.Structure):
     _fields_ = [("gridded_records", ctypes.c_int64), ("gridded_points", ctypes.c_int64),
-                ("work_items", ctypes.c_int64), ("tile", ctypes.c_int32),
-                ("path", ctypes.c_int32)]
+                ("work_items", ctypes.c_int64), ("flushes", ctypes.c_int64)]
 
 
 # Every symbol include/mtimager_amd.h declares.
@@ -122,7 +124,7 @@ def lib():
     L.mti_fft2d_inverse.argtypes = [vp, vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, vp]
     L.mti_finalize.restype = ctypes.c_int
     L.mti_finalize.argtypes = [vp, P(ImageParams), ctypes.c_int32, ctypes.c_int32,
-                               ctypes.c_int32, vp, vp, vp, vp]
+                               ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, vp, vp, vp, vp]
     _lib = L
     return L
 
@@ -203,9 +205,15 @@ class Context:
 
     # ---- a4-a7, a15, a21 --------------------------------------------------
     def grid(self, kern, params, vis, grids, norm, chan0=0, nchan=1, accumulate=False,
-             psf=False, weighting="natural", robust=0.0, stats=True, stream=None):
+             psf=False, weighting="natural", robust=0.0, layout="vup", precision=None,
+             stats=True, stream=None):
+        """grids: per channel [ny][nx][npol] (layout "vup", the reference's
+        Grid) or [npol][ny][nx] ("pvu"); complex64 or complex128 (precision
+        "c64"/"c128", taken from the tensor dtype by default)."""
+        if precision is None:
+            precision = "c128" if str(grids.dtype) in ("torch.complex128", "torch.float64") else "c64"
         o = GridOpts(int(chan0), int(nchan), 1 if accumulate else 0, 1 if psf else 0,
-                     WEIGHTING[weighting], float(robust))
+                     WEIGHTING[weighting], float(robust), LAYOUT[layout], GRID_PRECISION[precision])
         st = GridStats()
         rc = lib().mti_grid(self.h, kern.h, ctypes.byref(params), ctypes.byref(vis),
                             ctypes.byref(o), grids.data_ptr(), norm.data_ptr(),
@@ -221,10 +229,13 @@ class Context:
 
     # ---- a16-a19 ----------------------------------------------------------
     def finalize(self, params, npol, grids, norm, image, nchan=1, poltype="linear",
-                 stream=None):
+                 layout="vup", precision=None, stream=None):
+        if precision is None:
+            precision = "c128" if str(grids.dtype) in ("torch.complex128", "torch.float64") else "c64"
         rc = lib().mti_finalize(self.h, ctypes.byref(params), int(npol), POLTYPE[poltype],
-                                int(nchan), grids.data_ptr(), norm.data_ptr(),
-                                image.data_ptr(), _stream(stream))
+                                int(nchan), LAYOUT[layout], GRID_PRECISION[precision],
+                                grids.data_ptr(), norm.data_ptr(), image.data_ptr(),
+                                _stream(stream))
         self.check(rc, "mti_finalize")
 
 

@@ -142,13 +142,15 @@ GRID_RMS_TOL = 1e-5
 
 
 @pytest.mark.parametrize("psf", [False, True])
-def test_config1_dirty_image_and_psf(orc, mti, ctx, c1, c1_params, psf):
+@pytest.mark.parametrize("precision,layout", [("c64", "vup"), ("c128", "pvu")])
+def test_config1_dirty_image_and_psf(orc, mti, ctx, c1, c1_params, psf, precision, layout):
     dev = H.Dev(c1, torch)
     idx = H.gpu_sort(ctx, dev, c1.nant, torch)
     rows = idx.cpu().numpy()
     K = ctx.build_kernels(c1_params)
     OK = orc.Kernels(H.oracle_params(orc, c1_params))
-    g = H.gpu_image(mti, ctx, K, c1_params, dev, torch, [0, 1], idx, psf=psf)
+    g = H.gpu_image(mti, ctx, K, c1_params, dev, torch, [0, 1], idx, psf=psf, precision=precision,
+                    layout=layout)
     ng = 0
     for ci, chan in enumerate([0, 1]):
         r = H.oracle_image(orc, OK, H.oracle_params(orc, c1_params), c1, chan, rows, psf=psf)
@@ -156,7 +158,7 @@ def test_config1_dirty_image_and_psf(orc, mti, ctx, c1, c1_params, psf):
         np.testing.assert_allclose(g["norm"][ci], r["norm"], rtol=1e-6)
         assert H.rel_rms(g["grid"][ci], r["grid"]) < GRID_RMS_TOL
         e = H.image_errors(g["image"][ci][0], r["image"][0])
-        print(f"chan {chan} psf={psf} stokes I: {e}")
+        print(f"chan {chan} psf={psf} {precision}/{layout} stokes I: {e}")
         assert e["rel_rms"] < IMAGE_RMS_TOL
         ref_i = np.sqrt(np.mean(r["image"][0] ** 2))
         for q in range(1, 4):  # Q, U, V relative to Stokes I
@@ -208,6 +210,21 @@ def test_batch_of_channels_equals_single_calls(mti, ctx, c1, c1_params):
         s = H.gpu_image(mti, ctx, K, c1_params, dev, torch, [chan], idx)
         assert H.rel_rms(b["grid"][ci], s["grid"][0]) < 1e-6
         np.testing.assert_allclose(b["norm"][ci], s["norm"][0], rtol=1e-12)
+
+
+def test_planar_grid_layout_matches(orc, mti, ctx, c1, c1_params):
+    """[P][v][u] grids (what PreIFFTUVCorrection reorders to) give the same
+    grid and image as the reference's [v][u][P] layout."""
+    dev = H.Dev(c1, torch)
+    idx = H.gpu_sort(ctx, dev, c1.nant, torch)
+    K = ctx.build_kernels(c1_params)
+    a = H.gpu_image(mti, ctx, K, c1_params, dev, torch, [4], idx, layout="vup")
+    b = H.gpu_image(mti, ctx, K, c1_params, dev, torch, [4], idx, layout="pvu")
+    assert H.rel_rms(b["grid"][0], a["grid"][0]) < 1e-6
+    assert H.rel_rms(b["image"][0], a["image"][0]) < 1e-6
+    r = H.oracle_image(orc, orc.Kernels(H.oracle_params(orc, c1_params)),
+                       H.oracle_params(orc, c1_params), c1, 4, idx.cpu().numpy())
+    assert H.rel_rms(b["image"][0][0], r["image"][0]) < IMAGE_RMS_TOL
 
 
 def test_accumulate_over_chunks(mti, ctx, c1, c1_params):
