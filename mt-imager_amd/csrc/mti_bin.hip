@@ -15,6 +15,8 @@
 // in LDS, which gives the same sum.
 #include "mti_grid.hpp"
 
+#include <algorithm>
+
 namespace mti {
 
 struct BinOut {
@@ -102,77 +104,62 @@ __global__ void k_bin_only(VisView v, BinGeom g, int chan, int32_t *out)
     o[7] = b.conj;
 }
 
-// Wave-level segmented sum of v over runs of equal key in lane order; the
-// last lane of every run holding a valid key adds the run's sum with one
-// atomic.  Consecutive records of one baseline usually share a cell, so a wave
-// issues a few atomics instead of 64.
-__device__ __forceinline__ void wave_run_atomic_add(float *base, long long key, float v)
+__device__ __forceinline__ void bin_one(const BinPassArgs &a, int64_t t, unsigned long long &ngrid,
+                                        unsigned long long &npts)
 {
-    const int lane = threadIdx.x & 63;
-    const long long prev = __shfl_up(key, 1, 64);
-    const long long next = __shfl_down(key, 1, 64);
-    int head = (lane == 0) || (prev != key);
-    const bool tail = (lane == 63) || (next != key);
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const float vu = __shfl_up(v, d, 64);
-        const int hu = __shfl_up(head, d, 64);
-        if (lane >= d && !head) {
-            v += vu;
-            head |= hu;
+    const int64_t i = t / a.nb;
+    const int c_local = (int)(t - i * a.nb);
+    const int64_t e = (int64_t)c_local * a.v.nidx + i;
+    const int64_t row = a.v.row_index ? (int64_t)a.v.row_index[i] : i;
+    const int chan = a.chan0 + c_local;
+    BinOut b = bin_record(a.v, a.g, row, chan);
+    int4 d = make_int4((int)row, 0, 0, 0);
+    if (b.to_grid) {
+        d.y = (b.cy << 16) | b.cx;
+        d.z = b.conv_idx;
+        d.w = (b.plane << 16) | (b.conj << 15) | b.S;
+        ngrid += 1;
+        npts += (unsigned long long)b.S * b.S * a.v.npol;
+        const int P = a.v.npol;
+        for (int p = 0; p < P; p++) {
+            const bool fl = vis_flag(a.v, row, chan, p);
+            a.wt[e * P + p] = fl ? 0.0f : a.v.weight[row * a.v.w_rs + p];
+            float2 vv = make_float2(1.0f, 0.0f);
+            if (!a.psf) vv = *(const float2 *)(a.v.vis + 2 * (row * a.v.vis_rs + (int64_t)chan * a.v.vis_cs + p));
+            if (b.conj) vv.y = -vv.y;
+            a.wv[e * P + p] = vv;
         }
     }
-    if (tail && key >= 0 && v != 0.0f) unsafeAtomicAdd(base + key, v);
+    a.valid[e] = b.to_grid ? 1u : 0u;
+    a.desc[e] = d;
 }
 
-// Record e = c_local * nidx + i (channel-major, so a wave sees consecutive
-// records of one channel in baseline-major order).
+// Threads run over (row, channel) with the channel fastest, so a wave reads
+// one row's flags and visibilities contiguously (MeasurementSet order); the
+// record e = c_local * nidx + i is written channel-major (baseline-major
+// within a channel).  Gridded records also get their visibilities (PSF: 1+0i,
+// imaginary part negated for w < 0) and flag-masked weights, which the
+// compaction moves next to the descriptors.  Grid-stride loop: the gridded
+// record / point counts are reduced per block and added once.
 __global__ __launch_bounds__(256) void k_bin_pass(BinPassArgs a, int64_t nrec)
 {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool in = e < nrec;
-    bool grid = false;
-    unsigned long long pts = 0;
-    long long dcell = -1;
-    int64_t row = 0;
-    int chan = 0, c_local = 0;
-    if (in) {
-        c_local = (int)(e / a.v.nidx);
-        const int64_t i = e - (int64_t)c_local * a.v.nidx;
-        row = a.v.row_index ? (int64_t)a.v.row_index[i] : i;
-        chan = a.chan0 + c_local;
-        BinOut b = bin_record(a.v, a.g, row, chan);
-        int4 d = make_int4((int)row, 0, 0, 0);
-        if (b.to_grid) {
-            grid = true;
-            d.y = (b.cy << 16) | b.cx;
-            d.z = b.conv_idx;
-            d.w = (b.plane << 16) | (b.conj << 15) | b.S;
-            pts = (unsigned long long)b.S * b.S * a.v.npol;
-            dcell = (long long)(b.cy + b.S / 2) * a.g.nx + (b.cx + b.S / 2);
-        }
-        a.valid[e] = grid ? 1u : 0u;
-        a.desc[e] = d;
+    __shared__ unsigned long long s_st[4][2];
+    unsigned long long ngrid = 0, npts = 0;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nrec; t += (int64_t)gridDim.x * blockDim.x)
+        bin_one(a, t, ngrid, npts);
+    for (int o = 32; o >= 1; o >>= 1) {
+        ngrid += __shfl_xor(ngrid, o, 64);
+        npts += __shfl_xor(npts, o, 64);
     }
-    if (a.density) {
-        // weight density per (channel, correlation) at the record's nearest cell
-        const long long plane_sz = (long long)a.g.nx * a.g.ny;
-        for (int p = 0; p < a.v.npol; p++) {
-            long long key = -1;
-            float w = 0.0f;
-            if (grid && !vis_flag(a.v, row, chan, p)) {
-                key = ((long long)c_local * a.v.npol + p) * plane_sz + dcell;
-                w = a.v.weight[row * a.v.w_rs + p];
-            }
-            wave_run_atomic_add(a.density, key, w);
-        }
+    if ((threadIdx.x & 63) == 0) {
+        s_st[threadIdx.x >> 6][0] = ngrid;
+        s_st[threadIdx.x >> 6][1] = npts;
     }
-    const uint64_t g = __ballot(grid);
-    unsigned long long s = pts;
-    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
-    if ((threadIdx.x & 63) == 0 && g) {
-        atomicAdd(&a.stats[0], (unsigned long long)__popcll(g));
-        atomicAdd(&a.stats[1], s);
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const unsigned long long v = s_st[0][threadIdx.x] + s_st[1][threadIdx.x] + s_st[2][threadIdx.x] +
+                                     s_st[3][threadIdx.x];
+        if (v) atomicAdd(&a.stats[threadIdx.x], v);
     }
 }
 
@@ -222,7 +209,8 @@ BinGeom make_geom(const mti_kernels *k, const mti_image_params *p)
 mti_status launch_bin_pass(mti_ctx *ctx, const BinPassArgs &a, int64_t nrec, hipStream_t st)
 {
     if (nrec <= 0) return MTI_OK;
-    hipLaunchKernelGGL(k_bin_pass, dim3((unsigned)ceil_div(nrec, 256)), dim3(256), 0, st, a, nrec);
+    const int64_t blocks = std::min<int64_t>(ceil_div(nrec, 256), (int64_t)ctx->num_cus * 16);
+    hipLaunchKernelGGL(k_bin_pass, dim3((unsigned)blocks), dim3(256), 0, st, a, nrec);
     MTI_HIP(ctx, hipGetLastError());
     return MTI_OK;
 }

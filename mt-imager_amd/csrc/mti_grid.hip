@@ -4,27 +4,28 @@
 // grid[corner_y+sy][corner_x+sx][p] += K[conv_idx + sy*S + sx] * V_p, with
 // V_p = vis_p * w_p * (1-flag_p) and the imaginary part negated for w<0
 // (op/common.hcu:80-104); the normaliser is the sum of sums[k][o] * w_p *
-// (1-flag_p) (op/WProjPreProcessUVW.cu:84-108, comp/WImager.cpp:196-197).  Its
-// thread (bu,bv) owns the grid points congruent to (bu,bv) mod S and keeps a
-// register accumulator while consecutive records leave its point unchanged
-// (:102-129, :394-441), fed by a work plan of <= 440-record runs of equal
-// support (AccumulatedSum, CreateIndexAndReorder, ValueChangeDetect,
-// CreateBlockDataIndex, NotSameSupportManipulate).
+// (1-flag_p) (op/WProjPreProcessUVW.cu:84-108, comp/WImager.cpp:196-197).  The
+// reference keeps per-thread register accumulators fed by a work plan of runs
+// of equal support (AccumulatedSum, CreateIndexAndReorder, ValueChangeDetect,
+// CreateBlockDataIndex, NotSameSupportManipulate) and flushes with float
+// atomics to the grid in HBM.
 //
-// MI355X design:
-//   1. binning pass (mti_bin.hip): one record per (row, channel) in
-//      channel-major, baseline-major order; FP64 bins; weight density for
-//      uniform/Briggs with wave-segmented atomics;
-//   2. stable compaction of gridded records (exclusive scan + scatter) and
-//      the work list: items are runs of <= 512 records of one channel with
-//      equal support (scan of boundary flags);
-//   3. gridder: persistent 256-thread workgroups; an item's records (weighted
-//      visibilities, corner, kernel offset) are staged in LDS once; every
-//      lane owns tap classes (bu,bv) mod S -- several sub-streams of the run
-//      in parallel when S^2 is small, up to 5 classes per lane when it is
-//      large -- and accumulates in registers while its grid point is
-//      unchanged, flushing to HBM with float atomics only when it moves.
-//      The planar grid layout makes a wave's flush one contiguous segment.
+// MI355X design (tile ownership):
+//   1. binning pass (mti_bin.hip): one record per (row, channel), FP64 bins;
+//   2. stable compaction of gridded records (exclusive scan + scatter);
+//   3. counting sort of (record, tile) pairs: every record is listed under each
+//      32x32-cell tile its support box touches (wave-aggregated counters, one
+//      scan, scatter); a tile's list is cut into work units of <= TG_CHUNK
+//      records;
+//   4. uniform/Briggs: the weight density is built the same way, per tile in
+//      LDS, from the records whose nearest cell lies in the tile;
+//   5. gridder: one 512-thread workgroup per unit holds the tile of every
+//      correlation in LDS as FP64 (ds_add_f64), stages records in batches
+//      (weighted visibility, clipped box, kernel offset) and lets each wave
+//      take one record at a time with its lanes over the clipped taps; the tile
+//      is then added to the FP64 grid once -- plain read-modify-write when the
+//      unit is the tile's only one, FP64 atomics otherwise.
+// HBM sees each touched grid cell once per unit instead of once per tap.
 #include "mti_grid.hpp"
 
 #include <hip/hip_bf16.h>
@@ -35,32 +36,39 @@
 
 namespace mti {
 
-constexpr int RG_THREADS = 256;
-constexpr int RG_RECS = 512;     // records per item
-constexpr int RG_JMAX = 5;       // tap classes per lane
-constexpr int RG_CLASSES = RG_THREADS * RG_JMAX;
+constexpr int TG_T = 32;             // tile side in cells
+constexpr int TG_TT = TG_T * TG_T;
+constexpr int TG_THREADS = 512;      // 8 waves
+constexpr int TG_WAVES = TG_THREADS / 64;
+constexpr int TG_BATCH = 128;        // records staged per round
+constexpr int TG_LS = TG_T + 1;      // LDS row stride of the tile (cells)
+static_assert(TG_T * TG_LS < 2048 && TG_T < 64, "packed record fields");
+constexpr int TG_CHUNK = 2048;       // records per work unit
 
-struct RomeinArgs {
+struct TileArgs {
     VisView v;
     int chan0;
-    const int4 *cdesc;              // compacted record descriptors [M]
-    const int32_t *cchan;           // channel (local) of each compacted record
-    const uint32_t *item_start;     // [NI]
-    const uint32_t *n_items;        // device scalar NI
-    const uint32_t *n_rec;          // device scalar M
+    const int4 *cdesc;          // compacted record descriptors
+    const int32_t *cchan;       // local channel of each compacted record
+    const uint32_t *n_rec;      // device scalar: compacted records
+    float2 *wvis;               // [M][P] visibilities, weighted in place by k_record_prep
+    float *wt;                  // [M][P] natural weights, 0 where flagged
+    const uint32_t *pairs;      // record index per (tile, record) pair, grouped by tile
+    const int4 *units;          // {key, first pair, pairs, 1 if the tile's only unit}
+    const uint32_t *n_units;    // device scalar
     const void *table;
     const float *sums;
     const int2 *convinfo;
     int T2;
-    int nx, ny;
-    int64_t cs, ps;                 // complex strides of cell and correlation
-    int64_t chan_stride;            // complex elements per channel grid
-    double *grid;                   // FP64 accumulation grid
-    double *norm;                   // [nb][P]
+    int nx, ny, ntx, nty;
+    int64_t cs, ps;             // complex strides of cell and correlation
+    int64_t chan_stride;        // complex elements per channel grid
+    double *grid;               // FP64 accumulation grid
+    double *norm;               // [nb][P]
     int psf;
     int weighting;
-    const float *density;           // [nb][P][ny][nx]
-    const double *f2;               // [nb][P]
+    float *density;             // [nb][ny][nx][P] (written by the density pass)
+    const double *f2;           // [nb][P]
 };
 
 template <int PREC>
@@ -72,194 +80,418 @@ __device__ __forceinline__ float2 ksample(const void *table, int idx)
     return make_float2(__bfloat162float(h.x), __bfloat162float(h.y));
 }
 
-// Effective weight of correlation p of a record (flag, uniform/Briggs).
-__device__ __forceinline__ float eff_weight(const RomeinArgs &a, int64_t row, int chan, int c_local,
-                                            int p, int cell_x, int cell_y)
+__device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// ---- tile lists ---------------------------------------------------------------
+
+struct TileRange {
+    int tx0, ty0, ntx, nty;  // tiles touched by a support box
+};
+
+__device__ __forceinline__ TileRange tile_range(int4 d)
 {
-    if (vis_flag(a.v, row, chan, p)) return 0.0f;
-    float w = a.v.weight[row * a.v.w_rs + p];
-    if (a.weighting == MTI_WEIGHT_NATURAL) return w;
-    const int npol = a.v.npol;
-    const float d = a.density[(((int64_t)c_local * npol + p) * a.ny + cell_y) * a.nx + cell_x];
-    if (a.weighting == MTI_WEIGHT_UNIFORM) return d > 0.0f ? w / d : 0.0f;
-    return (float)((double)w / (1.0 + a.f2[c_local * npol + p] * (double)d));
+    const int cx = d.y & 0xffff, cy = (int)((unsigned)d.y >> 16), S = d.w & 0x7fff;
+    TileRange t;
+    t.tx0 = cx / TG_T;
+    t.ty0 = cy / TG_T;
+    t.ntx = S > 0 ? (cx + S - 1) / TG_T - t.tx0 + 1 : 0;  // a plane without samples grids nothing
+    t.nty = S > 0 ? (cy + S - 1) / TG_T - t.ty0 + 1 : 0;
+    return t;
 }
 
-template <int P>
-__device__ __forceinline__ void flush_point(double *g, int cell, int64_t cs, int64_t ps, const double *acc)
+// Runs of equal keys in consecutive lanes: head lanes and their run lengths.
+struct WaveRuns {
+    bool head;
+    int len;   // valid on head lanes
+    int hl;    // head lane of this lane's run
+};
+
+__device__ __forceinline__ WaveRuns wave_runs(int key)
 {
-    if (cell < 0) return;
-#pragma unroll
-    for (int p = 0; p < P; p++) {
-        double *c = g + 2 * ((int64_t)cell * cs + (int64_t)p * ps);
-        if (acc[2 * p] != 0.0) unsafeAtomicAdd(c, acc[2 * p]);
-        if (acc[2 * p + 1] != 0.0) unsafeAtomicAdd(c + 1, acc[2 * p + 1]);
+    const int lane = threadIdx.x & 63;
+    const int prev = __shfl_up(key, 1, 64);
+    WaveRuns r;
+    r.head = (lane == 0) || (prev != key);
+    const uint64_t heads = __ballot(r.head);
+    const uint64_t above = (lane == 63) ? 0ull : (heads & ~((2ull << lane) - 1ull));
+    const int end = above ? (__ffsll((unsigned long long)above) - 1) : 64;
+    r.len = end - lane;
+    const uint64_t upto = (lane == 63) ? heads : (heads & ((2ull << lane) - 1ull));
+    r.hl = 63 - __clzll((long long)upto);
+    return r;
+}
+
+// Key of tile (tx, ty) of local channel c.
+__device__ __forceinline__ int tile_key(int c, int ty, int tx, int nty, int ntx)
+{
+    return (c * nty + ty) * ntx + tx;
+}
+
+// Pass 0: count the pairs of every tile.  Pass 1: scatter record indices.
+template <int PASS>
+__global__ __launch_bounds__(256) void k_tile_pairs(const int4 *cdesc, const int32_t *cchan, const uint32_t *n_rec,
+                                                    int64_t cap, int nty, int ntx, uint32_t *cnt,
+                                                    const uint32_t *start, uint32_t *pairs)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if ((int64_t)blockIdx.x * blockDim.x >= (int64_t)*n_rec) return;  // whole block past the end
+    const bool in = j < cap && j < (int64_t)*n_rec;
+    TileRange t{0, 0, 0, 0};
+    int c = 0;
+    if (in) {
+        t = tile_range(cdesc[j]);
+        c = cchan[j];
+    }
+    const int nq = t.ntx * t.nty;
+    int qmax = nq;
+    for (int o = 32; o >= 1; o >>= 1) qmax = max(qmax, __shfl_xor(qmax, o, 64));
+    for (int q = 0; q < qmax; q++) {
+        int key = -1;
+        if (q < nq) {
+            const int dy = q / t.ntx, dx = q - dy * t.ntx;
+            key = tile_key(c, t.ty0 + dy, t.tx0 + dx, nty, ntx);
+        }
+        const WaveRuns r = wave_runs(key);
+        if (PASS == 0) {
+            if (r.head && key >= 0) atomicAdd(&cnt[key], (uint32_t)r.len);
+        } else {
+            uint32_t base = 0;
+            if (r.head && key >= 0) base = atomicAdd(&cnt[key], (uint32_t)r.len);
+            base = __shfl(base, r.hl, 64);
+            if (key >= 0) pairs[start[key] + base + (uint32_t)((threadIdx.x & 63) - r.hl)] = (uint32_t)j;
+        }
     }
 }
 
-// One work unit = (item, class chunk).  JMAX = tap classes per lane; UNR =
-// records whose kernel samples are loaded ahead of the accumulation.
-template <int P, int PREC, int JMAX, int UNR>
-__global__ __launch_bounds__(RG_THREADS) void k_grid_romein(RomeinArgs a, const uint32_t *unit_prefix,
-                                                           const uint32_t *n_units)
+// Units per tile (ceil(pairs / TG_CHUNK)); start is the exclusive scan of
+// the pair counts, total the sum.
+__global__ void k_unit_counts(const uint32_t *start, const uint32_t *total, int64_t nkeys, uint32_t *nunits)
 {
-    __shared__ int4 s_rec[RG_RECS];            // corner cell, kernel offset, corner mod S
-    __shared__ double s_vis[2 * P][RG_RECS];   // weighted visibility
-    __shared__ double s_norm[RG_THREADS / 64][P];
-    __shared__ int s_hdr[6];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint32_t NU = *n_units, NI = *a.n_items, M = *a.n_rec;
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nkeys) return;
+    const uint32_t e = (k + 1 < nkeys) ? start[k + 1] : *total;
+    nunits[k] = (e - start[k] + TG_CHUNK - 1) / TG_CHUNK;
+}
 
-    for (uint32_t unit = blockIdx.x; unit < NU; unit += gridDim.x) {
-        if (tid == 0) {
-            uint32_t lo = 0, hi = NI - 1;  // last item with unit_prefix <= unit
-            while (lo < hi) {
-                uint32_t mid = (lo + hi + 1) >> 1;
-                if (unit_prefix[mid] <= unit) lo = mid;
-                else hi = mid - 1;
-            }
-            const uint32_t st = a.item_start[lo];
-            const uint32_t en = (lo + 1 < NI) ? a.item_start[lo + 1] : M;
-            s_hdr[0] = (int)st;
-            s_hdr[1] = (int)(en - st);
-            s_hdr[2] = (int)(unit - unit_prefix[lo]);  // class chunk
-            s_hdr[3] = a.cdesc[st].w & 0x7fff;          // support
-            s_hdr[4] = a.cchan[st];                     // channel (local)
-        }
+__global__ void k_unit_list(const uint32_t *start, const uint32_t *total, const uint32_t *uprefix,
+                            const uint32_t *n_units, int64_t nkeys, int4 *units)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nkeys) return;
+    const uint32_t s = start[k];
+    const uint32_t e = (k + 1 < nkeys) ? start[k + 1] : *total;
+    const uint32_t u0 = uprefix[k];
+    const uint32_t u1 = (k + 1 < nkeys) ? uprefix[k + 1] : *n_units;
+    const int excl = (u1 - u0) == 1;
+    for (uint32_t u = u0; u < u1; u++) {
+        const uint32_t f = s + (u - u0) * TG_CHUNK;
+        units[u] = make_int4((int)k, (int)f, (int)min((uint32_t)TG_CHUNK, e - f), excl);
+    }
+}
+
+// ---- weight density (uniform / Briggs) ----------------------------------------
+
+// Per unit: the weights of the records whose nearest cell (box centre) lies in
+// the tile, summed per correlation in LDS and added to the density grid
+// [channel][v][u][P].
+template <int P>
+__global__ __launch_bounds__(256) void k_density_tiles(TileArgs a)
+{
+    __shared__ float s_d[TG_TT * P];
+    const int tid = threadIdx.x;
+    const uint32_t NU = *a.n_units;
+    for (uint32_t u = blockIdx.x; u < NU; u += gridDim.x) {
+        const int4 un = a.units[u];
+        const int key = un.x;
+        const int tx = key % a.ntx, t2 = key / a.ntx, ty = t2 % a.nty, c_local = t2 / a.nty;
+        const int x0 = tx * TG_T, y0 = ty * TG_T;
+        for (int i = tid; i < P * TG_TT; i += 256) s_d[i] = 0.0f;
         __syncthreads();
-        const int st = s_hdr[0], n = s_hdr[1], chunk = s_hdr[2], S = s_hdr[3], c_local = s_hdr[4];
-        const int chan = a.chan0 + c_local;
-        const int S2 = S * S;
-        if (S == 0) {  // plane without kernel samples: contributes nothing
-            __syncthreads();
-            continue;
-        }
-
-        // ---- stage the item's records -------------------------------------
-        double np[P];
-#pragma unroll
-        for (int p = 0; p < P; p++) np[p] = 0.0;
-        for (int j = tid; j < n; j += RG_THREADS) {
-            const int4 d = a.cdesc[st + j];
-            const int64_t row = d.x;
-            const int cx = d.y & 0xffff, cy = (int)((unsigned)d.y >> 16);
-            const int plane = (int)((unsigned)d.w >> 16);
-            const bool conj = (d.w >> 15) & 1;
-            s_rec[j] = make_int4(cy * a.nx + cx, d.z, (cx % S) | ((cy % S) << 16), 0);
-            const int o = (d.z - a.convinfo[plane].y) / S2;
-            const double csum = (double)a.sums[plane * a.T2 + o];
+        for (int i = tid; i < un.z; i += 256) {
+            const uint32_t j = a.pairs[un.y + i];
+            const int4 d = a.cdesc[j];
+            const int S = d.w & 0x7fff;
+            const int mx = (d.y & 0xffff) + S / 2 - x0, my = (int)((unsigned)d.y >> 16) + S / 2 - y0;
+            if (mx < 0 || mx >= TG_T || my < 0 || my >= TG_T) continue;
 #pragma unroll
             for (int p = 0; p < P; p++) {
-                const float w = eff_weight(a, row, chan, c_local, p, cx + S / 2, cy + S / 2);
-                double re = 1.0, im = 0.0;
-                if (!a.psf) {
-                    const float2 vv =
-                        *(const float2 *)(a.v.vis + 2 * (row * a.v.vis_rs + (int64_t)chan * a.v.vis_cs + p));
-                    re = vv.x;
-                    im = vv.y;
-                }
-                s_vis[2 * p][j] = re * (double)w;
-                s_vis[2 * p + 1][j] = conj ? -(im * (double)w) : im * (double)w;
-                np[p] += csum * (double)w;
+                const float w = a.wt[(int64_t)j * P + p];
+                if (w != 0.0f) atomicAdd(&s_d[(my * TG_T + mx) * P + p], w);
             }
         }
-        if (chunk == 0) {
+        __syncthreads();
+        float *dch = a.density + (int64_t)c_local * a.nx * a.ny * P;
+        for (int i = tid; i < TG_TT * P; i += 256) {
+            const int cell = i / P, p = i - cell * P;
+            const int gx = x0 + (cell % TG_T), gy = y0 + cell / TG_T;
+            const float v = s_d[i];
+            if (v == 0.0f || gx >= a.nx || gy >= a.ny) continue;
+            float *o = dch + ((int64_t)gy * a.nx + gx) * P + p;
+            if (un.w) *o += v;
+            else unsafeAtomicAdd(o, v);
+        }
+        __syncthreads();
+    }
+}
+
+// ---- per-record weights -------------------------------------------------------
+
+// Per compacted record: effective weights (natural, uniform 1/W or Briggs
+// 1/(1 + f^2 W) at the record's nearest cell), visibilities weighted in place
+// and the normaliser sums[k][o] * w_p, reduced per block and channel.
+template <int P>
+__global__ __launch_bounds__(256) void k_record_prep(TileArgs a)
+{
+    const uint32_t M = *a.n_rec;
+    const int lane = threadIdx.x & 63;
+    double np[P];
+#pragma unroll
+    for (int p = 0; p < P; p++) np[p] = 0.0;
+    int c_prev = -1;
+    for (uint32_t j0 = blockIdx.x * 256u; j0 < M; j0 += gridDim.x * 256u) {
+        const uint32_t j = j0 + threadIdx.x;
+        // records are channel-major: partials of the previous channel are
+        // flushed when a block starts on a new one
+        const int c_first = a.cchan[j0];
+        if (c_first != c_prev && c_prev >= 0) {
 #pragma unroll
             for (int p = 0; p < P; p++) {
                 double v = np[p];
                 for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-                if (lane == 0) s_norm[wave][p] = v;
+                if (lane == 0 && v != 0.0) unsafeAtomicAdd(&a.norm[c_prev * P + p], v);
+                np[p] = 0.0;
+            }
+        }
+        c_prev = c_first;
+        if (j < M) {
+            const int4 d = a.cdesc[j];
+            const int c_local = a.cchan[j];
+            const int cx = d.y & 0xffff, cy = (int)((unsigned)d.y >> 16);
+            const int plane = (int)((unsigned)d.w >> 16);
+            const int S = d.w & 0x7fff;
+            const double csum = S > 0 ? (double)a.sums[plane * a.T2 + (d.z - a.convinfo[plane].y) / (S * S)] : 0.0;
+            float dens[P];
+            if (a.weighting != MTI_WEIGHT_NATURAL) {
+                const float *dp = a.density + (((int64_t)c_local * a.ny + cy + S / 2) * a.nx + cx + S / 2) * P;
+#pragma unroll
+                for (int p = 0; p < P; p++) dens[p] = dp[p];
+            }
+#pragma unroll
+            for (int p = 0; p < P; p++) {
+                float w = a.wt[(int64_t)j * P + p];
+                if (a.weighting == MTI_WEIGHT_UNIFORM)
+                    w = dens[p] > 0.0f ? w / dens[p] : 0.0f;
+                else if (a.weighting == MTI_WEIGHT_BRIGGS)
+                    w = (float)((double)w / (1.0 + a.f2[c_local * P + p] * (double)dens[p]));
+                const float2 vv = a.wvis[(int64_t)j * P + p];
+                a.wvis[(int64_t)j * P + p] = make_float2(vv.x * w, vv.y * w);
+                if (c_local == c_first) np[p] += csum * (double)w;
+                else unsafeAtomicAdd(&a.norm[c_local * P + p], csum * (double)w);
+            }
+        }
+    }
+    if (c_prev >= 0) {
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            double v = np[p];
+            for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+            if (lane == 0 && v != 0.0) unsafeAtomicAdd(&a.norm[c_prev * P + p], v);
+        }
+    }
+}
+
+// ---- gridder ------------------------------------------------------------------
+
+__device__ __forceinline__ double wave_uniform_f64(double v)
+{
+    const int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
+    const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+
+// A wave's walk over its records (wave, wave + 8, ...) pass by pass.  The
+// record geometry is wave-uniform; (lx, ly) is the lane's tap within a pass of
+// rpp rows of w lanes.  s_rec: x = kernel offset of the clipped box's first
+// tap, y = LDS cell | w << 11 | h << 17, z = S, w = rpp | npass << 8.
+struct TapCursor {
+    int r, q, npass, rpp, h, S;
+    int kb_lx, cell_lx, ly;   // per lane
+    bool lane_ok;             // per lane: ly < rpp
+};
+
+__device__ __forceinline__ void cursor_record(TapCursor &c, int r, int nbat, const int4 *s_rec, const float *s_lw,
+                                              int lane)
+{
+    c.r = r;
+    c.q = 0;
+    if (r >= nbat) return;
+    const int4 rec = s_rec[r];
+    const int kb = wave_uniform(rec.x), pk = wave_uniform(rec.y);
+    c.S = wave_uniform(rec.z);
+    const int zz = wave_uniform(rec.w);
+    const int w = (pk >> 11) & 63;
+    c.h = (pk >> 17) & 63;
+    c.rpp = zz & 255;
+    c.npass = (zz >> 8) & 255;
+    const float rcp = s_lw[r];
+    const int ly = (int)(((float)lane + 0.5f) * rcp), lx = lane - ly * w;
+    c.ly = ly;
+    c.lane_ok = ly < c.rpp;
+    c.kb_lx = kb + lx;
+    c.cell_lx = (pk & 2047) + lx;
+}
+
+__device__ __forceinline__ void cursor_next(TapCursor &c, int nbat, const int4 *s_rec, const float *s_lw, int lane)
+{
+    if (c.r >= nbat) return;
+    if (++c.q < c.npass) return;
+    cursor_record(c, c.r + TG_WAVES, nbat, s_rec, s_lw, lane);
+}
+
+// One in-flight tap: record, LDS cell, lane active, kernel sample.
+struct TapSlot {
+    int r, cell;
+    bool on;
+    float2 k;
+};
+
+template <int PREC>
+__device__ __forceinline__ TapSlot cursor_issue(const TapCursor &c, int nbat, const void *table)
+{
+    TapSlot t;
+    t.r = c.r;
+    const int y = c.ly + c.q * c.rpp;
+    t.on = (c.r < nbat) && c.lane_ok && (y < c.h);
+    t.cell = c.cell_lx + y * TG_LS;
+    t.k = ksample<PREC>(table, t.on ? c.kb_lx + y * c.S : 0);  // index 0: a valid dummy
+    return t;
+}
+
+template <int P, int PREC>
+__global__ __launch_bounds__(TG_THREADS) void k_grid_tiles(TileArgs a)
+{
+    __shared__ double s_acc[2 * P * TG_T * TG_LS];  // [p][re,im][y][x], rows padded to TG_LS
+    __shared__ int4 s_rec[TG_BATCH];
+    __shared__ float s_lw[TG_BATCH];
+    __shared__ double s_vd[2 * P][TG_BATCH];        // weighted visibility
+    constexpr int ACC = TG_T * TG_LS;
+    const int tid = threadIdx.x, lane = tid & 63, wave = wave_uniform(tid >> 6);
+    const uint32_t NU = *a.n_units;
+
+    for (uint32_t u = blockIdx.x; u < NU; u += gridDim.x) {
+        const int4 un = a.units[u];
+        const int key = un.x, first = un.y, npairs = un.z, excl = un.w;
+        const int tx = key % a.ntx, t2 = key / a.ntx, ty = t2 % a.nty, c_local = t2 / a.nty;
+        const int x0 = tx * TG_T, y0 = ty * TG_T;
+        for (int i = tid; i < 2 * P * ACC; i += TG_THREADS) s_acc[i] = 0.0;
+
+        for (int b0 = 0; b0 < npairs; b0 += TG_BATCH) {
+            const int nbat = min(TG_BATCH, npairs - b0);
+            __syncthreads();  // LDS zeroed / previous batch consumed
+            // staging: four threads per record (geometry; one correlation each)
+            {
+                const int rr = tid >> 2, part = tid & 3;
+                if (rr < nbat) {
+                    const uint32_t j = a.pairs[first + b0 + rr];
+                    if (part < P) {
+                        const float2 v = a.wvis[(int64_t)j * P + part];
+                        s_vd[2 * part][rr] = v.x;
+                        s_vd[2 * part + 1][rr] = v.y;
+                    }
+                    if (part == 0) {
+                        const int4 d = a.cdesc[j];
+                        const int cx = d.y & 0xffff, cy = (int)((unsigned)d.y >> 16);
+                        const int S = d.w & 0x7fff;
+                        const int ix0 = max(cx, x0), ix1 = min(cx + S, x0 + TG_T);
+                        const int iy0 = max(cy, y0), iy1 = min(cy + S, y0 + TG_T);
+                        const int w = ix1 - ix0, h = iy1 - iy0;
+                        const int rpp = 64 / w;
+                        const int npass = (h + rpp - 1) / rpp;
+                        const int kb = d.z + (iy0 - cy) * S + (ix0 - cx);
+                        const int lb = (iy0 - y0) * TG_LS + (ix0 - x0);
+                        s_rec[rr] = make_int4(kb, lb | (w << 11) | (h << 17), S, rpp | (npass << 8));
+                        s_lw[rr] = 1.0f / (float)w;
+                    }
+                }
+            }
+            __syncthreads();
+            // taps: four steps in flight, the loop unrolled over the four slots
+            // so that no slot is copied
+            TapCursor cur;
+            cursor_record(cur, wave, nbat, s_rec, s_lw, lane);
+            TapSlot ta = cursor_issue<PREC>(cur, nbat, a.table);
+            cursor_next(cur, nbat, s_rec, s_lw, lane);
+            TapSlot tb = cursor_issue<PREC>(cur, nbat, a.table);
+            cursor_next(cur, nbat, s_rec, s_lw, lane);
+            TapSlot tc = cursor_issue<PREC>(cur, nbat, a.table);
+            cursor_next(cur, nbat, s_rec, s_lw, lane);
+            TapSlot td = cursor_issue<PREC>(cur, nbat, a.table);
+            cursor_next(cur, nbat, s_rec, s_lw, lane);
+            int vr_rec = -1;
+            double vr[P], vi[P];
+            int total = 0;  // steps of this wave (uniform)
+            for (int r = wave; r < nbat; r += TG_WAVES) total += (wave_uniform(s_rec[r].w) >> 8) & 255;
+            auto tap = [&](TapSlot &t) {
+                if (t.r != vr_rec && t.r < nbat) {
+                    vr_rec = t.r;
+#pragma unroll
+                    for (int p = 0; p < P; p++) {
+                        vr[p] = wave_uniform_f64(s_vd[2 * p][vr_rec]);
+                        vi[p] = wave_uniform_f64(s_vd[2 * p + 1][vr_rec]);
+                    }
+                }
+                if (t.on) {
+                    const double kx = t.k.x, ky = t.k.y;
+#pragma unroll
+                    for (int p = 0; p < P; p++) {
+                        unsafeAtomicAdd(&s_acc[(2 * p) * ACC + t.cell], vr[p] * kx - vi[p] * ky);
+                        unsafeAtomicAdd(&s_acc[(2 * p + 1) * ACC + t.cell], vi[p] * kx + vr[p] * ky);
+                    }
+                }
+                t = cursor_issue<PREC>(cur, nbat, a.table);
+                cursor_next(cur, nbat, s_rec, s_lw, lane);
+            };
+            for (int step = 0; step < total; step += 4) {
+                tap(ta);
+                tap(tb);
+                tap(tc);
+                tap(td);
             }
         }
         __syncthreads();
-        if (chunk == 0 && tid < P) {
-            double t = 0;
-            for (int w = 0; w < RG_THREADS / 64; w++) t += s_norm[w][tid];
-            if (t != 0.0) unsafeAtomicAdd(&a.norm[c_local * P + tid], t);
-        }
-
-        // ---- Romein walk ---------------------------------------------------
-        int r0, r1, cls0, J;
-        if (JMAX == 1) {
-            const int K = RG_THREADS / S2;  // parallel sub-streams of the run
-            const int sub = tid / S2;
-            cls0 = tid - sub * S2;
-            if (sub < K) {
-                J = 1;
-                r0 = (int)(((int64_t)n * sub) / K);
-                r1 = (int)(((int64_t)n * (sub + 1)) / K);
-            } else {
-                J = 0;
-                r0 = r1 = 0;
-            }
-        } else {
-            cls0 = chunk * RG_THREADS * JMAX + tid;
-            J = JMAX;
-            r0 = 0;
-            r1 = n;
-        }
+        // add the tile to the grid (loads of a cell's correlations issued together)
         double *g = a.grid + 2 * (int64_t)c_local * a.chan_stride;
-        int bx[JMAX], by[JMAX], cur[JMAX];
-        double acc[JMAX][2 * P];
+        for (int i = tid; i < TG_TT; i += TG_THREADS) {
+            const int lx = i % TG_T, ly = i / TG_T;
+            const int gx = x0 + lx, gy = y0 + ly;
+            if (gx >= a.nx || gy >= a.ny) continue;
+            const int64_t cell = (int64_t)gy * a.nx + gx;
+            const int li = ly * TG_LS + lx;
+            double re[P], im[P];
+            double2 old[P];
 #pragma unroll
-        for (int j = 0; j < JMAX; j++) {
-            const int c = cls0 + j * RG_THREADS;
-            const bool on = (j < J) && (c < S2);
-            by[j] = on ? c / S : -1;
-            bx[j] = on ? c - (c / S) * S : 0;
-            cur[j] = -1;
-#pragma unroll
-            for (int q = 0; q < 2 * P; q++) acc[j][q] = 0.0;
-        }
-        for (int r = r0; r < r1; r += UNR) {
-            int cellv[UNR][JMAX];
-            float2 kv[UNR][JMAX];
-#pragma unroll
-            for (int u = 0; u < UNR; u++) {
-                const int rr = r + u;
-                const int4 rec = s_rec[rr < r1 ? rr : r];
-                const int rx = rec.z & 0xffff, ry = rec.z >> 16;
-#pragma unroll
-                for (int j = 0; j < JMAX; j++) {
-                    int sx = bx[j] - rx, sy = by[j] - ry;
-                    sx += (sx < 0) ? S : 0;
-                    sy += (sy < 0) ? S : 0;
-                    cellv[u][j] = rec.x + sy * a.nx + sx;
-                    kv[u][j] = (by[j] >= 0 && rr < r1) ? ksample<PREC>(a.table, rec.y + sy * S + sx)
-                                                        : make_float2(0.0f, 0.0f);
-                }
+            for (int p = 0; p < P; p++) {
+                re[p] = s_acc[(2 * p) * ACC + li];
+                im[p] = s_acc[(2 * p + 1) * ACC + li];
+                old[p] = make_double2(0.0, 0.0);
             }
+            if (excl) {
 #pragma unroll
-            for (int u = 0; u < UNR; u++) {
-                const int rr = r + u;
-                if (rr >= r1) break;
-                double vr[P], vi[P];
+                for (int p = 0; p < P; p++)
+                    if (re[p] != 0.0 || im[p] != 0.0) old[p] = *(const double2 *)(g + 2 * (cell * a.cs + (int64_t)p * a.ps));
+#pragma unroll
+                for (int p = 0; p < P; p++)
+                    if (re[p] != 0.0 || im[p] != 0.0)
+                        *(double2 *)(g + 2 * (cell * a.cs + (int64_t)p * a.ps)) = make_double2(old[p].x + re[p], old[p].y + im[p]);
+            } else {
 #pragma unroll
                 for (int p = 0; p < P; p++) {
-                    vr[p] = s_vis[2 * p][rr];
-                    vi[p] = s_vis[2 * p + 1][rr];
-                }
-#pragma unroll
-                for (int j = 0; j < JMAX; j++) {
-                    if (by[j] < 0) continue;
-                    const int cell = cellv[u][j];
-                    if (cell != cur[j]) {
-                        flush_point<P>(g, cur[j], a.cs, a.ps, acc[j]);
-                        cur[j] = cell;
-#pragma unroll
-                        for (int q = 0; q < 2 * P; q++) acc[j][q] = 0.0;
-                    }
-                    const double kx = kv[u][j].x, ky = kv[u][j].y;
-#pragma unroll
-                    for (int p = 0; p < P; p++) {
-                        acc[j][2 * p] += vr[p] * kx - vi[p] * ky;
-                        acc[j][2 * p + 1] += vi[p] * kx + vr[p] * ky;
-                    }
+                    double *c = g + 2 * (cell * a.cs + (int64_t)p * a.ps);
+                    if (re[p] != 0.0) unsafeAtomicAdd(c, re[p]);
+                    if (im[p] != 0.0) unsafeAtomicAdd(c + 1, im[p]);
                 }
             }
         }
-#pragma unroll
-        for (int j = 0; j < JMAX; j++) flush_point<P>(g, cur[j], a.cs, a.ps, acc[j]);
         __syncthreads();
     }
 }
@@ -287,8 +519,9 @@ __global__ void k_fill_f64(double *p, int64_t n, double v)
 
 // Stable compaction.  pos is the exclusive scan of the valid flags, so
 // record e is kept exactly when the next position differs from its own.
-__global__ void k_compact(const uint32_t *pos, const uint32_t *n_rec, const int4 *desc, int64_t nrec,
-                          int64_t nidx, int4 *cdesc, int32_t *cchan)
+__global__ void k_compact(const uint32_t *pos, const uint32_t *n_rec, const int4 *desc, const float2 *wv,
+                          const float *wt, int64_t nrec, int64_t nidx, int P, int4 *cdesc, int32_t *cchan,
+                          float2 *cwv, float *cwt)
 {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nrec) return;
@@ -297,42 +530,9 @@ __global__ void k_compact(const uint32_t *pos, const uint32_t *n_rec, const int4
     if (next == j) return;
     cdesc[j] = desc[e];
     cchan[j] = (int32_t)(e / nidx);
-}
-
-// Item boundaries: channel change, support change, or every RG_RECS records.
-__global__ void k_item_flags(const int4 *cdesc, const int32_t *cchan, const uint32_t *n_rec, int64_t cap,
-                             uint32_t *flag)
-{
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= cap) return;
-    const int64_t M = *n_rec;
-    uint32_t f = 0;
-    if (j < M) {
-        f = (j == 0) || (j % RG_RECS == 0) || (cchan[j] != cchan[j - 1]) ||
-            ((cdesc[j].w & 0x7fff) != (cdesc[j - 1].w & 0x7fff));
-    }
-    flag[j] = f;
-}
-
-// ids = exclusive scan of the boundary flags: record j starts item ids[j]
-// when the next id differs.  Writes item_start and the work units of every
-// item for the two gridder variants: S^2 <= 256 (one tap class per lane,
-// sub-streams) or larger (chunks of RG_CLASSES classes).
-__global__ void k_item_starts(const uint32_t *ids, const uint32_t *n_items, const int4 *cdesc,
-                              const uint32_t *n_rec, int64_t cap, uint32_t *item_start, uint32_t *units_small,
-                              uint32_t *units_large)
-{
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= cap || j >= (int64_t)*n_rec) return;
-    const uint32_t id = ids[j];
-    const uint32_t next = (j + 1 < cap) ? ids[j + 1] : *n_items;
-    if (next == id) return;
-    item_start[id] = (uint32_t)j;
-    const int S = cdesc[j].w & 0x7fff;
-    if (S * S <= RG_THREADS) {
-        units_small[id] = 1;
-    } else {
-        units_large[id] = (uint32_t)((S * S + RG_CLASSES - 1) / RG_CLASSES);
+    for (int p = 0; p < P; p++) {
+        cwv[(int64_t)j * P + p] = wv[e * P + p];
+        cwt[(int64_t)j * P + p] = wt[e * P + p];
     }
 }
 
@@ -344,40 +544,39 @@ __global__ void k_round_grid(const double *work, float *out, int64_t n, int accu
         out[i] = accumulate ? (float)((double)out[i] + work[i]) : (float)work[i];
 }
 
-__global__ void k_fill_f64x(double *p, int64_t n)
+// Per (channel, correlation): sum W and sum W^2 of the density grid
+// [channel][cell][P]; one block row per channel.
+__global__ __launch_bounds__(256) void k_density_stats(const float *dens, int64_t cells, int P,
+                                                      double *acc /* [nb][P][2] */)
 {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = 0.0;
-}
-
-// Per (channel, pol) plane: sum W and sum W^2 of the density grid.
-__global__ __launch_bounds__(256) void k_density_stats(const float *dens, int64_t plane_sz,
-                                                      double *acc /* [planes][2] */)
-{
-    const int plane = blockIdx.y;
-    const float *d = dens + (int64_t)plane * plane_sz;
-    double s1 = 0, s2 = 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < plane_sz;
+    const int c = blockIdx.y;
+    const float *d = dens + (int64_t)c * cells * P;
+    double s1[4] = {0, 0, 0, 0}, s2[4] = {0, 0, 0, 0};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cells;
          i += (int64_t)gridDim.x * blockDim.x) {
-        double v = d[i];
-        s1 += v;
-        s2 += v * v;
+        for (int p = 0; p < P; p++) {
+            const double v = d[i * P + p];
+            s1[p] += v;
+            s2[p] += v * v;
+        }
     }
-    for (int o = 32; o >= 1; o >>= 1) {
-        s1 += __shfl_xor(s1, o, 64);
-        s2 += __shfl_xor(s2, o, 64);
-    }
-    __shared__ double r1[4], r2[4];
-    if ((threadIdx.x & 63) == 0) {
-        r1[threadIdx.x >> 6] = s1;
-        r2[threadIdx.x >> 6] = s2;
+    __shared__ double r[4][4][2];
+    for (int p = 0; p < P; p++) {
+        double a1 = s1[p], a2 = s2[p];
+        for (int o = 32; o >= 1; o >>= 1) {
+            a1 += __shfl_xor(a1, o, 64);
+            a2 += __shfl_xor(a2, o, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            r[threadIdx.x >> 6][p][0] = a1;
+            r[threadIdx.x >> 6][p][1] = a2;
+        }
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double t1 = r1[0] + r1[1] + r1[2] + r1[3], t2 = r2[0] + r2[1] + r2[2] + r2[3];
-        unsafeAtomicAdd(&acc[2 * plane], t1);
-        unsafeAtomicAdd(&acc[2 * plane + 1], t2);
+    if (threadIdx.x < 2 * P) {
+        const int p = threadIdx.x >> 1, k = threadIdx.x & 1;
+        const double t = r[0][p][k] + r[1][p][k] + r[2][p][k] + r[3][p][k];
+        unsafeAtomicAdd(&acc[2 * (c * P + p) + k], t);
     }
 }
 
@@ -391,174 +590,237 @@ __global__ void k_briggs_f2(const double *acc, int n, double robust, double *f2)
     f2[i] = sw2 > 0.0 ? (k * k) / (sw2 / sw) : 0.0;
 }
 
-template <int P, int PREC, int JMAX, int UNR>
-static mti_status launch_romein(mti_ctx *ctx, const RomeinArgs &a, const uint32_t *prefix,
-                                const uint32_t *nunits, hipStream_t st)
+template <typename K>
+static int persistent_blocks(mti_ctx *ctx, K kern, int threads)
 {
-    auto kern = k_grid_romein<P, PREC, JMAX, UNR>;
     int per_cu = 0;
-    MTI_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, RG_THREADS, 0));
-    if (per_cu < 1) per_cu = 1;
-    const int blocks = ctx->num_cus * per_cu;
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(RG_THREADS), 0, st, a, prefix, nunits);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    return ctx->num_cus * per_cu;
+}
+
+template <int P, int PREC>
+static mti_status launch_tiles(mti_ctx *ctx, const TileArgs &a, hipStream_t st)
+{
+    auto kern = k_grid_tiles<P, PREC>;
+    hipLaunchKernelGGL(kern, dim3(persistent_blocks(ctx, kern, TG_THREADS)), dim3(TG_THREADS), 0, st, a);
     MTI_HIP(ctx, hipGetLastError());
     return MTI_OK;
 }
 
-template <int P, int PREC>
-static mti_status romein_pair(mti_ctx *ctx, const RomeinArgs &a, const uint32_t *ps, const uint32_t *ns,
-                              const uint32_t *pl, const uint32_t *nl, hipStream_t st)
+template <int P>
+static mti_status dispatch_tiles(mti_ctx *ctx, int prec, const TileArgs &a, hipStream_t st)
 {
-    mti_status r = launch_romein<P, PREC, 1, 4>(ctx, a, ps, ns, st);
-    if (r != MTI_OK) return r;
-    return launch_romein<P, PREC, RG_JMAX, 2>(ctx, a, pl, nl, st);
+    if (prec == MTI_TABLE_F32) return launch_tiles<P, MTI_TABLE_F32>(ctx, a, st);
+    if (prec == MTI_TABLE_F16) return launch_tiles<P, MTI_TABLE_F16>(ctx, a, st);
+    return launch_tiles<P, MTI_TABLE_BF16>(ctx, a, st);
 }
 
 template <int P>
-static mti_status dispatch_romein(mti_ctx *ctx, int prec, const RomeinArgs &a, const uint32_t *ps,
-                                  const uint32_t *ns, const uint32_t *pl, const uint32_t *nl, hipStream_t st)
+static mti_status launch_prep(mti_ctx *ctx, const TileArgs &a, hipStream_t st)
 {
-    if (prec == MTI_TABLE_F32) return romein_pair<P, MTI_TABLE_F32>(ctx, a, ps, ns, pl, nl, st);
-    if (prec == MTI_TABLE_F16) return romein_pair<P, MTI_TABLE_F16>(ctx, a, ps, ns, pl, nl, st);
-    return romein_pair<P, MTI_TABLE_BF16>(ctx, a, ps, ns, pl, nl, st);
+    auto kern = k_record_prep<P>;
+    hipLaunchKernelGGL(kern, dim3(persistent_blocks(ctx, kern, 256)), dim3(256), 0, st, a);
+    MTI_HIP(ctx, hipGetLastError());
+    return MTI_OK;
+}
+
+template <int P>
+static mti_status launch_density(mti_ctx *ctx, const TileArgs &a, hipStream_t st)
+{
+    auto kern = k_density_tiles<P>;
+    hipLaunchKernelGGL(kern, dim3(persistent_blocks(ctx, kern, 256)), dim3(256), 0, st, a);
+    MTI_HIP(ctx, hipGetLastError());
+    return MTI_OK;
 }
 
 static inline size_t align_up(size_t x) { return (x + 255) / 256 * 256; }
+
+// Tiles a support box of side S can touch along one axis.
+static inline int tiles_per_axis(int S) { return 1 + (std::max(S, 1) - 1 + TG_T - 1) / TG_T; }
 
 // Grids channels [chan0, chan0+nb) into the FP64 grid G (already zeroed or
 // holding previous chunks); norm is accumulated.
 static mti_status grid_batch(mti_ctx *ctx, const mti_kernels *k, const mti_image_params *p,
                              const VisView &vw, const mti_grid_opts *o, int chan0, int nb, double *G,
-                             double *norm, unsigned long long *dstats, uint32_t *ni_total, hipStream_t st)
+                             double *norm, unsigned long long *dstats, uint32_t *nu_total, hipStream_t st)
 {
     const int P = vw.npol;
     const int64_t gsz = (int64_t)p->nx * p->ny;
     const int64_t nrec = vw.nidx * nb;
+    const int ntx = (p->nx + TG_T - 1) / TG_T, nty = (p->ny + TG_T - 1) / TG_T;
+    const int64_t nkeys = (int64_t)nb * ntx * nty;
+    const int tpa = tiles_per_axis(k->info.max_support);
+    const int64_t pair_cap = nrec * tpa * tpa;
+    MTI_CHECK(ctx, nkeys < (int64_t(1) << 31), "too many grid tiles per call");
+    MTI_CHECK(ctx, pair_cap < (int64_t(1) << 32), "too many (record, tile) pairs per call");
+    const int64_t unit_cap = nkeys + pair_cap / TG_CHUNK + 1;
     mti_status r;
-    // scratch: valid, flags, item_start, units_small, units_large, cchan (u32 x nrec) + scalars;
-    // desc + cdesc (int4 x nrec)
+    // scratch: valid, cchan (u32 x nrec); desc, cdesc (int4 x nrec);
+    // start, fill, nunits (u32 x nkeys); pairs (u32 x pair_cap); units (int4)
     const size_t u32n = align_up((size_t)nrec * 4);
-    void *w1 = nullptr, *w2 = nullptr;
-    r = scratch_get(ctx, SLOT_KEYS, u32n * 6 + 4096, &w1);
+    const size_t u32k = align_up((size_t)nkeys * 4);
+    void *w1 = nullptr, *w2 = nullptr, *w3 = nullptr, *w4 = nullptr;
+    r = scratch_get(ctx, SLOT_KEYS, u32n * 2 + 4096, &w1);
     if (r != MTI_OK) return r;
     r = scratch_get(ctx, SLOT_DESC, align_up((size_t)nrec * 16) * 2, &w2);
     if (r != MTI_OK) return r;
+    r = scratch_get(ctx, SLOT_BUCKETS, u32k * 3, &w3);
+    if (r != MTI_OK) return r;
+    r = scratch_get(ctx, SLOT_ITEMS, align_up((size_t)pair_cap * 4) + align_up((size_t)unit_cap * 16), &w4);
+    if (r != MTI_OK) return r;
     uint32_t *valid = (uint32_t *)w1;
-    uint32_t *flag = (uint32_t *)((char *)w1 + u32n);
-    uint32_t *istart = (uint32_t *)((char *)w1 + 2 * u32n);
-    uint32_t *us = (uint32_t *)((char *)w1 + 3 * u32n);
-    uint32_t *ul = (uint32_t *)((char *)w1 + 4 * u32n);
-    int32_t *cchan = (int32_t *)((char *)w1 + 5 * u32n);
-    uint32_t *scal = (uint32_t *)((char *)w1 + 6 * u32n);
-    uint32_t *M_d = scal, *NI_d = scal + 64, *NS_d = scal + 128, *NL_d = scal + 192;
+    int32_t *cchan = (int32_t *)((char *)w1 + u32n);
+    uint32_t *scal = (uint32_t *)((char *)w1 + 2 * u32n);
+    uint32_t *M_d = scal, *NP_d = scal + 64, *NU_d = scal + 128;
     int4 *desc = (int4 *)w2;
     int4 *cdesc = (int4 *)((char *)w2 + align_up((size_t)nrec * 16));
+    uint32_t *start = (uint32_t *)w3;
+    uint32_t *fill = (uint32_t *)((char *)w3 + u32k);
+    uint32_t *nunits = (uint32_t *)((char *)w3 + 2 * u32k);
+    uint32_t *pairs = (uint32_t *)w4;
+    int4 *units = (int4 *)((char *)w4 + align_up((size_t)pair_cap * 4));
 
-    float *density = nullptr;
-    double *dacc = nullptr, *f2 = nullptr;
-    if (o->weighting != MTI_WEIGHT_NATURAL) {
-        void *w4 = nullptr, *w5 = nullptr;
-        r = scratch_get(ctx, SLOT_DENSITY, (size_t)gsz * nb * P * sizeof(float), &w4);
-        if (r != MTI_OK) return r;
-        density = (float *)w4;
-        r = scratch_get(ctx, SLOT_NORM, (size_t)nb * P * 3 * sizeof(double) + 256, &w5);
-        if (r != MTI_OK) return r;
-        dacc = (double *)w5;
-        f2 = dacc + 2 * nb * P;
-        hipLaunchKernelGGL(k_fill_f32, dim3(4096), dim3(256), 0, st, density, gsz * nb * P, 0.0f);
-        hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(256), 0, st, dacc, (int64_t)nb * P * 2, 0.0);
-        MTI_HIP(ctx, hipGetLastError());
-    }
     BinPassArgs ba;
     ba.v = vw;
     ba.g = make_geom(k, p);
     ba.chan0 = chan0;
     ba.nb = nb;
+    ba.psf = o->psf;
     ba.valid = valid;
     ba.desc = desc;
-    ba.density = density;
     ba.stats = dstats;
+    // per-record visibilities and weights, binning order and compacted
+    void *w7 = nullptr, *w8 = nullptr;
+    const size_t wv_bytes = align_up((size_t)nrec * P * sizeof(float2));
+    const size_t wt_bytes = align_up((size_t)nrec * P * sizeof(float));
+    r = scratch_get(ctx, SLOT_RECVIS, wv_bytes + wt_bytes, &w7);
+    if (r != MTI_OK) return r;
+    r = scratch_get(ctx, SLOT_CRECVIS, wv_bytes + wt_bytes, &w8);
+    if (r != MTI_OK) return r;
+    ba.wv = (float2 *)w7;
+    ba.wt = (float *)((char *)w7 + wv_bytes);
+    float2 *cwv = (float2 *)w8;
+    float *cwt = (float *)((char *)w8 + wv_bytes);
     {
         StageTimer tm(ctx, MTI_STAGE_BIN, st);
         r = launch_bin_pass(ctx, ba, nrec, st);
         if (r != MTI_OK) return r;
-        if (density) {
-            dim3 gr(64, (unsigned)(nb * P));
-            hipLaunchKernelGGL(k_density_stats, gr, dim3(256), 0, st, density, gsz, dacc);
-            hipLaunchKernelGGL(k_briggs_f2, dim3((unsigned)ceil_div(nb * P, 64)), dim3(64), 0, st, dacc,
-                               nb * P, o->robust, f2);
-            MTI_HIP(ctx, hipGetLastError());
-        }
     }
     const unsigned gblocks = (unsigned)ceil_div(nrec, 256);
+    const unsigned kblocks = (unsigned)ceil_div(nkeys, 256);
     {
         StageTimer tm(ctx, MTI_STAGE_SORT_RECORDS, st);
         // stable compaction of gridded records (keeps baseline-major order)
         r = exclusive_scan_u32(ctx, valid, nrec, M_d, st);
         if (r != MTI_OK) return r;
-        hipLaunchKernelGGL(k_compact, dim3(gblocks), dim3(256), 0, st, valid, M_d, desc, nrec, vw.nidx, cdesc,
-                           cchan);
+        hipLaunchKernelGGL(k_compact, dim3(gblocks), dim3(256), 0, st, valid, M_d, desc, ba.wv, ba.wt, nrec,
+                           vw.nidx, P, cdesc, cchan, cwv, cwt);
+        // counting sort of (record, tile) pairs by tile
+        hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, st, start, nkeys, 0u);
+        hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, st, fill, nkeys, 0u);
+        hipLaunchKernelGGL(k_tile_pairs<0>, dim3(gblocks), dim3(256), 0, st, cdesc, cchan, M_d, nrec, nty, ntx,
+                           start, (const uint32_t *)nullptr, (uint32_t *)nullptr);
+        MTI_HIP(ctx, hipGetLastError());
+        r = exclusive_scan_u32(ctx, start, nkeys, NP_d, st);
+        if (r != MTI_OK) return r;
+        hipLaunchKernelGGL(k_tile_pairs<1>, dim3(gblocks), dim3(256), 0, st, cdesc, cchan, M_d, nrec, nty, ntx,
+                           fill, (const uint32_t *)start, pairs);
         MTI_HIP(ctx, hipGetLastError());
     }
     {
         StageTimer tm(ctx, MTI_STAGE_PLAN, st);
-        hipLaunchKernelGGL(k_item_flags, dim3(gblocks), dim3(256), 0, st, cdesc, cchan, M_d, nrec, flag);
+        hipLaunchKernelGGL(k_unit_counts, dim3(kblocks), dim3(256), 0, st, start, NP_d, nkeys, nunits);
         MTI_HIP(ctx, hipGetLastError());
-        r = exclusive_scan_u32(ctx, flag, nrec, NI_d, st);
+        r = exclusive_scan_u32(ctx, nunits, nkeys, NU_d, st);
         if (r != MTI_OK) return r;
-        hipLaunchKernelGGL(k_fill_u32, dim3(4096), dim3(256), 0, st, us, nrec, 0u);
-        hipLaunchKernelGGL(k_fill_u32, dim3(4096), dim3(256), 0, st, ul, nrec, 0u);
-        hipLaunchKernelGGL(k_item_starts, dim3(gblocks), dim3(256), 0, st, flag, NI_d, cdesc, M_d, nrec, istart,
-                           us, ul);
+        hipLaunchKernelGGL(k_unit_list, dim3(kblocks), dim3(256), 0, st, start, NP_d, nunits, NU_d, nkeys, units);
         MTI_HIP(ctx, hipGetLastError());
-        r = exclusive_scan_u32(ctx, us, nrec, NS_d, st);
-        if (r != MTI_OK) return r;
-        r = exclusive_scan_u32(ctx, ul, nrec, NL_d, st);
-        if (r != MTI_OK) return r;
     }
-    RomeinArgs ra;
-    ra.v = vw;
-    ra.chan0 = chan0;
-    ra.cdesc = cdesc;
-    ra.cchan = cchan;
-    ra.item_start = istart;
-    ra.n_items = NI_d;
-    ra.n_rec = M_d;
-    ra.table = k->d_table;
-    ra.sums = k->d_sums;
-    ra.convinfo = k->d_convinfo;
-    ra.T2 = k->info.T * k->info.T;
-    ra.nx = p->nx;
-    ra.ny = p->ny;
+    TileArgs ta;
+    ta.v = vw;
+    ta.chan0 = chan0;
+    ta.cdesc = cdesc;
+    ta.cchan = cchan;
+    ta.n_rec = M_d;
+    ta.wvis = cwv;
+    ta.wt = cwt;
+    ta.pairs = pairs;
+    ta.units = units;
+    ta.n_units = NU_d;
+    ta.table = k->d_table;
+    ta.sums = k->d_sums;
+    ta.convinfo = k->d_convinfo;
+    ta.T2 = k->info.T * k->info.T;
+    ta.nx = p->nx;
+    ta.ny = p->ny;
+    ta.ntx = ntx;
+    ta.nty = nty;
     if (o->grid_layout == MTI_GRID_VUP) {
-        ra.cs = P;
-        ra.ps = 1;
+        ta.cs = P;
+        ta.ps = 1;
     } else {
-        ra.cs = 1;
-        ra.ps = gsz;
+        ta.cs = 1;
+        ta.ps = gsz;
     }
-    ra.chan_stride = gsz * P;
-    ra.grid = G;
-    ra.norm = norm;
-    ra.psf = o->psf;
-    ra.weighting = o->weighting;
-    ra.density = density;
-    ra.f2 = f2;
+    ta.chan_stride = gsz * P;
+    ta.grid = G;
+    ta.norm = norm;
+    ta.psf = o->psf;
+    ta.weighting = o->weighting;
+    ta.density = nullptr;
+    ta.f2 = nullptr;
+    if (o->weighting != MTI_WEIGHT_NATURAL) {
+        StageTimer tm(ctx, MTI_STAGE_PLAN, st);
+        void *w5 = nullptr, *w6 = nullptr;
+        r = scratch_get(ctx, SLOT_DENSITY, (size_t)gsz * nb * P * sizeof(float), &w5);
+        if (r != MTI_OK) return r;
+        r = scratch_get(ctx, SLOT_NORM, (size_t)nb * P * 3 * sizeof(double) + 256, &w6);
+        if (r != MTI_OK) return r;
+        ta.density = (float *)w5;
+        double *dacc = (double *)w6;
+        double *f2 = dacc + 2 * nb * P;
+        ta.f2 = f2;
+        hipLaunchKernelGGL(k_fill_f32, dim3(8192), dim3(256), 0, st, ta.density, gsz * nb * P, 0.0f);
+        hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(256), 0, st, dacc, (int64_t)nb * P * 2, 0.0);
+        MTI_HIP(ctx, hipGetLastError());
+        switch (P) {
+        case 1: r = launch_density<1>(ctx, ta, st); break;
+        case 2: r = launch_density<2>(ctx, ta, st); break;
+        default: r = launch_density<4>(ctx, ta, st); break;
+        }
+        if (r != MTI_OK) return r;
+        hipLaunchKernelGGL(k_density_stats, dim3(256, (unsigned)nb), dim3(256), 0, st, ta.density, gsz, P, dacc);
+        hipLaunchKernelGGL(k_briggs_f2, dim3((unsigned)ceil_div(nb * P, 64)), dim3(64), 0, st, dacc, nb * P,
+                           o->robust, f2);
+        MTI_HIP(ctx, hipGetLastError());
+    }
+    {
+        StageTimer tm(ctx, MTI_STAGE_PLAN, st);
+        switch (P) {
+        case 1: r = launch_prep<1>(ctx, ta, st); break;
+        case 2: r = launch_prep<2>(ctx, ta, st); break;
+        default: r = launch_prep<4>(ctx, ta, st); break;
+        }
+        if (r != MTI_OK) return r;
+    }
     {
         StageTimer tm(ctx, MTI_STAGE_GRID, st);
         const int prec = k->info.precision;
         switch (P) {
-        case 1: r = dispatch_romein<1>(ctx, prec, ra, us, NS_d, ul, NL_d, st); break;
-        case 2: r = dispatch_romein<2>(ctx, prec, ra, us, NS_d, ul, NL_d, st); break;
-        default: r = dispatch_romein<4>(ctx, prec, ra, us, NS_d, ul, NL_d, st); break;
+        case 1: r = dispatch_tiles<1>(ctx, prec, ta, st); break;
+        case 2: r = dispatch_tiles<2>(ctx, prec, ta, st); break;
+        default: r = dispatch_tiles<4>(ctx, prec, ta, st); break;
         }
         if (r != MTI_OK) return r;
     }
-    if (ni_total) {
-        uint32_t ni = 0;
-        MTI_HIP(ctx, hipMemcpyAsync(&ni, NI_d, 4, hipMemcpyDeviceToHost, st));
+    if (nu_total) {
+        uint32_t h[2] = {0, 0};
+        MTI_HIP(ctx, hipMemcpyAsync(&h[0], NU_d, 4, hipMemcpyDeviceToHost, st));
+        MTI_HIP(ctx, hipMemcpyAsync(&h[1], NP_d, 4, hipMemcpyDeviceToHost, st));
         MTI_HIP(ctx, hipStreamSynchronize(st));
-        *ni_total += ni;
+        nu_total[0] += h[0];
+        nu_total[1] += h[1];
     }
     return MTI_OK;
 }
@@ -588,6 +850,7 @@ extern "C" mti_status mti_grid(mti_ctx *ctx, const mti_kernels *k, const mti_ima
     MTI_CHECK(ctx, o->grid_precision == MTI_GRID_C64 || o->grid_precision == MTI_GRID_C128,
               "bad grid_precision");
     MTI_CHECK(ctx, (int64_t)p->nx * p->ny < (int64_t(1) << 30), "grid above 2^30 cells");
+    MTI_CHECK(ctx, p->nx <= 65535 && p->ny <= 65535, "grid side above 65535");
     MTI_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = to_stream(stream);
 
@@ -595,12 +858,12 @@ extern "C" mti_status mti_grid(mti_ctx *ctx, const mti_kernels *k, const mti_ima
     const int64_t gsz = (int64_t)p->nx * p->ny;
     const int64_t chan_floats = gsz * P * 2;  // reals per channel grid
     VisView vw = make_view(v);
-    MTI_CHECK(ctx, vw.nidx * nb < (int64_t(1) << 31), "more than 2^31-1 records (rows x channels) per call");
+    MTI_CHECK(ctx, vw.nidx < (int64_t(1) << 31), "more than 2^31-1 rows per call");
 
     if (!o->accumulate) {
         hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(256), 0, st, norm, (int64_t)nb * P, 0.0);
         if (o->grid_precision == MTI_GRID_C128)
-            hipLaunchKernelGGL(k_fill_f64x, dim3(8192), dim3(256), 0, st, (double *)grids, chan_floats * nb);
+            hipLaunchKernelGGL(k_fill_f64, dim3(8192), dim3(256), 0, st, (double *)grids, chan_floats * nb, 0.0);
         MTI_HIP(ctx, hipGetLastError());
     }
     void *sw = nullptr;
@@ -609,27 +872,38 @@ extern "C" mti_status mti_grid(mti_ctx *ctx, const mti_kernels *k, const mti_ima
     unsigned long long *dstats = (unsigned long long *)sw;
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, (uint32_t *)dstats, 8, 0u);
     MTI_HIP(ctx, hipGetLastError());
-    uint32_t ni = 0;
+    uint32_t nu[2] = {0, 0};
+
+    // Channel batches keep records below 2^31 and (record, tile) pairs below 2^32.
+    const int tpa = tiles_per_axis(k->info.max_support);
+    const int64_t per_chan = std::max<int64_t>(1, vw.nidx * tpa * tpa);
+    const int64_t cap_chans = std::min<int64_t>(((int64_t(1) << 32) - 1) / per_chan,
+                                                ((int64_t(1) << 31) - 1) / std::max<int64_t>(1, vw.nidx));
+    MTI_CHECK(ctx, cap_chans >= 1, "one channel has too many (record, tile) pairs");
+    const int pair_batch = (int)std::min<int64_t>(nb, cap_chans);
 
     if (o->grid_precision == MTI_GRID_C128) {
-        if (vw.nidx > 0) {
-            r = grid_batch(ctx, k, p, vw, o, o->chan0, nb, (double *)grids, norm, dstats, stats ? &ni : nullptr, st);
+        for (int c0 = 0; c0 < nb && vw.nidx > 0; c0 += pair_batch) {
+            const int n = std::min(pair_batch, nb - c0);
+            r = grid_batch(ctx, k, p, vw, o, o->chan0 + c0, n, (double *)grids + (size_t)c0 * chan_floats,
+                           norm + (size_t)c0 * P, dstats, stats ? nu : nullptr, st);
             if (r != MTI_OK) return r;
         }
     } else {
         // FP64 work grid for a batch of channels, rounded once into the output
         const size_t chan_bytes = (size_t)chan_floats * sizeof(double);
-        const int batch = (int)std::max<size_t>(1, std::min<size_t>((size_t)nb, (size_t(8) << 30) / chan_bytes));
+        int batch = (int)std::max<size_t>(1, std::min<size_t>((size_t)nb, (size_t(8) << 30) / chan_bytes));
+        batch = std::min(batch, pair_batch);
         void *gw = nullptr;
         r = scratch_get(ctx, SLOT_GRID64, chan_bytes * batch, &gw);
         if (r != MTI_OK) return r;
         for (int c0 = 0; c0 < nb; c0 += batch) {
             const int n = std::min(batch, nb - c0);
-            hipLaunchKernelGGL(k_fill_f64x, dim3(8192), dim3(256), 0, st, (double *)gw, chan_floats * n);
+            hipLaunchKernelGGL(k_fill_f64, dim3(8192), dim3(256), 0, st, (double *)gw, chan_floats * n, 0.0);
             MTI_HIP(ctx, hipGetLastError());
             if (vw.nidx > 0) {
                 r = grid_batch(ctx, k, p, vw, o, o->chan0 + c0, n, (double *)gw, norm + (size_t)c0 * P, dstats,
-                               stats ? &ni : nullptr, st);
+                               stats ? nu : nullptr, st);
                 if (r != MTI_OK) return r;
             }
             hipLaunchKernelGGL(k_round_grid, dim3(8192), dim3(256), 0, st, (const double *)gw,
@@ -644,7 +918,8 @@ extern "C" mti_status mti_grid(mti_ctx *ctx, const mti_kernels *k, const mti_ima
         *stats = mti_grid_stats{};
         stats->gridded_records = (int64_t)h[0];
         stats->gridded_points = (int64_t)h[1];
-        stats->work_items = ni;
+        stats->work_items = nu[0];
+        stats->flushes = nu[1];
     }
     return MTI_OK;
 }

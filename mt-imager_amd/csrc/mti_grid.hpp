@@ -38,9 +38,11 @@ struct BinPassArgs {
     VisView v;
     BinGeom g;
     int chan0, nb;
+    int psf;
     uint32_t *valid;            // [nrec] 1 if the record is gridded
     int4 *desc;                 // [nrec]
-    float *density;             // [nb][npol][ny][nx] or null (uniform/Briggs)
+    float2 *wv;                 // [nrec][npol] visibilities of gridded records
+    float *wt;                  // [nrec][npol] weights, 0 where flagged
     unsigned long long *stats;  // [0] gridded records, [1] gridded points
 };
 
