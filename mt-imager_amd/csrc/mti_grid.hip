@@ -38,11 +38,10 @@ namespace mti {
 
 constexpr int TG_T = 32;             // tile side in cells
 constexpr int TG_TT = TG_T * TG_T;
-constexpr int TG_THREADS = 512;      // 8 waves
-constexpr int TG_WAVES = TG_THREADS / 64;
-constexpr int TG_BATCH = 128;        // records staged per round
-constexpr int TG_LS = TG_T + 1;      // LDS row stride of the tile (cells)
-static_assert(TG_T * TG_LS < 2048 && TG_T < 64, "packed record fields");
+constexpr int TG_THREADS = 1024;     // 16 waves, one 8x8 region each
+constexpr int TG_REGIONS = 16;
+constexpr int TG_BATCH = 256;        // records staged per round
+static_assert(TG_T == 32 && TG_THREADS == 64 * TG_REGIONS && TG_BATCH <= 256, "region layout");
 constexpr int TG_CHUNK = 2048;       // records per work unit
 
 struct TileArgs {
@@ -302,80 +301,49 @@ __global__ __launch_bounds__(256) void k_record_prep(TileArgs a)
 
 // ---- gridder ------------------------------------------------------------------
 
-__device__ __forceinline__ double wave_uniform_f64(double v)
-{
-    const int lo = __builtin_amdgcn_readfirstlane(__double2loint(v));
-    const int hi = __builtin_amdgcn_readfirstlane(__double2hiint(v));
-    return __hiloint2double(hi, lo);
-}
-
-// A wave's walk over its records (wave, wave + 8, ...) pass by pass.  The
-// record geometry is wave-uniform; (lx, ly) is the lane's tap within a pass of
-// rpp rows of w lanes.  s_rec: x = kernel offset of the clipped box's first
-// tap, y = LDS cell | w << 11 | h << 17, z = S, w = rpp | npass << 8.
-struct TapCursor {
-    int r, q, npass, rpp, h, S;
-    int kb_lx, cell_lx, ly;   // per lane
-    bool lane_ok;             // per lane: ly < rpp
-};
-
-__device__ __forceinline__ void cursor_record(TapCursor &c, int r, int nbat, const int4 *s_rec, const float *s_lw,
-                                              int lane)
-{
-    c.r = r;
-    c.q = 0;
-    if (r >= nbat) return;
-    const int4 rec = s_rec[r];
-    const int kb = wave_uniform(rec.x), pk = wave_uniform(rec.y);
-    c.S = wave_uniform(rec.z);
-    const int zz = wave_uniform(rec.w);
-    const int w = (pk >> 11) & 63;
-    c.h = (pk >> 17) & 63;
-    c.rpp = zz & 255;
-    c.npass = (zz >> 8) & 255;
-    const float rcp = s_lw[r];
-    const int ly = (int)(((float)lane + 0.5f) * rcp), lx = lane - ly * w;
-    c.ly = ly;
-    c.lane_ok = ly < c.rpp;
-    c.kb_lx = kb + lx;
-    c.cell_lx = (pk & 2047) + lx;
-}
-
-__device__ __forceinline__ void cursor_next(TapCursor &c, int nbat, const int4 *s_rec, const float *s_lw, int lane)
-{
-    if (c.r >= nbat) return;
-    if (++c.q < c.npass) return;
-    cursor_record(c, c.r + TG_WAVES, nbat, s_rec, s_lw, lane);
-}
-
-// One in-flight tap: record, LDS cell, lane active, kernel sample.
-struct TapSlot {
-    int r, cell;
+// One in-flight visit of a wave to a record: staged record, lane active,
+// kernel sample.
+struct Visit {
+    int r;
     bool on;
     float2 k;
 };
 
+// Visit i of the wave's region list: the lane's cell (gx, gy) against the
+// record's support box; kernel index 0 (a valid dummy) when outside.
 template <int PREC>
-__device__ __forceinline__ TapSlot cursor_issue(const TapCursor &c, int nbat, const void *table)
+__device__ __forceinline__ Visit visit_issue(int i, int n, const uint8_t *list, const int4 *s_rec, int gx, int gy,
+                                             const void *table)
 {
-    TapSlot t;
-    t.r = c.r;
-    const int y = c.ly + c.q * c.rpp;
-    t.on = (c.r < nbat) && c.lane_ok && (y < c.h);
-    t.cell = c.cell_lx + y * TG_LS;
-    t.k = ksample<PREC>(table, t.on ? c.kb_lx + y * c.S : 0);  // index 0: a valid dummy
-    return t;
+    Visit v;
+    v.r = -1;
+    v.on = false;
+    int idx = 0;
+    if (i < n) {
+        v.r = list[i];
+        const int4 rec = s_rec[v.r];  // corner x | y << 16, kernel offset, S
+        const int dx = gx - (rec.x & 0xffff), dy = gy - (int)((unsigned)rec.x >> 16);
+        v.on = (unsigned)dx < (unsigned)rec.z && (unsigned)dy < (unsigned)rec.z;
+        idx = v.on ? rec.y + dy * rec.z + dx : 0;
+    }
+    v.k = ksample<PREC>(table, idx);
+    return v;
 }
 
+// Gridder: one 1024-thread workgroup per unit; wave w owns the 8x8 region w
+// of the 32x32 tile and each lane one cell, accumulated in FP64 registers.
+// Records are staged in LDS per batch together with, for every region, the
+// list of records whose support box reaches it; a wave walks only its list,
+// with kernel samples loaded four visits ahead.  No LDS traffic per tap.
 template <int P, int PREC>
 __global__ __launch_bounds__(TG_THREADS) void k_grid_tiles(TileArgs a)
 {
-    __shared__ double s_acc[2 * P * TG_T * TG_LS];  // [p][re,im][y][x], rows padded to TG_LS
-    __shared__ int4 s_rec[TG_BATCH];
-    __shared__ float s_lw[TG_BATCH];
-    __shared__ double s_vd[2 * P][TG_BATCH];        // weighted visibility
-    constexpr int ACC = TG_T * TG_LS;
+    __shared__ int4 s_rec[TG_BATCH];                 // corner, kernel offset, S
+    __shared__ double s_vd[TG_BATCH][2 * P];         // weighted visibility
+    __shared__ uint8_t s_list[TG_REGIONS][TG_BATCH]; // records per region
+    __shared__ int s_cnt[TG_REGIONS];
     const int tid = threadIdx.x, lane = tid & 63, wave = wave_uniform(tid >> 6);
+    const int rx = (wave & 3) * 8 + (lane & 7), ry = (wave >> 2) * 8 + (lane >> 3);  // cell in the tile
     const uint32_t NU = *a.n_units;
 
     for (uint32_t u = blockIdx.x; u < NU; u += gridDim.x) {
@@ -383,116 +351,91 @@ __global__ __launch_bounds__(TG_THREADS) void k_grid_tiles(TileArgs a)
         const int key = un.x, first = un.y, npairs = un.z, excl = un.w;
         const int tx = key % a.ntx, t2 = key / a.ntx, ty = t2 % a.nty, c_local = t2 / a.nty;
         const int x0 = tx * TG_T, y0 = ty * TG_T;
-        for (int i = tid; i < 2 * P * ACC; i += TG_THREADS) s_acc[i] = 0.0;
+        const int gx = x0 + rx, gy = y0 + ry;
+        double acc[2 * P];
+#pragma unroll
+        for (int q = 0; q < 2 * P; q++) acc[q] = 0.0;
 
         for (int b0 = 0; b0 < npairs; b0 += TG_BATCH) {
             const int nbat = min(TG_BATCH, npairs - b0);
-            __syncthreads();  // LDS zeroed / previous batch consumed
-            // staging: four threads per record (geometry; one correlation each)
+            __syncthreads();  // previous batch consumed
+            if (tid < TG_REGIONS) s_cnt[tid] = 0;
+            __syncthreads();
+            // staging: four threads per record (one correlation each; the
+            // first also stages the geometry and the region lists)
             {
                 const int rr = tid >> 2, part = tid & 3;
                 if (rr < nbat) {
                     const uint32_t j = a.pairs[first + b0 + rr];
                     if (part < P) {
                         const float2 v = a.wvis[(int64_t)j * P + part];
-                        s_vd[2 * part][rr] = v.x;
-                        s_vd[2 * part + 1][rr] = v.y;
+                        s_vd[rr][2 * part] = v.x;
+                        s_vd[rr][2 * part + 1] = v.y;
                     }
                     if (part == 0) {
                         const int4 d = a.cdesc[j];
                         const int cx = d.y & 0xffff, cy = (int)((unsigned)d.y >> 16);
                         const int S = d.w & 0x7fff;
-                        const int ix0 = max(cx, x0), ix1 = min(cx + S, x0 + TG_T);
-                        const int iy0 = max(cy, y0), iy1 = min(cy + S, y0 + TG_T);
-                        const int w = ix1 - ix0, h = iy1 - iy0;
-                        const int rpp = 64 / w;
-                        const int npass = (h + rpp - 1) / rpp;
-                        const int kb = d.z + (iy0 - cy) * S + (ix0 - cx);
-                        const int lb = (iy0 - y0) * TG_LS + (ix0 - x0);
-                        s_rec[rr] = make_int4(kb, lb | (w << 11) | (h << 17), S, rpp | (npass << 8));
-                        s_lw[rr] = 1.0f / (float)w;
+                        s_rec[rr] = make_int4(d.y, d.z, S, 0);
+                        const int ix0 = max(cx, x0) - x0, ix1 = min(cx + S, x0 + TG_T) - x0;
+                        const int iy0 = max(cy, y0) - y0, iy1 = min(cy + S, y0 + TG_T) - y0;
+                        for (int gyr = iy0 >> 3; gyr <= (iy1 - 1) >> 3; gyr++)
+                            for (int gxr = ix0 >> 3; gxr <= (ix1 - 1) >> 3; gxr++) {
+                                const int reg = gyr * 4 + gxr;
+                                const int pos = atomicAdd(&s_cnt[reg], 1);
+                                s_list[reg][pos] = (uint8_t)rr;
+                            }
                     }
                 }
             }
             __syncthreads();
-            // taps: four steps in flight, the loop unrolled over the four slots
-            // so that no slot is copied
-            TapCursor cur;
-            cursor_record(cur, wave, nbat, s_rec, s_lw, lane);
-            TapSlot ta = cursor_issue<PREC>(cur, nbat, a.table);
-            cursor_next(cur, nbat, s_rec, s_lw, lane);
-            TapSlot tb = cursor_issue<PREC>(cur, nbat, a.table);
-            cursor_next(cur, nbat, s_rec, s_lw, lane);
-            TapSlot tc = cursor_issue<PREC>(cur, nbat, a.table);
-            cursor_next(cur, nbat, s_rec, s_lw, lane);
-            TapSlot td = cursor_issue<PREC>(cur, nbat, a.table);
-            cursor_next(cur, nbat, s_rec, s_lw, lane);
-            int vr_rec = -1;
-            double vr[P], vi[P];
-            int total = 0;  // steps of this wave (uniform)
-            for (int r = wave; r < nbat; r += TG_WAVES) total += (wave_uniform(s_rec[r].w) >> 8) & 255;
-            auto tap = [&](TapSlot &t) {
-                if (t.r != vr_rec && t.r < nbat) {
-                    vr_rec = t.r;
+            const int n = wave_uniform(s_cnt[wave]);
+            const uint8_t *list = s_list[wave];
+            Visit va = visit_issue<PREC>(0, n, list, s_rec, gx, gy, a.table);
+            Visit vb = visit_issue<PREC>(1, n, list, s_rec, gx, gy, a.table);
+            Visit vc = visit_issue<PREC>(2, n, list, s_rec, gx, gy, a.table);
+            Visit vd = visit_issue<PREC>(3, n, list, s_rec, gx, gy, a.table);
+            int next = 4;
+            auto visit = [&](Visit &v) {
+                if (v.on) {
+                    const double kx = v.k.x, ky = v.k.y;
 #pragma unroll
                     for (int p = 0; p < P; p++) {
-                        vr[p] = wave_uniform_f64(s_vd[2 * p][vr_rec]);
-                        vi[p] = wave_uniform_f64(s_vd[2 * p + 1][vr_rec]);
+                        const double vr = s_vd[v.r][2 * p], vi = s_vd[v.r][2 * p + 1];
+                        acc[2 * p] = fma(-vi, ky, fma(vr, kx, acc[2 * p]));
+                        acc[2 * p + 1] = fma(vr, ky, fma(vi, kx, acc[2 * p + 1]));
                     }
                 }
-                if (t.on) {
-                    const double kx = t.k.x, ky = t.k.y;
-#pragma unroll
-                    for (int p = 0; p < P; p++) {
-                        unsafeAtomicAdd(&s_acc[(2 * p) * ACC + t.cell], vr[p] * kx - vi[p] * ky);
-                        unsafeAtomicAdd(&s_acc[(2 * p + 1) * ACC + t.cell], vi[p] * kx + vr[p] * ky);
-                    }
-                }
-                t = cursor_issue<PREC>(cur, nbat, a.table);
-                cursor_next(cur, nbat, s_rec, s_lw, lane);
+                v = visit_issue<PREC>(next++, n, list, s_rec, gx, gy, a.table);
             };
-            for (int step = 0; step < total; step += 4) {
-                tap(ta);
-                tap(tb);
-                tap(tc);
-                tap(td);
+            for (int i = 0; i < n; i += 4) {
+                visit(va);
+                visit(vb);
+                visit(vc);
+                visit(vd);
             }
         }
-        __syncthreads();
-        // add the tile to the grid (loads of a cell's correlations issued together)
-        double *g = a.grid + 2 * (int64_t)c_local * a.chan_stride;
-        for (int i = tid; i < TG_TT; i += TG_THREADS) {
-            const int lx = i % TG_T, ly = i / TG_T;
-            const int gx = x0 + lx, gy = y0 + ly;
-            if (gx >= a.nx || gy >= a.ny) continue;
-            const int64_t cell = (int64_t)gy * a.nx + gx;
-            const int li = ly * TG_LS + lx;
-            double re[P], im[P];
-            double2 old[P];
-#pragma unroll
-            for (int p = 0; p < P; p++) {
-                re[p] = s_acc[(2 * p) * ACC + li];
-                im[p] = s_acc[(2 * p + 1) * ACC + li];
-                old[p] = make_double2(0.0, 0.0);
-            }
+        // add the cell to the grid
+        if (gx < a.nx && gy < a.ny) {
+            double *g = a.grid + 2 * ((int64_t)c_local * a.chan_stride + ((int64_t)gy * a.nx + gx) * a.cs);
             if (excl) {
+                double2 old[P];
 #pragma unroll
                 for (int p = 0; p < P; p++)
-                    if (re[p] != 0.0 || im[p] != 0.0) old[p] = *(const double2 *)(g + 2 * (cell * a.cs + (int64_t)p * a.ps));
+                    old[p] = (acc[2 * p] != 0.0 || acc[2 * p + 1] != 0.0) ? *(const double2 *)(g + 2 * p * a.ps)
+                                                                         : make_double2(0.0, 0.0);
 #pragma unroll
                 for (int p = 0; p < P; p++)
-                    if (re[p] != 0.0 || im[p] != 0.0)
-                        *(double2 *)(g + 2 * (cell * a.cs + (int64_t)p * a.ps)) = make_double2(old[p].x + re[p], old[p].y + im[p]);
+                    if (acc[2 * p] != 0.0 || acc[2 * p + 1] != 0.0)
+                        *(double2 *)(g + 2 * p * a.ps) = make_double2(old[p].x + acc[2 * p], old[p].y + acc[2 * p + 1]);
             } else {
 #pragma unroll
                 for (int p = 0; p < P; p++) {
-                    double *c = g + 2 * (cell * a.cs + (int64_t)p * a.ps);
-                    if (re[p] != 0.0) unsafeAtomicAdd(c, re[p]);
-                    if (im[p] != 0.0) unsafeAtomicAdd(c + 1, im[p]);
+                    if (acc[2 * p] != 0.0) unsafeAtomicAdd(g + 2 * p * a.ps, acc[2 * p]);
+                    if (acc[2 * p + 1] != 0.0) unsafeAtomicAdd(g + 2 * p * a.ps + 1, acc[2 * p + 1]);
                 }
             }
         }
-        __syncthreads();
     }
 }
 
