@@ -317,6 +317,344 @@ static FftGeom geom(int nx, int ny)
     return g;
 }
 
+// ---- FP64 transform on 4096-element blocks (finalize path) --------------------
+//
+// A block holds NF = 4096 / N independent inverse transforms of length N in one
+// LDS buffer (padded by one element every 16, so radix-16 writes are
+// conflict-free).  Stockham passes of radix 16 (then 8, 4 or 2) keep R
+// elements per butterfly in registers: every pass reads its inputs, waits,
+// and overwrites in place.  The first pass reads HBM through a loader and
+// the last pass hands results to a storer, so each block touches HBM once
+// each way.  256 threads; twiddles from the FP64 table exp(+2 pi i t / NT).
+
+constexpr int FB_E = 4096;
+constexpr int FB_THREADS = 256;
+constexpr int FB_LDS = FB_E + FB_E / 16;
+
+__device__ __forceinline__ int fb_pad(int i) { return i + (i >> 4); }
+
+__device__ __forceinline__ double2 c2add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 c2sub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 c2mul(double2 a, double2 b)
+{
+    return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
+}
+__device__ __forceinline__ double2 c2muli(double2 a) { return make_double2(-a.y, a.x); }  // * (+i)
+
+// Inverse DFT of length 4 in place: X[k] = sum_n x[n] i^(nk).
+__device__ __forceinline__ void idft4(double2 &x0, double2 &x1, double2 &x2, double2 &x3)
+{
+    const double2 a0 = c2add(x0, x2), a1 = c2sub(x0, x2), a2 = c2add(x1, x3), a3 = c2muli(c2sub(x1, x3));
+    x0 = c2add(a0, a2);
+    x2 = c2sub(a0, a2);
+    x1 = c2add(a1, a3);
+    x3 = c2sub(a1, a3);
+}
+
+// exp(+2 pi i m / 16) for m = 0..15, as double constants.
+__device__ __forceinline__ double2 w16(int m)
+{
+    constexpr double C1 = 0.92387953251128675613, S1 = 0.38268343236508977173, H = 0.70710678118654752440;
+    switch (m & 15) {
+    case 0: return make_double2(1.0, 0.0);
+    case 1: return make_double2(C1, S1);
+    case 2: return make_double2(H, H);
+    case 3: return make_double2(S1, C1);
+    case 4: return make_double2(0.0, 1.0);
+    case 5: return make_double2(-S1, C1);
+    case 6: return make_double2(-H, H);
+    case 7: return make_double2(-C1, S1);
+    case 8: return make_double2(-1.0, 0.0);
+    case 9: return make_double2(-C1, -S1);
+    case 10: return make_double2(-H, -H);
+    case 11: return make_double2(-S1, -C1);
+    case 12: return make_double2(0.0, -1.0);
+    case 13: return make_double2(S1, -C1);
+    case 14: return make_double2(H, -H);
+    default: return make_double2(C1, -S1);
+    }
+}
+
+// Inverse DFT of length R in place (natural order in and out).
+template <int R>
+__device__ __forceinline__ void idft(double2 (&v)[R])
+{
+    if constexpr (R == 2) {
+        const double2 a = v[0];
+        v[0] = c2add(a, v[1]);
+        v[1] = c2sub(a, v[1]);
+    } else if constexpr (R == 4) {
+        idft4(v[0], v[1], v[2], v[3]);
+    } else if constexpr (R == 8) {
+        // n = 2 n1 + n2, k = k1 + 4 k2
+        double2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+        idft4(e[0], e[1], e[2], e[3]);
+        idft4(o[0], o[1], o[2], o[3]);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const double2 t = c2mul(o[k], w16(2 * k));
+            v[k] = c2add(e[k], t);
+            v[k + 4] = c2sub(e[k], t);
+        }
+    } else {
+        static_assert(R == 16, "radix");
+        // n = 4 n1 + n2, k = k1 + 4 k2: DFT4 over n1, twiddle w16^(n2 k1), DFT4 over n2
+        double2 y[4][4];
+#pragma unroll
+        for (int n2 = 0; n2 < 4; n2++) {
+            y[n2][0] = v[n2];
+            y[n2][1] = v[4 + n2];
+            y[n2][2] = v[8 + n2];
+            y[n2][3] = v[12 + n2];
+            idft4(y[n2][0], y[n2][1], y[n2][2], y[n2][3]);
+#pragma unroll
+            for (int k1 = 1; k1 < 4; k1++)
+                if (n2 > 0) y[n2][k1] = c2mul(y[n2][k1], w16(n2 * k1));
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < 4; k1++) {
+            idft4(y[0][k1], y[1][k1], y[2][k1], y[3][k1]);
+            v[k1] = y[0][k1];
+            v[k1 + 4] = y[1][k1];
+            v[k1 + 8] = y[2][k1];
+            v[k1 + 12] = y[3][k1];
+        }
+    }
+}
+
+// LDS index of element n of transform f: row layout f*N + n (transforms of a
+// row block, lanes along n) or column layout n*NF + f (column blocks, lanes
+// along f so that HBM rows are read and written contiguously).
+template <int N, bool COL>
+__device__ __forceinline__ int fb_idx(int f, int n)
+{
+    return COL ? n * (FB_E / N) + f : fb_pad(f * N + n);
+}
+
+// One Stockham pass of radix R at stride NS over the block.  load(f, n) gives
+// element n of transform f (FIRST); store(f, k, v) takes output k (LAST).
+template <int N, bool COL, int R, int NS, bool FIRST, bool LAST, class Load, class Store>
+__device__ __forceinline__ void fb_pass(double2 *lds, const double2 *tw, int NT, const Load &load,
+                                        const Store &store)
+{
+    constexpr int PER = N / R;
+    constexpr int NF = FB_E / N;
+    constexpr int B = FB_E / (R * FB_THREADS);
+    static_assert(B >= 1, "block too small for the radix");
+    double2 v[B][R];
+#pragma unroll
+    for (int s = 0; s < B; s++) {
+        const int b = threadIdx.x + FB_THREADS * s;
+        const int f = COL ? b % NF : b / PER, j = COL ? b / NF : b % PER;
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            v[s][r] = FIRST ? load(f, j + PER * r) : lds[fb_idx<N, COL>(f, j + PER * r)];
+    }
+    if (!FIRST) __syncthreads();
+#pragma unroll
+    for (int s = 0; s < B; s++) {
+        const int b = threadIdx.x + FB_THREADS * s;
+        const int f = COL ? b % NF : b / PER, j = COL ? b / NF : b % PER;
+        const int k = j % NS;
+        if (NS > 1) {
+            const int step = NT / (NS * R);
+#pragma unroll
+            for (int r = 1; r < R; r++) v[s][r] = c2mul(v[s][r], tw[r * k * step]);
+        }
+        idft<R>(v[s]);
+        const int base = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            if (LAST) store(f, base + NS * r, v[s][r]);
+            else lds[fb_idx<N, COL>(f, base + NS * r)] = v[s][r];
+        }
+    }
+    if (!LAST) __syncthreads();
+}
+
+template <int N, bool COL, int NS, bool FIRST, class Load, class Store>
+__device__ __forceinline__ void fb_passes(double2 *lds, const double2 *tw, int NT, const Load &load,
+                                          const Store &store)
+{
+    constexpr int REM = N / NS;
+    constexpr int R = REM >= 16 ? 16 : REM;
+    fb_pass<N, COL, R, NS, FIRST, REM == R>(lds, tw, NT, load, store);
+    if constexpr (REM > R) fb_passes<N, COL, NS * R, false>(lds, tw, NT, load, store);
+}
+
+// Full inverse transforms of the block: load -> passes -> store.
+template <int N, bool COL, class Load, class Store>
+__device__ __forceinline__ void fb_ifft(double2 *lds, const double2 *tw, int NT, const Load &load,
+                                        const Store &store)
+{
+    static_assert(N >= 16 && N <= FB_E, "length");
+    fb_passes<N, COL, 1, true>(lds, tw, NT, load, store);
+}
+
+// Pass A: rows.  Block = NF = 4096/N rows (y0 .. y0+NF) of plane (c, p): grid
+// element (c, y, gx, p) at (c*nx*ny + y*nx + gx)*cs*... with the
+// PreIFFTUVCorrection reorder (ix = (gx - nx/2) mod nx, iy = (y - ny/2) mod ny)
+// into X[c][p][iy][kx].
+template <int N, typename TI>
+__global__ __launch_bounds__(FB_THREADS) void k_rows_fb(const TI *in, double2 *X, int ny, int P,
+                                                       const double2 *tw, int NT, int64_t cs, int64_t ps)
+{
+    __shared__ double2 lds[FB_LDS];
+    constexpr int NF = FB_E / N;
+    const int y0 = blockIdx.x * NF, p = blockIdx.y, c = blockIdx.z;
+    const TI *src = in + (size_t)c * N * ny * P + (size_t)p * ps;
+    double2 *dst = X + ((size_t)c * P + p) * N * ny;
+    auto load = [&](int f, int ix) {
+        const int gx = (ix + N / 2) & (N - 1);
+        const TI v = src[((size_t)(y0 + f) * N + gx) * cs];
+        return make_double2((double)v.x, (double)v.y);
+    };
+    auto store = [&](int f, int kx, double2 v) {
+        const int iy = (y0 + f - (ny >> 1)) & (ny - 1);
+        dst[(size_t)iy * N + kx] = v;
+    };
+    fb_ifft<N, false>(lds, tw, NT, load, store);
+}
+
+// Pass B1: length-N1 transforms down the columns (rows N2*n1 + n2) of NF =
+// 4096/N1 columns starting at x0, times exp(+2 pi i n2 k1 / ny), in place.
+template <int N1>
+__global__ __launch_bounds__(FB_THREADS) void k_cols1_fb(double2 *X, int nx, int ny, int N2, const double2 *tw,
+                                                        int NT)
+{
+    __shared__ double2 lds[FB_LDS];
+    constexpr int NF = FB_E / N1;
+    const int x0 = blockIdx.x * NF, n2 = blockIdx.y;
+    double2 *plane = X + (size_t)blockIdx.z * nx * ny;
+    const int mul = NT / ny;
+    // transform f = column x0 + f; loads with the column index fastest across lanes
+    auto load = [&](int f, int n1) { return plane[((size_t)N2 * n1 + n2) * nx + x0 + f]; };
+    auto store = [&](int f, int k1, double2 v) {
+        plane[((size_t)N2 * k1 + n2) * nx + x0 + f] = c2mul(v, tw[((n2 * k1) & (ny - 1)) * mul]);
+    };
+    fb_ifft<N1, true>(lds, tw, NT, load, store);
+}
+
+// Pass B2 + finalize: channel c, index k1, columns [x0, x0 + COLS) of all P
+// planes: rows N2*k1 + n2 -> length-N2 transforms -> ky = k1 + N1*k2, then
+// fftshift, normalise, Stokes, m flip and both corrections -> f32 image.
+template <int N2, int P>
+__global__ __launch_bounds__(FB_THREADS) void k_cols2_fb(const double2 *X, int nx, int ny, int N1,
+                                                        const double2 *tw, int NT, const double *norm,
+                                                        int poltype, const float *corr_x, const float *corr_y,
+                                                        float *image)
+{
+    __shared__ double2 lds[FB_LDS];
+    constexpr int NF = FB_E / N2;  // P * COLS transforms
+    constexpr int COLS = NF / P;
+    static_assert(COLS >= 1, "columns per block");
+    const int x0 = blockIdx.x * COLS, k1 = blockIdx.y, c = blockIdx.z;
+    auto load = [&](int f, int n2) {
+        const int p = f / COLS, cx = f - p * COLS;
+        return X[(((size_t)c * P + p) * ny + (size_t)N2 * k1 + n2) * nx + x0 + cx];
+    };
+    auto store = [&](int f, int k2, double2 v) { lds[k2 * NF + f] = v; };
+    fb_ifft<N2, true>(lds, tw, NT, load, store);
+    __syncthreads();
+    double inv[P];
+#pragma unroll
+    for (int p = 0; p < P; p++) inv[p] = 1.0 / norm[c * P + p];
+    float *img = image + (size_t)c * P * nx * ny;
+    for (int i = threadIdx.x; i < N2 * COLS; i += FB_THREADS) {
+        const int k2 = i / COLS, cx = i - k2 * COLS;
+        const int ky = k1 + N1 * k2, kx = x0 + cx;
+        const int jy = (ky + (ny >> 1)) & (ny - 1), jx = (kx + (nx >> 1)) & (nx - 1);
+        const int row = ny - 1 - jy;  // m axis flipped for FITS
+        double re[P], im[P];
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            const double2 v = lds[k2 * NF + p * COLS + cx];
+            re[p] = v.x * inv[p];
+            im[p] = v.y * inv[p];
+        }
+        double o[P];
+        if constexpr (P == 1) {
+            o[0] = re[0];
+        } else if constexpr (P == 2) {
+            o[0] = (re[0] + re[1]) * 0.5;
+            o[1] = (re[0] - re[1]) * 0.5;
+        } else {
+            if (poltype == MTI_POL_LINEAR) {
+                o[0] = (re[0] + re[3]) * 0.5;
+                o[1] = (re[0] - re[3]) * 0.5;
+                o[2] = (re[1] + re[2]) * 0.5;
+                o[3] = (im[1] - im[2]) * 0.5;
+            } else {
+                o[0] = (re[0] + re[3]) * 0.5;
+                o[1] = (re[1] + re[2]) * 0.5;
+                o[2] = (im[2] - im[1]) * 0.5;
+                o[3] = (re[0] - re[3]) * 0.5;
+            }
+        }
+        const double cf = (double)corr_x[jx] * (double)corr_y[row];
+#pragma unroll
+        for (int q = 0; q < P; q++) img[((size_t)q * ny + row) * nx + jx] = (float)(o[q] * cf);
+    }
+}
+
+// Host dispatch of the block-transform path (nx, N1, N2 >= 16 and the blocks
+// fit the image).
+static bool fb_ok(const FftGeom &g, int P)
+{
+    return g.nx >= 16 && g.N1 >= 16 && g.N2 >= 16 && FB_E / g.nx <= g.ny && FB_E / g.N1 <= g.nx &&
+           FB_E / (g.N2 * P) <= g.nx && FB_E / (g.N2 * P) >= 1;
+}
+
+template <typename TI>
+static mti_status fb_rows(mti_ctx *ctx, const FftGeom &g, const TI *G, double2 *X, int P, int nch,
+                          const double2 *tw, int64_t cs, int64_t ps, hipStream_t st)
+{
+    const dim3 gr((unsigned)(g.ny / (FB_E / g.nx)), (unsigned)P, (unsigned)nch);
+    switch (g.nx) {
+#define MTI_ROWS(NN) \
+    case NN: hipLaunchKernelGGL((k_rows_fb<NN, TI>), gr, dim3(FB_THREADS), 0, st, G, X, g.ny, P, tw, g.NT, cs, ps); break;
+        MTI_ROWS(16) MTI_ROWS(32) MTI_ROWS(64) MTI_ROWS(128) MTI_ROWS(256) MTI_ROWS(512) MTI_ROWS(1024)
+        MTI_ROWS(2048) MTI_ROWS(4096)
+#undef MTI_ROWS
+    default: return set_error(ctx, MTI_ERR_UNSUPPORTED, "row length");
+    }
+    MTI_HIP(ctx, hipGetLastError());
+    return MTI_OK;
+}
+
+static mti_status fb_cols1(mti_ctx *ctx, const FftGeom &g, double2 *X, int planes, const double2 *tw,
+                           hipStream_t st)
+{
+    const dim3 gr((unsigned)(g.nx / (FB_E / g.N1)), (unsigned)g.N2, (unsigned)planes);
+    switch (g.N1) {
+#define MTI_C1(NN) \
+    case NN: hipLaunchKernelGGL((k_cols1_fb<NN>), gr, dim3(FB_THREADS), 0, st, X, g.nx, g.ny, g.N2, tw, g.NT); break;
+        MTI_C1(16) MTI_C1(32) MTI_C1(64) MTI_C1(128)
+#undef MTI_C1
+    default: return set_error(ctx, MTI_ERR_UNSUPPORTED, "column split");
+    }
+    MTI_HIP(ctx, hipGetLastError());
+    return MTI_OK;
+}
+
+template <int P>
+static mti_status fb_cols2(mti_ctx *ctx, const FftGeom &g, const double2 *X, int nch, const double2 *tw,
+                           const double *norm, int poltype, const float *cx, const float *cy, float *image,
+                           hipStream_t st)
+{
+    const int cols = FB_E / (g.N2 * P);
+    const dim3 gr((unsigned)(g.nx / cols), (unsigned)g.N1, (unsigned)nch);
+    switch (g.N2) {
+#define MTI_C2(NN) \
+    case NN: hipLaunchKernelGGL((k_cols2_fb<NN, P>), gr, dim3(FB_THREADS), 0, st, X, g.nx, g.ny, g.N1, tw, g.NT, norm, poltype, cx, cy, image); break;
+        MTI_C2(16) MTI_C2(32) MTI_C2(64)
+#undef MTI_C2
+    default: return set_error(ctx, MTI_ERR_UNSUPPORTED, "column split");
+    }
+    MTI_HIP(ctx, hipGetLastError());
+    return MTI_OK;
+}
+
 template <int MODE, typename C, typename TI>
 static mti_status rows_pass(mti_ctx *ctx, const FftGeom &g, const TI *in, C *out, int P, int nch,
                             const double2 *tw, hipStream_t st, int64_t cs = 1, int64_t ps = 1)
@@ -368,6 +706,20 @@ static mti_status finalize_batch(mti_ctx *ctx, const FftGeom &g, const TI *G, do
     mti_status r;
     const int64_t cs = layout == MTI_GRID_VUP ? P : 1;
     const int64_t ps = layout == MTI_GRID_VUP ? 1 : (int64_t)g.nx * g.ny;
+    if (fb_ok(g, P)) {
+        {
+            StageTimer tm(ctx, MTI_STAGE_FFT_ROWS, st);
+            r = fb_rows<TI>(ctx, g, G, X, P, nb, tw, cs, ps, st);
+            if (r != MTI_OK) return r;
+        }
+        {
+            StageTimer tm(ctx, MTI_STAGE_FFT_COLS, st);
+            r = fb_cols1(ctx, g, X, nb * P, tw, st);
+            if (r != MTI_OK) return r;
+        }
+        StageTimer tm(ctx, MTI_STAGE_FINALIZE, st);
+        return fb_cols2<P>(ctx, g, X, nb, tw, nr, poltype, cx, cy, img, st);
+    }
     {
         StageTimer tm(ctx, MTI_STAGE_FFT_ROWS, st);
         r = rows_pass<0, double2, TI>(ctx, g, G, X, P, nb, tw, st, cs, ps);

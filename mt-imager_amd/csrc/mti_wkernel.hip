@@ -15,10 +15,11 @@
 //
 // MI355X design: the FFT of a zero-padded plane followed by a crop is
 // evaluated as a separable DFT restricted to the window actually kept (the
-// same numbers; the padded FFT is never materialised), in FP64 with an LDS-free
-// incremental twiddle walk; the support search uses the identity "centre row
-// of the 2-D DFT = 1-D DFT of the column sums".  Only one host sync (support
-// read-back) per build.
+// same numbers; the padded FFT is never materialised): two tiled FP64 complex
+// matrix products per plane, all planes of a batch in one launch, with the
+// DFT matrix generated from an FP64 twiddle table while tiles are staged.
+// The support search uses the identity "centre row of the 2-D DFT = 1-D DFT
+// of the column sums".  One host sync (support read-back) per build.
 #include "mti_internal.hpp"
 
 #include <hip/hip_bf16.h>
@@ -145,30 +146,6 @@ __global__ __launch_bounds__(SUP_THREADS) void k_support(const double2 *scr, Scr
     }
 }
 
-// Stage 1 of the windowed DFT along x for one plane:
-// A[iy][j] = sum_cx scr[iy][cx] * tw[(cx*(qmin+j)) mod L], j < W
-__global__ void k_stage1(const double2 *scr, ScreenGeom g, const double2 *tw, int qmin, int W,
-                         double2 *A)
-{
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int iy = blockIdx.y;
-    if (j >= W) return;
-    const int q = qmin + j;
-    const int qm = mod_pos(q, g.L);
-    int m = mod_pos(-(long long)g.h * q, g.L);
-    const double2 *row = scr + (size_t)iy * g.St;
-    double re = 0, im = 0;
-    for (int c = 0; c < g.St; c++) {
-        double2 t = tw[m];
-        double2 v = row[c];
-        re += v.x * t.x - v.y * t.y;
-        im += v.x * t.y + v.y * t.x;
-        m += qm;
-        if (m >= g.L) m -= g.L;
-    }
-    A[(size_t)iy * W + j] = make_double2(re, im);
-}
-
 template <int PREC>
 __device__ __forceinline__ void store_sample(void *table, int64_t idx, double re, double im)
 {
@@ -192,35 +169,6 @@ __device__ __forceinline__ float2 load_sample(const void *table, int64_t idx)
     if (PREC == MTI_TABLE_F16) return __half22float2(((const __half2 *)table)[idx]);
     __hip_bfloat162 v = ((const __hip_bfloat162 *)table)[idx];
     return make_float2(__bfloat162float(v.x), __bfloat162float(v.y));
-}
-
-// Stage 2 (along y) written straight into the [oy][ox][sy][sx] table layout.
-template <int PREC>
-__global__ void k_stage2(const double2 *A, ScreenGeom g, const double2 *tw, int qmin, int W,
-                         int S, int T, double inv_div, void *table, int64_t base)
-{
-    const int64_t S2 = (int64_t)S * S;
-    const int64_t n = (int64_t)T * T * S2;
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    const int o = (int)(e / S2);
-    const int t = (int)(e - (int64_t)o * S2);
-    const int oy = o / T, ox = o - oy * T;
-    const int sy = t / S, sx = t - sy * S;
-    const int jy = sy * g.s + oy, jx = sx * g.s + ox;
-    const int qy = qmin + jy;
-    const int qm = mod_pos(qy, g.L);
-    int m = mod_pos(-(long long)g.h * qy, g.L);
-    double re = 0, im = 0;
-    for (int c = 0; c < g.St; c++) {
-        double2 tt = tw[m];
-        double2 v = A[(size_t)c * W + jx];
-        re += v.x * tt.x - v.y * tt.y;
-        im += v.x * tt.y + v.y * tt.x;
-        m += qm;
-        if (m >= g.L) m -= g.L;
-    }
-    store_sample<PREC>(table, base + e, re * inv_div, im * inv_div);
 }
 
 // taper-only table (mode=normal): op/CasaGridSF.cu:107-149 + reorganize
@@ -260,6 +208,107 @@ __global__ void k_sums(const void *table, const int2 *convinfo, int nplanes, int
     sums[id] = (float)acc;
 }
 
+// The two windowed-DFT stages as tiled complex FP64 matrix products, all
+// planes of a batch in one launch (blockIdx.z = plane).  E[c][j] =
+// tw[((c - h) q_j) mod L] is generated while a tile is staged.
+//   stage 1: A[b][iy][j] = sum_cx scr[b][iy][cx] E[cx][j]            (St x Wk)
+//   stage 2: T[b][jy][jx] = sum_iy E[iy][jy] A[b][iy][jx]            (Wk x Wk)
+// Plane b uses the centred window j < Wk(S_b) of the widest window Wmax
+// (offset (Wmax - Wk)/2), qmin = -(S/2) s - s/2.
+constexpr int GT = 32;   // output tile
+constexpr int GK = 16;   // k chunk
+
+__device__ __forceinline__ double2 tw_at(const double2 *tw, int c, int h, int q, int L)
+{
+    return tw[mod_pos((long long)(c - h) * q, L)];
+}
+
+template <int STAGE>
+__global__ __launch_bounds__(256) void k_wdft(const double2 *src, ScreenGeom g, const double2 *tw,
+                                              const int *support, int k0, int Wmax, int T, double2 *dst)
+{
+    __shared__ double2 Xs[GT][GK + 1];
+    __shared__ double2 Ys[GK][GT + 1];
+    const int b = blockIdx.z;
+    const int S = support[k0 + b];
+    if (S == 0) return;
+    const int Wk = (S - 1) * g.s + T;
+    const int qmin = -(S / 2) * g.s - g.s / 2;
+    const int M = STAGE == 1 ? g.St : Wk, N = Wk, K = g.St;
+    const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
+    if (m0 >= M || n0 >= N) return;
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    const double2 *Sb = src + (size_t)b * g.St * (STAGE == 1 ? g.St : Wmax);
+    double2 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = make_double2(0.0, 0.0);
+    for (int k0c = 0; k0c < K; k0c += GK) {
+        // X tile [GT][GK]: stage 1 screen rows; stage 2 E[k][m] (transposed)
+        for (int e = tid; e < GT * GK; e += 256) {
+            const int mi = e / GK, ki = e - mi * GK;
+            const int m = m0 + mi, k = k0c + ki;
+            double2 v = make_double2(0.0, 0.0);
+            if (m < M && k < K) v = STAGE == 1 ? Sb[(size_t)m * g.St + k] : tw_at(tw, k, g.h, qmin + m, g.L);
+            Xs[mi][ki] = v;
+        }
+        // Y tile [GK][GT]: stage 1 E[k][n]; stage 2 A[k][n]
+        for (int e = tid; e < GK * GT; e += 256) {
+            const int ki = e / GT, ni = e - ki * GT;
+            const int k = k0c + ki, n = n0 + ni;
+            double2 v = make_double2(0.0, 0.0);
+            if (n < N && k < K) v = STAGE == 1 ? tw_at(tw, k, g.h, qmin + n, g.L) : Sb[(size_t)k * Wmax + n];
+            Ys[ki][ni] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ki = 0; ki < GK; ki++) {
+            double2 x[2], y[2];
+#pragma unroll
+            for (int i = 0; i < 2; i++) x[i] = Xs[ty * 2 + i][ki];
+#pragma unroll
+            for (int j = 0; j < 2; j++) y[j] = Ys[ki][tx * 2 + j];
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++) {
+                    acc[i][j].x = fma(x[i].x, y[j].x, fma(-x[i].y, y[j].y, acc[i][j].x));
+                    acc[i][j].y = fma(x[i].x, y[j].y, fma(x[i].y, y[j].x, acc[i][j].y));
+                }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int m = m0 + ty * 2 + i, n = n0 + tx * 2 + j;
+            if (m < M && n < N) dst[((size_t)b * (STAGE == 1 ? g.St : Wmax) + m) * Wmax + n] = acc[i][j];
+        }
+}
+
+// Stage-2 products (T[jy][jx], jy, jx < Wk) scaled by 1/divisor into the
+// [oy][ox][sy][sx] layout of plane k0 + b.
+template <int PREC>
+__global__ void k_wstore(const double2 *Tm, ScreenGeom g, const int *support, const int64_t *base, int k0,
+                         int Wmax, int T, double inv_div, void *table)
+{
+    const int b = blockIdx.y;
+    const int S = support[k0 + b];
+    const int64_t S2 = (int64_t)S * S;
+    const int64_t n = (int64_t)T * T * S2;
+    const double2 *Tb = Tm + (size_t)b * Wmax * Wmax;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        const int o = (int)(e / S2);
+        const int t = (int)(e - (int64_t)o * S2);
+        const int oy = o / T, ox = o - oy * T;
+        const int sy = t / S, sx = t - sy * S;
+        const double2 v = Tb[(size_t)(sy * g.s + oy) * Wmax + sx * g.s + ox];
+        store_sample<PREC>(table, base[k0 + b] + e, v.x * inv_div, v.y * inv_div);
+    }
+}
+
 double host_gridsf(double nu)
 {
     static const double P[2][5] = {
@@ -283,14 +332,6 @@ double host_gridsf(double nu)
 }
 
 size_t elem_bytes(int prec) { return prec == MTI_TABLE_F32 ? 8 : 4; }
-
-template <int PREC>
-void launch_stage2(dim3 gr, dim3 bl, hipStream_t st, const double2 *A, ScreenGeom g,
-                   const double2 *tw, int qmin, int W, int S, int T, double inv_div, void *table,
-                   int64_t base)
-{
-    hipLaunchKernelGGL(k_stage2<PREC>, gr, bl, 0, st, A, g, tw, qmin, W, S, T, inv_div, table, base);
-}
 
 }  // namespace
 
@@ -390,40 +431,42 @@ static mti_status build_impl(mti_ctx *ctx, const mti_image_params *p, int precis
             return set_error(ctx, MTI_ERR_UNSUPPORTED, "convolution table above 2^31 samples");
         MTI_HIP(ctx, hipMalloc(&K->d_table, std::max<size_t>(1, (size_t)tot) * eb));
 
-        // stage-1 buffer sized for the widest window
-        int Wmax = (std::max(smax, 1) - 1) * s + T;
-        size_t a_bytes = (size_t)St * Wmax * sizeof(double2);
+        // windowed DFT as two batched complex products (all planes of a batch)
+        const int Wmax = (std::max(smax, 1) - 1) * s + T;
+        int64_t *d_base = nullptr;
+        MTI_HIP(ctx, hipMalloc(&d_base, (size_t)W * sizeof(int64_t)));
+        MTI_HIP(ctx, hipMemcpyAsync(d_base, K->base.data(), (size_t)W * sizeof(int64_t), hipMemcpyHostToDevice, st));
+        const size_t a_plane = (size_t)St * Wmax * sizeof(double2), t_plane = (size_t)Wmax * Wmax * sizeof(double2);
         void *aw = nullptr;
-        r = scratch_get(ctx, SLOT_ITEMS, a_bytes, &aw);
-        if (r != MTI_OK) return r;
+        r = scratch_get(ctx, SLOT_ITEMS, (size_t)nb * (a_plane + t_plane) + 256, &aw);
+        if (r != MTI_OK) {
+            (void)hipFree(d_base);
+            return r;
+        }
         double2 *d_A = (double2 *)aw;
+        double2 *d_T = (double2 *)((char *)aw + (size_t)nb * a_plane);
         for (int k0 = 0; k0 < W; k0 += nb) {
             int n = std::min(nb, W - k0);
             dim3 sg((unsigned)ceil_div(St, 16), (unsigned)ceil_div(St, 16), (unsigned)n);
             hipLaunchKernelGGL(k_screen, sg, sb, 0, st, d_scr, g, K->info.w_increment, k0);
+            dim3 g1((unsigned)ceil_div(Wmax, GT), (unsigned)ceil_div(St, GT), (unsigned)n);
+            hipLaunchKernelGGL(k_wdft<1>, g1, dim3(256), 0, st, d_scr, g, d_tw, d_sup, k0, Wmax, T, d_A);
+            dim3 g2((unsigned)ceil_div(Wmax, GT), (unsigned)ceil_div(Wmax, GT), (unsigned)n);
+            hipLaunchKernelGGL(k_wdft<2>, g2, dim3(256), 0, st, d_A, g, d_tw, d_sup, k0, Wmax, T, d_T);
+            dim3 g3(64, (unsigned)n);
+            if (precision == MTI_TABLE_F32)
+                hipLaunchKernelGGL(k_wstore<MTI_TABLE_F32>, g3, dim3(256), 0, st, d_T, g, d_sup, d_base, k0, Wmax, T,
+                                   inv, K->d_table);
+            else if (precision == MTI_TABLE_F16)
+                hipLaunchKernelGGL(k_wstore<MTI_TABLE_F16>, g3, dim3(256), 0, st, d_T, g, d_sup, d_base, k0, Wmax, T,
+                                   inv, K->d_table);
+            else
+                hipLaunchKernelGGL(k_wstore<MTI_TABLE_BF16>, g3, dim3(256), 0, st, d_T, g, d_sup, d_base, k0, Wmax,
+                                   T, inv, K->d_table);
             MTI_HIP(ctx, hipGetLastError());
-            for (int b = 0; b < n; b++) {
-                int k = k0 + b, S = K->support[k];
-                if (S == 0) continue;
-                int qmin = -(S / 2) * s - s / 2;
-                int Wk = (S - 1) * s + T;
-                hipLaunchKernelGGL(k_stage1, dim3((unsigned)ceil_div(Wk, 64), (unsigned)St), dim3(64),
-                                   0, st, d_scr + (size_t)b * St * St, g, d_tw, qmin, Wk, d_A);
-                MTI_HIP(ctx, hipGetLastError());
-                int64_t ne = (int64_t)T * T * S * S;
-                dim3 gr((unsigned)ceil_div(ne, 256)), bl(256);
-                if (precision == MTI_TABLE_F32)
-                    launch_stage2<MTI_TABLE_F32>(gr, bl, st, d_A, g, d_tw, qmin, Wk, S, T, inv,
-                                                 K->d_table, K->base[k]);
-                else if (precision == MTI_TABLE_F16)
-                    launch_stage2<MTI_TABLE_F16>(gr, bl, st, d_A, g, d_tw, qmin, Wk, S, T, inv,
-                                                 K->d_table, K->base[k]);
-                else
-                    launch_stage2<MTI_TABLE_BF16>(gr, bl, st, d_A, g, d_tw, qmin, Wk, S, T, inv,
-                                                  K->d_table, K->base[k]);
-                MTI_HIP(ctx, hipGetLastError());
-            }
         }
+        MTI_HIP(ctx, hipStreamSynchronize(st));
+        MTI_HIP(ctx, hipFree(d_base));
     }
 
     const int W = K->info.nplanes;

@@ -334,9 +334,12 @@ def test_fft2d_inverse_matches_numpy(mti, ctx, nx, ny, nb):
     assert err < 1e-6, err
 
 
-@pytest.mark.parametrize("npol,poltype", [(1, "linear"), (2, "circular"), (4, "linear"), (4, "circular")])
-def test_finalize_matches_oracle(orc, mti, ctx, npol, poltype):
-    p = mti.make_params(256, 128, 3.0, 2.0, "wproj", 8, 8)
+@pytest.mark.parametrize("nx,ny,npol,poltype", [
+    (256, 128, 1, "linear"), (256, 128, 2, "circular"), (256, 128, 4, "linear"), (256, 128, 4, "circular"),
+    # block-transform path (radix-16 passes): rectangular, P=1/2/4, the config-2 size
+    (512, 256, 4, "linear"), (1024, 1024, 2, "linear"), (2048, 512, 1, "linear"), (4096, 4096, 4, "circular")])
+def test_finalize_matches_oracle(orc, mti, ctx, nx, ny, npol, poltype):
+    p = mti.make_params(nx, ny, 3.0, 2.0, "wproj", 8, 8)
     rng = np.random.default_rng(npol)
     g = rng.standard_normal((p.ny, p.nx, npol, 2))
     norm = rng.uniform(1.0, 3.0, npol)
