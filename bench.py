@@ -24,7 +24,7 @@ for _p in (ROOT, os.path.join(ROOT, "mt-imager_amd")):
         sys.path.insert(0, _p)
 
 METRIC = "Mvis/s gridded + p99 job latency, batched/served vs plain, at ≤1e-4 error"
-FP32_PEAK_TFLOPS = 157.3   # MI355X published FP32 peak
+FP64_PEAK_TFLOPS = 78.6    # MI355X published FP64 peak (vector and matrix)
 HBM_PEAK_GBS = 8000.0
 
 
@@ -171,11 +171,17 @@ def main():
 
     g_ms, g_n = stages["grid"]
     grid_avg_s = g_ms / max(g_n, 1) / 1e3
-    flops = 8.0 * points  # one complex multiply-add per tap per correlation
+    # gridder kernel: one complex multiply-add per tap per correlation, FP64
+    flops = 8.0 * points
     achieved = flops / grid_avg_s / 1e12 if grid_avg_s > 0 else 0.0
-    roofline = dict(bound="valu", kernel="k_grid_romein", achieved=round(achieved, 3),
-                    peak=FP32_PEAK_TFLOPS, unit="TFLOP/s", frac=round(achieved / FP32_PEAK_TFLOPS, 4),
-                    traffic=None, flops_per_launch=flops, avg_launch_ms=round(grid_avg_s * 1e3, 3))
+    # algorithmic bytes of the gridded records (complex64 vis + u8 flags per
+    # correlation, f32 weights per row-correlation + f64 uvw per row shared by
+    # the channels of a row)
+    rec_bytes = gridded * (9.0 * P + (4.0 * P + 24.0) / nchan)
+    roofline = dict(bound="valu", kernel="k_grid_tiles", achieved=round(achieved, 3),
+                    peak=FP64_PEAK_TFLOPS, unit="TFLOP/s", frac=round(achieved / FP64_PEAK_TFLOPS, 4),
+                    traffic=None, flops_per_launch=flops, avg_launch_ms=round(grid_avg_s * 1e3, 3),
+                    algorithmic_GBps=round(rec_bytes / grid_avg_s / 1e9, 1))
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -202,7 +208,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f64",
         "data": "synthetic",
         "config": {
             "workload": ("config2 full-pol cube: 64 ant / 8 km, 2016 baselines x 1800 steps, 64 ch "
