@@ -108,6 +108,9 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=60000)
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="run one elementwise negation over this many MiB before the timed region "
+                         "(a kernel of known HBM traffic for calibrating FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -138,6 +141,14 @@ def main():
         K.free()
         return st, K
 
+    if args.copy_probe_mb > 0:
+        n = args.copy_probe_mb * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        log(f"copy probe: {n * 8} bytes read, {n * 8} bytes written (neg kernel)")
+        del src, dst
     log("warmup")
     st, _ = step(stats=True)
     torch.cuda.synchronize()

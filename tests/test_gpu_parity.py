@@ -167,6 +167,36 @@ def test_config1_dirty_image_and_psf(orc, mti, ctx, c1, c1_params, psf, precisio
     assert g["stats"].gridded_records == ng
 
 
+# ---- config 3: precision forms --------------------------------------------
+# Kernel tables in f16 / bf16 on config-1 inputs (visibilities complex64,
+# FP64 accumulation and transform, FP64 binning and w-chirp phase); every
+# Stokes image and the PSF against the FP64 oracle with FP64 tables.
+PRECISION_FORM_TOL = {"f16": 1e-3, "bf16": 1e-2}
+
+
+@pytest.mark.parametrize("table", ["f16", "bf16"])
+def test_precision_forms_config1(orc, mti, ctx, c1, c1_params, table):
+    dev = H.Dev(c1, torch)
+    idx = H.gpu_sort(ctx, dev, c1.nant, torch)
+    rows = idx.cpu().numpy()
+    K = ctx.build_kernels(c1_params, precision=table)
+    OK = orc.Kernels(H.oracle_params(orc, c1_params))
+    op = H.oracle_params(orc, c1_params)
+    for psf in (False, True):
+        g = H.gpu_image(mti, ctx, K, c1_params, dev, torch, [0], idx, psf=psf)
+        r = H.oracle_image(orc, OK, op, c1, 0, rows, psf=psf)
+        ref_i = np.sqrt(np.mean(r["image"][0] ** 2))
+        for q, name in enumerate("IQUV"):
+            if psf and q > 0:
+                break
+            e = H.image_errors(g["image"][0][q], r["image"][q])
+            if q > 0:  # Q, U, V relative to Stokes I (unpolarised sky)
+                d = g["image"][0][q].astype(np.float64) - r["image"][q]
+                e["rel_rms_vs_I"] = float(np.sqrt(np.mean(d ** 2)) / ref_i)
+            print(f"precision form {table} {'PSF' if psf else 'Stokes ' + name}: {e}")
+            assert (e["rel_rms"] if q == 0 else e["rel_rms_vs_I"]) < PRECISION_FORM_TOL[table]
+
+
 def test_wide_field_wproj_image(orc, mti, ctx):
     """Wide field (4.3 deg), 64 planes, supports 7..11, P=1."""
     from mtimager_amd import synth
