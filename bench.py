@@ -14,6 +14,7 @@ channels.  value = gridded visibilities (row x channel samples) per second.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -67,12 +68,26 @@ def make_workload(torch, cfg, seed):
     return d
 
 
-def cpu_baseline(d, params_args, budget_rows):
-    """FP64 oracle on a bounded sample: one channel's gridding of
-    `budget_rows` baseline-major rows (Briggs weights included) and one
-    channel's finalize; extrapolated to the whole cube."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(d, params_args, budget_rows, threads):
+    """FP64 oracle on a bounded sample: per channel, Briggs weights + gridding
+    of `budget_rows` baseline-major rows and one finalize.  Timed on one
+    thread, then on `threads` threads with one channel each (the oracle is
+    reentrant C; ctypes releases the interpreter lock), and extrapolated to
+    the whole cube."""
     import numpy as np
     import oracle
+    from concurrent.futures import ThreadPoolExecutor
     from mtimager_amd import synth
     h = d["host"]
     rows = oracle.sort_index(h["a1"], h["a2"], d["nant"])
@@ -80,23 +95,216 @@ def cpu_baseline(d, params_args, budget_rows):
     nx, cell, W, s = params_args
     p = oracle.make_params(nx, nx, cell * synth.ARCSEC, cell * synth.ARCSEC, "wproj", W, s)
     K = oracle.Kernels(p)
-    chan = d["nchan"] // 2
+    n, P, nchan = len(sample), d["P"], d["nchan"]
     rng = np.random.default_rng(0)
-    n = len(sample)
-    P = d["P"]
     vis = (rng.standard_normal((n, 1, P)) + 1j * rng.standard_normal((n, 1, P))).astype(np.complex64)
     flags = np.zeros((n, 1, P), np.uint8)
     weight = np.ones((n, P), np.float32)
-    il = np.array([h["inv_lambda"][chan]])
+    uvw = np.ascontiguousarray(h["uvw"][sample])
+
+    def one(chan):
+        il = np.array([h["inv_lambda"][chan]])
+        t0 = time.perf_counter()
+        wts = oracle.weights(K, uvw, il, flags, weight, 0, "briggs", 0.0)
+        g, nrm, ng = oracle.grid(K, uvw, il, vis, flags, weight, 0, wts=wts)
+        t1 = time.perf_counter()
+        oracle.finalize(p, g, nrm)
+        return t1 - t0, time.perf_counter() - t1, ng
+
+    tg1, tf1, ng1 = one(nchan // 2)
+    chans = [int(c) for c in np.linspace(0, nchan - 1, threads)]
     t0 = time.perf_counter()
-    wts = oracle.weights(K, h["uvw"][sample], il, flags, weight, 0, "briggs", 0.0)
-    g, nrm, ng = oracle.grid(K, h["uvw"][sample], il, vis, flags, weight, 0, wts=wts)
-    t_grid = time.perf_counter() - t0
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        rs = list(ex.map(one, chans))
+    wall = time.perf_counter() - t0
+    tgN = float(np.mean([r[0] / max(r[2], 1) for r in rs]))  # s per gridded record, under contention
+    tfN = float(np.mean([r[1] for r in rs]))
+    return dict(per_record_s=tg1 / max(ng1, 1), finalize_s=tf1, sample_rows=n, sample_gridded=ng1,
+                per_record_s_threads=tgN, finalize_s_threads=tfN, threads=threads, wall_threads_s=wall)
+
+
+def snapshot_pool(torch, mti, njobs, seed=4):
+    """The snapshot-job mix (Stokes I, 10k-5M visibilities log-uniform,
+    256^2-2048^2, 8-64 w-planes, s 4/8) with its inputs resident on the GPU:
+    visibilities predicted from 20-source skies, 5% random flags, weights 1."""
+    from mtimager_amd import synth
+    specs = synth.snapshot_jobs(njobs, seed=seed)
+    log(f"{len(specs)} snapshot jobs, predicting visibilities on the GPU")
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    jobs = []
+    for sp in specs:
+        nrows, nch = len(sp["uvw"]), sp["nchan"]
+        vis = synth.predict_torch(sp["uvw"], sp["inv_lambda"], sp["src"], 1)
+        flags = (torch.rand((nrows, nch), generator=g, device="cuda") < 0.05).to(torch.uint8)[:, :, None].contiguous()
+        jobs.append(dict(
+            p=mti.make_params(sp["nx"], sp["nx"], sp["cell"], sp["cell"], "wproj", sp["wplanes"], sp["sampling"]),
+            key=(sp["nx"], sp["cell"], sp["wplanes"], sp["sampling"]), nch=nch, nx=sp["nx"], nrows=nrows,
+            uvw=torch.tensor(sp["uvw"], dtype=torch.float64, device="cuda"),
+            invl=torch.tensor(sp["inv_lambda"], dtype=torch.float64, device="cuda"),
+            vis=vis, flags=flags, weight=torch.ones((nrows, 1), dtype=torch.float32, device="cuda"),
+            a1=torch.tensor(sp["ant1"], device="cuda"), a2=torch.tensor(sp["ant2"], device="cuda")))
+    torch.cuda.synchronize()
+    return jobs
+
+
+def run_c4(args, torch, mti, jobs):
+    """Config 4: independent snapshot jobs (Stokes I) imaged one at a time
+    (plain: per job kernel tables, row sort, gridding, finalize) and batched
+    (kernel-table cache, jobs admitted in waves by an HBM budget, one
+    mti_grid_jobs + one mti_finalize_jobs call per wave).  Returns a dict."""
+    import numpy as np
+    ctx = mti.Context(0)
+    budget = args.c4_budget_gb * (1 << 30)
+
+    def prep(j):
+        idx = torch.empty(j["nrows"], dtype=torch.int32, device="cuda")
+        kept = ctx.sort_rows(j["a1"], j["a2"], 64, idx)
+        return mti.vis_ms(j["uvw"], j["invl"], j["vis"], j["flags"], j["weight"], idx[:kept])
+
+    def outputs(j):
+        nx, nch = j["nx"], j["nch"]
+        return (torch.empty((nch, nx, nx, 1, 2), dtype=torch.float64, device="cuda"),
+                torch.empty((nch, 1), dtype=torch.float64, device="cuda"),
+                torch.empty((nch, 1, nx, nx), dtype=torch.float32, device="cuda"))
+
+    def plain():
+        lat, gridded = [], 0
+        t0 = time.perf_counter()
+        for n, j in enumerate(jobs):
+            if n % 64 == 0:
+                log(f"c4: plain job {n}")
+            v = prep(j)
+            K = ctx.build_kernels(j["p"])
+            gr, nr, im = outputs(j)
+            st = ctx.grid(K, j["p"], v, gr, nr, chan0=0, nchan=j["nch"], precision="c128", stats=True)
+            ctx.finalize(j["p"], 1, gr, nr, im, nchan=j["nch"], precision="c128")
+            torch.cuda.synchronize()
+            K.free()
+            gridded += st.gridded_records
+            lat.append(time.perf_counter() - t0)
+        return time.perf_counter() - t0, gridded, lat
+
+    def batched():
+        lat, gridded, cache, wave, mem = [], 0, {}, [], 0
+        t0 = time.perf_counter()
+
+        def flush():
+            nonlocal gridded, wave, mem
+            if not wave:
+                return
+            js = [w[0] for w in wave]
+            st = ctx.grid_jobs(js)
+            ctx.finalize_jobs(js)
+            torch.cuda.synchronize()
+            log(f"c4: wave of {len(js)} jobs done")
+            gridded += st.gridded_records
+            lat.extend([time.perf_counter() - t0] * len(wave))
+            wave, mem = [], 0
+
+        for j in jobs:
+            if j["key"] not in cache:
+                cache[j["key"]] = ctx.build_kernels(j["p"])
+            need = j["nch"] * j["nx"] ** 2 * 20
+            if wave and mem + need > budget:
+                flush()
+            v = prep(j)
+            gr, nr, im = outputs(j)
+            wave.append((ctx.make_job(cache[j["key"]], j["p"], v, gr, nr, im, 0, j["nch"]), gr, nr, im))
+            mem += need
+        flush()
+        el = time.perf_counter() - t0
+        for K in cache.values():
+            K.free()
+        return el, gridded, lat, len(cache)
+
+    log("c4: warm-up (batched)")
+    batched()
+    log("c4: plain, one job at a time")
+    tp, gp, lp = plain()
+    log(f"c4: plain {tp:.2f} s")
+    tb, gb, lb, nk = batched()
+    log(f"c4: batched {tb:.2f} s ({nk} kernel-table sets)")
+    ctx.close()
+    assert gp == gb, (gp, gb)
+    lp, lb = np.array(lp) * 1e3, np.array(lb) * 1e3
+    return dict(jobs=len(jobs), gridded_vis=int(gb), kernel_sets=nk, hbm_budget_gb=args.c4_budget_gb,
+                plain=dict(mvis_s=round(gp / tp / 1e6, 3), seconds=round(tp, 3),
+                           latency_ms={"p50": round(float(np.percentile(lp, 50)), 1),
+                                       "p99": round(float(np.percentile(lp, 99)), 1)}),
+                batched=dict(mvis_s=round(gb / tb / 1e6, 3), seconds=round(tb, 3),
+                             latency_ms={"p50": round(float(np.percentile(lb, 50)), 1),
+                                         "p99": round(float(np.percentile(lb, 99)), 1)}),
+                speedup=round(tp / tb, 3))
+
+
+def _pct(a, q):
+    import numpy as np
+    return round(float(np.percentile(np.asarray(a) * 1e3, q)), 1) if len(a) else None
+
+
+def _serve_run(torch, mti, serve, pool, picks, times, plain, budget, cache, deadline_s):
+    """Feeds requests at their arrival times into a Server; returns its figures."""
+    reqs = []
+    for i, k in enumerate(picks):
+        j = pool[k]
+        reqs.append(serve.Request(i, j["p"], (j["uvw"], j["invl"], j["vis"], j["flags"], j["weight"]),
+                                  j["a1"], j["a2"], 64, j["nch"], npol=1, keep_image=False))
+    srv = serve.Server(budget, workers=2, max_wave=512, cache=cache, plain=plain)
     t0 = time.perf_counter()
-    oracle.finalize(p, g, nrm)
-    t_fin = time.perf_counter() - t0
-    per_rec = t_grid / max(ng, 1)
-    return dict(per_record_s=per_rec, finalize_s=t_fin, sample_rows=n, sample_gridded=ng)
+    for r, at in zip(reqs, times):
+        dt = t0 + at - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        srv.submit(r)
+    finished = srv.wait(timeout=max(1.0, t0 + deadline_s - time.perf_counter()))
+    t_end = time.perf_counter()
+    srv.close()
+    if srv.errors:
+        raise srv.errors[0]
+    done = [r for r in reqs if r.done is not None]
+    lat = [r.done - r.arrival for r in done]
+    # requests still queued at the deadline count with their waiting time so far
+    lat_all = lat + [t_end - r.arrival for r in reqs if r.done is None]
+    last = max((r.done for r in done), default=t_end)
+    return dict(requests=len(reqs), completed=len(done), saturated=not finished,
+                mvis_s=round(srv.gridded / max(last - t0, 1e-9) / 1e6, 3),
+                latency_ms={"p50": _pct(lat_all, 50), "p99": _pct(lat_all, 99)},
+                waves=len(srv.wave_sizes),
+                mean_wave=round(float(sum(srv.wave_sizes)) / max(1, len(srv.wave_sizes)), 2))
+
+
+def run_c5(args, torch, mti, pool):
+    """Config 5: Poisson streams of imaging requests drawn from the snapshot
+    mix, served by continuous batching (two workers, HBM-budget admission,
+    device-resident kernel-table cache) and by the plain one-job-at-a-time
+    path on the same arrivals.  Inputs are resident on the GPU; each request
+    images one pool job.  A run that has not drained by the deadline reports
+    saturated=true and counts queued requests with their waiting time."""
+    import numpy as np
+    from mtimager_amd import serve
+    budget = int(args.serve_budget_gb * (1 << 30))
+    cache = serve.KernelCache()
+    rng = np.random.default_rng(5)
+    log("c5: warm-up (every kernel-table set into the cache, code objects loaded)")
+    warm = list(range(len(pool)))
+    _serve_run(torch, mti, serve, pool, warm, [0.0] * len(warm), False, budget, cache, 600)
+    out = []
+    for rate in [float(x) for x in args.rates.split(",")]:
+        n = int(rng.poisson(rate * args.serve_seconds))
+        times = np.sort(rng.uniform(0.0, args.serve_seconds, n))
+        picks = rng.integers(0, len(pool), n)
+        log(f"c5: {rate:g} req/s, {n} requests, served")
+        sv = _serve_run(torch, mti, serve, pool, picks, times, False, budget, cache, args.serve_seconds * 3)
+        log(f"c5: {rate:g} req/s served {sv}")
+        pl = _serve_run(torch, mti, serve, pool, picks, times, True, budget, None, args.serve_seconds * 3)
+        log(f"c5: {rate:g} req/s plain {pl}")
+        out.append(dict(rate=rate, served=sv, plain=pl,
+                        speedup_mvis=round(sv["mvis_s"] / max(pl["mvis_s"], 1e-9), 3)))
+    res = dict(pool=len(pool), seconds=args.serve_seconds, kernel_sets=len(cache),
+               cache_hits=cache.hits, cache_misses=cache.misses, rates=out)
+    cache.clear()
+    return res
 
 
 def main():
@@ -104,10 +312,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", choices=["c2", "c1"])
+    ap.add_argument("--config", default="c2", choices=["c2", "c1", "c4", "c5"])
+    ap.add_argument("--rates", default="50,200,800", help="config 5: arrival rates (req/s)")
+    ap.add_argument("--serve-seconds", type=float, default=10.0, help="config 5: arrival window per rate")
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0, help="config 5: HBM admission budget")
+    ap.add_argument("--jobs", type=int, default=512, help="config 4: number of snapshot jobs")
+    ap.add_argument("--c4-budget-gb", type=float, default=64.0, help="config 4: HBM budget of a batched wave")
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=60000)
+    ap.add_argument("--cpu-threads", type=int, default=16, help="threads of the CPU baseline (one channel each)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="run one elementwise negation over this many MiB before the timed region "
                          "(a kernel of known HBM traffic for calibrating FETCH_SIZE/WRITE_SIZE)")
@@ -122,6 +336,31 @@ def main():
     import mtimager_amd as mti
 
     torch.cuda.set_device(0)
+    if args.config == "c5":
+        r = run_c5(args, torch, mti, snapshot_pool(torch, mti, args.jobs))
+        print(json.dumps({
+            "metric": METRIC, "value": r["rates"][-1]["served"]["mvis_s"], "unit": "Mvis/s", "n_gpus": 1,
+            "steps": len(r["rates"]), "warmup": 1, "ms_per_step": None,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": f"config5 serving: Poisson arrivals at {args.rates} req/s for "
+                                   f"{args.serve_seconds} s from a pool of {r['pool']} snapshot jobs, "
+                                   f"HBM budget {args.serve_budget_gb} GB",
+                       "path": "served vs plain"},
+            "latency_ms": r["rates"][-1]["served"]["latency_ms"], "c5": r}), flush=True)
+        return
+    if args.config == "c4":
+        r = run_c4(args, torch, mti, snapshot_pool(torch, mti, args.jobs))
+        print(json.dumps({
+            "metric": METRIC, "value": r["batched"]["mvis_s"], "unit": "Mvis/s", "n_gpus": 1,
+            "steps": 1, "warmup": 1, "ms_per_step": round(r["batched"]["seconds"] * 1e3, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": f"config4 batched snapshot imaging: {r['jobs']} jobs, Stokes I, "
+                                   "10k-5M vis, 256^2-2048^2, 8-64 w-planes, s 4/8",
+                       "path": "batched vs plain"},
+            "latency_ms": r["batched"]["latency_ms"], "c4": r}), flush=True)
+        return
     d = make_workload(torch, args.config, args.seed)
     p = mti.make_params(d["nx"], d["nx"], d["cell"], d["cell"], "wproj", d["W"], d["s"])
     ctx = mti.Context(0)
@@ -196,16 +435,22 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        log("cpu baseline (FP64 oracle, 1 thread)")
-        cb = cpu_baseline(d, (nx, d["cell"], d["W"], d["s"]), args.cpu_rows)
+        thr = max(1, min(args.cpu_threads, os.cpu_count() or 1))
+        log(f"cpu baseline (FP64 oracle, 1 and {thr} threads)")
+        cb = cpu_baseline(d, (nx, d["cell"], d["W"], d["s"]), args.cpu_rows, thr)
         per_ch_rows = gridded / nchan
-        t_job = nchan * (per_ch_rows * cb["per_record_s"] + cb["finalize_s"])
-        cpu = dict(value=round(gridded / t_job / 1e6, 4), unit="Mvis/s", cores=1, kind="port",
-                   sample=(f"FP64 oracle: Briggs weights + gridding of {cb['sample_rows']} baseline-major "
-                           f"rows of one channel ({cb['sample_gridded']} gridded, "
-                           f"{cb['per_record_s'] * 1e6:.2f} us/record) and one {nx}^2 x {P} finalize "
-                           f"({cb['finalize_s']:.1f} s), extrapolated to {nchan} channels; "
-                           "W-kernel generation not counted"))
+        t_job1 = nchan * (per_ch_rows * cb["per_record_s"] + cb["finalize_s"])
+        t_jobN = math.ceil(nchan / thr) * (per_ch_rows * cb["per_record_s_threads"] + cb["finalize_s_threads"])
+        cpu = dict(value=round(gridded / t_jobN / 1e6, 4), unit="Mvis/s", cores=thr, kind="port",
+                   single_thread_value=round(gridded / t_job1 / 1e6, 4), cpu_model=cpu_model(),
+                   host_cpus=os.cpu_count(),
+                   sample=(f"FP64 oracle: per channel, Briggs weights + gridding of {cb['sample_rows']} "
+                           f"baseline-major rows ({cb['sample_gridded']} gridded) and one {nx}^2 x {P} "
+                           f"finalize; 1 thread: {cb['per_record_s'] * 1e6:.2f} us/record + "
+                           f"{cb['finalize_s']:.1f} s finalize; {thr} threads (one channel each, "
+                           f"{cb['wall_threads_s']:.1f} s wall): {cb['per_record_s_threads'] * 1e6:.2f} "
+                           f"us/record + {cb['finalize_s_threads']:.1f} s finalize per thread; "
+                           f"extrapolated to {nchan} channels; W-kernel generation not counted"))
 
     lat = np.array(times) * 1e3
     out = {

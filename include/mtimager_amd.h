@@ -213,6 +213,33 @@ mti_status mti_grid(mti_ctx *ctx, const mti_kernels *k, const mti_image_params *
                     const mti_vis *v, const mti_grid_opts *opts, void *grids,
                     double *norm, mti_grid_stats *stats, void *stream);
 
+/* ---- batched jobs (snapshot / multi-field imaging) ------------------------
+ * No counterpart in the reference (it images one MeasurementSet per process,
+ * app/mtimager.cpp:130-170): many independent jobs, each with its own
+ * parameters, convolution functions, visibilities and outputs, gridded by one
+ * persistent launch per pass (passes split only at the record/tile limits)
+ * and finalized with batched transforms per image size.  All jobs of a call
+ * share npol and the table precision; grids are complex128. */
+typedef struct {
+    const mti_kernels *kernels;  /* tables built for `params`                  */
+    mti_image_params params;
+    mti_vis vis;
+    int32_t chan0;               /* first channel of `vis` imaged              */
+    int32_t nchan;               /* channels imaged (one grid/image each)      */
+    void   *grids;               /* complex128 [nchan] grids, opts.grid_layout */
+    double *norm;                /* f64 [nchan][npol]                          */
+    float  *image;               /* f32 [nchan][npol][ny][nx] (finalize)       */
+} mti_job;
+
+/* Grids every job (opts: accumulate, psf, weighting, robust, grid_layout;
+ * grid_precision must be MTI_GRID_C128; chan0/nchan come from the jobs). */
+mti_status mti_grid_jobs(mti_ctx *ctx, int32_t njobs, const mti_job *jobs,
+                         const mti_grid_opts *opts, mti_grid_stats *stats, void *stream);
+
+/* mti_finalize of every job's grids into its image. */
+mti_status mti_finalize_jobs(mti_ctx *ctx, int32_t njobs, const mti_job *jobs,
+                             int32_t poltype, int32_t grid_layout, void *stream);
+
 /* ---- a17: 2-D inverse FFT (IFFT2D, gafw/StandardOperators/IFFT2D.cpp:86-111) --
  * Unnormalised C2C inverse of nbatch planes complex64 [nbatch][ny][nx], in
  * place.  nx, ny powers of two in [2, 8192]. */

@@ -35,17 +35,12 @@ __device__ __forceinline__ double load_invl(const VisView &v, int64_t row, int c
     return v.invl_f32 ? (double)((const float *)v.invl)[o] : ((const double *)v.invl)[o];
 }
 
-__device__ BinOut bin_record(const VisView &v, const BinGeom &g, int64_t row, int chan)
+// The FP64 binning of one record from its row's (u, v, w) and 1/lambda.
+__device__ __forceinline__ BinOut bin_core(double u, double vv, double w, double f, const BinGeom &g)
 {
     BinOut b;
     b.to_grid = 0;
     b.plane = b.S = b.cx = b.cy = b.ox = b.oy = b.conj = b.conv_idx = 0;
-    bool any = false;
-    for (int p = 0; p < v.npol; p++)
-        if (!vis_flag(v, row, chan, p)) any = true;
-    if (!any) return b;
-    double u = load_uvw(v, row, 0), vv = load_uvw(v, row, 1), w = load_uvw(v, row, 2);
-    const double f = load_invl(v, row, chan);
     int conj = 0;
     if (w < 0.0) {
         u = -u;
@@ -87,6 +82,19 @@ __device__ BinOut bin_record(const VisView &v, const BinGeom &g, int64_t row, in
     return b;
 }
 
+__device__ BinOut bin_record(const VisView &v, const BinGeom &g, int64_t row, int chan)
+{
+    bool any = false;
+    for (int p = 0; p < v.npol; p++)
+        if (!vis_flag(v, row, chan, p)) any = true;
+    if (!any) {
+        BinOut b;
+        b.to_grid = b.plane = b.S = b.cx = b.cy = b.ox = b.oy = b.conj = b.conv_idx = 0;
+        return b;
+    }
+    return bin_core(load_uvw(v, row, 0), load_uvw(v, row, 1), load_uvw(v, row, 2), load_invl(v, row, chan), g);
+}
+
 __global__ void k_bin_only(VisView v, BinGeom g, int chan, int32_t *out)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -104,49 +112,79 @@ __global__ void k_bin_only(VisView v, BinGeom g, int chan, int32_t *out)
     o[7] = b.conj;
 }
 
-__device__ __forceinline__ void bin_one(const BinPassArgs &a, int64_t t, unsigned long long &ngrid,
-                                        unsigned long long &npts)
+// Rows and channels of one binning block: a thread owns one row of the
+// job's index and walks a run of BIN_CHANS channels.
+constexpr int BIN_ROWS = 256;
+constexpr int BIN_CHANS = 64;
+
+// One row of a job over channels [c0, c1): the row's (u, v, w) and weights
+// are loaded once; record e = rec0 + c_local * nidx + i is written
+// channel-major, so the lanes of a wave (consecutive i) store contiguous
+// descriptors, visibilities and weights for each channel.
+template <int P>
+__device__ __forceinline__ void bin_row(const BinPassArgs &a, const JobDev &jd, int job, int64_t i, int c0, int c1,
+                                        unsigned long long &ngrid, unsigned long long &npts)
 {
-    const int64_t i = t / a.nb;
-    const int c_local = (int)(t - i * a.nb);
-    const int64_t e = (int64_t)c_local * a.v.nidx + i;
-    const int64_t row = a.v.row_index ? (int64_t)a.v.row_index[i] : i;
-    const int chan = a.chan0 + c_local;
-    BinOut b = bin_record(a.v, a.g, row, chan);
-    int4 d = make_int4((int)row, 0, 0, 0);
-    if (b.to_grid) {
-        d.y = (b.cy << 16) | b.cx;
-        d.z = b.conv_idx;
-        d.w = (b.plane << 16) | (b.conj << 15) | b.S;
-        ngrid += 1;
-        npts += (unsigned long long)b.S * b.S * a.v.npol;
-        const int P = a.v.npol;
+    const VisView &v = jd.v;
+    const int64_t row = v.row_index ? (int64_t)v.row_index[i] : i;
+    const double u = load_uvw(v, row, 0), vv = load_uvw(v, row, 1), w = load_uvw(v, row, 2);
+    float wgt[P];
+#pragma unroll
+    for (int p = 0; p < P; p++) wgt[p] = v.weight[row * v.w_rs + p];
+    for (int c = c0; c < c1; c++) {
+        const int chan = jd.chan0 + c;
+        const int64_t e = jd.rec0 + (int64_t)c * v.nidx + i;
+        bool fl[P];
+        bool any = false;
+#pragma unroll
         for (int p = 0; p < P; p++) {
-            const bool fl = vis_flag(a.v, row, chan, p);
-            a.wt[e * P + p] = fl ? 0.0f : a.v.weight[row * a.v.w_rs + p];
-            float2 vv = make_float2(1.0f, 0.0f);
-            if (!a.psf) vv = *(const float2 *)(a.v.vis + 2 * (row * a.v.vis_rs + (int64_t)chan * a.v.vis_cs + p));
-            if (b.conj) vv.y = -vv.y;
-            a.wv[e * P + p] = vv;
+            fl[p] = vis_flag(v, row, chan, p);
+            any |= !fl[p];
         }
+        int4 d = make_int4((int)row, 0, 0, 0);
+        if (any) {
+            const BinOut b = bin_core(u, vv, w, load_invl(v, row, chan), jd.g);
+            if (b.to_grid) {
+                d.y = (b.cy << 16) | b.cx;
+                d.z = b.conv_idx;
+                d.w = (b.plane << 16) | (b.conj << 15) | b.S;
+                ngrid += 1;
+                npts += (unsigned long long)b.S * b.S * P;
+                const float *src = v.vis + 2 * (row * v.vis_rs + (int64_t)chan * v.vis_cs);
+#pragma unroll
+                for (int p = 0; p < P; p++) {
+                    float2 x = make_float2(1.0f, 0.0f);
+                    if (!jd.psf) x = *(const float2 *)(src + 2 * p);
+                    if (b.conj) x.y = -x.y;
+                    a.wv[e * P + p] = x;
+                    a.wt[e * P + p] = fl[p] ? 0.0f : wgt[p];
+                }
+            }
+        }
+        a.cjc[e] = (job << 16) | c;
+        a.desc[e] = d;
     }
-    a.valid[e] = b.to_grid ? 1u : 0u;
-    a.desc[e] = d;
 }
 
-// Threads run over (row, channel) with the channel fastest, so a wave reads
-// one row's flags and visibilities contiguously (MeasurementSet order); the
-// record e = c_local * nidx + i is written channel-major (baseline-major
-// within a channel).  Gridded records also get their visibilities (PSF: 1+0i,
-// imaginary part negated for w < 0) and flag-masked weights, which the
-// compaction moves next to the descriptors.  Grid-stride loop: the gridded
-// record / point counts are reduced per block and added once.
-__global__ __launch_bounds__(256) void k_bin_pass(BinPassArgs a, int64_t nrec)
+// Blocks are assigned to jobs by the block prefix js.blk0 (so the job is
+// uniform in a block); a job's blocks cover its rows in runs of BIN_ROWS
+// (fastest) and its channels in runs of BIN_CHANS.  Gridded records also get
+// their visibilities (PSF: 1+0i, imaginary part negated for w < 0) and
+// flag-masked weights.  Counts are reduced per block and added once.
+template <int P>
+__global__ __launch_bounds__(BIN_ROWS) void k_bin_pass(BinPassArgs a)
 {
-    __shared__ unsigned long long s_st[4][2];
+    __shared__ unsigned long long s_st[BIN_ROWS / 64][2];
+    const int job = job_of(a.js.blk0, a.js.njobs, (int64_t)blockIdx.x);
+    const JobDev jd = a.js.jobs[job];  // uniform: scalar loads into registers
+    const int64_t lb = (int64_t)blockIdx.x - a.js.blk0[job];
+    const int64_t nrb = (jd.v.nidx + BIN_ROWS - 1) / BIN_ROWS;
+    const int64_t rb = lb % nrb;
+    const int c0 = (int)(lb / nrb) * BIN_CHANS;
+    const int c1 = min(jd.nb, c0 + BIN_CHANS);
+    const int64_t i = rb * BIN_ROWS + threadIdx.x;
     unsigned long long ngrid = 0, npts = 0;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nrec; t += (int64_t)gridDim.x * blockDim.x)
-        bin_one(a, t, ngrid, npts);
+    if (i < jd.v.nidx) bin_row<P>(a, jd, job, i, c0, c1, ngrid, npts);
     for (int o = 32; o >= 1; o >>= 1) {
         ngrid += __shfl_xor(ngrid, o, 64);
         npts += __shfl_xor(npts, o, 64);
@@ -157,9 +195,9 @@ __global__ __launch_bounds__(256) void k_bin_pass(BinPassArgs a, int64_t nrec)
     }
     __syncthreads();
     if (threadIdx.x < 2) {
-        const unsigned long long v = s_st[0][threadIdx.x] + s_st[1][threadIdx.x] + s_st[2][threadIdx.x] +
-                                     s_st[3][threadIdx.x];
-        if (v) atomicAdd(&a.stats[threadIdx.x], v);
+        unsigned long long t = 0;
+        for (int k = 0; k < BIN_ROWS / 64; k++) t += s_st[k][threadIdx.x];
+        if (t) atomicAdd(&a.stats[threadIdx.x], t);
     }
 }
 
@@ -206,13 +244,24 @@ BinGeom make_geom(const mti_kernels *k, const mti_image_params *p)
     return g;
 }
 
-mti_status launch_bin_pass(mti_ctx *ctx, const BinPassArgs &a, int64_t nrec, hipStream_t st)
+mti_status launch_bin_pass(mti_ctx *ctx, const BinPassArgs &a, int64_t nblocks, hipStream_t st)
 {
-    if (nrec <= 0) return MTI_OK;
-    const int64_t blocks = std::min<int64_t>(ceil_div(nrec, 256), (int64_t)ctx->num_cus * 16);
-    hipLaunchKernelGGL(k_bin_pass, dim3((unsigned)blocks), dim3(256), 0, st, a, nrec);
+    if (nblocks <= 0) return MTI_OK;
+    MTI_CHECK(ctx, nblocks < (int64_t(1) << 31), "binning grid above 2^31 blocks");
+    switch (a.js.P) {
+    case 1: hipLaunchKernelGGL(k_bin_pass<1>, dim3((unsigned)nblocks), dim3(BIN_ROWS), 0, st, a); break;
+    case 2: hipLaunchKernelGGL(k_bin_pass<2>, dim3((unsigned)nblocks), dim3(BIN_ROWS), 0, st, a); break;
+    case 4: hipLaunchKernelGGL(k_bin_pass<4>, dim3((unsigned)nblocks), dim3(BIN_ROWS), 0, st, a); break;
+    default: return set_error(ctx, MTI_ERR_INVALID, "npol must be 1, 2 or 4");
+    }
     MTI_HIP(ctx, hipGetLastError());
     return MTI_OK;
+}
+
+// Binning blocks of a job with nidx rows and nb channels.
+int64_t bin_blocks(int64_t nidx, int nb)
+{
+    return ceil_div(nidx, (int64_t)BIN_ROWS) * ceil_div((int64_t)nb, (int64_t)BIN_CHANS);
 }
 
 }  // namespace mti

@@ -331,6 +331,16 @@ constexpr int FB_E = 4096;
 constexpr int FB_THREADS = 256;
 constexpr int FB_LDS = FB_E + FB_E / 16;
 
+// Per-channel inputs and outputs of the block-transform finalize (one entry
+// per channel of every job of a call with the same image size).
+struct ChanDesc {
+    const void *grid;       // the channel's grid (layout strides cs, ps)
+    const double *norm;     // [P]
+    float *image;           // [P][ny][nx]
+    const float *corr_x;    // [nx] sinc / taper correction
+    const float *corr_y;    // [ny]
+};
+
 __device__ __forceinline__ int fb_pad(int i) { return i + (i >> 4); }
 
 __device__ __forceinline__ double2 c2add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
@@ -491,18 +501,18 @@ __device__ __forceinline__ void fb_ifft(double2 *lds, const double2 *tw, int NT,
     fb_passes<N, COL, 1, true>(lds, tw, NT, load, store);
 }
 
-// Pass A: rows.  Block = NF = 4096/N rows (y0 .. y0+NF) of plane (c, p): grid
-// element (c, y, gx, p) at (c*nx*ny + y*nx + gx)*cs*... with the
-// PreIFFTUVCorrection reorder (ix = (gx - nx/2) mod nx, iy = (y - ny/2) mod ny)
-// into X[c][p][iy][kx].
+// Pass A: rows.  Block = NF = 4096/N rows (y0 .. y0+NF) of plane (c, p) of
+// the channel grid cd[c].grid (element (y, gx, p) at (y*nx + gx)*cs + p*ps),
+// with the PreIFFTUVCorrection reorder (ix = (gx - nx/2) mod nx,
+// iy = (y - ny/2) mod ny) into X[c][p][iy][kx].
 template <int N, typename TI>
-__global__ __launch_bounds__(FB_THREADS) void k_rows_fb(const TI *in, double2 *X, int ny, int P,
+__global__ __launch_bounds__(FB_THREADS) void k_rows_fb(const ChanDesc *cd, double2 *X, int ny, int P,
                                                        const double2 *tw, int NT, int64_t cs, int64_t ps)
 {
     __shared__ double2 lds[FB_LDS];
     constexpr int NF = FB_E / N;
     const int y0 = blockIdx.x * NF, p = blockIdx.y, c = blockIdx.z;
-    const TI *src = in + (size_t)c * N * ny * P + (size_t)p * ps;
+    const TI *src = (const TI *)cd[c].grid + (size_t)p * ps;
     double2 *dst = X + ((size_t)c * P + p) * N * ny;
     auto load = [&](int f, int ix) {
         const int gx = (ix + N / 2) & (N - 1);
@@ -540,9 +550,7 @@ __global__ __launch_bounds__(FB_THREADS) void k_cols1_fb(double2 *X, int nx, int
 // fftshift, normalise, Stokes, m flip and both corrections -> f32 image.
 template <int N2, int P>
 __global__ __launch_bounds__(FB_THREADS) void k_cols2_fb(const double2 *X, int nx, int ny, int N1,
-                                                        const double2 *tw, int NT, const double *norm,
-                                                        int poltype, const float *corr_x, const float *corr_y,
-                                                        float *image)
+                                                        const double2 *tw, int NT, const ChanDesc *cd, int poltype)
 {
     __shared__ double2 lds[FB_LDS];
     constexpr int NF = FB_E / N2;  // P * COLS transforms
@@ -556,10 +564,12 @@ __global__ __launch_bounds__(FB_THREADS) void k_cols2_fb(const double2 *X, int n
     auto store = [&](int f, int k2, double2 v) { lds[k2 * NF + f] = v; };
     fb_ifft<N2, true>(lds, tw, NT, load, store);
     __syncthreads();
+    const double *norm = cd[c].norm;
+    const float *corr_x = cd[c].corr_x, *corr_y = cd[c].corr_y;
     double inv[P];
 #pragma unroll
-    for (int p = 0; p < P; p++) inv[p] = 1.0 / norm[c * P + p];
-    float *img = image + (size_t)c * P * nx * ny;
+    for (int p = 0; p < P; p++) inv[p] = 1.0 / norm[p];
+    float *img = cd[c].image;
     for (int i = threadIdx.x; i < N2 * COLS; i += FB_THREADS) {
         const int k2 = i / COLS, cx = i - k2 * COLS;
         const int ky = k1 + N1 * k2, kx = x0 + cx;
@@ -606,7 +616,7 @@ static bool fb_ok(const FftGeom &g, int P)
 }
 
 template <typename TI>
-static mti_status fb_rows(mti_ctx *ctx, const FftGeom &g, const TI *G, double2 *X, int P, int nch,
+static mti_status fb_rows(mti_ctx *ctx, const FftGeom &g, const ChanDesc *G, double2 *X, int P, int nch,
                           const double2 *tw, int64_t cs, int64_t ps, hipStream_t st)
 {
     const dim3 gr((unsigned)(g.ny / (FB_E / g.nx)), (unsigned)P, (unsigned)nch);
@@ -639,14 +649,13 @@ static mti_status fb_cols1(mti_ctx *ctx, const FftGeom &g, double2 *X, int plane
 
 template <int P>
 static mti_status fb_cols2(mti_ctx *ctx, const FftGeom &g, const double2 *X, int nch, const double2 *tw,
-                           const double *norm, int poltype, const float *cx, const float *cy, float *image,
-                           hipStream_t st)
+                           const ChanDesc *cd, int poltype, hipStream_t st)
 {
     const int cols = FB_E / (g.N2 * P);
     const dim3 gr((unsigned)(g.nx / cols), (unsigned)g.N1, (unsigned)nch);
     switch (g.N2) {
 #define MTI_C2(NN) \
-    case NN: hipLaunchKernelGGL((k_cols2_fb<NN, P>), gr, dim3(FB_THREADS), 0, st, X, g.nx, g.ny, g.N1, tw, g.NT, norm, poltype, cx, cy, image); break;
+    case NN: hipLaunchKernelGGL((k_cols2_fb<NN, P>), gr, dim3(FB_THREADS), 0, st, X, g.nx, g.ny, g.N1, tw, g.NT, cd, poltype); break;
         MTI_C2(16) MTI_C2(32) MTI_C2(64)
 #undef MTI_C2
     default: return set_error(ctx, MTI_ERR_UNSUPPORTED, "column split");
@@ -698,6 +707,7 @@ static mti_status cols2(mti_ctx *ctx, const FftGeom &g, const C *X, int nch, con
     return MTI_OK;
 }
 
+// Old-path finalize (small images): contiguous channels of one job.
 template <int P, typename TI>
 static mti_status finalize_batch(mti_ctx *ctx, const FftGeom &g, const TI *G, double2 *X, int nb,
                                  const double2 *tw, const double *nr, int poltype, const float *cx,
@@ -706,20 +716,6 @@ static mti_status finalize_batch(mti_ctx *ctx, const FftGeom &g, const TI *G, do
     mti_status r;
     const int64_t cs = layout == MTI_GRID_VUP ? P : 1;
     const int64_t ps = layout == MTI_GRID_VUP ? 1 : (int64_t)g.nx * g.ny;
-    if (fb_ok(g, P)) {
-        {
-            StageTimer tm(ctx, MTI_STAGE_FFT_ROWS, st);
-            r = fb_rows<TI>(ctx, g, G, X, P, nb, tw, cs, ps, st);
-            if (r != MTI_OK) return r;
-        }
-        {
-            StageTimer tm(ctx, MTI_STAGE_FFT_COLS, st);
-            r = fb_cols1(ctx, g, X, nb * P, tw, st);
-            if (r != MTI_OK) return r;
-        }
-        StageTimer tm(ctx, MTI_STAGE_FINALIZE, st);
-        return fb_cols2<P>(ctx, g, X, nb, tw, nr, poltype, cx, cy, img, st);
-    }
     {
         StageTimer tm(ctx, MTI_STAGE_FFT_ROWS, st);
         r = rows_pass<0, double2, TI>(ctx, g, G, X, P, nb, tw, st, cs, ps);
@@ -732,6 +728,57 @@ static mti_status finalize_batch(mti_ctx *ctx, const FftGeom &g, const TI *G, do
     }
     StageTimer tm(ctx, MTI_STAGE_FINALIZE, st);
     return cols2<0, P, double2>(ctx, g, X, nb, tw, nr, poltype, cx, cy, img, nullptr, st);
+}
+
+// Block-transform finalize of nch channels described by cd (device).
+template <int P, typename TI>
+static mti_status finalize_blocks(mti_ctx *ctx, const FftGeom &g, const ChanDesc *cd, double2 *X, int nch,
+                                  const double2 *tw, int poltype, int layout, hipStream_t st)
+{
+    mti_status r;
+    const int64_t cs = layout == MTI_GRID_VUP ? P : 1;
+    const int64_t ps = layout == MTI_GRID_VUP ? 1 : (int64_t)g.nx * g.ny;
+    {
+        StageTimer tm(ctx, MTI_STAGE_FFT_ROWS, st);
+        r = fb_rows<TI>(ctx, g, cd, X, P, nch, tw, cs, ps, st);
+        if (r != MTI_OK) return r;
+    }
+    {
+        StageTimer tm(ctx, MTI_STAGE_FFT_COLS, st);
+        r = fb_cols1(ctx, g, X, nch * P, tw, st);
+        if (r != MTI_OK) return r;
+    }
+    StageTimer tm(ctx, MTI_STAGE_FINALIZE, st);
+    return fb_cols2<P>(ctx, g, X, nch, tw, cd, poltype, st);
+}
+
+template <typename TI>
+static mti_status finalize_blocks_p(mti_ctx *ctx, int P, const FftGeom &g, const ChanDesc *cd, double2 *X, int nch,
+                                    const double2 *tw, int poltype, int layout, hipStream_t st)
+{
+    if (P == 1) return finalize_blocks<1, TI>(ctx, g, cd, X, nch, tw, poltype, layout, st);
+    if (P == 2) return finalize_blocks<2, TI>(ctx, g, cd, X, nch, tw, poltype, layout, st);
+    return finalize_blocks<4, TI>(ctx, g, cd, X, nch, tw, poltype, layout, st);
+}
+
+// Channels per transform batch (FP64 scratch of at most 16 GB).
+static int transform_batch(const FftGeom &g, int P, int nch)
+{
+    const size_t plane_bytes = (size_t)g.nx * g.ny * sizeof(double2);
+    return (int)std::max<size_t>(1, std::min<size_t>((size_t)nch, (size_t(16) << 30) / (plane_bytes * P)));
+}
+
+// Uploads the channel descriptors of a call (once; the copy completes before
+// any batch is enqueued, so later batches never overwrite what earlier ones read).
+static mti_status upload_desc(mti_ctx *ctx, const std::vector<ChanDesc> &h, ChanDesc **d, hipStream_t st)
+{
+    void *w = nullptr;
+    mti_status r = scratch_get(ctx, SLOT_CHANDESC, std::max<size_t>(1, h.size()) * sizeof(ChanDesc), &w);
+    if (r != MTI_OK) return r;
+    MTI_HIP(ctx, hipMemcpyAsync(w, h.data(), h.size() * sizeof(ChanDesc), hipMemcpyHostToDevice, st));
+    MTI_HIP(ctx, hipStreamSynchronize(st));
+    *d = (ChanDesc *)w;
+    return MTI_OK;
 }
 
 }  // namespace mti
@@ -766,6 +813,15 @@ extern "C" mti_status mti_fft2d_inverse(mti_ctx *ctx, float *data, int32_t nx, i
     return MTI_OK;
 }
 
+static mti_status check_finalize_size(mti_ctx *ctx, const mti_image_params *p)
+{
+    // FP64 transform: the row pass holds rows of nx complex128 in LDS.
+    if (!is_pow2(p->nx) || !is_pow2(p->ny) || p->nx > 4096 || p->ny > 8192)
+        return set_error(ctx, MTI_ERR_UNSUPPORTED,
+                         "image.nx must be a power of two in [2,4096] and image.ny in [2,8192]");
+    return MTI_OK;
+}
+
 extern "C" mti_status mti_finalize(mti_ctx *ctx, const mti_image_params *p, int32_t npol, int32_t poltype,
                                    int32_t nchan, int32_t grid_layout, int32_t grid_precision,
                                    const void *grids, const double *norm, float *image, void *stream)
@@ -778,10 +834,8 @@ extern "C" mti_status mti_finalize(mti_ctx *ctx, const mti_image_params *p, int3
     MTI_CHECK(ctx, nchan >= 1 && grids && norm && image, "null pointer or nchan < 1");
     MTI_CHECK(ctx, grid_layout == MTI_GRID_VUP || grid_layout == MTI_GRID_PVU, "bad grid_layout");
     MTI_CHECK(ctx, grid_precision == MTI_GRID_C64 || grid_precision == MTI_GRID_C128, "bad grid_precision");
-    // FP64 transform: the row pass holds 2 rows of nx complex128 in LDS.
-    if (!is_pow2(p->nx) || !is_pow2(p->ny) || p->nx > 4096 || p->ny > 8192)
-        return set_error(ctx, MTI_ERR_UNSUPPORTED,
-                         "image.nx must be a power of two in [2,4096] and image.ny in [2,8192]");
+    r = check_finalize_size(ctx, p);
+    if (r != MTI_OK) return r;
     MTI_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = to_stream(stream);
     FftGeom g = geom(p->nx, p->ny);
@@ -790,7 +844,7 @@ extern "C" mti_status mti_finalize(mti_ctx *ctx, const mti_image_params *p, int3
     if (r != MTI_OK) return r;
     // scratch: FP64 transform buffer for a batch of channels + correction vectors
     const size_t plane_bytes = (size_t)p->nx * p->ny * sizeof(double2);
-    int batch = (int)std::max<size_t>(1, std::min<size_t>((size_t)nchan, (size_t(16) << 30) / (plane_bytes * npol)));
+    const int batch = transform_batch(g, npol, nchan);
     void *w = nullptr;
     r = scratch_get(ctx, SLOT_FFT, plane_bytes * npol * batch + (size_t)(p->nx + p->ny + 128) * 4, &w);
     if (r != MTI_OK) return r;
@@ -800,10 +854,29 @@ extern "C" mti_status mti_finalize(mti_ctx *ctx, const mti_image_params *p, int3
     hipLaunchKernelGGL(k_corr, dim3((unsigned)ceil_div(p->nx, 256)), dim3(256), 0, st, corr_x, p->nx, p->sampling);
     hipLaunchKernelGGL(k_corr, dim3((unsigned)ceil_div(p->ny, 256)), dim3(256), 0, st, corr_y, p->ny, p->sampling);
     MTI_HIP(ctx, hipGetLastError());
+    const size_t elem = grid_precision == MTI_GRID_C64 ? sizeof(float2) : sizeof(double2);
+    const size_t chan_grid = (size_t)p->nx * p->ny * npol;
+    if (fb_ok(g, npol)) {
+        std::vector<ChanDesc> h(nchan);
+        for (int c = 0; c < nchan; c++)
+            h[c] = ChanDesc{(const char *)grids + elem * chan_grid * c, norm + (size_t)c * npol,
+                            image + chan_grid * c, corr_x, corr_y};
+        ChanDesc *cd = nullptr;
+        r = upload_desc(ctx, h, &cd, st);
+        if (r != MTI_OK) return r;
+        for (int c0 = 0; c0 < nchan; c0 += batch) {
+            const int nb = std::min(batch, nchan - c0);
+            r = grid_precision == MTI_GRID_C64
+                    ? finalize_blocks_p<float2>(ctx, npol, g, cd + c0, X, nb, tw, poltype, grid_layout, st)
+                    : finalize_blocks_p<double2>(ctx, npol, g, cd + c0, X, nb, tw, poltype, grid_layout, st);
+            if (r != MTI_OK) return r;
+        }
+        return MTI_OK;
+    }
     for (int c0 = 0; c0 < nchan; c0 += batch) {
         const int nb = std::min(batch, nchan - c0);
-        const size_t off = (size_t)c0 * p->nx * p->ny * npol;
-        float *img = image + (size_t)c0 * npol * p->nx * p->ny;
+        const size_t off = (size_t)c0 * chan_grid;
+        float *img = image + (size_t)c0 * chan_grid;
         const double *nr = norm + (size_t)c0 * npol;
         if (grid_precision == MTI_GRID_C64) {
             const float2 *G = (const float2 *)grids + off;
@@ -823,6 +896,99 @@ extern "C" mti_status mti_finalize(mti_ctx *ctx, const mti_image_params *p, int3
                 r = finalize_batch<4>(ctx, g, G, X, nb, tw, nr, poltype, corr_x, corr_y, img, grid_layout, st);
         }
         if (r != MTI_OK) return r;
+    }
+    return MTI_OK;
+}
+
+extern "C" mti_status mti_finalize_jobs(mti_ctx *ctx, int32_t njobs, const mti_job *jobs, int32_t poltype,
+                                        int32_t grid_layout, void *stream)
+{
+    if (!ctx) return MTI_ERR_INVALID;
+    MTI_CHECK(ctx, njobs >= 0 && (njobs == 0 || jobs), "bad job list");
+    MTI_CHECK(ctx, poltype == MTI_POL_LINEAR || poltype == MTI_POL_CIRCULAR, "bad PolarizationType");
+    MTI_CHECK(ctx, grid_layout == MTI_GRID_VUP || grid_layout == MTI_GRID_PVU, "bad grid_layout");
+    if (njobs == 0) return MTI_OK;
+    const int P = jobs[0].vis.npol;
+    for (int i = 0; i < njobs; i++) {
+        mti_status r = validate_params(ctx, &jobs[i].params);
+        if (r != MTI_OK) return r;
+        r = check_finalize_size(ctx, &jobs[i].params);
+        if (r != MTI_OK) return r;
+        MTI_CHECK(ctx, jobs[i].vis.npol == P, "jobs of one call must share npol");
+        MTI_CHECK(ctx, jobs[i].grids && jobs[i].norm && jobs[i].image && jobs[i].nchan >= 1, "null job outputs");
+    }
+    MTI_CHECK(ctx, P == 1 || P == 2 || P == 4, "npol must be 1, 2 or 4");
+    MTI_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = to_stream(stream);
+    // correction vectors of every job (they depend on the size and sampling)
+    std::vector<size_t> coff(njobs + 1, 0);
+    for (int i = 0; i < njobs; i++) coff[i + 1] = coff[i] + (size_t)jobs[i].params.nx + jobs[i].params.ny + 64;
+    void *wc = nullptr;
+    mti_status r = scratch_get(ctx, SLOT_CORR, coff[njobs] * sizeof(float), &wc);
+    if (r != MTI_OK) return r;
+    float *corr = (float *)wc;
+    for (int i = 0; i < njobs; i++) {
+        const mti_image_params &p = jobs[i].params;
+        hipLaunchKernelGGL(k_corr, dim3((unsigned)ceil_div(p.nx, 256)), dim3(256), 0, st, corr + coff[i], p.nx,
+                           p.sampling);
+        hipLaunchKernelGGL(k_corr, dim3((unsigned)ceil_div(p.ny, 256)), dim3(256), 0, st, corr + coff[i] + p.nx,
+                           p.ny, p.sampling);
+    }
+    MTI_HIP(ctx, hipGetLastError());
+    // image sizes with the block transform: one descriptor list per size
+    std::vector<std::pair<int, int>> sizes;
+    for (int i = 0; i < njobs; i++) {
+        const std::pair<int, int> sz{jobs[i].params.nx, jobs[i].params.ny};
+        if (std::find(sizes.begin(), sizes.end(), sz) == sizes.end()) sizes.push_back(sz);
+    }
+    std::vector<ChanDesc> h;
+    std::vector<size_t> goff;
+    std::vector<int> gch;
+    for (auto sz : sizes) {
+        const FftGeom g = geom(sz.first, sz.second);
+        goff.push_back(h.size());
+        int n = 0;
+        if (fb_ok(g, P)) {
+            const size_t chan_grid = (size_t)sz.first * sz.second * P;
+            for (int i = 0; i < njobs; i++) {
+                const mti_job &j = jobs[i];
+                if (j.params.nx != sz.first || j.params.ny != sz.second) continue;
+                for (int c = 0; c < j.nchan; c++, n++)
+                    h.push_back(ChanDesc{(const double2 *)j.grids + chan_grid * c, j.norm + (size_t)c * P,
+                                         j.image + chan_grid * c, corr + coff[i], corr + coff[i] + sz.first});
+            }
+        }
+        gch.push_back(n);
+    }
+    ChanDesc *cd = nullptr;
+    r = upload_desc(ctx, h, &cd, st);
+    if (r != MTI_OK) return r;
+    for (size_t s = 0; s < sizes.size(); s++) {
+        const FftGeom g = geom(sizes[s].first, sizes[s].second);
+        if (!fb_ok(g, P)) {
+            // small images: one job at a time through the per-job finalize
+            for (int i = 0; i < njobs; i++) {
+                const mti_job &j = jobs[i];
+                if (j.params.nx != sizes[s].first || j.params.ny != sizes[s].second) continue;
+                r = mti_finalize(ctx, &j.params, P, poltype, j.nchan, grid_layout, MTI_GRID_C128, j.grids, j.norm,
+                                 j.image, stream);
+                if (r != MTI_OK) return r;
+            }
+            continue;
+        }
+        const double2 *tw = nullptr;
+        r = twiddle_table(ctx, g.NT, &tw, st);
+        if (r != MTI_OK) return r;
+        const int batch = transform_batch(g, P, gch[s]);
+        void *w = nullptr;
+        r = scratch_get(ctx, SLOT_FFT, (size_t)g.nx * g.ny * sizeof(double2) * P * batch, &w);
+        if (r != MTI_OK) return r;
+        for (int c0 = 0; c0 < gch[s]; c0 += batch) {
+            const int nb = std::min(batch, gch[s] - c0);
+            r = finalize_blocks_p<double2>(ctx, P, g, cd + goff[s] + c0, (double2 *)w, nb, tw, poltype, grid_layout,
+                                           st);
+            if (r != MTI_OK) return r;
+        }
     }
     return MTI_OK;
 }

@@ -1,4 +1,5 @@
-// mti_grid.hip -- convolutional gridding of many channels (rows a5-a7, a15, a21).
+// mti_grid.hip -- convolutional gridding of many channels and many jobs
+// (rows a5-a7, a15, a21).
 //
 // Reference: ConvolutionGridder (op/ConvolutionGridder.cu:183-550) scatters
 // grid[corner_y+sy][corner_x+sx][p] += K[conv_idx + sy*S + sx] * V_p, with
@@ -10,22 +11,27 @@
 // CreateBlockDataIndex, NotSameSupportManipulate) and flushes with float
 // atomics to the grid in HBM.
 //
-// MI355X design (tile ownership):
-//   1. binning pass (mti_bin.hip): one record per (row, channel), FP64 bins;
-//   2. stable compaction of gridded records (exclusive scan + scatter);
+// MI355X design (tile ownership), for one or many imaging jobs per pass:
+//   1. binning pass (mti_bin.hip): one record per (row, channel), FP64 bins,
+//      written in place (job, channel, baseline order); a record that is not
+//      gridded has support 0, so it yields no tile pairs and no weight;
+//   2. (no compaction: ungridded records are a few percent, and every later
+//      pass reaches records through tile pairs or skips support 0);
 //   3. counting sort of (record, tile) pairs: every record is listed under each
 //      32x32-cell tile its support box touches (wave-aggregated counters, one
 //      scan, scatter); a tile's list is cut into work units of <= TG_CHUNK
-//      records;
-//   4. uniform/Briggs: the weight density is built the same way, per tile in
-//      LDS, from the records whose nearest cell lies in the tile;
-//   5. gridder: one 512-thread workgroup per unit holds the tile of every
-//      correlation in LDS as FP64 (ds_add_f64), stages records in batches
-//      (weighted visibility, clipped box, kernel offset) and lets each wave
-//      take one record at a time with its lanes over the clipped taps; the tile
-//      is then added to the FP64 grid once -- plain read-modify-write when the
-//      unit is the tile's only one, FP64 atomics otherwise.
-// HBM sees each touched grid cell once per unit instead of once per tap.
+//      records.  Tile keys are numbered job-major, so the units of all jobs of
+//      a pass form one list and one persistent launch grids them all;
+//   4. uniform/Briggs: one block per tile key gathers the weights of the
+//      records whose nearest cell lies in the tile into LDS, adds the tile's
+//      sum W and sum W^2 to its channel and hands each such record its W (no
+//      density grid is stored);
+//   5. per-record effective weights and the normaliser;
+//   6. gridder: one 1024-thread workgroup per unit; wave w owns the 8x8 region
+//      w of the tile and each lane one cell, accumulated in FP64 registers
+//      (see k_grid_tiles); the tile is added to the FP64 grid once -- plain
+//      read-modify-write when the unit is the tile's only one, FP64 atomics
+//      otherwise.
 #include "mti_grid.hpp"
 
 #include <hip/hip_bf16.h>
@@ -33,6 +39,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 namespace mti {
 
@@ -44,42 +51,80 @@ constexpr int TG_BATCH = 256;        // records staged per round
 static_assert(TG_T == 32 && TG_THREADS == 64 * TG_REGIONS && TG_BATCH <= 256, "region layout");
 constexpr int TG_CHUNK = 2048;       // records per work unit
 
-struct TileArgs {
-    VisView v;
-    int chan0;
-    const int4 *cdesc;          // compacted record descriptors
-    const int32_t *cchan;       // local channel of each compacted record
-    const uint32_t *n_rec;      // device scalar: compacted records
+// Arrays of a gridding pass shared by the kernels after the binning pass.
+struct GridArgs {
+    JobSet js;
+    const int4 *cdesc;          // record descriptors (support 0: not gridded)
+    const int32_t *cjc;         // job << 16 | local channel of each record
+    const uint32_t *n_rec;      // device scalar: records
     float2 *wvis;               // [M][P] visibilities, weighted in place by k_record_prep
     float *wt;                  // [M][P] natural weights, 0 where flagged
+    float *rdens;               // [M][P] uniform/Briggs: weight density at the record's nearest cell
     const uint32_t *pairs;      // record index per (tile, record) pair, grouped by tile
     const int4 *units;          // {key, first pair, pairs, 1 if the tile's only unit}
     const uint32_t *n_units;    // device scalar
-    const void *table;
-    const float *sums;
-    const int2 *convinfo;
-    int T2;
-    int nx, ny, ntx, nty;
-    int64_t cs, ps;             // complex strides of cell and correlation
-    int64_t chan_stride;        // complex elements per channel grid
-    double *grid;               // FP64 accumulation grid
-    double *norm;               // [nb][P]
-    int psf;
-    int weighting;
-    float *density;             // [nb][ny][nx][P] (written by the density pass)
-    const double *f2;           // [nb][P]
 };
 
-template <int PREC>
-__device__ __forceinline__ float2 ksample(const void *table, int idx)
+// Global-memory views of pointers read from the device-resident job table.
+// Such pointers are generic to the compiler, and loads through a generic
+// pointer become FLAT instructions, which also count against the LDS wait
+// counter: every LDS wait would then drain the prefetched kernel samples.
+#define MTI_GLOBAL __attribute__((address_space(1)))
+
+template <typename T>
+__device__ __forceinline__ const MTI_GLOBAL T *gconst(const void *p)
 {
-    if (PREC == MTI_TABLE_F32) return ((const float2 *)table)[idx];
-    if (PREC == MTI_TABLE_F16) return __half22float2(((const __half2 *)table)[idx]);
-    __hip_bfloat162 h = ((const __hip_bfloat162 *)table)[idx];
-    return make_float2(__bfloat162float(h.x), __bfloat162float(h.y));
+    return (const MTI_GLOBAL T *)p;
 }
 
 __device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// A pointer known to be the same in every lane (kept in scalar registers).
+__device__ __forceinline__ const void *uniform_ptr(const void *p)
+{
+    const uint64_t v = (uint64_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (const void *)(((uint64_t)hi << 32) | lo);
+}
+
+// Kernel sample idx of a table (complex f32, f16 or bf16 pairs).  The byte
+// offset is 32-bit (tables are below 4 GiB, checked when they are built), so
+// the load takes the table base from scalar registers and one offset VGPR.
+template <int PREC>
+__device__ __forceinline__ float2 ksample(const void *table, uint32_t idx)
+{
+    constexpr uint32_t esz = PREC == MTI_TABLE_F32 ? 8u : 4u;
+    const MTI_GLOBAL char *p = gconst<char>(table) + idx * esz;
+    if (PREC == MTI_TABLE_F32) {
+        const uint64_t w = *(const MTI_GLOBAL uint64_t *)p;
+        return make_float2(__uint_as_float((uint32_t)w), __uint_as_float((uint32_t)(w >> 32)));
+    }
+    const uint32_t w = *(const MTI_GLOBAL uint32_t *)p;
+    if (PREC == MTI_TABLE_F16)
+        return make_float2(__half2float(__ushort_as_half((unsigned short)(w & 0xffffu))),
+                           __half2float(__ushort_as_half((unsigned short)(w >> 16))));
+    // bf16 is the high half of an f32
+    return make_float2(__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u));
+}
+
+// Tile coordinates of a global (job, channel, tile) key.
+struct TileAt {
+    int job, c_local, tx, ty;
+};
+
+__device__ __forceinline__ TileAt tile_at(const JobSet &js, int key)
+{
+    TileAt t;
+    t.job = job_of(js.key0, js.njobs, key);
+    const JobDev &jd = js.jobs[t.job];
+    const int k = key - (int)js.key0[t.job];
+    t.tx = k % jd.ntx;
+    const int r = k / jd.ntx;
+    t.ty = r % jd.nty;
+    t.c_local = r / jd.nty;
+    return t;
+}
 
 // ---- tile lists ---------------------------------------------------------------
 
@@ -120,26 +165,22 @@ __device__ __forceinline__ WaveRuns wave_runs(int key)
     return r;
 }
 
-// Key of tile (tx, ty) of local channel c.
-__device__ __forceinline__ int tile_key(int c, int ty, int tx, int nty, int ntx)
-{
-    return (c * nty + ty) * ntx + tx;
-}
-
 // Pass 0: count the pairs of every tile.  Pass 1: scatter record indices.
 template <int PASS>
-__global__ __launch_bounds__(256) void k_tile_pairs(const int4 *cdesc, const int32_t *cchan, const uint32_t *n_rec,
-                                                    int64_t cap, int nty, int ntx, uint32_t *cnt,
-                                                    const uint32_t *start, uint32_t *pairs)
+__global__ __launch_bounds__(256) void k_tile_pairs(GridArgs a, int64_t cap, uint32_t *cnt, const uint32_t *start,
+                                                    uint32_t *pairs)
 {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if ((int64_t)blockIdx.x * blockDim.x >= (int64_t)*n_rec) return;  // whole block past the end
-    const bool in = j < cap && j < (int64_t)*n_rec;
+    if ((int64_t)blockIdx.x * blockDim.x >= (int64_t)*a.n_rec) return;  // whole block past the end
+    const bool in = j < cap && j < (int64_t)*a.n_rec;
     TileRange t{0, 0, 0, 0};
-    int c = 0;
+    int base = 0, ntx = 1;
     if (in) {
-        t = tile_range(cdesc[j]);
-        c = cchan[j];
+        t = tile_range(a.cdesc[j]);
+        const int jc = a.cjc[j], job = jc >> 16, c = jc & 0xffff;
+        const JobDev &jd = a.js.jobs[job];
+        ntx = jd.ntx;
+        base = (int)a.js.key0[job] + c * jd.nty * jd.ntx;
     }
     const int nq = t.ntx * t.nty;
     int qmax = nq;
@@ -148,16 +189,16 @@ __global__ __launch_bounds__(256) void k_tile_pairs(const int4 *cdesc, const int
         int key = -1;
         if (q < nq) {
             const int dy = q / t.ntx, dx = q - dy * t.ntx;
-            key = tile_key(c, t.ty0 + dy, t.tx0 + dx, nty, ntx);
+            key = base + (t.ty0 + dy) * ntx + t.tx0 + dx;
         }
         const WaveRuns r = wave_runs(key);
         if (PASS == 0) {
             if (r.head && key >= 0) atomicAdd(&cnt[key], (uint32_t)r.len);
         } else {
-            uint32_t base = 0;
-            if (r.head && key >= 0) base = atomicAdd(&cnt[key], (uint32_t)r.len);
-            base = __shfl(base, r.hl, 64);
-            if (key >= 0) pairs[start[key] + base + (uint32_t)((threadIdx.x & 63) - r.hl)] = (uint32_t)j;
+            uint32_t b = 0;
+            if (r.head && key >= 0) b = atomicAdd(&cnt[key], (uint32_t)r.len);
+            b = __shfl(b, r.hl, 64);
+            if (key >= 0) pairs[start[key] + b + (uint32_t)((threadIdx.x & 63) - r.hl)] = (uint32_t)j;
         }
     }
 }
@@ -190,24 +231,61 @@ __global__ void k_unit_list(const uint32_t *start, const uint32_t *total, const 
 
 // ---- weight density (uniform / Briggs) ----------------------------------------
 
-// Per unit: the weights of the records whose nearest cell (box centre) lies in
-// the tile, summed per correlation in LDS and added to the density grid
-// [channel][v][u][P].
+// The density W of a cell is the sum of the weights of the records whose
+// nearest cell (box centre) it is, per channel and correlation.  A block
+// takes whole tiles (all pairs of a tile key), so the tile's densities are
+// complete in LDS: it adds sum W and sum W^2 of the tile's cells to the
+// channel's totals and gives every record whose nearest cell lies in the
+// tile its W (each record has exactly one such tile).  No density grid is
+// stored.  acc: [global channel][P][2] FP64 sums; jc0: channel prefix per job.
 template <int P>
-__global__ __launch_bounds__(256) void k_density_tiles(TileArgs a)
+__global__ __launch_bounds__(256) void k_density_keys(GridArgs a, const uint32_t *start, const uint32_t *np_total,
+                                                     int64_t nkeys, const int64_t *jc0, double *acc)
 {
     __shared__ float s_d[TG_TT * P];
-    const int tid = threadIdx.x;
-    const uint32_t NU = *a.n_units;
-    for (uint32_t u = blockIdx.x; u < NU; u += gridDim.x) {
-        const int4 un = a.units[u];
-        const int key = un.x;
-        const int tx = key % a.ntx, t2 = key / a.ntx, ty = t2 % a.nty, c_local = t2 / a.nty;
-        const int x0 = tx * TG_T, y0 = ty * TG_T;
+    __shared__ double s_r[4][2];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t NP = *np_total;
+    // per-thread partial sums of the channel being visited, flushed on change;
+    // thread tid only ever sees correlation p = tid % P (256 is a multiple of P)
+    const int pc = tid % P;
+    double s1 = 0.0, s2 = 0.0;
+    int ch_cur = -1;
+    auto flush = [&]() {
+        // block reduction of (s1, s2) per correlation, then 2P atomics
+        for (int q = 0; q < P; q++) {
+            double v1 = pc == q ? s1 : 0.0, v2 = pc == q ? s2 : 0.0;
+            for (int o = 32; o >= 1; o >>= 1) {
+                v1 += __shfl_xor(v1, o, 64);
+                v2 += __shfl_xor(v2, o, 64);
+            }
+            if (lane == 0) {
+                s_r[wave][0] = v1;
+                s_r[wave][1] = v2;
+            }
+            __syncthreads();
+            if (tid < 2) {
+                const double t = s_r[0][tid] + s_r[1][tid] + s_r[2][tid] + s_r[3][tid];
+                if (t != 0.0) unsafeAtomicAdd(&acc[2 * ((int64_t)ch_cur * P + q) + tid], t);
+            }
+            __syncthreads();
+        }
+        s1 = s2 = 0.0;
+    };
+    for (int64_t k = blockIdx.x; k < nkeys; k += gridDim.x) {
+        const uint32_t p0 = start[k], p1 = (k + 1 < nkeys) ? start[k + 1] : NP;
+        if (p0 == p1) continue;  // empty tile (uniform in the block)
+        const TileAt ta = tile_at(a.js, (int)k);
+        const int ch = (int)jc0[ta.job] + ta.c_local;
+        if (ch != ch_cur) {
+            if (ch_cur >= 0) flush();
+            ch_cur = ch;
+        }
+        const int x0 = ta.tx * TG_T, y0 = ta.ty * TG_T;
         for (int i = tid; i < P * TG_TT; i += 256) s_d[i] = 0.0f;
         __syncthreads();
-        for (int i = tid; i < un.z; i += 256) {
-            const uint32_t j = a.pairs[un.y + i];
+        for (uint32_t i = p0 + tid; i < p1; i += 256) {
+            const uint32_t j = a.pairs[i];
             const int4 d = a.cdesc[j];
             const int S = d.w & 0x7fff;
             const int mx = (d.y & 0xffff) + S / 2 - x0, my = (int)((unsigned)d.y >> 16) + S / 2 - y0;
@@ -219,84 +297,86 @@ __global__ __launch_bounds__(256) void k_density_tiles(TileArgs a)
             }
         }
         __syncthreads();
-        float *dch = a.density + (int64_t)c_local * a.nx * a.ny * P;
         for (int i = tid; i < TG_TT * P; i += 256) {
-            const int cell = i / P, p = i - cell * P;
-            const int gx = x0 + (cell % TG_T), gy = y0 + cell / TG_T;
-            const float v = s_d[i];
-            if (v == 0.0f || gx >= a.nx || gy >= a.ny) continue;
-            float *o = dch + ((int64_t)gy * a.nx + gx) * P + p;
-            if (un.w) *o += v;
-            else unsafeAtomicAdd(o, v);
+            const double v = s_d[i];
+            s1 += v;
+            s2 += v * v;
         }
-        __syncthreads();
+        for (uint32_t i = p0 + tid; i < p1; i += 256) {
+            const uint32_t j = a.pairs[i];
+            const int4 d = a.cdesc[j];
+            const int S = d.w & 0x7fff;
+            const int mx = (d.y & 0xffff) + S / 2 - x0, my = (int)((unsigned)d.y >> 16) + S / 2 - y0;
+            if (mx < 0 || mx >= TG_T || my < 0 || my >= TG_T) continue;
+#pragma unroll
+            for (int p = 0; p < P; p++) a.rdens[(int64_t)j * P + p] = s_d[(my * TG_T + mx) * P + p];
+        }
+        __syncthreads();  // s_d is cleared for the next tile
     }
+    if (ch_cur >= 0) flush();
 }
 
 // ---- per-record weights -------------------------------------------------------
 
-// Per compacted record: effective weights (natural, uniform 1/W or Briggs
+// Per gridded record: effective weights (natural, uniform 1/W or Briggs
 // 1/(1 + f^2 W) at the record's nearest cell), visibilities weighted in place
-// and the normaliser sums[k][o] * w_p, reduced per block and channel.
+// and the normaliser sums[k][o] * w_p, reduced per block and (job, channel).
 template <int P>
-__global__ __launch_bounds__(256) void k_record_prep(TileArgs a)
+__global__ __launch_bounds__(256) void k_record_prep(GridArgs a)
 {
     const uint32_t M = *a.n_rec;
     const int lane = threadIdx.x & 63;
     double np[P];
 #pragma unroll
     for (int p = 0; p < P; p++) np[p] = 0.0;
-    int c_prev = -1;
-    for (uint32_t j0 = blockIdx.x * 256u; j0 < M; j0 += gridDim.x * 256u) {
-        const uint32_t j = j0 + threadIdx.x;
-        // records are channel-major: partials of the previous channel are
-        // flushed when a block starts on a new one
-        const int c_first = a.cchan[j0];
-        if (c_first != c_prev && c_prev >= 0) {
-#pragma unroll
-            for (int p = 0; p < P; p++) {
-                double v = np[p];
-                for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-                if (lane == 0 && v != 0.0) unsafeAtomicAdd(&a.norm[c_prev * P + p], v);
-                np[p] = 0.0;
-            }
-        }
-        c_prev = c_first;
-        if (j < M) {
-            const int4 d = a.cdesc[j];
-            const int c_local = a.cchan[j];
-            const int cx = d.y & 0xffff, cy = (int)((unsigned)d.y >> 16);
-            const int plane = (int)((unsigned)d.w >> 16);
-            const int S = d.w & 0x7fff;
-            const double csum = S > 0 ? (double)a.sums[plane * a.T2 + (d.z - a.convinfo[plane].y) / (S * S)] : 0.0;
-            float dens[P];
-            if (a.weighting != MTI_WEIGHT_NATURAL) {
-                const float *dp = a.density + (((int64_t)c_local * a.ny + cy + S / 2) * a.nx + cx + S / 2) * P;
-#pragma unroll
-                for (int p = 0; p < P; p++) dens[p] = dp[p];
-            }
-#pragma unroll
-            for (int p = 0; p < P; p++) {
-                float w = a.wt[(int64_t)j * P + p];
-                if (a.weighting == MTI_WEIGHT_UNIFORM)
-                    w = dens[p] > 0.0f ? w / dens[p] : 0.0f;
-                else if (a.weighting == MTI_WEIGHT_BRIGGS)
-                    w = (float)((double)w / (1.0 + a.f2[c_local * P + p] * (double)dens[p]));
-                const float2 vv = a.wvis[(int64_t)j * P + p];
-                a.wvis[(int64_t)j * P + p] = make_float2(vv.x * w, vv.y * w);
-                if (c_local == c_first) np[p] += csum * (double)w;
-                else unsafeAtomicAdd(&a.norm[c_local * P + p], csum * (double)w);
-            }
-        }
-    }
-    if (c_prev >= 0) {
+    int jc_prev = -1;
+    auto flush = [&](int jc) {
+        const JobDev &jd = a.js.jobs[jc >> 16];
 #pragma unroll
         for (int p = 0; p < P; p++) {
             double v = np[p];
             for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-            if (lane == 0 && v != 0.0) unsafeAtomicAdd(&a.norm[c_prev * P + p], v);
+            if (lane == 0 && v != 0.0) unsafeAtomicAdd(&jd.norm[(jc & 0xffff) * P + p], v);
+            np[p] = 0.0;
+        }
+    };
+    for (uint32_t j0 = blockIdx.x * 256u; j0 < M; j0 += gridDim.x * 256u) {
+        const uint32_t j = j0 + threadIdx.x;
+        // records are ordered by (job, channel): partials are flushed when a
+        // block starts on a new one
+        const int jc_first = a.cjc[j0];
+        if (jc_first != jc_prev && jc_prev >= 0) flush(jc_prev);
+        jc_prev = jc_first;
+        const int4 d = j < M ? a.cdesc[j] : make_int4(0, 0, 0, 0);
+        const int S = d.w & 0x7fff;
+        // records that are not gridded (support 0) hold no visibilities or weights
+        if (S > 0) {
+            const int jc = a.cjc[j];
+            const JobDev &jd = a.js.jobs[jc >> 16];
+            const int c_local = jc & 0xffff;
+            const int cx = d.y & 0xffff, cy = (int)((unsigned)d.y >> 16);
+            const int plane = (int)((unsigned)d.w >> 16);
+            const double csum = (double)jd.sums[plane * jd.T2 + (d.z - jd.g.convinfo[plane].y) / (S * S)];
+            float dens[P];
+            if (jd.weighting != MTI_WEIGHT_NATURAL) {
+#pragma unroll
+                for (int p = 0; p < P; p++) dens[p] = a.rdens[(int64_t)j * P + p];
+            }
+#pragma unroll
+            for (int p = 0; p < P; p++) {
+                float w = a.wt[(int64_t)j * P + p];
+                if (jd.weighting == MTI_WEIGHT_UNIFORM)
+                    w = dens[p] > 0.0f ? w / dens[p] : 0.0f;
+                else if (jd.weighting == MTI_WEIGHT_BRIGGS)
+                    w = (float)((double)w / (1.0 + jd.f2[c_local * P + p] * (double)dens[p]));
+                const float2 vv = a.wvis[(int64_t)j * P + p];
+                a.wvis[(int64_t)j * P + p] = make_float2(vv.x * w, vv.y * w);
+                if (jc == jc_first) np[p] += csum * (double)w;
+                else unsafeAtomicAdd(&jd.norm[c_local * P + p], csum * (double)w);
+            }
         }
     }
+    if (jc_prev >= 0) flush(jc_prev);
 }
 
 // ---- gridder ------------------------------------------------------------------
@@ -304,27 +384,36 @@ __global__ __launch_bounds__(256) void k_record_prep(TileArgs a)
 // One in-flight visit of a wave to a record: staged record, lane active,
 // kernel sample.
 struct Visit {
-    int r;
+    int vo;  // byte offset of the record's visibilities in LDS
     bool on;
     float2 k;
 };
 
+// Region-list entry (12 bytes): the record's corner x | y << 16, its kernel
+// offset, and S | (visibility byte offset) << 16.
+struct ListEntry {
+    int xy, koff, s_vo;
+};
+
 // Visit i of the wave's region list: the lane's cell (gx, gy) against the
-// record's support box; kernel index 0 (a valid dummy) when outside.
+// record's support box; kernel index 0 (a valid dummy) when outside.  The
+// entry carries everything a visit needs, so it is one LDS read with no
+// dependent read; dy * S (|dy| < 2^23) uses the full-rate 24-bit multiply.
 template <int PREC>
-__device__ __forceinline__ Visit visit_issue(int i, int n, const uint8_t *list, const int4 *s_rec, int gx, int gy,
-                                             const void *table)
+__device__ __forceinline__ Visit visit_issue(int i, int n, const ListEntry *list, int gx, int gy, const void *table)
 {
     Visit v;
-    v.r = -1;
+    v.vo = 0;
     v.on = false;
-    int idx = 0;
+    uint32_t idx = 0;
     if (i < n) {
-        v.r = list[i];
-        const int4 rec = s_rec[v.r];  // corner x | y << 16, kernel offset, S
-        const int dx = gx - (rec.x & 0xffff), dy = gy - (int)((unsigned)rec.x >> 16);
-        v.on = (unsigned)dx < (unsigned)rec.z && (unsigned)dy < (unsigned)rec.z;
-        idx = v.on ? rec.y + dy * rec.z + dx : 0;
+        const ListEntry e = list[i];
+        const int S = e.s_vo & 0xffff;
+        const int dx = gx - (e.xy & 0xffff), dy = gy - (int)((unsigned)e.xy >> 16);
+        const uint32_t t = (uint32_t)(e.koff + __mul24(dy, S) + dx);
+        v.on = max((unsigned)dx, (unsigned)dy) < (unsigned)S;
+        v.vo = (int)((unsigned)e.s_vo >> 16);
+        idx = v.on ? t : 0u;
     }
     v.k = ksample<PREC>(table, idx);
     return v;
@@ -332,25 +421,28 @@ __device__ __forceinline__ Visit visit_issue(int i, int n, const uint8_t *list, 
 
 // Gridder: one 1024-thread workgroup per unit; wave w owns the 8x8 region w
 // of the 32x32 tile and each lane one cell, accumulated in FP64 registers.
-// Records are staged in LDS per batch together with, for every region, the
-// list of records whose support box reaches it; a wave walks only its list,
-// with kernel samples loaded four visits ahead.  No LDS traffic per tap.
+// Records are staged in LDS per batch: their weighted visibilities, and for
+// every region the list of records whose support box reaches it, each entry
+// a copy of the record's geometry.  A wave walks only its list, with kernel
+// samples loaded four visits ahead.  No LDS traffic per tap.
 template <int P, int PREC>
-__global__ __launch_bounds__(TG_THREADS) void k_grid_tiles(TileArgs a)
+__global__ __launch_bounds__(TG_THREADS) void k_grid_tiles(GridArgs a)
 {
-    __shared__ int4 s_rec[TG_BATCH];                 // corner, kernel offset, S
-    __shared__ double s_vd[TG_BATCH][2 * P];         // weighted visibility
-    __shared__ uint8_t s_list[TG_REGIONS][TG_BATCH]; // records per region
+    __shared__ __align__(16) double s_vd[TG_BATCH][2 * P];  // weighted visibility
+    __shared__ ListEntry s_list[TG_REGIONS][TG_BATCH];  // records per region
     __shared__ int s_cnt[TG_REGIONS];
+    static_assert(TG_BATCH * 2 * P * sizeof(double) < 65536, "visibility offsets fit 16 bits");
     const int tid = threadIdx.x, lane = tid & 63, wave = wave_uniform(tid >> 6);
     const int rx = (wave & 3) * 8 + (lane & 7), ry = (wave >> 2) * 8 + (lane >> 3);  // cell in the tile
     const uint32_t NU = *a.n_units;
 
     for (uint32_t u = blockIdx.x; u < NU; u += gridDim.x) {
         const int4 un = a.units[u];
-        const int key = un.x, first = un.y, npairs = un.z, excl = un.w;
-        const int tx = key % a.ntx, t2 = key / a.ntx, ty = t2 % a.nty, c_local = t2 / a.nty;
-        const int x0 = tx * TG_T, y0 = ty * TG_T;
+        const int first = un.y, npairs = un.z, excl = un.w;
+        const TileAt ta = tile_at(a.js, un.x);
+        const JobDev &jd = a.js.jobs[ta.job];
+        const void *table = uniform_ptr(jd.table);
+        const int x0 = ta.tx * TG_T, y0 = ta.ty * TG_T;
         const int gx = x0 + rx, gy = y0 + ry;
         double acc[2 * P];
 #pragma unroll
@@ -362,7 +454,7 @@ __global__ __launch_bounds__(TG_THREADS) void k_grid_tiles(TileArgs a)
             if (tid < TG_REGIONS) s_cnt[tid] = 0;
             __syncthreads();
             // staging: four threads per record (one correlation each; the
-            // first also stages the geometry and the region lists)
+            // first also appends the record to the lists of its regions)
             {
                 const int rr = tid >> 2, part = tid & 3;
                 if (rr < nbat) {
@@ -376,37 +468,43 @@ __global__ __launch_bounds__(TG_THREADS) void k_grid_tiles(TileArgs a)
                         const int4 d = a.cdesc[j];
                         const int cx = d.y & 0xffff, cy = (int)((unsigned)d.y >> 16);
                         const int S = d.w & 0x7fff;
-                        s_rec[rr] = make_int4(d.y, d.z, S, 0);
+                        ListEntry e;
+                        e.xy = d.y;
+                        e.koff = d.z;
+                        e.s_vo = S | ((rr * 2 * P * (int)sizeof(double)) << 16);
                         const int ix0 = max(cx, x0) - x0, ix1 = min(cx + S, x0 + TG_T) - x0;
                         const int iy0 = max(cy, y0) - y0, iy1 = min(cy + S, y0 + TG_T) - y0;
                         for (int gyr = iy0 >> 3; gyr <= (iy1 - 1) >> 3; gyr++)
                             for (int gxr = ix0 >> 3; gxr <= (ix1 - 1) >> 3; gxr++) {
                                 const int reg = gyr * 4 + gxr;
                                 const int pos = atomicAdd(&s_cnt[reg], 1);
-                                s_list[reg][pos] = (uint8_t)rr;
+                                s_list[reg][pos] = e;
                             }
                     }
                 }
             }
             __syncthreads();
             const int n = wave_uniform(s_cnt[wave]);
-            const uint8_t *list = s_list[wave];
-            Visit va = visit_issue<PREC>(0, n, list, s_rec, gx, gy, a.table);
-            Visit vb = visit_issue<PREC>(1, n, list, s_rec, gx, gy, a.table);
-            Visit vc = visit_issue<PREC>(2, n, list, s_rec, gx, gy, a.table);
-            Visit vd = visit_issue<PREC>(3, n, list, s_rec, gx, gy, a.table);
+            const ListEntry *list = s_list[wave];
+            Visit va = visit_issue<PREC>(0, n, list, gx, gy, table);
+            Visit vb = visit_issue<PREC>(1, n, list, gx, gy, table);
+            Visit vc = visit_issue<PREC>(2, n, list, gx, gy, table);
+            Visit vd = visit_issue<PREC>(3, n, list, gx, gy, table);
             int next = 4;
             auto visit = [&](Visit &v) {
                 if (v.on) {
                     const double kx = v.k.x, ky = v.k.y;
+                    // 16-byte aligned pairs: one ds_read_b128 per correlation
+                    const double2 *vis = (const double2 *)((const char *)&s_vd[0][0] + v.vo);
 #pragma unroll
                     for (int p = 0; p < P; p++) {
-                        const double vr = s_vd[v.r][2 * p], vi = s_vd[v.r][2 * p + 1];
+                        const double2 x = vis[p];
+                        const double vr = x.x, vi = x.y;
                         acc[2 * p] = fma(-vi, ky, fma(vr, kx, acc[2 * p]));
                         acc[2 * p + 1] = fma(vr, ky, fma(vi, kx, acc[2 * p + 1]));
                     }
                 }
-                v = visit_issue<PREC>(next++, n, list, s_rec, gx, gy, a.table);
+                v = visit_issue<PREC>(next++, n, list, gx, gy, table);
             };
             for (int i = 0; i < n; i += 4) {
                 visit(va);
@@ -416,35 +514,31 @@ __global__ __launch_bounds__(TG_THREADS) void k_grid_tiles(TileArgs a)
             }
         }
         // add the cell to the grid
-        if (gx < a.nx && gy < a.ny) {
-            double *g = a.grid + 2 * ((int64_t)c_local * a.chan_stride + ((int64_t)gy * a.nx + gx) * a.cs);
+        if (gx < jd.g.nx && gy < jd.g.ny) {
+            double *g = jd.grid + 2 * ((int64_t)ta.c_local * jd.chan_stride + ((int64_t)gy * jd.g.nx + gx) * jd.cs);
+            const int64_t ps = jd.ps;
             if (excl) {
                 double2 old[P];
 #pragma unroll
                 for (int p = 0; p < P; p++)
-                    old[p] = (acc[2 * p] != 0.0 || acc[2 * p + 1] != 0.0) ? *(const double2 *)(g + 2 * p * a.ps)
+                    old[p] = (acc[2 * p] != 0.0 || acc[2 * p + 1] != 0.0) ? *(const double2 *)(g + 2 * p * ps)
                                                                          : make_double2(0.0, 0.0);
 #pragma unroll
                 for (int p = 0; p < P; p++)
                     if (acc[2 * p] != 0.0 || acc[2 * p + 1] != 0.0)
-                        *(double2 *)(g + 2 * p * a.ps) = make_double2(old[p].x + acc[2 * p], old[p].y + acc[2 * p + 1]);
+                        *(double2 *)(g + 2 * p * ps) = make_double2(old[p].x + acc[2 * p], old[p].y + acc[2 * p + 1]);
             } else {
 #pragma unroll
                 for (int p = 0; p < P; p++) {
-                    if (acc[2 * p] != 0.0) unsafeAtomicAdd(g + 2 * p * a.ps, acc[2 * p]);
-                    if (acc[2 * p + 1] != 0.0) unsafeAtomicAdd(g + 2 * p * a.ps + 1, acc[2 * p + 1]);
+                    if (acc[2 * p] != 0.0) unsafeAtomicAdd(g + 2 * p * ps, acc[2 * p]);
+                    if (acc[2 * p + 1] != 0.0) unsafeAtomicAdd(g + 2 * p * ps + 1, acc[2 * p + 1]);
                 }
             }
         }
     }
 }
 
-__global__ void k_fill_f32(float *p, int64_t n, float v)
-{
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = v;
-}
+// ---- small kernels ------------------------------------------------------------
 
 __global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v)
 {
@@ -460,25 +554,6 @@ __global__ void k_fill_f64(double *p, int64_t n, double v)
         p[i] = v;
 }
 
-// Stable compaction.  pos is the exclusive scan of the valid flags, so
-// record e is kept exactly when the next position differs from its own.
-__global__ void k_compact(const uint32_t *pos, const uint32_t *n_rec, const int4 *desc, const float2 *wv,
-                          const float *wt, int64_t nrec, int64_t nidx, int P, int4 *cdesc, int32_t *cchan,
-                          float2 *cwv, float *cwt)
-{
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nrec) return;
-    const uint32_t j = pos[e];
-    const uint32_t next = (e + 1 < nrec) ? pos[e + 1] : *n_rec;
-    if (next == j) return;
-    cdesc[j] = desc[e];
-    cchan[j] = (int32_t)(e / nidx);
-    for (int p = 0; p < P; p++) {
-        cwv[(int64_t)j * P + p] = wv[e * P + p];
-        cwt[(int64_t)j * P + p] = wt[e * P + p];
-    }
-}
-
 // out = (accumulate ? out : 0) + work, FP64 work grid -> complex64 output.
 __global__ void k_round_grid(const double *work, float *out, int64_t n, int accumulate)
 {
@@ -487,50 +562,18 @@ __global__ void k_round_grid(const double *work, float *out, int64_t n, int accu
         out[i] = accumulate ? (float)((double)out[i] + work[i]) : (float)work[i];
 }
 
-// Per (channel, correlation): sum W and sum W^2 of the density grid
-// [channel][cell][P]; one block row per channel.
-__global__ __launch_bounds__(256) void k_density_stats(const float *dens, int64_t cells, int P,
-                                                      double *acc /* [nb][P][2] */)
+// Briggs f^2 = (5*10^-R)^2 / (sum W^2 / sum w) per (job, channel, correlation).
+__global__ void k_briggs_f2(JobSet js, const int64_t *jc0, int nch, const double *acc, double robust)
 {
-    const int c = blockIdx.y;
-    const float *d = dens + (int64_t)c * cells * P;
-    double s1[4] = {0, 0, 0, 0}, s2[4] = {0, 0, 0, 0};
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cells;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        for (int p = 0; p < P; p++) {
-            const double v = d[i * P + p];
-            s1[p] += v;
-            s2[p] += v * v;
-        }
-    }
-    __shared__ double r[4][4][2];
-    for (int p = 0; p < P; p++) {
-        double a1 = s1[p], a2 = s2[p];
-        for (int o = 32; o >= 1; o >>= 1) {
-            a1 += __shfl_xor(a1, o, 64);
-            a2 += __shfl_xor(a2, o, 64);
-        }
-        if ((threadIdx.x & 63) == 0) {
-            r[threadIdx.x >> 6][p][0] = a1;
-            r[threadIdx.x >> 6][p][1] = a2;
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x < 2 * P) {
-        const int p = threadIdx.x >> 1, k = threadIdx.x & 1;
-        const double t = r[0][p][k] + r[1][p][k] + r[2][p][k] + r[3][p][k];
-        unsafeAtomicAdd(&acc[2 * (c * P + p) + k], t);
-    }
-}
-
-// Briggs f^2 = (5*10^-R)^2 / (sum W^2 / sum w)
-__global__ void k_briggs_f2(const double *acc, int n, double robust, double *f2)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double sw = acc[2 * i], sw2 = acc[2 * i + 1];
-    double k = 5.0 * pow(10.0, -robust);
-    f2[i] = sw2 > 0.0 ? (k * k) / (sw2 / sw) : 0.0;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int P = js.P;
+    if (i >= nch * P) return;
+    const int y = i / P, p = i - y * P;
+    const int job = job_of(jc0, js.njobs, y);
+    const int c = y - (int)jc0[job];
+    const double sw = acc[2 * i], sw2 = acc[2 * i + 1];
+    const double k = 5.0 * pow(10.0, -robust);
+    js.jobs[job].f2[c * P + p] = sw2 > 0.0 ? (k * k) / (sw2 / sw) : 0.0;
 }
 
 template <typename K>
@@ -543,7 +586,7 @@ static int persistent_blocks(mti_ctx *ctx, K kern, int threads)
 }
 
 template <int P, int PREC>
-static mti_status launch_tiles(mti_ctx *ctx, const TileArgs &a, hipStream_t st)
+static mti_status launch_tiles(mti_ctx *ctx, const GridArgs &a, hipStream_t st)
 {
     auto kern = k_grid_tiles<P, PREC>;
     hipLaunchKernelGGL(kern, dim3(persistent_blocks(ctx, kern, TG_THREADS)), dim3(TG_THREADS), 0, st, a);
@@ -552,7 +595,7 @@ static mti_status launch_tiles(mti_ctx *ctx, const TileArgs &a, hipStream_t st)
 }
 
 template <int P>
-static mti_status dispatch_tiles(mti_ctx *ctx, int prec, const TileArgs &a, hipStream_t st)
+static mti_status dispatch_tiles(mti_ctx *ctx, int prec, const GridArgs &a, hipStream_t st)
 {
     if (prec == MTI_TABLE_F32) return launch_tiles<P, MTI_TABLE_F32>(ctx, a, st);
     if (prec == MTI_TABLE_F16) return launch_tiles<P, MTI_TABLE_F16>(ctx, a, st);
@@ -560,7 +603,7 @@ static mti_status dispatch_tiles(mti_ctx *ctx, int prec, const TileArgs &a, hipS
 }
 
 template <int P>
-static mti_status launch_prep(mti_ctx *ctx, const TileArgs &a, hipStream_t st)
+static mti_status launch_prep(mti_ctx *ctx, const GridArgs &a, hipStream_t st)
 {
     auto kern = k_record_prep<P>;
     hipLaunchKernelGGL(kern, dim3(persistent_blocks(ctx, kern, 256)), dim3(256), 0, st, a);
@@ -569,10 +612,12 @@ static mti_status launch_prep(mti_ctx *ctx, const TileArgs &a, hipStream_t st)
 }
 
 template <int P>
-static mti_status launch_density(mti_ctx *ctx, const TileArgs &a, hipStream_t st)
+static mti_status launch_density(mti_ctx *ctx, const GridArgs &a, const uint32_t *start, const uint32_t *np_total,
+                                 int64_t nkeys, const int64_t *jc0, double *acc, hipStream_t st)
 {
-    auto kern = k_density_tiles<P>;
-    hipLaunchKernelGGL(kern, dim3(persistent_blocks(ctx, kern, 256)), dim3(256), 0, st, a);
+    auto kern = k_density_keys<P>;
+    hipLaunchKernelGGL(kern, dim3(persistent_blocks(ctx, kern, 256)), dim3(256), 0, st, a, start, np_total, nkeys,
+                       jc0, acc);
     MTI_HIP(ctx, hipGetLastError());
     return MTI_OK;
 }
@@ -582,93 +627,173 @@ static inline size_t align_up(size_t x) { return (x + 255) / 256 * 256; }
 // Tiles a support box of side S can touch along one axis.
 static inline int tiles_per_axis(int S) { return 1 + (std::max(S, 1) - 1 + TG_T - 1) / TG_T; }
 
-// Grids channels [chan0, chan0+nb) into the FP64 grid G (already zeroed or
-// holding previous chunks); norm is accumulated.
-static mti_status grid_batch(mti_ctx *ctx, const mti_kernels *k, const mti_image_params *p,
-                             const VisView &vw, const mti_grid_opts *o, int chan0, int nb, double *G,
-                             double *norm, unsigned long long *dstats, uint32_t *nu_total, hipStream_t st)
+// One job (or channel range of a job) of a pass, host side.
+struct PassJob {
+    const mti_kernels *k;
+    mti_image_params p;
+    VisView vw;
+    int chan0, nb;
+    double *grid;      // FP64 grid of chan0 in the job's layout
+    int layout;
+    double *norm;      // [nb][P]
+};
+
+// Limits of one pass: records < 2^31, (record, tile) pairs < 2^32, keys < 2^31.
+static int64_t job_records(const PassJob &j) { return j.vw.nidx * j.nb; }
+static int64_t job_pairs(const PassJob &j)
 {
-    const int P = vw.npol;
-    const int64_t gsz = (int64_t)p->nx * p->ny;
-    const int64_t nrec = vw.nidx * nb;
-    const int ntx = (p->nx + TG_T - 1) / TG_T, nty = (p->ny + TG_T - 1) / TG_T;
-    const int64_t nkeys = (int64_t)nb * ntx * nty;
-    const int tpa = tiles_per_axis(k->info.max_support);
-    const int64_t pair_cap = nrec * tpa * tpa;
-    MTI_CHECK(ctx, nkeys < (int64_t(1) << 31), "too many grid tiles per call");
-    MTI_CHECK(ctx, pair_cap < (int64_t(1) << 32), "too many (record, tile) pairs per call");
+    const int t = tiles_per_axis(j.k->info.max_support);
+    return job_records(j) * t * t;
+}
+static int64_t job_keys(const PassJob &j)
+{
+    return (int64_t)j.nb * ((j.p.nx + TG_T - 1) / TG_T) * ((j.p.ny + TG_T - 1) / TG_T);
+}
+
+// Grids a set of jobs (the same npol and table precision) into their FP64
+// grids (zeroed or holding earlier chunks) and accumulates their normalisers.
+// nu_total[0] += work units, [1] += (record, tile) pairs.
+static mti_status grid_pass(mti_ctx *ctx, const std::vector<PassJob> &jobs, const mti_grid_opts *o, int P, int prec,
+                            unsigned long long *dstats, uint32_t *nu_total, hipStream_t st)
+{
+    const int nj = (int)jobs.size();
+    if (nj == 0) return MTI_OK;
+    std::vector<JobDev> jd(nj);
+    std::vector<int64_t> rec0(nj + 1, 0), key0(nj + 1, 0), blk0(nj + 1, 0), jc0(nj + 1, 0);
+    int64_t pair_cap = 0;
+    const bool weighted = o->weighting != MTI_WEIGHT_NATURAL;
+    for (int i = 0; i < nj; i++) {
+        const PassJob &j = jobs[i];
+        MTI_CHECK(ctx, j.k->info.precision == prec, "jobs of one call must share the table precision");
+        JobDev &d = jd[i];
+        d = JobDev{};
+        d.v = j.vw;
+        d.g = make_geom(j.k, &j.p);
+        d.chan0 = j.chan0;
+        d.nb = j.nb;
+        d.ntx = (j.p.nx + TG_T - 1) / TG_T;
+        d.nty = (j.p.ny + TG_T - 1) / TG_T;
+        d.table = j.k->d_table;
+        d.sums = j.k->d_sums;
+        d.T2 = j.k->info.T * j.k->info.T;
+        d.grid = j.grid;
+        const int64_t gsz = (int64_t)j.p.nx * j.p.ny;
+        d.cs = j.layout == MTI_GRID_VUP ? P : 1;
+        d.ps = j.layout == MTI_GRID_VUP ? 1 : gsz;
+        d.chan_stride = gsz * P;
+        d.norm = j.norm;
+        d.psf = o->psf;
+        d.weighting = o->weighting;
+        rec0[i + 1] = rec0[i] + job_records(j);
+        key0[i + 1] = key0[i] + job_keys(j);
+        blk0[i + 1] = blk0[i] + bin_blocks(j.vw.nidx, j.nb);
+        jc0[i + 1] = jc0[i] + j.nb;
+        pair_cap += job_pairs(j);
+        d.rec0 = rec0[i];
+        d.key0 = key0[i];
+    }
+    const int64_t nrec = rec0[nj], nkeys = key0[nj], nch = jc0[nj];
+    if (nrec == 0) return MTI_OK;
+    MTI_CHECK(ctx, nrec < (int64_t(1) << 31) && nkeys < (int64_t(1) << 31) && pair_cap < (int64_t(1) << 32),
+              "gridding pass above the record/tile limits");
     const int64_t unit_cap = nkeys + pair_cap / TG_CHUNK + 1;
     mti_status r;
-    // scratch: valid, cchan (u32 x nrec); desc, cdesc (int4 x nrec);
-    // start, fill, nunits (u32 x nkeys); pairs (u32 x pair_cap); units (int4)
-    const size_t u32n = align_up((size_t)nrec * 4);
-    const size_t u32k = align_up((size_t)nkeys * 4);
-    void *w1 = nullptr, *w2 = nullptr, *w3 = nullptr, *w4 = nullptr;
-    r = scratch_get(ctx, SLOT_KEYS, u32n * 2 + 4096, &w1);
-    if (r != MTI_OK) return r;
-    r = scratch_get(ctx, SLOT_DESC, align_up((size_t)nrec * 16) * 2, &w2);
-    if (r != MTI_OK) return r;
-    r = scratch_get(ctx, SLOT_BUCKETS, u32k * 3, &w3);
-    if (r != MTI_OK) return r;
-    r = scratch_get(ctx, SLOT_ITEMS, align_up((size_t)pair_cap * 4) + align_up((size_t)unit_cap * 16), &w4);
-    if (r != MTI_OK) return r;
-    uint32_t *valid = (uint32_t *)w1;
-    int32_t *cchan = (int32_t *)((char *)w1 + u32n);
-    uint32_t *scal = (uint32_t *)((char *)w1 + 2 * u32n);
+    // scratch
+    const size_t u32n = align_up((size_t)nrec * 4), u32k = align_up((size_t)nkeys * 4);
+    const size_t pre = align_up((size_t)(nj + 1) * 8);
+    const size_t tab = align_up((size_t)nj * sizeof(JobDev)) + 4 * pre + align_up((size_t)nch * P * 3 * sizeof(double));
+    const size_t wv_bytes = align_up((size_t)nrec * P * sizeof(float2));
+    const size_t wt_bytes = align_up((size_t)nrec * P * sizeof(float));
+    void *w1 = nullptr, *w2 = nullptr, *w3 = nullptr, *w4 = nullptr, *w6 = nullptr, *w7 = nullptr;
+    if ((r = scratch_get(ctx, SLOT_KEYS, u32n + 4096, &w1)) != MTI_OK) return r;
+    if ((r = scratch_get(ctx, SLOT_DESC, align_up((size_t)nrec * 16), &w2)) != MTI_OK) return r;
+    if ((r = scratch_get(ctx, SLOT_BUCKETS, u32k * 3, &w3)) != MTI_OK) return r;
+    if ((r = scratch_get(ctx, SLOT_ITEMS, align_up((size_t)pair_cap * 4) + align_up((size_t)unit_cap * 16), &w4)) !=
+        MTI_OK)
+        return r;
+    if ((r = scratch_get(ctx, SLOT_CRECVIS, wv_bytes + wt_bytes, &w6)) != MTI_OK) return r;
+    if ((r = scratch_get(ctx, SLOT_NORM, tab, &w7)) != MTI_OK) return r;
+    int32_t *cjc = (int32_t *)w1;
+    uint32_t *scal = (uint32_t *)((char *)w1 + u32n);
     uint32_t *M_d = scal, *NP_d = scal + 64, *NU_d = scal + 128;
-    int4 *desc = (int4 *)w2;
-    int4 *cdesc = (int4 *)((char *)w2 + align_up((size_t)nrec * 16));
+    int4 *cdesc = (int4 *)w2;
     uint32_t *start = (uint32_t *)w3;
     uint32_t *fill = (uint32_t *)((char *)w3 + u32k);
     uint32_t *nunits = (uint32_t *)((char *)w3 + 2 * u32k);
     uint32_t *pairs = (uint32_t *)w4;
     int4 *units = (int4 *)((char *)w4 + align_up((size_t)pair_cap * 4));
+    float2 *cwv = (float2 *)w6;
+    float *cwt = (float *)((char *)w6 + wv_bytes);
+    char *tp = (char *)w7;
+    JobDev *d_jobs = (JobDev *)tp;
+    tp += align_up((size_t)nj * sizeof(JobDev));
+    int64_t *d_rec0 = (int64_t *)tp;
+    int64_t *d_key0 = (int64_t *)(tp + pre);
+    int64_t *d_blk0 = (int64_t *)(tp + 2 * pre);
+    int64_t *d_jc0 = (int64_t *)(tp + 3 * pre);
+    double *dacc = (double *)(tp + 4 * pre);  // [nch][P][2] density sums, then f2 [nch][P]
+    double *f2 = dacc + 2 * nch * P;
+    float *rdens = nullptr;
+    if (weighted) {
+        void *wd = nullptr;
+        if ((r = scratch_get(ctx, SLOT_DENSITY, wt_bytes, &wd)) != MTI_OK) return r;
+        rdens = (float *)wd;
+        for (int i = 0; i < nj; i++) jd[i].f2 = f2 + jc0[i] * P;
+    }
+    // the job table (the host vectors live until the synchronisation below)
+    MTI_HIP(ctx, hipMemcpyAsync(d_jobs, jd.data(), (size_t)nj * sizeof(JobDev), hipMemcpyHostToDevice, st));
+    MTI_HIP(ctx, hipMemcpyAsync(d_rec0, rec0.data(), (size_t)(nj + 1) * 8, hipMemcpyHostToDevice, st));
+    MTI_HIP(ctx, hipMemcpyAsync(d_key0, key0.data(), (size_t)(nj + 1) * 8, hipMemcpyHostToDevice, st));
+    MTI_HIP(ctx, hipMemcpyAsync(d_blk0, blk0.data(), (size_t)(nj + 1) * 8, hipMemcpyHostToDevice, st));
+    MTI_HIP(ctx, hipMemcpyAsync(d_jc0, jc0.data(), (size_t)(nj + 1) * 8, hipMemcpyHostToDevice, st));
+    JobSet js;
+    js.jobs = d_jobs;
+    js.rec0 = d_rec0;
+    js.key0 = d_key0;
+    js.blk0 = d_blk0;
+    js.njobs = nj;
+    js.P = P;
 
     BinPassArgs ba;
-    ba.v = vw;
-    ba.g = make_geom(k, p);
-    ba.chan0 = chan0;
-    ba.nb = nb;
-    ba.psf = o->psf;
-    ba.valid = valid;
-    ba.desc = desc;
+    ba.js = js;
+    ba.cjc = cjc;
+    ba.desc = cdesc;
+    ba.wv = cwv;
+    ba.wt = cwt;
     ba.stats = dstats;
-    // per-record visibilities and weights, binning order and compacted
-    void *w7 = nullptr, *w8 = nullptr;
-    const size_t wv_bytes = align_up((size_t)nrec * P * sizeof(float2));
-    const size_t wt_bytes = align_up((size_t)nrec * P * sizeof(float));
-    r = scratch_get(ctx, SLOT_RECVIS, wv_bytes + wt_bytes, &w7);
-    if (r != MTI_OK) return r;
-    r = scratch_get(ctx, SLOT_CRECVIS, wv_bytes + wt_bytes, &w8);
-    if (r != MTI_OK) return r;
-    ba.wv = (float2 *)w7;
-    ba.wt = (float *)((char *)w7 + wv_bytes);
-    float2 *cwv = (float2 *)w8;
-    float *cwt = (float *)((char *)w8 + wv_bytes);
     {
         StageTimer tm(ctx, MTI_STAGE_BIN, st);
-        r = launch_bin_pass(ctx, ba, nrec, st);
+        r = launch_bin_pass(ctx, ba, blk0[nj], st);
         if (r != MTI_OK) return r;
     }
     const unsigned gblocks = (unsigned)ceil_div(nrec, 256);
     const unsigned kblocks = (unsigned)ceil_div(nkeys, 256);
+    GridArgs ga;
+    ga.js = js;
+    ga.cdesc = cdesc;
+    ga.cjc = cjc;
+    ga.n_rec = M_d;
+    ga.wvis = cwv;
+    ga.wt = cwt;
+    ga.rdens = rdens;
+    ga.pairs = pairs;
+    ga.units = units;
+    ga.n_units = NU_d;
     {
         StageTimer tm(ctx, MTI_STAGE_SORT_RECORDS, st);
-        // stable compaction of gridded records (keeps baseline-major order)
-        r = exclusive_scan_u32(ctx, valid, nrec, M_d, st);
-        if (r != MTI_OK) return r;
-        hipLaunchKernelGGL(k_compact, dim3(gblocks), dim3(256), 0, st, valid, M_d, desc, ba.wv, ba.wt, nrec,
-                           vw.nidx, P, cdesc, cchan, cwv, cwt);
+        // every record stays in place (job, channel, baseline order); records
+        // that are not gridded have support 0 and produce no pairs
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, M_d, (int64_t)1, (uint32_t)nrec);
         // counting sort of (record, tile) pairs by tile
         hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, st, start, nkeys, 0u);
         hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, st, fill, nkeys, 0u);
-        hipLaunchKernelGGL(k_tile_pairs<0>, dim3(gblocks), dim3(256), 0, st, cdesc, cchan, M_d, nrec, nty, ntx,
-                           start, (const uint32_t *)nullptr, (uint32_t *)nullptr);
+        hipLaunchKernelGGL(k_tile_pairs<0>, dim3(gblocks), dim3(256), 0, st, ga, nrec, start,
+                           (const uint32_t *)nullptr, (uint32_t *)nullptr);
         MTI_HIP(ctx, hipGetLastError());
         r = exclusive_scan_u32(ctx, start, nkeys, NP_d, st);
         if (r != MTI_OK) return r;
-        hipLaunchKernelGGL(k_tile_pairs<1>, dim3(gblocks), dim3(256), 0, st, cdesc, cchan, M_d, nrec, nty, ntx,
-                           fill, (const uint32_t *)start, pairs);
+        hipLaunchKernelGGL(k_tile_pairs<1>, dim3(gblocks), dim3(256), 0, st, ga, nrec, fill, (const uint32_t *)start,
+                           pairs);
         MTI_HIP(ctx, hipGetLastError());
     }
     {
@@ -679,93 +804,91 @@ static mti_status grid_batch(mti_ctx *ctx, const mti_kernels *k, const mti_image
         if (r != MTI_OK) return r;
         hipLaunchKernelGGL(k_unit_list, dim3(kblocks), dim3(256), 0, st, start, NP_d, nunits, NU_d, nkeys, units);
         MTI_HIP(ctx, hipGetLastError());
-    }
-    TileArgs ta;
-    ta.v = vw;
-    ta.chan0 = chan0;
-    ta.cdesc = cdesc;
-    ta.cchan = cchan;
-    ta.n_rec = M_d;
-    ta.wvis = cwv;
-    ta.wt = cwt;
-    ta.pairs = pairs;
-    ta.units = units;
-    ta.n_units = NU_d;
-    ta.table = k->d_table;
-    ta.sums = k->d_sums;
-    ta.convinfo = k->d_convinfo;
-    ta.T2 = k->info.T * k->info.T;
-    ta.nx = p->nx;
-    ta.ny = p->ny;
-    ta.ntx = ntx;
-    ta.nty = nty;
-    if (o->grid_layout == MTI_GRID_VUP) {
-        ta.cs = P;
-        ta.ps = 1;
-    } else {
-        ta.cs = 1;
-        ta.ps = gsz;
-    }
-    ta.chan_stride = gsz * P;
-    ta.grid = G;
-    ta.norm = norm;
-    ta.psf = o->psf;
-    ta.weighting = o->weighting;
-    ta.density = nullptr;
-    ta.f2 = nullptr;
-    if (o->weighting != MTI_WEIGHT_NATURAL) {
-        StageTimer tm(ctx, MTI_STAGE_PLAN, st);
-        void *w5 = nullptr, *w6 = nullptr;
-        r = scratch_get(ctx, SLOT_DENSITY, (size_t)gsz * nb * P * sizeof(float), &w5);
-        if (r != MTI_OK) return r;
-        r = scratch_get(ctx, SLOT_NORM, (size_t)nb * P * 3 * sizeof(double) + 256, &w6);
-        if (r != MTI_OK) return r;
-        ta.density = (float *)w5;
-        double *dacc = (double *)w6;
-        double *f2 = dacc + 2 * nb * P;
-        ta.f2 = f2;
-        hipLaunchKernelGGL(k_fill_f32, dim3(8192), dim3(256), 0, st, ta.density, gsz * nb * P, 0.0f);
-        hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(256), 0, st, dacc, (int64_t)nb * P * 2, 0.0);
-        MTI_HIP(ctx, hipGetLastError());
-        switch (P) {
-        case 1: r = launch_density<1>(ctx, ta, st); break;
-        case 2: r = launch_density<2>(ctx, ta, st); break;
-        default: r = launch_density<4>(ctx, ta, st); break;
+        if (weighted) {
+            hipLaunchKernelGGL(k_fill_f64, dim3(64), dim3(256), 0, st, dacc, (int64_t)nch * P * 2, 0.0);
+            MTI_HIP(ctx, hipGetLastError());
+            switch (P) {
+            case 1: r = launch_density<1>(ctx, ga, start, NP_d, nkeys, d_jc0, dacc, st); break;
+            case 2: r = launch_density<2>(ctx, ga, start, NP_d, nkeys, d_jc0, dacc, st); break;
+            default: r = launch_density<4>(ctx, ga, start, NP_d, nkeys, d_jc0, dacc, st); break;
+            }
+            if (r != MTI_OK) return r;
+            hipLaunchKernelGGL(k_briggs_f2, dim3((unsigned)ceil_div(nch * P, 64)), dim3(64), 0, st, js,
+                               (const int64_t *)d_jc0, (int)nch, (const double *)dacc, o->robust);
+            MTI_HIP(ctx, hipGetLastError());
         }
-        if (r != MTI_OK) return r;
-        hipLaunchKernelGGL(k_density_stats, dim3(256, (unsigned)nb), dim3(256), 0, st, ta.density, gsz, P, dacc);
-        hipLaunchKernelGGL(k_briggs_f2, dim3((unsigned)ceil_div(nb * P, 64)), dim3(64), 0, st, dacc, nb * P,
-                           o->robust, f2);
-        MTI_HIP(ctx, hipGetLastError());
-    }
-    {
-        StageTimer tm(ctx, MTI_STAGE_PLAN, st);
         switch (P) {
-        case 1: r = launch_prep<1>(ctx, ta, st); break;
-        case 2: r = launch_prep<2>(ctx, ta, st); break;
-        default: r = launch_prep<4>(ctx, ta, st); break;
+        case 1: r = launch_prep<1>(ctx, ga, st); break;
+        case 2: r = launch_prep<2>(ctx, ga, st); break;
+        default: r = launch_prep<4>(ctx, ga, st); break;
         }
         if (r != MTI_OK) return r;
     }
     {
         StageTimer tm(ctx, MTI_STAGE_GRID, st);
-        const int prec = k->info.precision;
         switch (P) {
-        case 1: r = dispatch_tiles<1>(ctx, prec, ta, st); break;
-        case 2: r = dispatch_tiles<2>(ctx, prec, ta, st); break;
-        default: r = dispatch_tiles<4>(ctx, prec, ta, st); break;
+        case 1: r = dispatch_tiles<1>(ctx, prec, ga, st); break;
+        case 2: r = dispatch_tiles<2>(ctx, prec, ga, st); break;
+        default: r = dispatch_tiles<4>(ctx, prec, ga, st); break;
         }
         if (r != MTI_OK) return r;
     }
+    uint32_t h[2] = {0, 0};
+    MTI_HIP(ctx, hipMemcpyAsync(&h[0], NU_d, 4, hipMemcpyDeviceToHost, st));
+    MTI_HIP(ctx, hipMemcpyAsync(&h[1], NP_d, 4, hipMemcpyDeviceToHost, st));
+    // also keeps the job table's host copies alive until the copies are done
+    MTI_HIP(ctx, hipStreamSynchronize(st));
     if (nu_total) {
-        uint32_t h[2] = {0, 0};
-        MTI_HIP(ctx, hipMemcpyAsync(&h[0], NU_d, 4, hipMemcpyDeviceToHost, st));
-        MTI_HIP(ctx, hipMemcpyAsync(&h[1], NP_d, 4, hipMemcpyDeviceToHost, st));
-        MTI_HIP(ctx, hipStreamSynchronize(st));
         nu_total[0] += h[0];
         nu_total[1] += h[1];
     }
     return MTI_OK;
+}
+
+static mti_status check_opts(mti_ctx *ctx, const mti_grid_opts *o)
+{
+    MTI_CHECK(ctx, o, "null options");
+    MTI_CHECK(ctx, o->weighting >= MTI_WEIGHT_NATURAL && o->weighting <= MTI_WEIGHT_BRIGGS, "bad weighting");
+    MTI_CHECK(ctx, o->grid_layout == MTI_GRID_VUP || o->grid_layout == MTI_GRID_PVU, "bad grid_layout");
+    MTI_CHECK(ctx, o->grid_precision == MTI_GRID_C64 || o->grid_precision == MTI_GRID_C128, "bad grid_precision");
+    return MTI_OK;
+}
+
+static mti_status check_job_geometry(mti_ctx *ctx, const mti_kernels *k, const mti_image_params *p, const mti_vis *v)
+{
+    mti_status r = validate_params(ctx, p);
+    if (r != MTI_OK) return r;
+    r = validate_vis(ctx, v);
+    if (r != MTI_OK) return r;
+    MTI_CHECK(ctx, k, "null kernels");
+    MTI_CHECK(ctx, k->info.sampling == p->sampling, "kernels built for another sampling");
+    MTI_CHECK(ctx, (int64_t)p->nx * p->ny < (int64_t(1) << 30), "grid above 2^30 cells");
+    MTI_CHECK(ctx, p->nx <= 65535 && p->ny <= 65535, "grid side above 65535");
+    return MTI_OK;
+}
+
+static mti_status stats_out(mti_ctx *ctx, unsigned long long *dstats, const uint32_t *nu, mti_grid_stats *stats,
+                            hipStream_t st)
+{
+    if (!stats) return MTI_OK;
+    unsigned long long h[2] = {0, 0};
+    MTI_HIP(ctx, hipMemcpyAsync(h, dstats, sizeof(h), hipMemcpyDeviceToHost, st));
+    MTI_HIP(ctx, hipStreamSynchronize(st));
+    *stats = mti_grid_stats{};
+    stats->gridded_records = (int64_t)h[0];
+    stats->gridded_points = (int64_t)h[1];
+    stats->work_items = nu[0];
+    stats->flushes = nu[1];
+    return MTI_OK;
+}
+
+// Largest channel count of one job piece that keeps a pass within its limits.
+static int64_t channels_per_pass(const PassJob &j)
+{
+    PassJob one = j;
+    one.nb = 1;
+    const int64_t pc = std::max<int64_t>(1, job_pairs(one));
+    return std::min<int64_t>(((int64_t(1) << 32) - 1) / pc, ((int64_t(1) << 31) - 1) / std::max<int64_t>(1, j.vw.nidx));
 }
 
 }  // namespace mti
@@ -777,23 +900,16 @@ extern "C" mti_status mti_grid(mti_ctx *ctx, const mti_kernels *k, const mti_ima
                                mti_grid_stats *stats, void *stream)
 {
     if (!ctx) return MTI_ERR_INVALID;
-    mti_status r = validate_params(ctx, p);
+    mti_status r = check_job_geometry(ctx, k, p, v);
     if (r != MTI_OK) return r;
-    r = validate_vis(ctx, v);
+    r = check_opts(ctx, o);
     if (r != MTI_OK) return r;
-    MTI_CHECK(ctx, k && o && grids && norm, "null kernels/options/outputs");
-    MTI_CHECK(ctx, k->info.sampling == p->sampling, "kernels built for another sampling");
-    MTI_CHECK(ctx, o->nchan >= 1, "opts.nchan must be >= 1");
+    MTI_CHECK(ctx, grids && norm, "null outputs");
+    MTI_CHECK(ctx, o->nchan >= 1 && o->nchan <= 65535, "opts.nchan must be in [1, 65535]");
     if (v->inv_lambda_per_row)
         MTI_CHECK(ctx, o->nchan == 1, "per-row 1/lambda implies a single channel per call");
     else
         MTI_CHECK(ctx, o->chan0 >= 0 && o->chan0 + o->nchan <= v->nchan, "channel range out of bounds");
-    MTI_CHECK(ctx, o->weighting >= MTI_WEIGHT_NATURAL && o->weighting <= MTI_WEIGHT_BRIGGS, "bad weighting");
-    MTI_CHECK(ctx, o->grid_layout == MTI_GRID_VUP || o->grid_layout == MTI_GRID_PVU, "bad grid_layout");
-    MTI_CHECK(ctx, o->grid_precision == MTI_GRID_C64 || o->grid_precision == MTI_GRID_C128,
-              "bad grid_precision");
-    MTI_CHECK(ctx, (int64_t)p->nx * p->ny < (int64_t(1) << 30), "grid above 2^30 cells");
-    MTI_CHECK(ctx, p->nx <= 65535 && p->ny <= 65535, "grid side above 65535");
     MTI_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = to_stream(stream);
 
@@ -817,19 +933,21 @@ extern "C" mti_status mti_grid(mti_ctx *ctx, const mti_kernels *k, const mti_ima
     MTI_HIP(ctx, hipGetLastError());
     uint32_t nu[2] = {0, 0};
 
-    // Channel batches keep records below 2^31 and (record, tile) pairs below 2^32.
-    const int tpa = tiles_per_axis(k->info.max_support);
-    const int64_t per_chan = std::max<int64_t>(1, vw.nidx * tpa * tpa);
-    const int64_t cap_chans = std::min<int64_t>(((int64_t(1) << 32) - 1) / per_chan,
-                                                ((int64_t(1) << 31) - 1) / std::max<int64_t>(1, vw.nidx));
+    // channel batches keep a pass within the record / pair limits
+    const PassJob base{k, *p, vw, o->chan0, 1, nullptr, o->grid_layout, nullptr};
+    const int64_t cap_chans = channels_per_pass(base);
     MTI_CHECK(ctx, cap_chans >= 1, "one channel has too many (record, tile) pairs");
     const int pair_batch = (int)std::min<int64_t>(nb, cap_chans);
+    const int prec = k->info.precision;
 
     if (o->grid_precision == MTI_GRID_C128) {
         for (int c0 = 0; c0 < nb && vw.nidx > 0; c0 += pair_batch) {
-            const int n = std::min(pair_batch, nb - c0);
-            r = grid_batch(ctx, k, p, vw, o, o->chan0 + c0, n, (double *)grids + (size_t)c0 * chan_floats,
-                           norm + (size_t)c0 * P, dstats, stats ? nu : nullptr, st);
+            PassJob j = base;
+            j.chan0 = o->chan0 + c0;
+            j.nb = std::min(pair_batch, nb - c0);
+            j.grid = (double *)grids + (size_t)c0 * chan_floats;
+            j.norm = norm + (size_t)c0 * P;
+            r = grid_pass(ctx, {j}, o, P, prec, dstats, nu, st);
             if (r != MTI_OK) return r;
         }
     } else {
@@ -845,8 +963,12 @@ extern "C" mti_status mti_grid(mti_ctx *ctx, const mti_kernels *k, const mti_ima
             hipLaunchKernelGGL(k_fill_f64, dim3(8192), dim3(256), 0, st, (double *)gw, chan_floats * n, 0.0);
             MTI_HIP(ctx, hipGetLastError());
             if (vw.nidx > 0) {
-                r = grid_batch(ctx, k, p, vw, o, o->chan0 + c0, n, (double *)gw, norm + (size_t)c0 * P, dstats,
-                               stats ? nu : nullptr, st);
+                PassJob j = base;
+                j.chan0 = o->chan0 + c0;
+                j.nb = n;
+                j.grid = (double *)gw;
+                j.norm = norm + (size_t)c0 * P;
+                r = grid_pass(ctx, {j}, o, P, prec, dstats, nu, st);
                 if (r != MTI_OK) return r;
             }
             hipLaunchKernelGGL(k_round_grid, dim3(8192), dim3(256), 0, st, (const double *)gw,
@@ -854,15 +976,90 @@ extern "C" mti_status mti_grid(mti_ctx *ctx, const mti_kernels *k, const mti_ima
             MTI_HIP(ctx, hipGetLastError());
         }
     }
-    if (stats) {
-        unsigned long long h[2] = {0, 0};
-        MTI_HIP(ctx, hipMemcpyAsync(h, dstats, sizeof(h), hipMemcpyDeviceToHost, st));
-        MTI_HIP(ctx, hipStreamSynchronize(st));
-        *stats = mti_grid_stats{};
-        stats->gridded_records = (int64_t)h[0];
-        stats->gridded_points = (int64_t)h[1];
-        stats->work_items = nu[0];
-        stats->flushes = nu[1];
+    return stats_out(ctx, dstats, nu, stats, st);
+}
+
+extern "C" mti_status mti_grid_jobs(mti_ctx *ctx, int32_t njobs, const mti_job *jobs, const mti_grid_opts *o,
+                                    mti_grid_stats *stats, void *stream)
+{
+    if (!ctx) return MTI_ERR_INVALID;
+    mti_status r = check_opts(ctx, o);
+    if (r != MTI_OK) return r;
+    MTI_CHECK(ctx, njobs >= 0 && njobs <= 65535 && (njobs == 0 || jobs), "njobs must be in [0, 65535]");
+    MTI_CHECK(ctx, o->grid_precision == MTI_GRID_C128, "mti_grid_jobs grids into complex128 grids");
+    MTI_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = to_stream(stream);
+    int P = 0, prec = -1;
+    std::vector<PassJob> all;
+    for (int i = 0; i < njobs; i++) {
+        const mti_job &j = jobs[i];
+        r = check_job_geometry(ctx, j.kernels, &j.params, &j.vis);
+        if (r != MTI_OK) return r;
+        MTI_CHECK(ctx, j.grids && j.norm, "null job outputs");
+        MTI_CHECK(ctx, j.nchan >= 1 && j.nchan <= 65535, "job nchan must be in [1, 65535]");
+        if (j.vis.inv_lambda_per_row)
+            MTI_CHECK(ctx, j.nchan == 1, "per-row 1/lambda implies a single channel per job");
+        else
+            MTI_CHECK(ctx, j.chan0 >= 0 && j.chan0 + j.nchan <= j.vis.nchan, "job channel range out of bounds");
+        if (i == 0) {
+            P = j.vis.npol;
+            prec = j.kernels->info.precision;
+        }
+        MTI_CHECK(ctx, j.vis.npol == P, "jobs of one call must share npol");
+        MTI_CHECK(ctx, j.kernels->info.precision == prec, "jobs of one call must share the table precision");
+        const PassJob pj{j.kernels, j.params, make_view(&j.vis), j.chan0, j.nchan, (double *)j.grids, o->grid_layout,
+                         j.norm};
+        MTI_CHECK(ctx, pj.vw.nidx < (int64_t(1) << 31), "more than 2^31-1 rows in a job");
+        MTI_CHECK(ctx, channels_per_pass(pj) >= 1, "one channel has too many (record, tile) pairs");
+        all.push_back(pj);
     }
-    return MTI_OK;
+    void *sw = nullptr;
+    r = scratch_get(ctx, SLOT_MISC2, 256, &sw);
+    if (r != MTI_OK) return r;
+    unsigned long long *dstats = (unsigned long long *)sw;
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, (uint32_t *)dstats, 8, 0u);
+    MTI_HIP(ctx, hipGetLastError());
+    uint32_t nu[2] = {0, 0};
+    // zero outputs
+    if (!o->accumulate) {
+        for (const PassJob &j : all) {
+            hipLaunchKernelGGL(k_fill_f64, dim3(1024), dim3(256), 0, st, j.grid,
+                               (int64_t)j.p.nx * j.p.ny * P * 2 * j.nb, 0.0);
+            hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(64), 0, st, j.norm, (int64_t)j.nb * P, 0.0);
+        }
+        MTI_HIP(ctx, hipGetLastError());
+    }
+    // passes: consecutive jobs while the pass stays within its limits (a job
+    // too large for one pass is split by channel)
+    std::vector<PassJob> pass;
+    int64_t in_r = 0, in_p = 0, in_k = 0;
+    for (const PassJob &j : all) {
+        const int64_t cap = channels_per_pass(j);
+        const int64_t gch = (int64_t)j.p.nx * j.p.ny * P;  // complex elements per channel grid
+        for (int c0 = 0; c0 < j.nb;) {
+            PassJob piece = j;
+            piece.chan0 = j.chan0 + c0;
+            piece.nb = (int)std::min<int64_t>(j.nb - c0, cap);
+            piece.grid = j.grid + 2 * gch * c0;
+            piece.norm = j.norm + (size_t)c0 * P;
+            if (!pass.empty() &&
+                (in_r + job_records(piece) >= (int64_t(1) << 31) || in_p + job_pairs(piece) >= (int64_t(1) << 32) ||
+                 in_k + job_keys(piece) >= (int64_t(1) << 31) || pass.size() >= 32767)) {
+                r = grid_pass(ctx, pass, o, P, prec, dstats, nu, st);
+                if (r != MTI_OK) return r;
+                pass.clear();
+                in_r = in_p = in_k = 0;
+            }
+            pass.push_back(piece);
+            in_r += job_records(piece);
+            in_p += job_pairs(piece);
+            in_k += job_keys(piece);
+            c0 += piece.nb;
+        }
+    }
+    if (!pass.empty()) {
+        r = grid_pass(ctx, pass, o, P, prec, dstats, nu, st);
+        if (r != MTI_OK) return r;
+    }
+    return stats_out(ctx, dstats, nu, stats, st);
 }

@@ -31,22 +31,65 @@ struct BinGeom {
     const int2 *convinfo;  // {support, base}
 };
 
+// One imaging job of a gridding call (device-resident table).  Records are
+// numbered job-major; within a job channel-major (c_local * nidx + i).
+struct JobDev {
+    VisView v;
+    BinGeom g;
+    int chan0, nb;           // first channel, channels gridded
+    int64_t rec0;            // first record of the job
+    int64_t key0;            // first (channel, tile) key of the job
+    int ntx, nty;            // tiles per row / column
+    const void *table;       // kernel samples (precision of the call)
+    const float *sums;       // [nplanes][T*T]
+    int T2;
+    double *grid;            // FP64 grid of channel chan0; channel c at grid + 2*c*chan_stride
+    int64_t cs, ps;          // complex strides of cell and correlation
+    int64_t chan_stride;     // complex elements per channel grid
+    double *norm;            // [nb][P]
+    double *f2;              // [nb][P] Briggs factor
+    int psf, weighting;
+};
+
+// The jobs of a call and prefix tables for locating them.
+struct JobSet {
+    const JobDev *jobs;
+    const int64_t *rec0;     // [njobs + 1] record prefix
+    const int64_t *key0;     // [njobs + 1] key prefix
+    const int64_t *blk0;     // [njobs + 1] binning-block prefix
+    int njobs;
+    int P;
+};
+
+// Largest j with prefix[j] <= x (prefix ascending, prefix[0] = 0).
+__device__ __forceinline__ int job_of(const int64_t *prefix, int n, int64_t x)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 // Record descriptor written by the binning pass (16 bytes):
 //   x = row, y = corner_y << 16 | corner_x, z = conv_idx,
 //   w = plane << 16 | conj << 15 | support
+// Every (job, channel, row) gets a record; one that is not gridded (all
+// correlations flagged, or its support box off the grid) has support 0 and
+// no visibilities or weights.
 struct BinPassArgs {
-    VisView v;
-    BinGeom g;
-    int chan0, nb;
-    int psf;
-    uint32_t *valid;            // [nrec] 1 if the record is gridded
+    JobSet js;
+    int32_t *cjc;               // [nrec] job << 16 | local channel
     int4 *desc;                 // [nrec]
     float2 *wv;                 // [nrec][npol] visibilities of gridded records
     float *wt;                  // [nrec][npol] weights, 0 where flagged
     unsigned long long *stats;  // [0] gridded records, [1] gridded points
 };
 
-mti_status launch_bin_pass(mti_ctx *ctx, const BinPassArgs &a, int64_t nrec, hipStream_t st);
+mti_status launch_bin_pass(mti_ctx *ctx, const BinPassArgs &a, int64_t nblocks, hipStream_t st);
+int64_t bin_blocks(int64_t nidx, int nb);
 VisView make_view(const mti_vis *v);
 BinGeom make_geom(const mti_kernels *k, const mti_image_params *p);
 

@@ -35,7 +35,7 @@ struct mti_ctx {
     };
     std::vector<StageEv> stage_events;
     std::vector<hipEvent_t> event_pool;
-    mti::Scratch scratch[14];
+    mti::Scratch scratch[16];
     // FFT twiddle tables keyed by length (FP64, exp(+2 pi i k / n)).
     std::vector<std::pair<int, double2 *>> twiddles;
 };
@@ -66,9 +66,11 @@ enum ScratchSlot {
     SLOT_NORM = 9,
     SLOT_GRID64 = 10,
     SLOT_MISC2 = 11,
-    SLOT_RECVIS = 12,   // per-record visibilities + weights (binning order)
-    SLOT_CRECVIS = 13,  // the same, compacted
-    SLOT_COUNT = 14
+    SLOT_RECVIS = 12,   // unused
+    SLOT_CRECVIS = 13,  // per-record visibilities + weights
+    SLOT_CHANDESC = 14, // per-channel descriptors of the finalize
+    SLOT_CORR = 15,     // per-job correction vectors of the batched finalize
+    SLOT_COUNT = 16
 };
 
 inline mti_status set_error(mti_ctx *ctx, mti_status code, const std::string &m)

@@ -61,6 +61,13 @@ class GridOpts(ctypes.Structure):
                 ("grid_layout", ctypes.c_int32), ("grid_precision", ctypes.c_int32)]
 
 
+class Job(ctypes.Structure):
+    """mti_job: one imaging job of a batched call."""
+    _fields_ = [("kernels", ctypes.c_void_p), ("params", ImageParams), ("vis", Vis),
+                ("chan0", ctypes.c_int32), ("nchan", ctypes.c_int32),
+                ("grids", ctypes.c_void_p), ("norm", ctypes.c_void_p), ("image", ctypes.c_void_p)]
+
+
 class GridStats(ctypes.Structure):
     _fields_ = [("gridded_records", ctypes.c_int64), ("gridded_points", ctypes.c_int64),
                 ("work_items", ctypes.c_int64), ("flushes", ctypes.c_int64)]
@@ -75,7 +82,7 @@ EXPORTS = ["mti_version", "mti_create", "mti_destroy", "mti_last_error",
            "mti_kernels_build", "mti_kernels_info", "mti_kernels_supports",
            "mti_kernels_table", "mti_kernels_sums", "mti_kernels_free",
            "mti_sort_rows", "mti_preprocess", "mti_grid", "mti_fft2d_inverse",
-           "mti_finalize"]
+           "mti_finalize", "mti_grid_jobs", "mti_finalize_jobs"]
 
 _lib = None
 
@@ -125,6 +132,10 @@ def lib():
     L.mti_finalize.restype = ctypes.c_int
     L.mti_finalize.argtypes = [vp, P(ImageParams), ctypes.c_int32, ctypes.c_int32,
                                ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, vp, vp, vp, vp]
+    L.mti_grid_jobs.restype = ctypes.c_int
+    L.mti_grid_jobs.argtypes = [vp, ctypes.c_int32, vp, P(GridOpts), P(GridStats), vp]
+    L.mti_finalize_jobs.restype = ctypes.c_int
+    L.mti_finalize_jobs.argtypes = [vp, ctypes.c_int32, vp, ctypes.c_int32, ctypes.c_int32, vp]
     _lib = L
     return L
 
@@ -237,6 +248,39 @@ class Context:
                                 grids.data_ptr(), norm.data_ptr(), image.data_ptr(),
                                 _stream(stream))
         self.check(rc, "mti_finalize")
+
+
+    # ---- batched jobs ---------------------------------------------------------
+    @staticmethod
+    def make_job(kern, params, vis, grids, norm, image=None, chan0=0, nchan=1):
+        """An mti_job; grids complex128 [nchan] grids, norm f64 [nchan][npol],
+        image f32 [nchan][npol][ny][nx] (for finalize_jobs)."""
+        j = Job()
+        j.kernels = kern.h
+        j.params = params
+        j.vis = vis
+        j.chan0, j.nchan = int(chan0), int(nchan)
+        j.grids, j.norm = grids.data_ptr(), norm.data_ptr()
+        j.image = image.data_ptr() if image is not None else None
+        j._keep = (kern, vis, grids, norm, image)
+        return j
+
+    def grid_jobs(self, jobs, accumulate=False, psf=False, weighting="natural", robust=0.0,
+                  layout="vup", stats=True, stream=None):
+        arr = (Job * len(jobs))(*jobs)
+        o = GridOpts(0, 0, 1 if accumulate else 0, 1 if psf else 0, WEIGHTING[weighting],
+                     float(robust), LAYOUT[layout], GRID_PRECISION["c128"])
+        st = GridStats()
+        rc = lib().mti_grid_jobs(self.h, len(jobs), ctypes.cast(arr, ctypes.c_void_p), ctypes.byref(o),
+                                 ctypes.byref(st) if stats else None, _stream(stream))
+        self.check(rc, "mti_grid_jobs")
+        return st
+
+    def finalize_jobs(self, jobs, poltype="linear", layout="vup", stream=None):
+        arr = (Job * len(jobs))(*jobs)
+        rc = lib().mti_finalize_jobs(self.h, len(jobs), ctypes.cast(arr, ctypes.c_void_p), POLTYPE[poltype],
+                                     LAYOUT[layout], _stream(stream))
+        self.check(rc, "mti_finalize_jobs")
 
 
 class Kernels:

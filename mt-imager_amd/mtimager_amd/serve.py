@@ -1,0 +1,268 @@
+"""Serving loop for imaging requests on one GPU: continuous batching over the
+batched C-ABI entry points (mti_grid_jobs + mti_finalize_jobs).
+
+The reference images one job at a time (one WImager DAG per channel,
+comp/WImager.cpp:167-234, driven by app/mtimager.cpp:151-170); it has no
+request loop.  This module adds one above the same operators:
+
+* a request carries its visibilities (already on the device), its imaging
+  parameters and its channel range;
+* admission is by an HBM budget: a request is admitted when the bytes of its
+  grids, image and per-record work space fit beside those of the requests
+  already admitted, so memory, not a queue length, bounds what is in flight;
+* W-kernel tables are built once per (nx, ny, cell, mode, w-planes,
+  oversampling, taper support) and kept on the device in a cache shared by
+  all workers;
+* each worker owns a library context and a HIP stream; whenever it is free it
+  takes every admitted request waiting (first come, first served) and images
+  them as one wave, so requests that arrive while a wave runs join the next
+  one and the waves of different workers overlap on the device.
+
+Results do not depend on which wave a request lands in: each job of a wave
+has its own grids, normaliser and density, and the FP64 accumulation differs
+from a solo run only in summation order.
+"""
+import collections
+import threading
+import time
+
+from . import Context, MTIError, vis_ms
+
+WORK_BYTES_PER_VIS = 64  # record descriptors, weighted vis, pairs and keys per (row, channel)
+
+
+def request_bytes(nx, ny, npol, nchan, nvis):
+    """Device bytes a request holds while admitted: complex128 grids, f64
+    normaliser, f32 image and the per-record work space of its wave."""
+    pix = int(nx) * int(ny) * int(npol) * int(nchan)
+    return pix * (16 + 4) + 8 * npol * nchan + WORK_BYTES_PER_VIS * int(nvis)
+
+
+class Request:
+    """One imaging request.  vis_args: (uvw, inv_lambda, vis, flags, weight)
+    device tensors in MeasurementSet order (see mtimager_amd.vis_ms);
+    ant1/ant2: the antenna columns of those rows (int32 device tensors), for
+    the baseline-major row order; params: ImageParams; nchan channels from
+    chan0; npol correlations.
+
+    After the request is done: image f32 [nchan][npol][ny][nx] and norm f64
+    [nchan][npol] (if keep_image), arrival/admitted/done perf_counter stamps,
+    error (an exception, or None), wave (the wave number it ran in) and
+    gridded (its gridded (row, channel) count when it ran alone in its wave;
+    a batched wave reports one count for all of its requests, summed in
+    Server.gridded)."""
+
+    __slots__ = ("rid", "params", "vis_args", "ant1", "ant2", "nant", "chan0", "nchan", "npol",
+                 "nvis", "keep_image", "arrival", "admitted", "done", "image", "norm",
+                 "gridded", "error", "nbytes", "wave")
+
+    def __init__(self, rid, params, vis_args, ant1, ant2, nant, nchan, npol=1, chan0=0,
+                 nvis=None, keep_image=True):
+        self.rid = rid
+        self.params = params
+        self.vis_args = vis_args
+        self.ant1, self.ant2, self.nant = ant1, ant2, int(nant)
+        self.chan0, self.nchan, self.npol = int(chan0), int(nchan), int(npol)
+        self.nvis = int(nvis) if nvis is not None else int(ant1.numel()) * int(nchan)
+        self.keep_image = keep_image
+        self.arrival = self.admitted = self.done = None
+        self.image = self.norm = None
+        self.gridded = 0
+        self.error = None
+        self.nbytes = request_bytes(params.nx, params.ny, npol, nchan, self.nvis)
+        self.wave = -1
+
+
+class KernelCache:
+    """Device-resident W-kernel tables keyed by the parameters they depend on.
+    Thread-safe; a miss builds on the caller's context and stream."""
+
+    def __init__(self, precision="f32"):
+        self.precision = precision
+        self._lock = threading.Lock()
+        self._tab = {}
+        self.hits = self.misses = 0
+
+    @staticmethod
+    def key(p):
+        return (p.nx, p.ny, p.cell_l, p.cell_m, p.mode, p.wplanes, p.sampling, p.taper_support)
+
+    def get(self, ctx, params, stream=None):
+        k = self.key(params)
+        with self._lock:
+            K = self._tab.get(k)
+            if K is not None:
+                self.hits += 1
+                return K
+            self.misses += 1
+            K = ctx.build_kernels(params, self.precision, stream)
+            self._tab[k] = K
+            return K
+
+    def __len__(self):
+        return len(self._tab)
+
+    def clear(self):
+        with self._lock:
+            for K in self._tab.values():
+                K.free()
+            self._tab.clear()
+
+
+class Server:
+    """Continuous-batching server.  submit() hands in a request (time-stamped
+    on arrival); workers image admitted requests in waves; wait() blocks until
+    every submitted request is done.
+
+    hbm_budget: bytes of grids/images/work space admitted at once;
+    workers: contexts (and streams) working concurrently;
+    max_wave: most requests in one wave;
+    cache: a KernelCache, or None to build the tables of every request afresh;
+    plain: the one-job-at-a-time baseline (one worker, waves of one request,
+    the single-job entry points mti_grid + mti_finalize, tables built per
+    request);
+    weighting/robust/poltype: imaging options shared by all requests."""
+
+    def __init__(self, hbm_budget, workers=2, max_wave=256, cache=None, device=0,
+                 weighting="natural", robust=0.0, poltype="linear", plain=False):
+        import torch
+        self.torch = torch
+        self.budget = int(hbm_budget)
+        self.plain = bool(plain)
+        if self.plain:
+            workers, max_wave, cache = min(int(workers), 1), 1, None
+        self.max_wave = int(max_wave)
+        self.cache = cache
+        self.device = device
+        self.weighting, self.robust, self.poltype = weighting, robust, poltype
+        self.gridded = 0
+        self.errors = []
+        self.wave_sizes = []
+        self._cv = threading.Condition()
+        self._queue = collections.deque()
+        self._in_use = 0
+        self._pending = 0
+        self._stop = False
+        self._threads = []
+        for _ in range(int(workers)):
+            t = threading.Thread(target=self._worker, daemon=True)
+            t.start()
+            self._threads.append(t)
+
+    # ---- client side --------------------------------------------------------
+    def submit(self, req, now=None):
+        if req.nbytes > self.budget:
+            raise MTIError(f"request {req.rid} needs {req.nbytes} bytes, above the HBM budget {self.budget}")
+        req.arrival = time.perf_counter() if now is None else now
+        with self._cv:
+            self._queue.append(req)
+            self._pending += 1
+            self._cv.notify_all()
+
+    def wait(self, timeout=None):
+        """Blocks until every submitted request is done; False on timeout."""
+        end = None if timeout is None else time.perf_counter() + timeout
+        with self._cv:
+            while self._pending:
+                left = None if end is None else end - time.perf_counter()
+                if left is not None and left <= 0:
+                    return False
+                self._cv.wait(left)
+        return True
+
+    def close(self):
+        """Stops the workers after their current wave (queued requests are dropped)."""
+        with self._cv:
+            self._stop = True
+            self._cv.notify_all()
+        for t in self._threads:
+            t.join()
+
+    # ---- admission (called with the lock held) --------------------------------
+    def _head_fits(self):
+        return bool(self._queue) and self._in_use + self._queue[0].nbytes <= self.budget
+
+    def _take_wave(self):
+        """Requests of the next wave: first come first served while they fit
+        the budget.  A request that does not fit waits at the head of the
+        queue, so later small requests cannot starve it."""
+        wave = []
+        while len(wave) < self.max_wave and self._head_fits():
+            r = self._queue.popleft()
+            self._in_use += r.nbytes
+            wave.append(r)
+        return wave
+
+    # ---- workers ------------------------------------------------------------
+    def _worker(self):
+        torch = self.torch
+        torch.cuda.set_device(self.device)
+        stream = torch.cuda.Stream(device=self.device)
+        ctx = Context(self.device)
+        try:
+            while True:
+                with self._cv:
+                    while not self._stop and not self._head_fits():
+                        self._cv.wait()
+                    if self._stop:
+                        return
+                    wave = self._take_wave()
+                    self.wave_sizes.append(len(wave))
+                    wid = len(self.wave_sizes)
+                t_adm = time.perf_counter()
+                for r in wave:
+                    r.admitted, r.wave = t_adm, wid
+                try:
+                    self._run_wave(ctx, stream, wave)
+                except Exception as e:  # the wave fails as a whole; its requests carry the error
+                    for r in wave:
+                        r.error = e
+                    self.errors.append(e)
+                t = time.perf_counter()
+                with self._cv:
+                    for r in wave:
+                        r.done = t
+                        self._in_use -= r.nbytes
+                        self._pending -= 1
+                    self._cv.notify_all()
+        finally:
+            ctx.close()
+
+    def _run_wave(self, ctx, stream, wave):
+        torch = self.torch
+        sid = stream.cuda_stream
+        with torch.cuda.stream(stream):
+            jobs, owned = [], []
+            for r in wave:
+                idx = torch.empty(r.ant1.numel(), dtype=torch.int32, device="cuda")
+                kept = ctx.sort_rows(r.ant1, r.ant2, r.nant, idx, stream=sid)
+                v = vis_ms(*r.vis_args, row_index=idx[:kept])
+                p = r.params
+                if self.cache is not None:
+                    K = self.cache.get(ctx, p, sid)
+                else:
+                    K = ctx.build_kernels(p, "f32", sid)
+                grids = torch.empty((r.nchan, p.ny, p.nx, r.npol, 2), dtype=torch.float64, device="cuda")
+                norm = torch.empty((r.nchan, r.npol), dtype=torch.float64, device="cuda")
+                image = torch.empty((r.nchan, r.npol, p.ny, p.nx), dtype=torch.float32, device="cuda")
+                jobs.append(ctx.make_job(K, p, v, grids, norm, image, r.chan0, r.nchan))
+                owned.append((r, K, v, grids, norm, image))
+            if self.plain:
+                r, K, v, grids, norm, image = owned[0]
+                st = ctx.grid(K, r.params, v, grids, norm, chan0=r.chan0, nchan=r.nchan,
+                              weighting=self.weighting, robust=self.robust, precision="c128", stream=sid)
+                ctx.finalize(r.params, r.npol, grids, norm, image, nchan=r.nchan, poltype=self.poltype,
+                             precision="c128", stream=sid)
+            else:
+                st = ctx.grid_jobs(jobs, weighting=self.weighting, robust=self.robust, stream=sid)
+                ctx.finalize_jobs(jobs, poltype=self.poltype, stream=sid)
+            stream.synchronize()
+        with self._cv:
+            self.gridded += st.gridded_records
+        if len(owned) == 1:
+            owned[0][0].gridded = st.gridded_records
+        for r, K, v, grids, norm, image in owned:
+            if r.keep_image:
+                r.image, r.norm = image, norm
+            if self.cache is None:
+                K.free()

@@ -171,3 +171,49 @@ def config2_geometry(seed=2):
     return make_observation(nant=64, radius_m=8000.0, dec_deg=-45.0, ha_hours=(-4.0, 4.0),
                             ntime=1800, freqs_hz=np.linspace(0.9e9, 1.7e9, 64), npol=4,
                             fov_rad=4096 * 1.5 * ARCSEC, pol_frac=0.1, seed=seed, predict=False)
+
+
+def uvw_baselines(enu, a1, a2, lat_deg, dec_deg, H):
+    """uvw [ntime*nbl][3] (time-major rows) of the baselines (a1, a2) at hour
+    angles H (radians)."""
+    lat, dec = np.deg2rad(lat_deg), np.deg2rad(dec_deg)
+    E, N, U = enu[:, 0], enu[:, 1], enu[:, 2]
+    xyz = np.stack([-np.sin(lat) * N + np.cos(lat) * U, E, np.cos(lat) * N + np.sin(lat) * U], axis=1)
+    B = xyz[a2] - xyz[a1]
+    sH, cH = np.sin(H)[:, None], np.cos(H)[:, None]
+    Bx, By, Bz = B[None, :, 0], B[None, :, 1], B[None, :, 2]
+    u = sH * Bx + cH * By
+    v = -np.sin(dec) * cH * Bx + np.sin(dec) * sH * By + np.cos(dec) * Bz
+    w = np.cos(dec) * cH * Bx - np.cos(dec) * sH * By + np.sin(dec) * Bz
+    return np.stack([u, v, w], axis=-1).reshape(-1, 3)
+
+
+def snapshot_jobs(njobs=512, seed=4, min_vis=1e4, max_vis=5e6):
+    """Independent snapshot imaging jobs (Stokes I, P=1): 32-2016 baselines of
+    a 64-antenna 8 km array, 1-64 channels in 1.3-1.5 GHz, visibility counts
+    log-uniform in [min_vis, max_vis] (time steps of 16 s fill the count),
+    images of 256^2..2048^2 at a common 0.85 deg field, w-planes from
+    {8, 16, 32, 64}, oversampling 4 or 8.  Returns host-side specs."""
+    rng = np.random.default_rng(seed)
+    enu = array_layout(64, 8000.0, seed)
+    pa1, pa2 = np.triu_indices(64, k=1)
+    jobs = []
+    for j in range(njobs):
+        nvis = float(np.exp(rng.uniform(np.log(min_vis), np.log(max_vis))))
+        nbl = int(rng.integers(32, 2017))
+        nch = int(rng.integers(1, 65))
+        ntime = max(1, int(round(nvis / (nbl * nch))))
+        nx = int(rng.choice([256, 512, 1024, 2048]))
+        W = int(rng.choice([8, 16, 32, 64]))
+        s = int(rng.choice([4, 8]))
+        sel = rng.choice(len(pa1), nbl, replace=False)
+        a1, a2 = pa1[sel].astype(np.int32), pa2[sel].astype(np.int32)
+        h0 = rng.uniform(-4.0, 4.0 - ntime * 16.0 / 3600.0) if ntime * 16.0 < 8 * 3600 else -4.0
+        H = np.deg2rad(15.0 * (h0 + np.arange(ntime) * 16.0 / 3600.0))
+        uvw = uvw_baselines(enu, a1, a2, -30.7, -45.0, H)
+        freqs = np.linspace(1.3e9, 1.5e9, nch) if nch > 1 else np.array([1.4e9])
+        jobs.append(dict(nx=nx, cell=1.5 * 2048 / nx, wplanes=W, sampling=s, nchan=nch,
+                         uvw=uvw, ant1=np.tile(a1, ntime), ant2=np.tile(a2, ntime),
+                         inv_lambda=freqs / C_LIGHT, src=sky(20, 3072 * ARCSEC, seed + j),
+                         seed=seed + j))
+    return jobs
