@@ -68,6 +68,24 @@ def make_workload(torch, cfg, seed):
     return d
 
 
+def measured_traffic(cfg):
+    """HBM bytes per gridder launch measured with rocprofv3 FETCH_SIZE and
+    WRITE_SIZE (scaled by a copy kernel of known size) for this workload, as
+    recorded in profiles/; None when no measurement exists for it."""
+    if cfg != "c2":
+        return None
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "k_grid_tiles_counters_config2.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -208,9 +226,11 @@ def run_c4(args, torch, mti, jobs):
             need = j["nch"] * j["nx"] ** 2 * 20
             if wave and mem + need > budget:
                 flush()
-            v = prep(j)
+            # the batched call puts each job's rows in baseline-major order itself
+            v = mti.vis_ms(j["uvw"], j["invl"], j["vis"], j["flags"], j["weight"])
             gr, nr, im = outputs(j)
-            wave.append((ctx.make_job(cache[j["key"]], j["p"], v, gr, nr, im, 0, j["nch"]), gr, nr, im))
+            wave.append((ctx.make_job(cache[j["key"]], j["p"], v, gr, nr, im, 0, j["nch"],
+                                      ant1=j["a1"], ant2=j["a2"], nant=64), gr, nr, im))
             mem += need
         flush()
         el = time.perf_counter() - t0
@@ -243,14 +263,17 @@ def _pct(a, q):
     return round(float(np.percentile(np.asarray(a) * 1e3, q)), 1) if len(a) else None
 
 
-def _serve_run(torch, mti, serve, pool, picks, times, plain, budget, cache, deadline_s):
-    """Feeds requests at their arrival times into a Server; returns its figures."""
+def _serve_feed(srv, pool, picks, times, deadline_s):
+    """Feeds requests (pool entries `picks`) into a running Server at their
+    arrival times; waits until they are done or the deadline passes, then
+    drops what was never admitted.  Returns the figures of this stream."""
+    from mtimager_amd import serve
     reqs = []
     for i, k in enumerate(picks):
         j = pool[k]
         reqs.append(serve.Request(i, j["p"], (j["uvw"], j["invl"], j["vis"], j["flags"], j["weight"]),
                                   j["a1"], j["a2"], 64, j["nch"], npol=1, keep_image=False))
-    srv = serve.Server(budget, workers=2, max_wave=512, cache=cache, plain=plain)
+    g0, w0 = srv.gridded, len(srv.wave_sizes)
     t0 = time.perf_counter()
     for r, at in zip(reqs, times):
         dt = t0 + at - time.perf_counter()
@@ -259,48 +282,59 @@ def _serve_run(torch, mti, serve, pool, picks, times, plain, budget, cache, dead
         srv.submit(r)
     finished = srv.wait(timeout=max(1.0, t0 + deadline_s - time.perf_counter()))
     t_end = time.perf_counter()
-    srv.close()
+    if not finished:
+        srv.cancel_queued()
+        srv.wait()
     if srv.errors:
         raise srv.errors[0]
-    done = [r for r in reqs if r.done is not None]
+    done = [r for r in reqs if r.done is not None and r.error is None]
     lat = [r.done - r.arrival for r in done]
-    # requests still queued at the deadline count with their waiting time so far
-    lat_all = lat + [t_end - r.arrival for r in reqs if r.done is None]
+    # requests dropped at the deadline count with their waiting time until then
+    lat_all = lat + [t_end - r.arrival for r in reqs if r.done is None or r.error is not None]
     last = max((r.done for r in done), default=t_end)
+    sizes = srv.wave_sizes[w0:]
     return dict(requests=len(reqs), completed=len(done), saturated=not finished,
-                mvis_s=round(srv.gridded / max(last - t0, 1e-9) / 1e6, 3),
+                mvis_s=round((srv.gridded - g0) / max(last - t0, 1e-9) / 1e6, 3),
                 latency_ms={"p50": _pct(lat_all, 50), "p99": _pct(lat_all, 99)},
-                waves=len(srv.wave_sizes),
-                mean_wave=round(float(sum(srv.wave_sizes)) / max(1, len(srv.wave_sizes)), 2))
+                waves=len(sizes), mean_wave=round(float(sum(sizes)) / max(1, len(sizes)), 2))
 
 
 def run_c5(args, torch, mti, pool):
     """Config 5: Poisson streams of imaging requests drawn from the snapshot
     mix, served by continuous batching (two workers, HBM-budget admission,
     device-resident kernel-table cache) and by the plain one-job-at-a-time
-    path on the same arrivals.  Inputs are resident on the GPU; each request
-    images one pool job.  A run that has not drained by the deadline reports
-    saturated=true and counts queued requests with their waiting time."""
+    path on the same arrivals.  Both servers are long-lived (warm contexts,
+    streams and memory pools) and see the rates in turn.  Inputs are resident
+    on the GPU; each request images one pool job.  A stream that has not
+    drained by its deadline reports saturated=true; requests never admitted
+    count with their waiting time until the deadline."""
     import numpy as np
     from mtimager_amd import serve
     budget = int(args.serve_budget_gb * (1 << 30))
     cache = serve.KernelCache()
+    served = serve.Server(budget, workers=2, max_wave=512, cache=cache)
+    plain = serve.Server(budget, plain=True)
     rng = np.random.default_rng(5)
-    log("c5: warm-up (every kernel-table set into the cache, code objects loaded)")
-    warm = list(range(len(pool)))
-    _serve_run(torch, mti, serve, pool, warm, [0.0] * len(warm), False, budget, cache, 600)
     out = []
-    for rate in [float(x) for x in args.rates.split(",")]:
-        n = int(rng.poisson(rate * args.serve_seconds))
-        times = np.sort(rng.uniform(0.0, args.serve_seconds, n))
-        picks = rng.integers(0, len(pool), n)
-        log(f"c5: {rate:g} req/s, {n} requests, served")
-        sv = _serve_run(torch, mti, serve, pool, picks, times, False, budget, cache, args.serve_seconds * 3)
-        log(f"c5: {rate:g} req/s served {sv}")
-        pl = _serve_run(torch, mti, serve, pool, picks, times, True, budget, None, args.serve_seconds * 3)
-        log(f"c5: {rate:g} req/s plain {pl}")
-        out.append(dict(rate=rate, served=sv, plain=pl,
-                        speedup_mvis=round(sv["mvis_s"] / max(pl["mvis_s"], 1e-9), 3)))
+    try:
+        log("c5: warm-up (every kernel-table set into the cache, memory pools, code objects)")
+        allj = np.arange(len(pool))
+        _serve_feed(served, pool, allj, np.zeros(len(pool)), 600)
+        _serve_feed(plain, pool, allj, np.zeros(len(pool)), 600)
+        for rate in [float(x) for x in args.rates.split(",")]:
+            n = int(rng.poisson(rate * args.serve_seconds))
+            times = np.sort(rng.uniform(0.0, args.serve_seconds, n))
+            picks = rng.integers(0, len(pool), n)
+            log(f"c5: {rate:g} req/s, {n} requests, served")
+            sv = _serve_feed(served, pool, picks, times, args.serve_seconds * 3)
+            log(f"c5: {rate:g} req/s served {sv}")
+            pl = _serve_feed(plain, pool, picks, times, args.serve_seconds * 3)
+            log(f"c5: {rate:g} req/s plain {pl}")
+            out.append(dict(rate=rate, served=sv, plain=pl,
+                            speedup_mvis=round(sv["mvis_s"] / max(pl["mvis_s"], 1e-9), 3)))
+    finally:
+        served.close()
+        plain.close()
     res = dict(pool=len(pool), seconds=args.serve_seconds, kernel_sets=len(cache),
                cache_hits=cache.hits, cache_misses=cache.misses, rates=out)
     cache.clear()
@@ -315,7 +349,9 @@ def main():
     ap.add_argument("--config", default="c2", choices=["c2", "c1", "c4", "c5"])
     ap.add_argument("--rates", default="50,200,800", help="config 5: arrival rates (req/s)")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="config 5: arrival window per rate")
-    ap.add_argument("--serve-budget-gb", type=float, default=200.0, help="config 5: HBM admission budget")
+    ap.add_argument("--serve-budget-gb", type=float, default=160.0,
+                    help="config 5: HBM admission budget (the rest holds the inputs, the library's "
+                         "scratch and allocator slack)")
     ap.add_argument("--jobs", type=int, default=512, help="config 4: number of snapshot jobs")
     ap.add_argument("--c4-budget-gb", type=float, default=64.0, help="config 4: HBM budget of a batched wave")
     ap.add_argument("--seed", type=int, default=2)
@@ -327,8 +363,8 @@ def main():
                          "(a kernel of known HBM traffic for calibrating FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              "(channels/jobs would shard one per GPU, see DESIGN.md).", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(channels and jobs would shard one per GPU).", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -430,7 +466,8 @@ def main():
     rec_bytes = gridded * (9.0 * P + (4.0 * P + 24.0) / nchan)
     roofline = dict(bound="valu", kernel="k_grid_tiles", achieved=round(achieved, 3),
                     peak=FP64_PEAK_TFLOPS, unit="TFLOP/s", frac=round(achieved / FP64_PEAK_TFLOPS, 4),
-                    traffic=None, flops_per_launch=flops, avg_launch_ms=round(grid_avg_s * 1e3, 3),
+                    traffic=measured_traffic(args.config), flops_per_launch=flops,
+                    avg_launch_ms=round(grid_avg_s * 1e3, 3),
                     algorithmic_GBps=round(rec_bytes / grid_avg_s / 1e9, 1))
 
     cpu = None
@@ -477,6 +514,9 @@ def main():
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                        "p99": round(float(np.percentile(lat, 99)), 3)},
         "stages_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in stages.items()},
+        # binning + record sort + plan + gridder only (the reference's gridding-rate scope)
+        "gridding_stage_mvis_s": round(gridded / (sum(stages[k][0] for k in ("bin", "sort_records", "plan", "grid"))
+                                                  / args.steps / 1e3) / 1e6, 3),
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

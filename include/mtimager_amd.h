@@ -229,6 +229,15 @@ typedef struct {
     void   *grids;               /* complex128 [nchan] grids, opts.grid_layout */
     double *norm;                /* f64 [nchan][npol]                          */
     float  *image;               /* f32 [nchan][npol][ny][nx] (finalize)       */
+    /* Optional: the antenna columns of vis's rows (int32 [vis.nrows], device)
+     * and the antenna count.  When ant1 is set, mti_grid_jobs puts the rows in
+     * baseline-major order itself, exactly as mti_sort_rows does
+     * (vis/SortIndex.cpp:46-87, autocorrelations dropped), keeping the kept
+     * count on the device: no host synchronisation per job.  vis.row_index
+     * must then be null.  Null ant1: vis.row_index (or MS order) is used. */
+    const int32_t *ant1;
+    const int32_t *ant2;
+    int32_t nant;
 } mti_job;
 
 /* Grids every job (opts: accumulate, psf, weighting, robust, grid_layout;

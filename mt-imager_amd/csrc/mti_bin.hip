@@ -1,4 +1,4 @@
-// mti_bin.hip -- binning of visibility records (row a4).
+// mti_bin.hip -- binning of visibility records.
 //
 // Restates op/WProjPreProcessUVW.cu:111-286 in FP64 (built with
 // -ffp-contract=off so every operation rounds exactly as the FP64 oracle's):
@@ -112,92 +112,117 @@ __global__ void k_bin_only(VisView v, BinGeom g, int chan, int32_t *out)
     o[7] = b.conj;
 }
 
-// Rows and channels of one binning block: a thread owns one row of the
-// job's index and walks a run of BIN_CHANS channels.
-constexpr int BIN_ROWS = 256;
-constexpr int BIN_CHANS = 64;
+// One binning block: BT_ROWS index entries of one job, all its channels in
+// chunks of BT_CHANS, 256 threads.  The rows' (u, v, w) and weights are staged
+// once; per chunk, phase 1 stages the flags and visibilities in LDS with loads
+// that run along each row (MS order: correlations, then channels,
+// contiguous), and phase 2 bins the records with the 64 lanes of a wave on 64
+// consecutive index entries of one channel, so every store of descriptors,
+// visibilities and weights is one contiguous run per channel (records are
+// numbered e = rec0 + c_local * nidx + i).
+constexpr int BT_THREADS = 256;
+constexpr int BT_ROWS = 64;
+constexpr int BT_CHANS = 16;
 
-// One row of a job over channels [c0, c1): the row's (u, v, w) and weights
-// are loaded once; record e = rec0 + c_local * nidx + i is written
-// channel-major, so the lanes of a wave (consecutive i) store contiguous
-// descriptors, visibilities and weights for each channel.
 template <int P>
-__device__ __forceinline__ void bin_row(const BinPassArgs &a, const JobDev &jd, int job, int64_t i, int c0, int c1,
-                                        unsigned long long &ngrid, unsigned long long &npts)
-{
-    const VisView &v = jd.v;
-    const int64_t row = v.row_index ? (int64_t)v.row_index[i] : i;
-    const double u = load_uvw(v, row, 0), vv = load_uvw(v, row, 1), w = load_uvw(v, row, 2);
-    float wgt[P];
-#pragma unroll
-    for (int p = 0; p < P; p++) wgt[p] = v.weight[row * v.w_rs + p];
-    for (int c = c0; c < c1; c++) {
-        const int chan = jd.chan0 + c;
-        const int64_t e = jd.rec0 + (int64_t)c * v.nidx + i;
-        bool fl[P];
-        bool any = false;
-#pragma unroll
-        for (int p = 0; p < P; p++) {
-            fl[p] = vis_flag(v, row, chan, p);
-            any |= !fl[p];
-        }
-        int4 d = make_int4((int)row, 0, 0, 0);
-        if (any) {
-            const BinOut b = bin_core(u, vv, w, load_invl(v, row, chan), jd.g);
-            if (b.to_grid) {
-                d.y = (b.cy << 16) | b.cx;
-                d.z = b.conv_idx;
-                d.w = (b.plane << 16) | (b.conj << 15) | b.S;
-                ngrid += 1;
-                npts += (unsigned long long)b.S * b.S * P;
-                const float *src = v.vis + 2 * (row * v.vis_rs + (int64_t)chan * v.vis_cs);
-#pragma unroll
-                for (int p = 0; p < P; p++) {
-                    float2 x = make_float2(1.0f, 0.0f);
-                    if (!jd.psf) x = *(const float2 *)(src + 2 * p);
-                    if (b.conj) x.y = -x.y;
-                    a.wv[e * P + p] = x;
-                    a.wt[e * P + p] = fl[p] ? 0.0f : wgt[p];
-                }
-            }
-        }
-        a.cjc[e] = (job << 16) | c;
-        a.desc[e] = d;
-    }
-}
+struct BinTile {
+    float2 vis[BT_CHANS][BT_ROWS][P];
+    uint8_t flag[BT_CHANS][BT_ROWS][P];
+    double uvw[BT_ROWS][3];
+    float wgt[BT_ROWS][P];
+    int row[BT_ROWS];
+    unsigned long long st[BT_THREADS / 64][2];
+};
 
-// Blocks are assigned to jobs by the block prefix js.blk0 (so the job is
-// uniform in a block); a job's blocks cover its rows in runs of BIN_ROWS
-// (fastest) and its channels in runs of BIN_CHANS.  Gridded records also get
-// their visibilities (PSF: 1+0i, imaginary part negated for w < 0) and
-// flag-masked weights.  Counts are reduced per block and added once.
 template <int P>
-__global__ __launch_bounds__(BIN_ROWS) void k_bin_pass(BinPassArgs a)
+__global__ __launch_bounds__(BT_THREADS) void k_bin_pass(BinPassArgs a)
 {
-    __shared__ unsigned long long s_st[BIN_ROWS / 64][2];
+    __shared__ BinTile<P> t;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int job = job_of(a.js.blk0, a.js.njobs, (int64_t)blockIdx.x);
     const JobDev jd = a.js.jobs[job];  // uniform: scalar loads into registers
-    const int64_t lb = (int64_t)blockIdx.x - a.js.blk0[job];
-    const int64_t nrb = (jd.v.nidx + BIN_ROWS - 1) / BIN_ROWS;
-    const int64_t rb = lb % nrb;
-    const int c0 = (int)(lb / nrb) * BIN_CHANS;
-    const int c1 = min(jd.nb, c0 + BIN_CHANS);
-    const int64_t i = rb * BIN_ROWS + threadIdx.x;
+    const VisView &v = jd.v;
+    const int64_t i0 = ((int64_t)blockIdx.x - a.js.blk0[job]) * BT_ROWS;
+    const int64_t nvalid = v.nkept ? (int64_t)*v.nkept : v.nidx;
+    const int nr = (int)max<int64_t>(0, min<int64_t>(BT_ROWS, nvalid - i0));  // rows of the tile to bin
+    const int64_t i = i0 + lane;
+
+    if (tid < nr) {
+        const int64_t row = v.row_index ? (int64_t)v.row_index[i0 + tid] : i0 + tid;
+        t.row[tid] = (int)row;
+        t.uvw[tid][0] = load_uvw(v, row, 0);
+        t.uvw[tid][1] = load_uvw(v, row, 1);
+        t.uvw[tid][2] = load_uvw(v, row, 2);
+#pragma unroll
+        for (int p = 0; p < P; p++) t.wgt[tid][p] = v.weight[row * v.w_rs + p];
+    }
     unsigned long long ngrid = 0, npts = 0;
-    if (i < jd.v.nidx) bin_row<P>(a, jd, job, i, c0, c1, ngrid, npts);
+    for (int c0 = 0; c0 < jd.nb; c0 += BT_CHANS) {
+        const int nc = min(jd.nb - c0, BT_CHANS);
+        __syncthreads();  // row data staged / previous chunk consumed
+        // phase 1: the chunk's flags and visibilities, along each row
+        constexpr int PER_ROW = BT_CHANS * P;
+        for (int q = tid; q < nr * PER_ROW; q += BT_THREADS) {
+            const int r = q / PER_ROW, rem = q - r * PER_ROW;
+            const int c = rem / P, p = rem - c * P;
+            if (c >= nc) continue;
+            const int64_t row = t.row[r];
+            const int chan = jd.chan0 + c0 + c;
+            t.flag[c][r][p] = vis_flag(v, row, chan, p) ? 1 : 0;
+            if (!jd.psf)
+                t.vis[c][r][p] = *(const float2 *)(v.vis + 2 * (row * v.vis_rs + (int64_t)chan * v.vis_cs + p));
+        }
+        __syncthreads();
+        // phase 2: wave w takes channels w, w + 4, ...; lane = index entry
+        if (i < v.nidx) {
+            for (int c = wave; c < nc; c += BT_THREADS / 64) {
+                const int cl = c0 + c;
+                const int64_t e = jd.rec0 + (int64_t)cl * v.nidx + i;
+                int4 d = make_int4(0, 0, 0, 0);
+                if (lane < nr) {
+                    d.x = t.row[lane];
+                    bool any = false;
+#pragma unroll
+                    for (int p = 0; p < P; p++) any |= !t.flag[c][lane][p];
+                    if (any) {
+                        const BinOut b = bin_core(t.uvw[lane][0], t.uvw[lane][1], t.uvw[lane][2],
+                                                  load_invl(v, (int64_t)t.row[lane], jd.chan0 + cl), jd.g);
+                        if (b.to_grid) {
+                            d.y = (b.cy << 16) | b.cx;
+                            d.z = b.conv_idx;
+                            d.w = (b.plane << 16) | (b.conj << 15) | b.S;
+                            ngrid += 1;
+                            npts += (unsigned long long)b.S * b.S * P;
+#pragma unroll
+                            for (int p = 0; p < P; p++) {
+                                float2 x = jd.psf ? make_float2(1.0f, 0.0f) : t.vis[c][lane][p];
+                                if (b.conj) x.y = -x.y;
+                                a.wv[e * P + p] = x;
+                                a.wt[e * P + p] = t.flag[c][lane][p] ? 0.0f : t.wgt[lane][p];
+                            }
+                        }
+                    }
+                }
+                // entries behind the kept rows (device row order) are not gridded
+                a.cjc[e] = (job << 16) | cl;
+                a.desc[e] = d;
+            }
+        }
+    }
     for (int o = 32; o >= 1; o >>= 1) {
         ngrid += __shfl_xor(ngrid, o, 64);
         npts += __shfl_xor(npts, o, 64);
     }
-    if ((threadIdx.x & 63) == 0) {
-        s_st[threadIdx.x >> 6][0] = ngrid;
-        s_st[threadIdx.x >> 6][1] = npts;
+    if (lane == 0) {
+        t.st[wave][0] = ngrid;
+        t.st[wave][1] = npts;
     }
     __syncthreads();
-    if (threadIdx.x < 2) {
-        unsigned long long t = 0;
-        for (int k = 0; k < BIN_ROWS / 64; k++) t += s_st[k][threadIdx.x];
-        if (t) atomicAdd(&a.stats[threadIdx.x], t);
+    if (tid < 2) {
+        unsigned long long s = 0;
+        for (int k = 0; k < BT_THREADS / 64; k++) s += t.st[k][tid];
+        // striped counters: blocks spread their adds over BIN_STAT_STRIPES addresses
+        if (s) atomicAdd(&a.stats[tid * BIN_STAT_STRIPES + blockIdx.x % BIN_STAT_STRIPES], s);
     }
 }
 
@@ -223,6 +248,7 @@ VisView make_view(const mti_vis *v)
     w.row_index = v->row_index;
     w.nidx = v->row_index ? v->nindex : v->nrows;
     w.npol = v->npol;
+    w.nkept = nullptr;
     return w;
 }
 
@@ -249,9 +275,9 @@ mti_status launch_bin_pass(mti_ctx *ctx, const BinPassArgs &a, int64_t nblocks, 
     if (nblocks <= 0) return MTI_OK;
     MTI_CHECK(ctx, nblocks < (int64_t(1) << 31), "binning grid above 2^31 blocks");
     switch (a.js.P) {
-    case 1: hipLaunchKernelGGL(k_bin_pass<1>, dim3((unsigned)nblocks), dim3(BIN_ROWS), 0, st, a); break;
-    case 2: hipLaunchKernelGGL(k_bin_pass<2>, dim3((unsigned)nblocks), dim3(BIN_ROWS), 0, st, a); break;
-    case 4: hipLaunchKernelGGL(k_bin_pass<4>, dim3((unsigned)nblocks), dim3(BIN_ROWS), 0, st, a); break;
+    case 1: hipLaunchKernelGGL(k_bin_pass<1>, dim3((unsigned)nblocks), dim3(BT_THREADS), 0, st, a); break;
+    case 2: hipLaunchKernelGGL(k_bin_pass<2>, dim3((unsigned)nblocks), dim3(BT_THREADS), 0, st, a); break;
+    case 4: hipLaunchKernelGGL(k_bin_pass<4>, dim3((unsigned)nblocks), dim3(BT_THREADS), 0, st, a); break;
     default: return set_error(ctx, MTI_ERR_INVALID, "npol must be 1, 2 or 4");
     }
     MTI_HIP(ctx, hipGetLastError());
@@ -261,7 +287,8 @@ mti_status launch_bin_pass(mti_ctx *ctx, const BinPassArgs &a, int64_t nblocks, 
 // Binning blocks of a job with nidx rows and nb channels.
 int64_t bin_blocks(int64_t nidx, int nb)
 {
-    return ceil_div(nidx, (int64_t)BIN_ROWS) * ceil_div((int64_t)nb, (int64_t)BIN_CHANS);
+    (void)nb;
+    return ceil_div(nidx, (int64_t)BT_ROWS);
 }
 
 }  // namespace mti

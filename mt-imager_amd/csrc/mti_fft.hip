@@ -1,4 +1,4 @@
-// mti_fft.hip -- 2-D inverse FFT and the image finalizer (rows a16-a19).
+// mti_fft.hip -- 2-D inverse FFT and the image finalizer.
 //
 // Reference (comp/ImageFinalizer.cpp:31-44): PreIFFTUVCorrection
 // ([v][u][P] -> [P][v][u] plus ifftshift, op/PreIFFTUVCorrection.cu:28-90),
@@ -9,19 +9,26 @@
 // (op/CasaGridSF.cu:174-199) and SincCorrector (op/SincCorrector.cu:29-58),
 // the last two evaluated on the flipped array exactly as the reference does.
 //
-// MI355X design, three HBM passes per channel:
-//   A  row transforms in LDS (radix-4/2 Stockham), fused with the uv-plane
-//      reorder/ifftshift;                          grid   -> X[p][v][u]
-//   B1 column transforms, four-step first half (length N1 over rows strided
-//      by N2, times the inter-step twiddle), in place;  tiles of 16 columns
-//      so every row segment read is 128 contiguous bytes;
-//   B2 four-step second half (length N2 over N2 contiguous rows) fused with
-//      fftshift + normalise + Stokes + flip + taper and sinc correction,
-//      all P correlations of a pixel in one workgroup  -> f32 image.
-// Twiddles come from an FP64-computed table exp(+2 pi i t / NT).
+// MI355X design.  Every Stokes image is real, so two of them share one
+// complex FP64 transform ("packed planes": the Hermitian parts of two Stokes
+// combinations of correlation planes, one as the real and one as the
+// imaginary part); a full-polarisation channel takes two transforms, and two
+// Stokes-I channels share one.  Per packed plane:
+//   A  row transforms on 4096/8192-element LDS blocks (radix-16 Stockham),
+//      rows y and -y together, fused with the uv-plane reorder/ifftshift,
+//      normalisation and Stokes coefficients;     grid -> X[pack][v][u]
+//   B  columns whole in one block when ny <= 256, else the four-step split
+//      B1 (length N1 over rows strided by N2, times the inter-step twiddle,
+//      in place) and B2 (length N2 over N2 contiguous rows); the last pass is
+//      fused with fftshift + m flip + taper and sinc correction and writes
+//      the real and imaginary parts as two f32 images.
+// Images below 16 pixels on a side take the unpacked per-correlation passes
+// (k_fft_rows, k_fft_cols1, k_fft_cols2).  Twiddles come from an
+// FP64-computed table exp(+2 pi i t / NT).
 #include "mti_internal.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 
 namespace mti {
@@ -331,14 +338,22 @@ constexpr int FB_E = 4096;
 constexpr int FB_THREADS = 256;
 constexpr int FB_LDS = FB_E + FB_E / 16;
 
-// Per-channel inputs and outputs of the block-transform finalize (one entry
-// per channel of every job of a call with the same image size).
-struct ChanDesc {
-    const void *grid;       // the channel's grid (layout strides cs, ps)
-    const double *norm;     // [P]
-    float *image;           // [P][ny][nx]
-    const float *corr_x;    // [nx] sinc / taper correction
-    const float *corr_y;    // [ny]
+// Packed Stokes planes.  Every output image is real (the Stokes formulas take
+// real parts), and Re IDFT(S) = IDFT(H) with H(y, x) = (S(y, x) +
+// conj(S(-y, -x))) / 2, the Hermitian part of S; two Hermitian planes then
+// share one complex transform, IDFT(H0 + i H1) = IDFT(H0) + i IDFT(H1), whose
+// real and imaginary parts are the two images.  A packed plane is built in
+// the row pass from two correlation planes A and B of one channel (or, for
+// one correlation, the planes of two channels): S_s = ca[s] A / na +
+// cb[s] B / nb for s = 0, 1.  A full-polarisation channel takes two
+// transforms instead of four; two Stokes-I channels share one.
+struct PackDesc {
+    const void *a, *b;        // element (0, 0) of planes A and B (b null: none)
+    const double *na, *nb;    // their normalisers (device)
+    double2 ca[2], cb[2];     // Stokes coefficients of s = 0 (real part) and 1 (imaginary part)
+    float *img[2];            // image planes [ny][nx] of s = 0, 1 (img[1] null: none)
+    const float *corr_x;      // [nx] sinc / taper correction
+    const float *corr_y;      // [ny]
 };
 
 __device__ __forceinline__ int fb_pad(int i) { return i + (i >> 4); }
@@ -435,35 +450,35 @@ __device__ __forceinline__ void idft(double2 (&v)[R])
 // LDS index of element n of transform f: row layout f*N + n (transforms of a
 // row block, lanes along n) or column layout n*NF + f (column blocks, lanes
 // along f so that HBM rows are read and written contiguously).
-template <int N, bool COL>
+template <int E, int N, bool COL>
 __device__ __forceinline__ int fb_idx(int f, int n)
 {
-    return COL ? n * (FB_E / N) + f : fb_pad(f * N + n);
+    return COL ? n * (E / N) + f : fb_pad(f * N + n);
 }
 
 // One Stockham pass of radix R at stride NS over the block.  load(f, n) gives
 // element n of transform f (FIRST); store(f, k, v) takes output k (LAST).
-template <int N, bool COL, int R, int NS, bool FIRST, bool LAST, class Load, class Store>
+template <int E, int TH, int N, bool COL, int R, int NS, bool FIRST, bool LAST, class Load, class Store>
 __device__ __forceinline__ void fb_pass(double2 *lds, const double2 *tw, int NT, const Load &load,
                                         const Store &store)
 {
     constexpr int PER = N / R;
-    constexpr int NF = FB_E / N;
-    constexpr int B = FB_E / (R * FB_THREADS);
+    constexpr int NF = E / N;
+    constexpr int B = E / (R * TH);
     static_assert(B >= 1, "block too small for the radix");
     double2 v[B][R];
 #pragma unroll
     for (int s = 0; s < B; s++) {
-        const int b = threadIdx.x + FB_THREADS * s;
+        const int b = threadIdx.x + TH * s;
         const int f = COL ? b % NF : b / PER, j = COL ? b / NF : b % PER;
 #pragma unroll
         for (int r = 0; r < R; r++)
-            v[s][r] = FIRST ? load(f, j + PER * r) : lds[fb_idx<N, COL>(f, j + PER * r)];
+            v[s][r] = FIRST ? load(f, j + PER * r) : lds[fb_idx<E, N, COL>(f, j + PER * r)];
     }
     if (!FIRST) __syncthreads();
 #pragma unroll
     for (int s = 0; s < B; s++) {
-        const int b = threadIdx.x + FB_THREADS * s;
+        const int b = threadIdx.x + TH * s;
         const int f = COL ? b % NF : b / PER, j = COL ? b / NF : b % PER;
         const int k = j % NS;
         if (NS > 1) {
@@ -476,54 +491,84 @@ __device__ __forceinline__ void fb_pass(double2 *lds, const double2 *tw, int NT,
 #pragma unroll
         for (int r = 0; r < R; r++) {
             if (LAST) store(f, base + NS * r, v[s][r]);
-            else lds[fb_idx<N, COL>(f, base + NS * r)] = v[s][r];
+            else lds[fb_idx<E, N, COL>(f, base + NS * r)] = v[s][r];
         }
     }
     if (!LAST) __syncthreads();
 }
 
-template <int N, bool COL, int NS, bool FIRST, class Load, class Store>
+template <int E, int TH, int N, bool COL, int NS, bool FIRST, class Load, class Store>
 __device__ __forceinline__ void fb_passes(double2 *lds, const double2 *tw, int NT, const Load &load,
                                           const Store &store)
 {
     constexpr int REM = N / NS;
     constexpr int R = REM >= 16 ? 16 : REM;
-    fb_pass<N, COL, R, NS, FIRST, REM == R>(lds, tw, NT, load, store);
-    if constexpr (REM > R) fb_passes<N, COL, NS * R, false>(lds, tw, NT, load, store);
+    fb_pass<E, TH, N, COL, R, NS, FIRST, REM == R>(lds, tw, NT, load, store);
+    if constexpr (REM > R) fb_passes<E, TH, N, COL, NS * R, false>(lds, tw, NT, load, store);
 }
 
-// Full inverse transforms of the block: load -> passes -> store.
-template <int N, bool COL, class Load, class Store>
+// Full inverse transforms of a block of E elements (E / N transforms of
+// length N) on TH threads: load -> passes -> store.
+template <int N, bool COL, int E = FB_E, int TH = FB_THREADS, class Load, class Store>
 __device__ __forceinline__ void fb_ifft(double2 *lds, const double2 *tw, int NT, const Load &load,
                                         const Store &store)
 {
-    static_assert(N >= 16 && N <= FB_E, "length");
-    fb_passes<N, COL, 1, true>(lds, tw, NT, load, store);
+    static_assert(N >= 16 && N <= E, "length");
+    fb_passes<E, TH, N, COL, 1, true>(lds, tw, NT, load, store);
 }
 
-// Pass A: rows.  Block = NF = 4096/N rows (y0 .. y0+NF) of plane (c, p) of
-// the channel grid cd[c].grid (element (y, gx, p) at (y*nx + gx)*cs + p*ps),
-// with the PreIFFTUVCorrection reorder (ix = (gx - nx/2) mod nx,
-// iy = (y - ny/2) mod ny) into X[c][p][iy][kx].
+// Row pass of packed planes.  Block size E = max(4096, 2N) elements (256 or
+// 512 threads); a block transforms E / (2N) pairs of rows (y, -y mod ny) of
+// packed plane blockIdx.y, built on the fly from the grid planes with the
+// PreIFFTUVCorrection reorder (transform row y, column x holds grid cell
+// ((y + ny/2) mod ny, (x + N/2) mod N)); the pair makes every conj(S(-y, -x))
+// a read within the block.  Rows 0 and ny/2 pair with themselves.
+constexpr int pk_e(int N) { return 2 * N > FB_E ? 2 * N : FB_E; }
+
 template <int N, typename TI>
-__global__ __launch_bounds__(FB_THREADS) void k_rows_fb(const ChanDesc *cd, double2 *X, int ny, int P,
-                                                       const double2 *tw, int NT, int64_t cs, int64_t ps)
+__global__ __launch_bounds__(pk_e(N) / 16) void k_rows_pk(const PackDesc *pd, double2 *X, int ny, int64_t cs,
+                                                         const double2 *tw, int NT)
 {
-    __shared__ double2 lds[FB_LDS];
-    constexpr int NF = FB_E / N;
-    const int y0 = blockIdx.x * NF, p = blockIdx.y, c = blockIdx.z;
-    const TI *src = (const TI *)cd[c].grid + (size_t)p * ps;
-    double2 *dst = X + ((size_t)c * P + p) * N * ny;
-    auto load = [&](int f, int ix) {
-        const int gx = (ix + N / 2) & (N - 1);
-        const TI v = src[((size_t)(y0 + f) * N + gx) * cs];
+    constexpr int E = pk_e(N), TH = E / 16, PB = E / (2 * N);
+    __shared__ double2 lds[E + E / 16];
+    const PackDesc &d = pd[blockIdx.y];
+    const TI *A = (const TI *)d.a, *B = (const TI *)d.b;
+    const double ia = 1.0 / *d.na, ib = B ? 1.0 / *d.nb : 0.0;
+    const double2 ka0 = make_double2(d.ca[0].x * ia, d.ca[0].y * ia), ka1 = make_double2(d.ca[1].x * ia, d.ca[1].y * ia);
+    const double2 kb0 = make_double2(d.cb[0].x * ib, d.cb[0].y * ib), kb1 = make_double2(d.cb[1].x * ib, d.cb[1].y * ib);
+    double2 *dst = X + (size_t)blockIdx.y * N * ny;
+    const int npairs = ny / 2 + 1;
+    auto elem = [&](const TI *P0, int y, int x) {
+        const TI v = P0[((size_t)((y + (ny >> 1)) & (ny - 1)) * N + ((x + N / 2) & (N - 1))) * cs];
         return make_double2((double)v.x, (double)v.y);
     };
-    auto store = [&](int f, int kx, double2 v) {
-        const int iy = (y0 + f - (ny >> 1)) & (ny - 1);
-        dst[(size_t)iy * N + kx] = v;
+    auto load = [&](int f, int x) {
+        const int pair = blockIdx.x * PB + (f >> 1);
+        if (pair >= npairs) return make_double2(0.0, 0.0);
+        const int y = (f & 1) ? ((ny - pair) & (ny - 1)) : pair;
+        const int ym = (ny - y) & (ny - 1), xm = (N - x) & (N - 1);
+        const double2 a0 = elem(A, y, x), a1 = elem(A, ym, xm);
+        double2 b0 = make_double2(0.0, 0.0), b1 = b0;
+        if (B) {
+            b0 = elem(B, y, x);
+            b1 = elem(B, ym, xm);
+        }
+        // S_s at (y, x) and at (-y, -x)
+        const double2 s0 = c2add(c2mul(ka0, a0), c2mul(kb0, b0)), s0m = c2add(c2mul(ka0, a1), c2mul(kb0, b1));
+        const double2 s1 = c2add(c2mul(ka1, a0), c2mul(kb1, b0)), s1m = c2add(c2mul(ka1, a1), c2mul(kb1, b1));
+        // H_s = (S_s + conj(S_s(-y,-x))) / 2;  Z = H_0 + i H_1
+        const double h0r = 0.5 * (s0.x + s0m.x), h0i = 0.5 * (s0.y - s0m.y);
+        const double h1r = 0.5 * (s1.x + s1m.x), h1i = 0.5 * (s1.y - s1m.y);
+        return make_double2(h0r - h1i, h0i + h1r);
     };
-    fb_ifft<N, false>(lds, tw, NT, load, store);
+    auto store = [&](int f, int kx, double2 v) {
+        const int pair = blockIdx.x * PB + (f >> 1);
+        if (pair >= npairs) return;
+        const int y = (f & 1) ? ((ny - pair) & (ny - 1)) : pair;
+        if ((f & 1) && y == pair) return;  // a self-paired row is written once
+        dst[(size_t)y * N + kx] = v;
+    };
+    fb_ifft<N, false, E, TH>(lds, tw, NT, load, store);
 }
 
 // Pass B1: length-N1 transforms down the columns (rows N2*n1 + n2) of NF =
@@ -545,87 +590,96 @@ __global__ __launch_bounds__(FB_THREADS) void k_cols1_fb(double2 *X, int nx, int
     fb_ifft<N1, true>(lds, tw, NT, load, store);
 }
 
-// Pass B2 + finalize: channel c, index k1, columns [x0, x0 + COLS) of all P
-// planes: rows N2*k1 + n2 -> length-N2 transforms -> ky = k1 + N1*k2, then
-// fftshift, normalise, Stokes, m flip and both corrections -> f32 image.
-template <int N2, int P>
-__global__ __launch_bounds__(FB_THREADS) void k_cols2_fb(const double2 *X, int nx, int ny, int N1,
-                                                        const double2 *tw, int NT, const ChanDesc *cd, int poltype)
+// Finalize of packed column transforms held in LDS: output k of column
+// x0 + cx at lds[k * NF + cx] is image row ky = ky_of(k); fftshift, the m flip
+// and both corrections; the real part is image s = 0, the imaginary part s = 1.
+template <int NF, int TH, class KyOf>
+__device__ __forceinline__ void pack_store(const double2 *lds, int nk, int x0, int nx, int ny, const PackDesc &d,
+                                           const KyOf &ky_of)
 {
-    __shared__ double2 lds[FB_LDS];
-    constexpr int NF = FB_E / N2;  // P * COLS transforms
-    constexpr int COLS = NF / P;
-    static_assert(COLS >= 1, "columns per block");
-    const int x0 = blockIdx.x * COLS, k1 = blockIdx.y, c = blockIdx.z;
-    auto load = [&](int f, int n2) {
-        const int p = f / COLS, cx = f - p * COLS;
-        return X[(((size_t)c * P + p) * ny + (size_t)N2 * k1 + n2) * nx + x0 + cx];
-    };
-    auto store = [&](int f, int k2, double2 v) { lds[k2 * NF + f] = v; };
-    fb_ifft<N2, true>(lds, tw, NT, load, store);
-    __syncthreads();
-    const double *norm = cd[c].norm;
-    const float *corr_x = cd[c].corr_x, *corr_y = cd[c].corr_y;
-    double inv[P];
-#pragma unroll
-    for (int p = 0; p < P; p++) inv[p] = 1.0 / norm[p];
-    float *img = cd[c].image;
-    for (int i = threadIdx.x; i < N2 * COLS; i += FB_THREADS) {
-        const int k2 = i / COLS, cx = i - k2 * COLS;
-        const int ky = k1 + N1 * k2, kx = x0 + cx;
+    float *i0 = d.img[0], *i1 = d.img[1];
+    for (int i = threadIdx.x; i < nk * NF; i += TH) {
+        const int k = i / NF, cx = i - k * NF;
+        const int ky = ky_of(k), kx = x0 + cx;
         const int jy = (ky + (ny >> 1)) & (ny - 1), jx = (kx + (nx >> 1)) & (nx - 1);
         const int row = ny - 1 - jy;  // m axis flipped for FITS
-        double re[P], im[P];
-#pragma unroll
-        for (int p = 0; p < P; p++) {
-            const double2 v = lds[k2 * NF + p * COLS + cx];
-            re[p] = v.x * inv[p];
-            im[p] = v.y * inv[p];
-        }
-        double o[P];
-        if constexpr (P == 1) {
-            o[0] = re[0];
-        } else if constexpr (P == 2) {
-            o[0] = (re[0] + re[1]) * 0.5;
-            o[1] = (re[0] - re[1]) * 0.5;
-        } else {
-            if (poltype == MTI_POL_LINEAR) {
-                o[0] = (re[0] + re[3]) * 0.5;
-                o[1] = (re[0] - re[3]) * 0.5;
-                o[2] = (re[1] + re[2]) * 0.5;
-                o[3] = (im[1] - im[2]) * 0.5;
-            } else {
-                o[0] = (re[0] + re[3]) * 0.5;
-                o[1] = (re[1] + re[2]) * 0.5;
-                o[2] = (im[2] - im[1]) * 0.5;
-                o[3] = (re[0] - re[3]) * 0.5;
-            }
-        }
-        const double cf = (double)corr_x[jx] * (double)corr_y[row];
-#pragma unroll
-        for (int q = 0; q < P; q++) img[((size_t)q * ny + row) * nx + jx] = (float)(o[q] * cf);
+        const double2 v = lds[k * NF + cx];
+        const double cf = (double)d.corr_x[jx] * (double)d.corr_y[row];
+        const size_t o = (size_t)row * nx + jx;
+        i0[o] = (float)(v.x * cf);
+        if (i1) i1[o] = (float)(v.y * cf);
     }
 }
 
-// Host dispatch of the block-transform path (nx, N1, N2 >= 16 and the blocks
-// fit the image).
-static bool fb_ok(const FftGeom &g, int P)
+// Pass B2 + finalize: packed plane blockIdx.z, index k1, NF = 4096 / N2
+// columns from x0: rows N2*k1 + n2 -> length-N2 transforms -> ky = k1 + N1*k2.
+template <int N2>
+__global__ __launch_bounds__(FB_THREADS) void k_cols2_pk(const double2 *X, int nx, int ny, int N1,
+                                                        const double2 *tw, int NT, const PackDesc *pd)
 {
-    return g.nx >= 16 && g.N1 >= 16 && g.N2 >= 16 && FB_E / g.nx <= g.ny && FB_E / g.N1 <= g.nx &&
-           FB_E / (g.N2 * P) <= g.nx && FB_E / (g.N2 * P) >= 1;
+    __shared__ double2 lds[FB_LDS];
+    constexpr int NF = FB_E / N2;
+    const int x0 = blockIdx.x * NF, k1 = blockIdx.y;
+    const double2 *plane = X + (size_t)blockIdx.z * nx * ny;
+    auto load = [&](int f, int n2) { return plane[((size_t)N2 * k1 + n2) * nx + x0 + f]; };
+    auto store = [&](int f, int k2, double2 v) { lds[k2 * NF + f] = v; };
+    fb_ifft<N2, true>(lds, tw, NT, load, store);
+    __syncthreads();
+    pack_store<NF, FB_THREADS>(lds, N2, x0, nx, ny, pd[blockIdx.z], [&](int k2) { return k1 + N1 * k2; });
+}
+
+// Whole-column pass + finalize (ny <= FBC_E / 32): a block holds NF >= 32
+// columns of packed plane blockIdx.z (128 KB of LDS, rows read in runs of NF
+// elements) and transforms them completely, so the transform takes two HBM
+// passes instead of three.
+constexpr int FBC_E = 8192;
+constexpr int FBC_THREADS = 512;
+
+template <int N>
+__global__ __launch_bounds__(FBC_THREADS) void k_cols_pk(const double2 *X, int nx, int ny, const double2 *tw, int NT,
+                                                        const PackDesc *pd)
+{
+    __shared__ double2 lds[FBC_E];
+    constexpr int NF = FBC_E / N;
+    static_assert(NF >= 32, "columns per block");
+    const int x0 = blockIdx.x * NF;
+    const double2 *plane = X + (size_t)blockIdx.z * nx * ny;
+    auto load = [&](int f, int n) { return plane[(size_t)n * nx + x0 + f]; };
+    auto store = [&](int f, int k, double2 v) { lds[k * NF + f] = v; };
+    fb_ifft<N, true, FBC_E, FBC_THREADS>(lds, tw, NT, load, store);
+    __syncthreads();
+    pack_store<NF, FBC_THREADS>(lds, N, x0, nx, ny, pd[blockIdx.z], [](int k) { return k; });
+}
+
+// Host dispatch of the packed block-transform path: rows of 16-4096 elements;
+// columns whole when a block spans at least 32 of them (ny <= 256: image rows
+// written in runs of >= 128 bytes; narrower blocks write the f32 image in
+// pieces too small for HBM and lose to the two-pass split), else four-step
+// (N1, N2 >= 16, a 4096-element block of columns fits the image width).
+static bool fbc_ok(const FftGeom &g) { return g.ny >= 16 && g.ny * 32 <= FBC_E && g.nx >= FBC_E / g.ny; }
+
+static bool pk_ok(const FftGeom &g)
+{
+    return g.nx >= 16 && g.nx <= 4096 && g.ny >= 16 &&
+           (fbc_ok(g) || (g.N1 >= 16 && g.N2 >= 16 && FB_E / g.N1 <= g.nx && FB_E / g.N2 <= g.nx));
 }
 
 template <typename TI>
-static mti_status fb_rows(mti_ctx *ctx, const FftGeom &g, const ChanDesc *G, double2 *X, int P, int nch,
-                          const double2 *tw, int64_t cs, int64_t ps, hipStream_t st)
+static mti_status pk_rows(mti_ctx *ctx, const FftGeom &g, const PackDesc *pd, int npk, double2 *X, int64_t cs,
+                          const double2 *tw, hipStream_t st)
 {
-    const dim3 gr((unsigned)(g.ny / (FB_E / g.nx)), (unsigned)P, (unsigned)nch);
+    const int npairs = g.ny / 2 + 1;
     switch (g.nx) {
-#define MTI_ROWS(NN) \
-    case NN: hipLaunchKernelGGL((k_rows_fb<NN, TI>), gr, dim3(FB_THREADS), 0, st, G, X, g.ny, P, tw, g.NT, cs, ps); break;
-        MTI_ROWS(16) MTI_ROWS(32) MTI_ROWS(64) MTI_ROWS(128) MTI_ROWS(256) MTI_ROWS(512) MTI_ROWS(1024)
-        MTI_ROWS(2048) MTI_ROWS(4096)
-#undef MTI_ROWS
+#define MTI_PR(NN)                                                                                          \
+    case NN: {                                                                                              \
+        constexpr int PB = pk_e(NN) / (2 * NN);                                                             \
+        const dim3 gr((unsigned)ceil_div(npairs, PB), (unsigned)npk);                                       \
+        hipLaunchKernelGGL((k_rows_pk<NN, TI>), gr, dim3(pk_e(NN) / 16), 0, st, pd, X, g.ny, cs, tw, g.NT); \
+        break;                                                                                              \
+    }
+        MTI_PR(16) MTI_PR(32) MTI_PR(64) MTI_PR(128) MTI_PR(256) MTI_PR(512) MTI_PR(1024) MTI_PR(2048)
+        MTI_PR(4096)
+#undef MTI_PR
     default: return set_error(ctx, MTI_ERR_UNSUPPORTED, "row length");
     }
     MTI_HIP(ctx, hipGetLastError());
@@ -647,18 +701,31 @@ static mti_status fb_cols1(mti_ctx *ctx, const FftGeom &g, double2 *X, int plane
     return MTI_OK;
 }
 
-template <int P>
-static mti_status fb_cols2(mti_ctx *ctx, const FftGeom &g, const double2 *X, int nch, const double2 *tw,
-                           const ChanDesc *cd, int poltype, hipStream_t st)
+static mti_status pk_cols2(mti_ctx *ctx, const FftGeom &g, const double2 *X, int npk, const double2 *tw,
+                           const PackDesc *pd, hipStream_t st)
 {
-    const int cols = FB_E / (g.N2 * P);
-    const dim3 gr((unsigned)(g.nx / cols), (unsigned)g.N1, (unsigned)nch);
+    const dim3 gr((unsigned)(g.nx / (FB_E / g.N2)), (unsigned)g.N1, (unsigned)npk);
     switch (g.N2) {
 #define MTI_C2(NN) \
-    case NN: hipLaunchKernelGGL((k_cols2_fb<NN, P>), gr, dim3(FB_THREADS), 0, st, X, g.nx, g.ny, g.N1, tw, g.NT, cd, poltype); break;
+    case NN: hipLaunchKernelGGL((k_cols2_pk<NN>), gr, dim3(FB_THREADS), 0, st, X, g.nx, g.ny, g.N1, tw, g.NT, pd); break;
         MTI_C2(16) MTI_C2(32) MTI_C2(64)
 #undef MTI_C2
     default: return set_error(ctx, MTI_ERR_UNSUPPORTED, "column split");
+    }
+    MTI_HIP(ctx, hipGetLastError());
+    return MTI_OK;
+}
+
+static mti_status pk_cols(mti_ctx *ctx, const FftGeom &g, const double2 *X, int npk, const double2 *tw,
+                          const PackDesc *pd, hipStream_t st)
+{
+    const dim3 gr((unsigned)(g.nx / (FBC_E / g.ny)), 1u, (unsigned)npk);
+    switch (g.ny) {
+#define MTI_CF(NN) \
+    case NN: hipLaunchKernelGGL((k_cols_pk<NN>), gr, dim3(FBC_THREADS), 0, st, X, g.nx, g.ny, tw, g.NT, pd); break;
+        MTI_CF(16) MTI_CF(32) MTI_CF(64) MTI_CF(128) MTI_CF(256)
+#undef MTI_CF
+    default: return set_error(ctx, MTI_ERR_UNSUPPORTED, "column length");
     }
     MTI_HIP(ctx, hipGetLastError());
     return MTI_OK;
@@ -730,54 +797,138 @@ static mti_status finalize_batch(mti_ctx *ctx, const FftGeom &g, const TI *G, do
     return cols2<0, P, double2>(ctx, g, X, nb, tw, nr, poltype, cx, cy, img, nullptr, st);
 }
 
-// Block-transform finalize of nch channels described by cd (device).
-template <int P, typename TI>
-static mti_status finalize_blocks(mti_ctx *ctx, const FftGeom &g, const ChanDesc *cd, double2 *X, int nch,
-                                  const double2 *tw, int poltype, int layout, hipStream_t st)
+// One channel of a block-transform finalize.
+struct ChanRef {
+    const void *grid;   // the channel's grid (layout strides cs, ps)
+    const double *norm; // [P]
+    float *image;       // [P][ny][nx]
+    const float *corr_x, *corr_y;
+};
+
+// Packed planes of a list of channels of one image size (the Stokes formulas
+// of op/PostIFFTStokesConverter.cu:51-59, 73-80 as coefficients of the
+// correlation planes: linear I = Re(XX+YY)/2, Q = Re(XX-YY)/2,
+// U = Re(XY+YX)/2, V = Im(XY-YX)/2 = Re(-i(XY-YX))/2; circular
+// I = Re(RR+LL)/2, Q = Re(RL+LR)/2, U = Im(LR-RL)/2, V = Re(RR-LL)/2;
+// two correlations (a+b)/2, (a-b)/2; one correlation pairs channels).
+static void make_packs(const std::vector<ChanRef> &ch, int P, int poltype, int layout, int nx, int ny, size_t elem,
+                       std::vector<PackDesc> &out)
+{
+    const int64_t plane = (int64_t)nx * ny;
+    const int64_t ps = layout == MTI_GRID_VUP ? 1 : plane;  // complex elements between correlation planes
+    auto pl = [&](const ChanRef &c, int p) { return (const void *)((const char *)c.grid + (size_t)(p * ps) * elem); };
+    const double2 h = make_double2(0.5, 0.0), mh = make_double2(-0.5, 0.0);
+    const double2 ih = make_double2(0.0, 0.5), mih = make_double2(0.0, -0.5), one = make_double2(1.0, 0.0);
+    const double2 zero = make_double2(0.0, 0.0);
+    if (P == 1) {
+        // a pack has one pair of correction vectors: only channels that share
+        // them (same size and oversampling) are paired; the stable order keeps
+        // the channels of a job together
+        std::vector<size_t> ord(ch.size());
+        for (size_t i = 0; i < ch.size(); i++) ord[i] = i;
+        std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) {
+            return std::less<const float *>()(ch[x].corr_x, ch[y].corr_x);
+        });
+        for (size_t i = 0; i < ord.size();) {
+            const ChanRef &c0 = ch[ord[i]];
+            const bool two = i + 1 < ord.size() && ch[ord[i + 1]].corr_x == c0.corr_x &&
+                             ch[ord[i + 1]].corr_y == c0.corr_y;
+            const ChanRef &c1 = two ? ch[ord[i + 1]] : c0;
+            i += two ? 2 : 1;
+            PackDesc d{pl(c0, 0), two ? pl(c1, 0) : nullptr, c0.norm, two ? c1.norm : nullptr,
+                       {one, zero}, {zero, one}, {c0.image, two ? c1.image : nullptr}, c0.corr_x, c0.corr_y};
+            out.push_back(d);
+        }
+        return;
+    }
+    for (const ChanRef &c : ch) {
+        float *im[4] = {c.image, c.image + plane, c.image + 2 * plane, c.image + 3 * plane};
+        if (P == 2) {
+            out.push_back(PackDesc{pl(c, 0), pl(c, 1), c.norm, c.norm + 1, {h, h}, {h, mh}, {im[0], im[1]},
+                                   c.corr_x, c.corr_y});
+        } else if (poltype == MTI_POL_LINEAR) {
+            out.push_back(PackDesc{pl(c, 0), pl(c, 3), c.norm, c.norm + 3, {h, h}, {h, mh}, {im[0], im[1]},
+                                   c.corr_x, c.corr_y});  // I, Q from XX, YY
+            out.push_back(PackDesc{pl(c, 1), pl(c, 2), c.norm + 1, c.norm + 2, {h, mih}, {h, ih}, {im[2], im[3]},
+                                   c.corr_x, c.corr_y});  // U, V from XY, YX
+        } else {
+            out.push_back(PackDesc{pl(c, 0), pl(c, 3), c.norm, c.norm + 3, {h, h}, {h, mh}, {im[0], im[3]},
+                                   c.corr_x, c.corr_y});  // I, V from RR, LL
+            out.push_back(PackDesc{pl(c, 1), pl(c, 2), c.norm + 1, c.norm + 2, {h, ih}, {h, mih}, {im[1], im[2]},
+                                   c.corr_x, c.corr_y});  // Q, U from RL, LR
+        }
+    }
+}
+
+// Transform + finalize of npk packed planes (device descriptors pd) into X.
+template <typename TI>
+static mti_status finalize_packs(mti_ctx *ctx, const FftGeom &g, const PackDesc *pd, int npk, double2 *X, int64_t cs,
+                                 const double2 *tw, hipStream_t st)
 {
     mti_status r;
-    const int64_t cs = layout == MTI_GRID_VUP ? P : 1;
-    const int64_t ps = layout == MTI_GRID_VUP ? 1 : (int64_t)g.nx * g.ny;
     {
         StageTimer tm(ctx, MTI_STAGE_FFT_ROWS, st);
-        r = fb_rows<TI>(ctx, g, cd, X, P, nch, tw, cs, ps, st);
+        r = pk_rows<TI>(ctx, g, pd, npk, X, cs, tw, st);
         if (r != MTI_OK) return r;
+    }
+    if (fbc_ok(g)) {
+        StageTimer tm(ctx, MTI_STAGE_FINALIZE, st);
+        return pk_cols(ctx, g, X, npk, tw, pd, st);
     }
     {
         StageTimer tm(ctx, MTI_STAGE_FFT_COLS, st);
-        r = fb_cols1(ctx, g, X, nch * P, tw, st);
+        r = fb_cols1(ctx, g, X, npk, tw, st);
         if (r != MTI_OK) return r;
     }
     StageTimer tm(ctx, MTI_STAGE_FINALIZE, st);
-    return fb_cols2<P>(ctx, g, X, nch, tw, cd, poltype, st);
+    return pk_cols2(ctx, g, X, npk, tw, pd, st);
 }
 
-template <typename TI>
-static mti_status finalize_blocks_p(mti_ctx *ctx, int P, const FftGeom &g, const ChanDesc *cd, double2 *X, int nch,
-                                    const double2 *tw, int poltype, int layout, hipStream_t st)
-{
-    if (P == 1) return finalize_blocks<1, TI>(ctx, g, cd, X, nch, tw, poltype, layout, st);
-    if (P == 2) return finalize_blocks<2, TI>(ctx, g, cd, X, nch, tw, poltype, layout, st);
-    return finalize_blocks<4, TI>(ctx, g, cd, X, nch, tw, poltype, layout, st);
-}
-
-// Channels per transform batch (FP64 scratch of at most 16 GB).
-static int transform_batch(const FftGeom &g, int P, int nch)
+// Planes per transform batch (FP64 scratch of at most 16 GB).
+static int transform_batch(const FftGeom &g, int planes)
 {
     const size_t plane_bytes = (size_t)g.nx * g.ny * sizeof(double2);
-    return (int)std::max<size_t>(1, std::min<size_t>((size_t)nch, (size_t(16) << 30) / (plane_bytes * P)));
+    return (int)std::max<size_t>(1, std::min<size_t>((size_t)planes, (size_t(16) << 30) / plane_bytes));
 }
 
-// Uploads the channel descriptors of a call (once; the copy completes before
-// any batch is enqueued, so later batches never overwrite what earlier ones read).
-static mti_status upload_desc(mti_ctx *ctx, const std::vector<ChanDesc> &h, ChanDesc **d, hipStream_t st)
+// Uploads the descriptors of a call (once; the copy completes before any
+// batch is enqueued, so later batches never overwrite what earlier ones read).
+template <typename D>
+static mti_status upload_desc(mti_ctx *ctx, const std::vector<D> &h, D **d, hipStream_t st)
 {
     void *w = nullptr;
-    mti_status r = scratch_get(ctx, SLOT_CHANDESC, std::max<size_t>(1, h.size()) * sizeof(ChanDesc), &w);
+    mti_status r = scratch_get(ctx, SLOT_CHANDESC, std::max<size_t>(1, h.size()) * sizeof(D), &w);
     if (r != MTI_OK) return r;
-    MTI_HIP(ctx, hipMemcpyAsync(w, h.data(), h.size() * sizeof(ChanDesc), hipMemcpyHostToDevice, st));
+    MTI_HIP(ctx, hipMemcpyAsync(w, h.data(), h.size() * sizeof(D), hipMemcpyHostToDevice, st));
     MTI_HIP(ctx, hipStreamSynchronize(st));
-    *d = (ChanDesc *)w;
+    *d = (D *)w;
+    return MTI_OK;
+}
+
+// Transform + finalize of a channel list in batches of planes.
+static mti_status finalize_channels(mti_ctx *ctx, const FftGeom &g, const std::vector<ChanRef> &ch, int P,
+                                    int poltype, int layout, bool c64, hipStream_t st)
+{
+    std::vector<PackDesc> h;
+    make_packs(ch, P, poltype, layout, g.nx, g.ny, c64 ? sizeof(float2) : sizeof(double2), h);
+    PackDesc *pd = nullptr;
+    mti_status r = upload_desc(ctx, h, &pd, st);
+    if (r != MTI_OK) return r;
+    const double2 *tw = nullptr;
+    r = twiddle_table(ctx, g.NT, &tw, st);
+    if (r != MTI_OK) return r;
+    const int npk = (int)h.size();
+    const int batch = transform_batch(g, npk);
+    void *w = nullptr;
+    r = scratch_get(ctx, SLOT_FFT, (size_t)g.nx * g.ny * sizeof(double2) * batch, &w);
+    if (r != MTI_OK) return r;
+    const int64_t cs = layout == MTI_GRID_VUP ? P : 1;
+    for (int k0 = 0; k0 < npk; k0 += batch) {
+        const int nk = std::min(batch, npk - k0);
+        r = c64 ? finalize_packs<float2>(ctx, g, pd + k0, nk, (double2 *)w, cs, tw, st)
+                : finalize_packs<double2>(ctx, g, pd + k0, nk, (double2 *)w, cs, tw, st);
+        if (r != MTI_OK) return r;
+    }
     return MTI_OK;
 }
 
@@ -842,37 +993,31 @@ extern "C" mti_status mti_finalize(mti_ctx *ctx, const mti_image_params *p, int3
     const double2 *tw = nullptr;
     r = twiddle_table(ctx, g.NT, &tw, st);
     if (r != MTI_OK) return r;
-    // scratch: FP64 transform buffer for a batch of channels + correction vectors
-    const size_t plane_bytes = (size_t)p->nx * p->ny * sizeof(double2);
-    const int batch = transform_batch(g, npol, nchan);
-    void *w = nullptr;
-    r = scratch_get(ctx, SLOT_FFT, plane_bytes * npol * batch + (size_t)(p->nx + p->ny + 128) * 4, &w);
+    // correction vectors
+    void *wc = nullptr;
+    r = scratch_get(ctx, SLOT_FCORR, (size_t)(p->nx + p->ny + 128) * 4, &wc);
     if (r != MTI_OK) return r;
-    double2 *X = (double2 *)w;
-    float *corr_x = (float *)((char *)w + plane_bytes * npol * batch);
+    float *corr_x = (float *)wc;
     float *corr_y = corr_x + p->nx + 64;
     hipLaunchKernelGGL(k_corr, dim3((unsigned)ceil_div(p->nx, 256)), dim3(256), 0, st, corr_x, p->nx, p->sampling);
     hipLaunchKernelGGL(k_corr, dim3((unsigned)ceil_div(p->ny, 256)), dim3(256), 0, st, corr_y, p->ny, p->sampling);
     MTI_HIP(ctx, hipGetLastError());
     const size_t elem = grid_precision == MTI_GRID_C64 ? sizeof(float2) : sizeof(double2);
     const size_t chan_grid = (size_t)p->nx * p->ny * npol;
-    if (fb_ok(g, npol)) {
-        std::vector<ChanDesc> h(nchan);
+    if (pk_ok(g)) {
+        std::vector<ChanRef> ch(nchan);
         for (int c = 0; c < nchan; c++)
-            h[c] = ChanDesc{(const char *)grids + elem * chan_grid * c, norm + (size_t)c * npol,
+            ch[c] = ChanRef{(const char *)grids + elem * chan_grid * c, norm + (size_t)c * npol,
                             image + chan_grid * c, corr_x, corr_y};
-        ChanDesc *cd = nullptr;
-        r = upload_desc(ctx, h, &cd, st);
-        if (r != MTI_OK) return r;
-        for (int c0 = 0; c0 < nchan; c0 += batch) {
-            const int nb = std::min(batch, nchan - c0);
-            r = grid_precision == MTI_GRID_C64
-                    ? finalize_blocks_p<float2>(ctx, npol, g, cd + c0, X, nb, tw, poltype, grid_layout, st)
-                    : finalize_blocks_p<double2>(ctx, npol, g, cd + c0, X, nb, tw, poltype, grid_layout, st);
-            if (r != MTI_OK) return r;
-        }
-        return MTI_OK;
+        return finalize_channels(ctx, g, ch, npol, poltype, grid_layout, grid_precision == MTI_GRID_C64, st);
     }
+    // small images: one FP64 plane per correlation, transform and finalize per channel batch
+    const size_t plane_bytes = (size_t)p->nx * p->ny * sizeof(double2);
+    const int batch = std::max(1, transform_batch(g, npol * nchan) / npol);
+    void *w = nullptr;
+    r = scratch_get(ctx, SLOT_FFT, plane_bytes * npol * batch, &w);
+    if (r != MTI_OK) return r;
+    double2 *X = (double2 *)w;
     for (int c0 = 0; c0 < nchan; c0 += batch) {
         const int nb = std::min(batch, nchan - c0);
         const size_t off = (size_t)c0 * chan_grid;
@@ -920,75 +1065,63 @@ extern "C" mti_status mti_finalize_jobs(mti_ctx *ctx, int32_t njobs, const mti_j
     MTI_CHECK(ctx, P == 1 || P == 2 || P == 4, "npol must be 1, 2 or 4");
     MTI_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = to_stream(stream);
-    // correction vectors of every job (they depend on the size and sampling)
-    std::vector<size_t> coff(njobs + 1, 0);
-    for (int i = 0; i < njobs; i++) coff[i + 1] = coff[i] + (size_t)jobs[i].params.nx + jobs[i].params.ny + 64;
-    void *wc = nullptr;
-    mti_status r = scratch_get(ctx, SLOT_CORR, coff[njobs] * sizeof(float), &wc);
-    if (r != MTI_OK) return r;
-    float *corr = (float *)wc;
+    // correction vectors, once per distinct (nx, ny, sampling): they depend on
+    // nothing else, and jobs that share them can share packed planes
+    std::vector<std::array<int, 3>> ckey;
+    std::vector<size_t> cbase, coff(njobs);
+    size_t ctot = 0;
     for (int i = 0; i < njobs; i++) {
         const mti_image_params &p = jobs[i].params;
-        hipLaunchKernelGGL(k_corr, dim3((unsigned)ceil_div(p.nx, 256)), dim3(256), 0, st, corr + coff[i], p.nx,
-                           p.sampling);
-        hipLaunchKernelGGL(k_corr, dim3((unsigned)ceil_div(p.ny, 256)), dim3(256), 0, st, corr + coff[i] + p.nx,
-                           p.ny, p.sampling);
+        const std::array<int, 3> key{p.nx, p.ny, p.sampling};
+        size_t k = std::find(ckey.begin(), ckey.end(), key) - ckey.begin();
+        if (k == ckey.size()) {
+            ckey.push_back(key);
+            cbase.push_back(ctot);
+            ctot += (size_t)p.nx + p.ny + 64;
+        }
+        coff[i] = cbase[k];
+    }
+    void *wc = nullptr;
+    mti_status r = scratch_get(ctx, SLOT_CORR, ctot * sizeof(float), &wc);
+    if (r != MTI_OK) return r;
+    float *corr = (float *)wc;
+    for (size_t k = 0; k < ckey.size(); k++) {
+        const int nx = ckey[k][0], ny = ckey[k][1], smp = ckey[k][2];
+        hipLaunchKernelGGL(k_corr, dim3((unsigned)ceil_div(nx, 256)), dim3(256), 0, st, corr + cbase[k], nx, smp);
+        hipLaunchKernelGGL(k_corr, dim3((unsigned)ceil_div(ny, 256)), dim3(256), 0, st, corr + cbase[k] + nx, ny,
+                           smp);
     }
     MTI_HIP(ctx, hipGetLastError());
-    // image sizes with the block transform: one descriptor list per size
+    // one packed-plane transform per image size over the channels of all its jobs
     std::vector<std::pair<int, int>> sizes;
     for (int i = 0; i < njobs; i++) {
         const std::pair<int, int> sz{jobs[i].params.nx, jobs[i].params.ny};
         if (std::find(sizes.begin(), sizes.end(), sz) == sizes.end()) sizes.push_back(sz);
     }
-    std::vector<ChanDesc> h;
-    std::vector<size_t> goff;
-    std::vector<int> gch;
     for (auto sz : sizes) {
         const FftGeom g = geom(sz.first, sz.second);
-        goff.push_back(h.size());
-        int n = 0;
-        if (fb_ok(g, P)) {
-            const size_t chan_grid = (size_t)sz.first * sz.second * P;
-            for (int i = 0; i < njobs; i++) {
-                const mti_job &j = jobs[i];
-                if (j.params.nx != sz.first || j.params.ny != sz.second) continue;
-                for (int c = 0; c < j.nchan; c++, n++)
-                    h.push_back(ChanDesc{(const double2 *)j.grids + chan_grid * c, j.norm + (size_t)c * P,
-                                         j.image + chan_grid * c, corr + coff[i], corr + coff[i] + sz.first});
-            }
-        }
-        gch.push_back(n);
-    }
-    ChanDesc *cd = nullptr;
-    r = upload_desc(ctx, h, &cd, st);
-    if (r != MTI_OK) return r;
-    for (size_t s = 0; s < sizes.size(); s++) {
-        const FftGeom g = geom(sizes[s].first, sizes[s].second);
-        if (!fb_ok(g, P)) {
+        if (!pk_ok(g)) {
             // small images: one job at a time through the per-job finalize
             for (int i = 0; i < njobs; i++) {
                 const mti_job &j = jobs[i];
-                if (j.params.nx != sizes[s].first || j.params.ny != sizes[s].second) continue;
+                if (j.params.nx != sz.first || j.params.ny != sz.second) continue;
                 r = mti_finalize(ctx, &j.params, P, poltype, j.nchan, grid_layout, MTI_GRID_C128, j.grids, j.norm,
                                  j.image, stream);
                 if (r != MTI_OK) return r;
             }
             continue;
         }
-        const double2 *tw = nullptr;
-        r = twiddle_table(ctx, g.NT, &tw, st);
-        if (r != MTI_OK) return r;
-        const int batch = transform_batch(g, P, gch[s]);
-        void *w = nullptr;
-        r = scratch_get(ctx, SLOT_FFT, (size_t)g.nx * g.ny * sizeof(double2) * P * batch, &w);
-        if (r != MTI_OK) return r;
-        for (int c0 = 0; c0 < gch[s]; c0 += batch) {
-            const int nb = std::min(batch, gch[s] - c0);
-            r = finalize_blocks_p<double2>(ctx, P, g, cd + goff[s] + c0, (double2 *)w, nb, tw, poltype, grid_layout,
-                                           st);
-            if (r != MTI_OK) return r;
+        const size_t chan_grid = (size_t)sz.first * sz.second * P;
+        std::vector<ChanRef> ch;
+        for (int i = 0; i < njobs; i++) {
+            const mti_job &j = jobs[i];
+            if (j.params.nx != sz.first || j.params.ny != sz.second) continue;
+            for (int c = 0; c < j.nchan; c++)
+                ch.push_back(ChanRef{(const double2 *)j.grids + chan_grid * c, j.norm + (size_t)c * P,
+                                     j.image + chan_grid * c, corr + coff[i], corr + coff[i] + sz.first});
         }
+        r = finalize_channels(ctx, g, ch, P, poltype, grid_layout, false, st);
+        if (r != MTI_OK) return r;
     }
     return MTI_OK;
 }

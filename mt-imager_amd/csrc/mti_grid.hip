@@ -1,5 +1,4 @@
-// mti_grid.hip -- convolutional gridding of many channels and many jobs
-// (rows a5-a7, a15, a21).
+// mti_grid.hip -- convolutional gridding of many channels and many jobs.
 //
 // Reference: ConvolutionGridder (op/ConvolutionGridder.cu:183-550) scatters
 // grid[corner_y+sy][corner_x+sx][p] += K[conv_idx + sy*S + sx] * V_p, with
@@ -29,10 +28,13 @@
 //   5. per-record effective weights and the normaliser;
 //   6. gridder: one 1024-thread workgroup per unit; wave w owns the 8x8 region
 //      w of the tile and each lane one cell, accumulated in FP64 registers
-//      (see k_grid_tiles); the tile is added to the FP64 grid once -- plain
-//      read-modify-write when the unit is the tile's only one, FP64 atomics
+//      (see k_grid_tiles); the tile is added to the FP64 grid once -- stored
+//      when the unit is the tile's only one and the grid was not pre-filled
+//      (k_zero_tiles zeroes every other tile first), read-modify-write when
+//      the unit is the tile's only one of an accumulating call, FP64 atomics
 //      otherwise.
 #include "mti_grid.hpp"
+#include "mti_sort.hpp"
 
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
@@ -60,9 +62,11 @@ struct GridArgs {
     float2 *wvis;               // [M][P] visibilities, weighted in place by k_record_prep
     float *wt;                  // [M][P] natural weights, 0 where flagged
     float *rdens;               // [M][P] uniform/Briggs: weight density at the record's nearest cell
-    const uint32_t *pairs;      // record index per (tile, record) pair, grouped by tile
+    const uint32_t *pairs;      // record index (| PAIR_HOME) per (tile, record) pair, grouped by tile
     const int4 *units;          // {key, first pair, pairs, 1 if the tile's only unit}
     const uint32_t *n_units;    // device scalar
+    int fresh;                  // grids not pre-filled: a tile's only unit stores its cells,
+                                // every other tile was zeroed by k_zero_tiles
 };
 
 // Global-memory views of pointers read from the device-resident job table.
@@ -165,6 +169,10 @@ __device__ __forceinline__ WaveRuns wave_runs(int key)
     return r;
 }
 
+// A pair is the record index; bit 31 marks the record's home tile, the one
+// holding its nearest cell (box centre), which the weight density uses.
+constexpr uint32_t PAIR_HOME = 0x80000000u, PAIR_INDEX = 0x7fffffffu;
+
 // Pass 0: count the pairs of every tile.  Pass 1: scatter record indices.
 template <int PASS>
 __global__ __launch_bounds__(256) void k_tile_pairs(GridArgs a, int64_t cap, uint32_t *cnt, const uint32_t *start,
@@ -174,9 +182,12 @@ __global__ __launch_bounds__(256) void k_tile_pairs(GridArgs a, int64_t cap, uin
     if ((int64_t)blockIdx.x * blockDim.x >= (int64_t)*a.n_rec) return;  // whole block past the end
     const bool in = j < cap && j < (int64_t)*a.n_rec;
     TileRange t{0, 0, 0, 0};
-    int base = 0, ntx = 1;
+    int base = 0, ntx = 1, home = -1;
     if (in) {
-        t = tile_range(a.cdesc[j]);
+        const int4 d = a.cdesc[j];
+        t = tile_range(d);
+        const int S = d.w & 0x7fff;
+        home = ((((unsigned)d.y >> 16) + S / 2) / TG_T - t.ty0) * t.ntx + ((d.y & 0xffff) + S / 2) / TG_T - t.tx0;
         const int jc = a.cjc[j], job = jc >> 16, c = jc & 0xffff;
         const JobDev &jd = a.js.jobs[job];
         ntx = jd.ntx;
@@ -198,7 +209,8 @@ __global__ __launch_bounds__(256) void k_tile_pairs(GridArgs a, int64_t cap, uin
             uint32_t b = 0;
             if (r.head && key >= 0) b = atomicAdd(&cnt[key], (uint32_t)r.len);
             b = __shfl(b, r.hl, 64);
-            if (key >= 0) pairs[start[key] + b + (uint32_t)((threadIdx.x & 63) - r.hl)] = (uint32_t)j;
+            if (key >= 0)
+                pairs[start[key] + b + (uint32_t)((threadIdx.x & 63) - r.hl)] = (uint32_t)j | (q == home ? PAIR_HOME : 0u);
         }
     }
 }
@@ -285,11 +297,12 @@ __global__ __launch_bounds__(256) void k_density_keys(GridArgs a, const uint32_t
         for (int i = tid; i < P * TG_TT; i += 256) s_d[i] = 0.0f;
         __syncthreads();
         for (uint32_t i = p0 + tid; i < p1; i += 256) {
-            const uint32_t j = a.pairs[i];
+            const uint32_t pj = a.pairs[i];
+            if (!(pj & PAIR_HOME)) continue;  // only the record's home tile gathers it
+            const uint32_t j = pj & PAIR_INDEX;
             const int4 d = a.cdesc[j];
             const int S = d.w & 0x7fff;
             const int mx = (d.y & 0xffff) + S / 2 - x0, my = (int)((unsigned)d.y >> 16) + S / 2 - y0;
-            if (mx < 0 || mx >= TG_T || my < 0 || my >= TG_T) continue;
 #pragma unroll
             for (int p = 0; p < P; p++) {
                 const float w = a.wt[(int64_t)j * P + p];
@@ -303,11 +316,12 @@ __global__ __launch_bounds__(256) void k_density_keys(GridArgs a, const uint32_t
             s2 += v * v;
         }
         for (uint32_t i = p0 + tid; i < p1; i += 256) {
-            const uint32_t j = a.pairs[i];
+            const uint32_t pj = a.pairs[i];
+            if (!(pj & PAIR_HOME)) continue;
+            const uint32_t j = pj & PAIR_INDEX;
             const int4 d = a.cdesc[j];
             const int S = d.w & 0x7fff;
             const int mx = (d.y & 0xffff) + S / 2 - x0, my = (int)((unsigned)d.y >> 16) + S / 2 - y0;
-            if (mx < 0 || mx >= TG_T || my < 0 || my >= TG_T) continue;
 #pragma unroll
             for (int p = 0; p < P; p++) a.rdens[(int64_t)j * P + p] = s_d[(my * TG_T + mx) * P + p];
         }
@@ -458,7 +472,7 @@ __global__ __launch_bounds__(TG_THREADS) void k_grid_tiles(GridArgs a)
             {
                 const int rr = tid >> 2, part = tid & 3;
                 if (rr < nbat) {
-                    const uint32_t j = a.pairs[first + b0 + rr];
+                    const uint32_t j = a.pairs[first + b0 + rr] & PAIR_INDEX;
                     if (part < P) {
                         const float2 v = a.wvis[(int64_t)j * P + part];
                         s_vd[rr][2 * part] = v.x;
@@ -517,7 +531,10 @@ __global__ __launch_bounds__(TG_THREADS) void k_grid_tiles(GridArgs a)
         if (gx < jd.g.nx && gy < jd.g.ny) {
             double *g = jd.grid + 2 * ((int64_t)ta.c_local * jd.chan_stride + ((int64_t)gy * jd.g.nx + gx) * jd.cs);
             const int64_t ps = jd.ps;
-            if (excl) {
+            if (excl && a.fresh) {
+#pragma unroll
+                for (int p = 0; p < P; p++) *(double2 *)(g + 2 * p * ps) = make_double2(acc[2 * p], acc[2 * p + 1]);
+            } else if (excl) {
                 double2 old[P];
 #pragma unroll
                 for (int p = 0; p < P; p++)
@@ -539,6 +556,37 @@ __global__ __launch_bounds__(TG_THREADS) void k_grid_tiles(GridArgs a)
 }
 
 // ---- small kernels ------------------------------------------------------------
+
+// Zeroes the tiles that no single unit owns (no unit at all, or several units
+// adding with atomics), so that a tile's only unit can store its cells
+// without a prior fill of the grid.  uprefix is the exclusive scan of the
+// units per tile key, total the unit count.  Blocks stride over the keys.
+__global__ __launch_bounds__(256) void k_zero_tiles(JobSet js, const uint32_t *uprefix, const uint32_t *total,
+                                                   int64_t nkeys)
+{
+    const int P = js.P;
+    for (int64_t k = blockIdx.x; k < nkeys; k += gridDim.x) {
+        const uint32_t u1 = (k + 1 < nkeys) ? uprefix[k + 1] : *total;
+        if (u1 - uprefix[k] == 1) continue;
+        const TileAt ta = tile_at(js, (int)k);
+        const JobDev &jd = js.jobs[ta.job];
+        const int nx = jd.g.nx, ny = jd.g.ny, x0 = ta.tx * TG_T, y0 = ta.ty * TG_T;
+        const int w = min(TG_T, nx - x0), h = min(TG_T, ny - y0), cells = w * h;
+        double2 *g = (double2 *)jd.grid + (int64_t)ta.c_local * jd.chan_stride;
+        for (int i = threadIdx.x; i < cells * P; i += blockDim.x) {
+            int p, cell;
+            if (jd.ps == 1) {  // [v][u][P]: correlations of a cell adjacent
+                p = i % P;
+                cell = i / P;
+            } else {           // [P][v][u]: cells of a plane adjacent
+                p = i / cells;
+                cell = i - p * cells;
+            }
+            const int cy = cell / w, cx = cell - cy * w;
+            g[((int64_t)(y0 + cy) * nx + x0 + cx) * jd.cs + p * jd.ps] = make_double2(0.0, 0.0);
+        }
+    }
+}
 
 __global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v)
 {
@@ -654,7 +702,7 @@ static int64_t job_keys(const PassJob &j)
 // grids (zeroed or holding earlier chunks) and accumulates their normalisers.
 // nu_total[0] += work units, [1] += (record, tile) pairs.
 static mti_status grid_pass(mti_ctx *ctx, const std::vector<PassJob> &jobs, const mti_grid_opts *o, int P, int prec,
-                            unsigned long long *dstats, uint32_t *nu_total, hipStream_t st)
+                            bool fresh, unsigned long long *dstats, uint32_t *nu_total, hipStream_t st)
 {
     const int nj = (int)jobs.size();
     if (nj == 0) return MTI_OK;
@@ -693,7 +741,15 @@ static mti_status grid_pass(mti_ctx *ctx, const std::vector<PassJob> &jobs, cons
         d.key0 = key0[i];
     }
     const int64_t nrec = rec0[nj], nkeys = key0[nj], nch = jc0[nj];
-    if (nrec == 0) return MTI_OK;
+    if (nrec == 0) {
+        // nothing to grid: a grid that was not pre-filled is all zeros
+        if (fresh)
+            for (const PassJob &j : jobs)
+                hipLaunchKernelGGL(k_fill_f64, dim3(1024), dim3(256), 0, st, j.grid,
+                                   (int64_t)j.p.nx * j.p.ny * P * 2 * j.nb, 0.0);
+        MTI_HIP(ctx, hipGetLastError());
+        return MTI_OK;
+    }
     MTI_CHECK(ctx, nrec < (int64_t(1) << 31) && nkeys < (int64_t(1) << 31) && pair_cap < (int64_t(1) << 32),
               "gridding pass above the record/tile limits");
     const int64_t unit_cap = nkeys + pair_cap / TG_CHUNK + 1;
@@ -779,6 +835,7 @@ static mti_status grid_pass(mti_ctx *ctx, const std::vector<PassJob> &jobs, cons
     ga.pairs = pairs;
     ga.units = units;
     ga.n_units = NU_d;
+    ga.fresh = fresh ? 1 : 0;
     {
         StageTimer tm(ctx, MTI_STAGE_SORT_RECORDS, st);
         // every record stays in place (job, channel, baseline order); records
@@ -804,6 +861,11 @@ static mti_status grid_pass(mti_ctx *ctx, const std::vector<PassJob> &jobs, cons
         if (r != MTI_OK) return r;
         hipLaunchKernelGGL(k_unit_list, dim3(kblocks), dim3(256), 0, st, start, NP_d, nunits, NU_d, nkeys, units);
         MTI_HIP(ctx, hipGetLastError());
+        if (fresh) {
+            hipLaunchKernelGGL(k_zero_tiles, dim3((unsigned)std::min<int64_t>(nkeys, 16384)), dim3(256), 0, st, js,
+                               (const uint32_t *)nunits, (const uint32_t *)NU_d, nkeys);
+            MTI_HIP(ctx, hipGetLastError());
+        }
         if (weighted) {
             hipLaunchKernelGGL(k_fill_f64, dim3(64), dim3(256), 0, st, dacc, (int64_t)nch * P * 2, 0.0);
             MTI_HIP(ctx, hipGetLastError());
@@ -871,12 +933,17 @@ static mti_status stats_out(mti_ctx *ctx, unsigned long long *dstats, const uint
                             hipStream_t st)
 {
     if (!stats) return MTI_OK;
-    unsigned long long h[2] = {0, 0};
+    unsigned long long h[2 * BIN_STAT_STRIPES];
     MTI_HIP(ctx, hipMemcpyAsync(h, dstats, sizeof(h), hipMemcpyDeviceToHost, st));
     MTI_HIP(ctx, hipStreamSynchronize(st));
+    unsigned long long s[2] = {0, 0};
+    for (int k = 0; k < BIN_STAT_STRIPES; k++) {
+        s[0] += h[k];
+        s[1] += h[BIN_STAT_STRIPES + k];
+    }
     *stats = mti_grid_stats{};
-    stats->gridded_records = (int64_t)h[0];
-    stats->gridded_points = (int64_t)h[1];
+    stats->gridded_records = (int64_t)s[0];
+    stats->gridded_points = (int64_t)s[1];
     stats->work_items = nu[0];
     stats->flushes = nu[1];
     return MTI_OK;
@@ -921,15 +988,16 @@ extern "C" mti_status mti_grid(mti_ctx *ctx, const mti_kernels *k, const mti_ima
 
     if (!o->accumulate) {
         hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(256), 0, st, norm, (int64_t)nb * P, 0.0);
-        if (o->grid_precision == MTI_GRID_C128)
+        // with rows, the gridding pass zeroes exactly the tiles it does not store whole
+        if (o->grid_precision == MTI_GRID_C128 && vw.nidx == 0)
             hipLaunchKernelGGL(k_fill_f64, dim3(8192), dim3(256), 0, st, (double *)grids, chan_floats * nb, 0.0);
         MTI_HIP(ctx, hipGetLastError());
     }
     void *sw = nullptr;
-    r = scratch_get(ctx, SLOT_MISC2, 256, &sw);
+    r = scratch_get(ctx, SLOT_MISC2, 2 * BIN_STAT_STRIPES * sizeof(unsigned long long), &sw);
     if (r != MTI_OK) return r;
     unsigned long long *dstats = (unsigned long long *)sw;
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, (uint32_t *)dstats, 8, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, st, (uint32_t *)dstats, 4 * BIN_STAT_STRIPES, 0u);
     MTI_HIP(ctx, hipGetLastError());
     uint32_t nu[2] = {0, 0};
 
@@ -947,7 +1015,7 @@ extern "C" mti_status mti_grid(mti_ctx *ctx, const mti_kernels *k, const mti_ima
             j.nb = std::min(pair_batch, nb - c0);
             j.grid = (double *)grids + (size_t)c0 * chan_floats;
             j.norm = norm + (size_t)c0 * P;
-            r = grid_pass(ctx, {j}, o, P, prec, dstats, nu, st);
+            r = grid_pass(ctx, {j}, o, P, prec, !o->accumulate, dstats, nu, st);
             if (r != MTI_OK) return r;
         }
     } else {
@@ -960,16 +1028,18 @@ extern "C" mti_status mti_grid(mti_ctx *ctx, const mti_kernels *k, const mti_ima
         if (r != MTI_OK) return r;
         for (int c0 = 0; c0 < nb; c0 += batch) {
             const int n = std::min(batch, nb - c0);
-            hipLaunchKernelGGL(k_fill_f64, dim3(8192), dim3(256), 0, st, (double *)gw, chan_floats * n, 0.0);
-            MTI_HIP(ctx, hipGetLastError());
             if (vw.nidx > 0) {
+                // the work grid is never pre-filled: the pass zeroes what it does not store
                 PassJob j = base;
                 j.chan0 = o->chan0 + c0;
                 j.nb = n;
                 j.grid = (double *)gw;
                 j.norm = norm + (size_t)c0 * P;
-                r = grid_pass(ctx, {j}, o, P, prec, dstats, nu, st);
+                r = grid_pass(ctx, {j}, o, P, prec, true, dstats, nu, st);
                 if (r != MTI_OK) return r;
+            } else {
+                hipLaunchKernelGGL(k_fill_f64, dim3(8192), dim3(256), 0, st, (double *)gw, chan_floats * n, 0.0);
+                MTI_HIP(ctx, hipGetLastError());
             }
             hipLaunchKernelGGL(k_round_grid, dim3(8192), dim3(256), 0, st, (const double *)gw,
                                (float *)grids + (size_t)c0 * chan_floats, chan_floats * n, o->accumulate);
@@ -991,8 +1061,14 @@ extern "C" mti_status mti_grid_jobs(mti_ctx *ctx, int32_t njobs, const mti_job *
     hipStream_t st = to_stream(stream);
     int P = 0, prec = -1;
     std::vector<PassJob> all;
+    int64_t sort_rows_total = 0;  // rows of the jobs whose order the call computes
     for (int i = 0; i < njobs; i++) {
         const mti_job &j = jobs[i];
+        if (j.ant1) {
+            MTI_CHECK(ctx, j.ant2 && !j.vis.row_index, "a job with ant1 needs ant2 and no row_index");
+            MTI_CHECK(ctx, j.nant >= 1 && j.nant <= 46340, "job antenna count out of range");
+            sort_rows_total += j.vis.nrows;
+        }
         r = check_job_geometry(ctx, j.kernels, &j.params, &j.vis);
         if (r != MTI_OK) return r;
         MTI_CHECK(ctx, j.grids && j.norm, "null job outputs");
@@ -1014,19 +1090,40 @@ extern "C" mti_status mti_grid_jobs(mti_ctx *ctx, int32_t njobs, const mti_job *
         all.push_back(pj);
     }
     void *sw = nullptr;
-    r = scratch_get(ctx, SLOT_MISC2, 256, &sw);
+    r = scratch_get(ctx, SLOT_MISC2, 2 * BIN_STAT_STRIPES * sizeof(unsigned long long), &sw);
     if (r != MTI_OK) return r;
     unsigned long long *dstats = (unsigned long long *)sw;
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, (uint32_t *)dstats, 8, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, st, (uint32_t *)dstats, 4 * BIN_STAT_STRIPES, 0u);
     MTI_HIP(ctx, hipGetLastError());
-    uint32_t nu[2] = {0, 0};
-    // zero outputs
-    if (!o->accumulate) {
-        for (const PassJob &j : all) {
-            hipLaunchKernelGGL(k_fill_f64, dim3(1024), dim3(256), 0, st, j.grid,
-                               (int64_t)j.p.nx * j.p.ny * P * 2 * j.nb, 0.0);
-            hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(64), 0, st, j.norm, (int64_t)j.nb * P, 0.0);
+    // baseline-major row order of the jobs that ask for it, kept counts on the
+    // device (index and counts live in SLOT_RECVIS for the whole call)
+    if (sort_rows_total > 0) {
+        StageTimer tm(ctx, MTI_STAGE_SORT_ROWS, st);
+        size_t maxrows = 0;
+        for (int i = 0; i < njobs; i++)
+            if (jobs[i].ant1) maxrows = std::max<size_t>(maxrows, (size_t)jobs[i].vis.nrows);
+        const size_t idx_bytes = align_up((size_t)sort_rows_total * 4) + align_up((size_t)njobs * 8);
+        void *wi = nullptr, *wk = nullptr;
+        if ((r = scratch_get(ctx, SLOT_RECVIS, idx_bytes, &wi)) != MTI_OK) return r;
+        if ((r = scratch_get(ctx, SLOT_KEYS, maxrows * 12 + 4096, &wk)) != MTI_OK) return r;
+        int32_t *ip = (int32_t *)wi;
+        unsigned long long *kp = (unsigned long long *)((char *)wi + align_up((size_t)sort_rows_total * 4));
+        for (int i = 0; i < njobs; i++) {
+            if (!jobs[i].ant1) continue;
+            const int64_t n = jobs[i].vis.nrows;
+            r = sort_rows_device(ctx, jobs[i].ant1, jobs[i].ant2, n, jobs[i].nant, ip, (uint32_t *)wk, kp + i, st);
+            if (r != MTI_OK) return r;
+            all[i].vw.row_index = ip;
+            all[i].vw.nidx = n;
+            all[i].vw.nkept = kp + i;
+            ip += n;
         }
+    }
+    uint32_t nu[2] = {0, 0};
+    // zero the normalisers; the grids are zeroed by the passes (tiles not stored whole)
+    if (!o->accumulate) {
+        for (const PassJob &j : all)
+            hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(64), 0, st, j.norm, (int64_t)j.nb * P, 0.0);
         MTI_HIP(ctx, hipGetLastError());
     }
     // passes: consecutive jobs while the pass stays within its limits (a job
@@ -1045,7 +1142,7 @@ extern "C" mti_status mti_grid_jobs(mti_ctx *ctx, int32_t njobs, const mti_job *
             if (!pass.empty() &&
                 (in_r + job_records(piece) >= (int64_t(1) << 31) || in_p + job_pairs(piece) >= (int64_t(1) << 32) ||
                  in_k + job_keys(piece) >= (int64_t(1) << 31) || pass.size() >= 32767)) {
-                r = grid_pass(ctx, pass, o, P, prec, dstats, nu, st);
+                r = grid_pass(ctx, pass, o, P, prec, !o->accumulate, dstats, nu, st);
                 if (r != MTI_OK) return r;
                 pass.clear();
                 in_r = in_p = in_k = 0;
@@ -1058,7 +1155,7 @@ extern "C" mti_status mti_grid_jobs(mti_ctx *ctx, int32_t njobs, const mti_job *
         }
     }
     if (!pass.empty()) {
-        r = grid_pass(ctx, pass, o, P, prec, dstats, nu, st);
+        r = grid_pass(ctx, pass, o, P, prec, !o->accumulate, dstats, nu, st);
         if (r != MTI_OK) return r;
     }
     return stats_out(ctx, dstats, nu, stats, st);

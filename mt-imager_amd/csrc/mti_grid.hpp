@@ -22,6 +22,9 @@ struct VisView {
     const int32_t *row_index;
     int64_t nidx;  // records per channel
     int npol;
+    // device count of leading row_index entries to grid (the rest are rows a
+    // device-side row sort put behind the kept ones), or null: all nidx
+    const unsigned long long *nkept;
 };
 
 struct BinGeom {
@@ -79,13 +82,15 @@ __device__ __forceinline__ int job_of(const int64_t *prefix, int n, int64_t x)
 // Every (job, channel, row) gets a record; one that is not gridded (all
 // correlations flagged, or its support box off the grid) has support 0 and
 // no visibilities or weights.
+constexpr int BIN_STAT_STRIPES = 32;
+
 struct BinPassArgs {
     JobSet js;
     int32_t *cjc;               // [nrec] job << 16 | local channel
     int4 *desc;                 // [nrec]
     float2 *wv;                 // [nrec][npol] visibilities of gridded records
     float *wt;                  // [nrec][npol] weights, 0 where flagged
-    unsigned long long *stats;  // [0] gridded records, [1] gridded points
+    unsigned long long *stats;  // [2][BIN_STAT_STRIPES]: gridded records, gridded points
 };
 
 mti_status launch_bin_pass(mti_ctx *ctx, const BinPassArgs &a, int64_t nblocks, hipStream_t st);

@@ -21,6 +21,28 @@ struct Status {
     std::string msg;
 };
 
+// Scratch slots.
+enum ScratchSlot {
+    SLOT_KEYS = 0,
+    SLOT_DESC = 1,
+    SLOT_SORTED = 2,
+    SLOT_BUCKETS = 3,
+    SLOT_ITEMS = 4,
+    SLOT_DENSITY = 5,
+    SLOT_MISC = 6,
+    SLOT_FFT = 7,
+    SLOT_RADIX = 8,
+    SLOT_NORM = 9,
+    SLOT_GRID64 = 10,
+    SLOT_MISC2 = 11,
+    SLOT_RECVIS = 12,   // row order and kept counts of jobs sorted inside mti_grid_jobs
+    SLOT_CRECVIS = 13,  // per-record visibilities + weights
+    SLOT_CHANDESC = 14, // descriptors of the finalize (packed planes)
+    SLOT_CORR = 15,     // per-job correction vectors of the batched finalize
+    SLOT_FCORR = 16,    // correction vectors of mti_finalize
+    SLOT_COUNT = 17
+};
+
 }  // namespace mti
 
 struct mti_ctx {
@@ -35,7 +57,7 @@ struct mti_ctx {
     };
     std::vector<StageEv> stage_events;
     std::vector<hipEvent_t> event_pool;
-    mti::Scratch scratch[16];
+    mti::Scratch scratch[mti::SLOT_COUNT];
     // FFT twiddle tables keyed by length (FP64, exp(+2 pi i k / n)).
     std::vector<std::pair<int, double2 *>> twiddles;
 };
@@ -52,26 +74,6 @@ struct mti_kernels {
 
 namespace mti {
 
-// Scratch slots.
-enum ScratchSlot {
-    SLOT_KEYS = 0,
-    SLOT_DESC = 1,
-    SLOT_SORTED = 2,
-    SLOT_BUCKETS = 3,
-    SLOT_ITEMS = 4,
-    SLOT_DENSITY = 5,
-    SLOT_MISC = 6,
-    SLOT_FFT = 7,
-    SLOT_RADIX = 8,
-    SLOT_NORM = 9,
-    SLOT_GRID64 = 10,
-    SLOT_MISC2 = 11,
-    SLOT_RECVIS = 12,   // unused
-    SLOT_CRECVIS = 13,  // per-record visibilities + weights
-    SLOT_CHANDESC = 14, // per-channel descriptors of the finalize
-    SLOT_CORR = 15,     // per-job correction vectors of the batched finalize
-    SLOT_COUNT = 16
-};
 
 inline mti_status set_error(mti_ctx *ctx, mti_status code, const std::string &m)
 {
@@ -94,6 +96,8 @@ inline mti_status set_error(mti_ctx *ctx, mti_status code, const std::string &m)
 
 inline mti_status scratch_get(mti_ctx *ctx, int slot, size_t bytes, void **out)
 {
+    if (slot < 0 || slot >= SLOT_COUNT)
+        return set_error(ctx, MTI_ERR_INVALID, "scratch slot out of range");
     Scratch &s = ctx->scratch[slot];
     if (s.bytes < bytes) {
         if (s.ptr) {

@@ -1,5 +1,5 @@
 // mti_sort.hip -- stable LSD radix sort of (uint32 key, int32 value) pairs and
-// the baseline-major row order (row a1).
+// the baseline-major row order.
 //
 // vis/SortIndex.cpp:46-87 buckets rows by baseline key Nant^2 - a1*Nant - a2 - 1
 // on one host thread and keeps time order inside a baseline.  Here the same
@@ -138,6 +138,27 @@ __global__ void k_zero_u64(unsigned long long *p, int n)
     if (i < n) p[i] = 0ull;
 }
 
+mti_status sort_rows_device(mti_ctx *ctx, const int32_t *ant1, const int32_t *ant2, int64_t nrows, int nant,
+                            int32_t *index_out, uint32_t *tmp, unsigned long long *kept, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, kept, 1);
+    if (nrows == 0) return MTI_OK;
+    uint32_t *k0 = tmp, *k1 = tmp + nrows;
+    int32_t *v1 = (int32_t *)(k1 + nrows);
+    hipLaunchKernelGGL(k_baseline_keys, dim3((unsigned)ceil_div(nrows, 256)), dim3(256), 0, st, ant1, ant2, nrows,
+                       nant, k0, index_out, kept);
+    MTI_HIP(ctx, hipGetLastError());
+    const uint32_t nk = (uint32_t)nant * (uint32_t)nant;
+    int bits = 1;
+    while (bits < 32 && (nk >> bits) != 0) bits++;
+    bool in_tmp = false;
+    mti_status r = radix_sort_pairs(ctx, k0, index_out, k1, v1, nrows, bits, st, &in_tmp);
+    if (r != MTI_OK) return r;
+    if (in_tmp)
+        MTI_HIP(ctx, hipMemcpyAsync(index_out, v1, (size_t)nrows * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
+    return MTI_OK;
+}
+
 }  // namespace mti
 
 using namespace mti;
@@ -158,27 +179,14 @@ extern "C" mti_status mti_sort_rows(mti_ctx *ctx, const int32_t *ant1, const int
         return MTI_OK;
     }
     void *p = nullptr;
-    size_t need = (size_t)nrows * 16 + 64;
+    size_t need = (size_t)nrows * 12 + 64;
     mti_status r = scratch_get(ctx, SLOT_KEYS, need, &p);
     if (r != MTI_OK) return r;
-    uint32_t *k0 = (uint32_t *)p;
-    uint32_t *k1 = k0 + nrows;
-    int32_t *v1 = (int32_t *)(k1 + nrows);
-    unsigned long long *kept = (unsigned long long *)(((uintptr_t)(v1 + nrows) + 15) & ~uintptr_t(15));
+    uint32_t *tmp = (uint32_t *)p;
+    unsigned long long *kept = (unsigned long long *)(((uintptr_t)(tmp + 3 * nrows) + 15) & ~uintptr_t(15));
     // the value array in slot 0 is the caller's output buffer
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, kept, 1);
-    hipLaunchKernelGGL(k_baseline_keys, dim3((unsigned)ceil_div(nrows, 256)), dim3(256), 0, st, ant1,
-                       ant2, nrows, nant, k0, index_out, kept);
-    MTI_HIP(ctx, hipGetLastError());
-    uint32_t nk = (uint32_t)nant * (uint32_t)nant;
-    int bits = 1;
-    while (bits < 32 && (nk >> bits) != 0) bits++;
-    bool in_tmp = false;
-    r = radix_sort_pairs(ctx, k0, index_out, k1, v1, nrows, bits, st, &in_tmp);
+    r = sort_rows_device(ctx, ant1, ant2, nrows, nant, index_out, tmp, kept, st);
     if (r != MTI_OK) return r;
-    if (in_tmp)
-        MTI_HIP(ctx, hipMemcpyAsync(index_out, v1, (size_t)nrows * sizeof(int32_t),
-                                    hipMemcpyDeviceToDevice, st));
     unsigned long long h = 0;
     MTI_HIP(ctx, hipMemcpyAsync(&h, kept, sizeof(h), hipMemcpyDeviceToHost, st));
     MTI_HIP(ctx, hipStreamSynchronize(st));

@@ -1,4 +1,4 @@
-// mti_wkernel.hip -- convolution-function tables on the device (rows a8-a14).
+// mti_wkernel.hip -- convolution-function tables on the device.
 //
 // W-projection kernels (cfg/WProjectionConvFunction.cpp:186-291,
 // cfg/WPlaneConvFunctionCalculator.cpp:31-148):

@@ -65,7 +65,8 @@ class Job(ctypes.Structure):
     """mti_job: one imaging job of a batched call."""
     _fields_ = [("kernels", ctypes.c_void_p), ("params", ImageParams), ("vis", Vis),
                 ("chan0", ctypes.c_int32), ("nchan", ctypes.c_int32),
-                ("grids", ctypes.c_void_p), ("norm", ctypes.c_void_p), ("image", ctypes.c_void_p)]
+                ("grids", ctypes.c_void_p), ("norm", ctypes.c_void_p), ("image", ctypes.c_void_p),
+                ("ant1", ctypes.c_void_p), ("ant2", ctypes.c_void_p), ("nant", ctypes.c_int32)]
 
 
 class GridStats(ctypes.Structure):
@@ -252,9 +253,11 @@ class Context:
 
     # ---- batched jobs ---------------------------------------------------------
     @staticmethod
-    def make_job(kern, params, vis, grids, norm, image=None, chan0=0, nchan=1):
+    def make_job(kern, params, vis, grids, norm, image=None, chan0=0, nchan=1, ant1=None, ant2=None, nant=0):
         """An mti_job; grids complex128 [nchan] grids, norm f64 [nchan][npol],
-        image f32 [nchan][npol][ny][nx] (for finalize_jobs)."""
+        image f32 [nchan][npol][ny][nx] (for finalize_jobs).  With ant1/ant2
+        (int32 device tensors of vis's rows) and nant, grid_jobs computes the
+        baseline-major row order itself (vis must then have no row index)."""
         j = Job()
         j.kernels = kern.h
         j.params = params
@@ -262,7 +265,9 @@ class Context:
         j.chan0, j.nchan = int(chan0), int(nchan)
         j.grids, j.norm = grids.data_ptr(), norm.data_ptr()
         j.image = image.data_ptr() if image is not None else None
-        j._keep = (kern, vis, grids, norm, image)
+        if ant1 is not None:
+            j.ant1, j.ant2, j.nant = ant1.data_ptr(), ant2.data_ptr(), int(nant)
+        j._keep = (kern, vis, grids, norm, image, ant1, ant2)
         return j
 
     def grid_jobs(self, jobs, accumulate=False, psf=False, weighting="natural", robust=0.0,

@@ -138,6 +138,8 @@ class Server:
         self.gridded = 0
         self.errors = []
         self.wave_sizes = []
+        # per wave: requests, visibilities, pixels, host preparation and device seconds
+        self.wave_log = []
         self._cv = threading.Condition()
         self._queue = collections.deque()
         self._in_use = 0
@@ -169,6 +171,18 @@ class Server:
                     return False
                 self._cv.wait(left)
         return True
+
+    def cancel_queued(self):
+        """Drops the requests not yet admitted (their error is set); returns
+        how many.  Waves already running finish normally."""
+        with self._cv:
+            dropped = list(self._queue)
+            self._queue.clear()
+            self._pending -= len(dropped)
+            self._cv.notify_all()
+        for r in dropped:
+            r.error = MTIError("cancelled before admission")
+        return len(dropped)
 
     def close(self):
         """Stops the workers after their current wave (queued requests are dropped)."""
@@ -228,25 +242,45 @@ class Server:
         finally:
             ctx.close()
 
+    def _outputs(self, r, stream):
+        """A request's grids, normaliser and image.  They come from the
+        allocator's pool of the device's default stream and are recorded on
+        the worker's stream, so the workers share one pool of freed blocks
+        (separate per-stream pools would each keep their peak)."""
+        torch = self.torch
+        p = r.params
+        with torch.cuda.stream(torch.cuda.default_stream(self.device)):
+            out = (torch.empty((r.nchan, p.ny, p.nx, r.npol, 2), dtype=torch.float64, device="cuda"),
+                   torch.empty((r.nchan, r.npol), dtype=torch.float64, device="cuda"),
+                   torch.empty((r.nchan, r.npol, p.ny, p.nx), dtype=torch.float32, device="cuda"))
+        for t in out:
+            t.record_stream(stream)
+        return out
+
     def _run_wave(self, ctx, stream, wave):
         torch = self.torch
         sid = stream.cuda_stream
+        t0 = time.perf_counter()
         with torch.cuda.stream(stream):
             jobs, owned = [], []
             for r in wave:
-                idx = torch.empty(r.ant1.numel(), dtype=torch.int32, device="cuda")
-                kept = ctx.sort_rows(r.ant1, r.ant2, r.nant, idx, stream=sid)
-                v = vis_ms(*r.vis_args, row_index=idx[:kept])
+                if self.plain:
+                    idx = torch.empty(r.ant1.numel(), dtype=torch.int32, device="cuda")
+                    kept = ctx.sort_rows(r.ant1, r.ant2, r.nant, idx, stream=sid)
+                    v = vis_ms(*r.vis_args, row_index=idx[:kept])
+                else:
+                    # the batched call sorts the rows itself, with no host synchronisation
+                    v = vis_ms(*r.vis_args)
                 p = r.params
                 if self.cache is not None:
                     K = self.cache.get(ctx, p, sid)
                 else:
                     K = ctx.build_kernels(p, "f32", sid)
-                grids = torch.empty((r.nchan, p.ny, p.nx, r.npol, 2), dtype=torch.float64, device="cuda")
-                norm = torch.empty((r.nchan, r.npol), dtype=torch.float64, device="cuda")
-                image = torch.empty((r.nchan, r.npol, p.ny, p.nx), dtype=torch.float32, device="cuda")
-                jobs.append(ctx.make_job(K, p, v, grids, norm, image, r.chan0, r.nchan))
+                grids, norm, image = self._outputs(r, stream)
+                jobs.append(ctx.make_job(K, p, v, grids, norm, image, r.chan0, r.nchan,
+                                         ant1=r.ant1, ant2=r.ant2, nant=r.nant))
                 owned.append((r, K, v, grids, norm, image))
+            t1 = time.perf_counter()
             if self.plain:
                 r, K, v, grids, norm, image = owned[0]
                 st = ctx.grid(K, r.params, v, grids, norm, chan0=r.chan0, nchan=r.nchan,
@@ -257,8 +291,11 @@ class Server:
                 st = ctx.grid_jobs(jobs, weighting=self.weighting, robust=self.robust, stream=sid)
                 ctx.finalize_jobs(jobs, poltype=self.poltype, stream=sid)
             stream.synchronize()
+        t2 = time.perf_counter()
         with self._cv:
             self.gridded += st.gridded_records
+            self.wave_log.append((len(wave), sum(r.nvis for r in wave),
+                                  sum(r.params.nx * r.params.ny * r.nchan * r.npol for r in wave), t1 - t0, t2 - t1))
         if len(owned) == 1:
             owned[0][0].gridded = st.gridded_records
         for r, K, v, grids, norm, image in owned:

@@ -129,3 +129,14 @@ def test_kernel_cache_builds_once_under_concurrency():
     for t in ts:
         t.join()
     assert len(ctx.built) == 1 and all(k is out[0] for k in out)
+
+
+def test_cancel_queued_drops_unadmitted_requests():
+    a, b = _req(0, 256, 1, 10), _req(1, 256, 1, 10)
+    s = _server(a.nbytes + 10)
+    s.submit(a)
+    s.submit(b)
+    with s._cv:
+        assert [r.rid for r in s._take_wave()] == [0]
+    assert s.cancel_queued() == 1
+    assert b.error is not None and s._pending == 1 and not s._queue
