@@ -156,7 +156,7 @@ const char *mti_last_error(const mti_ctx *ctx);
 mti_status  mti_set_profiling(mti_ctx *ctx, int32_t enable);
 mti_status  mti_stage_times(mti_ctx *ctx, double *ms, int64_t *launches);
 
-/* ---- a8-a14: convolution functions --------------------------------------
+/* ---- convolution functions ----------------------------------------------
  * Replaces WProjectionConvFunction::calculate (cfg/WProjectionConvFunction.cpp:186-291)
  * with its operators casagridsf (op/CasaGridSF.cu:151-173), WProjPreFFTConvFunc
  * (op/WProjPreFFTconvFunc.cu:34-105), multiFFT2D + ZeroToCentre2DShift +
@@ -178,7 +178,7 @@ const void  *mti_kernels_table(const mti_kernels *k);
 const float *mti_kernels_sums(const mti_kernels *k);
 void         mti_kernels_free(mti_kernels *k);
 
-/* ---- a1: baseline-major row order ----------------------------------------
+/* ---- baseline-major row order --------------------------------------------
  * Replaces SortIndex::sortFuncThread (vis/SortIndex.cpp:46-87): stable order by
  * key Nant^2 - a1*Nant - a2 - 1, autocorrelations dropped.
  * index_out must hold nrows entries; *kept_out receives the count (blocks). */
@@ -186,7 +186,7 @@ mti_status mti_sort_rows(mti_ctx *ctx, const int32_t *ant1, const int32_t *ant2,
                          int64_t nrows, int32_t nant, int32_t *index_out,
                          int64_t *kept_out, void *stream);
 
-/* ---- a4: binning (PreProcessUVW) ------------------------------------------
+/* ---- binning (PreProcessUVW) ----------------------------------------------
  * Replaces WProjPreProcessUVW (op/WProjPreProcessUVW.cu:111-324) for one
  * channel: per record (row order of vis->row_index) writes
  *   bins[r] = {to_grid, plane, support, corner_x, corner_y, off_x, off_y, conj}
@@ -196,7 +196,7 @@ mti_status mti_preprocess(mti_ctx *ctx, const mti_kernels *k,
                           const mti_image_params *p, const mti_vis *v,
                           int32_t chan, int32_t *bins, void *stream);
 
-/* ---- a4-a7, a15, a21: grid channels ----------------------------------------
+/* ---- grid channels --------------------------------------------------------
  * Replaces the WImager DAG (comp/WImager.cpp:167-234: PreProcessUVW, the
  * AccumulatedSum/CreateIndexAndReorder/CreateBlockDataIndex/
  * NotSameSupportManipulate work plan, CreateCompressionPlan/
@@ -249,13 +249,13 @@ mti_status mti_grid_jobs(mti_ctx *ctx, int32_t njobs, const mti_job *jobs,
 mti_status mti_finalize_jobs(mti_ctx *ctx, int32_t njobs, const mti_job *jobs,
                              int32_t poltype, int32_t grid_layout, void *stream);
 
-/* ---- a17: 2-D inverse FFT (IFFT2D, gafw/StandardOperators/IFFT2D.cpp:86-111) --
+/* ---- 2-D inverse FFT (IFFT2D, gafw/StandardOperators/IFFT2D.cpp:86-111) --
  * Unnormalised C2C inverse of nbatch planes complex64 [nbatch][ny][nx], in
  * place.  nx, ny powers of two in [2, 8192]. */
 mti_status mti_fft2d_inverse(mti_ctx *ctx, float *data, int32_t nx, int32_t ny,
                              int32_t nbatch, void *stream);
 
-/* ---- a16-a19: finalize (ImageFinalizer, comp/ImageFinalizer.cpp:31-44) -----
+/* ---- finalize (ImageFinalizer, comp/ImageFinalizer.cpp:31-44) -----
  * PreIFFTUVCorrection + IFFT2D + ZeroToCentre2DShift + StokesConverter
  * (normalise, Stokes, m-flip) + casagridsf taper correction + SincCorrector.
  * The transform runs in FP64; the image is FP32.

@@ -179,14 +179,14 @@ private:
 
 // Creates the operators of the imaging path by the reference's names
 // (ImagerFactoryHelper::createOperator); unknown names give nullptr.
-//   "SortIndex"           a1  antenna1/2 int [N] -> int index [N], real_longint [1] kept
-//   "PreProcessUVW"       a4  WImager inputs -> int [N][8] bins
-//   "ConvolutionGridder"  a4-a7, a15, a21: the WImager gridding DAG
+//   "SortIndex"           antenna1/2 int [N] -> int index [N], real_longint [1] kept
+//   "PreProcessUVW"       WImager inputs -> int [N][8] bins
+//   "ConvolutionGridder"  the WImager gridding DAG
 //                         inputs UVW [3][N] f32, VIS [N][2P] f32, FREQUENCY [N] f32,
 //                         WEIGHT [N][P] f32, FLAGS [N][P] int;
 //                         outputs Grid complex_float [v][u][P], NormalizerValue [P]
-//   "IFFT2D"              a17 complex_float [..][ny][nx], unnormalised inverse
-//   "ImageFinalizer"      a16-a19: UVImage complex_float [v][u][P] + NormalizerValue
+//   "IFFT2D"              complex_float [..][ny][nx], unnormalised inverse
+//   "ImageFinalizer"      UVImage complex_float [v][u][P] + NormalizerValue
 //                         -> real_float [P][v][u] (Stokes)
 class ImagerFactoryHelper {
 public:

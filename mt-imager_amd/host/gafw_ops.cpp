@@ -191,7 +191,7 @@ int weighting_of(const Properties &p)
 
 // ---- operators ----------------------------------------------------------------
 
-// a1 (vis/SortIndex.cpp:46-87): inputs ANTENNA1, ANTENNA2 int [N]; output 0
+// SortIndex (vis/SortIndex.cpp:46-87): inputs ANTENNA1, ANTENNA2 int [N]; output 0
 // int [N] (the first `kept` entries valid), output 1 real_longint [1] = kept;
 // property nant.  Blocks for the kept count like the reference's host sort.
 class SortIndexOp : public ArrayOperator {
@@ -227,7 +227,7 @@ private:
     int64_t kept_ = 0;
 };
 
-// a4 (op/WProjPreProcessUVW.cu:111-324): WImager inputs -> int [N][8]
+// PreProcessUVW (op/WProjPreProcessUVW.cu:111-324): WImager inputs -> int [N][8]
 // {to_grid, plane, support, corner_x, corner_y, off_x, off_y, conj}.
 class PreProcessUVWOp : public ArrayOperator {
 public:
@@ -252,7 +252,7 @@ public:
     }
 };
 
-// a4-a7, a15, a21: the WImager gridding DAG (comp/WImager.cpp:167-234).
+// The WImager gridding DAG (comp/WImager.cpp:167-234).
 // Output 0 Grid complex_float [v][u][P] (or complex_double), output 1
 // NormalizerValue [P] real_float (or real_double).  Properties: psf,
 // initialize_output (default true; false adds to the outputs like the
@@ -311,7 +311,7 @@ public:
     }
 };
 
-// a17 (std/IFFT2D.cpp:86-111): unnormalised inverse C2C over the last two
+// IFFT2D (std/IFFT2D.cpp:86-111): unnormalised inverse C2C over the last two
 // dimensions of a complex_float array; out-of-place calls copy first.
 class IFFT2DOp : public ArrayOperator {
 public:
@@ -343,7 +343,7 @@ public:
     }
 };
 
-// a16-a19 (comp/ImageFinalizer.cpp:31-44): UVImage [v][u][P] (complex_float or
+// ImageFinalizer (comp/ImageFinalizer.cpp:31-44): UVImage [v][u][P] (complex_float or
 // complex_double) + NormalizerValue [P] (real_float or real_double) ->
 // real_float [P][v][u]; property PolarizationType (linear | circular).
 class ImageFinalizerOp : public ArrayOperator {

@@ -215,7 +215,7 @@ class Context:
                                   int(chan), bins_out.data_ptr(), _stream(stream))
         self.check(rc, "mti_preprocess")
 
-    # ---- a4-a7, a15, a21 --------------------------------------------------
+    # ---- gridding ---------------------------------------------------------
     def grid(self, kern, params, vis, grids, norm, chan0=0, nchan=1, accumulate=False,
              psf=False, weighting="natural", robust=0.0, layout="vup", precision=None,
              stats=True, stream=None):

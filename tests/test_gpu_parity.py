@@ -136,7 +136,7 @@ def test_binning_half_integer_ties_random(orc, mti, ctx):
     np.testing.assert_array_equal(bins.cpu().numpy(), exp)
 
 
-# ---- a4-a7, a15-a19: images ----------------------------------------------------
+# ---- images ------------------------------------------------------------------
 IMAGE_RMS_TOL = 1e-4   # relative RMS vs the FP64 oracle
 GRID_RMS_TOL = 1e-5
 
