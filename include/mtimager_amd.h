@@ -18,6 +18,7 @@
  */
 #ifndef MTIMAGER_AMD_H
 #define MTIMAGER_AMD_H
+#include <stddef.h>
 #include <stdint.h>
 #ifdef __cplusplus
 extern "C" {
@@ -268,6 +269,49 @@ mti_status mti_finalize(mti_ctx *ctx, const mti_image_params *p, int32_t npol,
                         int32_t poltype, int32_t nchan, int32_t grid_layout,
                         int32_t grid_precision, const void *grids,
                         const double *norm, float *image, void *stream);
+
+/* ---- output shapes and scratch sizes (validate) ----------------------------
+ * The validate() half of the reference's operator contract
+ * (GPUArrayOperator::validate, gpu/GPUArrayOperator.h:27-43): output
+ * dimensions, and the scratch bytes a call takes (ValidationData.bufferSize,
+ * gpu/ValidationData.h:25-34), from parameters and sizes only -- no context
+ * and no device.  Dimensions are row-major, last fastest; grid elements are
+ * complex.  MTI_ERR_INVALID when the same arguments would fail validation. */
+mti_status mti_grid_out_shape(const mti_image_params *p, int32_t npol, int32_t nchan,
+                              int32_t grid_layout, int64_t grid_dims[4], int64_t norm_dims[2]);
+mti_status mti_finalize_out_shape(const mti_image_params *p, int32_t npol, int32_t nchan,
+                                  int64_t image_dims[4]);
+mti_status mti_preprocess_out_shape(int64_t nrecords, int64_t bins_dims[2]);
+
+/* Device workspace bytes a call with the same arguments takes.  Pointers in
+ * the argument structs are not read (only counts, sizes and parameters; the
+ * job variants read each job's kernel-table description). */
+mti_status mti_sort_rows_workspace_size(int64_t nrows, size_t *bytes);
+mti_status mti_grid_workspace_size(const mti_kernel_info *kernels, const mti_image_params *p,
+                                   const mti_vis *v, const mti_grid_opts *opts, size_t *bytes);
+mti_status mti_grid_jobs_workspace_size(int32_t njobs, const mti_job *jobs,
+                                        const mti_grid_opts *opts, size_t *bytes);
+mti_status mti_finalize_workspace_size(const mti_image_params *p, int32_t npol, int32_t nchan,
+                                       size_t *bytes);
+mti_status mti_finalize_jobs_workspace_size(int32_t njobs, const mti_job *jobs, size_t *bytes);
+mti_status mti_fft2d_workspace_size(int32_t nx, int32_t ny, int32_t nbatch, size_t *bytes);
+
+/* Caller-owned scratch, as the reference's engine hands each operator the
+ * buffer its validate() asked for (GPUSubmissionData.bufferGPUPointer,
+ * gpu/GPUSubmissionData.h:27-47).  After mti_set_workspace(ctx, ptr, bytes),
+ * every mti_sort_rows, mti_grid, mti_grid_jobs, mti_finalize,
+ * mti_finalize_jobs and mti_fft2d_inverse call on ctx lays its scratch out in
+ * [ptr, ptr + bytes) and allocates no device memory; a call that needs more
+ * than `bytes` (its *_workspace_size) fails with MTI_ERR_NOMEM before any
+ * work is enqueued.  ptr = NULL with bytes = 0 returns to scratch owned by the
+ * context.  Calls sharing a workspace must be ordered (one stream, or
+ * synchronised).  mti_kernels_build stays outside: its tables and scratch are
+ * sized by the supports it finds on the device (the reference's UnknownSize
+ * buffer mode) and come from the context. */
+mti_status mti_set_workspace(mti_ctx *ctx, void *ptr, size_t bytes);
+/* Device bytes the context itself holds: scratch it allocated and its cached
+ * FFT twiddle tables. */
+mti_status mti_context_device_bytes(const mti_ctx *ctx, size_t *bytes);
 
 #ifdef __cplusplus
 }

@@ -38,6 +38,48 @@ extern "C" void mti_destroy(mti_ctx *ctx)
     delete ctx;
 }
 
+extern "C" mti_status mti_set_workspace(mti_ctx *ctx, void *ptr, size_t bytes)
+{
+    if (!ctx) return MTI_ERR_INVALID;
+    MTI_CHECK(ctx, ctx->call_depth == 0, "mti_set_workspace during a call");
+    MTI_CHECK(ctx, (ptr == nullptr) == (bytes == 0), "workspace pointer and size must be set (or cleared) together");
+    ctx->ws_ptr = (char *)ptr;
+    ctx->ws_bytes = bytes;
+    return MTI_OK;
+}
+
+extern "C" mti_status mti_context_device_bytes(const mti_ctx *ctx, size_t *bytes)
+{
+    if (!ctx || !bytes) return MTI_ERR_INVALID;
+    size_t t = 0;
+    for (const auto &s : ctx->scratch) t += s.bytes;
+    for (const auto &tw : ctx->twiddles) t += (size_t)tw.first * sizeof(double2);
+    *bytes = t;
+    return MTI_OK;
+}
+
+extern "C" mti_status mti_grid_out_shape(const mti_image_params *p, int32_t npol, int32_t nchan, int32_t grid_layout,
+                                         int64_t grid_dims[4], int64_t norm_dims[2])
+{
+    if (!p || !grid_dims || !norm_dims || mti::validate_params(nullptr, p) != MTI_OK ||
+        !(npol == 1 || npol == 2 || npol == 4) || nchan < 1 ||
+        !(grid_layout == MTI_GRID_VUP || grid_layout == MTI_GRID_PVU))
+        return MTI_ERR_INVALID;
+    const int64_t vup[4] = {nchan, p->ny, p->nx, npol}, pvu[4] = {nchan, npol, p->ny, p->nx};
+    for (int i = 0; i < 4; i++) grid_dims[i] = grid_layout == MTI_GRID_VUP ? vup[i] : pvu[i];
+    norm_dims[0] = nchan;
+    norm_dims[1] = npol;
+    return MTI_OK;
+}
+
+extern "C" mti_status mti_preprocess_out_shape(int64_t nrecords, int64_t bins_dims[2])
+{
+    if (!bins_dims || nrecords < 0 || nrecords >= (int64_t(1) << 31)) return MTI_ERR_INVALID;
+    bins_dims[0] = nrecords;
+    bins_dims[1] = 8;
+    return MTI_OK;
+}
+
 extern "C" const char *mti_last_error(const mti_ctx *ctx)
 {
     return ctx ? ctx->last_error.c_str() : "null context";
@@ -225,14 +267,22 @@ static mti_status scan_rec(mti_ctx *ctx, uint32_t *d, int64_t n, uint32_t *scrat
     return MTI_OK;
 }
 
+// scratch words of a scan of n: one for the original last element + tile
+// sums per level
+static int64_t scan_words(int64_t n)
+{
+    int64_t need = 2;
+    for (int64_t m = ceil_div(n, SCAN_TILE); m > 1; m = ceil_div(m, SCAN_TILE)) need += m + 1;
+    return need;
+}
+
+void plan_scan(int64_t n, ScratchPlan &pl) { pl.need(SLOT_MISC, (size_t)scan_words(n) * sizeof(uint32_t)); }
+
 mti_status exclusive_scan_u32(mti_ctx *ctx, uint32_t *d_data, int64_t n, uint32_t *d_total,
                               hipStream_t st)
 {
-    // scratch: one word for the original last element + tile sums per level
-    int64_t need = 2;
-    for (int64_t m = ceil_div(n, SCAN_TILE); m > 1; m = ceil_div(m, SCAN_TILE)) need += m + 1;
     void *p = nullptr;
-    mti_status r = scratch_get(ctx, SLOT_MISC, (size_t)need * sizeof(uint32_t), &p);
+    mti_status r = scratch_get(ctx, SLOT_MISC, (size_t)scan_words(n) * sizeof(uint32_t), &p);
     if (r != MTI_OK) return r;
     uint32_t *w = (uint32_t *)p;
     hipLaunchKernelGGL(k_copy_last, dim3(1), dim3(64), 0, st, d_data, n, w);

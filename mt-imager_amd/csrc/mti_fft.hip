@@ -284,6 +284,17 @@ static mti_status twiddle_table(mti_ctx *ctx, int NT, const double2 **out, hipSt
         double a = 2.0 * M_PI * (double)t / (double)NT;
         h[t] = make_double2(std::cos(a), std::sin(a));
     }
+    if (ctx->ws_active) {
+        // a call in a caller workspace allocates nothing: the table goes to
+        // its twiddle slot for this call (same values as the cached tables)
+        void *w = nullptr;
+        mti_status r = scratch_get(ctx, SLOT_TWIDDLE, (size_t)NT * sizeof(double2), &w);
+        if (r != MTI_OK) return r;
+        MTI_HIP(ctx, hipMemcpyAsync(w, h.data(), NT * sizeof(double2), hipMemcpyHostToDevice, st));
+        MTI_HIP(ctx, hipStreamSynchronize(st));
+        *out = (const double2 *)w;
+        return MTI_OK;
+    }
     double2 *d = nullptr;
     MTI_HIP(ctx, hipMalloc(&d, NT * sizeof(double2)));
     MTI_HIP(ctx, hipMemcpyAsync(d, h.data(), NT * sizeof(double2), hipMemcpyHostToDevice, st));
@@ -932,6 +943,69 @@ static mti_status finalize_channels(mti_ctx *ctx, const FftGeom &g, const std::v
     return MTI_OK;
 }
 
+// ---- scratch plans (the requests of each entry point, largest per slot) ----
+static void plan_twiddles(int NT, ScratchPlan &pl) { pl.need(SLOT_TWIDDLE, (size_t)NT * sizeof(double2)); }
+
+static void plan_finalize_channels(const FftGeom &g, int npk, ScratchPlan &pl)
+{
+    plan_twiddles(g.NT, pl);
+    pl.need(SLOT_CHANDESC, (size_t)std::max(1, npk) * sizeof(PackDesc));
+    pl.need(SLOT_FFT, (size_t)g.nx * g.ny * sizeof(double2) * transform_batch(g, npk));
+}
+
+// packed planes of nchan channels of P correlations (one correlation: two
+// channels per pack, all of them sharing one correction here)
+static int pack_count(int P, int nchan) { return P == 1 ? (nchan + 1) / 2 : P == 2 ? nchan : 2 * nchan; }
+
+static void plan_finalize(const mti_image_params *p, int npol, int nchan, ScratchPlan &pl)
+{
+    const FftGeom g = geom(p->nx, p->ny);
+    plan_twiddles(g.NT, pl);
+    pl.need(SLOT_FCORR, (size_t)(p->nx + p->ny + 128) * 4);
+    if (pk_ok(g)) {
+        plan_finalize_channels(g, pack_count(npol, nchan), pl);
+    } else {
+        const int batch = std::max(1, transform_batch(g, npol * nchan) / npol);
+        pl.need(SLOT_FFT, (size_t)p->nx * p->ny * sizeof(double2) * npol * batch);
+    }
+}
+
+static void plan_fft2d(const FftGeom &g, int nbatch, ScratchPlan &pl)
+{
+    plan_twiddles(g.NT, pl);
+    pl.need(SLOT_FFT, (size_t)nbatch * g.nx * g.ny * sizeof(float2));
+}
+
+static void plan_finalize_jobs(int32_t njobs, const mti_job *jobs, int P, ScratchPlan &pl)
+{
+    std::vector<std::array<int, 3>> ckey;
+    size_t ctot = 0;
+    std::vector<std::pair<int, int>> sizes;
+    for (int i = 0; i < njobs; i++) {
+        const mti_image_params &p = jobs[i].params;
+        const std::array<int, 3> key{p.nx, p.ny, p.sampling};
+        if (std::find(ckey.begin(), ckey.end(), key) == ckey.end()) {
+            ckey.push_back(key);
+            ctot += (size_t)p.nx + p.ny + 64;
+        }
+        const std::pair<int, int> sz{p.nx, p.ny};
+        if (std::find(sizes.begin(), sizes.end(), sz) == sizes.end()) sizes.push_back(sz);
+    }
+    pl.need(SLOT_CORR, ctot * sizeof(float));
+    for (auto sz : sizes) {
+        const FftGeom g = geom(sz.first, sz.second);
+        int nch = 0;
+        for (int i = 0; i < njobs; i++)
+            if (jobs[i].params.nx == sz.first && jobs[i].params.ny == sz.second) {
+                nch += jobs[i].nchan;
+                if (!pk_ok(g)) plan_finalize(&jobs[i].params, P, jobs[i].nchan, pl);
+            }
+        // one correlation: at most one pack per channel (channels pair only
+        // within a correction group)
+        if (pk_ok(g)) plan_finalize_channels(g, P == 1 ? nch : pack_count(P, nch), pl);
+    }
+}
+
 }  // namespace mti
 
 using namespace mti;
@@ -946,8 +1020,15 @@ extern "C" mti_status mti_fft2d_inverse(mti_ctx *ctx, float *data, int32_t nx, i
     MTI_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = to_stream(stream);
     FftGeom g = geom(nx, ny);
+    mti_status r;
+    CallScope cs(ctx);
+    if (cs.planning()) {
+        ScratchPlan pl;
+        plan_fft2d(g, nbatch, pl);
+        if ((r = cs.begin(pl)) != MTI_OK) return r;
+    }
     const double2 *tw = nullptr;
-    mti_status r = twiddle_table(ctx, g.NT, &tw, st);
+    r = twiddle_table(ctx, g.NT, &tw, st);
     if (r != MTI_OK) return r;
     float2 *X = (float2 *)data;
     r = rows_pass<1, float2, float2>(ctx, g, X, X, 1, nbatch, tw, st);
@@ -990,6 +1071,12 @@ extern "C" mti_status mti_finalize(mti_ctx *ctx, const mti_image_params *p, int3
     MTI_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = to_stream(stream);
     FftGeom g = geom(p->nx, p->ny);
+    CallScope cs(ctx);
+    if (cs.planning()) {
+        ScratchPlan pl;
+        plan_finalize(p, npol, nchan, pl);
+        if ((r = cs.begin(pl)) != MTI_OK) return r;
+    }
     const double2 *tw = nullptr;
     r = twiddle_table(ctx, g.NT, &tw, st);
     if (r != MTI_OK) return r;
@@ -1065,6 +1152,13 @@ extern "C" mti_status mti_finalize_jobs(mti_ctx *ctx, int32_t njobs, const mti_j
     MTI_CHECK(ctx, P == 1 || P == 2 || P == 4, "npol must be 1, 2 or 4");
     MTI_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = to_stream(stream);
+    CallScope cs(ctx);
+    if (cs.planning()) {
+        ScratchPlan pl;
+        plan_finalize_jobs(njobs, jobs, P, pl);
+        mti_status rp = cs.begin(pl);
+        if (rp != MTI_OK) return rp;
+    }
     // correction vectors, once per distinct (nx, ny, sampling): they depend on
     // nothing else, and jobs that share them can share packed planes
     std::vector<std::array<int, 3>> ckey;
@@ -1123,5 +1217,58 @@ extern "C" mti_status mti_finalize_jobs(mti_ctx *ctx, int32_t njobs, const mti_j
         r = finalize_channels(ctx, g, ch, P, poltype, grid_layout, false, st);
         if (r != MTI_OK) return r;
     }
+    return MTI_OK;
+}
+
+extern "C" mti_status mti_fft2d_workspace_size(int32_t nx, int32_t ny, int32_t nbatch, size_t *bytes)
+{
+    if (!bytes || !is_pow2(nx) || !is_pow2(ny) || nx > 8192 || ny > 8192 || nbatch < 1) return MTI_ERR_INVALID;
+    ScratchPlan pl;
+    plan_fft2d(geom(nx, ny), nbatch, pl);
+    *bytes = pl.total();
+    return MTI_OK;
+}
+
+extern "C" mti_status mti_finalize_workspace_size(const mti_image_params *p, int32_t npol, int32_t nchan,
+                                                  size_t *bytes)
+{
+    if (!bytes || !p || validate_params(nullptr, p) != MTI_OK || check_finalize_size(nullptr, p) != MTI_OK ||
+        !(npol == 1 || npol == 2 || npol == 4) || nchan < 1)
+        return MTI_ERR_INVALID;
+    ScratchPlan pl;
+    plan_finalize(p, npol, nchan, pl);
+    *bytes = pl.total();
+    return MTI_OK;
+}
+
+extern "C" mti_status mti_finalize_jobs_workspace_size(int32_t njobs, const mti_job *jobs, size_t *bytes)
+{
+    if (!bytes || njobs < 0 || (njobs > 0 && !jobs)) return MTI_ERR_INVALID;
+    if (njobs == 0) {
+        *bytes = ScratchPlan().total();
+        return MTI_OK;
+    }
+    const int P = jobs[0].vis.npol;
+    if (!(P == 1 || P == 2 || P == 4)) return MTI_ERR_INVALID;
+    for (int i = 0; i < njobs; i++)
+        if (validate_params(nullptr, &jobs[i].params) != MTI_OK || check_finalize_size(nullptr, &jobs[i].params) != MTI_OK ||
+            jobs[i].vis.npol != P || jobs[i].nchan < 1)
+            return MTI_ERR_INVALID;
+    ScratchPlan pl;
+    plan_finalize_jobs(njobs, jobs, P, pl);
+    *bytes = pl.total();
+    return MTI_OK;
+}
+
+extern "C" mti_status mti_finalize_out_shape(const mti_image_params *p, int32_t npol, int32_t nchan,
+                                             int64_t image_dims[4])
+{
+    if (!p || !image_dims || validate_params(nullptr, p) != MTI_OK || check_finalize_size(nullptr, p) != MTI_OK ||
+        !(npol == 1 || npol == 2 || npol == 4) || nchan < 1)
+        return MTI_ERR_INVALID;
+    image_dims[0] = nchan;
+    image_dims[1] = npol;
+    image_dims[2] = p->ny;
+    image_dims[3] = p->nx;
     return MTI_OK;
 }

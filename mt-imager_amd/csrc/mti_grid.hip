@@ -698,6 +698,47 @@ static int64_t job_keys(const PassJob &j)
     return (int64_t)j.nb * ((j.p.nx + TG_T - 1) / TG_T) * ((j.p.ny + TG_T - 1) / TG_T);
 }
 
+// Scratch of one gridding pass over a set of job pieces; grid_pass lays its
+// buffers out from these sizes and plan_pass asks for the same.
+struct PassSizes {
+    int64_t nrec = 0, nkeys = 0, nch = 0, pair_cap = 0, unit_cap = 0;
+    size_t u32n = 0, u32k = 0, pre = 0, tab = 0, wv_bytes = 0, wt_bytes = 0;
+};
+
+static PassSizes pass_sizes(const std::vector<PassJob> &jobs, int P)
+{
+    PassSizes z;
+    const int nj = (int)jobs.size();
+    for (const PassJob &j : jobs) {
+        z.nrec += job_records(j);
+        z.nkeys += job_keys(j);
+        z.nch += j.nb;
+        z.pair_cap += job_pairs(j);
+    }
+    z.unit_cap = z.nkeys + z.pair_cap / TG_CHUNK + 1;
+    z.u32n = align_up((size_t)z.nrec * 4);
+    z.u32k = align_up((size_t)z.nkeys * 4);
+    z.pre = align_up((size_t)(nj + 1) * 8);
+    z.tab = align_up((size_t)nj * sizeof(JobDev)) + 4 * z.pre + align_up((size_t)z.nch * P * 3 * sizeof(double));
+    z.wv_bytes = align_up((size_t)z.nrec * P * sizeof(float2));
+    z.wt_bytes = align_up((size_t)z.nrec * P * sizeof(float));
+    return z;
+}
+
+static void plan_pass(const std::vector<PassJob> &jobs, int P, bool weighted, ScratchPlan &pl)
+{
+    const PassSizes z = pass_sizes(jobs, P);
+    if (jobs.empty() || z.nrec == 0) return;
+    pl.need(SLOT_KEYS, z.u32n + 4096);
+    pl.need(SLOT_DESC, align_up((size_t)z.nrec * 16));
+    pl.need(SLOT_BUCKETS, z.u32k * 3);
+    pl.need(SLOT_ITEMS, align_up((size_t)z.pair_cap * 4) + align_up((size_t)z.unit_cap * 16));
+    pl.need(SLOT_CRECVIS, z.wv_bytes + z.wt_bytes);
+    pl.need(SLOT_NORM, z.tab);
+    if (weighted) pl.need(SLOT_DENSITY, z.wt_bytes);
+    plan_scan(z.nkeys, pl);
+}
+
 // Grids a set of jobs (the same npol and table precision) into their FP64
 // grids (zeroed or holding earlier chunks) and accumulates their normalisers.
 // nu_total[0] += work units, [1] += (record, tile) pairs.
@@ -740,7 +781,8 @@ static mti_status grid_pass(mti_ctx *ctx, const std::vector<PassJob> &jobs, cons
         d.rec0 = rec0[i];
         d.key0 = key0[i];
     }
-    const int64_t nrec = rec0[nj], nkeys = key0[nj], nch = jc0[nj];
+    const PassSizes z = pass_sizes(jobs, P);
+    const int64_t nrec = z.nrec, nkeys = z.nkeys, nch = z.nch;
     if (nrec == 0) {
         // nothing to grid: a grid that was not pre-filled is all zeros
         if (fresh)
@@ -752,14 +794,11 @@ static mti_status grid_pass(mti_ctx *ctx, const std::vector<PassJob> &jobs, cons
     }
     MTI_CHECK(ctx, nrec < (int64_t(1) << 31) && nkeys < (int64_t(1) << 31) && pair_cap < (int64_t(1) << 32),
               "gridding pass above the record/tile limits");
-    const int64_t unit_cap = nkeys + pair_cap / TG_CHUNK + 1;
+    const int64_t unit_cap = z.unit_cap;
     mti_status r;
-    // scratch
-    const size_t u32n = align_up((size_t)nrec * 4), u32k = align_up((size_t)nkeys * 4);
-    const size_t pre = align_up((size_t)(nj + 1) * 8);
-    const size_t tab = align_up((size_t)nj * sizeof(JobDev)) + 4 * pre + align_up((size_t)nch * P * 3 * sizeof(double));
-    const size_t wv_bytes = align_up((size_t)nrec * P * sizeof(float2));
-    const size_t wt_bytes = align_up((size_t)nrec * P * sizeof(float));
+    // scratch (the sizes plan_pass asks for)
+    const size_t u32n = z.u32n, u32k = z.u32k, pre = z.pre, tab = z.tab;
+    const size_t wv_bytes = z.wv_bytes, wt_bytes = z.wt_bytes;
     void *w1 = nullptr, *w2 = nullptr, *w3 = nullptr, *w4 = nullptr, *w6 = nullptr, *w7 = nullptr;
     if ((r = scratch_get(ctx, SLOT_KEYS, u32n + 4096, &w1)) != MTI_OK) return r;
     if ((r = scratch_get(ctx, SLOT_DESC, align_up((size_t)nrec * 16), &w2)) != MTI_OK) return r;
@@ -958,9 +997,125 @@ static int64_t channels_per_pass(const PassJob &j)
     return std::min<int64_t>(((int64_t(1) << 32) - 1) / pc, ((int64_t(1) << 31) - 1) / std::max<int64_t>(1, j.vw.nidx));
 }
 
+// Passes of a batched call: consecutive job pieces while the pass stays
+// within its limits (a job too large for one pass is split by channel).
+template <class F>
+static mti_status for_each_pass(const std::vector<PassJob> &all, int P, F fn)
+{
+    std::vector<PassJob> pass;
+    int64_t in_r = 0, in_p = 0, in_k = 0;
+    for (const PassJob &j : all) {
+        const int64_t cap = channels_per_pass(j);
+        const int64_t gch = (int64_t)j.p.nx * j.p.ny * P;  // complex elements per channel grid
+        for (int c0 = 0; c0 < j.nb;) {
+            PassJob piece = j;
+            piece.chan0 = j.chan0 + c0;
+            piece.nb = (int)std::min<int64_t>(j.nb - c0, cap);
+            piece.grid = j.grid ? j.grid + 2 * gch * c0 : nullptr;
+            piece.norm = j.norm ? j.norm + (size_t)c0 * P : nullptr;
+            if (!pass.empty() &&
+                (in_r + job_records(piece) >= (int64_t(1) << 31) || in_p + job_pairs(piece) >= (int64_t(1) << 32) ||
+                 in_k + job_keys(piece) >= (int64_t(1) << 31) || pass.size() >= 32767)) {
+                mti_status r = fn(pass);
+                if (r != MTI_OK) return r;
+                pass.clear();
+                in_r = in_p = in_k = 0;
+            }
+            pass.push_back(piece);
+            in_r += job_records(piece);
+            in_p += job_pairs(piece);
+            in_k += job_keys(piece);
+            c0 += piece.nb;
+        }
+    }
+    return pass.empty() ? MTI_OK : fn(pass);
+}
+
+// The scratch plan of one mti_grid_jobs call: statistics, the in-call row
+// sorts and every pass.
+static void plan_grid_jobs(const std::vector<PassJob> &all, int32_t njobs, const mti_job *jobs, int P, bool weighted,
+                           ScratchPlan &pl)
+{
+    pl.need(SLOT_MISC2, 2 * BIN_STAT_STRIPES * sizeof(unsigned long long));
+    int64_t sort_rows_total = 0;
+    size_t maxrows = 0;
+    for (int i = 0; i < njobs; i++)
+        if (jobs[i].ant1) {
+            sort_rows_total += jobs[i].vis.nrows;
+            maxrows = std::max<size_t>(maxrows, (size_t)jobs[i].vis.nrows);
+            plan_sort_rows_device(jobs[i].vis.nrows, pl);
+        }
+    if (sort_rows_total > 0) {
+        pl.need(SLOT_RECVIS, align_up((size_t)sort_rows_total * 4) + align_up((size_t)njobs * 8));
+        pl.need(SLOT_KEYS, maxrows * 12 + 4096);
+    }
+    for_each_pass(all, P, [&](const std::vector<PassJob> &pass) {
+        plan_pass(pass, P, weighted, pl);
+        return MTI_OK;
+    });
+}
+
+// Channel batches of one mti_grid call: passes of at most *pair_batch
+// channels (record / pair limits); complex64 output goes through an FP64
+// work grid of *batch channels (at most 8 GB).
+static void grid_batches(const PassJob &base, int nb, int P, const mti_grid_opts *o, int *pair_batch, int *batch)
+{
+    *pair_batch = (int)std::min<int64_t>(nb, std::max<int64_t>(1, channels_per_pass(base)));
+    *batch = *pair_batch;
+    if (o->grid_precision == MTI_GRID_C64) {
+        const size_t chan_bytes = (size_t)base.p.nx * base.p.ny * P * 2 * sizeof(double);
+        const int b = (int)std::max<size_t>(1, std::min<size_t>((size_t)nb, (size_t(8) << 30) / chan_bytes));
+        *batch = std::min(b, *pair_batch);
+    }
+}
+
+// The scratch plan of one mti_grid call (the same batches and passes).
+static void plan_grid(const PassJob &base, int nb, int P, const mti_grid_opts *o, ScratchPlan &pl)
+{
+    pl.need(SLOT_MISC2, 2 * BIN_STAT_STRIPES * sizeof(unsigned long long));
+    int pair_batch, batch;
+    grid_batches(base, nb, P, o, &pair_batch, &batch);
+    const bool weighted = o->weighting != MTI_WEIGHT_NATURAL;
+    const int step = o->grid_precision == MTI_GRID_C128 ? pair_batch : batch;
+    if (o->grid_precision == MTI_GRID_C64)
+        pl.need(SLOT_GRID64, (size_t)base.p.nx * base.p.ny * P * 2 * sizeof(double) * batch);
+    for (int c0 = 0; c0 < nb && base.vw.nidx > 0; c0 += step) {
+        PassJob j = base;
+        j.nb = std::min(step, nb - c0);
+        plan_pass({j}, P, weighted, pl);
+    }
+}
+
+// Sizes-only checks of a workspace query (pointers are not needed).
+static mti_status check_sizes(const mti_image_params *p, const mti_vis *v, int32_t chan0, int32_t nchan)
+{
+    if (!p || !v || validate_params(nullptr, p) != MTI_OK) return MTI_ERR_INVALID;
+    if (!(v->npol == 1 || v->npol == 2 || v->npol == 4) || v->nrows < 0 || v->nrows >= (int64_t(1) << 31) ||
+        v->nchan < 1 || (v->row_index && (v->nindex < 0 || v->nindex > v->nrows)))
+        return MTI_ERR_INVALID;
+    if (nchan < 1 || nchan > 65535 || (!v->inv_lambda_per_row && (chan0 < 0 || chan0 + nchan > v->nchan)))
+        return MTI_ERR_INVALID;
+    if ((int64_t)p->nx * p->ny >= (int64_t(1) << 30)) return MTI_ERR_INVALID;
+    return MTI_OK;
+}
+
 }  // namespace mti
 
 using namespace mti;
+
+extern "C" mti_status mti_grid_workspace_size(const mti_kernel_info *ki, const mti_image_params *p, const mti_vis *v,
+                                              const mti_grid_opts *o, size_t *bytes)
+{
+    if (!ki || !bytes || !o || check_opts(nullptr, o) != MTI_OK || check_sizes(p, v, o->chan0, o->nchan) != MTI_OK)
+        return MTI_ERR_INVALID;
+    mti_kernels kk;
+    kk.info = *ki;
+    const PassJob base{&kk, *p, make_view(v), o->chan0, 1, nullptr, o->grid_layout, nullptr};
+    ScratchPlan pl;
+    plan_grid(base, o->nchan, v->npol, o, pl);
+    *bytes = pl.total();
+    return MTI_OK;
+}
 
 extern "C" mti_status mti_grid(mti_ctx *ctx, const mti_kernels *k, const mti_image_params *p,
                                const mti_vis *v, const mti_grid_opts *o, void *grids, double *norm,
@@ -985,6 +1140,12 @@ extern "C" mti_status mti_grid(mti_ctx *ctx, const mti_kernels *k, const mti_ima
     const int64_t chan_floats = gsz * P * 2;  // reals per channel grid
     VisView vw = make_view(v);
     MTI_CHECK(ctx, vw.nidx < (int64_t(1) << 31), "more than 2^31-1 rows per call");
+    CallScope cs(ctx);
+    if (cs.planning()) {
+        ScratchPlan pl;
+        plan_grid(PassJob{k, *p, vw, o->chan0, 1, nullptr, o->grid_layout, nullptr}, nb, P, o, pl);
+        if ((r = cs.begin(pl)) != MTI_OK) return r;
+    }
 
     if (!o->accumulate) {
         hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(256), 0, st, norm, (int64_t)nb * P, 0.0);
@@ -1003,9 +1164,9 @@ extern "C" mti_status mti_grid(mti_ctx *ctx, const mti_kernels *k, const mti_ima
 
     // channel batches keep a pass within the record / pair limits
     const PassJob base{k, *p, vw, o->chan0, 1, nullptr, o->grid_layout, nullptr};
-    const int64_t cap_chans = channels_per_pass(base);
-    MTI_CHECK(ctx, cap_chans >= 1, "one channel has too many (record, tile) pairs");
-    const int pair_batch = (int)std::min<int64_t>(nb, cap_chans);
+    MTI_CHECK(ctx, channels_per_pass(base) >= 1, "one channel has too many (record, tile) pairs");
+    int pair_batch, batch;
+    grid_batches(base, nb, P, o, &pair_batch, &batch);
     const int prec = k->info.precision;
 
     if (o->grid_precision == MTI_GRID_C128) {
@@ -1021,8 +1182,6 @@ extern "C" mti_status mti_grid(mti_ctx *ctx, const mti_kernels *k, const mti_ima
     } else {
         // FP64 work grid for a batch of channels, rounded once into the output
         const size_t chan_bytes = (size_t)chan_floats * sizeof(double);
-        int batch = (int)std::max<size_t>(1, std::min<size_t>((size_t)nb, (size_t(8) << 30) / chan_bytes));
-        batch = std::min(batch, pair_batch);
         void *gw = nullptr;
         r = scratch_get(ctx, SLOT_GRID64, chan_bytes * batch, &gw);
         if (r != MTI_OK) return r;
@@ -1047,6 +1206,28 @@ extern "C" mti_status mti_grid(mti_ctx *ctx, const mti_kernels *k, const mti_ima
         }
     }
     return stats_out(ctx, dstats, nu, stats, st);
+}
+
+extern "C" mti_status mti_grid_jobs_workspace_size(int32_t njobs, const mti_job *jobs, const mti_grid_opts *o,
+                                                   size_t *bytes)
+{
+    if (!bytes || !o || check_opts(nullptr, o) != MTI_OK || njobs < 0 || njobs > 65535 || (njobs > 0 && !jobs))
+        return MTI_ERR_INVALID;
+    std::vector<PassJob> all;
+    int P = 0;
+    for (int i = 0; i < njobs; i++) {
+        const mti_job &j = jobs[i];
+        if (!j.kernels || check_sizes(&j.params, &j.vis, j.chan0, j.nchan) != MTI_OK) return MTI_ERR_INVALID;
+        if (i == 0) P = j.vis.npol;
+        if (j.vis.npol != P) return MTI_ERR_INVALID;
+        all.push_back(PassJob{j.kernels, j.params, make_view(&j.vis), j.chan0, j.nchan, nullptr, o->grid_layout,
+                              nullptr});
+        if (channels_per_pass(all.back()) < 1) return MTI_ERR_INVALID;
+    }
+    ScratchPlan pl;
+    plan_grid_jobs(all, njobs, jobs, P, o->weighting != MTI_WEIGHT_NATURAL, pl);
+    *bytes = pl.total();
+    return MTI_OK;
 }
 
 extern "C" mti_status mti_grid_jobs(mti_ctx *ctx, int32_t njobs, const mti_job *jobs, const mti_grid_opts *o,
@@ -1089,6 +1270,12 @@ extern "C" mti_status mti_grid_jobs(mti_ctx *ctx, int32_t njobs, const mti_job *
         MTI_CHECK(ctx, channels_per_pass(pj) >= 1, "one channel has too many (record, tile) pairs");
         all.push_back(pj);
     }
+    CallScope cs(ctx);
+    if (cs.planning()) {
+        ScratchPlan pl;
+        plan_grid_jobs(all, njobs, jobs, P, o->weighting != MTI_WEIGHT_NATURAL, pl);
+        if ((r = cs.begin(pl)) != MTI_OK) return r;
+    }
     void *sw = nullptr;
     r = scratch_get(ctx, SLOT_MISC2, 2 * BIN_STAT_STRIPES * sizeof(unsigned long long), &sw);
     if (r != MTI_OK) return r;
@@ -1126,37 +1313,9 @@ extern "C" mti_status mti_grid_jobs(mti_ctx *ctx, int32_t njobs, const mti_job *
             hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(64), 0, st, j.norm, (int64_t)j.nb * P, 0.0);
         MTI_HIP(ctx, hipGetLastError());
     }
-    // passes: consecutive jobs while the pass stays within its limits (a job
-    // too large for one pass is split by channel)
-    std::vector<PassJob> pass;
-    int64_t in_r = 0, in_p = 0, in_k = 0;
-    for (const PassJob &j : all) {
-        const int64_t cap = channels_per_pass(j);
-        const int64_t gch = (int64_t)j.p.nx * j.p.ny * P;  // complex elements per channel grid
-        for (int c0 = 0; c0 < j.nb;) {
-            PassJob piece = j;
-            piece.chan0 = j.chan0 + c0;
-            piece.nb = (int)std::min<int64_t>(j.nb - c0, cap);
-            piece.grid = j.grid + 2 * gch * c0;
-            piece.norm = j.norm + (size_t)c0 * P;
-            if (!pass.empty() &&
-                (in_r + job_records(piece) >= (int64_t(1) << 31) || in_p + job_pairs(piece) >= (int64_t(1) << 32) ||
-                 in_k + job_keys(piece) >= (int64_t(1) << 31) || pass.size() >= 32767)) {
-                r = grid_pass(ctx, pass, o, P, prec, !o->accumulate, dstats, nu, st);
-                if (r != MTI_OK) return r;
-                pass.clear();
-                in_r = in_p = in_k = 0;
-            }
-            pass.push_back(piece);
-            in_r += job_records(piece);
-            in_p += job_pairs(piece);
-            in_k += job_keys(piece);
-            c0 += piece.nb;
-        }
-    }
-    if (!pass.empty()) {
-        r = grid_pass(ctx, pass, o, P, prec, !o->accumulate, dstats, nu, st);
-        if (r != MTI_OK) return r;
-    }
+    r = for_each_pass(all, P, [&](const std::vector<PassJob> &pass) {
+        return grid_pass(ctx, pass, o, P, prec, !o->accumulate, dstats, nu, st);
+    });
+    if (r != MTI_OK) return r;
     return stats_out(ctx, dstats, nu, stats, st);
 }

@@ -40,7 +40,29 @@ enum ScratchSlot {
     SLOT_CHANDESC = 14, // descriptors of the finalize (packed planes)
     SLOT_CORR = 15,     // per-job correction vectors of the batched finalize
     SLOT_FCORR = 16,    // correction vectors of mti_finalize
-    SLOT_COUNT = 17
+    SLOT_TWIDDLE = 17,  // FFT twiddles of a call that runs in a caller workspace
+    SLOT_COUNT = 18
+};
+
+// Device scratch one call takes: the largest request of each slot.
+struct ScratchPlan {
+    size_t bytes[SLOT_COUNT] = {};
+    void need(int slot, size_t b)
+    {
+        if (b > bytes[slot]) bytes[slot] = b;
+    }
+    void merge(const ScratchPlan &o)
+    {
+        for (int i = 0; i < SLOT_COUNT; i++) need(i, o.bytes[i]);
+    }
+    // bytes of a caller workspace holding every slot 256-byte aligned
+    size_t total() const
+    {
+        size_t t = 256;  // room to align the workspace base
+        for (size_t b : bytes)
+            if (b) t += (b + 255) / 256 * 256;
+        return t;
+    }
 };
 
 }  // namespace mti
@@ -60,6 +82,14 @@ struct mti_ctx {
     mti::Scratch scratch[mti::SLOT_COUNT];
     // FFT twiddle tables keyed by length (FP64, exp(+2 pi i k / n)).
     std::vector<std::pair<int, double2 *>> twiddles;
+    // caller workspace (mti_set_workspace): while a planned call runs
+    // (ws_active), every slot is a fixed part of it and nothing is allocated
+    char *ws_ptr = nullptr;
+    size_t ws_bytes = 0;
+    bool ws_active = false;
+    int call_depth = 0;
+    size_t ws_off[mti::SLOT_COUNT] = {};
+    size_t ws_len[mti::SLOT_COUNT] = {};
 };
 
 struct mti_kernels {
@@ -98,6 +128,16 @@ inline mti_status scratch_get(mti_ctx *ctx, int slot, size_t bytes, void **out)
 {
     if (slot < 0 || slot >= SLOT_COUNT)
         return set_error(ctx, MTI_ERR_INVALID, "scratch slot out of range");
+    if (ctx->ws_active) {
+        // a slot of the caller workspace, sized by the call's plan
+        if (bytes > ctx->ws_len[slot])
+            return set_error(ctx, MTI_ERR_INVALID,
+                             "scratch request of " + std::to_string(bytes) + " bytes in slot " +
+                                 std::to_string(slot) + " above the call's workspace plan (" +
+                                 std::to_string(ctx->ws_len[slot]) + ")");
+        *out = ctx->ws_ptr + ctx->ws_off[slot];
+        return MTI_OK;
+    }
     Scratch &s = ctx->scratch[slot];
     if (s.bytes < bytes) {
         if (s.ptr) {
@@ -116,6 +156,38 @@ inline mti_status scratch_get(mti_ctx *ctx, int slot, size_t bytes, void **out)
     *out = s.ptr;
     return MTI_OK;
 }
+
+// One entry-point call.  The outermost call on a context with a caller
+// workspace lays the workspace out by the call's plan (begin); nested
+// entry-point calls run inside that layout.  Without a workspace it does
+// nothing: scratch comes from the context.
+struct CallScope {
+    mti_ctx *ctx;
+    bool outer;
+    explicit CallScope(mti_ctx *c) : ctx(c), outer(c->call_depth == 0) { c->call_depth++; }
+    ~CallScope()
+    {
+        ctx->call_depth--;
+        if (outer) ctx->ws_active = false;
+    }
+    bool planning() const { return outer && ctx->ws_ptr != nullptr; }
+    mti_status begin(const ScratchPlan &pl)
+    {
+        const uintptr_t base = ((uintptr_t)ctx->ws_ptr + 255) & ~uintptr_t(255);
+        size_t off = base - (uintptr_t)ctx->ws_ptr;
+        for (int i = 0; i < SLOT_COUNT; i++) {
+            ctx->ws_off[i] = off;
+            ctx->ws_len[i] = pl.bytes[i];
+            off += (pl.bytes[i] + 255) / 256 * 256;
+        }
+        if (off > ctx->ws_bytes)
+            return set_error(ctx, MTI_ERR_NOMEM,
+                             "caller workspace of " + std::to_string(ctx->ws_bytes) + " bytes is below the " +
+                                 std::to_string(pl.total()) + " bytes this call needs");
+        ctx->ws_active = true;
+        return MTI_OK;
+    }
+};
 
 inline hipStream_t to_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
@@ -147,6 +219,7 @@ mti_status validate_vis(mti_ctx *ctx, const mti_vis *v);
 // in *d_total (device).  Uses scratch SLOT_MISC.
 mti_status exclusive_scan_u32(mti_ctx *ctx, uint32_t *d_data, int64_t n,
                               uint32_t *d_total, hipStream_t st);
+void plan_scan(int64_t n, ScratchPlan &pl);
 
 }  // namespace mti
 

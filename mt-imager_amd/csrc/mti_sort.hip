@@ -81,6 +81,17 @@ __global__ __launch_bounds__(RX_THREADS) void k_radix_scatter(const uint32_t *ke
     }
 }
 
+static int64_t radix_counts(int64_t n) { return ceil_div(n, RX_ITEMS) * 256; }
+
+void plan_radix_sort(int64_t n, ScratchPlan &pl)
+{
+    if (n <= 0) return;
+    pl.need(SLOT_RADIX, (size_t)(radix_counts(n) + 64) * sizeof(uint32_t));
+    plan_scan(radix_counts(n), pl);
+}
+
+void plan_sort_rows_device(int64_t nrows, ScratchPlan &pl) { plan_radix_sort(nrows, pl); }
+
 mti_status radix_sort_pairs(mti_ctx *ctx, uint32_t *keys, int32_t *vals, uint32_t *keys_tmp,
                             int32_t *vals_tmp, int64_t n, int key_bits, hipStream_t st,
                             bool *result_in_tmp)
@@ -89,7 +100,7 @@ mti_status radix_sort_pairs(mti_ctx *ctx, uint32_t *keys, int32_t *vals, uint32_
     if (n <= 0) return MTI_OK;
     MTI_CHECK(ctx, n < (int64_t(1) << 32), "radix sort above 2^32 elements");
     const int nblocks = (int)ceil_div(n, RX_ITEMS);
-    const int64_t ncounts = (int64_t)nblocks * 256;
+    const int64_t ncounts = radix_counts(n);
     void *p = nullptr;
     mti_status r = scratch_get(ctx, SLOT_RADIX, (size_t)(ncounts + 64) * sizeof(uint32_t), &p);
     if (r != MTI_OK) return r;
@@ -163,6 +174,24 @@ mti_status sort_rows_device(mti_ctx *ctx, const int32_t *ant1, const int32_t *an
 
 using namespace mti;
 
+namespace mti {
+static void plan_sort_rows(int64_t nrows, ScratchPlan &pl)
+{
+    if (nrows <= 0) return;
+    pl.need(SLOT_KEYS, (size_t)nrows * 12 + 64);
+    plan_sort_rows_device(nrows, pl);
+}
+}  // namespace mti
+
+extern "C" mti_status mti_sort_rows_workspace_size(int64_t nrows, size_t *bytes)
+{
+    if (!bytes || nrows < 0 || nrows >= (int64_t(1) << 31)) return MTI_ERR_INVALID;
+    ScratchPlan pl;
+    plan_sort_rows(nrows, pl);
+    *bytes = pl.total();
+    return MTI_OK;
+}
+
 extern "C" mti_status mti_sort_rows(mti_ctx *ctx, const int32_t *ant1, const int32_t *ant2,
                                     int64_t nrows, int32_t nant, int32_t *index_out,
                                     int64_t *kept_out, void *stream)
@@ -173,6 +202,13 @@ extern "C" mti_status mti_sort_rows(mti_ctx *ctx, const int32_t *ant1, const int
     MTI_CHECK(ctx, kept_out && (nrows == 0 || (ant1 && ant2 && index_out)), "null pointer");
     MTI_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = to_stream(stream);
+    CallScope cs(ctx);
+    if (cs.planning()) {
+        ScratchPlan pl;
+        plan_sort_rows(nrows, pl);
+        mti_status rp = cs.begin(pl);
+        if (rp != MTI_OK) return rp;
+    }
     StageTimer tm(ctx, MTI_STAGE_SORT_ROWS, st);
     if (nrows == 0) {
         *kept_out = 0;

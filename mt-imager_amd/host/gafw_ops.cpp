@@ -189,6 +189,52 @@ int weighting_of(const Properties &p)
     throw ValidationException("weighting must be natural, uniform or briggs: " + w);
 }
 
+// The engine's operator buffer (sized by validate()) serves one call: bytes
+// [0, NORM_STAGE) hold the FP64 normaliser staging, the rest is the library's
+// workspace for the call, so the library allocates nothing itself.
+constexpr size_t NORM_STAGE = 256;
+
+class WorkspaceScope {
+public:
+    WorkspaceScope(mti_ctx *ctx, const SubmissionData &d, size_t need, const std::string &op) : ctx_(ctx)
+    {
+        if (!d.bufferGPUPointer || d.buffersize < NORM_STAGE + need)
+            throw GpuException(op + ": the operator buffer is missing or smaller than validate() asked for");
+        throw_status(mti_set_workspace(ctx, (char *)d.bufferGPUPointer + NORM_STAGE, d.buffersize - NORM_STAGE), ctx,
+                     op);
+    }
+    ~WorkspaceScope() { (void)mti_set_workspace(ctx_, nullptr, 0); }
+    WorkspaceScope(const WorkspaceScope &) = delete;
+    WorkspaceScope &operator=(const WorkspaceScope &) = delete;
+
+private:
+    mti_ctx *ctx_;
+};
+
+// Sizes-only description of a chunk for the workspace queries.
+mti_vis chunk_sizes(const ChunkShape &c)
+{
+    mti_vis v{};
+    v.nrows = c.N;
+    v.nchan = 1;
+    v.npol = c.P;
+    v.inv_lambda_per_row = 1;
+    return v;
+}
+
+mti_kernel_info info_of(const mti_kernels *k)
+{
+    mti_kernel_info i{};
+    (void)mti_kernels_info(k, &i);
+    return i;
+}
+
+size_t query(mti_status s, size_t bytes, const std::string &op)
+{
+    if (s != MTI_OK) throw ValidationException(op + ": arguments rejected by the workspace query");
+    return bytes;
+}
+
 // ---- operators ----------------------------------------------------------------
 
 // SortIndex (vis/SortIndex.cpp:46-87): inputs ANTENNA1, ANTENNA2 int [N]; output 0
@@ -198,7 +244,7 @@ class SortIndexOp : public ArrayOperator {
 public:
     using ArrayOperator::ArrayOperator;
     void validate(const std::vector<ArrayDescriptor> &in, std::vector<ArrayDescriptor> &out, const Properties &p,
-                  ValidationData &) override
+                  ValidationData &vd) override
     {
         expect(in.size() == 2, name_, "needs ANTENNA1 and ANTENNA2");
         expect(in[0].dim.getNoOfDimensions() == 1 && in[0].dim == in[1].dim, name_, "antenna columns must be [N]");
@@ -209,10 +255,15 @@ public:
         out[0].type = real_int;
         out[1].dim = ArrayDimensions({1});
         out[1].type = real_longint;
+        size_t ws = 0;
+        vd.bufferSize = NORM_STAGE + query(mti_sort_rows_workspace_size(in[0].dim.getX(), &ws), ws, name_);
     }
     void submitToGPU(SubmissionData &d) override
     {
         mti_ctx *ctx = device_context(current_device());
+        size_t ws = 0;
+        query(mti_sort_rows_workspace_size(d.inputs[0].dim.getX(), &ws), ws, name_);
+        WorkspaceScope wsc(ctx, d, ws, name_);
         throw_status(mti_sort_rows(ctx, (const int32_t *)d.inputs[0].pointer, (const int32_t *)d.inputs[1].pointer,
                                    d.inputs[0].dim.getX(), d.params.getIntProperty("nant"),
                                    (int32_t *)d.outputs[0].pointer, &kept_, d.stream),
@@ -275,39 +326,54 @@ public:
         out[1].dim = ArrayDimensions({c.P});
         out[1].type = nt;
         vd.bufferMode = BufferMode::Normal;
-        vd.bufferSize = nt == real_float ? (size_t)c.P * sizeof(double) : 0;  // FP64 normaliser staging
+        // FP64 normaliser staging + the gridding workspace
+        expect(conv_->kernels() != nullptr, name_, "convolution function not calculated");
+        const mti_grid_opts o = opts(p, gt);
+        const mti_vis v = chunk_sizes(c);
+        size_t ws = 0;
+        const mti_kernel_info ki = info_of(conv_->kernels());
+        vd.bufferSize = NORM_STAGE + query(mti_grid_workspace_size(&ki, &ip, &v, &o, &ws), ws, name_);
     }
     void submitToGPU(SubmissionData &d) override
     {
         const ChunkShape c{d.inputs[0].dim.getX(), (int)d.inputs[3].dim.getX()};
         mti_vis v = chunk_vis(d.inputs, c);
-        mti_grid_opts o{};
-        o.chan0 = 0;
-        o.nchan = 1;
-        o.accumulate = d.params.getBoolProperty("initialize_output", true) ? 0 : 1;
-        o.psf = d.params.getBoolProperty("psf", false) ? 1 : 0;
-        o.weighting = weighting_of(d.params);
-        o.robust = d.params.getDoubleProperty("robust", 0.0);
-        o.grid_layout = MTI_GRID_VUP;
-        o.grid_precision = d.outputs[0].type == complex_double ? MTI_GRID_C128 : MTI_GRID_C64;
+        const mti_grid_opts o = opts(d.params, d.outputs[0].type);
         hipStream_t st = (hipStream_t)d.stream;
+        mti_ctx *ctx = device_context(current_device());
+        size_t ws = 0;
+        const mti_kernel_info ki = info_of(conv_->kernels());
+        query(mti_grid_workspace_size(&ki, &conv_->imageParams(), &v, &o, &ws), ws, name_);
+        WorkspaceScope wsc(ctx, d, ws, name_);
         double *norm = (double *)d.outputs[1].pointer;
         const bool f32_norm = d.outputs[1].type != real_double;
         if (f32_norm) {
-            if (!d.bufferGPUPointer || d.buffersize < (size_t)c.P * sizeof(double))
-                throw GpuException(name_ + ": the validation buffer for the FP64 normaliser is missing");
             norm = (double *)d.bufferGPUPointer;
             if (o.accumulate)
                 hipLaunchKernelGGL(k_f32_to_f64, dim3(1), dim3(64), 0, st, (const float *)d.outputs[1].pointer, norm,
                                    c.P);
         }
-        mti_ctx *ctx = device_context(current_device());
         throw_status(mti_grid(ctx, conv_->kernels(), &conv_->imageParams(), &v, &o, d.outputs[0].pointer, norm,
                               nullptr, d.stream),
                      ctx, name_);
         if (f32_norm)
             hipLaunchKernelGGL(k_f64_to_f32, dim3(1), dim3(64), 0, st, norm, (float *)d.outputs[1].pointer, c.P);
         hip_check(hipGetLastError(), "normaliser conversion");
+    }
+
+private:
+    static mti_grid_opts opts(const Properties &p, StoreType grid_type)
+    {
+        mti_grid_opts o{};
+        o.chan0 = 0;
+        o.nchan = 1;
+        o.accumulate = p.getBoolProperty("initialize_output", true) ? 0 : 1;
+        o.psf = p.getBoolProperty("psf", false) ? 1 : 0;
+        o.weighting = weighting_of(p);
+        o.robust = p.getDoubleProperty("robust", 0.0);
+        o.grid_layout = MTI_GRID_VUP;
+        o.grid_precision = grid_type == complex_double ? MTI_GRID_C128 : MTI_GRID_C64;
+        return o;
     }
 };
 
@@ -317,7 +383,7 @@ class IFFT2DOp : public ArrayOperator {
 public:
     using ArrayOperator::ArrayOperator;
     void validate(const std::vector<ArrayDescriptor> &in, std::vector<ArrayDescriptor> &out, const Properties &,
-                  ValidationData &) override
+                  ValidationData &vd) override
     {
         expect(in.size() == 1, name_, "one input");
         expect(in[0].type == complex_float, name_, "input must be complex_float");
@@ -328,16 +394,23 @@ public:
         out.resize(1);
         out[0].dim = in[0].dim;
         out[0].type = complex_float;
+        size_t ws = 0;
+        vd.bufferSize = NORM_STAGE + query(mti_fft2d_workspace_size((int)nx, (int)ny,
+                                                                    (int)(in[0].dim.total() / (nx * ny)), &ws),
+                                           ws, name_);
     }
     void submitToGPU(SubmissionData &d) override
     {
         const ArrayDimensions &dim = d.inputs[0].dim;
         const long long nx = dim.getX(), ny = dim.getY(), nb = dim.total() / (nx * ny);
+        mti_ctx *ctx = device_context(current_device());
+        size_t ws = 0;
+        query(mti_fft2d_workspace_size((int)nx, (int)ny, (int)nb, &ws), ws, name_);
+        WorkspaceScope wsc(ctx, d, ws, name_);
         if (d.outputs[0].pointer != d.inputs[0].pointer)
             hip_check(hipMemcpyAsync(d.outputs[0].pointer, d.inputs[0].pointer, (size_t)dim.total() * 8,
                                      hipMemcpyDeviceToDevice, (hipStream_t)d.stream),
                       "IFFT2D copy");
-        mti_ctx *ctx = device_context(current_device());
         throw_status(mti_fft2d_inverse(ctx, (float *)d.outputs[0].pointer, (int)nx, (int)ny, (int)nb, d.stream), ctx,
                      name_);
     }
@@ -364,7 +437,11 @@ public:
         out.resize(1);
         out[0].dim = ArrayDimensions({P, in[0].dim.getZ(), in[0].dim.getY()});
         out[0].type = real_float;
-        vd.bufferSize = in[1].type == real_float ? (size_t)P * sizeof(double) : 0;
+        mti_image_params ip = conv_ ? conv_->imageParams() : image_params_from(p);
+        ip.ny = (int)in[0].dim.getZ();
+        ip.nx = (int)in[0].dim.getY();
+        size_t ws = 0;
+        vd.bufferSize = NORM_STAGE + query(mti_finalize_workspace_size(&ip, (int)P, 1, &ws), ws, name_);
     }
     void submitToGPU(SubmissionData &d) override
     {
@@ -372,10 +449,12 @@ public:
         const int P = (int)d.inputs[0].dim.getX();
         ip.ny = (int)d.inputs[0].dim.getZ();
         ip.nx = (int)d.inputs[0].dim.getY();
+        mti_ctx *ctx = device_context(current_device());
+        size_t ws = 0;
+        query(mti_finalize_workspace_size(&ip, P, 1, &ws), ws, name_);
+        WorkspaceScope wsc(ctx, d, ws, name_);
         const double *norm = (const double *)d.inputs[1].pointer;
         if (d.inputs[1].type == real_float) {
-            if (!d.bufferGPUPointer || d.buffersize < (size_t)P * sizeof(double))
-                throw GpuException(name_ + ": the validation buffer for the FP64 normaliser is missing");
             hipLaunchKernelGGL(k_f32_to_f64, dim3(1), dim3(64), 0, (hipStream_t)d.stream,
                                (const float *)d.inputs[1].pointer, (double *)d.bufferGPUPointer, P);
             hip_check(hipGetLastError(), "normaliser conversion");
@@ -383,7 +462,6 @@ public:
         }
         const int pt = d.params.getStringProperty("PolarizationType", "linear") == "circular" ? MTI_POL_CIRCULAR
                                                                                               : MTI_POL_LINEAR;
-        mti_ctx *ctx = device_context(current_device());
         throw_status(mti_finalize(ctx, &ip, P, pt, 1, MTI_GRID_VUP,
                                   d.inputs[0].type == complex_double ? MTI_GRID_C128 : MTI_GRID_C64,
                                   d.inputs[0].pointer, norm, (float *)d.outputs[0].pointer, d.stream),

@@ -83,7 +83,11 @@ EXPORTS = ["mti_version", "mti_create", "mti_destroy", "mti_last_error",
            "mti_kernels_build", "mti_kernels_info", "mti_kernels_supports",
            "mti_kernels_table", "mti_kernels_sums", "mti_kernels_free",
            "mti_sort_rows", "mti_preprocess", "mti_grid", "mti_fft2d_inverse",
-           "mti_finalize", "mti_grid_jobs", "mti_finalize_jobs"]
+           "mti_finalize", "mti_grid_jobs", "mti_finalize_jobs",
+           "mti_grid_out_shape", "mti_finalize_out_shape", "mti_preprocess_out_shape",
+           "mti_sort_rows_workspace_size", "mti_grid_workspace_size", "mti_grid_jobs_workspace_size",
+           "mti_finalize_workspace_size", "mti_finalize_jobs_workspace_size", "mti_fft2d_workspace_size",
+           "mti_set_workspace", "mti_context_device_bytes"]
 
 _lib = None
 
@@ -137,6 +141,22 @@ def lib():
     L.mti_grid_jobs.argtypes = [vp, ctypes.c_int32, vp, P(GridOpts), P(GridStats), vp]
     L.mti_finalize_jobs.restype = ctypes.c_int
     L.mti_finalize_jobs.argtypes = [vp, ctypes.c_int32, vp, ctypes.c_int32, ctypes.c_int32, vp]
+    i32, i64, sz = ctypes.c_int32, ctypes.c_int64, ctypes.c_size_t
+    for name, args in [
+            ("mti_grid_out_shape", [P(ImageParams), i32, i32, i32, P(i64), P(i64)]),
+            ("mti_finalize_out_shape", [P(ImageParams), i32, i32, P(i64)]),
+            ("mti_preprocess_out_shape", [i64, P(i64)]),
+            ("mti_sort_rows_workspace_size", [i64, P(sz)]),
+            ("mti_grid_workspace_size", [P(KernelInfo), P(ImageParams), P(Vis), P(GridOpts), P(sz)]),
+            ("mti_grid_jobs_workspace_size", [i32, vp, P(GridOpts), P(sz)]),
+            ("mti_finalize_workspace_size", [P(ImageParams), i32, i32, P(sz)]),
+            ("mti_finalize_jobs_workspace_size", [i32, vp, P(sz)]),
+            ("mti_fft2d_workspace_size", [i32, i32, i32, P(sz)]),
+            ("mti_set_workspace", [vp, vp, sz]),
+            ("mti_context_device_bytes", [vp, P(sz)])]:
+        f = getattr(L, name)
+        f.restype = ctypes.c_int
+        f.argtypes = args
     _lib = L
     return L
 
@@ -152,6 +172,77 @@ def make_params(nx, ny, cell_l, cell_m=None, mode="wproj", wplanes=1, sampling=4
     scale = ARCSEC if units == "arcsec" else 1.0
     return ImageParams(int(nx), int(ny), float(cell_l) * scale, float(cell_m) * scale,
                        MODE[mode], int(wplanes), int(sampling), int(taper_support))
+
+
+def _grid_opts(chan0=0, nchan=1, accumulate=False, psf=False, weighting="natural", robust=0.0,
+               layout="vup", precision="c128"):
+    return GridOpts(int(chan0), int(nchan), 1 if accumulate else 0, 1 if psf else 0,
+                    WEIGHTING[weighting], float(robust), LAYOUT[layout], GRID_PRECISION[precision])
+
+
+def _query(fn, *args):
+    out = ctypes.c_size_t(0)
+    rc = fn(*args, ctypes.byref(out))
+    if rc != 0:
+        raise MTIError(f"{fn.__name__}: invalid arguments (status {rc})")
+    return int(out.value)
+
+
+# ---- output shapes and workspace sizes (no context, no device) ------------------
+def grid_out_shape(params, npol, nchan, layout="vup"):
+    """(grid dims, normaliser dims) of mti_grid: [nchan][ny][nx][npol] complex
+    ("vup") or [nchan][npol][ny][nx] ("pvu"); [nchan][npol]."""
+    g, n = (ctypes.c_int64 * 4)(), (ctypes.c_int64 * 2)()
+    rc = lib().mti_grid_out_shape(ctypes.byref(params), int(npol), int(nchan), LAYOUT[layout], g, n)
+    if rc != 0:
+        raise MTIError(f"mti_grid_out_shape: invalid arguments (status {rc})")
+    return tuple(g), tuple(n)
+
+
+def finalize_out_shape(params, npol, nchan):
+    d = (ctypes.c_int64 * 4)()
+    rc = lib().mti_finalize_out_shape(ctypes.byref(params), int(npol), int(nchan), d)
+    if rc != 0:
+        raise MTIError(f"mti_finalize_out_shape: invalid arguments (status {rc})")
+    return tuple(d)
+
+
+def preprocess_out_shape(nrecords):
+    d = (ctypes.c_int64 * 2)()
+    rc = lib().mti_preprocess_out_shape(int(nrecords), d)
+    if rc != 0:
+        raise MTIError(f"mti_preprocess_out_shape: invalid arguments (status {rc})")
+    return tuple(d)
+
+
+def sort_rows_workspace_size(nrows):
+    return _query(lib().mti_sort_rows_workspace_size, int(nrows))
+
+
+def grid_workspace_size(kernel_info, params, vis, **opts):
+    """Workspace bytes of Context.grid with the same arguments (opts as for
+    grid: chan0, nchan, weighting, layout, precision, ...)."""
+    return _query(lib().mti_grid_workspace_size, ctypes.byref(kernel_info), ctypes.byref(params),
+                  ctypes.byref(vis), ctypes.byref(_grid_opts(**opts)))
+
+
+def grid_jobs_workspace_size(jobs, **opts):
+    arr = (Job * len(jobs))(*jobs)
+    return _query(lib().mti_grid_jobs_workspace_size, len(jobs), ctypes.cast(arr, ctypes.c_void_p),
+                  ctypes.byref(_grid_opts(**opts)))
+
+
+def finalize_workspace_size(params, npol, nchan):
+    return _query(lib().mti_finalize_workspace_size, ctypes.byref(params), int(npol), int(nchan))
+
+
+def finalize_jobs_workspace_size(jobs):
+    arr = (Job * len(jobs))(*jobs)
+    return _query(lib().mti_finalize_jobs_workspace_size, len(jobs), ctypes.cast(arr, ctypes.c_void_p))
+
+
+def fft2d_workspace_size(nx, ny, nbatch=1):
+    return _query(lib().mti_fft2d_workspace_size, int(nx), int(ny), int(nbatch))
 
 
 def _stream(stream):
@@ -197,10 +288,28 @@ class Context:
         return {s: (float(ms[i]), int(n[i])) for i, s in enumerate(STAGES)}
 
     # ---- a8-a14 ---------------------------------------------------------
+    def set_workspace(self, buf):
+        """Caller-owned scratch: every later sort/grid/finalize/transform call
+        on this context lays its scratch out in `buf` (a device tensor, kept
+        referenced here) and allocates nothing; None returns to scratch owned
+        by the context."""
+        if buf is None:
+            self.check(lib().mti_set_workspace(self.h, None, 0), "mti_set_workspace")
+        else:
+            nbytes = buf.numel() * buf.element_size()
+            self.check(lib().mti_set_workspace(self.h, buf.data_ptr(), nbytes), "mti_set_workspace")
+        self._workspace = buf
+
+    def device_bytes(self):
+        """Device bytes the context itself holds (its scratch and twiddle tables)."""
+        out = ctypes.c_size_t(0)
+        self.check(lib().mti_context_device_bytes(self.h, ctypes.byref(out)), "mti_context_device_bytes")
+        return int(out.value)
+
     def build_kernels(self, params, precision="f32", stream=None):
         return Kernels(self, params, precision, stream)
 
-    # ---- a1 -------------------------------------------------------------
+    # ---- row order --------------------------------------------------------
     def sort_rows(self, ant1, ant2, nant, index_out, stream=None):
         kept = ctypes.c_int64(0)
         rc = lib().mti_sort_rows(self.h, ant1.data_ptr(), ant2.data_ptr(), ant1.numel(),
@@ -209,7 +318,7 @@ class Context:
         self.check(rc, "mti_sort_rows")
         return int(kept.value)
 
-    # ---- a4 -------------------------------------------------------------
+    # ---- binning ----------------------------------------------------------
     def preprocess(self, kern, params, vis, chan, bins_out, stream=None):
         rc = lib().mti_preprocess(self.h, kern.h, ctypes.byref(params), ctypes.byref(vis),
                                   int(chan), bins_out.data_ptr(), _stream(stream))
@@ -224,8 +333,7 @@ class Context:
         "c64"/"c128", taken from the tensor dtype by default)."""
         if precision is None:
             precision = "c128" if str(grids.dtype) in ("torch.complex128", "torch.float64") else "c64"
-        o = GridOpts(int(chan0), int(nchan), 1 if accumulate else 0, 1 if psf else 0,
-                     WEIGHTING[weighting], float(robust), LAYOUT[layout], GRID_PRECISION[precision])
+        o = _grid_opts(chan0, nchan, accumulate, psf, weighting, robust, layout, precision)
         st = GridStats()
         rc = lib().mti_grid(self.h, kern.h, ctypes.byref(params), ctypes.byref(vis),
                             ctypes.byref(o), grids.data_ptr(), norm.data_ptr(),
@@ -233,13 +341,13 @@ class Context:
         self.check(rc, "mti_grid")
         return st
 
-    # ---- a17 ------------------------------------------------------------
+    # ---- transform --------------------------------------------------------
     def fft2d_inverse(self, data, nx, ny, nbatch=1, stream=None):
         rc = lib().mti_fft2d_inverse(self.h, data.data_ptr(), int(nx), int(ny), int(nbatch),
                                      _stream(stream))
         self.check(rc, "mti_fft2d_inverse")
 
-    # ---- a16-a19 ----------------------------------------------------------
+    # ---- finalize -----------------------------------------------------------
     def finalize(self, params, npol, grids, norm, image, nchan=1, poltype="linear",
                  layout="vup", precision=None, stream=None):
         if precision is None:

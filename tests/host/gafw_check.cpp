@@ -125,7 +125,14 @@ static int run_validate()
     fin->validate(ii, out, p, vd);
     CHECK(out.size() == 1 && out[0].dim == ArrayDimensions({4, 256, 128}) && out[0].type == real_float,
           "ImageFinalizer output is real_float [P][v][u]");
-    CHECK(vd.bufferSize == 4 * sizeof(double), "ImageFinalizer asks for an FP64 normaliser buffer");
+    {
+        mti_image_params fp = image_params_from(p);
+        fp.ny = 256;
+        fp.nx = 128;
+        size_t ws = 0;
+        CHECK(mti_finalize_workspace_size(&fp, 4, 1, &ws) == MTI_OK && ws > 0 && vd.bufferSize == 256 + ws,
+              "ImageFinalizer asks for the normaliser staging and the finalize workspace");
+    }
     ii[1].dim = ArrayDimensions({2});
     CHECK(throws<ValidationException>([&] { fin->validate(ii, out, p, vd); }), "normaliser length must be P");
     Properties pc = p;
