@@ -86,6 +86,25 @@ def measured_traffic(cfg):
     return None
 
 
+def achievable_peaks():
+    """Peaks re-measured on this GPU with libmti_probe: a device copy of 2 GiB
+    (read + write bytes per second) and an FP64 FMA loop (8 independent chains
+    per thread, 8192 x 256 threads)."""
+    import ctypes
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "mt-imager_amd", "libmti_probe.so")
+    if not os.path.exists(path):
+        return None
+    L = ctypes.CDLL(path)
+    L.mtp_copy_gbps.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    L.mtp_fp64_fma_tflops.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    gbps, tflops = ctypes.c_double(0), ctypes.c_double(0)
+    if L.mtp_copy_gbps(2 << 30, 5, ctypes.byref(gbps)) != 0:
+        return None
+    if L.mtp_fp64_fma_tflops(8192, 2048, 5, ctypes.byref(tflops)) != 0:
+        return None
+    return dict(copy_GBps=round(gbps.value, 1), fp64_fma_TFLOPs=round(tflops.value, 2))
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -238,6 +257,42 @@ def run_c4(args, torch, mti, jobs):
             K.free()
         return el, gridded, lat, len(cache)
 
+    tune_info = None
+    if not args.no_autotune:
+        # the project's autotuner picks the gridder's work-unit size and the
+        # transform's column strategy on a sample of the mix (every 8th job,
+        # up to half a wave's budget, imaged as one wave); plain and batched
+        # runs below both use the choice
+        from mtimager_amd import autotune
+        sample, mem = [], 0
+        for j in jobs[::8]:
+            need = j["nch"] * j["nx"] ** 2 * 20
+            if mem + need <= budget / 2:
+                sample.append(j)
+                mem += need
+        kc = {}
+        tjobs = []
+        for j in sample:
+            if j["key"] not in kc:
+                kc[j["key"]] = ctx.build_kernels(j["p"])
+            gr, nr, im = outputs(j)
+            v = mti.vis_ms(j["uvw"], j["invl"], j["vis"], j["flags"], j["weight"])
+            tjobs.append(ctx.make_job(kc[j["key"]], j["p"], v, gr, nr, im, 0, j["nch"],
+                                      ant1=j["a1"], ant2=j["a2"], nant=64))
+
+        def tune_run():
+            ctx.grid_jobs(tjobs, stats=False)
+            ctx.finalize_jobs(tjobs)
+
+        log(f"c4: autotuning on {len(sample)} jobs")
+        best, timings = autotune.tune(ctx, tune_run, repeats=2)
+        log(f"c4: tuning {best}")
+        tune_info = dict(sample_jobs=len(sample), strategy="coordinate", chosen=best,
+                         candidates_ms=[dict(c, ms=round(t * 1e3, 2)) for c, t in timings])
+        del tjobs
+        for K in kc.values():
+            K.free()
+        torch.cuda.synchronize()
     log("c4: warm-up (batched)")
     batched()
     log("c4: plain, one job at a time")
@@ -249,6 +304,7 @@ def run_c4(args, torch, mti, jobs):
     assert gp == gb, (gp, gb)
     lp, lb = np.array(lp) * 1e3, np.array(lb) * 1e3
     return dict(jobs=len(jobs), gridded_vis=int(gb), kernel_sets=nk, hbm_budget_gb=args.c4_budget_gb,
+                autotune=tune_info,
                 plain=dict(mvis_s=round(gp / tp / 1e6, 3), seconds=round(tp, 3),
                            latency_ms={"p50": round(float(np.percentile(lp, 50)), 1),
                                        "p99": round(float(np.percentile(lp, 99)), 1)}),
@@ -356,6 +412,7 @@ def main():
     ap.add_argument("--c4-budget-gb", type=float, default=64.0, help="config 4: HBM budget of a batched wave")
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="config 4: keep the default tuning")
     ap.add_argument("--cpu-rows", type=int, default=60000)
     ap.add_argument("--cpu-threads", type=int, default=16, help="threads of the CPU baseline (one channel each)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -424,6 +481,8 @@ def main():
         torch.cuda.synchronize()
         log(f"copy probe: {n * 8} bytes read, {n * 8} bytes written (neg kernel)")
         del src, dst
+    peaks = achievable_peaks()
+    log(f"achievable peaks: {peaks}")
     log("warmup")
     st, _ = step(stats=True)
     torch.cuda.synchronize()
@@ -468,7 +527,9 @@ def main():
                     peak=FP64_PEAK_TFLOPS, unit="TFLOP/s", frac=round(achieved / FP64_PEAK_TFLOPS, 4),
                     traffic=measured_traffic(args.config), flops_per_launch=flops,
                     avg_launch_ms=round(grid_avg_s * 1e3, 3),
-                    algorithmic_GBps=round(rec_bytes / grid_avg_s / 1e9, 1))
+                    algorithmic_GBps=round(rec_bytes / grid_avg_s / 1e9, 1),
+                    achievable=peaks,
+                    frac_of_achievable=(round(achieved / peaks["fp64_fma_TFLOPs"], 4) if peaks else None))
 
     cpu = None
     if not args.no_cpu_baseline:

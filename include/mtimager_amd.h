@@ -270,6 +270,24 @@ mti_status mti_finalize(mti_ctx *ctx, const mti_image_params *p, int32_t npol,
                         int32_t grid_precision, const void *grids,
                         const double *norm, float *image, void *stream);
 
+/* ---- launch-shape tuning -----------------------------------------------------
+ * Choices of how a context lays work onto the GPU, picked per workload by the
+ * autotuner (mtimager_amd/autotune.py).  The reference fixes its gridder
+ * blocks at <= 440 / <= 110 records (op/common.hcu:26-27, used by
+ * NotSameSupportManipulate, op/NotSameSupportManipulate.cu:32-97); results do
+ * not depend on these beyond FP64 summation order.  The tile itself is fixed
+ * at 32 x 32 cells (one lane per cell of a 1024-lane workgroup). */
+typedef struct {
+    int32_t unit_pairs;       /* (record, tile) pairs per gridder work unit:
+                                 power of two in [256, 16384]; 0 = 2048      */
+    int32_t whole_columns_ny; /* for ny = 256, where both transforms apply:
+                                 column passes whole in one block when
+                                 ny <= this, else the four-step split; 0 or a
+                                 power of two in [16, 256] (default 256)     */
+} mti_tuning;
+mti_status mti_set_tuning(mti_ctx *ctx, const mti_tuning *t);
+mti_status mti_get_tuning(const mti_ctx *ctx, mti_tuning *t);
+
 /* ---- output shapes and scratch sizes (validate) ----------------------------
  * The validate() half of the reference's operator contract
  * (GPUArrayOperator::validate, gpu/GPUArrayOperator.h:27-43): output

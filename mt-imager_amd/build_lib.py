@@ -66,7 +66,26 @@ def build(verbose=False):
     if p.returncode != 0:
         raise RuntimeError("link failed:\n" + p.stdout + p.stderr)
     os.replace(tmp, LIB)
+    build_probe()
     return LIB
+
+
+PROBE_SRC = os.path.join(HERE, "probe", "mti_probe.hip")
+PROBE_LIB = os.path.join(HERE, "libmti_probe.so")
+
+
+def build_probe():
+    """The benchmark's achievable-peak probes (stream copy, FP64 FMA loop)."""
+    tmp = PROBE_LIB + ".tmp"
+    flags = list(COMMON)
+    i = flags.index("-I")
+    del flags[i:i + 2]
+    cmd = [HIPCC] + flags + ["-shared", "-o", tmp, PROBE_SRC]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError("probe build failed:\n" + p.stdout + p.stderr)
+    os.replace(tmp, PROBE_LIB)
+    return PROBE_LIB
 
 
 if __name__ == "__main__":

@@ -48,6 +48,28 @@ extern "C" mti_status mti_set_workspace(mti_ctx *ctx, void *ptr, size_t bytes)
     return MTI_OK;
 }
 
+extern "C" mti_status mti_set_tuning(mti_ctx *ctx, const mti_tuning *t)
+{
+    if (!ctx) return MTI_ERR_INVALID;
+    MTI_CHECK(ctx, t, "null tuning");
+    const int u = t->unit_pairs ? t->unit_pairs : 2048;
+    MTI_CHECK(ctx, u >= 256 && u <= 16384 && (u & (u - 1)) == 0, "unit_pairs must be a power of two in [256, 16384]");
+    const int w = t->whole_columns_ny;
+    MTI_CHECK(ctx, w == 0 || (w >= 16 && w <= 256 && (w & (w - 1)) == 0),
+              "whole_columns_ny must be 0 or a power of two in [16, 256]");
+    ctx->unit_pairs = u;
+    ctx->whole_columns_ny = w;
+    return MTI_OK;
+}
+
+extern "C" mti_status mti_get_tuning(const mti_ctx *ctx, mti_tuning *t)
+{
+    if (!ctx || !t) return MTI_ERR_INVALID;
+    t->unit_pairs = ctx->unit_pairs;
+    t->whole_columns_ny = ctx->whole_columns_ny;
+    return MTI_OK;
+}
+
 extern "C" mti_status mti_context_device_bytes(const mti_ctx *ctx, size_t *bytes)
 {
     if (!ctx || !bytes) return MTI_ERR_INVALID;

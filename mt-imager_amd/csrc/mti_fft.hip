@@ -669,10 +669,18 @@ __global__ __launch_bounds__(FBC_THREADS) void k_cols_pk(const double2 *X, int n
 // (N1, N2 >= 16, a 4096-element block of columns fits the image width).
 static bool fbc_ok(const FftGeom &g) { return g.ny >= 16 && g.ny * 32 <= FBC_E && g.nx >= FBC_E / g.ny; }
 
-static bool pk_ok(const FftGeom &g)
+static bool split_ok(const FftGeom &g)
 {
-    return g.nx >= 16 && g.nx <= 4096 && g.ny >= 16 &&
-           (fbc_ok(g) || (g.N1 >= 16 && g.N2 >= 16 && FB_E / g.N1 <= g.nx && FB_E / g.N2 <= g.nx));
+    return g.N1 >= 16 && g.N2 >= 16 && FB_E / g.N1 <= g.nx && FB_E / g.N2 <= g.nx;
+}
+
+static bool pk_ok(const FftGeom &g) { return g.nx >= 16 && g.nx <= 4096 && g.ny >= 16 && (fbc_ok(g) || split_ok(g)); }
+
+// Whole columns in one pass, or the four-step split: where both apply (ny =
+// 256) the context's tuning picks.
+static bool whole_columns(const mti_ctx *ctx, const FftGeom &g)
+{
+    return fbc_ok(g) && (!split_ok(g) || g.ny <= ctx->whole_columns_ny);
 }
 
 template <typename TI>
@@ -882,7 +890,7 @@ static mti_status finalize_packs(mti_ctx *ctx, const FftGeom &g, const PackDesc 
         r = pk_rows<TI>(ctx, g, pd, npk, X, cs, tw, st);
         if (r != MTI_OK) return r;
     }
-    if (fbc_ok(g)) {
+    if (whole_columns(ctx, g)) {
         StageTimer tm(ctx, MTI_STAGE_FINALIZE, st);
         return pk_cols(ctx, g, X, npk, tw, pd, st);
     }

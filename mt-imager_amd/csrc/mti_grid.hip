@@ -18,7 +18,7 @@
 //      pass reaches records through tile pairs or skips support 0);
 //   3. counting sort of (record, tile) pairs: every record is listed under each
 //      32x32-cell tile its support box touches (wave-aggregated counters, one
-//      scan, scatter); a tile's list is cut into work units of <= TG_CHUNK
+//      scan, scatter); a tile's list is cut into work units of <= unit_pairs
 //      records.  Tile keys are numbered job-major, so the units of all jobs of
 //      a pass form one list and one persistent launch grids them all;
 //   4. uniform/Briggs: one block per tile key gathers the weights of the
@@ -51,7 +51,8 @@ constexpr int TG_THREADS = 1024;     // 16 waves, one 8x8 region each
 constexpr int TG_REGIONS = 16;
 constexpr int TG_BATCH = 256;        // records staged per round
 static_assert(TG_T == 32 && TG_THREADS == 64 * TG_REGIONS && TG_BATCH <= 256, "region layout");
-constexpr int TG_CHUNK = 2048;       // records per work unit
+constexpr int TG_CHUNK = 2048;       // default (record, tile) pairs per work unit
+constexpr int TG_CHUNK_MIN = 256;    // smallest unit a context may be tuned to
 
 // Arrays of a gridding pass shared by the kernels after the binning pass.
 struct GridArgs {
@@ -215,18 +216,19 @@ __global__ __launch_bounds__(256) void k_tile_pairs(GridArgs a, int64_t cap, uin
     }
 }
 
-// Units per tile (ceil(pairs / TG_CHUNK)); start is the exclusive scan of
+// Units per tile (ceil(pairs / chunk)); start is the exclusive scan of
 // the pair counts, total the sum.
-__global__ void k_unit_counts(const uint32_t *start, const uint32_t *total, int64_t nkeys, uint32_t *nunits)
+__global__ void k_unit_counts(const uint32_t *start, const uint32_t *total, int64_t nkeys, uint32_t chunk,
+                              uint32_t *nunits)
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nkeys) return;
     const uint32_t e = (k + 1 < nkeys) ? start[k + 1] : *total;
-    nunits[k] = (e - start[k] + TG_CHUNK - 1) / TG_CHUNK;
+    nunits[k] = (e - start[k] + chunk - 1) / chunk;
 }
 
 __global__ void k_unit_list(const uint32_t *start, const uint32_t *total, const uint32_t *uprefix,
-                            const uint32_t *n_units, int64_t nkeys, int4 *units)
+                            const uint32_t *n_units, int64_t nkeys, uint32_t chunk, int4 *units)
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nkeys) return;
@@ -236,8 +238,8 @@ __global__ void k_unit_list(const uint32_t *start, const uint32_t *total, const 
     const uint32_t u1 = (k + 1 < nkeys) ? uprefix[k + 1] : *n_units;
     const int excl = (u1 - u0) == 1;
     for (uint32_t u = u0; u < u1; u++) {
-        const uint32_t f = s + (u - u0) * TG_CHUNK;
-        units[u] = make_int4((int)k, (int)f, (int)min((uint32_t)TG_CHUNK, e - f), excl);
+        const uint32_t f = s + (u - u0) * chunk;
+        units[u] = make_int4((int)k, (int)f, (int)min(chunk, e - f), excl);
     }
 }
 
@@ -705,7 +707,7 @@ struct PassSizes {
     size_t u32n = 0, u32k = 0, pre = 0, tab = 0, wv_bytes = 0, wt_bytes = 0;
 };
 
-static PassSizes pass_sizes(const std::vector<PassJob> &jobs, int P)
+static PassSizes pass_sizes(const std::vector<PassJob> &jobs, int P, int chunk)
 {
     PassSizes z;
     const int nj = (int)jobs.size();
@@ -715,7 +717,7 @@ static PassSizes pass_sizes(const std::vector<PassJob> &jobs, int P)
         z.nch += j.nb;
         z.pair_cap += job_pairs(j);
     }
-    z.unit_cap = z.nkeys + z.pair_cap / TG_CHUNK + 1;
+    z.unit_cap = z.nkeys + z.pair_cap / chunk + 1;
     z.u32n = align_up((size_t)z.nrec * 4);
     z.u32k = align_up((size_t)z.nkeys * 4);
     z.pre = align_up((size_t)(nj + 1) * 8);
@@ -727,7 +729,8 @@ static PassSizes pass_sizes(const std::vector<PassJob> &jobs, int P)
 
 static void plan_pass(const std::vector<PassJob> &jobs, int P, bool weighted, ScratchPlan &pl)
 {
-    const PassSizes z = pass_sizes(jobs, P);
+    // the smallest unit a context can be tuned to: the most units
+    const PassSizes z = pass_sizes(jobs, P, TG_CHUNK_MIN);
     if (jobs.empty() || z.nrec == 0) return;
     pl.need(SLOT_KEYS, z.u32n + 4096);
     pl.need(SLOT_DESC, align_up((size_t)z.nrec * 16));
@@ -781,7 +784,8 @@ static mti_status grid_pass(mti_ctx *ctx, const std::vector<PassJob> &jobs, cons
         d.rec0 = rec0[i];
         d.key0 = key0[i];
     }
-    const PassSizes z = pass_sizes(jobs, P);
+    const int chunk = ctx->unit_pairs;
+    const PassSizes z = pass_sizes(jobs, P, chunk);
     const int64_t nrec = z.nrec, nkeys = z.nkeys, nch = z.nch;
     if (nrec == 0) {
         // nothing to grid: a grid that was not pre-filled is all zeros
@@ -894,11 +898,13 @@ static mti_status grid_pass(mti_ctx *ctx, const std::vector<PassJob> &jobs, cons
     }
     {
         StageTimer tm(ctx, MTI_STAGE_PLAN, st);
-        hipLaunchKernelGGL(k_unit_counts, dim3(kblocks), dim3(256), 0, st, start, NP_d, nkeys, nunits);
+        hipLaunchKernelGGL(k_unit_counts, dim3(kblocks), dim3(256), 0, st, start, NP_d, nkeys, (uint32_t)chunk,
+                           nunits);
         MTI_HIP(ctx, hipGetLastError());
         r = exclusive_scan_u32(ctx, nunits, nkeys, NU_d, st);
         if (r != MTI_OK) return r;
-        hipLaunchKernelGGL(k_unit_list, dim3(kblocks), dim3(256), 0, st, start, NP_d, nunits, NU_d, nkeys, units);
+        hipLaunchKernelGGL(k_unit_list, dim3(kblocks), dim3(256), 0, st, start, NP_d, nunits, NU_d, nkeys,
+                           (uint32_t)chunk, units);
         MTI_HIP(ctx, hipGetLastError());
         if (fresh) {
             hipLaunchKernelGGL(k_zero_tiles, dim3((unsigned)std::min<int64_t>(nkeys, 16384)), dim3(256), 0, st, js,

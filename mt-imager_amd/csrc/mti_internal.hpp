@@ -90,6 +90,9 @@ struct mti_ctx {
     int call_depth = 0;
     size_t ws_off[mti::SLOT_COUNT] = {};
     size_t ws_len[mti::SLOT_COUNT] = {};
+    // launch-shape tuning (mti_set_tuning)
+    int unit_pairs = 2048;       // (record, tile) pairs per gridder work unit
+    int whole_columns_ny = 256;  // largest ny transformed by whole columns when the split also applies
 };
 
 struct mti_kernels {

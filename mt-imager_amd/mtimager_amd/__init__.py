@@ -74,6 +74,10 @@ class GridStats(ctypes.Structure):
                 ("work_items", ctypes.c_int64), ("flushes", ctypes.c_int64)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("unit_pairs", ctypes.c_int32), ("whole_columns_ny", ctypes.c_int32)]
+
+
 # Every symbol include/mtimager_amd.h declares.
 STAGES = ["sort_rows", "kernels", "bin", "sort_records", "plan", "grid", "fft_rows",
           "fft_cols", "finalize"]
@@ -87,7 +91,7 @@ EXPORTS = ["mti_version", "mti_create", "mti_destroy", "mti_last_error",
            "mti_grid_out_shape", "mti_finalize_out_shape", "mti_preprocess_out_shape",
            "mti_sort_rows_workspace_size", "mti_grid_workspace_size", "mti_grid_jobs_workspace_size",
            "mti_finalize_workspace_size", "mti_finalize_jobs_workspace_size", "mti_fft2d_workspace_size",
-           "mti_set_workspace", "mti_context_device_bytes"]
+           "mti_set_workspace", "mti_context_device_bytes", "mti_set_tuning", "mti_get_tuning"]
 
 _lib = None
 
@@ -153,7 +157,9 @@ def lib():
             ("mti_finalize_jobs_workspace_size", [i32, vp, P(sz)]),
             ("mti_fft2d_workspace_size", [i32, i32, i32, P(sz)]),
             ("mti_set_workspace", [vp, vp, sz]),
-            ("mti_context_device_bytes", [vp, P(sz)])]:
+            ("mti_context_device_bytes", [vp, P(sz)]),
+            ("mti_set_tuning", [vp, P(Tuning)]),
+            ("mti_get_tuning", [vp, P(Tuning)])]:
         f = getattr(L, name)
         f.restype = ctypes.c_int
         f.argtypes = args
@@ -299,6 +305,22 @@ class Context:
             nbytes = buf.numel() * buf.element_size()
             self.check(lib().mti_set_workspace(self.h, buf.data_ptr(), nbytes), "mti_set_workspace")
         self._workspace = buf
+
+    def set_tuning(self, unit_pairs=None, whole_columns_ny=None):
+        """Launch-shape tuning of this context (see mtimager_amd.autotune);
+        arguments left None keep their current value."""
+        t = self.get_tuning()
+        if unit_pairs is not None:
+            t["unit_pairs"] = int(unit_pairs)
+        if whole_columns_ny is not None:
+            t["whole_columns_ny"] = int(whole_columns_ny)
+        self.check(lib().mti_set_tuning(self.h, ctypes.byref(Tuning(t["unit_pairs"], t["whole_columns_ny"]))),
+                   "mti_set_tuning")
+
+    def get_tuning(self):
+        t = Tuning()
+        self.check(lib().mti_get_tuning(self.h, ctypes.byref(t)), "mti_get_tuning")
+        return {"unit_pairs": int(t.unit_pairs), "whole_columns_ny": int(t.whole_columns_ny)}
 
     def device_bytes(self):
         """Device bytes the context itself holds (its scratch and twiddle tables)."""
