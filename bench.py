@@ -413,6 +413,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="config 4: keep the default tuning")
+    ap.add_argument("--xcd-chunk", type=int, default=None, help="configs 1/2: gridder units per XCD run")
+    ap.add_argument("--unit-pairs", type=int, default=None, help="configs 1/2: gridder work-unit size")
     ap.add_argument("--cpu-rows", type=int, default=60000)
     ap.add_argument("--cpu-threads", type=int, default=16, help="threads of the CPU baseline (one channel each)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -457,6 +459,7 @@ def main():
     d = make_workload(torch, args.config, args.seed)
     p = mti.make_params(d["nx"], d["nx"], d["cell"], d["cell"], "wproj", d["W"], d["s"])
     ctx = mti.Context(0)
+    ctx.set_tuning(unit_pairs=args.unit_pairs, xcd_chunk=args.xcd_chunk)
     nrows, nchan, P, nx = d["nrows"], d["nchan"], d["P"], d["nx"]
     grids = torch.empty((nchan, P, nx, nx, 2), dtype=torch.float64, device="cuda")
     norm = torch.empty((nchan, P), dtype=torch.float64, device="cuda")
@@ -571,6 +574,7 @@ def main():
             "config1: 27 ant / 3 km, 351 baselines x 300 steps, 8 ch, P=4, 512^2 @ 2 arcsec, 32 planes, s=4",
             "rows": nrows, "channels": nchan, "gridded_vis": gridded, "gridded_points": points,
             "supports": [int(sup.min()), int(sup.max())], "path": "plain single-job",
+            "tuning": ctx.get_tuning(),
         },
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                        "p99": round(float(np.percentile(lat, 99)), 3)},

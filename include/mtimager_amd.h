@@ -270,6 +270,23 @@ mti_status mti_finalize(mti_ctx *ctx, const mti_image_params *p, int32_t npol,
                         int32_t grid_precision, const void *grids,
                         const double *norm, float *image, void *stream);
 
+/* ---- captured batched imaging (HIP graphs) -----------------------------------
+ * No counterpart in the reference (it plans each cuFFT call afresh,
+ * gafw/StandardOperators/IFFT2D.cpp:86-111): one mti_grid_jobs call (and,
+ * with finalize != 0, one mti_finalize_jobs call with poltype) of a fixed job
+ * list captured once into a HIP graph.  The graph owns its workspace and the
+ * pinned copies of its host tables; every mti_graph_launch images the current
+ * contents of the jobs' input buffers into their grids/normalisers/images
+ * with no host synchronisation and no allocation (no statistics are read
+ * back).  The jobs' buffers, kernel tables and the context must outlive the
+ * graph; launches of one graph must be ordered. */
+typedef struct mti_graph mti_graph;
+mti_status mti_graph_create(mti_ctx *ctx, int32_t njobs, const mti_job *jobs,
+                            const mti_grid_opts *opts, int32_t finalize, int32_t poltype,
+                            mti_graph **out);
+mti_status mti_graph_launch(mti_graph *g, void *stream);
+void       mti_graph_destroy(mti_graph *g);
+
 /* ---- launch-shape tuning -----------------------------------------------------
  * Choices of how a context lays work onto the GPU, picked per workload by the
  * autotuner (mtimager_amd/autotune.py).  The reference fixes its gridder
@@ -284,6 +301,12 @@ typedef struct {
                                  column passes whole in one block when
                                  ny <= this, else the four-step split; 0 or a
                                  power of two in [16, 256] (default 256)     */
+    int32_t xcd_chunk;        /* gridder work units dealt to each of the 8
+                                 XCDs in runs of this many consecutive units
+                                 (neighbouring tiles share w-planes, so one
+                                 XCD's L2 serves their kernel samples); 0 =
+                                 plain round-robin, else a power of two in
+                                 [1, 4096]; default 0                         */
 } mti_tuning;
 mti_status mti_set_tuning(mti_ctx *ctx, const mti_tuning *t);
 mti_status mti_get_tuning(const mti_ctx *ctx, mti_tuning *t);

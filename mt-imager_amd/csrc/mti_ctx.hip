@@ -57,8 +57,12 @@ extern "C" mti_status mti_set_tuning(mti_ctx *ctx, const mti_tuning *t)
     const int w = t->whole_columns_ny;
     MTI_CHECK(ctx, w == 0 || (w >= 16 && w <= 256 && (w & (w - 1)) == 0),
               "whole_columns_ny must be 0 or a power of two in [16, 256]");
+    const int x = t->xcd_chunk;
+    MTI_CHECK(ctx, x == 0 || (x >= 1 && x <= 4096 && (x & (x - 1)) == 0),
+              "xcd_chunk must be 0 or a power of two in [1, 4096]");
     ctx->unit_pairs = u;
     ctx->whole_columns_ny = w;
+    ctx->xcd_chunk = x;
     return MTI_OK;
 }
 
@@ -67,6 +71,7 @@ extern "C" mti_status mti_get_tuning(const mti_ctx *ctx, mti_tuning *t)
     if (!ctx || !t) return MTI_ERR_INVALID;
     t->unit_pairs = ctx->unit_pairs;
     t->whole_columns_ny = ctx->whole_columns_ny;
+    t->xcd_chunk = ctx->xcd_chunk;
     return MTI_OK;
 }
 
@@ -149,7 +154,7 @@ static hipEvent_t pool_event(mti_ctx *ctx)
 
 StageTimer::StageTimer(mti_ctx *c, int s, hipStream_t stream) : ctx(c), st(stream), stage(s)
 {
-    if (!ctx || !ctx->profiling) return;
+    if (!ctx || !ctx->profiling || ctx->capturing) return;
     a = pool_event(ctx);
     b = pool_event(ctx);
     if (a) (void)hipEventRecord(a, st);

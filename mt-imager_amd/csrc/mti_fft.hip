@@ -290,8 +290,8 @@ static mti_status twiddle_table(mti_ctx *ctx, int NT, const double2 **out, hipSt
         void *w = nullptr;
         mti_status r = scratch_get(ctx, SLOT_TWIDDLE, (size_t)NT * sizeof(double2), &w);
         if (r != MTI_OK) return r;
-        MTI_HIP(ctx, hipMemcpyAsync(w, h.data(), NT * sizeof(double2), hipMemcpyHostToDevice, st));
-        MTI_HIP(ctx, hipStreamSynchronize(st));
+        if ((r = upload(ctx, w, h.data(), NT * sizeof(double2), st)) != MTI_OK) return r;
+        if ((r = host_sync(ctx, st)) != MTI_OK) return r;
         *out = (const double2 *)w;
         return MTI_OK;
     }
@@ -918,8 +918,8 @@ static mti_status upload_desc(mti_ctx *ctx, const std::vector<D> &h, D **d, hipS
     void *w = nullptr;
     mti_status r = scratch_get(ctx, SLOT_CHANDESC, std::max<size_t>(1, h.size()) * sizeof(D), &w);
     if (r != MTI_OK) return r;
-    MTI_HIP(ctx, hipMemcpyAsync(w, h.data(), h.size() * sizeof(D), hipMemcpyHostToDevice, st));
-    MTI_HIP(ctx, hipStreamSynchronize(st));
+    if ((r = upload(ctx, w, h.data(), h.size() * sizeof(D), st)) != MTI_OK) return r;
+    if ((r = host_sync(ctx, st)) != MTI_OK) return r;
     *d = (D *)w;
     return MTI_OK;
 }

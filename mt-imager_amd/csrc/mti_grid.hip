@@ -68,7 +68,10 @@ struct GridArgs {
     const uint32_t *n_units;    // device scalar
     int fresh;                  // grids not pre-filled: a tile's only unit stores its cells,
                                 // every other tile was zeroed by k_zero_tiles
+    int xcd_log2;               // gridder units dealt to each XCD in runs of 2^xcd_log2 (-1: round-robin)
 };
+
+constexpr uint32_t MTI_XCDS = 8;  // XCDs of an MI355X; workgroup b runs on XCD b % 8
 
 // Global-memory views of pointers read from the device-resident job table.
 // Such pointers are generic to the compiler, and loads through a generic
@@ -441,8 +444,11 @@ __device__ __forceinline__ Visit visit_issue(int i, int n, const ListEntry *list
 // every region the list of records whose support box reaches it, each entry
 // a copy of the record's geometry.  A wave walks only its list, with kernel
 // samples loaded four visits ahead.  No LDS traffic per tap.
+// f32 tables: two 1024-lane workgroups per CU (8 waves per SIMD, at most 64
+// VGPRs) is a requirement, not a hope -- one more register halves the
+// occupancy.  The f16/bf16 forms convert samples and need more registers.
 template <int P, int PREC>
-__global__ __launch_bounds__(TG_THREADS) void k_grid_tiles(GridArgs a)
+__global__ __launch_bounds__(TG_THREADS, PREC == MTI_TABLE_F32 ? 8 : 1) void k_grid_tiles(GridArgs a)
 {
     __shared__ __align__(16) double s_vd[TG_BATCH][2 * P];  // weighted visibility
     __shared__ ListEntry s_list[TG_REGIONS][TG_BATCH];  // records per region
@@ -451,8 +457,18 @@ __global__ __launch_bounds__(TG_THREADS) void k_grid_tiles(GridArgs a)
     const int tid = threadIdx.x, lane = tid & 63, wave = wave_uniform(tid >> 6);
     const int rx = (wave & 3) * 8 + (lane & 7), ry = (wave >> 2) * 8 + (lane >> 3);  // cell in the tile
     const uint32_t NU = *a.n_units;
-
-    for (uint32_t u = blockIdx.x; u < NU; u += gridDim.x) {
+    // XCD-aware order: runs of 2^lg consecutive units (neighbouring tiles of
+    // one channel, whose records share w-planes) are dealt to the XCDs in turn
+    // and an XCD's workgroups walk its runs in order, so that XCD's L2 serves
+    // the kernel samples the run has in common.  (xcd, i) -> u is a bijection:
+    // every unit is gridded once.  lg < 0 (or a grid not a multiple of 8):
+    // round-robin.  All of it is workgroup-uniform (scalar registers).
+    const int lg = (gridDim.x % MTI_XCDS) ? -1 : a.xcd_log2;
+    const uint32_t xcd = blockIdx.x % MTI_XCDS;
+    const uint32_t step = lg >= 0 ? gridDim.x / MTI_XCDS : gridDim.x;
+    for (uint32_t i = lg >= 0 ? blockIdx.x / MTI_XCDS : blockIdx.x;; i += step) {
+        const uint32_t u = lg >= 0 ? ((((i >> lg) * MTI_XCDS + xcd) << lg) | (i & ((1u << lg) - 1u))) : i;
+        if (u >= NU) break;
         const int4 un = a.units[u];
         const int first = un.y, npairs = un.z, excl = un.w;
         const TileAt ta = tile_at(a.js, un.x);
@@ -840,11 +856,11 @@ static mti_status grid_pass(mti_ctx *ctx, const std::vector<PassJob> &jobs, cons
         for (int i = 0; i < nj; i++) jd[i].f2 = f2 + jc0[i] * P;
     }
     // the job table (the host vectors live until the synchronisation below)
-    MTI_HIP(ctx, hipMemcpyAsync(d_jobs, jd.data(), (size_t)nj * sizeof(JobDev), hipMemcpyHostToDevice, st));
-    MTI_HIP(ctx, hipMemcpyAsync(d_rec0, rec0.data(), (size_t)(nj + 1) * 8, hipMemcpyHostToDevice, st));
-    MTI_HIP(ctx, hipMemcpyAsync(d_key0, key0.data(), (size_t)(nj + 1) * 8, hipMemcpyHostToDevice, st));
-    MTI_HIP(ctx, hipMemcpyAsync(d_blk0, blk0.data(), (size_t)(nj + 1) * 8, hipMemcpyHostToDevice, st));
-    MTI_HIP(ctx, hipMemcpyAsync(d_jc0, jc0.data(), (size_t)(nj + 1) * 8, hipMemcpyHostToDevice, st));
+    if ((r = upload(ctx, d_jobs, jd.data(), (size_t)nj * sizeof(JobDev), st)) != MTI_OK) return r;
+    if ((r = upload(ctx, d_rec0, rec0.data(), (size_t)(nj + 1) * 8, st)) != MTI_OK) return r;
+    if ((r = upload(ctx, d_key0, key0.data(), (size_t)(nj + 1) * 8, st)) != MTI_OK) return r;
+    if ((r = upload(ctx, d_blk0, blk0.data(), (size_t)(nj + 1) * 8, st)) != MTI_OK) return r;
+    if ((r = upload(ctx, d_jc0, jc0.data(), (size_t)(nj + 1) * 8, st)) != MTI_OK) return r;
     JobSet js;
     js.jobs = d_jobs;
     js.rec0 = d_rec0;
@@ -879,6 +895,9 @@ static mti_status grid_pass(mti_ctx *ctx, const std::vector<PassJob> &jobs, cons
     ga.units = units;
     ga.n_units = NU_d;
     ga.fresh = fresh ? 1 : 0;
+    ga.xcd_log2 = -1;
+    for (int k = 0; k <= 12; k++)
+        if (ctx->xcd_chunk == (1 << k)) ga.xcd_log2 = k;
     {
         StageTimer tm(ctx, MTI_STAGE_SORT_RECORDS, st);
         // every record stays in place (job, channel, baseline order); records
@@ -940,6 +959,7 @@ static mti_status grid_pass(mti_ctx *ctx, const std::vector<PassJob> &jobs, cons
         }
         if (r != MTI_OK) return r;
     }
+    if (ctx->capturing) return MTI_OK;  // the tables are staged; no counts read back
     uint32_t h[2] = {0, 0};
     MTI_HIP(ctx, hipMemcpyAsync(&h[0], NU_d, 4, hipMemcpyDeviceToHost, st));
     MTI_HIP(ctx, hipMemcpyAsync(&h[1], NP_d, 4, hipMemcpyDeviceToHost, st));

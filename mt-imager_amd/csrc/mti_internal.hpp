@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -93,6 +94,13 @@ struct mti_ctx {
     // launch-shape tuning (mti_set_tuning)
     int unit_pairs = 2048;       // (record, tile) pairs per gridder work unit
     int whole_columns_ny = 256;  // largest ny transformed by whole columns when the split also applies
+    int xcd_chunk = 0;           // gridder units dealt to each XCD in runs of this many (0: round-robin)
+    // HIP-graph capture (mti_graph_create): calls enqueue device work only --
+    // no host synchronisation, no statistics read-back, host tables staged
+    // in pinned memory that lives as long as the graph
+    bool capturing = false;
+    char *stage_ptr = nullptr;
+    size_t stage_bytes = 0, stage_used = 0, stage_need = 0;
 };
 
 struct mti_kernels {
@@ -191,6 +199,36 @@ struct CallScope {
         return MTI_OK;
     }
 };
+
+// Host-to-device copy of a call's small tables (job tables, descriptors,
+// twiddles).  Outside graph capture the call synchronises before the host
+// copy goes away; while a graph is captured the bytes go through the
+// graph's pinned staging, which the captured copy then reads at every launch.
+inline mti_status upload(mti_ctx *ctx, void *dst, const void *src, size_t bytes, hipStream_t st)
+{
+    if (bytes == 0) return MTI_OK;
+    if (ctx->capturing) {
+        const size_t off = (ctx->stage_used + 255) / 256 * 256;
+        ctx->stage_used = ctx->stage_need = off + bytes;
+        // past the end of the staging: keep counting and record nothing; the
+        // capture is then discarded and repeated with stage_need bytes
+        if (off + bytes > ctx->stage_bytes) return MTI_OK;
+        std::memcpy(ctx->stage_ptr + off, src, bytes);
+        src = ctx->stage_ptr + off;
+    }
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return set_error(ctx, MTI_ERR_HIP, std::string("upload: ") + hipGetErrorString(e));
+    return MTI_OK;
+}
+
+// Waits for the stream unless a graph is being captured.
+inline mti_status host_sync(mti_ctx *ctx, hipStream_t st)
+{
+    if (ctx->capturing) return MTI_OK;
+    hipError_t e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return set_error(ctx, MTI_ERR_HIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+    return MTI_OK;
+}
 
 inline hipStream_t to_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 

@@ -75,7 +75,7 @@ class GridStats(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("unit_pairs", ctypes.c_int32), ("whole_columns_ny", ctypes.c_int32)]
+    _fields_ = [("unit_pairs", ctypes.c_int32), ("whole_columns_ny", ctypes.c_int32), ("xcd_chunk", ctypes.c_int32)]
 
 
 # Every symbol include/mtimager_amd.h declares.
@@ -91,7 +91,8 @@ EXPORTS = ["mti_version", "mti_create", "mti_destroy", "mti_last_error",
            "mti_grid_out_shape", "mti_finalize_out_shape", "mti_preprocess_out_shape",
            "mti_sort_rows_workspace_size", "mti_grid_workspace_size", "mti_grid_jobs_workspace_size",
            "mti_finalize_workspace_size", "mti_finalize_jobs_workspace_size", "mti_fft2d_workspace_size",
-           "mti_set_workspace", "mti_context_device_bytes", "mti_set_tuning", "mti_get_tuning"]
+           "mti_set_workspace", "mti_context_device_bytes", "mti_set_tuning", "mti_get_tuning",
+           "mti_graph_create", "mti_graph_launch", "mti_graph_destroy"]
 
 _lib = None
 
@@ -159,10 +160,14 @@ def lib():
             ("mti_set_workspace", [vp, vp, sz]),
             ("mti_context_device_bytes", [vp, P(sz)]),
             ("mti_set_tuning", [vp, P(Tuning)]),
-            ("mti_get_tuning", [vp, P(Tuning)])]:
+            ("mti_get_tuning", [vp, P(Tuning)]),
+            ("mti_graph_create", [vp, i32, vp, P(GridOpts), i32, i32, P(vp)]),
+            ("mti_graph_launch", [vp, vp])]:
         f = getattr(L, name)
         f.restype = ctypes.c_int
         f.argtypes = args
+    L.mti_graph_destroy.argtypes = [vp]
+    L.mti_graph_destroy.restype = None
     _lib = L
     return L
 
@@ -306,21 +311,21 @@ class Context:
             self.check(lib().mti_set_workspace(self.h, buf.data_ptr(), nbytes), "mti_set_workspace")
         self._workspace = buf
 
-    def set_tuning(self, unit_pairs=None, whole_columns_ny=None):
+    def set_tuning(self, unit_pairs=None, whole_columns_ny=None, xcd_chunk=None):
         """Launch-shape tuning of this context (see mtimager_amd.autotune);
         arguments left None keep their current value."""
         t = self.get_tuning()
-        if unit_pairs is not None:
-            t["unit_pairs"] = int(unit_pairs)
-        if whole_columns_ny is not None:
-            t["whole_columns_ny"] = int(whole_columns_ny)
-        self.check(lib().mti_set_tuning(self.h, ctypes.byref(Tuning(t["unit_pairs"], t["whole_columns_ny"]))),
-                   "mti_set_tuning")
+        for k, v in (("unit_pairs", unit_pairs), ("whole_columns_ny", whole_columns_ny), ("xcd_chunk", xcd_chunk)):
+            if v is not None:
+                t[k] = int(v)
+        self.check(lib().mti_set_tuning(self.h, ctypes.byref(Tuning(t["unit_pairs"], t["whole_columns_ny"],
+                                                                    t["xcd_chunk"]))), "mti_set_tuning")
 
     def get_tuning(self):
         t = Tuning()
         self.check(lib().mti_get_tuning(self.h, ctypes.byref(t)), "mti_get_tuning")
-        return {"unit_pairs": int(t.unit_pairs), "whole_columns_ny": int(t.whole_columns_ny)}
+        return {"unit_pairs": int(t.unit_pairs), "whole_columns_ny": int(t.whole_columns_ny),
+                "xcd_chunk": int(t.xcd_chunk)}
 
     def device_bytes(self):
         """Device bytes the context itself holds (its scratch and twiddle tables)."""
@@ -411,11 +416,46 @@ class Context:
         self.check(rc, "mti_grid_jobs")
         return st
 
+    def capture_jobs(self, jobs, accumulate=False, psf=False, weighting="natural", robust=0.0, layout="vup",
+                     finalize=True, poltype="linear"):
+        """A Graph of grid_jobs (+ finalize_jobs) over these jobs: captured
+        once, launched any number of times on the buffers' current contents."""
+        return Graph(self, jobs, _grid_opts(0, 0, accumulate, psf, weighting, robust, layout, "c128"),
+                     finalize, poltype)
+
     def finalize_jobs(self, jobs, poltype="linear", layout="vup", stream=None):
         arr = (Job * len(jobs))(*jobs)
         rc = lib().mti_finalize_jobs(self.h, len(jobs), ctypes.cast(arr, ctypes.c_void_p), POLTYPE[poltype],
                                      LAYOUT[layout], _stream(stream))
         self.check(rc, "mti_finalize_jobs")
+
+
+class Graph:
+    """mti_graph: batched imaging of a fixed job list captured into a HIP
+    graph (the jobs' tensors, kernel tables and the context must outlive it)."""
+
+    def __init__(self, ctx, jobs, opts, finalize=True, poltype="linear"):
+        self.ctx = ctx
+        self.h = ctypes.c_void_p()
+        arr = (Job * len(jobs))(*jobs)
+        rc = lib().mti_graph_create(ctx.h, len(jobs), ctypes.cast(arr, ctypes.c_void_p), ctypes.byref(opts),
+                                    1 if finalize else 0, POLTYPE[poltype], ctypes.byref(self.h))
+        ctx.check(rc, "mti_graph_create")
+        self._keep = list(jobs)
+
+    def launch(self, stream=None):
+        self.ctx.check(lib().mti_graph_launch(self.h, _stream(stream)), "mti_graph_launch")
+
+    def free(self):
+        if self.h:
+            lib().mti_graph_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class Kernels:

@@ -1,7 +1,9 @@
 """Autotuner for a context's launch-shape choices (mti_tuning): the gridder's
 work-unit size (how many (record, tile) pairs one workgroup takes, which sets
 how a crowded tile splits over workgroups and how many tiles need atomic
-flushes) and the transform's column strategy where two apply.
+flushes), how work units are dealt to the 8 XCDs (runs of neighbouring tiles
+per XCD, for L2 reuse of kernel samples) and the transform's column strategy
+where two apply.
 
 Each candidate is set on the context, run once untimed, then timed `repeats`
 times on the caller's representative workload; the fastest stays set.  The
@@ -19,10 +21,11 @@ import time
 
 # the knobs of mti_tuning and the values searched by default
 DEFAULT_SPACE = {
+    "xcd_chunk": [0, 8, 32, 128],
     "unit_pairs": [512, 1024, 2048, 4096, 8192],
     "whole_columns_ny": [0, 256],
 }
-DEFAULTS = {"unit_pairs": 2048, "whole_columns_ny": 256}
+DEFAULTS = {"unit_pairs": 2048, "whole_columns_ny": 256, "xcd_chunk": 0}
 
 
 def grid_candidates(space):

@@ -36,19 +36,20 @@ def _image(mti, c, job, weighting="briggs"):
 def test_tuning_is_validated(mti):
     c = mti.Context(0)
     try:
-        assert c.get_tuning() == {"unit_pairs": 2048, "whole_columns_ny": 256}
+        assert c.get_tuning() == {"unit_pairs": 2048, "whole_columns_ny": 256, "xcd_chunk": 0}
         for bad in (dict(unit_pairs=300), dict(unit_pairs=128), dict(unit_pairs=32768), dict(whole_columns_ny=100),
-                    dict(whole_columns_ny=512)):
+                    dict(whole_columns_ny=512), dict(xcd_chunk=3), dict(xcd_chunk=8192)):
             with pytest.raises(mti.MTIError):
                 c.set_tuning(**bad)
-        c.set_tuning(unit_pairs=512, whole_columns_ny=0)
-        assert c.get_tuning() == {"unit_pairs": 512, "whole_columns_ny": 0}
+        c.set_tuning(unit_pairs=512, whole_columns_ny=0, xcd_chunk=16)
+        assert c.get_tuning() == {"unit_pairs": 512, "whole_columns_ny": 0, "xcd_chunk": 16}
     finally:
         c.close()
 
 
 @pytest.mark.parametrize("tuning", [dict(unit_pairs=256), dict(unit_pairs=16384), dict(whole_columns_ny=0),
-                                    dict(unit_pairs=1024, whole_columns_ny=128)])
+                                    dict(unit_pairs=1024, whole_columns_ny=128), dict(xcd_chunk=1),
+                                    dict(xcd_chunk=32, unit_pairs=256), dict(xcd_chunk=4096)])
 def test_results_do_not_depend_on_tuning(mti, ctx, job, tuning):
     g0, n0, i0, s0 = _image(mti, ctx, job)
     c = mti.Context(0)
