@@ -185,6 +185,61 @@ def snapshot_pool(torch, mti, njobs, seed=4):
     return jobs
 
 
+def graph_replay(ctx, torch, mti, jobs, outputs, max_jobs=64, reps=7):
+    """A wave of the mix's small jobs (images up to 512^2, at most 3e5
+    visibilities) imaged eagerly (mti_grid_jobs + mti_finalize_jobs, with
+    their host synchronisations) and as a captured HIP graph; median wall
+    time per wave of each, and whether the replay equals the eager images."""
+    import numpy as np
+    small = [j for j in jobs if j["nx"] <= 512 and j["nrows"] * j["nch"] <= 300000][:max_jobs]
+    if not small:
+        return None
+    kc = {}
+    for j in small:
+        if j["key"] not in kc:
+            kc[j["key"]] = ctx.build_kernels(j["p"])
+
+    def make():
+        out, js = [], []
+        for j in small:
+            gr, nr, im = outputs(j)
+            v = mti.vis_ms(j["uvw"], j["invl"], j["vis"], j["flags"], j["weight"])
+            js.append(ctx.make_job(kc[j["key"]], j["p"], v, gr, nr, im, 0, j["nch"],
+                                   ant1=j["a1"], ant2=j["a2"], nant=64))
+            out.append(im)
+        return js, out
+
+    ejobs, eimg = make()
+    gjobs, gimg = make()
+
+    def eager():
+        ctx.grid_jobs(ejobs, stats=False)
+        ctx.finalize_jobs(ejobs)
+
+    g = ctx.capture_jobs(gjobs)
+
+    def timed(fn):
+        ts = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)) * 1e3
+
+    te = timed(eager)
+    tg = timed(g.launch)
+    same = all(float((a - b).abs().max()) <= 1e-6 * max(float(b.abs().max()), 1e-30) for a, b in zip(gimg, eimg))
+    g.free()
+    for K in kc.values():
+        K.free()
+    nvis = sum(j["nrows"] * j["nch"] for j in small)
+    log(f"c4: wave of {len(small)} small jobs: eager {te:.2f} ms, graph {tg:.2f} ms")
+    return dict(jobs=len(small), visibilities=int(nvis), eager_ms=round(te, 3), graph_ms=round(tg, 3),
+                speedup=round(te / tg, 3), replay_equals_eager=bool(same))
+
+
 def run_c4(args, torch, mti, jobs):
     """Config 4: independent snapshot jobs (Stokes I) imaged one at a time
     (plain: per job kernel tables, row sort, gridding, finalize) and batched
@@ -300,11 +355,12 @@ def run_c4(args, torch, mti, jobs):
     log(f"c4: plain {tp:.2f} s")
     tb, gb, lb, nk = batched()
     log(f"c4: batched {tb:.2f} s ({nk} kernel-table sets)")
+    graph_info = graph_replay(ctx, torch, mti, jobs, outputs)
     ctx.close()
     assert gp == gb, (gp, gb)
     lp, lb = np.array(lp) * 1e3, np.array(lb) * 1e3
     return dict(jobs=len(jobs), gridded_vis=int(gb), kernel_sets=nk, hbm_budget_gb=args.c4_budget_gb,
-                autotune=tune_info,
+                autotune=tune_info, graph=graph_info,
                 plain=dict(mvis_s=round(gp / tp / 1e6, 3), seconds=round(tp, 3),
                            latency_ms={"p50": round(float(np.percentile(lp, 50)), 1),
                                        "p99": round(float(np.percentile(lp, 99)), 1)}),
