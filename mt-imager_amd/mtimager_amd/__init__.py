@@ -549,6 +549,30 @@ def vis_ms(uvw, inv_lambda, vis, flags, weight, row_index=None):
     return v
 
 
+def vis_chan_major(uvw, inv_lambda, vis, flags, weight, row_index=None):
+    """mti_vis for channel-major torch tensors on the GPU (a block of channels
+    uploaded as one contiguous piece): uvw f64 [nrows][3], inv_lambda f64
+    [nchan], vis complex64 [nchan][nrows][npol], flags u8 [nchan][nrows][npol],
+    weight f32 [nrows][npol]."""
+    nchan, nrows, npol = flags.shape
+    for t in (uvw, inv_lambda, vis, flags, weight):
+        assert t.is_contiguous() and t.is_cuda
+    v = Vis()
+    v.nrows, v.nchan, v.npol = nrows, nchan, npol
+    v.uvw, v.uvw_f32, v.uvw_row_stride, v.uvw_comp_stride = uvw.data_ptr(), 0, 3, 1
+    v.inv_lambda, v.inv_lambda_f32, v.inv_lambda_per_row = inv_lambda.data_ptr(), 0, 0
+    v.vis, v.vis_row_stride, v.vis_chan_stride = vis.data_ptr(), npol, nrows * npol
+    v.flags, v.flags_i32 = flags.data_ptr(), 0
+    v.flags_row_stride, v.flags_chan_stride = npol, nrows * npol
+    v.weight, v.weight_row_stride = weight.data_ptr(), npol
+    if row_index is not None:
+        v.row_index, v.nindex = row_index.data_ptr(), row_index.numel()
+    else:
+        v.row_index, v.nindex = None, 0
+    v._keep = (uvw, inv_lambda, vis, flags, weight, row_index)
+    return v
+
+
 def vis_reference_chunk(uvw_t, inv_lambda_rows, vis, flags_i32, weight):
     """mti_vis for the reference's per-channel chunk layout
     (vis/StorageManager.h:147-150): uvw f32 [3][N], 1/lambda f32 [N],
