@@ -92,7 +92,8 @@ EXPORTS = ["mti_version", "mti_create", "mti_destroy", "mti_last_error",
            "mti_sort_rows_workspace_size", "mti_grid_workspace_size", "mti_grid_jobs_workspace_size",
            "mti_finalize_workspace_size", "mti_finalize_jobs_workspace_size", "mti_fft2d_workspace_size",
            "mti_set_workspace", "mti_context_device_bytes", "mti_set_tuning", "mti_get_tuning",
-           "mti_graph_create", "mti_graph_launch", "mti_graph_destroy"]
+           "mti_graph_create", "mti_graph_launch", "mti_graph_destroy",
+           "mti_fits_create", "mti_fits_write_channel", "mti_fits_close"]
 
 _lib = None
 

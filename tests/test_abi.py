@@ -1,5 +1,6 @@
 """The C-ABI library loads without a GPU and exports every symbol that
-include/mtimager_amd.h declares (no compute calls here)."""
+include/mtimager_amd.h and include/mtimager_amd_fits.h declare (no compute
+calls here)."""
 import ctypes
 import os
 import re
@@ -8,9 +9,12 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def _declared():
-    src = open(os.path.join(ROOT, "include", "mtimager_amd.h")).read()
-    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return set(re.findall(r"\b(mti_[a-z0-9_]+)\s*\(", src))
+    names = set()
+    for h in ("mtimager_amd.h", "mtimager_amd_fits.h"):
+        src = open(os.path.join(ROOT, "include", h)).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        names |= set(re.findall(r"\b(mti_[a-z0-9_]+)\s*\(", src))
+    return names
 
 
 def test_header_and_binding_agree(mti):
