@@ -120,6 +120,14 @@ typedef struct {
     double  robust;           /* Briggs robustness R                          */
     int32_t grid_layout;      /* mti_grid_layout of `grids`                   */
     int32_t grid_precision;   /* mti_grid_precision of `grids`                */
+    /* Optional (NULL: complete grids).  Sparse output for callers that only
+     * finalize the grids: u8 [nchan][ceil(ny/32)][ceil(nx/32)] (device),
+     * written by the call -- 1 where some record's support box reaches the
+     * 32 x 32-cell tile, 0 where none does; tiles marked 0 are then left
+     * unwritten (undefined) instead of zero-filled.  Needs complex128 grids
+     * and accumulate = 0; mti_finalize_sparse reads those tiles as zeros.
+     * No counterpart in the reference (its grids are always complete). */
+    uint8_t *tile_mask;
 } mti_grid_opts;
 
 typedef struct {
@@ -239,6 +247,10 @@ typedef struct {
     const int32_t *ant1;
     const int32_t *ant2;
     int32_t nant;
+    /* Optional tile mask of the job's grids, as mti_grid_opts.tile_mask
+     * (u8 [nchan][ceil(ny/32)][ceil(nx/32)]): written by mti_grid_jobs
+     * (accumulate = 0), read by mti_finalize_jobs. */
+    uint8_t *tile_mask;
 } mti_job;
 
 /* Grids every job (opts: accumulate, psf, weighting, robust, grid_layout;
@@ -269,6 +281,13 @@ mti_status mti_finalize(mti_ctx *ctx, const mti_image_params *p, int32_t npol,
                         int32_t poltype, int32_t nchan, int32_t grid_layout,
                         int32_t grid_precision, const void *grids,
                         const double *norm, float *image, void *stream);
+/* mti_finalize of sparse grids: tiles that tile_mask (from mti_grid with
+ * opts.tile_mask) marks 0 are read as zeros and never loaded. */
+mti_status mti_finalize_sparse(mti_ctx *ctx, const mti_image_params *p, int32_t npol,
+                               int32_t poltype, int32_t nchan, int32_t grid_layout,
+                               int32_t grid_precision, const void *grids,
+                               const uint8_t *tile_mask, const double *norm, float *image,
+                               void *stream);
 
 /* ---- captured batched imaging (HIP graphs) -----------------------------------
  * No counterpart in the reference (it plans each cuFFT call afresh,

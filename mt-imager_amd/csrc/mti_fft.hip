@@ -365,6 +365,7 @@ struct PackDesc {
     float *img[2];            // image planes [ny][nx] of s = 0, 1 (img[1] null: none)
     const float *corr_x;      // [nx] sinc / taper correction
     const float *corr_y;      // [ny]
+    const uint8_t *ma, *mb;   // tile masks of planes A and B ([ny/32][nx/32]; null: complete grids)
 };
 
 __device__ __forceinline__ int fb_pad(int i) { return i + (i >> 4); }
@@ -540,8 +541,9 @@ template <int N, typename TI>
 __global__ __launch_bounds__(pk_e(N) / 16) void k_rows_pk(const PackDesc *pd, double2 *X, int ny, int64_t cs,
                                                          const double2 *tw, int NT)
 {
-    constexpr int E = pk_e(N), TH = E / 16, PB = E / (2 * N);
+    constexpr int E = pk_e(N), TH = E / 16, PB = E / (2 * N), NF = E / N, NTX = (N + 31) / 32;
     __shared__ double2 lds[E + E / 16];
+    __shared__ uint8_t s_mask[2][NF][NTX];  // tile masks of the block's rows (sparse grids)
     const PackDesc &d = pd[blockIdx.y];
     const TI *A = (const TI *)d.a, *B = (const TI *)d.b;
     const double ia = 1.0 / *d.na, ib = B ? 1.0 / *d.nb : 0.0;
@@ -549,8 +551,27 @@ __global__ __launch_bounds__(pk_e(N) / 16) void k_rows_pk(const PackDesc *pd, do
     const double2 kb0 = make_double2(d.cb[0].x * ib, d.cb[0].y * ib), kb1 = make_double2(d.cb[1].x * ib, d.cb[1].y * ib);
     double2 *dst = X + (size_t)blockIdx.y * N * ny;
     const int npairs = ny / 2 + 1;
-    auto elem = [&](const TI *P0, int y, int x) {
-        const TI v = P0[((size_t)((y + (ny >> 1)) & (ny - 1)) * N + ((x + N / 2) & (N - 1))) * cs];
+    // sparse grids: the mask rows of the block's transform rows, so that
+    // cells of tiles no record reached are read as zeros (never loaded)
+    const bool masked = d.ma != nullptr;
+    if (masked) {
+        for (int i = threadIdx.x; i < 2 * NF * NTX; i += TH) {
+            const int ab = i / (NF * NTX), f = (i / NTX) % NF, t = i % NTX;
+            const uint8_t *m = ab ? d.mb : d.ma;
+            const int pair = blockIdx.x * PB + (f >> 1);
+            uint8_t v = 0;
+            if (m && pair < npairs) {
+                const int y = (f & 1) ? ((ny - pair) & (ny - 1)) : pair;
+                v = m[(((y + (ny >> 1)) & (ny - 1)) >> 5) * NTX + t];
+            }
+            s_mask[ab][f][t] = v;
+        }
+        __syncthreads();
+    }
+    auto elem = [&](const TI *P0, int ab, int f, int y, int x) {
+        const int gx = (x + N / 2) & (N - 1);
+        if (masked && !s_mask[ab][f][gx >> 5]) return make_double2(0.0, 0.0);
+        const TI v = P0[((size_t)((y + (ny >> 1)) & (ny - 1)) * N + gx) * cs];
         return make_double2((double)v.x, (double)v.y);
     };
     auto load = [&](int f, int x) {
@@ -558,11 +579,11 @@ __global__ __launch_bounds__(pk_e(N) / 16) void k_rows_pk(const PackDesc *pd, do
         if (pair >= npairs) return make_double2(0.0, 0.0);
         const int y = (f & 1) ? ((ny - pair) & (ny - 1)) : pair;
         const int ym = (ny - y) & (ny - 1), xm = (N - x) & (N - 1);
-        const double2 a0 = elem(A, y, x), a1 = elem(A, ym, xm);
+        const double2 a0 = elem(A, 0, f, y, x), a1 = elem(A, 0, f ^ 1, ym, xm);
         double2 b0 = make_double2(0.0, 0.0), b1 = b0;
         if (B) {
-            b0 = elem(B, y, x);
-            b1 = elem(B, ym, xm);
+            b0 = elem(B, 1, f, y, x);
+            b1 = elem(B, 1, f ^ 1, ym, xm);
         }
         // S_s at (y, x) and at (-y, -x)
         const double2 s0 = c2add(c2mul(ka0, a0), c2mul(kb0, b0)), s0m = c2add(c2mul(ka0, a1), c2mul(kb0, b1));
@@ -822,6 +843,7 @@ struct ChanRef {
     const double *norm; // [P]
     float *image;       // [P][ny][nx]
     const float *corr_x, *corr_y;
+    const uint8_t *mask = nullptr;  // tile mask of sparse grids (null: complete)
 };
 
 // Packed planes of a list of channels of one image size (the Stokes formulas
@@ -855,7 +877,8 @@ static void make_packs(const std::vector<ChanRef> &ch, int P, int poltype, int l
             const ChanRef &c1 = two ? ch[ord[i + 1]] : c0;
             i += two ? 2 : 1;
             PackDesc d{pl(c0, 0), two ? pl(c1, 0) : nullptr, c0.norm, two ? c1.norm : nullptr,
-                       {one, zero}, {zero, one}, {c0.image, two ? c1.image : nullptr}, c0.corr_x, c0.corr_y};
+                       {one, zero}, {zero, one}, {c0.image, two ? c1.image : nullptr}, c0.corr_x, c0.corr_y,
+                       c0.mask, two ? c1.mask : nullptr};
             out.push_back(d);
         }
         return;
@@ -864,17 +887,17 @@ static void make_packs(const std::vector<ChanRef> &ch, int P, int poltype, int l
         float *im[4] = {c.image, c.image + plane, c.image + 2 * plane, c.image + 3 * plane};
         if (P == 2) {
             out.push_back(PackDesc{pl(c, 0), pl(c, 1), c.norm, c.norm + 1, {h, h}, {h, mh}, {im[0], im[1]},
-                                   c.corr_x, c.corr_y});
+                                   c.corr_x, c.corr_y, c.mask, c.mask});
         } else if (poltype == MTI_POL_LINEAR) {
             out.push_back(PackDesc{pl(c, 0), pl(c, 3), c.norm, c.norm + 3, {h, h}, {h, mh}, {im[0], im[1]},
-                                   c.corr_x, c.corr_y});  // I, Q from XX, YY
+                                   c.corr_x, c.corr_y, c.mask, c.mask});  // I, Q from XX, YY
             out.push_back(PackDesc{pl(c, 1), pl(c, 2), c.norm + 1, c.norm + 2, {h, mih}, {h, ih}, {im[2], im[3]},
-                                   c.corr_x, c.corr_y});  // U, V from XY, YX
+                                   c.corr_x, c.corr_y, c.mask, c.mask});  // U, V from XY, YX
         } else {
             out.push_back(PackDesc{pl(c, 0), pl(c, 3), c.norm, c.norm + 3, {h, h}, {h, mh}, {im[0], im[3]},
-                                   c.corr_x, c.corr_y});  // I, V from RR, LL
+                                   c.corr_x, c.corr_y, c.mask, c.mask});  // I, V from RR, LL
             out.push_back(PackDesc{pl(c, 1), pl(c, 2), c.norm + 1, c.norm + 2, {h, ih}, {h, mih}, {im[1], im[2]},
-                                   c.corr_x, c.corr_y});  // Q, U from RL, LR
+                                   c.corr_x, c.corr_y, c.mask, c.mask});  // Q, U from RL, LR
         }
     }
 }
@@ -1066,6 +1089,15 @@ extern "C" mti_status mti_finalize(mti_ctx *ctx, const mti_image_params *p, int3
                                    int32_t nchan, int32_t grid_layout, int32_t grid_precision,
                                    const void *grids, const double *norm, float *image, void *stream)
 {
+    return mti_finalize_sparse(ctx, p, npol, poltype, nchan, grid_layout, grid_precision, grids, nullptr, norm,
+                               image, stream);
+}
+
+extern "C" mti_status mti_finalize_sparse(mti_ctx *ctx, const mti_image_params *p, int32_t npol, int32_t poltype,
+                                          int32_t nchan, int32_t grid_layout, int32_t grid_precision,
+                                          const void *grids, const uint8_t *tile_mask, const double *norm,
+                                          float *image, void *stream)
+{
     if (!ctx) return MTI_ERR_INVALID;
     mti_status r = validate_params(ctx, p);
     if (r != MTI_OK) return r;
@@ -1099,13 +1131,16 @@ extern "C" mti_status mti_finalize(mti_ctx *ctx, const mti_image_params *p, int3
     MTI_HIP(ctx, hipGetLastError());
     const size_t elem = grid_precision == MTI_GRID_C64 ? sizeof(float2) : sizeof(double2);
     const size_t chan_grid = (size_t)p->nx * p->ny * npol;
+    const size_t tiles = (size_t)((p->nx + 31) / 32) * ((p->ny + 31) / 32);
     if (pk_ok(g)) {
         std::vector<ChanRef> ch(nchan);
         for (int c = 0; c < nchan; c++)
             ch[c] = ChanRef{(const char *)grids + elem * chan_grid * c, norm + (size_t)c * npol,
-                            image + chan_grid * c, corr_x, corr_y};
+                            image + chan_grid * c, corr_x, corr_y, tile_mask ? tile_mask + tiles * c : nullptr};
         return finalize_channels(ctx, g, ch, npol, poltype, grid_layout, grid_precision == MTI_GRID_C64, st);
     }
+    if (tile_mask)
+        return set_error(ctx, MTI_ERR_UNSUPPORTED, "sparse grids need the packed transform (images of 16+ pixels)");
     // small images: one FP64 plane per correlation, transform and finalize per channel batch
     const size_t plane_bytes = (size_t)p->nx * p->ny * sizeof(double2);
     const int batch = std::max(1, transform_batch(g, npol * nchan) / npol);
@@ -1207,8 +1242,8 @@ extern "C" mti_status mti_finalize_jobs(mti_ctx *ctx, int32_t njobs, const mti_j
             for (int i = 0; i < njobs; i++) {
                 const mti_job &j = jobs[i];
                 if (j.params.nx != sz.first || j.params.ny != sz.second) continue;
-                r = mti_finalize(ctx, &j.params, P, poltype, j.nchan, grid_layout, MTI_GRID_C128, j.grids, j.norm,
-                                 j.image, stream);
+                r = mti_finalize_sparse(ctx, &j.params, P, poltype, j.nchan, grid_layout, MTI_GRID_C128, j.grids,
+                                        j.tile_mask, j.norm, j.image, stream);
                 if (r != MTI_OK) return r;
             }
             continue;
@@ -1218,9 +1253,11 @@ extern "C" mti_status mti_finalize_jobs(mti_ctx *ctx, int32_t njobs, const mti_j
         for (int i = 0; i < njobs; i++) {
             const mti_job &j = jobs[i];
             if (j.params.nx != sz.first || j.params.ny != sz.second) continue;
+            const size_t tiles = (size_t)((sz.first + 31) / 32) * ((sz.second + 31) / 32);
             for (int c = 0; c < j.nchan; c++)
                 ch.push_back(ChanRef{(const double2 *)j.grids + chan_grid * c, j.norm + (size_t)c * P,
-                                     j.image + chan_grid * c, corr + coff[i], corr + coff[i] + sz.first});
+                                     j.image + chan_grid * c, corr + coff[i], corr + coff[i] + sz.first,
+                                     j.tile_mask ? j.tile_mask + tiles * c : nullptr});
         }
         r = finalize_channels(ctx, g, ch, P, poltype, grid_layout, false, st);
         if (r != MTI_OK) return r;

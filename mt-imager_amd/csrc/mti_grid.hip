@@ -579,15 +579,24 @@ __global__ __launch_bounds__(TG_THREADS, PREC == MTI_TABLE_F32 ? 8 : 1) void k_g
 // adding with atomics), so that a tile's only unit can store its cells
 // without a prior fill of the grid.  uprefix is the exclusive scan of the
 // units per tile key, total the unit count.  Blocks stride over the keys.
+// A job with a tile mask leaves the tiles no unit reaches unwritten and marks
+// every tile (1: written); only its tiles of several units are zeroed.
 __global__ __launch_bounds__(256) void k_zero_tiles(JobSet js, const uint32_t *uprefix, const uint32_t *total,
                                                    int64_t nkeys)
 {
     const int P = js.P;
     for (int64_t k = blockIdx.x; k < nkeys; k += gridDim.x) {
         const uint32_t u1 = (k + 1 < nkeys) ? uprefix[k + 1] : *total;
-        if (u1 - uprefix[k] == 1) continue;
+        const uint32_t nu = u1 - uprefix[k];
+        if (nu == 1 && !js.masks) continue;
         const TileAt ta = tile_at(js, (int)k);
         const JobDev &jd = js.jobs[ta.job];
+        if (jd.tile_mask) {
+            if (threadIdx.x == 0) jd.tile_mask[k - js.key0[ta.job]] = nu > 0 ? 1 : 0;
+            if (nu <= 1) continue;
+        } else if (nu == 1) {
+            continue;
+        }
         const int nx = jd.g.nx, ny = jd.g.ny, x0 = ta.tx * TG_T, y0 = ta.ty * TG_T;
         const int w = min(TG_T, nx - x0), h = min(TG_T, ny - y0), cells = w * h;
         double2 *g = (double2 *)jd.grid + (int64_t)ta.c_local * jd.chan_stride;
@@ -610,6 +619,12 @@ __global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_fill_u8(uint8_t *p, int64_t n, uint8_t v)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = v;
 }
 
@@ -702,6 +717,7 @@ struct PassJob {
     double *grid;      // FP64 grid of chan0 in the job's layout
     int layout;
     double *norm;      // [nb][P]
+    uint8_t *mask;     // [nb][nty][ntx] tile mask (null: complete grids)
 };
 
 // Limits of one pass: records < 2^31, (record, tile) pairs < 2^32, keys < 2^31.
@@ -714,6 +730,12 @@ static int64_t job_pairs(const PassJob &j)
 static int64_t job_keys(const PassJob &j)
 {
     return (int64_t)j.nb * ((j.p.nx + TG_T - 1) / TG_T) * ((j.p.ny + TG_T - 1) / TG_T);
+}
+
+static PassJob one_channel(PassJob j)
+{
+    j.nb = 1;
+    return j;
 }
 
 // Scratch of one gridding pass over a set of job pieces; grid_pass lays its
@@ -792,6 +814,7 @@ static mti_status grid_pass(mti_ctx *ctx, const std::vector<PassJob> &jobs, cons
         d.norm = j.norm;
         d.psf = o->psf;
         d.weighting = o->weighting;
+        d.tile_mask = j.mask;
         rec0[i + 1] = rec0[i] + job_records(j);
         key0[i + 1] = key0[i] + job_keys(j);
         blk0[i + 1] = blk0[i] + bin_blocks(j.vw.nidx, j.nb);
@@ -804,11 +827,16 @@ static mti_status grid_pass(mti_ctx *ctx, const std::vector<PassJob> &jobs, cons
     const PassSizes z = pass_sizes(jobs, P, chunk);
     const int64_t nrec = z.nrec, nkeys = z.nkeys, nch = z.nch;
     if (nrec == 0) {
-        // nothing to grid: a grid that was not pre-filled is all zeros
+        // nothing to grid: a grid that was not pre-filled is all zeros (or,
+        // with a tile mask, all of its tiles are marked empty)
         if (fresh)
-            for (const PassJob &j : jobs)
-                hipLaunchKernelGGL(k_fill_f64, dim3(1024), dim3(256), 0, st, j.grid,
-                                   (int64_t)j.p.nx * j.p.ny * P * 2 * j.nb, 0.0);
+            for (const PassJob &j : jobs) {
+                if (j.mask)
+                    hipLaunchKernelGGL(k_fill_u8, dim3(64), dim3(256), 0, st, j.mask, job_keys(j), (uint8_t)0);
+                else
+                    hipLaunchKernelGGL(k_fill_f64, dim3(1024), dim3(256), 0, st, j.grid,
+                                       (int64_t)j.p.nx * j.p.ny * P * 2 * j.nb, 0.0);
+            }
         MTI_HIP(ctx, hipGetLastError());
         return MTI_OK;
     }
@@ -868,6 +896,9 @@ static mti_status grid_pass(mti_ctx *ctx, const std::vector<PassJob> &jobs, cons
     js.blk0 = d_blk0;
     js.njobs = nj;
     js.P = P;
+    js.masks = 0;
+    for (const PassJob &j : jobs)
+        if (j.mask) js.masks = 1;
 
     BinPassArgs ba;
     ba.js = js;
@@ -1039,6 +1070,7 @@ static mti_status for_each_pass(const std::vector<PassJob> &all, int P, F fn)
             piece.nb = (int)std::min<int64_t>(j.nb - c0, cap);
             piece.grid = j.grid ? j.grid + 2 * gch * c0 : nullptr;
             piece.norm = j.norm ? j.norm + (size_t)c0 * P : nullptr;
+            piece.mask = j.mask ? j.mask + (size_t)c0 * job_keys(one_channel(j)) : nullptr;
             if (!pass.empty() &&
                 (in_r + job_records(piece) >= (int64_t(1) << 31) || in_p + job_pairs(piece) >= (int64_t(1) << 32) ||
                  in_k + job_keys(piece) >= (int64_t(1) << 31) || pass.size() >= 32767)) {
@@ -1154,6 +1186,11 @@ extern "C" mti_status mti_grid(mti_ctx *ctx, const mti_kernels *k, const mti_ima
     if (r != MTI_OK) return r;
     MTI_CHECK(ctx, grids && norm, "null outputs");
     MTI_CHECK(ctx, o->nchan >= 1 && o->nchan <= 65535, "opts.nchan must be in [1, 65535]");
+    if (o->tile_mask) {
+        MTI_CHECK(ctx, !o->accumulate, "a tile mask (sparse grids) needs accumulate = 0");
+        if (o->grid_precision != MTI_GRID_C128)
+            return set_error(ctx, MTI_ERR_UNSUPPORTED, "a tile mask (sparse grids) needs complex128 grids");
+    }
     if (v->inv_lambda_per_row)
         MTI_CHECK(ctx, o->nchan == 1, "per-row 1/lambda implies a single channel per call");
     else
@@ -1176,8 +1213,13 @@ extern "C" mti_status mti_grid(mti_ctx *ctx, const mti_kernels *k, const mti_ima
     if (!o->accumulate) {
         hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(256), 0, st, norm, (int64_t)nb * P, 0.0);
         // with rows, the gridding pass zeroes exactly the tiles it does not store whole
-        if (o->grid_precision == MTI_GRID_C128 && vw.nidx == 0)
-            hipLaunchKernelGGL(k_fill_f64, dim3(8192), dim3(256), 0, st, (double *)grids, chan_floats * nb, 0.0);
+        if (o->grid_precision == MTI_GRID_C128 && vw.nidx == 0) {
+            if (o->tile_mask)
+                hipLaunchKernelGGL(k_fill_u8, dim3(64), dim3(256), 0, st, o->tile_mask,
+                                   (int64_t)nb * ((p->nx + TG_T - 1) / TG_T) * ((p->ny + TG_T - 1) / TG_T), (uint8_t)0);
+            else
+                hipLaunchKernelGGL(k_fill_f64, dim3(8192), dim3(256), 0, st, (double *)grids, chan_floats * nb, 0.0);
+        }
         MTI_HIP(ctx, hipGetLastError());
     }
     void *sw = nullptr;
@@ -1202,6 +1244,7 @@ extern "C" mti_status mti_grid(mti_ctx *ctx, const mti_kernels *k, const mti_ima
             j.nb = std::min(pair_batch, nb - c0);
             j.grid = (double *)grids + (size_t)c0 * chan_floats;
             j.norm = norm + (size_t)c0 * P;
+            j.mask = o->tile_mask ? o->tile_mask + (size_t)c0 * job_keys(one_channel(base)) : nullptr;
             r = grid_pass(ctx, {j}, o, P, prec, !o->accumulate, dstats, nu, st);
             if (r != MTI_OK) return r;
         }
@@ -1290,8 +1333,9 @@ extern "C" mti_status mti_grid_jobs(mti_ctx *ctx, int32_t njobs, const mti_job *
         }
         MTI_CHECK(ctx, j.vis.npol == P, "jobs of one call must share npol");
         MTI_CHECK(ctx, j.kernels->info.precision == prec, "jobs of one call must share the table precision");
+        if (j.tile_mask) MTI_CHECK(ctx, !o->accumulate, "a job tile mask (sparse grids) needs accumulate = 0");
         const PassJob pj{j.kernels, j.params, make_view(&j.vis), j.chan0, j.nchan, (double *)j.grids, o->grid_layout,
-                         j.norm};
+                         j.norm, j.tile_mask};
         MTI_CHECK(ctx, pj.vw.nidx < (int64_t(1) << 31), "more than 2^31-1 rows in a job");
         MTI_CHECK(ctx, channels_per_pass(pj) >= 1, "one channel has too many (record, tile) pairs");
         all.push_back(pj);

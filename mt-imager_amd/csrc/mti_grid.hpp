@@ -52,6 +52,7 @@ struct JobDev {
     double *norm;            // [nb][P]
     double *f2;              // [nb][P] Briggs factor
     int psf, weighting;
+    uint8_t *tile_mask;      // [nb][nty][ntx]: 1 where a record reaches the tile; null: complete grids
 };
 
 // The jobs of a call and prefix tables for locating them.
@@ -62,6 +63,7 @@ struct JobSet {
     const int64_t *blk0;     // [njobs + 1] binning-block prefix
     int njobs;
     int P;
+    int masks;               // some job leaves untouched tiles unwritten (tile_mask)
 };
 
 // Largest j with prefix[j] <= x (prefix ascending, prefix[0] = 0).

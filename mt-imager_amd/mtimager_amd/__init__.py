@@ -58,7 +58,8 @@ class GridOpts(ctypes.Structure):
     _fields_ = [("chan0", ctypes.c_int32), ("nchan", ctypes.c_int32),
                 ("accumulate", ctypes.c_int32), ("psf", ctypes.c_int32),
                 ("weighting", ctypes.c_int32), ("robust", ctypes.c_double),
-                ("grid_layout", ctypes.c_int32), ("grid_precision", ctypes.c_int32)]
+                ("grid_layout", ctypes.c_int32), ("grid_precision", ctypes.c_int32),
+                ("tile_mask", ctypes.c_void_p)]
 
 
 class Job(ctypes.Structure):
@@ -66,7 +67,8 @@ class Job(ctypes.Structure):
     _fields_ = [("kernels", ctypes.c_void_p), ("params", ImageParams), ("vis", Vis),
                 ("chan0", ctypes.c_int32), ("nchan", ctypes.c_int32),
                 ("grids", ctypes.c_void_p), ("norm", ctypes.c_void_p), ("image", ctypes.c_void_p),
-                ("ant1", ctypes.c_void_p), ("ant2", ctypes.c_void_p), ("nant", ctypes.c_int32)]
+                ("ant1", ctypes.c_void_p), ("ant2", ctypes.c_void_p), ("nant", ctypes.c_int32),
+                ("tile_mask", ctypes.c_void_p)]
 
 
 class GridStats(ctypes.Structure):
@@ -93,7 +95,7 @@ EXPORTS = ["mti_version", "mti_create", "mti_destroy", "mti_last_error",
            "mti_finalize_workspace_size", "mti_finalize_jobs_workspace_size", "mti_fft2d_workspace_size",
            "mti_set_workspace", "mti_context_device_bytes", "mti_set_tuning", "mti_get_tuning",
            "mti_graph_create", "mti_graph_launch", "mti_graph_destroy",
-           "mti_fits_create", "mti_fits_write_channel", "mti_fits_close"]
+           "mti_fits_create", "mti_fits_write_channel", "mti_fits_close", "mti_finalize_sparse"]
 
 _lib = None
 
@@ -163,7 +165,8 @@ def lib():
             ("mti_set_tuning", [vp, P(Tuning)]),
             ("mti_get_tuning", [vp, P(Tuning)]),
             ("mti_graph_create", [vp, i32, vp, P(GridOpts), i32, i32, P(vp)]),
-            ("mti_graph_launch", [vp, vp])]:
+            ("mti_graph_launch", [vp, vp]),
+            ("mti_finalize_sparse", [vp, P(ImageParams), i32, i32, i32, i32, i32, vp, vp, vp, vp, vp])]:
         f = getattr(L, name)
         f.restype = ctypes.c_int
         f.argtypes = args
@@ -187,9 +190,15 @@ def make_params(nx, ny, cell_l, cell_m=None, mode="wproj", wplanes=1, sampling=4
 
 
 def _grid_opts(chan0=0, nchan=1, accumulate=False, psf=False, weighting="natural", robust=0.0,
-               layout="vup", precision="c128"):
+               layout="vup", precision="c128", tile_mask=None):
     return GridOpts(int(chan0), int(nchan), 1 if accumulate else 0, 1 if psf else 0,
-                    WEIGHTING[weighting], float(robust), LAYOUT[layout], GRID_PRECISION[precision])
+                    WEIGHTING[weighting], float(robust), LAYOUT[layout], GRID_PRECISION[precision],
+                    tile_mask.data_ptr() if tile_mask is not None else None)
+
+
+def tile_mask_shape(params, nchan):
+    """Shape of a tile mask (sparse grids): [nchan][ceil(ny/32)][ceil(nx/32)], uint8."""
+    return (int(nchan), (params.ny + 31) // 32, (params.nx + 31) // 32)
 
 
 def _query(fn, *args):
@@ -355,13 +364,16 @@ class Context:
     # ---- gridding ---------------------------------------------------------
     def grid(self, kern, params, vis, grids, norm, chan0=0, nchan=1, accumulate=False,
              psf=False, weighting="natural", robust=0.0, layout="vup", precision=None,
-             stats=True, stream=None):
+             stats=True, stream=None, tile_mask=None):
         """grids: per channel [ny][nx][npol] (layout "vup", the reference's
         Grid) or [npol][ny][nx] ("pvu"); complex64 or complex128 (precision
-        "c64"/"c128", taken from the tensor dtype by default)."""
+        "c64"/"c128", taken from the tensor dtype by default).  tile_mask
+        (uint8 device tensor of tile_mask_shape): sparse grids -- tiles no
+        record reaches are marked 0 and left unwritten (finalize with the
+        same mask)."""
         if precision is None:
             precision = "c128" if str(grids.dtype) in ("torch.complex128", "torch.float64") else "c64"
-        o = _grid_opts(chan0, nchan, accumulate, psf, weighting, robust, layout, precision)
+        o = _grid_opts(chan0, nchan, accumulate, psf, weighting, robust, layout, precision, tile_mask)
         st = GridStats()
         rc = lib().mti_grid(self.h, kern.h, ctypes.byref(params), ctypes.byref(vis),
                             ctypes.byref(o), grids.data_ptr(), norm.data_ptr(),
@@ -377,19 +389,20 @@ class Context:
 
     # ---- finalize -----------------------------------------------------------
     def finalize(self, params, npol, grids, norm, image, nchan=1, poltype="linear",
-                 layout="vup", precision=None, stream=None):
+                 layout="vup", precision=None, stream=None, tile_mask=None):
         if precision is None:
             precision = "c128" if str(grids.dtype) in ("torch.complex128", "torch.float64") else "c64"
-        rc = lib().mti_finalize(self.h, ctypes.byref(params), int(npol), POLTYPE[poltype],
-                                int(nchan), LAYOUT[layout], GRID_PRECISION[precision],
-                                grids.data_ptr(), norm.data_ptr(), image.data_ptr(),
-                                _stream(stream))
+        rc = lib().mti_finalize_sparse(self.h, ctypes.byref(params), int(npol), POLTYPE[poltype],
+                                       int(nchan), LAYOUT[layout], GRID_PRECISION[precision],
+                                       grids.data_ptr(), tile_mask.data_ptr() if tile_mask is not None else None,
+                                       norm.data_ptr(), image.data_ptr(), _stream(stream))
         self.check(rc, "mti_finalize")
 
 
     # ---- batched jobs ---------------------------------------------------------
     @staticmethod
-    def make_job(kern, params, vis, grids, norm, image=None, chan0=0, nchan=1, ant1=None, ant2=None, nant=0):
+    def make_job(kern, params, vis, grids, norm, image=None, chan0=0, nchan=1, ant1=None, ant2=None, nant=0,
+                 tile_mask=None):
         """An mti_job; grids complex128 [nchan] grids, norm f64 [nchan][npol],
         image f32 [nchan][npol][ny][nx] (for finalize_jobs).  With ant1/ant2
         (int32 device tensors of vis's rows) and nant, grid_jobs computes the
@@ -403,7 +416,9 @@ class Context:
         j.image = image.data_ptr() if image is not None else None
         if ant1 is not None:
             j.ant1, j.ant2, j.nant = ant1.data_ptr(), ant2.data_ptr(), int(nant)
-        j._keep = (kern, vis, grids, norm, image, ant1, ant2)
+        if tile_mask is not None:
+            j.tile_mask = tile_mask.data_ptr()
+        j._keep = (kern, vis, grids, norm, image, ant1, ant2, tile_mask)
         return j
 
     def grid_jobs(self, jobs, accumulate=False, psf=False, weighting="natural", robust=0.0,
