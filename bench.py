@@ -105,7 +105,7 @@ def achievable_peaks():
     return dict(copy_GBps=round(gbps.value, 1), fp64_fma_TFLOPs=round(tflops.value, 2))
 
 
-def end_to_end(torch, mti, ctx, d, p, grids, norm, image, ref_image, group=8):
+def end_to_end(torch, mti, ctx, d, p, grids, norm, image, ref_image, group=8, tmask=None):
     """One imaging job from pinned host memory: the job's inputs (channel-major,
     so a block of channels is one contiguous copy) go to the device on a copy
     stream in blocks of `group` channels, double-buffered, while the compute
@@ -159,10 +159,10 @@ def end_to_end(torch, mti, ctx, d, p, grids, norm, image, ref_image, group=8):
         v = mti.vis_chan_major(small_d["uvw"], small_d["invl"][c0:c0 + n], vis_d[b][:n], flg_d[b][:n],
                                small_d["weight"], idx[:kept])
         ctx.grid(K, p, v, grids[c0:c0 + n], norm[c0:c0 + n], chan0=0, nchan=n, weighting="briggs", robust=0.0,
-                 layout="pvu", precision="c128", stats=False)
+                 layout="pvu", precision="c128", stats=False, tile_mask=None if tmask is None else tmask[c0:c0 + n])
         done[b] = torch.cuda.Event()
         done[b].record(gs)
-    ctx.finalize(p, P, grids, norm, image, nchan=nchan, layout="pvu", precision="c128")
+    ctx.finalize(p, P, grids, norm, image, nchan=nchan, layout="pvu", precision="c128", tile_mask=tmask)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     K.free()
@@ -270,10 +270,10 @@ def graph_replay(ctx, torch, mti, jobs, outputs, max_jobs=64, reps=7):
     def make():
         out, js = [], []
         for j in small:
-            gr, nr, im = outputs(j)
+            gr, nr, im, mk = outputs(j)
             v = mti.vis_ms(j["uvw"], j["invl"], j["vis"], j["flags"], j["weight"])
             js.append(ctx.make_job(kc[j["key"]], j["p"], v, gr, nr, im, 0, j["nch"],
-                                   ant1=j["a1"], ant2=j["a2"], nant=64))
+                                   ant1=j["a1"], ant2=j["a2"], nant=64, tile_mask=mk))
             out.append(im)
         return js, out
 
@@ -323,10 +323,13 @@ def run_c4(args, torch, mti, jobs):
         return mti.vis_ms(j["uvw"], j["invl"], j["vis"], j["flags"], j["weight"], idx[:kept])
 
     def outputs(j):
+        # grids are intermediates here: sparse (tile mask) unless asked otherwise
         nx, nch = j["nx"], j["nch"]
+        mask = None if args.complete_grids else \
+            torch.empty(mti.tile_mask_shape(j["p"], nch), dtype=torch.uint8, device="cuda")
         return (torch.empty((nch, nx, nx, 1, 2), dtype=torch.float64, device="cuda"),
                 torch.empty((nch, 1), dtype=torch.float64, device="cuda"),
-                torch.empty((nch, 1, nx, nx), dtype=torch.float32, device="cuda"))
+                torch.empty((nch, 1, nx, nx), dtype=torch.float32, device="cuda"), mask)
 
     def plain():
         lat, gridded = [], 0
@@ -336,9 +339,10 @@ def run_c4(args, torch, mti, jobs):
                 log(f"c4: plain job {n}")
             v = prep(j)
             K = ctx.build_kernels(j["p"])
-            gr, nr, im = outputs(j)
-            st = ctx.grid(K, j["p"], v, gr, nr, chan0=0, nchan=j["nch"], precision="c128", stats=True)
-            ctx.finalize(j["p"], 1, gr, nr, im, nchan=j["nch"], precision="c128")
+            gr, nr, im, mk = outputs(j)
+            st = ctx.grid(K, j["p"], v, gr, nr, chan0=0, nchan=j["nch"], precision="c128", stats=True,
+                          tile_mask=mk)
+            ctx.finalize(j["p"], 1, gr, nr, im, nchan=j["nch"], precision="c128", tile_mask=mk)
             torch.cuda.synchronize()
             K.free()
             gridded += st.gridded_records
@@ -370,9 +374,9 @@ def run_c4(args, torch, mti, jobs):
                 flush()
             # the batched call puts each job's rows in baseline-major order itself
             v = mti.vis_ms(j["uvw"], j["invl"], j["vis"], j["flags"], j["weight"])
-            gr, nr, im = outputs(j)
+            gr, nr, im, mk = outputs(j)
             wave.append((ctx.make_job(cache[j["key"]], j["p"], v, gr, nr, im, 0, j["nch"],
-                                      ant1=j["a1"], ant2=j["a2"], nant=64), gr, nr, im))
+                                      ant1=j["a1"], ant2=j["a2"], nant=64, tile_mask=mk), gr, nr, im, mk))
             mem += need
         flush()
         el = time.perf_counter() - t0
@@ -398,10 +402,10 @@ def run_c4(args, torch, mti, jobs):
         for j in sample:
             if j["key"] not in kc:
                 kc[j["key"]] = ctx.build_kernels(j["p"])
-            gr, nr, im = outputs(j)
+            gr, nr, im, mk = outputs(j)
             v = mti.vis_ms(j["uvw"], j["invl"], j["vis"], j["flags"], j["weight"])
             tjobs.append(ctx.make_job(kc[j["key"]], j["p"], v, gr, nr, im, 0, j["nch"],
-                                      ant1=j["a1"], ant2=j["a2"], nant=64))
+                                      ant1=j["a1"], ant2=j["a2"], nant=64, tile_mask=mk))
 
         def tune_run():
             ctx.grid_jobs(tjobs, stats=False)
@@ -538,6 +542,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="config 4: keep the default tuning")
     ap.add_argument("--no-e2e", action="store_true", help="configs 1/2: skip the pinned-host end-to-end run")
+    ap.add_argument("--complete-grids", action="store_true",
+                    help="zero-fill and transform every grid tile (default: sparse grids with a tile mask)")
     ap.add_argument("--xcd-chunk", type=int, default=None, help="configs 1/2: gridder units per XCD run")
     ap.add_argument("--unit-pairs", type=int, default=None, help="configs 1/2: gridder work-unit size")
     ap.add_argument("--cpu-rows", type=int, default=60000)
@@ -590,14 +596,18 @@ def main():
     norm = torch.empty((nchan, P), dtype=torch.float64, device="cuda")
     image = torch.empty((nchan, P, nx, nx), dtype=torch.float32, device="cuda")
     idx = torch.empty(nrows, dtype=torch.int32, device="cuda")
+    # the grids are intermediates: sparse (tiles no record reaches are neither
+    # zero-filled nor read by the transform) unless --complete-grids
+    tmask = None if args.complete_grids else \
+        torch.empty(mti.tile_mask_shape(p, nchan), dtype=torch.uint8, device="cuda")
 
     def step(stats=False):
         kept = ctx.sort_rows(d["a1"], d["a2"], d["nant"], idx)
         K = ctx.build_kernels(p)
         v = mti.vis_ms(d["uvw"], d["invl"], d["vis"], d["flags"], d["weight"], idx[:kept])
         st = ctx.grid(K, p, v, grids, norm, chan0=0, nchan=nchan, weighting="briggs", robust=0.0,
-                      layout="pvu", precision="c128", stats=stats)
-        ctx.finalize(p, P, grids, norm, image, nchan=nchan, layout="pvu", precision="c128")
+                      layout="pvu", precision="c128", stats=stats, tile_mask=tmask)
+        ctx.finalize(p, P, grids, norm, image, nchan=nchan, layout="pvu", precision="c128", tile_mask=tmask)
         K.free()
         return st, K
 
@@ -662,7 +672,7 @@ def main():
     e2e = None
     if not args.no_e2e:
         ref_image = image.clone()
-        e2e = end_to_end(torch, mti, ctx, d, p, grids, norm, image, ref_image)
+        e2e = end_to_end(torch, mti, ctx, d, p, grids, norm, image, ref_image, tmask=tmask)
         e2e["mvis_s"] = round(gridded / e2e["seconds"] / 1e6, 3)
         log(f"end-to-end: {e2e}")
         del ref_image
@@ -707,7 +717,7 @@ def main():
             "config1: 27 ant / 3 km, 351 baselines x 300 steps, 8 ch, P=4, 512^2 @ 2 arcsec, 32 planes, s=4",
             "rows": nrows, "channels": nchan, "gridded_vis": gridded, "gridded_points": points,
             "supports": [int(sup.min()), int(sup.max())], "path": "plain single-job",
-            "tuning": ctx.get_tuning(),
+            "tuning": ctx.get_tuning(), "grids": "complete" if args.complete_grids else "sparse (tile mask)",
         },
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                        "p99": round(float(np.percentile(lat, 99)), 3)},

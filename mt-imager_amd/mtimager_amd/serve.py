@@ -249,10 +249,13 @@ class Server:
         (separate per-stream pools would each keep their peak)."""
         torch = self.torch
         p = r.params
+        from . import tile_mask_shape
         with torch.cuda.stream(torch.cuda.default_stream(self.device)):
             out = (torch.empty((r.nchan, p.ny, p.nx, r.npol, 2), dtype=torch.float64, device="cuda"),
                    torch.empty((r.nchan, r.npol), dtype=torch.float64, device="cuda"),
-                   torch.empty((r.nchan, r.npol, p.ny, p.nx), dtype=torch.float32, device="cuda"))
+                   torch.empty((r.nchan, r.npol, p.ny, p.nx), dtype=torch.float32, device="cuda"),
+                   # the grids are internal: sparse, with a tile mask
+                   torch.empty(tile_mask_shape(p, r.nchan), dtype=torch.uint8, device="cuda"))
         for t in out:
             t.record_stream(stream)
         return out
@@ -276,17 +279,18 @@ class Server:
                     K = self.cache.get(ctx, p, sid)
                 else:
                     K = ctx.build_kernels(p, "f32", sid)
-                grids, norm, image = self._outputs(r, stream)
+                grids, norm, image, mask = self._outputs(r, stream)
                 jobs.append(ctx.make_job(K, p, v, grids, norm, image, r.chan0, r.nchan,
-                                         ant1=r.ant1, ant2=r.ant2, nant=r.nant))
-                owned.append((r, K, v, grids, norm, image))
+                                         ant1=r.ant1, ant2=r.ant2, nant=r.nant, tile_mask=mask))
+                owned.append((r, K, v, grids, norm, image, mask))
             t1 = time.perf_counter()
             if self.plain:
-                r, K, v, grids, norm, image = owned[0]
+                r, K, v, grids, norm, image, mask = owned[0]
                 st = ctx.grid(K, r.params, v, grids, norm, chan0=r.chan0, nchan=r.nchan,
-                              weighting=self.weighting, robust=self.robust, precision="c128", stream=sid)
+                              weighting=self.weighting, robust=self.robust, precision="c128", stream=sid,
+                              tile_mask=mask)
                 ctx.finalize(r.params, r.npol, grids, norm, image, nchan=r.nchan, poltype=self.poltype,
-                             precision="c128", stream=sid)
+                             precision="c128", stream=sid, tile_mask=mask)
             else:
                 st = ctx.grid_jobs(jobs, weighting=self.weighting, robust=self.robust, stream=sid)
                 ctx.finalize_jobs(jobs, poltype=self.poltype, stream=sid)
@@ -298,7 +302,7 @@ class Server:
                                   sum(r.params.nx * r.params.ny * r.nchan * r.npol for r in wave), t1 - t0, t2 - t1))
         if len(owned) == 1:
             owned[0][0].gridded = st.gridded_records
-        for r, K, v, grids, norm, image in owned:
+        for r, K, v, grids, norm, image, mask in owned:
             if r.keep_image:
                 r.image, r.norm = image, norm
             if self.cache is None:
