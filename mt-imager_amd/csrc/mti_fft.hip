@@ -30,6 +30,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <type_traits>
 
 namespace mti {
 
@@ -537,6 +538,40 @@ __device__ __forceinline__ void fb_ifft(double2 *lds, const double2 *tw, int NT,
 // a read within the block.  Rows 0 and ny/2 pair with themselves.
 constexpr int pk_e(int N) { return 2 * N > FB_E ? 2 * N : FB_E; }
 
+// Grid elements loaded through global-address-space pointers: the grid
+// pointers come from a descriptor in memory, and through a generic pointer
+// a load becomes a FLAT instruction, which also counts against the LDS wait
+// counter -- every LDS read of the tile mask would then wait for all grid
+// loads in flight.  Raw POD types (HIP vector types cannot be copied in that
+// address space by the host compile pass).
+#define MTI_GLOBAL __attribute__((address_space(1)))
+struct Raw128 {
+    unsigned long long lo, hi;
+};
+template <typename TI>
+using RawOf = typename std::conditional<sizeof(TI) == 16, Raw128, unsigned long long>::type;
+
+// Element i of a grid of complex128 (Raw128) or complex64 (one 64-bit word).
+__device__ __forceinline__ void raw_load(const void *base, size_t i, Raw128 &t)
+{
+    const MTI_GLOBAL unsigned long long *p = (const MTI_GLOBAL unsigned long long *)base + 2 * i;
+    t.lo = p[0];
+    t.hi = p[1];
+}
+__device__ __forceinline__ void raw_load(const void *base, size_t i, unsigned long long &t)
+{
+    t = ((const MTI_GLOBAL unsigned long long *)base)[i];
+}
+
+__device__ __forceinline__ double2 raw_to_c(Raw128 t)
+{
+    return make_double2(__longlong_as_double((long long)t.lo), __longlong_as_double((long long)t.hi));
+}
+__device__ __forceinline__ double2 raw_to_c(unsigned long long t)
+{
+    return make_double2((double)__uint_as_float((unsigned)t), (double)__uint_as_float((unsigned)(t >> 32)));
+}
+
 template <int N, typename TI>
 __global__ __launch_bounds__(pk_e(N) / 16) void k_rows_pk(const PackDesc *pd, double2 *X, int ny, int64_t cs,
                                                          const double2 *tw, int NT)
@@ -551,28 +586,28 @@ __global__ __launch_bounds__(pk_e(N) / 16) void k_rows_pk(const PackDesc *pd, do
     const double2 kb0 = make_double2(d.cb[0].x * ib, d.cb[0].y * ib), kb1 = make_double2(d.cb[1].x * ib, d.cb[1].y * ib);
     double2 *dst = X + (size_t)blockIdx.y * N * ny;
     const int npairs = ny / 2 + 1;
-    // sparse grids: the mask rows of the block's transform rows, so that
-    // cells of tiles no record reached are read as zeros (never loaded)
-    const bool masked = d.ma != nullptr;
-    if (masked) {
-        for (int i = threadIdx.x; i < 2 * NF * NTX; i += TH) {
-            const int ab = i / (NF * NTX), f = (i / NTX) % NF, t = i % NTX;
-            const uint8_t *m = ab ? d.mb : d.ma;
-            const int pair = blockIdx.x * PB + (f >> 1);
-            uint8_t v = 0;
-            if (m && pair < npairs) {
-                const int y = (f & 1) ? ((ny - pair) & (ny - 1)) : pair;
-                v = m[(((y + (ny >> 1)) & (ny - 1)) >> 5) * NTX + t];
-            }
-            s_mask[ab][f][t] = v;
+    // the tile-mask rows of the block's transform rows (all 1 for complete
+    // grids): cells of tiles no record reached are read as zeros, never loaded
+    for (int i = threadIdx.x; i < 2 * NF * NTX; i += TH) {
+        const int ab = i / (NF * NTX), f = (i / NTX) % NF, t = i % NTX;
+        const uint8_t *m = ab ? d.mb : d.ma;
+        const int pair = blockIdx.x * PB + (f >> 1);
+        uint8_t v = 1;
+        if (m && pair < npairs) {
+            const int y = (f & 1) ? ((ny - pair) & (ny - 1)) : pair;
+            v = m[(((y + (ny >> 1)) & (ny - 1)) >> 5) * NTX + t];
         }
-        __syncthreads();
+        s_mask[ab][f][t] = v;
     }
+    __syncthreads();
+    // grid loads through global-address-space pointers (see raw_load)
+    using Raw = RawOf<TI>;
     auto elem = [&](const TI *P0, int ab, int f, int y, int x) {
         const int gx = (x + N / 2) & (N - 1);
-        if (masked && !s_mask[ab][f][gx >> 5]) return make_double2(0.0, 0.0);
-        const TI v = P0[((size_t)((y + (ny >> 1)) & (ny - 1)) * N + gx) * cs];
-        return make_double2((double)v.x, (double)v.y);
+        if (!s_mask[ab][f][gx >> 5]) return make_double2(0.0, 0.0);
+        Raw t;
+        raw_load(P0, ((size_t)((y + (ny >> 1)) & (ny - 1)) * N + gx) * cs, t);
+        return raw_to_c(t);
     };
     auto load = [&](int f, int x) {
         const int pair = blockIdx.x * PB + (f >> 1);
