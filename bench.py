@@ -315,6 +315,7 @@ def run_c4(args, torch, mti, jobs):
     mti_grid_jobs + one mti_finalize_jobs call per wave).  Returns a dict."""
     import numpy as np
     ctx = mti.Context(0)
+    ctx.set_transform_storage(args.transform_storage)
     budget = args.c4_budget_gb * (1 << 30)
 
     def prep(j):
@@ -432,6 +433,7 @@ def run_c4(args, torch, mti, jobs):
     assert gp == gb, (gp, gb)
     lp, lb = np.array(lp) * 1e3, np.array(lb) * 1e3
     return dict(jobs=len(jobs), gridded_vis=int(gb), kernel_sets=nk, hbm_budget_gb=args.c4_budget_gb,
+                transform_storage=args.transform_storage,
                 autotune=tune_info, graph=graph_info,
                 plain=dict(mvis_s=round(gp / tp / 1e6, 3), seconds=round(tp, 3),
                            latency_ms={"p50": round(float(np.percentile(lp, 50)), 1),
@@ -496,8 +498,8 @@ def run_c5(args, torch, mti, pool):
     from mtimager_amd import serve
     budget = int(args.serve_budget_gb * (1 << 30))
     cache = serve.KernelCache()
-    served = serve.Server(budget, workers=2, max_wave=512, cache=cache)
-    plain = serve.Server(budget, plain=True)
+    served = serve.Server(budget, workers=2, max_wave=512, cache=cache, transform_storage=args.transform_storage)
+    plain = serve.Server(budget, plain=True, transform_storage=args.transform_storage)
     rng = np.random.default_rng(5)
     out = []
     try:
@@ -546,6 +548,8 @@ def main():
                     help="zero-fill and transform every grid tile (default: sparse grids with a tile mask)")
     ap.add_argument("--xcd-chunk", type=int, default=None, help="configs 1/2: gridder units per XCD run")
     ap.add_argument("--unit-pairs", type=int, default=None, help="configs 1/2: gridder work-unit size")
+    ap.add_argument("--transform-storage", choices=["c128", "c64"], default="c64",
+                    help="element type of the packed planes between the finalize's transform passes")
     ap.add_argument("--cpu-rows", type=int, default=60000)
     ap.add_argument("--cpu-threads", type=int, default=16, help="threads of the CPU baseline (one channel each)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -591,6 +595,7 @@ def main():
     p = mti.make_params(d["nx"], d["nx"], d["cell"], d["cell"], "wproj", d["W"], d["s"])
     ctx = mti.Context(0)
     ctx.set_tuning(unit_pairs=args.unit_pairs, xcd_chunk=args.xcd_chunk)
+    ctx.set_transform_storage(args.transform_storage)
     nrows, nchan, P, nx = d["nrows"], d["nchan"], d["P"], d["nx"]
     grids = torch.empty((nchan, P, nx, nx, 2), dtype=torch.float64, device="cuda")
     norm = torch.empty((nchan, P), dtype=torch.float64, device="cuda")
@@ -669,6 +674,25 @@ def main():
                     achievable=peaks,
                     frac_of_achievable=(round(achieved / peaks["fp64_fma_TFLOPs"], 4) if peaks else None))
 
+    storage_err = None
+    if args.transform_storage == "c64":
+        # the last step's grids finalized again through complex128 planes: the
+        # difference the complex64 planes make, per channel and Stokes image
+        # relative to the RMS of the channel's Stokes I (worst over all)
+        ctx.set_transform_storage("c128")
+        ref = torch.empty_like(image)
+        ctx.finalize(p, P, grids, norm, ref, nchan=nchan, layout="pvu", precision="c128", tile_mask=tmask)
+        torch.cuda.synchronize()
+        ctx.set_transform_storage("c64")
+        errs = []
+        for c in range(nchan):
+            dd = image[c].double() - ref[c].double()
+            ri = ref[c, 0].double().pow(2).mean().sqrt()
+            errs.append(float((dd.pow(2).mean(dim=(1, 2)).sqrt() / ri).max()))
+        storage_err = max(errs)
+        del ref, dd
+        log(f"complex64 transform storage: max relative RMS vs complex128 storage {storage_err:.3e}")
+
     e2e = None
     if not args.no_e2e:
         ref_image = image.clone()
@@ -718,7 +742,11 @@ def main():
             "rows": nrows, "channels": nchan, "gridded_vis": gridded, "gridded_points": points,
             "supports": [int(sup.min()), int(sup.max())], "path": "plain single-job",
             "tuning": ctx.get_tuning(), "grids": "complete" if args.complete_grids else "sparse (tile mask)",
+            "transform_storage": args.transform_storage,
         },
+        "image_error_vs_c128_storage": (None if storage_err is None else
+                                        {"max_rel_rms": float(f"{storage_err:.3e}"),
+                                         "over": "every channel and Stokes image, relative to Stokes I RMS"}),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                        "p99": round(float(np.percentile(lat, 99)), 3)},
         "stages_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in stages.items()},

@@ -330,6 +330,20 @@ typedef struct {
 mti_status mti_set_tuning(mti_ctx *ctx, const mti_tuning *t);
 mti_status mti_get_tuning(const mti_ctx *ctx, mti_tuning *t);
 
+/* ---- transform storage (a precision form) -----------------------------------
+ * The packed planes that the finalize's transform passes hand to each other
+ * (mti_finalize, mti_finalize_sparse, mti_finalize_jobs, and graphs captured
+ * while the setting holds) are stored as complex128 (MTI_GRID_C128, the
+ * default) or complex64 (MTI_GRID_C64); butterflies, twiddles and
+ * corrections are FP64 either way.  complex64 is the element type of the
+ * reference's transform arrays (IFFT2D on complex64,
+ * gafw/StandardOperators/IFFT2D.cpp:86-111) and moves half the bytes between
+ * passes; its images differ from the complex128 form by the rounding of two
+ * intermediate stores (relative RMS ~1e-7).  The small-image path (nx or ny
+ * below 16) always stores complex128. */
+mti_status mti_set_transform_storage(mti_ctx *ctx, int32_t grid_precision);
+mti_status mti_get_transform_storage(const mti_ctx *ctx, int32_t *grid_precision);
+
 /* ---- output shapes and scratch sizes (validate) ----------------------------
  * The validate() half of the reference's operator contract
  * (GPUArrayOperator::validate, gpu/GPUArrayOperator.h:27-43): output

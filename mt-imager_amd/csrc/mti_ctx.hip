@@ -75,6 +75,23 @@ extern "C" mti_status mti_get_tuning(const mti_ctx *ctx, mti_tuning *t)
     return MTI_OK;
 }
 
+extern "C" mti_status mti_set_transform_storage(mti_ctx *ctx, int32_t grid_precision)
+{
+    if (!ctx) return MTI_ERR_INVALID;
+    MTI_CHECK(ctx, ctx->call_depth == 0, "mti_set_transform_storage during a call");
+    MTI_CHECK(ctx, grid_precision == MTI_GRID_C64 || grid_precision == MTI_GRID_C128,
+              "transform storage must be MTI_GRID_C64 or MTI_GRID_C128");
+    ctx->xform_c64 = grid_precision == MTI_GRID_C64 ? 1 : 0;
+    return MTI_OK;
+}
+
+extern "C" mti_status mti_get_transform_storage(const mti_ctx *ctx, int32_t *grid_precision)
+{
+    if (!ctx || !grid_precision) return MTI_ERR_INVALID;
+    *grid_precision = ctx->xform_c64 ? MTI_GRID_C64 : MTI_GRID_C128;
+    return MTI_OK;
+}
+
 extern "C" mti_status mti_context_device_bytes(const mti_ctx *ctx, size_t *bytes)
 {
     if (!ctx || !bytes) return MTI_ERR_INVALID;

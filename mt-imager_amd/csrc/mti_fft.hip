@@ -379,6 +379,23 @@ __device__ __forceinline__ double2 c2mul(double2 a, double2 b)
 }
 __device__ __forceinline__ double2 c2muli(double2 a) { return make_double2(-a.y, a.x); }  // * (+i)
 
+// Packed planes X between the passes are stored as complex128 or complex64
+// (XT); the arithmetic is FP64 either way.
+__device__ __forceinline__ double2 x_ld(double2 v) { return v; }
+__device__ __forceinline__ double2 x_ld(float2 v) { return make_double2(v.x, v.y); }
+template <typename XT>
+__device__ __forceinline__ XT x_st(double2 v);
+template <>
+__device__ __forceinline__ double2 x_st<double2>(double2 v)
+{
+    return v;
+}
+template <>
+__device__ __forceinline__ float2 x_st<float2>(double2 v)
+{
+    return make_float2((float)v.x, (float)v.y);
+}
+
 // Inverse DFT of length 4 in place: X[k] = sum_n x[n] i^(nk).
 __device__ __forceinline__ void idft4(double2 &x0, double2 &x1, double2 &x2, double2 &x3)
 {
@@ -572,8 +589,8 @@ __device__ __forceinline__ double2 raw_to_c(unsigned long long t)
     return make_double2((double)__uint_as_float((unsigned)t), (double)__uint_as_float((unsigned)(t >> 32)));
 }
 
-template <int N, typename TI>
-__global__ __launch_bounds__(pk_e(N) / 16) void k_rows_pk(const PackDesc *pd, double2 *X, int ny, int64_t cs,
+template <int N, typename TI, typename XT>
+__global__ __launch_bounds__(pk_e(N) / 16) void k_rows_pk(const PackDesc *pd, XT *X, int ny, int64_t cs,
                                                          const double2 *tw, int NT)
 {
     constexpr int E = pk_e(N), TH = E / 16, PB = E / (2 * N), NF = E / N, NTX = (N + 31) / 32;
@@ -584,7 +601,7 @@ __global__ __launch_bounds__(pk_e(N) / 16) void k_rows_pk(const PackDesc *pd, do
     const double ia = 1.0 / *d.na, ib = B ? 1.0 / *d.nb : 0.0;
     const double2 ka0 = make_double2(d.ca[0].x * ia, d.ca[0].y * ia), ka1 = make_double2(d.ca[1].x * ia, d.ca[1].y * ia);
     const double2 kb0 = make_double2(d.cb[0].x * ib, d.cb[0].y * ib), kb1 = make_double2(d.cb[1].x * ib, d.cb[1].y * ib);
-    double2 *dst = X + (size_t)blockIdx.y * N * ny;
+    XT *dst = X + (size_t)blockIdx.y * N * ny;
     const int npairs = ny / 2 + 1;
     // the tile-mask rows of the block's transform rows (all 1 for complete
     // grids): cells of tiles no record reached are read as zeros, never loaded
@@ -633,26 +650,25 @@ __global__ __launch_bounds__(pk_e(N) / 16) void k_rows_pk(const PackDesc *pd, do
         if (pair >= npairs) return;
         const int y = (f & 1) ? ((ny - pair) & (ny - 1)) : pair;
         if ((f & 1) && y == pair) return;  // a self-paired row is written once
-        dst[(size_t)y * N + kx] = v;
+        dst[(size_t)y * N + kx] = x_st<XT>(v);
     };
     fb_ifft<N, false, E, TH>(lds, tw, NT, load, store);
 }
 
 // Pass B1: length-N1 transforms down the columns (rows N2*n1 + n2) of NF =
 // 4096/N1 columns starting at x0, times exp(+2 pi i n2 k1 / ny), in place.
-template <int N1>
-__global__ __launch_bounds__(FB_THREADS) void k_cols1_fb(double2 *X, int nx, int ny, int N2, const double2 *tw,
-                                                        int NT)
+template <int N1, typename XT>
+__global__ __launch_bounds__(FB_THREADS) void k_cols1_fb(XT *X, int nx, int ny, int N2, const double2 *tw, int NT)
 {
     __shared__ double2 lds[FB_LDS];
     constexpr int NF = FB_E / N1;
     const int x0 = blockIdx.x * NF, n2 = blockIdx.y;
-    double2 *plane = X + (size_t)blockIdx.z * nx * ny;
+    XT *plane = X + (size_t)blockIdx.z * nx * ny;
     const int mul = NT / ny;
     // transform f = column x0 + f; loads with the column index fastest across lanes
-    auto load = [&](int f, int n1) { return plane[((size_t)N2 * n1 + n2) * nx + x0 + f]; };
+    auto load = [&](int f, int n1) { return x_ld(plane[((size_t)N2 * n1 + n2) * nx + x0 + f]); };
     auto store = [&](int f, int k1, double2 v) {
-        plane[((size_t)N2 * k1 + n2) * nx + x0 + f] = c2mul(v, tw[((n2 * k1) & (ny - 1)) * mul]);
+        plane[((size_t)N2 * k1 + n2) * nx + x0 + f] = x_st<XT>(c2mul(v, tw[((n2 * k1) & (ny - 1)) * mul]));
     };
     fb_ifft<N1, true>(lds, tw, NT, load, store);
 }
@@ -680,15 +696,15 @@ __device__ __forceinline__ void pack_store(const double2 *lds, int nk, int x0, i
 
 // Pass B2 + finalize: packed plane blockIdx.z, index k1, NF = 4096 / N2
 // columns from x0: rows N2*k1 + n2 -> length-N2 transforms -> ky = k1 + N1*k2.
-template <int N2>
-__global__ __launch_bounds__(FB_THREADS) void k_cols2_pk(const double2 *X, int nx, int ny, int N1,
-                                                        const double2 *tw, int NT, const PackDesc *pd)
+template <int N2, typename XT>
+__global__ __launch_bounds__(FB_THREADS) void k_cols2_pk(const XT *X, int nx, int ny, int N1, const double2 *tw,
+                                                        int NT, const PackDesc *pd)
 {
     __shared__ double2 lds[FB_LDS];
     constexpr int NF = FB_E / N2;
     const int x0 = blockIdx.x * NF, k1 = blockIdx.y;
-    const double2 *plane = X + (size_t)blockIdx.z * nx * ny;
-    auto load = [&](int f, int n2) { return plane[((size_t)N2 * k1 + n2) * nx + x0 + f]; };
+    const XT *plane = X + (size_t)blockIdx.z * nx * ny;
+    auto load = [&](int f, int n2) { return x_ld(plane[((size_t)N2 * k1 + n2) * nx + x0 + f]); };
     auto store = [&](int f, int k2, double2 v) { lds[k2 * NF + f] = v; };
     fb_ifft<N2, true>(lds, tw, NT, load, store);
     __syncthreads();
@@ -702,16 +718,16 @@ __global__ __launch_bounds__(FB_THREADS) void k_cols2_pk(const double2 *X, int n
 constexpr int FBC_E = 8192;
 constexpr int FBC_THREADS = 512;
 
-template <int N>
-__global__ __launch_bounds__(FBC_THREADS) void k_cols_pk(const double2 *X, int nx, int ny, const double2 *tw, int NT,
+template <int N, typename XT>
+__global__ __launch_bounds__(FBC_THREADS) void k_cols_pk(const XT *X, int nx, int ny, const double2 *tw, int NT,
                                                         const PackDesc *pd)
 {
     __shared__ double2 lds[FBC_E];
     constexpr int NF = FBC_E / N;
     static_assert(NF >= 32, "columns per block");
     const int x0 = blockIdx.x * NF;
-    const double2 *plane = X + (size_t)blockIdx.z * nx * ny;
-    auto load = [&](int f, int n) { return plane[(size_t)n * nx + x0 + f]; };
+    const XT *plane = X + (size_t)blockIdx.z * nx * ny;
+    auto load = [&](int f, int n) { return x_ld(plane[(size_t)n * nx + x0 + f]); };
     auto store = [&](int f, int k, double2 v) { lds[k * NF + f] = v; };
     fb_ifft<N, true, FBC_E, FBC_THREADS>(lds, tw, NT, load, store);
     __syncthreads();
@@ -739,8 +755,8 @@ static bool whole_columns(const mti_ctx *ctx, const FftGeom &g)
     return fbc_ok(g) && (!split_ok(g) || g.ny <= ctx->whole_columns_ny);
 }
 
-template <typename TI>
-static mti_status pk_rows(mti_ctx *ctx, const FftGeom &g, const PackDesc *pd, int npk, double2 *X, int64_t cs,
+template <typename TI, typename XT>
+static mti_status pk_rows(mti_ctx *ctx, const FftGeom &g, const PackDesc *pd, int npk, XT *X, int64_t cs,
                           const double2 *tw, hipStream_t st)
 {
     const int npairs = g.ny / 2 + 1;
@@ -749,7 +765,7 @@ static mti_status pk_rows(mti_ctx *ctx, const FftGeom &g, const PackDesc *pd, in
     case NN: {                                                                                              \
         constexpr int PB = pk_e(NN) / (2 * NN);                                                             \
         const dim3 gr((unsigned)ceil_div(npairs, PB), (unsigned)npk);                                       \
-        hipLaunchKernelGGL((k_rows_pk<NN, TI>), gr, dim3(pk_e(NN) / 16), 0, st, pd, X, g.ny, cs, tw, g.NT); \
+        hipLaunchKernelGGL((k_rows_pk<NN, TI, XT>), gr, dim3(pk_e(NN) / 16), 0, st, pd, X, g.ny, cs, tw, g.NT); \
         break;                                                                                              \
     }
         MTI_PR(16) MTI_PR(32) MTI_PR(64) MTI_PR(128) MTI_PR(256) MTI_PR(512) MTI_PR(1024) MTI_PR(2048)
@@ -761,13 +777,13 @@ static mti_status pk_rows(mti_ctx *ctx, const FftGeom &g, const PackDesc *pd, in
     return MTI_OK;
 }
 
-static mti_status fb_cols1(mti_ctx *ctx, const FftGeom &g, double2 *X, int planes, const double2 *tw,
-                           hipStream_t st)
+template <typename XT>
+static mti_status fb_cols1(mti_ctx *ctx, const FftGeom &g, XT *X, int planes, const double2 *tw, hipStream_t st)
 {
     const dim3 gr((unsigned)(g.nx / (FB_E / g.N1)), (unsigned)g.N2, (unsigned)planes);
     switch (g.N1) {
 #define MTI_C1(NN) \
-    case NN: hipLaunchKernelGGL((k_cols1_fb<NN>), gr, dim3(FB_THREADS), 0, st, X, g.nx, g.ny, g.N2, tw, g.NT); break;
+    case NN: hipLaunchKernelGGL((k_cols1_fb<NN, XT>), gr, dim3(FB_THREADS), 0, st, X, g.nx, g.ny, g.N2, tw, g.NT); break;
         MTI_C1(16) MTI_C1(32) MTI_C1(64) MTI_C1(128)
 #undef MTI_C1
     default: return set_error(ctx, MTI_ERR_UNSUPPORTED, "column split");
@@ -776,13 +792,16 @@ static mti_status fb_cols1(mti_ctx *ctx, const FftGeom &g, double2 *X, int plane
     return MTI_OK;
 }
 
-static mti_status pk_cols2(mti_ctx *ctx, const FftGeom &g, const double2 *X, int npk, const double2 *tw,
+template <typename XT>
+static mti_status pk_cols2(mti_ctx *ctx, const FftGeom &g, const XT *X, int npk, const double2 *tw,
                            const PackDesc *pd, hipStream_t st)
 {
     const dim3 gr((unsigned)(g.nx / (FB_E / g.N2)), (unsigned)g.N1, (unsigned)npk);
     switch (g.N2) {
 #define MTI_C2(NN) \
-    case NN: hipLaunchKernelGGL((k_cols2_pk<NN>), gr, dim3(FB_THREADS), 0, st, X, g.nx, g.ny, g.N1, tw, g.NT, pd); break;
+    case NN:                                                                                                \
+        hipLaunchKernelGGL((k_cols2_pk<NN, XT>), gr, dim3(FB_THREADS), 0, st, X, g.nx, g.ny, g.N1, tw, g.NT, pd); \
+        break;
         MTI_C2(16) MTI_C2(32) MTI_C2(64)
 #undef MTI_C2
     default: return set_error(ctx, MTI_ERR_UNSUPPORTED, "column split");
@@ -791,13 +810,14 @@ static mti_status pk_cols2(mti_ctx *ctx, const FftGeom &g, const double2 *X, int
     return MTI_OK;
 }
 
-static mti_status pk_cols(mti_ctx *ctx, const FftGeom &g, const double2 *X, int npk, const double2 *tw,
+template <typename XT>
+static mti_status pk_cols(mti_ctx *ctx, const FftGeom &g, const XT *X, int npk, const double2 *tw,
                           const PackDesc *pd, hipStream_t st)
 {
     const dim3 gr((unsigned)(g.nx / (FBC_E / g.ny)), 1u, (unsigned)npk);
     switch (g.ny) {
 #define MTI_CF(NN) \
-    case NN: hipLaunchKernelGGL((k_cols_pk<NN>), gr, dim3(FBC_THREADS), 0, st, X, g.nx, g.ny, tw, g.NT, pd); break;
+    case NN: hipLaunchKernelGGL((k_cols_pk<NN, XT>), gr, dim3(FBC_THREADS), 0, st, X, g.nx, g.ny, tw, g.NT, pd); break;
         MTI_CF(16) MTI_CF(32) MTI_CF(64) MTI_CF(128) MTI_CF(256)
 #undef MTI_CF
     default: return set_error(ctx, MTI_ERR_UNSUPPORTED, "column length");
@@ -937,15 +957,16 @@ static void make_packs(const std::vector<ChanRef> &ch, int P, int poltype, int l
     }
 }
 
-// Transform + finalize of npk packed planes (device descriptors pd) into X.
-template <typename TI>
-static mti_status finalize_packs(mti_ctx *ctx, const FftGeom &g, const PackDesc *pd, int npk, double2 *X, int64_t cs,
+// Transform + finalize of npk packed planes (device descriptors pd) through
+// the planes X (complex128 or complex64, XT).
+template <typename TI, typename XT>
+static mti_status finalize_packs(mti_ctx *ctx, const FftGeom &g, const PackDesc *pd, int npk, XT *X, int64_t cs,
                                  const double2 *tw, hipStream_t st)
 {
     mti_status r;
     {
         StageTimer tm(ctx, MTI_STAGE_FFT_ROWS, st);
-        r = pk_rows<TI>(ctx, g, pd, npk, X, cs, tw, st);
+        r = pk_rows<TI, XT>(ctx, g, pd, npk, X, cs, tw, st);
         if (r != MTI_OK) return r;
     }
     if (whole_columns(ctx, g)) {
@@ -995,15 +1016,22 @@ static mti_status finalize_channels(mti_ctx *ctx, const FftGeom &g, const std::v
     r = twiddle_table(ctx, g.NT, &tw, st);
     if (r != MTI_OK) return r;
     const int npk = (int)h.size();
+    // the batch is sized for complex128 planes (the workspace plans do the
+    // same), so complex64 planes take half of what was planned
     const int batch = transform_batch(g, npk);
+    const bool x64 = ctx->xform_c64 != 0;
     void *w = nullptr;
-    r = scratch_get(ctx, SLOT_FFT, (size_t)g.nx * g.ny * sizeof(double2) * batch, &w);
+    r = scratch_get(ctx, SLOT_FFT, (size_t)g.nx * g.ny * (x64 ? sizeof(float2) : sizeof(double2)) * batch, &w);
     if (r != MTI_OK) return r;
     const int64_t cs = layout == MTI_GRID_VUP ? P : 1;
     for (int k0 = 0; k0 < npk; k0 += batch) {
         const int nk = std::min(batch, npk - k0);
-        r = c64 ? finalize_packs<float2>(ctx, g, pd + k0, nk, (double2 *)w, cs, tw, st)
-                : finalize_packs<double2>(ctx, g, pd + k0, nk, (double2 *)w, cs, tw, st);
+        if (x64)
+            r = c64 ? finalize_packs<float2>(ctx, g, pd + k0, nk, (float2 *)w, cs, tw, st)
+                    : finalize_packs<double2>(ctx, g, pd + k0, nk, (float2 *)w, cs, tw, st);
+        else
+            r = c64 ? finalize_packs<float2>(ctx, g, pd + k0, nk, (double2 *)w, cs, tw, st)
+                    : finalize_packs<double2>(ctx, g, pd + k0, nk, (double2 *)w, cs, tw, st);
         if (r != MTI_OK) return r;
     }
     return MTI_OK;

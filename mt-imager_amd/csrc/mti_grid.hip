@@ -578,39 +578,53 @@ __global__ __launch_bounds__(TG_THREADS, PREC == MTI_TABLE_F32 ? 8 : 1) void k_g
 // Zeroes the tiles that no single unit owns (no unit at all, or several units
 // adding with atomics), so that a tile's only unit can store its cells
 // without a prior fill of the grid.  uprefix is the exclusive scan of the
-// units per tile key, total the unit count.  Blocks stride over the keys.
-// A job with a tile mask leaves the tiles no unit reaches unwritten and marks
-// every tile (1: written); only its tiles of several units are zeroed.
+// units per tile key, total the unit count.  A job with a tile mask leaves
+// the tiles no unit reaches unwritten and marks every tile (1: written);
+// only its tiles of several units are zeroed.  A lane per key finds what its
+// tile needs; the wave then zeroes the tiles its lanes found, all 64 lanes on
+// one tile at a time.
 __global__ __launch_bounds__(256) void k_zero_tiles(JobSet js, const uint32_t *uprefix, const uint32_t *total,
                                                    int64_t nkeys)
 {
-    const int P = js.P;
-    for (int64_t k = blockIdx.x; k < nkeys; k += gridDim.x) {
-        const uint32_t u1 = (k + 1 < nkeys) ? uprefix[k + 1] : *total;
-        const uint32_t nu = u1 - uprefix[k];
-        if (nu == 1 && !js.masks) continue;
-        const TileAt ta = tile_at(js, (int)k);
-        const JobDev &jd = js.jobs[ta.job];
-        if (jd.tile_mask) {
-            if (threadIdx.x == 0) jd.tile_mask[k - js.key0[ta.job]] = nu > 0 ? 1 : 0;
-            if (nu <= 1) continue;
-        } else if (nu == 1) {
-            continue;
-        }
-        const int nx = jd.g.nx, ny = jd.g.ny, x0 = ta.tx * TG_T, y0 = ta.ty * TG_T;
-        const int w = min(TG_T, nx - x0), h = min(TG_T, ny - y0), cells = w * h;
-        double2 *g = (double2 *)jd.grid + (int64_t)ta.c_local * jd.chan_stride;
-        for (int i = threadIdx.x; i < cells * P; i += blockDim.x) {
-            int p, cell;
-            if (jd.ps == 1) {  // [v][u][P]: correlations of a cell adjacent
-                p = i % P;
-                cell = i / P;
-            } else {           // [P][v][u]: cells of a plane adjacent
-                p = i / cells;
-                cell = i - p * cells;
+    const int P = js.P, lane = threadIdx.x & 63;
+    const uint32_t NU = *total;
+    const int64_t step = (int64_t)gridDim.x * blockDim.x;
+    // k0: the wave's first key (the same in every lane of the wave)
+    for (int64_t k0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); k0 < nkeys; k0 += step) {
+        const int64_t k = k0 + lane;
+        bool zero = false;
+        if (k < nkeys) {
+            const uint32_t nu = ((k + 1 < nkeys) ? uprefix[k + 1] : NU) - uprefix[k];
+            if (js.masks || nu != 1) {
+                const int job = job_of(js.key0, js.njobs, k);
+                const JobDev &jd = js.jobs[job];
+                if (jd.tile_mask) {
+                    jd.tile_mask[k - js.key0[job]] = nu > 0 ? 1 : 0;
+                    zero = nu > 1;
+                } else {
+                    zero = nu != 1;
+                }
             }
-            const int cy = cell / w, cx = cell - cy * w;
-            g[((int64_t)(y0 + cy) * nx + x0 + cx) * jd.cs + p * jd.ps] = make_double2(0.0, 0.0);
+        }
+        for (uint64_t todo = __ballot(zero); todo; todo &= todo - 1) {
+            const int l = __ffsll((unsigned long long)todo) - 1;
+            const TileAt ta = tile_at(js, (int)(k0 + l));
+            const JobDev &jd = js.jobs[ta.job];
+            const int nx = jd.g.nx, ny = jd.g.ny, x0 = ta.tx * TG_T, y0 = ta.ty * TG_T;
+            const int w = min(TG_T, nx - x0), h = min(TG_T, ny - y0), cells = w * h;
+            double2 *g = (double2 *)jd.grid + (int64_t)ta.c_local * jd.chan_stride;
+            for (int i = lane; i < cells * P; i += 64) {
+                int p, cell;
+                if (jd.ps == 1) {  // [v][u][P]: correlations of a cell adjacent
+                    p = i % P;
+                    cell = i / P;
+                } else {           // [P][v][u]: cells of a plane adjacent
+                    p = i / cells;
+                    cell = i - p * cells;
+                }
+                const int cy = cell / w, cx = cell - cy * w;
+                g[((int64_t)(y0 + cy) * nx + x0 + cx) * jd.cs + p * jd.ps] = make_double2(0.0, 0.0);
+            }
         }
     }
 }
@@ -957,8 +971,9 @@ static mti_status grid_pass(mti_ctx *ctx, const std::vector<PassJob> &jobs, cons
                            (uint32_t)chunk, units);
         MTI_HIP(ctx, hipGetLastError());
         if (fresh) {
-            hipLaunchKernelGGL(k_zero_tiles, dim3((unsigned)std::min<int64_t>(nkeys, 16384)), dim3(256), 0, st, js,
-                               (const uint32_t *)nunits, (const uint32_t *)NU_d, nkeys);
+            const unsigned zblocks = (unsigned)std::min<int64_t>(ceil_div(nkeys, 256), 8192);
+            hipLaunchKernelGGL(k_zero_tiles, dim3(zblocks), dim3(256), 0, st, js, (const uint32_t *)nunits,
+                               (const uint32_t *)NU_d, nkeys);
             MTI_HIP(ctx, hipGetLastError());
         }
         if (weighted) {

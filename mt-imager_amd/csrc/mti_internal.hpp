@@ -95,6 +95,9 @@ struct mti_ctx {
     int unit_pairs = 2048;       // (record, tile) pairs per gridder work unit
     int whole_columns_ny = 256;  // largest ny transformed by whole columns when the split also applies
     int xcd_chunk = 0;           // gridder units dealt to each XCD in runs of this many (0: round-robin)
+    // packed planes between the transform passes stored as complex64 (1) or
+    // complex128 (0) (mti_set_transform_storage)
+    int xform_c64 = 0;
     // HIP-graph capture (mti_graph_create): calls enqueue device work only --
     // no host synchronisation, no statistics read-back, host tables staged
     // in pinned memory that lives as long as the graph

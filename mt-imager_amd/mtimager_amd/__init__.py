@@ -95,7 +95,8 @@ EXPORTS = ["mti_version", "mti_create", "mti_destroy", "mti_last_error",
            "mti_finalize_workspace_size", "mti_finalize_jobs_workspace_size", "mti_fft2d_workspace_size",
            "mti_set_workspace", "mti_context_device_bytes", "mti_set_tuning", "mti_get_tuning",
            "mti_graph_create", "mti_graph_launch", "mti_graph_destroy",
-           "mti_fits_create", "mti_fits_write_channel", "mti_fits_close", "mti_finalize_sparse"]
+           "mti_fits_create", "mti_fits_write_channel", "mti_fits_close", "mti_finalize_sparse",
+           "mti_set_transform_storage", "mti_get_transform_storage"]
 
 _lib = None
 
@@ -164,6 +165,8 @@ def lib():
             ("mti_context_device_bytes", [vp, P(sz)]),
             ("mti_set_tuning", [vp, P(Tuning)]),
             ("mti_get_tuning", [vp, P(Tuning)]),
+            ("mti_set_transform_storage", [vp, i32]),
+            ("mti_get_transform_storage", [vp, P(i32)]),
             ("mti_graph_create", [vp, i32, vp, P(GridOpts), i32, i32, P(vp)]),
             ("mti_graph_launch", [vp, vp]),
             ("mti_finalize_sparse", [vp, P(ImageParams), i32, i32, i32, i32, i32, vp, vp, vp, vp, vp])]:
@@ -336,6 +339,17 @@ class Context:
         self.check(lib().mti_get_tuning(self.h, ctypes.byref(t)), "mti_get_tuning")
         return {"unit_pairs": int(t.unit_pairs), "whole_columns_ny": int(t.whole_columns_ny),
                 "xcd_chunk": int(t.xcd_chunk)}
+
+    def set_transform_storage(self, precision):
+        """Element type of the packed planes between the finalize's transform
+        passes: "c128" (default) or "c64" (half the bytes between passes; the
+        arithmetic stays FP64)."""
+        self.check(lib().mti_set_transform_storage(self.h, GRID_PRECISION[precision]), "mti_set_transform_storage")
+
+    def get_transform_storage(self):
+        out = ctypes.c_int32(0)
+        self.check(lib().mti_get_transform_storage(self.h, ctypes.byref(out)), "mti_get_transform_storage")
+        return {v: k for k, v in GRID_PRECISION.items()}[int(out.value)]
 
     def device_bytes(self):
         """Device bytes the context itself holds (its scratch and twiddle tables)."""

@@ -121,10 +121,12 @@ class Server:
     plain: the one-job-at-a-time baseline (one worker, waves of one request,
     the single-job entry points mti_grid + mti_finalize, tables built per
     request);
-    weighting/robust/poltype: imaging options shared by all requests."""
+    weighting/robust/poltype: imaging options shared by all requests;
+    transform_storage: "c128" or "c64", the element type of the packed
+    planes between the finalize's transform passes (Context.set_transform_storage)."""
 
     def __init__(self, hbm_budget, workers=2, max_wave=256, cache=None, device=0,
-                 weighting="natural", robust=0.0, poltype="linear", plain=False):
+                 weighting="natural", robust=0.0, poltype="linear", plain=False, transform_storage="c128"):
         import torch
         self.torch = torch
         self.budget = int(hbm_budget)
@@ -135,6 +137,7 @@ class Server:
         self.cache = cache
         self.device = device
         self.weighting, self.robust, self.poltype = weighting, robust, poltype
+        self.transform_storage = transform_storage
         self.gridded = 0
         self.errors = []
         self.wave_sizes = []
@@ -213,6 +216,7 @@ class Server:
         torch.cuda.set_device(self.device)
         stream = torch.cuda.Stream(device=self.device)
         ctx = Context(self.device)
+        ctx.set_transform_storage(self.transform_storage)
         try:
             while True:
                 with self._cv:
