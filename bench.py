@@ -312,7 +312,8 @@ def run_c4(args, torch, mti, jobs):
     """Config 4: independent snapshot jobs (Stokes I) imaged one at a time
     (plain: per job kernel tables, row sort, gridding, finalize) and batched
     (kernel-table cache, jobs admitted in waves by an HBM budget, one
-    mti_grid_jobs + one mti_finalize_jobs call per wave).  Returns a dict."""
+    mti_grid_jobs + one mti_finalize_jobs call per wave, waves on two workers
+    with a context and stream each).  Returns a dict."""
     import numpy as np
     ctx = mti.Context(0)
     ctx.set_transform_storage(args.transform_storage)
@@ -350,40 +351,85 @@ def run_c4(args, torch, mti, jobs):
             lat.append(time.perf_counter() - t0)
         return time.perf_counter() - t0, gridded, lat
 
+    # two wave workers, each with its own context and stream: one wave's host
+    # work (job tables, pack descriptors, the calls' synchronisations)
+    # overlaps the other wave's kernels
+    wctx = [mti.Context(0) for _ in range(2)]
+    for c in wctx:
+        c.set_transform_storage(args.transform_storage)
+    wstreams = [torch.cuda.Stream() for _ in wctx]
+
     def batched():
-        lat, gridded, cache, wave, mem = [], 0, {}, [], 0
+        """Waves cut by the HBM budget in arrival order (kernel tables from a
+        cache built as jobs arrive), each one mti_grid_jobs + one
+        mti_finalize_jobs call on whichever worker is free."""
+        import queue
+        import threading
+        for c in wctx:
+            c.set_tuning(**ctx.get_tuning())
+        lat, cache, errors = [], {}, []
+        gridded = [0]
+        lock = threading.Lock()
+        # waves dealt in turn (wave k to worker k mod 2): each worker sees the
+        # same waves in every run, so its stream's cached memory fits them
+        qs = [queue.Queue(maxsize=2) for _ in wctx]
         t0 = time.perf_counter()
 
-        def flush():
-            nonlocal gridded, wave, mem
-            if not wave:
-                return
-            js = [w[0] for w in wave]
-            st = ctx.grid_jobs(js)
-            ctx.finalize_jobs(js)
-            torch.cuda.synchronize()
-            log(f"c4: wave of {len(js)} jobs done")
-            gridded += st.gridded_records
-            lat.extend([time.perf_counter() - t0] * len(wave))
-            wave, mem = [], 0
+        def worker(c, s, q):
+            with torch.cuda.stream(s):
+                while True:
+                    wave = q.get()
+                    if wave is None:
+                        return
+                    if errors:
+                        continue
+                    try:
+                        items = []
+                        for j in wave:
+                            # the batched call puts each job's rows in baseline-major order itself
+                            v = mti.vis_ms(j["uvw"], j["invl"], j["vis"], j["flags"], j["weight"])
+                            gr, nr, im, mk = outputs(j)
+                            items.append((c.make_job(cache[j["key"]], j["p"], v, gr, nr, im, 0, j["nch"],
+                                                     ant1=j["a1"], ant2=j["a2"], nant=64, tile_mask=mk),
+                                          gr, nr, im, mk))
+                        js = [it[0] for it in items]
+                        st = c.grid_jobs(js)  # on the thread's current stream, s
+                        c.finalize_jobs(js)
+                        s.synchronize()
+                        with lock:
+                            gridded[0] += st.gridded_records
+                            lat.extend([time.perf_counter() - t0] * len(wave))
+                        log(f"c4: wave of {len(js)} jobs done")
+                        del items, js
+                    except Exception as e:  # reported by the main thread
+                        errors.append(e)
 
+        ths = [threading.Thread(target=worker, args=(c, s, q)) for c, s, q in zip(wctx, wstreams, qs)]
+        for t in ths:
+            t.start()
+        wave, mem, nw = [], 0, 0
         for j in jobs:
             if j["key"] not in cache:
-                cache[j["key"]] = ctx.build_kernels(j["p"])
+                cache[j["key"]] = ctx.build_kernels(j["p"])  # complete on return
             need = j["nch"] * j["nx"] ** 2 * 20
             if wave and mem + need > budget:
-                flush()
-            # the batched call puts each job's rows in baseline-major order itself
-            v = mti.vis_ms(j["uvw"], j["invl"], j["vis"], j["flags"], j["weight"])
-            gr, nr, im, mk = outputs(j)
-            wave.append((ctx.make_job(cache[j["key"]], j["p"], v, gr, nr, im, 0, j["nch"],
-                                      ant1=j["a1"], ant2=j["a2"], nant=64, tile_mask=mk), gr, nr, im, mk))
+                qs[nw % len(qs)].put(wave)
+                nw += 1
+                wave, mem = [], 0
+            wave.append(j)
             mem += need
-        flush()
+        if wave:
+            qs[nw % len(qs)].put(wave)
+        for q in qs:
+            q.put(None)
+        for t in ths:
+            t.join()
+        if errors:
+            raise errors[0]
         el = time.perf_counter() - t0
         for K in cache.values():
             K.free()
-        return el, gridded, lat, len(cache)
+        return el, gridded[0], lat, len(cache)
 
     tune_info = None
     if not args.no_autotune:
@@ -421,6 +467,10 @@ def run_c4(args, torch, mti, jobs):
         for K in kc.values():
             K.free()
         torch.cuda.synchronize()
+    # memory the allocator keeps for the default stream is of no use to the
+    # workers' streams
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
     log("c4: warm-up (batched)")
     batched()
     log("c4: plain, one job at a time")
@@ -429,6 +479,8 @@ def run_c4(args, torch, mti, jobs):
     tb, gb, lb, nk = batched()
     log(f"c4: batched {tb:.2f} s ({nk} kernel-table sets)")
     graph_info = graph_replay(ctx, torch, mti, jobs, outputs)
+    for c in wctx:
+        c.close()
     ctx.close()
     assert gp == gb, (gp, gb)
     lp, lb = np.array(lp) * 1e3, np.array(lb) * 1e3
