@@ -96,26 +96,6 @@ __device__ __forceinline__ const void *uniform_ptr(const void *p)
     return (const void *)(((uint64_t)hi << 32) | lo);
 }
 
-// Kernel sample idx of a table (complex f32, f16 or bf16 pairs).  The byte
-// offset is 32-bit (tables are below 4 GiB, checked when they are built), so
-// the load takes the table base from scalar registers and one offset VGPR.
-template <int PREC>
-__device__ __forceinline__ float2 ksample(const void *table, uint32_t idx)
-{
-    constexpr uint32_t esz = PREC == MTI_TABLE_F32 ? 8u : 4u;
-    const MTI_GLOBAL char *p = gconst<char>(table) + idx * esz;
-    if (PREC == MTI_TABLE_F32) {
-        const uint64_t w = *(const MTI_GLOBAL uint64_t *)p;
-        return make_float2(__uint_as_float((uint32_t)w), __uint_as_float((uint32_t)(w >> 32)));
-    }
-    const uint32_t w = *(const MTI_GLOBAL uint32_t *)p;
-    if (PREC == MTI_TABLE_F16)
-        return make_float2(__half2float(__ushort_as_half((unsigned short)(w & 0xffffu))),
-                           __half2float(__ushort_as_half((unsigned short)(w >> 16))));
-    // bf16 is the high half of an f32
-    return make_float2(__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u));
-}
-
 // Tile coordinates of a global (job, channel, tile) key.
 struct TileAt {
     int job, c_local, tx, ty;
@@ -388,8 +368,10 @@ __global__ __launch_bounds__(256) void k_record_prep(GridArgs a)
                     w = dens[p] > 0.0f ? w / dens[p] : 0.0f;
                 else if (jd.weighting == MTI_WEIGHT_BRIGGS)
                     w = (float)((double)w / (1.0 + jd.f2[c_local * P + p] * (double)dens[p]));
+                // a zero weight (flagged) gives an exact zero, whatever the
+                // visibility holds (a flagged sample may be NaN)
                 const float2 vv = a.wvis[(int64_t)j * P + p];
-                a.wvis[(int64_t)j * P + p] = make_float2(vv.x * w, vv.y * w);
+                a.wvis[(int64_t)j * P + p] = w != 0.0f ? make_float2(vv.x * w, vv.y * w) : make_float2(0.0f, 0.0f);
                 if (jc == jc_first) np[p] += csum * (double)w;
                 else unsafeAtomicAdd(&jd.norm[c_local * P + p], csum * (double)w);
             }
@@ -400,13 +382,40 @@ __global__ __launch_bounds__(256) void k_record_prep(GridArgs a)
 
 // ---- gridder ------------------------------------------------------------------
 
-// One in-flight visit of a wave to a record: staged record, lane active,
-// kernel sample.
+// One in-flight visit of a wave to a record: staged record, visit inside the
+// list (the same in every lane), the lane's kernel sample.
 struct Visit {
     int vo;  // byte offset of the record's visibilities in LDS
-    bool on;
+    bool live;
     float2 k;
 };
+
+// Kernel samples are read through a buffer resource over the job's table:
+// the lane's byte offset is one 32-bit register (no 64-bit address per
+// visit), and the buffer's range check makes an offset at or past the
+// table's end read as zero -- a lane outside the record's support box reads
+// a zero sample and adds exactly nothing (staged visibilities are finite:
+// zero-weight samples are staged as zeros), so a visit needs no branch on
+// the lane.  Tables are at most 2^32 - 16 bytes, so KS_OUTSIDE is past every
+// table.
+constexpr uint32_t KS_OUTSIDE = 0xfffffff0u;
+constexpr int MTI_BUFFER_DWORD3 = 0x00020000;  // raw buffer, gfx9-family dword 3
+
+typedef unsigned int mti_u32x2 __attribute__((ext_vector_type(2)));
+
+template <int PREC>
+__device__ __forceinline__ float2 ksample_buf(__amdgpu_buffer_rsrc_t tab, uint32_t off)
+{
+    if (PREC == MTI_TABLE_F32) {
+        const mti_u32x2 w = __builtin_amdgcn_raw_buffer_load_b64(tab, off, 0, 0);
+        return make_float2(__uint_as_float(w.x), __uint_as_float(w.y));
+    }
+    const uint32_t w = __builtin_amdgcn_raw_buffer_load_b32(tab, off, 0, 0);
+    if (PREC == MTI_TABLE_F16)
+        return make_float2(__half2float(__ushort_as_half((unsigned short)(w & 0xffffu))),
+                           __half2float(__ushort_as_half((unsigned short)(w >> 16))));
+    return make_float2(__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u));
+}
 
 // Region-list entry (12 bytes): the record's corner x | y << 16, its kernel
 // offset, and S | (visibility byte offset) << 16.
@@ -415,26 +424,27 @@ struct ListEntry {
 };
 
 // Visit i of the wave's region list: the lane's cell (gx, gy) against the
-// record's support box; kernel index 0 (a valid dummy) when outside.  The
+// record's support box; a zero sample when outside (or past the list).  The
 // entry carries everything a visit needs, so it is one LDS read with no
 // dependent read; dy * S (|dy| < 2^23) uses the full-rate 24-bit multiply.
 template <int PREC>
-__device__ __forceinline__ Visit visit_issue(int i, int n, const ListEntry *list, int gx, int gy, const void *table)
+__device__ __forceinline__ Visit visit_issue(int i, int n, const ListEntry *list, int gx, int gy,
+                                             __amdgpu_buffer_rsrc_t tab)
 {
+    constexpr uint32_t esz = PREC == MTI_TABLE_F32 ? 8u : 4u;
     Visit v;
     v.vo = 0;
-    v.on = false;
-    uint32_t idx = 0;
-    if (i < n) {
+    v.live = i < n;
+    uint32_t off = KS_OUTSIDE;
+    if (v.live) {
         const ListEntry e = list[i];
         const int S = e.s_vo & 0xffff;
         const int dx = gx - (e.xy & 0xffff), dy = gy - (int)((unsigned)e.xy >> 16);
         const uint32_t t = (uint32_t)(e.koff + __mul24(dy, S) + dx);
-        v.on = max((unsigned)dx, (unsigned)dy) < (unsigned)S;
         v.vo = (int)((unsigned)e.s_vo >> 16);
-        idx = v.on ? t : 0u;
+        off = max((unsigned)dx, (unsigned)dy) < (unsigned)S ? t * esz : KS_OUTSIDE;
     }
-    v.k = ksample<PREC>(table, idx);
+    v.k = ksample_buf<PREC>(tab, off);
     return v;
 }
 
@@ -473,7 +483,9 @@ __global__ __launch_bounds__(TG_THREADS, PREC == MTI_TABLE_F32 ? 8 : 1) void k_g
         const int first = un.y, npairs = un.z, excl = un.w;
         const TileAt ta = tile_at(a.js, un.x);
         const JobDev &jd = a.js.jobs[ta.job];
-        const void *table = uniform_ptr(jd.table);
+        const __amdgpu_buffer_rsrc_t table = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<void *>(uniform_ptr(jd.table)), (short)0, (int)wave_uniform((int)jd.table_bytes),
+            MTI_BUFFER_DWORD3);
         const int x0 = ta.tx * TG_T, y0 = ta.ty * TG_T;
         const int gx = x0 + rx, gy = y0 + ry;
         double acc[2 * P];
@@ -524,7 +536,7 @@ __global__ __launch_bounds__(TG_THREADS, PREC == MTI_TABLE_F32 ? 8 : 1) void k_g
             Visit vd = visit_issue<PREC>(3, n, list, gx, gy, table);
             int next = 4;
             auto visit = [&](Visit &v) {
-                if (v.on) {
+                if (v.live) {
                     const double kx = v.k.x, ky = v.k.y;
                     // 16-byte aligned pairs: one ds_read_b128 per correlation
                     const double2 *vis = (const double2 *)((const char *)&s_vd[0][0] + v.vo);
@@ -818,6 +830,7 @@ static mti_status grid_pass(mti_ctx *ctx, const std::vector<PassJob> &jobs, cons
         d.ntx = (j.p.nx + TG_T - 1) / TG_T;
         d.nty = (j.p.ny + TG_T - 1) / TG_T;
         d.table = j.k->d_table;
+        d.table_bytes = (uint32_t)((uint64_t)j.k->info.total * (prec == MTI_TABLE_F32 ? 8u : 4u));
         d.sums = j.k->d_sums;
         d.T2 = j.k->info.T * j.k->info.T;
         d.grid = j.grid;

@@ -44,6 +44,7 @@ struct JobDev {
     int64_t key0;            // first (channel, tile) key of the job
     int ntx, nty;            // tiles per row / column
     const void *table;       // kernel samples (precision of the call)
+    uint32_t table_bytes;    // its size (at most 2^32 - 16)
     const float *sums;       // [nplanes][T*T]
     int T2;
     double *grid;            // FP64 grid of channel chan0; channel c at grid + 2*c*chan_stride

@@ -357,8 +357,9 @@ static mti_status build_impl(mti_ctx *ctx, const mti_image_params *p, int precis
         K->support = {S};
         K->base = {0};
         K->info.total = (int64_t)T * T * S * S;
-        // the gridder addresses samples with 32-bit byte offsets
-        if ((uint64_t)K->info.total * (uint64_t)eb >= (uint64_t(1) << 32))
+        // the gridder addresses samples with 32-bit byte offsets, and offsets
+        // from 2^32 - 16 up must lie beyond the table
+        if ((uint64_t)K->info.total * (uint64_t)eb > (uint64_t(1) << 32) - 16)
             return set_error(ctx, MTI_ERR_UNSUPPORTED, "convolution table above 4 GiB");
     } else {
         const int W = p->wplanes;
@@ -430,8 +431,9 @@ static mti_status build_impl(mti_ctx *ctx, const mti_image_params *p, int precis
         }
         K->info.total = tot;
         K->info.max_support = smax;
-        // the gridder addresses samples with 32-bit byte offsets
-        if ((uint64_t)tot * (uint64_t)eb >= (uint64_t(1) << 32))
+        // the gridder addresses samples with 32-bit byte offsets, and offsets
+        // from 2^32 - 16 up must lie beyond the table
+        if ((uint64_t)tot * (uint64_t)eb > (uint64_t(1) << 32) - 16)
             return set_error(ctx, MTI_ERR_UNSUPPORTED, "convolution table above 4 GiB");
         MTI_HIP(ctx, hipMalloc(&K->d_table, std::max<size_t>(1, (size_t)tot) * eb));
 

@@ -343,6 +343,36 @@ def test_normal_mode_image(orc, mti, ctx, c1):
     assert H.rel_rms(g["image"][0][0], r["image"][0]) < IMAGE_RMS_TOL
 
 
+@pytest.mark.parametrize("weighting", ["natural", "briggs"])
+def test_flagged_samples_holding_nan_grid_as_zeros(mti, ctx, c1, c1_params, weighting):
+    """A flagged correlation contributes nothing whatever its visibility holds
+    (real data often carries NaN there): the grids and normalisers equal those
+    of the same data with zeros in the flagged samples (to the order of FP64
+    atomic sums)."""
+    dev = H.Dev(c1, torch)
+    idx = H.gpu_sort(ctx, dev, c1.nant, torch)
+    K = ctx.build_kernels(c1_params)
+    fl = dev.flags.bool()[:, :2]
+    vz = dev.vis[:, :2].clone()
+    vz[fl] = 0
+    vn = vz.clone()
+    vn[fl] = complex(float("nan"), float("nan"))
+    out = []
+    for vis in (vz, vn):
+        v = mti.vis_ms(dev.uvw, dev.invl[:2].contiguous(), vis.contiguous(), dev.flags[:, :2].contiguous(),
+                       dev.weight, idx)
+        g = torch.empty((2, c1_params.ny, c1_params.nx, dev.npol, 2), dtype=torch.float64, device="cuda")
+        n = torch.empty((2, dev.npol), dtype=torch.float64, device="cuda")
+        ctx.grid(K, c1_params, v, g, n, nchan=2, weighting=weighting, precision="c128")
+        torch.cuda.synchronize()
+        out.append((g.cpu().numpy(), n.cpu().numpy()))
+    assert fl.any() and np.isfinite(out[1][0]).all()
+    # the same records and weights: equal up to the order of FP64 atomics
+    # (normaliser sums across blocks, tiles split over work units)
+    np.testing.assert_allclose(out[1][1], out[0][1], rtol=1e-12)
+    assert H.rel_rms(out[1][0], out[0][0]) < 1e-12
+
+
 def test_all_flagged_and_empty(mti, ctx, c1_params):
     p = c1_params
     K = ctx.build_kernels(p)
