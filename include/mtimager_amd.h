@@ -124,8 +124,8 @@ typedef struct {
      * finalize the grids: u8 [nchan][ceil(ny/32)][ceil(nx/32)] (device),
      * written by the call -- 1 where some record's support box reaches the
      * 32 x 32-cell tile, 0 where none does; tiles marked 0 are then left
-     * unwritten (undefined) instead of zero-filled.  Needs complex128 grids
-     * and accumulate = 0; mti_finalize_sparse reads those tiles as zeros.
+     * unwritten (undefined) instead of zero-filled.  Needs accumulate = 0;
+     * mti_finalize_sparse reads those tiles as zeros.
      * No counterpart in the reference (its grids are always complete). */
     uint8_t *tile_mask;
 } mti_grid_opts;
@@ -228,14 +228,16 @@ mti_status mti_grid(mti_ctx *ctx, const mti_kernels *k, const mti_image_params *
  * parameters, convolution functions, visibilities and outputs, gridded by one
  * persistent launch per pass (passes split only at the record/tile limits)
  * and finalized with batched transforms per image size.  All jobs of a call
- * share npol and the table precision; grids are complex128. */
+ * share npol and the table precision; each job's grids are complex128, or
+ * complex64 with grids_c64. */
 typedef struct {
     const mti_kernels *kernels;  /* tables built for `params`                  */
     mti_image_params params;
     mti_vis vis;
     int32_t chan0;               /* first channel of `vis` imaged              */
     int32_t nchan;               /* channels imaged (one grid/image each)      */
-    void   *grids;               /* complex128 [nchan] grids, opts.grid_layout */
+    void   *grids;               /* [nchan] grids, opts.grid_layout; complex128,
+                                    or complex64 if grids_c64                  */
     double *norm;                /* f64 [nchan][npol]                          */
     float  *image;               /* f32 [nchan][npol][ny][nx] (finalize)       */
     /* Optional: the antenna columns of vis's rows (int32 [vis.nrows], device)
@@ -251,10 +253,14 @@ typedef struct {
      * (u8 [nchan][ceil(ny/32)][ceil(nx/32)]): written by mti_grid_jobs
      * (accumulate = 0), read by mti_finalize_jobs. */
     uint8_t *tile_mask;
+    /* 1: grids are complex64 (FP64 sums rounded once as a cell is added, float
+     * atomics where a tile is split over work units); 0: complex128. */
+    int32_t grids_c64;
 } mti_job;
 
 /* Grids every job (opts: accumulate, psf, weighting, robust, grid_layout;
- * grid_precision must be MTI_GRID_C128; chan0/nchan come from the jobs). */
+ * opts.grid_precision is not used -- each job's grids_c64 gives its grids'
+ * element type; chan0/nchan come from the jobs). */
 mti_status mti_grid_jobs(mti_ctx *ctx, int32_t njobs, const mti_job *jobs,
                          const mti_grid_opts *opts, mti_grid_stats *stats, void *stream);
 

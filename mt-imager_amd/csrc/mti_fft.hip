@@ -617,26 +617,29 @@ __global__ __launch_bounds__(pk_e(N) / 16) void k_rows_pk(const PackDesc *pd, XT
         s_mask[ab][f][t] = v;
     }
     __syncthreads();
-    // grid loads through global-address-space pointers (see raw_load)
+    // grid loads through global-address-space pointers (see raw_load); an
+    // element comes back as raw bits (all zero bits in a tile the mask leaves
+    // out: +0 in either precision) and is widened only after all four loads
+    // are issued, so no conversion inside the mask branch waits for its load
     using Raw = RawOf<TI>;
     auto elem = [&](const TI *P0, int ab, int f, int y, int x) {
         const int gx = (x + N / 2) & (N - 1);
-        if (!s_mask[ab][f][gx >> 5]) return make_double2(0.0, 0.0);
-        Raw t;
-        raw_load(P0, ((size_t)((y + (ny >> 1)) & (ny - 1)) * N + gx) * cs, t);
-        return raw_to_c(t);
+        Raw t{};
+        if (s_mask[ab][f][gx >> 5]) raw_load(P0, ((size_t)((y + (ny >> 1)) & (ny - 1)) * N + gx) * cs, t);
+        return t;
     };
     auto load = [&](int f, int x) {
         const int pair = blockIdx.x * PB + (f >> 1);
         if (pair >= npairs) return make_double2(0.0, 0.0);
         const int y = (f & 1) ? ((ny - pair) & (ny - 1)) : pair;
         const int ym = (ny - y) & (ny - 1), xm = (N - x) & (N - 1);
-        const double2 a0 = elem(A, 0, f, y, x), a1 = elem(A, 0, f ^ 1, ym, xm);
-        double2 b0 = make_double2(0.0, 0.0), b1 = b0;
+        const Raw ra0 = elem(A, 0, f, y, x), ra1 = elem(A, 0, f ^ 1, ym, xm);
+        Raw rb0{}, rb1{};
         if (B) {
-            b0 = elem(B, 1, f, y, x);
-            b1 = elem(B, 1, f ^ 1, ym, xm);
+            rb0 = elem(B, 1, f, y, x);
+            rb1 = elem(B, 1, f ^ 1, ym, xm);
         }
+        const double2 a0 = raw_to_c(ra0), a1 = raw_to_c(ra1), b0 = raw_to_c(rb0), b1 = raw_to_c(rb1);
         // S_s at (y, x) and at (-y, -x)
         const double2 s0 = c2add(c2mul(ka0, a0), c2mul(kb0, b0)), s0m = c2add(c2mul(ka0, a1), c2mul(kb0, b1));
         const double2 s1 = c2add(c2mul(ka1, a0), c2mul(kb1, b0)), s1m = c2add(c2mul(ka1, a1), c2mul(kb1, b1));
@@ -1254,6 +1257,7 @@ extern "C" mti_status mti_finalize_jobs(mti_ctx *ctx, int32_t njobs, const mti_j
         if (r != MTI_OK) return r;
         MTI_CHECK(ctx, jobs[i].vis.npol == P, "jobs of one call must share npol");
         MTI_CHECK(ctx, jobs[i].grids && jobs[i].norm && jobs[i].image && jobs[i].nchan >= 1, "null job outputs");
+        MTI_CHECK(ctx, jobs[i].grids_c64 == 0 || jobs[i].grids_c64 == 1, "job grids_c64 must be 0 or 1");
     }
     MTI_CHECK(ctx, P == 1 || P == 2 || P == 4, "npol must be 1, 2 or 4");
     MTI_HIP(ctx, hipSetDevice(ctx->device));
@@ -1292,37 +1296,44 @@ extern "C" mti_status mti_finalize_jobs(mti_ctx *ctx, int32_t njobs, const mti_j
                            smp);
     }
     MTI_HIP(ctx, hipGetLastError());
-    // one packed-plane transform per image size over the channels of all its jobs
-    std::vector<std::pair<int, int>> sizes;
+    // one packed-plane transform per image size (and grid element type) over
+    // the channels of all its jobs
+    std::vector<std::array<int, 3>> sizes;
     for (int i = 0; i < njobs; i++) {
-        const std::pair<int, int> sz{jobs[i].params.nx, jobs[i].params.ny};
+        const std::array<int, 3> sz{jobs[i].params.nx, jobs[i].params.ny, jobs[i].grids_c64 ? 1 : 0};
         if (std::find(sizes.begin(), sizes.end(), sz) == sizes.end()) sizes.push_back(sz);
     }
-    for (auto sz : sizes) {
-        const FftGeom g = geom(sz.first, sz.second);
+    for (const auto &sz : sizes) {
+        const FftGeom g = geom(sz[0], sz[1]);
+        const bool c64 = sz[2] != 0;
+        auto member = [&](const mti_job &j) {
+            return j.params.nx == sz[0] && j.params.ny == sz[1] && (j.grids_c64 != 0) == c64;
+        };
         if (!pk_ok(g)) {
             // small images: one job at a time through the per-job finalize
             for (int i = 0; i < njobs; i++) {
                 const mti_job &j = jobs[i];
-                if (j.params.nx != sz.first || j.params.ny != sz.second) continue;
-                r = mti_finalize_sparse(ctx, &j.params, P, poltype, j.nchan, grid_layout, MTI_GRID_C128, j.grids,
-                                        j.tile_mask, j.norm, j.image, stream);
+                if (!member(j)) continue;
+                r = mti_finalize_sparse(ctx, &j.params, P, poltype, j.nchan, grid_layout,
+                                        c64 ? MTI_GRID_C64 : MTI_GRID_C128, j.grids, j.tile_mask, j.norm, j.image,
+                                        stream);
                 if (r != MTI_OK) return r;
             }
             continue;
         }
-        const size_t chan_grid = (size_t)sz.first * sz.second * P;
+        const size_t chan_grid = (size_t)sz[0] * sz[1] * P;
+        const size_t elem = c64 ? sizeof(float2) : sizeof(double2);
         std::vector<ChanRef> ch;
         for (int i = 0; i < njobs; i++) {
             const mti_job &j = jobs[i];
-            if (j.params.nx != sz.first || j.params.ny != sz.second) continue;
-            const size_t tiles = (size_t)((sz.first + 31) / 32) * ((sz.second + 31) / 32);
+            if (!member(j)) continue;
+            const size_t tiles = (size_t)((sz[0] + 31) / 32) * ((sz[1] + 31) / 32);
             for (int c = 0; c < j.nchan; c++)
-                ch.push_back(ChanRef{(const double2 *)j.grids + chan_grid * c, j.norm + (size_t)c * P,
-                                     j.image + chan_grid * c, corr + coff[i], corr + coff[i] + sz.first,
+                ch.push_back(ChanRef{(const char *)j.grids + chan_grid * c * elem, j.norm + (size_t)c * P,
+                                     j.image + chan_grid * c, corr + coff[i], corr + coff[i] + sz[0],
                                      j.tile_mask ? j.tile_mask + tiles * c : nullptr});
         }
-        r = finalize_channels(ctx, g, ch, P, poltype, grid_layout, false, st);
+        r = finalize_channels(ctx, g, ch, P, poltype, grid_layout, c64, st);
         if (r != MTI_OK) return r;
     }
     return MTI_OK;

@@ -448,6 +448,43 @@ __device__ __forceinline__ Visit visit_issue(int i, int n, const ListEntry *list
     return v;
 }
 
+// Adds a cell's FP64 sums acc (re, im per correlation) to the grid at g
+// (correlations ps complex elements apart): stored when the unit is the
+// tile's only one and the grid was not pre-filled, read-modify-write when it
+// is the only one of an accumulating call, atomics otherwise.  complex64
+// grids round each sum once (the read-modify-write adds in FP64 first).
+template <int P, typename R>
+__device__ __forceinline__ void flush_cell(R *g, int64_t ps, const double (&acc)[2 * P], int excl, int fresh)
+{
+    if (excl && fresh) {
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            g[2 * p * ps] = (R)acc[2 * p];
+            g[2 * p * ps + 1] = (R)acc[2 * p + 1];
+        }
+    } else if (excl) {
+        R old[2 * P];
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            const bool nz = acc[2 * p] != 0.0 || acc[2 * p + 1] != 0.0;
+            old[2 * p] = nz ? g[2 * p * ps] : (R)0;
+            old[2 * p + 1] = nz ? g[2 * p * ps + 1] : (R)0;
+        }
+#pragma unroll
+        for (int p = 0; p < P; p++)
+            if (acc[2 * p] != 0.0 || acc[2 * p + 1] != 0.0) {
+                g[2 * p * ps] = (R)((double)old[2 * p] + acc[2 * p]);
+                g[2 * p * ps + 1] = (R)((double)old[2 * p + 1] + acc[2 * p + 1]);
+            }
+    } else {
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            if (acc[2 * p] != 0.0) unsafeAtomicAdd(g + 2 * p * ps, (R)acc[2 * p]);
+            if (acc[2 * p + 1] != 0.0) unsafeAtomicAdd(g + 2 * p * ps + 1, (R)acc[2 * p + 1]);
+        }
+    }
+}
+
 // Gridder: one 1024-thread workgroup per unit; wave w owns the 8x8 region w
 // of the 32x32 tile and each lane one cell, accumulated in FP64 registers.
 // Records are staged in LDS per batch: their weighted visibilities, and for
@@ -559,28 +596,11 @@ __global__ __launch_bounds__(TG_THREADS, PREC == MTI_TABLE_F32 ? 8 : 1) void k_g
         }
         // add the cell to the grid
         if (gx < jd.g.nx && gy < jd.g.ny) {
-            double *g = jd.grid + 2 * ((int64_t)ta.c_local * jd.chan_stride + ((int64_t)gy * jd.g.nx + gx) * jd.cs);
-            const int64_t ps = jd.ps;
-            if (excl && a.fresh) {
-#pragma unroll
-                for (int p = 0; p < P; p++) *(double2 *)(g + 2 * p * ps) = make_double2(acc[2 * p], acc[2 * p + 1]);
-            } else if (excl) {
-                double2 old[P];
-#pragma unroll
-                for (int p = 0; p < P; p++)
-                    old[p] = (acc[2 * p] != 0.0 || acc[2 * p + 1] != 0.0) ? *(const double2 *)(g + 2 * p * ps)
-                                                                         : make_double2(0.0, 0.0);
-#pragma unroll
-                for (int p = 0; p < P; p++)
-                    if (acc[2 * p] != 0.0 || acc[2 * p + 1] != 0.0)
-                        *(double2 *)(g + 2 * p * ps) = make_double2(old[p].x + acc[2 * p], old[p].y + acc[2 * p + 1]);
-            } else {
-#pragma unroll
-                for (int p = 0; p < P; p++) {
-                    if (acc[2 * p] != 0.0) unsafeAtomicAdd(g + 2 * p * ps, acc[2 * p]);
-                    if (acc[2 * p + 1] != 0.0) unsafeAtomicAdd(g + 2 * p * ps + 1, acc[2 * p + 1]);
-                }
-            }
+            const int64_t e = (int64_t)ta.c_local * jd.chan_stride + ((int64_t)gy * jd.g.nx + gx) * jd.cs;
+            if (jd.grid_c64)
+                flush_cell<P>((float *)jd.grid + 2 * e, jd.ps, acc, excl, a.fresh);
+            else
+                flush_cell<P>((double *)jd.grid + 2 * e, jd.ps, acc, excl, a.fresh);
         }
     }
 }
@@ -624,7 +644,7 @@ __global__ __launch_bounds__(256) void k_zero_tiles(JobSet js, const uint32_t *u
             const JobDev &jd = js.jobs[ta.job];
             const int nx = jd.g.nx, ny = jd.g.ny, x0 = ta.tx * TG_T, y0 = ta.ty * TG_T;
             const int w = min(TG_T, nx - x0), h = min(TG_T, ny - y0), cells = w * h;
-            double2 *g = (double2 *)jd.grid + (int64_t)ta.c_local * jd.chan_stride;
+            const int64_t g0 = (int64_t)ta.c_local * jd.chan_stride;
             for (int i = lane; i < cells * P; i += 64) {
                 int p, cell;
                 if (jd.ps == 1) {  // [v][u][P]: correlations of a cell adjacent
@@ -635,7 +655,11 @@ __global__ __launch_bounds__(256) void k_zero_tiles(JobSet js, const uint32_t *u
                     cell = i - p * cells;
                 }
                 const int cy = cell / w, cx = cell - cy * w;
-                g[((int64_t)(y0 + cy) * nx + x0 + cx) * jd.cs + p * jd.ps] = make_double2(0.0, 0.0);
+                const int64_t e = g0 + ((int64_t)(y0 + cy) * nx + x0 + cx) * jd.cs + p * jd.ps;
+                if (jd.grid_c64)
+                    ((float2 *)jd.grid)[e] = make_float2(0.0f, 0.0f);
+                else
+                    ((double2 *)jd.grid)[e] = make_double2(0.0, 0.0);
             }
         }
     }
@@ -659,14 +683,6 @@ __global__ void k_fill_f64(double *p, int64_t n, double v)
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
         p[i] = v;
-}
-
-// out = (accumulate ? out : 0) + work, FP64 work grid -> complex64 output.
-__global__ void k_round_grid(const double *work, float *out, int64_t n, int accumulate)
-{
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        out[i] = accumulate ? (float)((double)out[i] + work[i]) : (float)work[i];
 }
 
 // Briggs f^2 = (5*10^-R)^2 / (sum W^2 / sum w) per (job, channel, correlation).
@@ -740,10 +756,11 @@ struct PassJob {
     mti_image_params p;
     VisView vw;
     int chan0, nb;
-    double *grid;      // FP64 grid of chan0 in the job's layout
+    void *grid;        // grid of chan0 in the job's layout (complex128, or complex64 if c64)
     int layout;
     double *norm;      // [nb][P]
     uint8_t *mask;     // [nb][nty][ntx] tile mask (null: complete grids)
+    int c64 = 0;       // complex64 grids
 };
 
 // Limits of one pass: records < 2^31, (record, tile) pairs < 2^32, keys < 2^31.
@@ -834,6 +851,7 @@ static mti_status grid_pass(mti_ctx *ctx, const std::vector<PassJob> &jobs, cons
         d.sums = j.k->d_sums;
         d.T2 = j.k->info.T * j.k->info.T;
         d.grid = j.grid;
+        d.grid_c64 = j.c64;
         const int64_t gsz = (int64_t)j.p.nx * j.p.ny;
         d.cs = j.layout == MTI_GRID_VUP ? P : 1;
         d.ps = j.layout == MTI_GRID_VUP ? 1 : gsz;
@@ -860,8 +878,11 @@ static mti_status grid_pass(mti_ctx *ctx, const std::vector<PassJob> &jobs, cons
             for (const PassJob &j : jobs) {
                 if (j.mask)
                     hipLaunchKernelGGL(k_fill_u8, dim3(64), dim3(256), 0, st, j.mask, job_keys(j), (uint8_t)0);
+                else if (j.c64)  // 0.0f is all zero bits
+                    hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, st, (uint32_t *)j.grid,
+                                       (int64_t)j.p.nx * j.p.ny * P * 2 * j.nb, 0u);
                 else
-                    hipLaunchKernelGGL(k_fill_f64, dim3(1024), dim3(256), 0, st, j.grid,
+                    hipLaunchKernelGGL(k_fill_f64, dim3(1024), dim3(256), 0, st, (double *)j.grid,
                                        (int64_t)j.p.nx * j.p.ny * P * 2 * j.nb, 0.0);
             }
         MTI_HIP(ctx, hipGetLastError());
@@ -1096,7 +1117,7 @@ static mti_status for_each_pass(const std::vector<PassJob> &all, int P, F fn)
             PassJob piece = j;
             piece.chan0 = j.chan0 + c0;
             piece.nb = (int)std::min<int64_t>(j.nb - c0, cap);
-            piece.grid = j.grid ? j.grid + 2 * gch * c0 : nullptr;
+            piece.grid = j.grid ? (char *)j.grid + (size_t)gch * c0 * (j.c64 ? 8 : 16) : nullptr;
             piece.norm = j.norm ? j.norm + (size_t)c0 * P : nullptr;
             piece.mask = j.mask ? j.mask + (size_t)c0 * job_keys(one_channel(j)) : nullptr;
             if (!pass.empty() &&
@@ -1141,30 +1162,18 @@ static void plan_grid_jobs(const std::vector<PassJob> &all, int32_t njobs, const
     });
 }
 
-// Channel batches of one mti_grid call: passes of at most *pair_batch
-// channels (record / pair limits); complex64 output goes through an FP64
-// work grid of *batch channels (at most 8 GB).
-static void grid_batches(const PassJob &base, int nb, int P, const mti_grid_opts *o, int *pair_batch, int *batch)
+// Channels per pass of one mti_grid call (record / pair limits).
+static int grid_batch(const PassJob &base, int nb)
 {
-    *pair_batch = (int)std::min<int64_t>(nb, std::max<int64_t>(1, channels_per_pass(base)));
-    *batch = *pair_batch;
-    if (o->grid_precision == MTI_GRID_C64) {
-        const size_t chan_bytes = (size_t)base.p.nx * base.p.ny * P * 2 * sizeof(double);
-        const int b = (int)std::max<size_t>(1, std::min<size_t>((size_t)nb, (size_t(8) << 30) / chan_bytes));
-        *batch = std::min(b, *pair_batch);
-    }
+    return (int)std::min<int64_t>(nb, std::max<int64_t>(1, channels_per_pass(base)));
 }
 
 // The scratch plan of one mti_grid call (the same batches and passes).
 static void plan_grid(const PassJob &base, int nb, int P, const mti_grid_opts *o, ScratchPlan &pl)
 {
     pl.need(SLOT_MISC2, 2 * BIN_STAT_STRIPES * sizeof(unsigned long long));
-    int pair_batch, batch;
-    grid_batches(base, nb, P, o, &pair_batch, &batch);
     const bool weighted = o->weighting != MTI_WEIGHT_NATURAL;
-    const int step = o->grid_precision == MTI_GRID_C128 ? pair_batch : batch;
-    if (o->grid_precision == MTI_GRID_C64)
-        pl.need(SLOT_GRID64, (size_t)base.p.nx * base.p.ny * P * 2 * sizeof(double) * batch);
+    const int step = grid_batch(base, nb);
     for (int c0 = 0; c0 < nb && base.vw.nidx > 0; c0 += step) {
         PassJob j = base;
         j.nb = std::min(step, nb - c0);
@@ -1214,11 +1223,7 @@ extern "C" mti_status mti_grid(mti_ctx *ctx, const mti_kernels *k, const mti_ima
     if (r != MTI_OK) return r;
     MTI_CHECK(ctx, grids && norm, "null outputs");
     MTI_CHECK(ctx, o->nchan >= 1 && o->nchan <= 65535, "opts.nchan must be in [1, 65535]");
-    if (o->tile_mask) {
-        MTI_CHECK(ctx, !o->accumulate, "a tile mask (sparse grids) needs accumulate = 0");
-        if (o->grid_precision != MTI_GRID_C128)
-            return set_error(ctx, MTI_ERR_UNSUPPORTED, "a tile mask (sparse grids) needs complex128 grids");
-    }
+    if (o->tile_mask) MTI_CHECK(ctx, !o->accumulate, "a tile mask (sparse grids) needs accumulate = 0");
     if (v->inv_lambda_per_row)
         MTI_CHECK(ctx, o->nchan == 1, "per-row 1/lambda implies a single channel per call");
     else
@@ -1227,26 +1232,32 @@ extern "C" mti_status mti_grid(mti_ctx *ctx, const mti_kernels *k, const mti_ima
     hipStream_t st = to_stream(stream);
 
     const int P = v->npol, nb = o->nchan;
+    const int c64 = o->grid_precision == MTI_GRID_C64 ? 1 : 0;
     const int64_t gsz = (int64_t)p->nx * p->ny;
-    const int64_t chan_floats = gsz * P * 2;  // reals per channel grid
+    const int64_t chan_reals = gsz * P * 2;  // reals per channel grid
     VisView vw = make_view(v);
     MTI_CHECK(ctx, vw.nidx < (int64_t(1) << 31), "more than 2^31-1 rows per call");
+    // channel batches keep a pass within the record / pair limits
+    PassJob base{k, *p, vw, o->chan0, 1, nullptr, o->grid_layout, nullptr};
+    base.c64 = c64;
     CallScope cs(ctx);
     if (cs.planning()) {
         ScratchPlan pl;
-        plan_grid(PassJob{k, *p, vw, o->chan0, 1, nullptr, o->grid_layout, nullptr}, nb, P, o, pl);
+        plan_grid(base, nb, P, o, pl);
         if ((r = cs.begin(pl)) != MTI_OK) return r;
     }
 
     if (!o->accumulate) {
         hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(256), 0, st, norm, (int64_t)nb * P, 0.0);
         // with rows, the gridding pass zeroes exactly the tiles it does not store whole
-        if (o->grid_precision == MTI_GRID_C128 && vw.nidx == 0) {
+        if (vw.nidx == 0) {
             if (o->tile_mask)
                 hipLaunchKernelGGL(k_fill_u8, dim3(64), dim3(256), 0, st, o->tile_mask,
                                    (int64_t)nb * ((p->nx + TG_T - 1) / TG_T) * ((p->ny + TG_T - 1) / TG_T), (uint8_t)0);
+            else if (c64)  // 0.0f is all zero bits
+                hipLaunchKernelGGL(k_fill_u32, dim3(8192), dim3(256), 0, st, (uint32_t *)grids, chan_reals * nb, 0u);
             else
-                hipLaunchKernelGGL(k_fill_f64, dim3(8192), dim3(256), 0, st, (double *)grids, chan_floats * nb, 0.0);
+                hipLaunchKernelGGL(k_fill_f64, dim3(8192), dim3(256), 0, st, (double *)grids, chan_reals * nb, 0.0);
         }
         MTI_HIP(ctx, hipGetLastError());
     }
@@ -1258,49 +1269,18 @@ extern "C" mti_status mti_grid(mti_ctx *ctx, const mti_kernels *k, const mti_ima
     MTI_HIP(ctx, hipGetLastError());
     uint32_t nu[2] = {0, 0};
 
-    // channel batches keep a pass within the record / pair limits
-    const PassJob base{k, *p, vw, o->chan0, 1, nullptr, o->grid_layout, nullptr};
     MTI_CHECK(ctx, channels_per_pass(base) >= 1, "one channel has too many (record, tile) pairs");
-    int pair_batch, batch;
-    grid_batches(base, nb, P, o, &pair_batch, &batch);
+    const int pair_batch = grid_batch(base, nb);
     const int prec = k->info.precision;
-
-    if (o->grid_precision == MTI_GRID_C128) {
-        for (int c0 = 0; c0 < nb && vw.nidx > 0; c0 += pair_batch) {
-            PassJob j = base;
-            j.chan0 = o->chan0 + c0;
-            j.nb = std::min(pair_batch, nb - c0);
-            j.grid = (double *)grids + (size_t)c0 * chan_floats;
-            j.norm = norm + (size_t)c0 * P;
-            j.mask = o->tile_mask ? o->tile_mask + (size_t)c0 * job_keys(one_channel(base)) : nullptr;
-            r = grid_pass(ctx, {j}, o, P, prec, !o->accumulate, dstats, nu, st);
-            if (r != MTI_OK) return r;
-        }
-    } else {
-        // FP64 work grid for a batch of channels, rounded once into the output
-        const size_t chan_bytes = (size_t)chan_floats * sizeof(double);
-        void *gw = nullptr;
-        r = scratch_get(ctx, SLOT_GRID64, chan_bytes * batch, &gw);
+    for (int c0 = 0; c0 < nb && vw.nidx > 0; c0 += pair_batch) {
+        PassJob j = base;
+        j.chan0 = o->chan0 + c0;
+        j.nb = std::min(pair_batch, nb - c0);
+        j.grid = (char *)grids + (size_t)c0 * chan_reals * (c64 ? 4 : 8);
+        j.norm = norm + (size_t)c0 * P;
+        j.mask = o->tile_mask ? o->tile_mask + (size_t)c0 * job_keys(one_channel(base)) : nullptr;
+        r = grid_pass(ctx, {j}, o, P, prec, !o->accumulate, dstats, nu, st);
         if (r != MTI_OK) return r;
-        for (int c0 = 0; c0 < nb; c0 += batch) {
-            const int n = std::min(batch, nb - c0);
-            if (vw.nidx > 0) {
-                // the work grid is never pre-filled: the pass zeroes what it does not store
-                PassJob j = base;
-                j.chan0 = o->chan0 + c0;
-                j.nb = n;
-                j.grid = (double *)gw;
-                j.norm = norm + (size_t)c0 * P;
-                r = grid_pass(ctx, {j}, o, P, prec, true, dstats, nu, st);
-                if (r != MTI_OK) return r;
-            } else {
-                hipLaunchKernelGGL(k_fill_f64, dim3(8192), dim3(256), 0, st, (double *)gw, chan_floats * n, 0.0);
-                MTI_HIP(ctx, hipGetLastError());
-            }
-            hipLaunchKernelGGL(k_round_grid, dim3(8192), dim3(256), 0, st, (const double *)gw,
-                               (float *)grids + (size_t)c0 * chan_floats, chan_floats * n, o->accumulate);
-            MTI_HIP(ctx, hipGetLastError());
-        }
     }
     return stats_out(ctx, dstats, nu, stats, st);
 }
@@ -1334,7 +1314,6 @@ extern "C" mti_status mti_grid_jobs(mti_ctx *ctx, int32_t njobs, const mti_job *
     mti_status r = check_opts(ctx, o);
     if (r != MTI_OK) return r;
     MTI_CHECK(ctx, njobs >= 0 && njobs <= 65535 && (njobs == 0 || jobs), "njobs must be in [0, 65535]");
-    MTI_CHECK(ctx, o->grid_precision == MTI_GRID_C128, "mti_grid_jobs grids into complex128 grids");
     MTI_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = to_stream(stream);
     int P = 0, prec = -1;
@@ -1362,8 +1341,9 @@ extern "C" mti_status mti_grid_jobs(mti_ctx *ctx, int32_t njobs, const mti_job *
         MTI_CHECK(ctx, j.vis.npol == P, "jobs of one call must share npol");
         MTI_CHECK(ctx, j.kernels->info.precision == prec, "jobs of one call must share the table precision");
         if (j.tile_mask) MTI_CHECK(ctx, !o->accumulate, "a job tile mask (sparse grids) needs accumulate = 0");
-        const PassJob pj{j.kernels, j.params, make_view(&j.vis), j.chan0, j.nchan, (double *)j.grids, o->grid_layout,
-                         j.norm, j.tile_mask};
+        MTI_CHECK(ctx, j.grids_c64 == 0 || j.grids_c64 == 1, "job grids_c64 must be 0 or 1");
+        const PassJob pj{j.kernels, j.params, make_view(&j.vis), j.chan0, j.nchan, j.grids, o->grid_layout,
+                         j.norm, j.tile_mask, j.grids_c64};
         MTI_CHECK(ctx, pj.vw.nidx < (int64_t(1) << 31), "more than 2^31-1 rows in a job");
         MTI_CHECK(ctx, channels_per_pass(pj) >= 1, "one channel has too many (record, tile) pairs");
         all.push_back(pj);

@@ -47,7 +47,9 @@ struct JobDev {
     uint32_t table_bytes;    // its size (at most 2^32 - 16)
     const float *sums;       // [nplanes][T*T]
     int T2;
-    double *grid;            // FP64 grid of channel chan0; channel c at grid + 2*c*chan_stride
+    void *grid;              // grid of channel chan0, complex128 (or complex64 if grid_c64);
+                             // channel c at complex element c*chan_stride
+    int grid_c64;
     int64_t cs, ps;          // complex strides of cell and correlation
     int64_t chan_stride;     // complex elements per channel grid
     double *norm;            // [nb][P]

@@ -34,15 +34,14 @@ enum ScratchSlot {
     SLOT_FFT = 7,
     SLOT_RADIX = 8,
     SLOT_NORM = 9,
-    SLOT_GRID64 = 10,
-    SLOT_MISC2 = 11,
-    SLOT_RECVIS = 12,   // row order and kept counts of jobs sorted inside mti_grid_jobs
-    SLOT_CRECVIS = 13,  // per-record visibilities + weights
-    SLOT_CHANDESC = 14, // descriptors of the finalize (packed planes)
-    SLOT_CORR = 15,     // per-job correction vectors of the batched finalize
-    SLOT_FCORR = 16,    // correction vectors of mti_finalize
-    SLOT_TWIDDLE = 17,  // FFT twiddles of a call that runs in a caller workspace
-    SLOT_COUNT = 18
+    SLOT_MISC2 = 10,
+    SLOT_RECVIS = 11,   // row order and kept counts of jobs sorted inside mti_grid_jobs
+    SLOT_CRECVIS = 12,  // per-record visibilities + weights
+    SLOT_CHANDESC = 13, // descriptors of the finalize (packed planes)
+    SLOT_CORR = 14,     // per-job correction vectors of the batched finalize
+    SLOT_FCORR = 15,    // correction vectors of mti_finalize
+    SLOT_TWIDDLE = 16,  // FFT twiddles of a call that runs in a caller workspace
+    SLOT_COUNT = 17
 };
 
 // Device scratch one call takes: the largest request of each slot.

@@ -68,7 +68,7 @@ class Job(ctypes.Structure):
                 ("chan0", ctypes.c_int32), ("nchan", ctypes.c_int32),
                 ("grids", ctypes.c_void_p), ("norm", ctypes.c_void_p), ("image", ctypes.c_void_p),
                 ("ant1", ctypes.c_void_p), ("ant2", ctypes.c_void_p), ("nant", ctypes.c_int32),
-                ("tile_mask", ctypes.c_void_p)]
+                ("tile_mask", ctypes.c_void_p), ("grids_c64", ctypes.c_int32)]
 
 
 class GridStats(ctypes.Structure):
@@ -417,16 +417,19 @@ class Context:
     @staticmethod
     def make_job(kern, params, vis, grids, norm, image=None, chan0=0, nchan=1, ant1=None, ant2=None, nant=0,
                  tile_mask=None):
-        """An mti_job; grids complex128 [nchan] grids, norm f64 [nchan][npol],
-        image f32 [nchan][npol][ny][nx] (for finalize_jobs).  With ant1/ant2
-        (int32 device tensors of vis's rows) and nant, grid_jobs computes the
-        baseline-major row order itself (vis must then have no row index)."""
+        """An mti_job; grids [nchan] grids, complex128 (complex128 or float64
+        tensors) or complex64 (complex64 or float32 tensors), norm f64
+        [nchan][npol], image f32 [nchan][npol][ny][nx] (for finalize_jobs).
+        With ant1/ant2 (int32 device tensors of vis's rows) and nant, grid_jobs
+        computes the baseline-major row order itself (vis must then have no
+        row index)."""
         j = Job()
         j.kernels = kern.h
         j.params = params
         j.vis = vis
         j.chan0, j.nchan = int(chan0), int(nchan)
         j.grids, j.norm = grids.data_ptr(), norm.data_ptr()
+        j.grids_c64 = 0 if str(grids.dtype) in ("torch.complex128", "torch.float64") else 1
         j.image = image.data_ptr() if image is not None else None
         if ant1 is not None:
             j.ant1, j.ant2, j.nant = ant1.data_ptr(), ant2.data_ptr(), int(nant)

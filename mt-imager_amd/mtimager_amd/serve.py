@@ -31,11 +31,12 @@ from . import Context, MTIError, vis_ms
 WORK_BYTES_PER_VIS = 64  # record descriptors, weighted vis, pairs and keys per (row, channel)
 
 
-def request_bytes(nx, ny, npol, nchan, nvis):
-    """Device bytes a request holds while admitted: complex128 grids, f64
-    normaliser, f32 image and the per-record work space of its wave."""
+def request_bytes(nx, ny, npol, nchan, nvis, grid_bytes=16):
+    """Device bytes a request holds while admitted: grids (complex128: 16
+    bytes a cell and correlation, complex64: 8), f64 normaliser, f32 image and
+    the per-record work space of its wave."""
     pix = int(nx) * int(ny) * int(npol) * int(nchan)
-    return pix * (16 + 4) + 8 * npol * nchan + WORK_BYTES_PER_VIS * int(nvis)
+    return pix * (int(grid_bytes) + 4) + 8 * npol * nchan + WORK_BYTES_PER_VIS * int(nvis)
 
 
 class Request:
@@ -123,10 +124,14 @@ class Server:
     request);
     weighting/robust/poltype: imaging options shared by all requests;
     transform_storage: "c128" or "c64", the element type of the packed
-    planes between the finalize's transform passes (Context.set_transform_storage)."""
+    planes between the finalize's transform passes (Context.set_transform_storage);
+    grid_precision: "c128" or "c64", the element type of the requests' grids
+    (complex64 grids take half the device memory, so more requests fit the
+    budget)."""
 
     def __init__(self, hbm_budget, workers=2, max_wave=256, cache=None, device=0,
-                 weighting="natural", robust=0.0, poltype="linear", plain=False, transform_storage="c128"):
+                 weighting="natural", robust=0.0, poltype="linear", plain=False, transform_storage="c128",
+                 grid_precision="c128"):
         import torch
         self.torch = torch
         self.budget = int(hbm_budget)
@@ -138,6 +143,9 @@ class Server:
         self.device = device
         self.weighting, self.robust, self.poltype = weighting, robust, poltype
         self.transform_storage = transform_storage
+        if grid_precision not in ("c128", "c64"):
+            raise ValueError(f"grid_precision {grid_precision!r}")
+        self.grid_precision = grid_precision
         self.gridded = 0
         self.errors = []
         self.wave_sizes = []
@@ -156,6 +164,9 @@ class Server:
 
     # ---- client side --------------------------------------------------------
     def submit(self, req, now=None):
+        p = req.params
+        req.nbytes = request_bytes(p.nx, p.ny, req.npol, req.nchan, req.nvis,
+                                   8 if self.grid_precision == "c64" else 16)
         if req.nbytes > self.budget:
             raise MTIError(f"request {req.rid} needs {req.nbytes} bytes, above the HBM budget {self.budget}")
         req.arrival = time.perf_counter() if now is None else now
@@ -255,7 +266,8 @@ class Server:
         p = r.params
         from . import tile_mask_shape
         with torch.cuda.stream(torch.cuda.default_stream(self.device)):
-            out = (torch.empty((r.nchan, p.ny, p.nx, r.npol, 2), dtype=torch.float64, device="cuda"),
+            gdt = torch.float32 if self.grid_precision == "c64" else torch.float64
+            out = (torch.empty((r.nchan, p.ny, p.nx, r.npol, 2), dtype=gdt, device="cuda"),
                    torch.empty((r.nchan, r.npol), dtype=torch.float64, device="cuda"),
                    torch.empty((r.nchan, r.npol, p.ny, p.nx), dtype=torch.float32, device="cuda"),
                    # the grids are internal: sparse, with a tile mask
@@ -291,10 +303,10 @@ class Server:
             if self.plain:
                 r, K, v, grids, norm, image, mask = owned[0]
                 st = ctx.grid(K, r.params, v, grids, norm, chan0=r.chan0, nchan=r.nchan,
-                              weighting=self.weighting, robust=self.robust, precision="c128", stream=sid,
+                              weighting=self.weighting, robust=self.robust, precision=self.grid_precision, stream=sid,
                               tile_mask=mask)
                 ctx.finalize(r.params, r.npol, grids, norm, image, nchan=r.nchan, poltype=self.poltype,
-                             precision="c128", stream=sid, tile_mask=mask)
+                             precision=self.grid_precision, stream=sid, tile_mask=mask)
             else:
                 st = ctx.grid_jobs(jobs, weighting=self.weighting, robust=self.robust, stream=sid)
                 ctx.finalize_jobs(jobs, poltype=self.poltype, stream=sid)

@@ -254,16 +254,25 @@ def test_batch_of_channels_equals_single_calls(mti, ctx, c1, c1_params):
         np.testing.assert_allclose(b["norm"][ci], s["norm"][0], rtol=1e-12)
 
 
-def test_planar_grid_layout_matches(orc, mti, ctx, c1, c1_params):
+# Two runs of one call agree to the order of atomic sums in tiles split over
+# work units: FP64 atomics (complex128 grids) leave images equal to ~1e-12;
+# float atomics (complex64 grids, as the reference's) change the last bits of
+# those cells, which the taper correction amplifies at the field edges to
+# ~2e-5 of the image RMS.
+RERUN_IMAGE_TOL = {"c128": 1e-6, "c64": 5e-5}
+
+
+@pytest.mark.parametrize("precision", ["c128", "c64"])
+def test_planar_grid_layout_matches(orc, mti, ctx, c1, c1_params, precision):
     """[P][v][u] grids (what PreIFFTUVCorrection reorders to) give the same
     grid and image as the reference's [v][u][P] layout."""
     dev = H.Dev(c1, torch)
     idx = H.gpu_sort(ctx, dev, c1.nant, torch)
     K = ctx.build_kernels(c1_params)
-    a = H.gpu_image(mti, ctx, K, c1_params, dev, torch, [4], idx, layout="vup")
-    b = H.gpu_image(mti, ctx, K, c1_params, dev, torch, [4], idx, layout="pvu")
+    a = H.gpu_image(mti, ctx, K, c1_params, dev, torch, [4], idx, layout="vup", precision=precision)
+    b = H.gpu_image(mti, ctx, K, c1_params, dev, torch, [4], idx, layout="pvu", precision=precision)
     assert H.rel_rms(b["grid"][0], a["grid"][0]) < 1e-6
-    assert H.rel_rms(b["image"][0], a["image"][0]) < 1e-6
+    assert H.rel_rms(b["image"][0], a["image"][0]) < RERUN_IMAGE_TOL[precision]
     r = H.oracle_image(orc, orc.Kernels(H.oracle_params(orc, c1_params)),
                        H.oracle_params(orc, c1_params), c1, 4, idx.cpu().numpy())
     assert H.rel_rms(b["image"][0][0], r["image"][0]) < IMAGE_RMS_TOL

@@ -119,9 +119,7 @@ def test_sparse_grids_are_refused_where_they_cannot_apply(mti, ctx):
     v = mti.vis_ms(d.uvw, d.invl, d.vis, d.flags, d.weight)
     mask = torch.empty(mti.tile_mask_shape(p, 1), dtype=torch.uint8, device="cuda")
     n = torch.zeros((1, 1), dtype=torch.float64, device="cuda")
-    g128 = torch.zeros((1, 256, 256, 1), dtype=torch.complex128, device="cuda")
-    with pytest.raises(mti.MTIError, match="accumulate"):
-        ctx.grid(K, p, v, g128, n, nchan=1, accumulate=True, precision="c128", tile_mask=mask)
-    g64 = torch.zeros((1, 256, 256, 1), dtype=torch.complex64, device="cuda")
-    with pytest.raises(mti.MTIError, match="complex128"):
-        ctx.grid(K, p, v, g64, n, nchan=1, precision="c64", tile_mask=mask)
+    for prec, dt in (("c128", torch.complex128), ("c64", torch.complex64)):
+        g = torch.zeros((1, 256, 256, 1), dtype=dt, device="cuda")
+        with pytest.raises(mti.MTIError, match="accumulate"):
+            ctx.grid(K, p, v, g, n, nchan=1, accumulate=True, precision=prec, tile_mask=mask)

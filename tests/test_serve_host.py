@@ -36,6 +36,16 @@ def test_request_bytes():
     assert r.nbytes == serve.request_bytes(512, 512, 1, 4, 400)
 
 
+def test_complex64_grids_halve_the_grid_bytes():
+    assert serve.request_bytes(256, 128, 2, 3, 1000, grid_bytes=8) == 256 * 128 * 2 * 3 * 12 + 8 * 2 * 3 + 64 * 1000
+    r = _req(0, 512, 4, 100)
+    s = serve.Server(1 << 40, workers=0, grid_precision="c64")
+    s.submit(r)
+    assert r.nbytes == serve.request_bytes(512, 512, 1, 4, 400, grid_bytes=8)
+    with pytest.raises(ValueError):
+        serve.Server(1 << 40, workers=0, grid_precision="c32")
+
+
 def test_admission_fifo_within_budget():
     a, b, c = _req(0, 256, 1, 10), _req(1, 256, 1, 10), _req(2, 256, 1, 10)
     s = _server(2 * a.nbytes + 10)
