@@ -49,6 +49,12 @@ constexpr int TG_T = 32;             // tile side in cells
 constexpr int TG_TT = TG_T * TG_T;
 constexpr int TG_THREADS = 1024;     // 16 waves, one 8x8 region each
 constexpr int TG_REGIONS = 16;
+// A wave's region: TG_RW x TG_RH cells (64 lanes).  16 x 4 regions read each
+// support-box row of kernel samples as a longer contiguous run, but a box
+// touches more of them: measured slower than 8 x 8 (config-2 gridder 147.5 ms
+// against 141).
+constexpr int TG_RW = 8, TG_RH = 8;
+static_assert(TG_RW * TG_RH == 64 && (TG_T / TG_RW) * (TG_T / TG_RH) == TG_REGIONS, "region shape");
 constexpr int TG_BATCH = 256;        // records staged per round
 static_assert(TG_T == 32 && TG_THREADS == 64 * TG_REGIONS && TG_BATCH <= 256, "region layout");
 constexpr int TG_CHUNK = 2048;       // default (record, tile) pairs per work unit
@@ -502,7 +508,8 @@ __global__ __launch_bounds__(TG_THREADS, PREC == MTI_TABLE_F32 ? 8 : 1) void k_g
     __shared__ int s_cnt[TG_REGIONS];
     static_assert(TG_BATCH * 2 * P * sizeof(double) < 65536, "visibility offsets fit 16 bits");
     const int tid = threadIdx.x, lane = tid & 63, wave = wave_uniform(tid >> 6);
-    const int rx = (wave & 3) * 8 + (lane & 7), ry = (wave >> 2) * 8 + (lane >> 3);  // cell in the tile
+    constexpr int RX = TG_T / TG_RW;  // regions along a tile row
+    const int rx = (wave % RX) * TG_RW + (lane % TG_RW), ry = (wave / RX) * TG_RH + (lane / TG_RW);  // cell in the tile
     const uint32_t NU = *a.n_units;
     // XCD-aware order: runs of 2^lg consecutive units (neighbouring tiles of
     // one channel, whose records share w-planes) are dealt to the XCDs in turn
@@ -555,9 +562,9 @@ __global__ __launch_bounds__(TG_THREADS, PREC == MTI_TABLE_F32 ? 8 : 1) void k_g
                         e.s_vo = S | ((rr * 2 * P * (int)sizeof(double)) << 16);
                         const int ix0 = max(cx, x0) - x0, ix1 = min(cx + S, x0 + TG_T) - x0;
                         const int iy0 = max(cy, y0) - y0, iy1 = min(cy + S, y0 + TG_T) - y0;
-                        for (int gyr = iy0 >> 3; gyr <= (iy1 - 1) >> 3; gyr++)
-                            for (int gxr = ix0 >> 3; gxr <= (ix1 - 1) >> 3; gxr++) {
-                                const int reg = gyr * 4 + gxr;
+                        for (int gyr = iy0 / TG_RH; gyr <= (iy1 - 1) / TG_RH; gyr++)
+                            for (int gxr = ix0 / TG_RW; gxr <= (ix1 - 1) / TG_RW; gxr++) {
+                                const int reg = gyr * (TG_T / TG_RW) + gxr;
                                 const int pos = atomicAdd(&s_cnt[reg], 1);
                                 s_list[reg][pos] = e;
                             }
