@@ -75,7 +75,7 @@ def measured_traffic(cfg):
     if cfg != "c2":
         return None
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        "k_grid_tiles_counters_config2.md")
+                        "k_grid_tiles_counters_fullpol_cube.md")
     try:
         with open(path) as f:
             for line in f:
@@ -309,7 +309,7 @@ def graph_replay(ctx, torch, mti, jobs, outputs, max_jobs=64, reps=7):
 
 
 def run_c4(args, torch, mti, jobs):
-    """Config 4: independent snapshot jobs (Stokes I) imaged one at a time
+    """The snapshot mix: independent jobs (Stokes I) imaged one at a time
     (plain: per job kernel tables, row sort, gridding, finalize) and batched
     (kernel-table cache, jobs admitted in waves by an HBM budget, one
     mti_grid_jobs + one mti_finalize_jobs call per wave, waves on two workers
@@ -539,7 +539,7 @@ def _serve_feed(srv, pool, picks, times, deadline_s):
 
 
 def run_c5(args, torch, mti, pool):
-    """Config 5: Poisson streams of imaging requests drawn from the snapshot
+    """Serving: Poisson streams of imaging requests drawn from the snapshot
     mix, served by continuous batching (two workers, HBM-budget admission,
     device-resident kernel-table cache) and by the plain one-job-at-a-time
     path on the same arrivals.  Both servers are long-lived (warm contexts,
@@ -588,27 +588,27 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=["c2", "c1", "c4", "c5"])
-    ap.add_argument("--rates", default="50,200,800", help="config 5: arrival rates (req/s)")
-    ap.add_argument("--serve-seconds", type=float, default=10.0, help="config 5: arrival window per rate")
+    ap.add_argument("--rates", default="50,200,800", help="serving: arrival rates (req/s)")
+    ap.add_argument("--serve-seconds", type=float, default=10.0, help="serving: arrival window per rate")
     ap.add_argument("--serve-budget-gb", type=float, default=160.0,
-                    help="config 5: HBM admission budget (the rest holds the inputs, the library's "
+                    help="serving: HBM admission budget (the rest holds the inputs, the library's "
                          "scratch and allocator slack)")
-    ap.add_argument("--jobs", type=int, default=512, help="config 4: number of snapshot jobs")
-    ap.add_argument("--c4-budget-gb", type=float, default=64.0, help="config 4: HBM budget of a batched wave")
+    ap.add_argument("--jobs", type=int, default=512, help="snapshot mix: number of snapshot jobs")
+    ap.add_argument("--c4-budget-gb", type=float, default=64.0, help="snapshot mix: HBM budget of a batched wave")
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-autotune", action="store_true", help="config 4: keep the default tuning")
-    ap.add_argument("--no-e2e", action="store_true", help="configs 1/2: skip the pinned-host end-to-end run")
+    ap.add_argument("--no-autotune", action="store_true", help="snapshot mix: keep the default tuning")
+    ap.add_argument("--no-e2e", action="store_true", help="cube runs: skip the pinned-host end-to-end run")
     ap.add_argument("--complete-grids", action="store_true",
                     help="zero-fill and transform every grid tile (default: sparse grids with a tile mask)")
-    ap.add_argument("--xcd-chunk", type=int, default=None, help="configs 1/2: gridder units per XCD run")
-    ap.add_argument("--unit-pairs", type=int, default=None, help="configs 1/2: gridder work-unit size")
+    ap.add_argument("--xcd-chunk", type=int, default=None, help="cube runs: gridder units per XCD run")
+    ap.add_argument("--unit-pairs", type=int, default=None, help="cube runs: gridder work-unit size")
     ap.add_argument("--transform-storage", choices=["c128", "c64"], default="c64",
                     help="element type of the packed planes between the finalize's transform passes")
     ap.add_argument("--grid-precision", choices=["c128", "c64"], default="c64",
                     help="element type of the grids the gridder writes")
     ap.add_argument("--no-form-check", action="store_true",
-                    help="configs 1/2: skip imaging the job again on the complex128 path")
+                    help="cube runs: skip imaging the job again on the complex128 path")
     ap.add_argument("--cpu-rows", type=int, default=60000)
     ap.add_argument("--cpu-threads", type=int, default=16, help="threads of the CPU baseline (one channel each)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -632,7 +632,7 @@ def main():
             "steps": len(r["rates"]), "warmup": 1, "ms_per_step": None,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
             "data": "synthetic",
-            "config": {"workload": f"config5 serving: Poisson arrivals at {args.rates} req/s for "
+            "config": {"workload": f"serving: Poisson arrivals at {args.rates} req/s for "
                                    f"{args.serve_seconds} s from a pool of {r['pool']} snapshot jobs, "
                                    f"HBM budget {args.serve_budget_gb} GB",
                        "path": "served vs plain"},
@@ -645,7 +645,7 @@ def main():
             "steps": 1, "warmup": 1, "ms_per_step": round(r["batched"]["seconds"] * 1e3, 3),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
             "data": "synthetic",
-            "config": {"workload": f"config4 batched snapshot imaging: {r['jobs']} jobs, Stokes I, "
+            "config": {"workload": f"batched snapshot imaging: {r['jobs']} jobs, Stokes I, "
                                    "10k-5M vis, 256^2-2048^2, 8-64 w-planes, s 4/8",
                        "path": "batched vs plain"},
             "latency_ms": r["batched"]["latency_ms"], "c4": r}), flush=True)
@@ -808,10 +808,10 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": ("config2 full-pol cube: 64 ant / 8 km, 2016 baselines x 1800 steps, 64 ch "
+            "workload": ("full-pol cube: 64 ant / 8 km, 2016 baselines x 1800 steps, 64 ch "
                          "0.9-1.7 GHz, P=4, 4096^2 @ 1.5 arcsec, 256 w-planes, s=8, Briggs 0, IQUV")
             if args.config == "c2" else
-            "config1: 27 ant / 3 km, 351 baselines x 300 steps, 8 ch, P=4, 512^2 @ 2 arcsec, 32 planes, s=4",
+            "small array: 27 ant / 3 km, 351 baselines x 300 steps, 8 ch, P=4, 512^2 @ 2 arcsec, 32 planes, s=4",
             "rows": nrows, "channels": nchan, "gridded_vis": gridded, "gridded_points": points,
             "supports": [int(sup.min()), int(sup.max())], "path": "plain single-job",
             "tuning": ctx.get_tuning(), "grids": "complete" if args.complete_grids else "sparse (tile mask)",

@@ -51,8 +51,8 @@ constexpr int TG_THREADS = 1024;     // 16 waves, one 8x8 region each
 constexpr int TG_REGIONS = 16;
 // A wave's region: TG_RW x TG_RH cells (64 lanes).  16 x 4 regions read each
 // support-box row of kernel samples as a longer contiguous run, but a box
-// touches more of them: measured slower than 8 x 8 (config-2 gridder 147.5 ms
-// against 141).
+// touches more of them: measured slower than 8 x 8 (the 4096^2 full-polarisation
+// cube's gridder 147.5 ms against 141).
 constexpr int TG_RW = 8, TG_RH = 8;
 static_assert(TG_RW * TG_RH == 64 && (TG_T / TG_RW) * (TG_T / TG_RH) == TG_REGIONS, "region shape");
 constexpr int TG_BATCH = 256;        // records staged per round

@@ -133,7 +133,7 @@ def make_observation(nant=27, radius_m=3000.0, lat_deg=-30.7, dec_deg=-30.0,
                      ha_hours=(-2.0, 2.0), ntime=300, freqs_hz=None, npol=4,
                      nsrc=20, fov_rad=None, pol_frac=0.0, noise=0.01, flag_frac=0.05,
                      seed=1, with_autocorr=True, predict=True):
-    """Config-1 defaults: 27 antennas in a 3 km disk, 300 steps over +-2 h at
+    """Small-array defaults: 27 antennas in a 3 km disk, 300 steps over +-2 h at
     dec -30, 8 channels over 1.4-1.5 GHz, 20 point sources, sigma 0.01 Jy,
     5% flags, weights 1."""
     if freqs_hz is None:
@@ -158,14 +158,14 @@ def make_observation(nant=27, radius_m=3000.0, lat_deg=-30.7, dec_deg=-30.0,
                        vis=vis, flags=flags, weight=weight, nant=nant, src=src, npol=npol)
 
 
-def config1(seed=1, **kw):
+def small_array_obs(seed=1, **kw):
     """27 antennas / 3 km / 300 steps / +-2 h / dec -30 / 8 ch 1.4-1.5 GHz /
     P=4 / 512^2 at 2 arcsec, 32 planes, s=4."""
     kw.setdefault("fov_rad", 512 * 2 * ARCSEC)
     return make_observation(seed=seed, **kw)
 
 
-def config2_geometry(seed=2):
+def large_array_geometry(seed=2):
     """64 antennas / 8 km / 1800 steps / +-4 h / dec -45 / 64 ch 0.9-1.7 GHz
     (uvw, antennas and frequencies only; visibilities are predicted on the GPU)."""
     return make_observation(nant=64, radius_m=8000.0, dec_deg=-45.0, ha_hours=(-4.0, 4.0),

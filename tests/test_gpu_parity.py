@@ -13,9 +13,9 @@ torch = pytest.importorskip("torch")
 
 @pytest.fixture(scope="module")
 def c1(orc):
-    """Config-1 observation (27 ant, 3 km, 300 steps, 8 ch, P=4), seed 1."""
+    """Small-array observation (27 ant, 3 km, 300 steps, 8 ch, P=4), seed 1."""
     from mtimager_amd import synth
-    return synth.config1(seed=1)
+    return synth.small_array_obs(seed=1)
 
 
 @pytest.fixture(scope="module")
@@ -48,7 +48,7 @@ def test_sort_rows_random_and_empty(orc, mti, ctx):
 
 # ---- a8-a14 ----------------------------------------------------------------
 @pytest.mark.parametrize("cfg", [
-    dict(nx=512, cell=2.0, W=32, s=4, mode="wproj"),      # config 1 geometry
+    dict(nx=512, cell=2.0, W=32, s=4, mode="wproj"),      # the small-array geometry
     dict(nx=256, cell=60.0, W=64, s=4, mode="wproj"),     # wide field, growing supports
     dict(nx=128, cell=120.0, W=16, s=5, mode="wproj"),    # odd sampling
     dict(nx=256, cell=2.0, W=1, s=4, mode="normal"),      # taper only
@@ -95,7 +95,7 @@ def test_binning_known_answers(mti, ctx, s, cases):
     np.testing.assert_array_equal(got, exp[:, :8])
 
 
-def test_binning_bit_exact_config1(orc, mti, ctx, c1, c1_params):
+def test_binning_bit_exact_small_array(orc, mti, ctx, c1, c1_params):
     dev = H.Dev(c1, torch)
     idx = H.gpu_sort(ctx, dev, c1.nant, torch)
     K = ctx.build_kernels(c1_params)
@@ -143,7 +143,7 @@ GRID_RMS_TOL = 1e-5
 
 @pytest.mark.parametrize("psf", [False, True])
 @pytest.mark.parametrize("precision,layout", [("c64", "vup"), ("c128", "pvu")])
-def test_config1_dirty_image_and_psf(orc, mti, ctx, c1, c1_params, psf, precision, layout):
+def test_small_array_dirty_image_and_psf(orc, mti, ctx, c1, c1_params, psf, precision, layout):
     dev = H.Dev(c1, torch)
     idx = H.gpu_sort(ctx, dev, c1.nant, torch)
     rows = idx.cpu().numpy()
@@ -167,8 +167,8 @@ def test_config1_dirty_image_and_psf(orc, mti, ctx, c1, c1_params, psf, precisio
     assert g["stats"].gridded_records == ng
 
 
-# ---- config 3: precision forms --------------------------------------------
-# Kernel tables in f16 / bf16 on config-1 inputs (visibilities complex64,
+# ---- precision forms of the kernel tables --------------------------------------------
+# Kernel tables in f16 / bf16 on the small-array inputs (visibilities complex64,
 # FP64 accumulation and transform, FP64 binning and w-chirp phase); every
 # Stokes image and the PSF against the FP64 oracle with FP64 tables.
 # Rounding a table sample (f16: 2^-11, bf16: 2^-8 relative) is noise that
@@ -185,7 +185,7 @@ def _inner(img):
 
 
 @pytest.mark.parametrize("table", ["f16", "bf16"])
-def test_precision_forms_config1(orc, mti, ctx, c1, c1_params, table):
+def test_precision_forms_small_array(orc, mti, ctx, c1, c1_params, table):
     dev = H.Dev(c1, torch)
     idx = H.gpu_sort(ctx, dev, c1.nant, torch)
     rows = idx.cpu().numpy()
@@ -329,7 +329,7 @@ def test_reference_chunk_layout(orc, mti, ctx, c1, c1_params):
 @pytest.mark.parametrize("npol", [1, 2])
 def test_fewer_correlations(orc, mti, ctx, npol):
     from mtimager_amd import synth
-    ob = synth.config1(seed=2, npol=npol, ntime=100, freqs_hz=[1.45e9])
+    ob = synth.small_array_obs(seed=2, npol=npol, ntime=100, freqs_hz=[1.45e9])
     p = mti.make_params(512, 512, 2.0, 2.0, "wproj", 32, 4)
     dev = H.Dev(ob, torch)
     idx = H.gpu_sort(ctx, dev, ob.nant, torch)
@@ -417,7 +417,7 @@ def test_fft2d_inverse_matches_numpy(mti, ctx, nx, ny, nb):
 
 @pytest.mark.parametrize("nx,ny,npol,poltype", [
     (256, 128, 1, "linear"), (256, 128, 2, "circular"), (256, 128, 4, "linear"), (256, 128, 4, "circular"),
-    # block-transform path (radix-16 passes): rectangular, P=1/2/4, the config-2 size
+    # block-transform path (radix-16 passes): rectangular, P=1/2/4, the 4096^2 cube size
     (512, 256, 4, "linear"), (1024, 1024, 2, "linear"), (2048, 512, 1, "linear"), (4096, 4096, 4, "circular")])
 def test_finalize_matches_oracle(orc, mti, ctx, nx, ny, npol, poltype):
     p = mti.make_params(nx, ny, 3.0, 2.0, "wproj", 8, 8)
@@ -441,7 +441,7 @@ def test_invalid_parameters_raise(mti, ctx):
         ctx.fft2d_inverse(torch.zeros(10, dtype=torch.complex64, device="cuda"), 5, 2)
 
 
-# ---- batched jobs (config 4 path) ------------------------------------------
+# ---- batched jobs (the snapshot-mix path) ------------------------------------------
 BATCH_JOBS = [  # nx, cell arcsec, w-planes, sampling, antennas, steps, channels, seed
     (256, 8.0, 8, 4, 12, 40, 3, 11),
     (512, 2.0, 16, 8, 20, 25, 2, 12),

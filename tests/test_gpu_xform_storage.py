@@ -124,13 +124,13 @@ def test_batched_jobs_match_complex128_storage(mti, ctx, ctx64):
 
 
 @pytest.mark.parametrize("psf", [False, True])
-def test_config1_dirty_image_and_psf(orc, mti, ctx64, psf):
-    """Config 1 (512^2, 32 planes, s=4, natural weighting), complex128 grids
+def test_small_array_dirty_image_and_psf(orc, mti, ctx64, psf):
+    """The small array (512^2, 32 planes, s=4, natural weighting), complex128 grids
     finalized through complex64 planes: every Stokes image and the PSF against
     the FP64 oracle.  The taper correction amplifies edge pixels by up to
     1/psi(1)^2, so this is where storage rounding shows most."""
     from mtimager_amd import synth
-    c1 = synth.config1(seed=1)
+    c1 = synth.small_array_obs(seed=1)
     p = mti.make_params(512, 512, 2.0, 2.0, "wproj", 32, 4)
     dev = H.Dev(c1, torch)
     idx = H.gpu_sort(ctx64, dev, c1.nant, torch)
@@ -140,7 +140,7 @@ def test_config1_dirty_image_and_psf(orc, mti, ctx64, psf):
     for ci in range(2):
         r = H.oracle_image(orc, OK, H.oracle_params(orc, p), c1, ci, idx.cpu().numpy(), psf=psf)
         e = H.image_errors(g["image"][ci][0], r["image"][0])
-        print(f"config 1 chan {ci} psf={psf}, complex64 storage, stokes I: {e}")
+        print(f"small array chan {ci} psf={psf}, complex64 storage, stokes I: {e}")
         assert e["rel_rms"] < IMAGE_RMS_TOL
         ref_i = np.sqrt(np.mean(r["image"][0] ** 2))
         for q in range(1, 4):
