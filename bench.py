@@ -63,6 +63,8 @@ def make_workload(torch, cfg, seed):
              weight=torch.ones((nrows, P), dtype=torch.float32, device="cuda"),
              a1=torch.tensor(a1, device="cuda"), a2=torch.tensor(a2, device="cuda"),
              nant=nant, nrows=nrows, nchan=nchan, P=P, nx=nx, cell=cell, W=W, s=s,
+             # the cube is imaged with Briggs robust 0 weighting, the small array with natural
+             weighting="briggs" if cfg == "c2" else "natural",
              host=dict(uvw=uvw, a1=a1, a2=a2, inv_lambda=inv_lambda, src=src))
     torch.cuda.synchronize()
     return d
@@ -158,7 +160,7 @@ def end_to_end(torch, mti, ctx, d, p, grids, norm, image, ref_image, group=8, tm
         b = gi % 2
         v = mti.vis_chan_major(small_d["uvw"], small_d["invl"][c0:c0 + n], vis_d[b][:n], flg_d[b][:n],
                                small_d["weight"], idx[:kept])
-        ctx.grid(K, p, v, grids[c0:c0 + n], norm[c0:c0 + n], chan0=0, nchan=n, weighting="briggs", robust=0.0,
+        ctx.grid(K, p, v, grids[c0:c0 + n], norm[c0:c0 + n], chan0=0, nchan=n, weighting=d["weighting"], robust=0.0,
                  layout="pvu", stats=False, tile_mask=None if tmask is None else tmask[c0:c0 + n])
         done[b] = torch.cuda.Event()
         done[b].record(gs)
@@ -185,7 +187,7 @@ def cpu_model():
 
 
 def cpu_baseline(d, params_args, budget_rows, threads):
-    """FP64 oracle on a bounded sample: per channel, Briggs weights + gridding
+    """FP64 oracle on a bounded sample: per channel, weights (the workload's) + gridding
     of `budget_rows` baseline-major rows and one finalize.  Timed on one
     thread, then on `threads` threads with one channel each (the oracle is
     reentrant C; ctypes releases the interpreter lock), and extrapolated to
@@ -210,7 +212,7 @@ def cpu_baseline(d, params_args, budget_rows, threads):
     def one(chan):
         il = np.array([h["inv_lambda"][chan]])
         t0 = time.perf_counter()
-        wts = oracle.weights(K, uvw, il, flags, weight, 0, "briggs", 0.0)
+        wts = None if d["weighting"] == "natural" else oracle.weights(K, uvw, il, flags, weight, 0, d["weighting"], 0.0)
         g, nrm, ng = oracle.grid(K, uvw, il, vis, flags, weight, 0, wts=wts)
         t1 = time.perf_counter()
         oracle.finalize(p, g, nrm)
@@ -670,7 +672,7 @@ def main():
         kept = ctx.sort_rows(d["a1"], d["a2"], d["nant"], idx)
         K = ctx.build_kernels(p)
         v = mti.vis_ms(d["uvw"], d["invl"], d["vis"], d["flags"], d["weight"], idx[:kept])
-        st = ctx.grid(K, p, v, grids, norm, chan0=0, nchan=nchan, weighting="briggs", robust=0.0,
+        st = ctx.grid(K, p, v, grids, norm, chan0=0, nchan=nchan, weighting=d["weighting"], robust=0.0,
                       layout="pvu", stats=stats, tile_mask=tmask)
         ctx.finalize(p, P, grids, norm, image, nchan=nchan, layout="pvu", tile_mask=tmask)
         K.free()
@@ -749,7 +751,7 @@ def main():
         kept = ctx.sort_rows(d["a1"], d["a2"], d["nant"], idx)
         K = ctx.build_kernels(p)
         v = mti.vis_ms(d["uvw"], d["invl"], d["vis"], d["flags"], d["weight"], idx[:kept])
-        ctx.grid(K, p, v, g128, n128, chan0=0, nchan=nchan, weighting="briggs", robust=0.0, layout="pvu",
+        ctx.grid(K, p, v, g128, n128, chan0=0, nchan=nchan, weighting=d["weighting"], robust=0.0, layout="pvu",
                  stats=False, tile_mask=m128)
         ctx.finalize(p, P, g128, n128, ref, nchan=nchan, layout="pvu", tile_mask=m128)
         torch.cuda.synchronize()
@@ -785,7 +787,7 @@ def main():
         cpu = dict(value=round(gridded / t_jobN / 1e6, 4), unit="Mvis/s", cores=thr, kind="port",
                    single_thread_value=round(gridded / t_job1 / 1e6, 4), cpu_model=cpu_model(),
                    host_cpus=os.cpu_count(),
-                   sample=(f"FP64 oracle: per channel, Briggs weights + gridding of {cb['sample_rows']} "
+                   sample=(f"FP64 oracle: per channel, {d['weighting']} weights + gridding of {cb['sample_rows']} "
                            f"baseline-major rows ({cb['sample_gridded']} gridded) and one {nx}^2 x {P} "
                            f"finalize; 1 thread: {cb['per_record_s'] * 1e6:.2f} us/record + "
                            f"{cb['finalize_s']:.1f} s finalize; {thr} threads (one channel each, "
@@ -811,7 +813,8 @@ def main():
             "workload": ("full-pol cube: 64 ant / 8 km, 2016 baselines x 1800 steps, 64 ch "
                          "0.9-1.7 GHz, P=4, 4096^2 @ 1.5 arcsec, 256 w-planes, s=8, Briggs 0, IQUV")
             if args.config == "c2" else
-            "small array: 27 ant / 3 km, 351 baselines x 300 steps, 8 ch, P=4, 512^2 @ 2 arcsec, 32 planes, s=4",
+            "small array: 27 ant / 3 km, 351 baselines x 300 steps, 8 ch, P=4, 512^2 @ 2 arcsec, 32 planes, s=4, "
+            "natural weighting",
             "rows": nrows, "channels": nchan, "gridded_vis": gridded, "gridded_points": points,
             "supports": [int(sup.min()), int(sup.max())], "path": "plain single-job",
             "tuning": ctx.get_tuning(), "grids": "complete" if args.complete_grids else "sparse (tile mask)",
