@@ -123,3 +123,22 @@ def test_sparse_grids_are_refused_where_they_cannot_apply(mti, ctx):
         g = torch.zeros((1, 256, 256, 1), dtype=dt, device="cuda")
         with pytest.raises(mti.MTIError, match="accumulate"):
             ctx.grid(K, p, v, g, n, nchan=1, accumulate=True, precision=prec, tile_mask=mask)
+
+
+def test_batched_sparse_jobs_refuse_accumulate(mti, ctx):
+    """mti_grid_jobs refuses a job tile mask with accumulate (the unmarked
+    tiles would keep whatever they held), before any work."""
+    ob = _obs(256, 8.0, 1, 1, 77, nant=8, ntime=10)
+    d = H.Dev(ob, torch)
+    p = mti.make_params(256, 256, 8.0, 8.0, "wproj", 8, 4)
+    K = ctx.build_kernels(p)
+    g = torch.full((1, 256, 256, 1), 3.0 + 0j, dtype=torch.complex64, device="cuda")
+    n = torch.zeros((1, 1), dtype=torch.float64, device="cuda")
+    im = torch.zeros((1, 1, 256, 256), dtype=torch.float32, device="cuda")
+    mask = torch.empty(mti.tile_mask_shape(p, 1), dtype=torch.uint8, device="cuda")
+    v = mti.vis_ms(d.uvw, d.invl, d.vis, d.flags, d.weight)
+    job = ctx.make_job(K, p, v, g, n, im, 0, 1, ant1=d.ant1, ant2=d.ant2, nant=ob.nant, tile_mask=mask)
+    with pytest.raises(mti.MTIError, match="accumulate"):
+        ctx.grid_jobs([job], accumulate=True)
+    torch.cuda.synchronize()
+    assert float((g - 3.0).abs().max()) == 0.0
