@@ -270,7 +270,9 @@ mti_status mti_finalize_jobs(mti_ctx *ctx, int32_t njobs, const mti_job *jobs,
 
 /* ---- 2-D inverse FFT (IFFT2D, gafw/StandardOperators/IFFT2D.cpp:86-111) --
  * Unnormalised C2C inverse of nbatch planes complex64 [nbatch][ny][nx], in
- * place.  nx, ny powers of two in [2, 8192]. */
+ * place, any nx, ny in [1, 32768] (the reference plans cuFFT for any
+ * rows x cols).  FP64 arithmetic: mixed-radix passes (radices 2-16, 3, 5, 7,
+ * 11, 13), Bluestein for a length with a larger prime factor. */
 mti_status mti_fft2d_inverse(mti_ctx *ctx, float *data, int32_t nx, int32_t ny,
                              int32_t nbatch, void *stream);
 
@@ -282,7 +284,10 @@ mti_status mti_fft2d_inverse(mti_ctx *ctx, float *data, int32_t nx, int32_t ny,
  *           complex128 (grid_precision); read only
  *   norm  : f64 [nchan][npol]
  *   image : f32 [nchan][npol][ny][nx]  (Stokes I,Q,U,V / I,Q / I,V / I)
- * nx: power of two in [2,4096]; ny: power of two in [2,8192]. */
+ * Any image size that validates (2..32768 per side): powers of two with
+ * 16 <= nx <= 4096 and ny >= 16 take LDS-blocked passes, every other size
+ * mixed-radix HBM passes (Bluestein for a prime factor above 13); odd sizes
+ * shift as the reference's OriginToCorner/OriginToCentre (op/common.hcu:105-136). */
 mti_status mti_finalize(mti_ctx *ctx, const mti_image_params *p, int32_t npol,
                         int32_t poltype, int32_t nchan, int32_t grid_layout,
                         int32_t grid_precision, const void *grids,

@@ -35,6 +35,8 @@ extern "C" void mti_destroy(mti_ctx *ctx)
         if (s.ptr) (void)hipFree(s.ptr);
     for (auto &t : ctx->twiddles)
         if (t.second) (void)hipFree(t.second);
+    for (auto &t : ctx->gen_tables)
+        if (t.ptr) (void)hipFree(t.ptr);
     delete ctx;
 }
 
@@ -98,6 +100,7 @@ extern "C" mti_status mti_context_device_bytes(const mti_ctx *ctx, size_t *bytes
     size_t t = 0;
     for (const auto &s : ctx->scratch) t += s.bytes;
     for (const auto &tw : ctx->twiddles) t += (size_t)tw.first * sizeof(double2);
+    for (const auto &gt : ctx->gen_tables) t += gt.bytes;
     *bytes = t;
     return MTI_OK;
 }

@@ -22,234 +22,24 @@
 //      in place) and B2 (length N2 over N2 contiguous rows); the last pass is
 //      fused with fftshift + m flip + taper and sinc correction and writes
 //      the real and imaginary parts as two f32 images.
-// Images below 16 pixels on a side take the unpacked per-correlation passes
-// (k_fft_rows, k_fft_cols1, k_fft_cols2).  Twiddles come from an
-// FP64-computed table exp(+2 pi i t / NT).
+// Other sizes (a side that is not a power of two, rows above 4096 or a side
+// below 16) take the general path: the same packed planes built in one HBM
+// pass, each axis transformed by mixed-radix Stockham passes through HBM
+// (radices 16, 8, 4, 2, 3, 5, 7, 11, 13) or, for a length with a larger prime
+// factor, by Bluestein's chirp-z algorithm over a power-of-two length, and a
+// last pass that writes the images.  Twiddles come from FP64-computed tables
+// exp(+2 pi i t / L).
 #include "mti_internal.hpp"
 
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <mutex>
+#include <string>
 #include <type_traits>
 
 namespace mti {
 
-constexpr int FFT_THREADS = 256;
-constexpr int COLS = 16;  // columns per tile in the column passes
-
-template <typename C>
-__device__ __forceinline__ C cmk(decltype(C::x) x, decltype(C::x) y)
-{
-    C r;
-    r.x = x;
-    r.y = y;
-    return r;
-}
-
-template <typename C>
-__device__ __forceinline__ C cmul(C a, C b)
-{
-    return cmk<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-
-template <typename C>
-__device__ __forceinline__ C tw_load(const double2 *tw, int i)
-{
-    double2 t = tw[i];
-    return cmk<C>((decltype(C::x))t.x, (decltype(C::x))t.y);
-}
-
-// Batch of nb inverse FFTs of length n (power of two) stored contiguously in
-// LDS buffer a (stride n); b is a scratch buffer of the same size.  Returns
-// the buffer holding the result.  tw[t] = exp(+2 pi i t / NT), tw_mul = NT/n.
-// C is float2 or double2 (the arithmetic precision).
-template <typename C>
-__device__ C *lds_ifft(C *a, C *b, int n, int logn, int nb, const double2 *tw, int tw_mul)
-{
-    int Ns = 1;
-    while (Ns < n) {
-        const int R = (n / Ns >= 4) ? 4 : 2;
-        const int per = n / R;
-        const int logper = logn - (R == 4 ? 2 : 1);
-        const int twstep = (n / (Ns * R)) * tw_mul;
-        for (int idx = threadIdx.x; idx < per * nb; idx += blockDim.x) {
-            const int bt = idx >> logper;
-            const int j = idx & (per - 1);
-            const C *src = a + bt * n;
-            C *dst = b + bt * n;
-            const int k = j & (Ns - 1);
-            const int idxD = (j - k) * R + k;
-            if (R == 4) {
-                C v0 = src[j], v1 = src[j + per], v2 = src[j + 2 * per], v3 = src[j + 3 * per];
-                if (Ns > 1) {
-                    v1 = cmul(v1, tw_load<C>(tw, k * twstep));
-                    v2 = cmul(v2, tw_load<C>(tw, 2 * k * twstep));
-                    v3 = cmul(v3, tw_load<C>(tw, 3 * k * twstep));
-                }
-                const C s02 = cmk<C>(v0.x + v2.x, v0.y + v2.y);
-                const C d02 = cmk<C>(v0.x - v2.x, v0.y - v2.y);
-                const C s13 = cmk<C>(v1.x + v3.x, v1.y + v3.y);
-                const C d13 = cmk<C>(v1.x - v3.x, v1.y - v3.y);
-                // i * d13 = (-d13.y, d13.x)
-                dst[idxD] = cmk<C>(s02.x + s13.x, s02.y + s13.y);
-                dst[idxD + Ns] = cmk<C>(d02.x - d13.y, d02.y + d13.x);
-                dst[idxD + 2 * Ns] = cmk<C>(s02.x - s13.x, s02.y - s13.y);
-                dst[idxD + 3 * Ns] = cmk<C>(d02.x + d13.y, d02.y - d13.x);
-            } else {
-                C v0 = src[j], v1 = src[j + per];
-                if (Ns > 1) v1 = cmul(v1, tw_load<C>(tw, k * twstep));
-                dst[idxD] = cmk<C>(v0.x + v1.x, v0.y + v1.y);
-                dst[idxD + Ns] = cmk<C>(v0.x - v1.x, v0.y - v1.y);
-            }
-        }
-        __syncthreads();
-        C *t = a;
-        a = b;
-        b = t;
-        Ns *= R;
-    }
-    return a;
-}
-
-// Pass A.  MODE 0: from the gridder layout [c][gy][gx][P] (complex64) with
-// the PreIFFTUVCorrection reorder (ix = (gx - nx/2) mod nx,
-// iy = (gy - ny/2) mod ny) into X[c][p][iy][kx] (element type C).
-// MODE 1: plain in-place rows of [c][y][x] (C = float2 only).
-template <int MODE, typename C, typename TI>
-__global__ __launch_bounds__(FFT_THREADS) void k_fft_rows(const TI *in, C *out, int nx, int lognx,
-                                                         int ny, int P, const double2 *tw, int tw_mul,
-                                                         int64_t cs, int64_t ps)
-{
-    extern __shared__ char smem[];
-    C *a = (C *)smem, *b = a + nx;
-    const int y = blockIdx.x, p = blockIdx.y, c = blockIdx.z;
-    const int hx = nx >> 1;
-    if (MODE == 0) {
-        // grid element (c, y, gx, p) at c*nx*ny*P + (y*nx + gx)*cs + p*ps
-        const TI *src = in + (size_t)c * nx * ny * P + (size_t)y * nx * cs + (size_t)p * ps;
-        for (int ix = threadIdx.x; ix < nx; ix += blockDim.x) {
-            const int gx = (ix + hx) & (nx - 1);
-            const TI v = src[(size_t)gx * cs];
-            a[ix] = cmk<C>(v.x, v.y);
-        }
-    } else {
-        const TI *src = in + ((size_t)c * ny + y) * nx;
-        for (int ix = threadIdx.x; ix < nx; ix += blockDim.x) {
-            const TI v = src[ix];
-            a[ix] = cmk<C>(v.x, v.y);
-        }
-    }
-    __syncthreads();
-    C *r = lds_ifft<C>(a, b, nx, lognx, 1, tw, tw_mul);
-    C *dst;
-    if (MODE == 0) {
-        const int iy = (y - (ny >> 1)) & (ny - 1);
-        dst = out + (((size_t)c * P + p) * ny + iy) * nx;
-    } else {
-        dst = out + ((size_t)c * ny + y) * nx;
-    }
-    for (int kx = threadIdx.x; kx < nx; kx += blockDim.x) dst[kx] = r[kx];
-}
-
-// Pass B1: for plane blockIdx.z, columns [x0, x0+cols), index n2:
-// rows N2*n1 + n2 -> length-N1 FFT -> * exp(+2 pi i n2 k1 / ny) -> rows N2*k1 + n2.
-template <typename C>
-__global__ __launch_bounds__(FFT_THREADS) void k_fft_cols1(C *X, int nx, int ny, int N1, int logN1, int N2,
-                                                          const double2 *tw, int NT)
-{
-    extern __shared__ char smem[];
-    const int cols = min(COLS, nx);
-    C *a = (C *)smem, *b = a + cols * N1;
-    const int x0 = blockIdx.x * cols, n2 = blockIdx.y;
-    C *plane = X + (size_t)blockIdx.z * nx * ny;
-    for (int i = threadIdx.x; i < N1 * cols; i += blockDim.x) {
-        const int n1 = i / cols, cx = i - n1 * cols;
-        a[cx * N1 + n1] = plane[((size_t)N2 * n1 + n2) * nx + x0 + cx];
-    }
-    __syncthreads();
-    C *r = lds_ifft<C>(a, b, N1, logN1, cols, tw, NT / N1);
-    const int mul = NT / ny;
-    for (int i = threadIdx.x; i < N1 * cols; i += blockDim.x) {
-        const int k1 = i / cols, cx = i - k1 * cols;
-        C v = r[cx * N1 + k1];
-        const int t = ((n2 * k1) & (ny - 1)) * mul;
-        v = cmul(v, tw_load<C>(tw, t));
-        plane[((size_t)N2 * k1 + n2) * nx + x0 + cx] = v;
-    }
-}
-
-// Pass B2 (+ finalize): for channel c, index k1, columns [x0, x0+cols):
-// rows N2*k1 + n2 -> length-N2 FFT -> ky = k1 + N1*k2.  MODE 0 writes the
-// finalized image (fftshift, normalise, Stokes, m flip, corrections);
-// MODE 1 writes the raw transform to out[c][ky][kx] (complex64).
-template <int MODE, int P, typename C>
-__global__ __launch_bounds__(FFT_THREADS) void k_fft_cols2(const C *X, int nx, int ny, int N1, int N2,
-                                                          int logN2, const double2 *tw, int NT,
-                                                          const double *norm, int poltype,
-                                                          const float *corr_x, const float *corr_y,
-                                                          float *image, float2 *out)
-{
-    extern __shared__ char smem[];
-    const int cols = min(COLS, nx);
-    C *a = (C *)smem, *b = a + P * cols * N2;
-    const int x0 = blockIdx.x * cols, k1 = blockIdx.y, c = blockIdx.z;
-    for (int i = threadIdx.x; i < P * N2 * cols; i += blockDim.x) {
-        const int p = i / (N2 * cols);
-        const int rem = i - p * N2 * cols;
-        const int n2 = rem / cols, cx = rem - n2 * cols;
-        const C *plane = X + ((size_t)c * P + p) * nx * ny;
-        a[(p * cols + cx) * N2 + n2] = plane[((size_t)N2 * k1 + n2) * nx + x0 + cx];
-    }
-    __syncthreads();
-    C *r = lds_ifft<C>(a, b, N2, logN2, P * cols, tw, NT / N2);
-    if (MODE == 1) {
-        for (int i = threadIdx.x; i < N2 * cols; i += blockDim.x) {
-            const int k2 = i / cols, cx = i - k2 * cols;
-            const int ky = k1 + N1 * k2;
-            const C v = r[cx * N2 + k2];
-            out[((size_t)c * ny + ky) * nx + x0 + cx] = make_float2((float)v.x, (float)v.y);
-        }
-        return;
-    }
-    typedef decltype(C::x) R;
-    R inv[P];
-#pragma unroll
-    for (int p = 0; p < P; p++) inv[p] = (R)(1.0 / norm[c * P + p]);
-    for (int i = threadIdx.x; i < N2 * cols; i += blockDim.x) {
-        const int k2 = i / cols, cx = i - k2 * cols;
-        const int ky = k1 + N1 * k2, kx = x0 + cx;
-        const int jy = (ky + (ny >> 1)) & (ny - 1), jx = (kx + (nx >> 1)) & (nx - 1);
-        const int row = ny - 1 - jy;  // m axis flipped for FITS
-        R re[P], im[P];
-#pragma unroll
-        for (int p = 0; p < P; p++) {
-            const C v = r[(p * cols + cx) * N2 + k2];
-            re[p] = v.x * inv[p];
-            im[p] = v.y * inv[p];
-        }
-        R o[P];
-        if (P == 1) {
-            o[0] = re[0];
-        } else if (P == 2) {
-            o[0] = (re[0] + re[1]) * (R)0.5;
-            o[1] = (re[0] - re[1]) * (R)0.5;
-        } else if (poltype == MTI_POL_LINEAR) {
-            o[0] = (re[0] + re[3]) * (R)0.5;
-            o[1] = (re[0] - re[3]) * (R)0.5;
-            o[2] = (re[1] + re[2]) * (R)0.5;
-            o[3] = (im[1] - im[2]) * (R)0.5;
-        } else {
-            o[0] = (re[0] + re[3]) * (R)0.5;
-            o[1] = (re[1] + re[2]) * (R)0.5;
-            o[2] = (im[2] - im[1]) * (R)0.5;
-            o[3] = (re[0] - re[3]) * (R)0.5;
-        }
-        const R cf = (R)corr_x[jx] * (R)corr_y[row];
-        float *img = image + (size_t)c * P * nx * ny;
-#pragma unroll
-        for (int q = 0; q < P; q++) img[((size_t)q * ny + row) * nx + jx] = (float)(o[q] * cf);
-    }
-}
 
 // 1-D correction vectors: sinc(x)/psi(|i-n/2|/(n/2)), x = (i-n/2)/(n s)
 __global__ void k_corr(float *corr, int n, int s)
@@ -305,19 +95,8 @@ static mti_status twiddle_table(mti_ctx *ctx, int NT, const double2 **out, hipSt
     return MTI_OK;
 }
 
-template <typename K>
-static mti_status set_lds(mti_ctx *ctx, K kern, size_t bytes)
-{
-    if (bytes > 160 * 1024)
-        return set_error(ctx, MTI_ERR_UNSUPPORTED, "FFT tile exceeds the 160 KB LDS of a gfx950 CU");
-    if (bytes > 64 * 1024)
-        MTI_HIP(ctx, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         (int)bytes));
-    return MTI_OK;
-}
-
 struct FftGeom {
-    int nx, ny, lognx, N1, N2, logN1, logN2, NT, cols;
+    int nx, ny, lognx, N1, N2, logN1, logN2, NT;
 };
 
 static FftGeom geom(int nx, int ny)
@@ -332,7 +111,6 @@ static FftGeom geom(int nx, int ny)
     g.N1 = 1 << g.logN1;
     g.N2 = 1 << g.logN2;
     g.NT = std::max(nx, ny);
-    g.cols = std::min(COLS, nx);
     return g;
 }
 
@@ -749,7 +527,10 @@ static bool split_ok(const FftGeom &g)
     return g.N1 >= 16 && g.N2 >= 16 && FB_E / g.N1 <= g.nx && FB_E / g.N2 <= g.nx;
 }
 
-static bool pk_ok(const FftGeom &g) { return g.nx >= 16 && g.nx <= 4096 && g.ny >= 16 && (fbc_ok(g) || split_ok(g)); }
+static bool pk_ok(const FftGeom &g)
+{
+    return is_pow2(g.nx) && is_pow2(g.ny) && g.nx >= 16 && g.nx <= 4096 && g.ny >= 16 && (fbc_ok(g) || split_ok(g));
+}
 
 // Whole columns in one pass, or the four-step split: where both apply (ny =
 // 256) the context's tuning picks.
@@ -829,71 +610,6 @@ static mti_status pk_cols(mti_ctx *ctx, const FftGeom &g, const XT *X, int npk, 
     return MTI_OK;
 }
 
-template <int MODE, typename C, typename TI>
-static mti_status rows_pass(mti_ctx *ctx, const FftGeom &g, const TI *in, C *out, int P, int nch,
-                            const double2 *tw, hipStream_t st, int64_t cs = 1, int64_t ps = 1)
-{
-    auto kern = k_fft_rows<MODE, C, TI>;
-    size_t lds = (size_t)2 * g.nx * sizeof(C);
-    mti_status r = set_lds(ctx, kern, lds);
-    if (r != MTI_OK) return r;
-    hipLaunchKernelGGL(kern, dim3((unsigned)g.ny, (unsigned)P, (unsigned)nch), dim3(FFT_THREADS), lds, st, in,
-                       out, g.nx, g.lognx, g.ny, P, tw, g.NT / g.nx, cs, ps);
-    MTI_HIP(ctx, hipGetLastError());
-    return MTI_OK;
-}
-
-template <typename C>
-static mti_status cols1(mti_ctx *ctx, const FftGeom &g, C *X, int planes, const double2 *tw, hipStream_t st)
-{
-    auto kern = k_fft_cols1<C>;
-    size_t lds = (size_t)2 * g.cols * g.N1 * sizeof(C);
-    mti_status r = set_lds(ctx, kern, lds);
-    if (r != MTI_OK) return r;
-    dim3 gr((unsigned)(g.nx / g.cols), (unsigned)g.N2, (unsigned)planes);
-    hipLaunchKernelGGL(kern, gr, dim3(FFT_THREADS), lds, st, X, g.nx, g.ny, g.N1, g.logN1, g.N2, tw, g.NT);
-    MTI_HIP(ctx, hipGetLastError());
-    return MTI_OK;
-}
-
-template <int MODE, int P, typename C>
-static mti_status cols2(mti_ctx *ctx, const FftGeom &g, const C *X, int nch, const double2 *tw,
-                        const double *norm, int poltype, const float *cx, const float *cy, float *image,
-                        float2 *out, hipStream_t st)
-{
-    auto kern = k_fft_cols2<MODE, P, C>;
-    size_t lds = (size_t)2 * P * g.cols * g.N2 * sizeof(C);
-    mti_status r = set_lds(ctx, kern, lds);
-    if (r != MTI_OK) return r;
-    dim3 gr((unsigned)(g.nx / g.cols), (unsigned)g.N1, (unsigned)nch);
-    hipLaunchKernelGGL(kern, gr, dim3(FFT_THREADS), lds, st, X, g.nx, g.ny, g.N1, g.N2, g.logN2, tw, g.NT, norm,
-                       poltype, cx, cy, image, out);
-    MTI_HIP(ctx, hipGetLastError());
-    return MTI_OK;
-}
-
-// Old-path finalize (small images): contiguous channels of one job.
-template <int P, typename TI>
-static mti_status finalize_batch(mti_ctx *ctx, const FftGeom &g, const TI *G, double2 *X, int nb,
-                                 const double2 *tw, const double *nr, int poltype, const float *cx,
-                                 const float *cy, float *img, int layout, hipStream_t st)
-{
-    mti_status r;
-    const int64_t cs = layout == MTI_GRID_VUP ? P : 1;
-    const int64_t ps = layout == MTI_GRID_VUP ? 1 : (int64_t)g.nx * g.ny;
-    {
-        StageTimer tm(ctx, MTI_STAGE_FFT_ROWS, st);
-        r = rows_pass<0, double2, TI>(ctx, g, G, X, P, nb, tw, st, cs, ps);
-        if (r != MTI_OK) return r;
-    }
-    {
-        StageTimer tm(ctx, MTI_STAGE_FFT_COLS, st);
-        r = cols1<double2>(ctx, g, X, nb * P, tw, st);
-        if (r != MTI_OK) return r;
-    }
-    StageTimer tm(ctx, MTI_STAGE_FINALIZE, st);
-    return cols2<0, P, double2>(ctx, g, X, nb, tw, nr, poltype, cx, cy, img, nullptr, st);
-}
 
 // One channel of a block-transform finalize.
 struct ChanRef {
@@ -992,6 +708,480 @@ static int transform_batch(const FftGeom &g, int planes)
     return (int)std::max<size_t>(1, std::min<size_t>((size_t)planes, (size_t(16) << 30) / plane_bytes));
 }
 
+// ---- general sizes: HBM passes, mixed radix and Bluestein ------------------
+//
+// Element (b, i, v) of a batch of transforms along one axis sits at
+// b * bs + i * is + v: rows of a plane are (b = plane row, i = x, V = 1,
+// is = 1, bs = nx); columns are (b = plane, i = y, v = x, V = nx, is = nx,
+// bs = nx * ny), so a column pass reads and writes along image rows.
+// Per pass each thread runs one radix-R butterfly of a Stockham step: inputs
+// i = j + r L / R (r < R), twiddles exp(+2 pi i r k / (Ns R)) with
+// k = j mod Ns (Ns = product of the radices before this pass), outputs
+// (j - k) R + k + r Ns -- the self-sorting form, so the result needs no
+// digit reversal.
+
+// Inverse DFT of length R (any R up to 16; the powers of two take the fixed
+// butterflies above).  tw holds exp(+2 pi i t / L) with R dividing L.
+template <int R>
+__device__ __forceinline__ void idft_any(double2 (&v)[R], const double2 *__restrict__ tw, int L)
+{
+    if constexpr (R == 2 || R == 4 || R == 8 || R == 16) {
+        idft<R>(v);
+    } else {
+        const int st = L / R;
+        double2 w[R];
+        w[0] = make_double2(1.0, 0.0);
+#pragma unroll
+        for (int m = 1; m < R; m++) w[m] = tw[m * st];
+        double2 o[R];
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+            double2 acc = v[0];
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                const double2 t = w[(r * q) % R];
+                acc.x = fma(v[r].x, t.x, fma(-v[r].y, t.y, acc.x));
+                acc.y = fma(v[r].x, t.y, fma(v[r].y, t.x, acc.y));
+            }
+            o[q] = acc;
+        }
+#pragma unroll
+        for (int q = 0; q < R; q++) v[q] = o[q];
+    }
+}
+
+constexpr int GEN_THREADS = 256;
+
+// One Stockham pass of radix R over nb * V transforms of length L.  Every
+// index fits 32 bits: a batch holds at most 2^30 elements per buffer.
+template <int R>
+__global__ __launch_bounds__(GEN_THREADS) void k_gen_pass(const double2 *__restrict__ src, double2 *__restrict__ dst,
+                                                          int L, int Ns, int V, int64_t is, int64_t bs, int nb,
+                                                          const double2 *__restrict__ tw)
+{
+    const int per = L / R;
+    const uint32_t total = (uint32_t)nb * (uint32_t)V * (uint32_t)per;
+    const int step = L / (Ns * R);
+    for (uint32_t t = blockIdx.x * GEN_THREADS + threadIdx.x; t < total; t += gridDim.x * GEN_THREADS) {
+        const uint32_t v = t % (uint32_t)V, rest = t / (uint32_t)V;
+        const uint32_t j = rest % (uint32_t)per, b = rest / (uint32_t)per;
+        const double2 *s = src + (int64_t)b * bs + v;
+        double2 *d = dst + (int64_t)b * bs + v;
+        double2 x[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) x[r] = s[(int64_t)(j + r * per) * is];
+        const int k = (int)(j % (uint32_t)Ns);
+        if (Ns > 1) {
+#pragma unroll
+            for (int r = 1; r < R; r++) x[r] = c2mul(x[r], tw[r * k * step]);
+        }
+        idft_any<R>(x, tw, L);
+        const int64_t base = (int64_t)(j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; r++) d[(base + (int64_t)r * Ns) * is] = x[r];
+    }
+}
+
+// Bluestein, for a length N with a prime factor above 13: with
+// c_n = exp(+i pi n^2 / N), X_k = c_k sum_n (x_n c_n) conj(c_(k-n)), a
+// circular convolution of length M (a power of two >= 2N - 1).  With only
+// inverse transforms I: T = I(a); T_k *= D_k (D_k = B_(-k) / M, B the
+// forward DFT of b_m = conj(c_m), built on the host); T = I(T); the
+// convolution at n is T_((-n) mod M).  Layout of T: (b, m, v) at
+// b * M * V + m * V + v.
+__global__ __launch_bounds__(GEN_THREADS) void k_blue_pre(const double2 *__restrict__ X, double2 *__restrict__ T,
+                                                          int N, int M, int V, int64_t is, int64_t bs, int nb,
+                                                          const double2 *__restrict__ chirp)
+{
+    const uint32_t total = (uint32_t)nb * (uint32_t)M * (uint32_t)V;
+    for (uint32_t t = blockIdx.x * GEN_THREADS + threadIdx.x; t < total; t += gridDim.x * GEN_THREADS) {
+        const uint32_t v = t % (uint32_t)V, rest = t / (uint32_t)V;
+        const uint32_t m = rest % (uint32_t)M, b = rest / (uint32_t)M;
+        double2 a = make_double2(0.0, 0.0);
+        if ((int)m < N) a = c2mul(X[(int64_t)b * bs + (int64_t)m * is + v], chirp[m]);
+        T[t] = a;
+    }
+}
+
+__global__ __launch_bounds__(GEN_THREADS) void k_blue_mid(double2 *__restrict__ T, int M, int V, int nb,
+                                                          const double2 *__restrict__ D)
+{
+    const uint32_t total = (uint32_t)nb * (uint32_t)M * (uint32_t)V;
+    for (uint32_t t = blockIdx.x * GEN_THREADS + threadIdx.x; t < total; t += gridDim.x * GEN_THREADS)
+        T[t] = c2mul(T[t], D[(t / (uint32_t)V) % (uint32_t)M]);
+}
+
+__global__ __launch_bounds__(GEN_THREADS) void k_blue_post(const double2 *__restrict__ T, double2 *__restrict__ X,
+                                                           int N, int M, int V, int64_t is, int64_t bs, int nb,
+                                                           const double2 *__restrict__ chirp)
+{
+    const uint32_t total = (uint32_t)nb * (uint32_t)N * (uint32_t)V;
+    for (uint32_t t = blockIdx.x * GEN_THREADS + threadIdx.x; t < total; t += gridDim.x * GEN_THREADS) {
+        const uint32_t v = t % (uint32_t)V, rest = t / (uint32_t)V;
+        const uint32_t n = rest % (uint32_t)N, b = rest / (uint32_t)N;
+        const uint32_t m = n ? (uint32_t)M - n : 0u;
+        X[(int64_t)b * bs + (int64_t)n * is + v] = c2mul(T[((int64_t)b * M + m) * V + v], chirp[n]);
+    }
+}
+
+// Packed plane Z = H0 + i H1 of pack blockIdx.y at every (y, x) of the
+// transform, from the grids with the PreIFFTUVCorrection reorder (transform
+// index i holds grid cell (i + n/2) mod n on each axis, op/common.hcu:122-136
+// for odd and even n) -- the load of k_rows_pk for any size.
+template <typename TI>
+__global__ __launch_bounds__(GEN_THREADS) void k_gen_pack(const PackDesc *pd, double2 *X, int nx, int ny, int64_t cs)
+{
+    const PackDesc &d = pd[blockIdx.y];
+    const TI *A = (const TI *)d.a, *B = (const TI *)d.b;
+    const double ia = 1.0 / *d.na, ib = B ? 1.0 / *d.nb : 0.0;
+    const double2 ka0 = make_double2(d.ca[0].x * ia, d.ca[0].y * ia), ka1 = make_double2(d.ca[1].x * ia, d.ca[1].y * ia);
+    const double2 kb0 = make_double2(d.cb[0].x * ib, d.cb[0].y * ib), kb1 = make_double2(d.cb[1].x * ib, d.cb[1].y * ib);
+    const int ntx = (nx + 31) / 32, hx = nx / 2, hy = ny / 2;
+    const int64_t n = (int64_t)nx * ny;
+    double2 *dst = X + (int64_t)blockIdx.y * n;
+    auto elem = [&](const TI *P0, const uint8_t *m, int gy, int gx) {
+        if (m && !m[(gy >> 5) * ntx + (gx >> 5)]) return make_double2(0.0, 0.0);
+        const TI v = P0[((int64_t)gy * nx + gx) * cs];
+        return make_double2((double)v.x, (double)v.y);
+    };
+    for (int64_t i = (int64_t)blockIdx.x * GEN_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * GEN_THREADS) {
+        const int y = (int)(i / nx), x = (int)(i - (int64_t)y * nx);
+        const int ym = y ? ny - y : 0, xm = x ? nx - x : 0;
+        int gy = y + hy, gx = x + hx, gym = ym + hy, gxm = xm + hx;
+        if (gy >= ny) gy -= ny;
+        if (gx >= nx) gx -= nx;
+        if (gym >= ny) gym -= ny;
+        if (gxm >= nx) gxm -= nx;
+        const double2 a0 = elem(A, d.ma, gy, gx), a1 = elem(A, d.ma, gym, gxm);
+        double2 b0 = make_double2(0.0, 0.0), b1 = b0;
+        if (B) {
+            b0 = elem(B, d.mb, gy, gx);
+            b1 = elem(B, d.mb, gym, gxm);
+        }
+        const double2 s0 = c2add(c2mul(ka0, a0), c2mul(kb0, b0)), s0m = c2add(c2mul(ka0, a1), c2mul(kb0, b1));
+        const double2 s1 = c2add(c2mul(ka1, a0), c2mul(kb1, b0)), s1m = c2add(c2mul(ka1, a1), c2mul(kb1, b1));
+        const double h0r = 0.5 * (s0.x + s0m.x), h0i = 0.5 * (s0.y - s0m.y);
+        const double h1r = 0.5 * (s1.x + s1m.x), h1i = 0.5 * (s1.y - s1m.y);
+        dst[i] = make_double2(h0r - h1i, h0i + h1r);
+    }
+}
+
+// Images of pack blockIdx.y: fftshift (output index (k + n/2) mod n,
+// ZeroToCentre2DShift.cu:31-91 for odd and even n), the m flip and both
+// corrections; the real part is image 0, the imaginary part image 1.
+__global__ __launch_bounds__(GEN_THREADS) void k_gen_store(const PackDesc *pd, const double2 *X, int nx, int ny)
+{
+    const PackDesc &d = pd[blockIdx.y];
+    const int64_t n = (int64_t)nx * ny;
+    const double2 *src = X + (int64_t)blockIdx.y * n;
+    float *i0 = d.img[0], *i1 = d.img[1];
+    const int hx = nx / 2, hy = ny / 2;
+    for (int64_t i = (int64_t)blockIdx.x * GEN_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * GEN_THREADS) {
+        const int ky = (int)(i / nx), kx = (int)(i - (int64_t)ky * nx);
+        int jy = ky + hy, jx = kx + hx;
+        if (jy >= ny) jy -= ny;
+        if (jx >= nx) jx -= nx;
+        const int row = ny - 1 - jy;  // m axis flipped for FITS
+        const double2 v = src[i];
+        const double cf = (double)d.corr_x[jx] * (double)d.corr_y[row];
+        const int64_t o = (int64_t)row * nx + jx;
+        i0[o] = (float)(v.x * cf);
+        if (i1) i1[o] = (float)(v.y * cf);
+    }
+}
+
+// complex64 <-> complex128 planes of mti_fft2d_inverse.
+__global__ __launch_bounds__(GEN_THREADS) void k_gen_widen(const float2 *a, double2 *X, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * GEN_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * GEN_THREADS)
+        X[i] = make_double2((double)a[i].x, (double)a[i].y);
+}
+__global__ __launch_bounds__(GEN_THREADS) void k_gen_narrow(const double2 *X, float2 *a, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * GEN_THREADS + threadIdx.x; i < n; i += (int64_t)gridDim.x * GEN_THREADS)
+        a[i] = make_float2((float)X[i].x, (float)X[i].y);
+}
+
+// One axis of the general transform (host side).
+struct GenAxis {
+    int N = 1;                 // transform length
+    int L = 1;                 // length of the Stockham passes: N, or M for Bluestein
+    bool blue = false;
+    std::vector<int> radices;  // factors of L in pass order
+    size_t tw = 0, chirp = 0, dk = 0;  // offsets (elements) in the table blob: tw[L], chirp[N], D[L]
+};
+
+struct GenPlan {
+    GenAxis x, y;
+    size_t elems = 0;  // double2 elements of the table blob
+};
+
+// Radices 16, 8, 4, 2, 3, 5, 7, 11, 13; false if a larger prime factor is left.
+static bool gen_factor(int L, std::vector<int> &out)
+{
+    out.clear();
+    while (L % 16 == 0) {
+        out.push_back(16);
+        L /= 16;
+    }
+    for (int r : {8, 4, 2})
+        if (L % r == 0) {
+            out.push_back(r);
+            L /= r;
+        }
+    for (int r : {3, 5, 7, 11, 13})
+        while (L % r == 0) {
+            out.push_back(r);
+            L /= r;
+        }
+    return L == 1;
+}
+
+static GenAxis gen_axis(int N, size_t &off)
+{
+    GenAxis a;
+    a.N = N;
+    if (N <= 1) return a;
+    if (gen_factor(N, a.radices)) {
+        a.L = N;
+    } else {
+        a.blue = true;
+        int M = 1;
+        while (M < 2 * N - 1) M <<= 1;
+        a.L = M;
+        gen_factor(M, a.radices);
+    }
+    a.tw = off;
+    off += a.L;
+    if (a.blue) {
+        a.chirp = off;
+        off += N;
+        a.dk = off;
+        off += a.L;
+    }
+    return a;
+}
+
+static GenPlan gen_plan(int nx, int ny)
+{
+    GenPlan g;
+    size_t off = 0;
+    g.x = gen_axis(nx, off);
+    g.y = gen_axis(ny, off);
+    g.elems = std::max<size_t>(off, 1);
+    return g;
+}
+
+// Forward DFT of a power-of-two length on the host (FP64, radix 2, twiddles
+// from cos/sin of the exact angle).
+static void host_fft_fwd(std::vector<double2> &a)
+{
+    const int n = (int)a.size();
+    for (int i = 1, j = 0; i < n; i++) {
+        int bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) std::swap(a[i], a[j]);
+    }
+    for (int len = 2; len <= n; len <<= 1) {
+        const int h = len / 2;
+        for (int k = 0; k < h; k++) {
+            const double ang = -2.0 * M_PI * (double)k / (double)len;
+            const double wr = std::cos(ang), wi = std::sin(ang);
+            for (int i = 0; i < n; i += len) {
+                double2 &u = a[i + k], &v = a[i + k + h];
+                const double tr = v.x * wr - v.y * wi, ti = v.x * wi + v.y * wr;
+                v = make_double2(u.x - tr, u.y - ti);
+                u = make_double2(u.x + tr, u.y + ti);
+            }
+        }
+    }
+}
+
+static void gen_fill(const GenAxis &a, std::vector<double2> &h)
+{
+    if (a.N <= 1) return;
+    for (int t = 0; t < a.L; t++) {
+        const double ang = 2.0 * M_PI * (double)t / (double)a.L;
+        h[a.tw + t] = make_double2(std::cos(ang), std::sin(ang));
+    }
+    if (!a.blue) return;
+    const int N = a.N, M = a.L;
+    for (int n = 0; n < N; n++) {
+        const int64_t q = ((int64_t)n * n) % (2 * (int64_t)N);  // exact argument
+        const double ang = M_PI * (double)q / (double)N;
+        h[a.chirp + n] = make_double2(std::cos(ang), std::sin(ang));
+    }
+    std::vector<double2> b(M, make_double2(0.0, 0.0));
+    for (int n = 0; n < N; n++) {
+        const double2 c = h[a.chirp + n];
+        b[n] = make_double2(c.x, -c.y);
+        if (n) b[M - n] = b[n];
+    }
+    host_fft_fwd(b);
+    for (int k = 0; k < M; k++) {
+        const double2 v = b[k ? M - k : 0];
+        h[a.dk + k] = make_double2(v.x / M, v.y / M);
+    }
+}
+
+// The table blob of (nx, ny), cached per size on the context (or, in a
+// caller workspace, uploaded into its slot for the call).
+static mti_status gen_tables(mti_ctx *ctx, const GenPlan &g, const double2 **out, hipStream_t st)
+{
+    for (auto &t : ctx->gen_tables)
+        if (t.nx == g.x.N && t.ny == g.y.N) {
+            *out = t.ptr;
+            return MTI_OK;
+        }
+    std::vector<double2> h(g.elems, make_double2(0.0, 0.0));
+    gen_fill(g.x, h);
+    gen_fill(g.y, h);
+    const size_t bytes = g.elems * sizeof(double2);
+    if (ctx->ws_active) {
+        void *w = nullptr;
+        mti_status r = scratch_get(ctx, SLOT_GTAB, bytes, &w);
+        if (r != MTI_OK) return r;
+        if ((r = upload(ctx, w, h.data(), bytes, st)) != MTI_OK) return r;
+        if ((r = host_sync(ctx, st)) != MTI_OK) return r;
+        *out = (const double2 *)w;
+        return MTI_OK;
+    }
+    double2 *d = nullptr;
+    MTI_HIP(ctx, hipMalloc(&d, bytes));
+    MTI_HIP(ctx, hipMemcpyAsync(d, h.data(), bytes, hipMemcpyHostToDevice, st));
+    MTI_HIP(ctx, hipStreamSynchronize(st));
+    ctx->gen_tables.push_back({g.x.N, g.y.N, d, bytes});
+    *out = d;
+    return MTI_OK;
+}
+
+static unsigned gen_blocks(uint64_t threads)
+{
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((threads + GEN_THREADS - 1) / GEN_THREADS, 1u << 20));
+}
+
+static mti_status gen_pass(mti_ctx *ctx, int R, const double2 *src, double2 *dst, int L, int Ns, int V, int64_t is,
+                           int64_t bs, int nb, const double2 *tw, hipStream_t st)
+{
+    const dim3 gr(gen_blocks((uint64_t)nb * V * (L / R))), bl(GEN_THREADS);
+    switch (R) {
+#define MTI_GP(RR) \
+    case RR: hipLaunchKernelGGL(k_gen_pass<RR>, gr, bl, 0, st, src, dst, L, Ns, V, is, bs, nb, tw); break;
+        MTI_GP(2) MTI_GP(3) MTI_GP(4) MTI_GP(5) MTI_GP(7) MTI_GP(8) MTI_GP(11) MTI_GP(13) MTI_GP(16)
+#undef MTI_GP
+    default: return set_error(ctx, MTI_ERR_UNSUPPORTED, "FFT radix");
+    }
+    MTI_HIP(ctx, hipGetLastError());
+    return MTI_OK;
+}
+
+// Stockham passes over a length-L layout; the data moves between *a and *b
+// and ends in *a.
+static mti_status gen_passes(mti_ctx *ctx, const std::vector<int> &radices, double2 *&a, double2 *&b, int L, int V,
+                             int64_t is, int64_t bs, int nb, const double2 *tw, hipStream_t st)
+{
+    int Ns = 1;
+    for (int R : radices) {
+        mti_status r = gen_pass(ctx, R, a, b, L, Ns, V, is, bs, nb, tw, st);
+        if (r != MTI_OK) return r;
+        std::swap(a, b);
+        Ns *= R;
+    }
+    return MTI_OK;
+}
+
+// One axis of nb * V transforms held in *X (ping-pong buffer *Y; Bluestein
+// buffers T, U); the result ends in *X.
+static mti_status gen_axis_run(mti_ctx *ctx, const GenAxis &ax, const double2 *tab, double2 *&X, double2 *&Y,
+                               double2 *T, double2 *U, int V, int64_t is, int64_t bs, int nb, hipStream_t st)
+{
+    if (ax.N <= 1) return MTI_OK;
+    const double2 *tw = tab + ax.tw;
+    if (!ax.blue) return gen_passes(ctx, ax.radices, X, Y, ax.L, V, is, bs, nb, tw, st);
+    const int N = ax.N, M = ax.L;
+    const uint64_t tot = (uint64_t)nb * M * V;
+    hipLaunchKernelGGL(k_blue_pre, dim3(gen_blocks(tot)), dim3(GEN_THREADS), 0, st, X, T, N, M, V, is, bs, nb,
+                       tab + ax.chirp);
+    MTI_HIP(ctx, hipGetLastError());
+    double2 *a = T, *b = U;
+    mti_status r = gen_passes(ctx, ax.radices, a, b, M, V, (int64_t)V, (int64_t)M * V, nb, tw, st);
+    if (r != MTI_OK) return r;
+    hipLaunchKernelGGL(k_blue_mid, dim3(gen_blocks(tot)), dim3(GEN_THREADS), 0, st, a, M, V, nb, tab + ax.dk);
+    MTI_HIP(ctx, hipGetLastError());
+    if ((r = gen_passes(ctx, ax.radices, a, b, M, V, (int64_t)V, (int64_t)M * V, nb, tw, st)) != MTI_OK) return r;
+    hipLaunchKernelGGL(k_blue_post, dim3(gen_blocks((uint64_t)nb * N * V)), dim3(GEN_THREADS), 0, st, a, X, N, M, V,
+                       is, bs, nb, tab + ax.chirp);
+    MTI_HIP(ctx, hipGetLastError());
+    return MTI_OK;
+}
+
+// Unnormalised 2-D inverse transform of nb planes [nb][ny][nx] held in *X.
+static mti_status gen_transform(mti_ctx *ctx, const GenPlan &g, const double2 *tab, double2 *&X, double2 *&Y,
+                                double2 *T, double2 *U, int nb, hipStream_t st)
+{
+    const int nx = g.x.N, ny = g.y.N;
+    mti_status r = gen_axis_run(ctx, g.x, tab, X, Y, T, U, 1, 1, nx, nb * ny, st);
+    if (r != MTI_OK) return r;
+    return gen_axis_run(ctx, g.y, tab, X, Y, T, U, nx, nx, (int64_t)nx * ny, nb, st);
+}
+
+// Scratch of one general-path batch of nb planes: X, Y and (Bluestein) T, U.
+static size_t gen_plane_elems(const GenPlan &g) { return (size_t)g.x.N * g.y.N; }
+static size_t gen_blue_elems(const GenPlan &g)
+{
+    size_t t = 0;
+    if (g.x.blue) t = std::max(t, (size_t)g.x.L * g.y.N);
+    if (g.y.blue) t = std::max(t, (size_t)g.y.L * g.x.N);
+    return t;
+}
+static size_t gen_batch_bytes(const GenPlan &g, int nb)
+{
+    return (2 * gen_plane_elems(g) + 2 * gen_blue_elems(g)) * sizeof(double2) * (size_t)nb;
+}
+// Planes per general-path batch: at most 16 GB of scratch (2^30 elements).
+static int gen_batch(const GenPlan &g, int planes)
+{
+    const size_t per = gen_batch_bytes(g, 1);
+    return (int)std::max<size_t>(1, std::min<size_t>((size_t)planes, (size_t(16) << 30) / per));
+}
+static void gen_buffers(const GenPlan &g, int nb, void *w, double2 **X, double2 **Y, double2 **T, double2 **U)
+{
+    double2 *p = (double2 *)w;
+    const size_t pe = gen_plane_elems(g) * nb, be = gen_blue_elems(g) * nb;
+    *X = p;
+    *Y = p + pe;
+    *T = p + 2 * pe;
+    *U = p + 2 * pe + be;
+}
+
+// General-path finalize of npk packs (device descriptors pd).
+template <typename TI>
+static mti_status gen_finalize_packs(mti_ctx *ctx, const GenPlan &g, const double2 *tab, const PackDesc *pd,
+                                     int npk, void *w, int64_t cs, hipStream_t st)
+{
+    const int nx = g.x.N, ny = g.y.N;
+    double2 *X, *Y, *T, *U;
+    gen_buffers(g, npk, w, &X, &Y, &T, &U);
+    const dim3 gr(gen_blocks((uint64_t)nx * ny / 4 + 1), (unsigned)npk);
+    {
+        StageTimer tm(ctx, MTI_STAGE_FFT_ROWS, st);
+        hipLaunchKernelGGL(k_gen_pack<TI>, gr, dim3(GEN_THREADS), 0, st, pd, X, nx, ny, cs);
+        MTI_HIP(ctx, hipGetLastError());
+        mti_status r = gen_axis_run(ctx, g.x, tab, X, Y, T, U, 1, 1, nx, npk * ny, st);
+        if (r != MTI_OK) return r;
+    }
+    {
+        StageTimer tm(ctx, MTI_STAGE_FFT_COLS, st);
+        mti_status r = gen_axis_run(ctx, g.y, tab, X, Y, T, U, nx, nx, (int64_t)nx * ny, npk, st);
+        if (r != MTI_OK) return r;
+    }
+    StageTimer tm(ctx, MTI_STAGE_FINALIZE, st);
+    hipLaunchKernelGGL(k_gen_store, gr, dim3(GEN_THREADS), 0, st, pd, X, nx, ny);
+    MTI_HIP(ctx, hipGetLastError());
+    return MTI_OK;
+}
+
 // Uploads the descriptors of a call (once; the copy completes before any
 // batch is enqueued, so later batches never overwrite what earlier ones read).
 template <typename D>
@@ -1015,10 +1205,27 @@ static mti_status finalize_channels(mti_ctx *ctx, const FftGeom &g, const std::v
     PackDesc *pd = nullptr;
     mti_status r = upload_desc(ctx, h, &pd, st);
     if (r != MTI_OK) return r;
+    const int npk = (int)h.size();
+    const int64_t cs = layout == MTI_GRID_VUP ? P : 1;
+    if (!pk_ok(g)) {
+        // general sizes: complex128 planes, HBM passes
+        const GenPlan gp = gen_plan(g.nx, g.ny);
+        const double2 *tab = nullptr;
+        if ((r = gen_tables(ctx, gp, &tab, st)) != MTI_OK) return r;
+        const int batch = gen_batch(gp, npk);
+        void *w = nullptr;
+        if ((r = scratch_get(ctx, SLOT_FFT, gen_batch_bytes(gp, batch), &w)) != MTI_OK) return r;
+        for (int k0 = 0; k0 < npk; k0 += batch) {
+            const int nk = std::min(batch, npk - k0);
+            r = c64 ? gen_finalize_packs<float2>(ctx, gp, tab, pd + k0, nk, w, cs, st)
+                    : gen_finalize_packs<double2>(ctx, gp, tab, pd + k0, nk, w, cs, st);
+            if (r != MTI_OK) return r;
+        }
+        return MTI_OK;
+    }
     const double2 *tw = nullptr;
     r = twiddle_table(ctx, g.NT, &tw, st);
     if (r != MTI_OK) return r;
-    const int npk = (int)h.size();
     // the batch is sized for complex128 planes (the workspace plans do the
     // same), so complex64 planes take half of what was planned
     const int batch = transform_batch(g, npk);
@@ -1026,7 +1233,6 @@ static mti_status finalize_channels(mti_ctx *ctx, const FftGeom &g, const std::v
     void *w = nullptr;
     r = scratch_get(ctx, SLOT_FFT, (size_t)g.nx * g.ny * (x64 ? sizeof(float2) : sizeof(double2)) * batch, &w);
     if (r != MTI_OK) return r;
-    const int64_t cs = layout == MTI_GRID_VUP ? P : 1;
     for (int k0 = 0; k0 < npk; k0 += batch) {
         const int nk = std::min(batch, npk - k0);
         if (x64)
@@ -1043,10 +1249,20 @@ static mti_status finalize_channels(mti_ctx *ctx, const FftGeom &g, const std::v
 // ---- scratch plans (the requests of each entry point, largest per slot) ----
 static void plan_twiddles(int NT, ScratchPlan &pl) { pl.need(SLOT_TWIDDLE, (size_t)NT * sizeof(double2)); }
 
+static void plan_gen(const GenPlan &gp, int planes, ScratchPlan &pl)
+{
+    pl.need(SLOT_GTAB, gp.elems * sizeof(double2));
+    pl.need(SLOT_FFT, gen_batch_bytes(gp, gen_batch(gp, planes)));
+}
+
 static void plan_finalize_channels(const FftGeom &g, int npk, ScratchPlan &pl)
 {
-    plan_twiddles(g.NT, pl);
     pl.need(SLOT_CHANDESC, (size_t)std::max(1, npk) * sizeof(PackDesc));
+    if (!pk_ok(g)) {
+        plan_gen(gen_plan(g.nx, g.ny), npk, pl);
+        return;
+    }
+    plan_twiddles(g.NT, pl);
     pl.need(SLOT_FFT, (size_t)g.nx * g.ny * sizeof(double2) * transform_batch(g, npk));
 }
 
@@ -1057,21 +1273,11 @@ static int pack_count(int P, int nchan) { return P == 1 ? (nchan + 1) / 2 : P ==
 static void plan_finalize(const mti_image_params *p, int npol, int nchan, ScratchPlan &pl)
 {
     const FftGeom g = geom(p->nx, p->ny);
-    plan_twiddles(g.NT, pl);
     pl.need(SLOT_FCORR, (size_t)(p->nx + p->ny + 128) * 4);
-    if (pk_ok(g)) {
-        plan_finalize_channels(g, pack_count(npol, nchan), pl);
-    } else {
-        const int batch = std::max(1, transform_batch(g, npol * nchan) / npol);
-        pl.need(SLOT_FFT, (size_t)p->nx * p->ny * sizeof(double2) * npol * batch);
-    }
+    plan_finalize_channels(g, pack_count(npol, nchan), pl);
 }
 
-static void plan_fft2d(const FftGeom &g, int nbatch, ScratchPlan &pl)
-{
-    plan_twiddles(g.NT, pl);
-    pl.need(SLOT_FFT, (size_t)nbatch * g.nx * g.ny * sizeof(float2));
-}
+static void plan_fft2d(int nx, int ny, int nbatch, ScratchPlan &pl) { plan_gen(gen_plan(nx, ny), nbatch, pl); }
 
 static void plan_finalize_jobs(int32_t njobs, const mti_job *jobs, int P, ScratchPlan &pl)
 {
@@ -1093,13 +1299,10 @@ static void plan_finalize_jobs(int32_t njobs, const mti_job *jobs, int P, Scratc
         const FftGeom g = geom(sz.first, sz.second);
         int nch = 0;
         for (int i = 0; i < njobs; i++)
-            if (jobs[i].params.nx == sz.first && jobs[i].params.ny == sz.second) {
-                nch += jobs[i].nchan;
-                if (!pk_ok(g)) plan_finalize(&jobs[i].params, P, jobs[i].nchan, pl);
-            }
+            if (jobs[i].params.nx == sz.first && jobs[i].params.ny == sz.second) nch += jobs[i].nchan;
         // one correlation: at most one pack per channel (channels pair only
         // within a correction group)
-        if (pk_ok(g)) plan_finalize_channels(g, P == 1 ? nch : pack_count(P, nch), pl);
+        plan_finalize_channels(g, P == 1 ? nch : pack_count(P, nch), pl);
     }
 }
 
@@ -1111,43 +1314,36 @@ extern "C" mti_status mti_fft2d_inverse(mti_ctx *ctx, float *data, int32_t nx, i
                                         void *stream)
 {
     if (!ctx) return MTI_ERR_INVALID;
-    if (!is_pow2(nx) || !is_pow2(ny) || nx > 8192 || ny > 8192)
-        return set_error(ctx, MTI_ERR_UNSUPPORTED, "FFT sizes must be powers of two in [2, 8192]");
+    MTI_CHECK(ctx, nx >= 1 && ny >= 1 && nx <= 32768 && ny <= 32768, "FFT sizes must be in [1, 32768]");
     MTI_CHECK(ctx, data && nbatch >= 1, "null data or nbatch < 1");
     MTI_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = to_stream(stream);
-    FftGeom g = geom(nx, ny);
+    const GenPlan gp = gen_plan(nx, ny);
     mti_status r;
     CallScope cs(ctx);
     if (cs.planning()) {
         ScratchPlan pl;
-        plan_fft2d(g, nbatch, pl);
+        plan_fft2d(nx, ny, nbatch, pl);
         if ((r = cs.begin(pl)) != MTI_OK) return r;
     }
-    const double2 *tw = nullptr;
-    r = twiddle_table(ctx, g.NT, &tw, st);
-    if (r != MTI_OK) return r;
-    float2 *X = (float2 *)data;
-    r = rows_pass<1, float2, float2>(ctx, g, X, X, 1, nbatch, tw, st);
-    if (r != MTI_OK) return r;
-    r = cols1<float2>(ctx, g, X, nbatch, tw, st);
-    if (r != MTI_OK) return r;
+    const double2 *tab = nullptr;
+    if ((r = gen_tables(ctx, gp, &tab, st)) != MTI_OK) return r;
+    const int batch = gen_batch(gp, nbatch);
     void *w = nullptr;
-    size_t bytes = (size_t)nbatch * nx * ny * sizeof(float2);
-    r = scratch_get(ctx, SLOT_FFT, bytes, &w);
-    if (r != MTI_OK) return r;
-    r = cols2<1, 1, float2>(ctx, g, X, nbatch, tw, nullptr, 0, nullptr, nullptr, nullptr, (float2 *)w, st);
-    if (r != MTI_OK) return r;
-    MTI_HIP(ctx, hipMemcpyAsync(data, w, bytes, hipMemcpyDeviceToDevice, st));
-    return MTI_OK;
-}
-
-static mti_status check_finalize_size(mti_ctx *ctx, const mti_image_params *p)
-{
-    // FP64 transform: the row pass holds rows of nx complex128 in LDS.
-    if (!is_pow2(p->nx) || !is_pow2(p->ny) || p->nx > 4096 || p->ny > 8192)
-        return set_error(ctx, MTI_ERR_UNSUPPORTED,
-                         "image.nx must be a power of two in [2,4096] and image.ny in [2,8192]");
+    if ((r = scratch_get(ctx, SLOT_FFT, gen_batch_bytes(gp, batch), &w)) != MTI_OK) return r;
+    const int64_t plane = (int64_t)nx * ny;
+    for (int b0 = 0; b0 < nbatch; b0 += batch) {
+        const int nb = std::min(batch, nbatch - b0);
+        double2 *X, *Y, *T, *U;
+        gen_buffers(gp, nb, w, &X, &Y, &T, &U);
+        float2 *a = (float2 *)data + (int64_t)b0 * plane;
+        const int64_t n = plane * nb;
+        hipLaunchKernelGGL(k_gen_widen, dim3(gen_blocks((uint64_t)n / 4 + 1)), dim3(GEN_THREADS), 0, st, a, X, n);
+        MTI_HIP(ctx, hipGetLastError());
+        if ((r = gen_transform(ctx, gp, tab, X, Y, T, U, nb, st)) != MTI_OK) return r;
+        hipLaunchKernelGGL(k_gen_narrow, dim3(gen_blocks((uint64_t)n / 4 + 1)), dim3(GEN_THREADS), 0, st, X, a, n);
+        MTI_HIP(ctx, hipGetLastError());
+    }
     return MTI_OK;
 }
 
@@ -1172,8 +1368,6 @@ extern "C" mti_status mti_finalize_sparse(mti_ctx *ctx, const mti_image_params *
     MTI_CHECK(ctx, nchan >= 1 && grids && norm && image, "null pointer or nchan < 1");
     MTI_CHECK(ctx, grid_layout == MTI_GRID_VUP || grid_layout == MTI_GRID_PVU, "bad grid_layout");
     MTI_CHECK(ctx, grid_precision == MTI_GRID_C64 || grid_precision == MTI_GRID_C128, "bad grid_precision");
-    r = check_finalize_size(ctx, p);
-    if (r != MTI_OK) return r;
     MTI_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = to_stream(stream);
     FftGeom g = geom(p->nx, p->ny);
@@ -1183,9 +1377,6 @@ extern "C" mti_status mti_finalize_sparse(mti_ctx *ctx, const mti_image_params *
         plan_finalize(p, npol, nchan, pl);
         if ((r = cs.begin(pl)) != MTI_OK) return r;
     }
-    const double2 *tw = nullptr;
-    r = twiddle_table(ctx, g.NT, &tw, st);
-    if (r != MTI_OK) return r;
     // correction vectors
     void *wc = nullptr;
     r = scratch_get(ctx, SLOT_FCORR, (size_t)(p->nx + p->ny + 128) * 4, &wc);
@@ -1198,47 +1389,11 @@ extern "C" mti_status mti_finalize_sparse(mti_ctx *ctx, const mti_image_params *
     const size_t elem = grid_precision == MTI_GRID_C64 ? sizeof(float2) : sizeof(double2);
     const size_t chan_grid = (size_t)p->nx * p->ny * npol;
     const size_t tiles = (size_t)((p->nx + 31) / 32) * ((p->ny + 31) / 32);
-    if (pk_ok(g)) {
-        std::vector<ChanRef> ch(nchan);
-        for (int c = 0; c < nchan; c++)
-            ch[c] = ChanRef{(const char *)grids + elem * chan_grid * c, norm + (size_t)c * npol,
-                            image + chan_grid * c, corr_x, corr_y, tile_mask ? tile_mask + tiles * c : nullptr};
-        return finalize_channels(ctx, g, ch, npol, poltype, grid_layout, grid_precision == MTI_GRID_C64, st);
-    }
-    if (tile_mask)
-        return set_error(ctx, MTI_ERR_UNSUPPORTED, "sparse grids need the packed transform (images of 16+ pixels)");
-    // small images: one FP64 plane per correlation, transform and finalize per channel batch
-    const size_t plane_bytes = (size_t)p->nx * p->ny * sizeof(double2);
-    const int batch = std::max(1, transform_batch(g, npol * nchan) / npol);
-    void *w = nullptr;
-    r = scratch_get(ctx, SLOT_FFT, plane_bytes * npol * batch, &w);
-    if (r != MTI_OK) return r;
-    double2 *X = (double2 *)w;
-    for (int c0 = 0; c0 < nchan; c0 += batch) {
-        const int nb = std::min(batch, nchan - c0);
-        const size_t off = (size_t)c0 * chan_grid;
-        float *img = image + (size_t)c0 * chan_grid;
-        const double *nr = norm + (size_t)c0 * npol;
-        if (grid_precision == MTI_GRID_C64) {
-            const float2 *G = (const float2 *)grids + off;
-            if (npol == 1)
-                r = finalize_batch<1>(ctx, g, G, X, nb, tw, nr, poltype, corr_x, corr_y, img, grid_layout, st);
-            else if (npol == 2)
-                r = finalize_batch<2>(ctx, g, G, X, nb, tw, nr, poltype, corr_x, corr_y, img, grid_layout, st);
-            else
-                r = finalize_batch<4>(ctx, g, G, X, nb, tw, nr, poltype, corr_x, corr_y, img, grid_layout, st);
-        } else {
-            const double2 *G = (const double2 *)grids + off;
-            if (npol == 1)
-                r = finalize_batch<1>(ctx, g, G, X, nb, tw, nr, poltype, corr_x, corr_y, img, grid_layout, st);
-            else if (npol == 2)
-                r = finalize_batch<2>(ctx, g, G, X, nb, tw, nr, poltype, corr_x, corr_y, img, grid_layout, st);
-            else
-                r = finalize_batch<4>(ctx, g, G, X, nb, tw, nr, poltype, corr_x, corr_y, img, grid_layout, st);
-        }
-        if (r != MTI_OK) return r;
-    }
-    return MTI_OK;
+    std::vector<ChanRef> ch(nchan);
+    for (int c = 0; c < nchan; c++)
+        ch[c] = ChanRef{(const char *)grids + elem * chan_grid * c, norm + (size_t)c * npol, image + chan_grid * c,
+                        corr_x, corr_y, tile_mask ? tile_mask + tiles * c : nullptr};
+    return finalize_channels(ctx, g, ch, npol, poltype, grid_layout, grid_precision == MTI_GRID_C64, st);
 }
 
 extern "C" mti_status mti_finalize_jobs(mti_ctx *ctx, int32_t njobs, const mti_job *jobs, int32_t poltype,
@@ -1252,8 +1407,6 @@ extern "C" mti_status mti_finalize_jobs(mti_ctx *ctx, int32_t njobs, const mti_j
     const int P = jobs[0].vis.npol;
     for (int i = 0; i < njobs; i++) {
         mti_status r = validate_params(ctx, &jobs[i].params);
-        if (r != MTI_OK) return r;
-        r = check_finalize_size(ctx, &jobs[i].params);
         if (r != MTI_OK) return r;
         MTI_CHECK(ctx, jobs[i].vis.npol == P, "jobs of one call must share npol");
         MTI_CHECK(ctx, jobs[i].grids && jobs[i].norm && jobs[i].image && jobs[i].nchan >= 1, "null job outputs");
@@ -1309,18 +1462,6 @@ extern "C" mti_status mti_finalize_jobs(mti_ctx *ctx, int32_t njobs, const mti_j
         auto member = [&](const mti_job &j) {
             return j.params.nx == sz[0] && j.params.ny == sz[1] && (j.grids_c64 != 0) == c64;
         };
-        if (!pk_ok(g)) {
-            // small images: one job at a time through the per-job finalize
-            for (int i = 0; i < njobs; i++) {
-                const mti_job &j = jobs[i];
-                if (!member(j)) continue;
-                r = mti_finalize_sparse(ctx, &j.params, P, poltype, j.nchan, grid_layout,
-                                        c64 ? MTI_GRID_C64 : MTI_GRID_C128, j.grids, j.tile_mask, j.norm, j.image,
-                                        stream);
-                if (r != MTI_OK) return r;
-            }
-            continue;
-        }
         const size_t chan_grid = (size_t)sz[0] * sz[1] * P;
         const size_t elem = c64 ? sizeof(float2) : sizeof(double2);
         std::vector<ChanRef> ch;
@@ -1341,9 +1482,9 @@ extern "C" mti_status mti_finalize_jobs(mti_ctx *ctx, int32_t njobs, const mti_j
 
 extern "C" mti_status mti_fft2d_workspace_size(int32_t nx, int32_t ny, int32_t nbatch, size_t *bytes)
 {
-    if (!bytes || !is_pow2(nx) || !is_pow2(ny) || nx > 8192 || ny > 8192 || nbatch < 1) return MTI_ERR_INVALID;
+    if (!bytes || nx < 1 || ny < 1 || nx > 32768 || ny > 32768 || nbatch < 1) return MTI_ERR_INVALID;
     ScratchPlan pl;
-    plan_fft2d(geom(nx, ny), nbatch, pl);
+    plan_fft2d(nx, ny, nbatch, pl);
     *bytes = pl.total();
     return MTI_OK;
 }
@@ -1351,7 +1492,7 @@ extern "C" mti_status mti_fft2d_workspace_size(int32_t nx, int32_t ny, int32_t n
 extern "C" mti_status mti_finalize_workspace_size(const mti_image_params *p, int32_t npol, int32_t nchan,
                                                   size_t *bytes)
 {
-    if (!bytes || !p || validate_params(nullptr, p) != MTI_OK || check_finalize_size(nullptr, p) != MTI_OK ||
+    if (!bytes || !p || validate_params(nullptr, p) != MTI_OK ||
         !(npol == 1 || npol == 2 || npol == 4) || nchan < 1)
         return MTI_ERR_INVALID;
     ScratchPlan pl;
@@ -1370,7 +1511,7 @@ extern "C" mti_status mti_finalize_jobs_workspace_size(int32_t njobs, const mti_
     const int P = jobs[0].vis.npol;
     if (!(P == 1 || P == 2 || P == 4)) return MTI_ERR_INVALID;
     for (int i = 0; i < njobs; i++)
-        if (validate_params(nullptr, &jobs[i].params) != MTI_OK || check_finalize_size(nullptr, &jobs[i].params) != MTI_OK ||
+        if (validate_params(nullptr, &jobs[i].params) != MTI_OK ||
             jobs[i].vis.npol != P || jobs[i].nchan < 1)
             return MTI_ERR_INVALID;
     ScratchPlan pl;
@@ -1382,7 +1523,7 @@ extern "C" mti_status mti_finalize_jobs_workspace_size(int32_t njobs, const mti_
 extern "C" mti_status mti_finalize_out_shape(const mti_image_params *p, int32_t npol, int32_t nchan,
                                              int64_t image_dims[4])
 {
-    if (!p || !image_dims || validate_params(nullptr, p) != MTI_OK || check_finalize_size(nullptr, p) != MTI_OK ||
+    if (!p || !image_dims || validate_params(nullptr, p) != MTI_OK ||
         !(npol == 1 || npol == 2 || npol == 4) || nchan < 1)
         return MTI_ERR_INVALID;
     image_dims[0] = nchan;

@@ -41,7 +41,8 @@ enum ScratchSlot {
     SLOT_CORR = 14,     // per-job correction vectors of the batched finalize
     SLOT_FCORR = 15,    // correction vectors of mti_finalize
     SLOT_TWIDDLE = 16,  // FFT twiddles of a call that runs in a caller workspace
-    SLOT_COUNT = 17
+    SLOT_GTAB = 17,     // general-size transform tables of a call in a caller workspace
+    SLOT_COUNT = 18
 };
 
 // Device scratch one call takes: the largest request of each slot.
@@ -82,6 +83,13 @@ struct mti_ctx {
     mti::Scratch scratch[mti::SLOT_COUNT];
     // FFT twiddle tables keyed by length (FP64, exp(+2 pi i k / n)).
     std::vector<std::pair<int, double2 *>> twiddles;
+    // general-size transform tables (twiddles, Bluestein chirps) keyed by (nx, ny)
+    struct GenTable {
+        int nx, ny;
+        double2 *ptr;
+        size_t bytes;
+    };
+    std::vector<GenTable> gen_tables;
     // caller workspace (mti_set_workspace): while a planned call runs
     // (ws_active), every slot is a fixed part of it and nothing is allocated
     char *ws_ptr = nullptr;

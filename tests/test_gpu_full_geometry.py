@@ -58,23 +58,63 @@ def test_full_geometry_binning_bit_exact(orc, mti, ctx, full):
         print(f"chan {chan}: {got[:, 0].sum()} of {len(rows)} records gridded, planes up to {got[:, 1].max()}")
 
 
-def test_full_geometry_briggs_iquv_images(orc, mti, ctx, full):
+@pytest.fixture(scope="module")
+def oracle_chans(orc, full):
+    """The oracle's grids, normalisers and IQUV images of both channels."""
+    ob, p, OK, idx = (full[k] for k in ("ob", "p", "OK", "idx"))
+    rows = idx.cpu().numpy()
+    op = H.oracle_params(orc, p)
+    return [H.oracle_image(orc, OK, op, ob, chan, rows, weighting="briggs", robust=0.0) for chan in (0, 1)]
+
+
+def _check_iquv(img, refs, label):
+    for chan in (0, 1):
+        r = refs[chan]
+        ref_i = np.sqrt(np.mean(r["image"][0] ** 2))
+        for q, name in enumerate("IQUV"):
+            d = img[chan][q].astype(np.float64) - r["image"][q]
+            e = H.image_errors(img[chan][q], r["image"][q])
+            e["rel_rms_vs_I"] = float(np.sqrt(np.mean(d ** 2)) / ref_i)
+            print(f"{label} chan {chan} Stokes {name}: {e}")
+            assert e["rel_rms_vs_I"] < IMAGE_RMS_TOL
+
+
+def test_full_geometry_briggs_iquv_images(orc, mti, ctx, full, oracle_chans):
     ob, p, K, OK, dev, idx = (full[k] for k in ("ob", "p", "K", "OK", "dev", "idx"))
     g = H.gpu_image(mti, ctx, K, p, dev, torch, [0, 1], idx, weighting="briggs", robust=0.0,
                     layout="pvu", precision="c128")
-    rows = idx.cpu().numpy()
-    op = H.oracle_params(orc, p)
     ng = 0
     for chan in (0, 1):
-        r = H.oracle_image(orc, OK, op, ob, chan, rows, weighting="briggs", robust=0.0)
+        r = oracle_chans[chan]
         ng += r["gridded"]
         np.testing.assert_allclose(g["norm"][chan], r["norm"], rtol=1e-6)
         assert H.rel_rms(g["grid"][chan], r["grid"]) < GRID_RMS_TOL
-        ref_i = np.sqrt(np.mean(r["image"][0] ** 2))
-        for q, name in enumerate("IQUV"):
-            d = g["image"][chan][q].astype(np.float64) - r["image"][q]
-            e = H.image_errors(g["image"][chan][q], r["image"][q])
-            e["rel_rms_vs_I"] = float(np.sqrt(np.mean(d ** 2)) / ref_i)
-            print(f"full geometry chan {chan} Stokes {name}: {e}")
-            assert e["rel_rms_vs_I"] < IMAGE_RMS_TOL
+    _check_iquv(g["image"], oracle_chans, "full geometry")
     assert g["stats"].gridded_records == ng
+
+
+@pytest.mark.parametrize("layout", ["pvu", "vup"])
+def test_full_geometry_bench_form_iquv_images(mti, ctx, full, oracle_chans, layout):
+    """The form the benchmark times, against the oracle: complex64 grids written
+    by the gridder (the reference's Grid element type, comp/WImager.cpp:231-232;
+    FP64 sums rounded once per work unit, float atomics where a tile is split
+    over units), sparse grids (tile mask) and complex64 planes between the
+    transform passes."""
+    p, K, dev, idx = (full[k] for k in ("p", "K", "dev", "idx"))
+    v = mti.vis_ms(dev.uvw, dev.invl, dev.vis, dev.flags, dev.weight, idx)
+    shape = (2, 4, p.ny, p.nx, 2) if layout == "pvu" else (2, p.ny, p.nx, 4, 2)
+    grids = torch.full(shape, float("nan"), dtype=torch.float32, device="cuda")
+    norm = torch.empty((2, 4), dtype=torch.float64, device="cuda")
+    tmask = torch.empty(mti.tile_mask_shape(p, 2), dtype=torch.uint8, device="cuda")
+    img = torch.empty((2, 4, p.ny, p.nx), dtype=torch.float32, device="cuda")
+    ctx.set_transform_storage("c64")
+    try:
+        ctx.grid(K, p, v, grids, norm, chan0=0, nchan=2, weighting="briggs", robust=0.0, layout=layout,
+                 precision="c64", tile_mask=tmask)
+        ctx.finalize(p, 4, grids, norm, img, nchan=2, layout=layout, tile_mask=tmask)
+        torch.cuda.synchronize()
+    finally:
+        ctx.set_transform_storage("c128")
+    out = img.cpu().numpy()
+    assert np.isfinite(out).all()
+    _check_iquv(out, oracle_chans, f"bench form ({layout})")

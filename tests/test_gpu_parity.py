@@ -438,7 +438,7 @@ def test_invalid_parameters_raise(mti, ctx):
     with pytest.raises(mti.MTIError, match="odd"):
         ctx.build_kernels(bad)
     with pytest.raises(mti.MTIError):
-        ctx.fft2d_inverse(torch.zeros(10, dtype=torch.complex64, device="cuda"), 5, 2)
+        ctx.fft2d_inverse(torch.zeros(10, dtype=torch.complex64, device="cuda"), 0, 2)
 
 
 # ---- batched jobs (the snapshot-mix path) ------------------------------------------

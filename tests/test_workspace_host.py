@@ -44,19 +44,23 @@ def test_shapes_reject_what_validation_rejects(bad):
         M.finalize_out_shape(_params(), a["npol"], a["nchan"])
 
 
-def test_finalize_shape_needs_power_of_two_sizes():
+def test_finalize_shape_takes_any_size():
+    # the reference plans its transform for any rows x cols (IFFT2D.cpp:86-111)
+    assert M.finalize_out_shape(M.make_params(300, 257, 2.0, 2.0, "wproj", 8, 4), 1, 1) == (1, 1, 257, 300)
     with pytest.raises(M.MTIError):
-        M.finalize_out_shape(M.make_params(300, 256, 2.0, 2.0, "wproj", 8, 4), 1, 1)
+        M.finalize_out_shape(M.make_params(1, 256, 2.0, 2.0, "wproj", 8, 4), 1, 1)
 
 
-def test_fft2d_workspace_holds_the_out_of_place_batch():
-    # the column pass writes nbatch complex64 planes out of place, plus twiddles
-    for nx, ny, nb in [(64, 128, 3), (4096, 4096, 1)]:
+def test_fft2d_workspace_holds_the_batch_planes():
+    # FP64 planes X, Y of every transform of the batch plus the tables;
+    # Bluestein lengths (a prime factor above 13) add two planes of 2^k >= 2n-1
+    for nx, ny, nb in [(64, 128, 3), (4096, 4096, 1), (100, 64, 1)]:
         ws = M.fft2d_workspace_size(nx, ny, nb)
-        assert ws >= nb * nx * ny * 8 + max(nx, ny) * 16
-        assert ws < nb * nx * ny * 8 + max(nx, ny) * 16 + 4096
+        assert 2 * nb * nx * ny * 16 <= ws < 2 * nb * nx * ny * 16 + (nx + ny) * 16 + 4096
+    ws = M.fft2d_workspace_size(17 * 3, 8, 1)  # 51 = 3 * 17: Bluestein over 128
+    assert ws >= (2 * 51 * 8 + 2 * 128 * 8) * 16
     with pytest.raises(M.MTIError):
-        M.fft2d_workspace_size(100, 64, 1)
+        M.fft2d_workspace_size(0, 64, 1)
 
 
 def test_sort_rows_workspace_grows_with_rows():
