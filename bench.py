@@ -186,6 +186,47 @@ def cpu_model():
     return "unknown"
 
 
+def cpu_baseline_full(d, params_args, threads):
+    """FP64 oracle on the whole job (small workloads): row sort, W-kernel
+    tables, then per channel the workload's weights, gridding of every row and
+    the finalize -- once on one thread, then with the channels spread over
+    `threads` threads (the oracle is reentrant C; ctypes releases the
+    interpreter lock).  Returns seconds and gridded records."""
+    import numpy as np
+    import oracle
+    from concurrent.futures import ThreadPoolExecutor
+    from mtimager_amd import synth
+    h = d["host"]
+    vis = d["vis"].cpu().numpy()
+    flags = d["flags"].cpu().numpy()
+    weight = d["weight"].cpu().numpy()
+    nx, cell, W, s = params_args
+    nchan = d["nchan"]
+
+    def job(nthreads):
+        t0 = time.perf_counter()
+        rows = oracle.sort_index(h["a1"], h["a2"], d["nant"])
+        p = oracle.make_params(nx, nx, cell * synth.ARCSEC, cell * synth.ARCSEC, "wproj", W, s)
+        K = oracle.Kernels(p)
+        uvw, vr, fr, wr = (np.ascontiguousarray(a[rows]) for a in (h["uvw"], vis, flags, weight))
+        t_k = time.perf_counter() - t0
+
+        def one(chan):
+            wts = None if d["weighting"] == "natural" else oracle.weights(K, uvw, h["inv_lambda"], fr, wr, chan,
+                                                                          d["weighting"], 0.0)
+            g, nrm, ng = oracle.grid(K, uvw, h["inv_lambda"], vr, fr, wr, chan, wts=wts)
+            oracle.finalize(p, g, nrm)
+            return ng
+
+        with ThreadPoolExecutor(max_workers=nthreads) as ex:
+            ng = sum(ex.map(one, range(nchan)))
+        return time.perf_counter() - t0, t_k, ng
+
+    t1, k1, ng1 = job(1)
+    tN, kN, ngN = job(threads)
+    return dict(seconds_1=t1, seconds_n=tN, kernels_s=k1, gridded=ng1, threads=threads)
+
+
 def cpu_baseline(d, params_args, budget_rows, threads):
     """FP64 oracle on a bounded sample: per channel, weights (the workload's) + gridding
     of `budget_rows` baseline-major rows and one finalize.  Timed on one
@@ -487,7 +528,12 @@ def run_c4(args, torch, mti, jobs):
     ctx.close()
     assert gp == gb, (gp, gb)
     lp, lb = np.array(lp) * 1e3, np.array(lb) * 1e3
-    return dict(jobs=len(jobs), gridded_vis=int(gb), kernel_sets=nk, hbm_budget_gb=args.c4_budget_gb,
+    offered = sum(j["nrows"] * j["nch"] for j in jobs)
+    return dict(jobs=len(jobs), offered_vis=int(offered), gridded_vis=int(gb),
+                gridded_fraction=round(gb / max(offered, 1), 4),
+                field=("one 0.85 deg field for every job (cell = 1.5 arcsec x 2048 / nx): on the smaller "
+                       "grids the longest baselines fall outside the uv plane and are not gridded"),
+                kernel_sets=nk, hbm_budget_gb=args.c4_budget_gb,
                 transform_storage=args.transform_storage, grid_precision=args.grid_precision,
                 autotune=tune_info, graph=graph_info,
                 plain=dict(mvis_s=round(gp / tp / 1e6, 3), seconds=round(tp, 3),
@@ -598,6 +644,13 @@ def main():
     ap.add_argument("--jobs", type=int, default=512, help="snapshot mix: number of snapshot jobs")
     ap.add_argument("--c4-budget-gb", type=float, default=64.0, help="snapshot mix: HBM budget of a batched wave")
     ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--no-c4", action="store_true",
+                    help="cube runs: skip the nested snapshot-mix object (batched vs plain)")
+    ap.add_argument("--no-c5", action="store_true",
+                    help="cube runs: skip the nested serving object (served vs plain)")
+    ap.add_argument("--c5-rate", type=float, default=800.0, help="cube runs: arrival rate of the nested serving run")
+    ap.add_argument("--c5-seconds", type=float, default=10.0,
+                    help="cube runs: arrival window of the nested serving run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="snapshot mix: keep the default tuning")
     ap.add_argument("--no-e2e", action="store_true", help="cube runs: skip the pinned-host end-to-end run")
@@ -777,7 +830,19 @@ def main():
         del ref_image
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.config == "c1":
+        thr = max(1, min(args.cpu_threads, os.cpu_count() or 1, nchan))
+        log(f"cpu baseline (FP64 oracle, whole job, 1 and {thr} threads)")
+        cb = cpu_baseline_full(d, (nx, d["cell"], d["W"], d["s"]), thr)
+        cpu = dict(value=round(cb["gridded"] / cb["seconds_n"] / 1e6, 4), unit="Mvis/s", cores=thr, kind="port",
+                   single_thread_value=round(cb["gridded"] / cb["seconds_1"] / 1e6, 4), cpu_model=cpu_model(),
+                   host_cpus=os.cpu_count(),
+                   sample=(f"FP64 oracle (gcc -O3 -march=x86-64-v3, no FP contraction), the whole job timed, "
+                           f"no extrapolation: row sort, W-kernel tables ({cb['kernels_s']:.2f} s), then per "
+                           f"channel {d['weighting']} weighting, gridding of every row and the {nx}^2 x {P} "
+                           f"finalize; {cb['gridded']} gridded records; 1 thread {cb['seconds_1']:.2f} s, "
+                           f"{thr} threads (channels spread over them) {cb['seconds_n']:.2f} s"))
+    elif not args.no_cpu_baseline:
         thr = max(1, min(args.cpu_threads, os.cpu_count() or 1))
         log(f"cpu baseline (FP64 oracle, 1 and {thr} threads)")
         cb = cpu_baseline(d, (nx, d["cell"], d["W"], d["s"]), args.cpu_rows, thr)
@@ -796,6 +861,32 @@ def main():
                            f"extrapolated to {nchan} channels; W-kernel generation not counted"))
 
     lat = np.array(times) * 1e3
+    tuning = ctx.get_tuning()
+    nested = {}
+    if args.config == "c2" and not (args.no_c4 and args.no_c5):
+        # the metric's "batched/served vs plain": the snapshot mix (batched
+        # vs one job at a time) and a serving stream (continuous batching vs
+        # one request at a time), after the cube's buffers are released
+        del grids, image, norm, idx, tmask, d
+        torch.cuda.synchronize()
+        ctx.close()  # the cube's scratch
+        torch.cuda.empty_cache()
+        pool = snapshot_pool(torch, mti, args.jobs)
+        if not args.no_c4:
+            r4 = run_c4(args, torch, mti, pool)
+            if r4.get("autotune"):
+                r4["autotune"] = dict(r4["autotune"], candidates=len(r4["autotune"].pop("candidates_ms")))
+            nested["c4"] = r4
+        if not args.no_c5:
+            a5 = argparse.Namespace(**vars(args))
+            a5.rates, a5.serve_seconds = f"{args.c5_rate:g}", args.c5_seconds
+            r5 = run_c5(a5, torch, mti, pool)
+            rr = r5["rates"][0]
+            nested["c5"] = dict(rate_req_s=rr["rate"], seconds=r5["seconds"], hbm_budget_gb=args.serve_budget_gb,
+                                pool=r5["pool"], served=rr["served"], plain=rr["plain"],
+                                speedup_mvis=rr["speedup_mvis"], kernel_sets=r5["kernel_sets"],
+                                cache_hits=r5["cache_hits"], cache_misses=r5["cache_misses"])
+        del pool
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -817,7 +908,7 @@ def main():
             "natural weighting",
             "rows": nrows, "channels": nchan, "gridded_vis": gridded, "gridded_points": points,
             "supports": [int(sup.min()), int(sup.max())], "path": "plain single-job",
-            "tuning": ctx.get_tuning(), "grids": "complete" if args.complete_grids else "sparse (tile mask)",
+            "tuning": tuning, "grids": "complete" if args.complete_grids else "sparse (tile mask)",
             "grid_precision": args.grid_precision, "transform_storage": args.transform_storage,
         },
         "image_error_vs_fp64_path": (None if form_err is None else
@@ -835,6 +926,7 @@ def main():
         "end_to_end": e2e,
         "cpu_baseline": cpu,
     }
+    out.update(nested)
     print(json.dumps(out), flush=True)
     ctx.close()
 

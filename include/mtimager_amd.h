@@ -185,6 +185,10 @@ mti_status mti_kernels_supports(const mti_kernels *k, int32_t *support, int64_t 
  * sums f32 [nplanes][T*T]. */
 const void  *mti_kernels_table(const mti_kernels *k);
 const float *mti_kernels_sums(const mti_kernels *k);
+/* Device {support, base} int32 pairs per plane: the convolution position data
+ * (getConvFunctionPositionData, cfg/WProjectionConvFunction.cpp:258-265) that
+ * PreProcessUVW takes as input 0 (mti_preprocess_reference). */
+const int32_t *mti_kernels_position(const mti_kernels *k);
 void         mti_kernels_free(mti_kernels *k);
 
 /* ---- baseline-major row order --------------------------------------------
@@ -299,6 +303,74 @@ mti_status mti_finalize_sparse(mti_ctx *ctx, const mti_image_params *p, int32_t 
                                int32_t grid_precision, const void *grids,
                                const uint8_t *tile_mask, const double *norm, float *image,
                                void *stream);
+
+/* ---- the reference's operators with their own array contracts ---------------
+ * For callers that keep the reference's operator DAG by name
+ * (ImagerFactoryHelper, op/ImagerFactoryHelper.cpp:45-87, and the GAFW
+ * standard operators).  Device pointers, no scratch, enqueued on `stream`;
+ * FP64 arithmetic rounded once to the reference's output types. */
+
+/* PreProcessUVW parameters (op/WProjPreProcessUVW.cpp:49-56). */
+typedef struct {
+    int32_t rows, columns;           /* uvImage.rows / uvImage.columns                */
+    double  u_increment;             /* uvImage.u_increment, 1 / (cell_l * columns)   */
+    double  v_increment;             /* uvImage.v_increment, 1 / (cell_m * rows)      */
+    int32_t sampling;                /* ConvolutionFunction.sampling                  */
+    double  wsquareincrement;        /* ConvolutionFunction.wsquareincrement          */
+    int32_t wplanes;                 /* planes of the convolution-function data       */
+} mti_preprocess_ref_params;
+
+/* PreProcessUVW outputs 0-6 (op/WProjPreProcessUVW.cpp:70-84), N records. */
+typedef struct {
+    int32_t *gridding_data;   /* int4 [N]: corner_x, corner_y * columns + corner_x,
+                                 -(corner_x % S), -(corner_y % S); zeros if not gridded */
+    int32_t *conv_index;      /* [N] base + oy S^2 T + ox S^2 (0 if not gridded)  */
+    int32_t *to_grid;         /* [N] 1: grid this record                          */
+    int32_t *to_compress;     /* [N] 1: same kernel and corner as the record before */
+    int32_t *support;         /* [N] S (0 if not gridded)                         */
+    float   *take_conjugate;  /* [N] -1 where w < 0 (some correlation unflagged), else 1 */
+    float   *conv_sum;        /* [P][N] sums[plane][oy T + ox] * weight * (1 - flag) */
+} mti_preprocess_ref_outputs;
+
+/* Replaces WProjPreProcessUVW (op/WProjPreProcessUVW.cu:111-324) with its own
+ * inputs and outputs: convinfo = input 0 ({support, base} int32 pairs per
+ * plane), sums = input 1 (f32 [wplanes][T*T]), v = inputs 2-5 (any layout
+ * mti_vis describes; the reference's is uvw f32 [3][N], per-record f32 1/lambda,
+ * weights f32 [N][P], int flags [N][P]).  Binning in FP64 (bins equal
+ * mti_preprocess's); each correlation's sum uses its own flag. */
+mti_status mti_preprocess_reference(mti_ctx *ctx, const mti_preprocess_ref_params *rp,
+                                    const int32_t *convinfo, const float *sums, const mti_vis *v,
+                                    int32_t chan, const mti_preprocess_ref_outputs *out, void *stream);
+
+/* PreIFFTUVCorrection (op/PreIFFTUVCorrection.cu:28-90): complex [rows][cols][npol]
+ * -> [npol][rows][cols], each axis moved by OriginToCorner (op/common.hcu:122-136).
+ * c128: complex128 elements, else complex64.  Out of place. */
+mti_status mti_uv_reorder(const void *in, void *out, int32_t rows, int32_t cols, int32_t npol,
+                          int32_t c128, void *stream);
+/* ZeroToCentre2DShift (gafw/StandardOperators/ZeroToCentre2DShift.cu:66-119):
+ * every [rows][cols] plane moved by OriginToCentre on both axes.  Out of place. */
+mti_status mti_centre_shift(const void *in, void *out, int64_t planes, int32_t rows, int32_t cols,
+                            int32_t c128, void *stream);
+/* StokesConverter (op/PostIFFTStokesConverter.cu:42-234): complex64 [npol][rows][cols]
+ * and the f32 normaliser [npol] -> f32 Stokes [npol][rows][cols], rows flipped. */
+mti_status mti_stokes(const float *in, const float *norm, float *out, int32_t npol, int32_t rows,
+                      int32_t cols, int32_t poltype, void *stream);
+/* casagridsf, lmPlanePostCorrection=true (op/CasaGridSF.cu:151-199): every
+ * [rows][cols] plane of `in` divided by psi(|j-rows/2|/(rows/2)) psi(|i-cols/2|/(cols/2))
+ * (generate=false), or, with in = NULL, that product written (generate=true). f64: real_double. */
+mti_status mti_taper_correct(const void *in, void *out, int64_t planes, int32_t rows, int32_t cols,
+                             int32_t f64, void *stream);
+/* casagridsf, lmPlanePostCorrection=false (op/CasaGridSF.cu:107-149): the oversampled
+ * taper-only gridding function of support S and sampling s, [dim][dim] with dim = S s
+ * (odd s) or S s + 1, real or complex (complex_out), f32 or f64. */
+mti_status mti_taper_kernel(void *out, int32_t support, int32_t sampling, int32_t complex_out,
+                            int32_t f64, void *stream);
+/* SincCorrector (op/SincCorrector.cu:29-71): f32 planes times pi x / sin(pi x) per axis. */
+mti_status mti_sinc_correct(const float *in, float *out, int64_t planes, int32_t rows, int32_t cols,
+                            int32_t sampling, void *stream);
+/* The f32 NormalizerValue of the reference's WImager (comp/WImager.cpp:196-197)
+ * from the library's FP64 normalisers: n values rounded once. */
+mti_status mti_normalizer_f32(const double *norm, float *out, int64_t n, void *stream);
 
 /* ---- captured batched imaging (HIP graphs) -----------------------------------
  * No counterpart in the reference (it plans each cuFFT call afresh,

@@ -14,7 +14,11 @@
 //   ImagerFactoryHelper    mt-imager lib/operators/ImagerFactoryHelper.cpp:45-87
 //   ConvolutionFunctionGenerator
 //                          mt-imager lib/components/ConvolutionFunctionsGenerator/
-//                          ConvolutionFunctionGenerator.h:23-32
+//                          ConvolutionFunctionGenerator.h:23-32 (WProjectionConvFunction,
+//                          OnlyTaperConvFunction)
+//   Module, WImager, ImageFinalizer
+//                          pil/Module.h:28-46; mt-imager lib/components/WImager.cpp:150-313,
+//                          ImageFinalizer.cpp:31-86
 //   ValidationException / GpuException: the reference throws ValidationException from
 //   validate() and CudaException from submitToGPU (GPUafw.h:28-32).
 //
@@ -161,6 +165,7 @@ public:
     void calculateConvFunction();
     const void *getConvFunction() const;                       // device table
     std::vector<std::pair<int, long long>> getConvFunctionPositionData() const;  // {support, base}
+    const int32_t *getConvFunctionPositionDevice() const;      // device int [planes][2] {support, base}
     const float *getConvFunctionSumData() const;               // device [planes][T*T]
     int getMaxSupport() const;
     int getSampling() const;
@@ -177,17 +182,30 @@ private:
     mti_kernels *k_ = nullptr;
 };
 
-// Creates the operators of the imaging path by the reference's names
-// (ImagerFactoryHelper::createOperator); unknown names give nullptr.
-//   "SortIndex"           antenna1/2 int [N] -> int index [N], real_longint [1] kept
-//   "PreProcessUVW"       WImager inputs -> int [N][8] bins
-//   "ConvolutionGridder"  the WImager gridding DAG
-//                         inputs UVW [3][N] f32, VIS [N][2P] f32, FREQUENCY [N] f32,
-//                         WEIGHT [N][P] f32, FLAGS [N][P] int;
-//                         outputs Grid complex_float [v][u][P], NormalizerValue [P]
-//   "IFFT2D"              complex_float [..][ny][nx], unnormalised inverse
-//   "ImageFinalizer"      UVImage complex_float [v][u][P] + NormalizerValue
-//                         -> real_float [P][v][u] (Stokes)
+// Creates operators by the reference's names (ImagerFactoryHelper::createOperator,
+// op/ImagerFactoryHelper.cpp:45-87, and the GAFW standard operators of
+// StandardOperators/GPUOperatorsFactoryHelper.cu:79-131), each with the
+// reference's own inputs, outputs, types and parameters:
+//   "PreProcessUVW"        6 inputs (convolution position data int [W][2],
+//                          sums [W][T*T], UVW [3][N], FREQUENCY [N], WEIGHT [N][P],
+//                          FLAGS [N][P]) -> 7 outputs (op/WProjPreProcessUVW.cpp:39-86)
+//   "PreIFFTUVCorrection"  complex [v][u][P] -> complex [P][v][u]
+//   "IFFT2D"               complex_float [..][ny][nx], unnormalised inverse, any size
+//   "ZeroToCentre2DShift"  complex [..][ny][nx] -> same
+//   "StokesConverter"      complex_float [P][v][u] + real_float [P] -> real_float [P][v][u]
+//   "casagridsf"           its three modes (lmPlanePostCorrection, generate)
+//   "SincCorrector"        real_float [..][v][u] -> same
+//   "SortIndex"            (vis/SortIndex.cpp:46-87; not a reference operator)
+//                          antenna1/2 int [N] -> int index [N], real_longint [1] kept
+// Names whose reference contracts consume arrays this build does not
+// produce -- the gridding plan and its consumer (CreateIndexAndReorder,
+// NotSameSupportManipulate, CreateBlockDataIndex, CreateCompressionPlan,
+// CompressVisibilities, ConvolutionGridder, AccumulatedSum, ValueChangeDetect,
+// Sum, GriddedPointsCalculator) and the convolution-function chain
+// (WProjPreFFTConvFunc, multiFFT2D, ConvFunctionSupportFind,
+// ConvFunctionReOrganize, SumConvolutionFunctions, ArraysToVector,
+// SingleMultiply) -- give nullptr, as unknown names do: their work is done
+// by the WImager module and the convolution-function generator below.
 class ImagerFactoryHelper {
 public:
     explicit ImagerFactoryHelper(int device = 0) : device_(device) {}
@@ -195,6 +213,80 @@ public:
 
 private:
     int device_;
+};
+
+// The reference's names for the two convolution-function generators.
+typedef ConvolutionFunctionGenerator WProjectionConvFunction;
+typedef ConvolutionFunctionGenerator OnlyTaperConvFunction;
+
+// A module: inputs bound with setInput, calculate() enqueues the work on the
+// module's stream, getOutput(i) describes output i (device memory owned by
+// the module, valid until the module is destroyed).  GAFW::Module,
+// pil/Module.h:28-46, without an engine: inputs are array descriptors.
+class Module {
+public:
+    virtual ~Module() = default;
+    virtual void reset() = 0;
+    virtual void calculate() = 0;
+    virtual void setInput(int inputNo, const ArrayDescriptor &a) = 0;
+    virtual const ArrayDescriptor &getOutput(int outputNo) const = 0;
+    void setStream(void *stream) { stream_ = stream; }
+
+protected:
+    void *stream_ = nullptr;  // hipStream_t
+};
+
+// The W-projection imager of one channel (comp/WImager.cpp:150-313).
+// Inputs 0 UVW [3][N] real_float, 1 VISIBILITY [N][2P] real_float,
+// 2 FREQUENCY [N] real_float (1/lambda), 3 WEIGHT [N][P] real_float,
+// 4 FLAGS [N][P] real_int.  Outputs 0 Grid [v][u][P] complex_float
+// (conf "Grid.type" = complex_double for complex128), 1 NormalizerValue [P]
+// real_float.  Every calculate() grids the current chunk and adds it to the
+// outputs (the reference's overwrite() chunks, WImager.cpp:271-290), the first
+// one after construction or reset() starting from zero.  conf: psf,
+// weighting, robust.
+class WImager : public Module {
+public:
+    WImager(std::shared_ptr<ConvolutionFunctionGenerator> gen, const Properties &conf, int npol);
+    ~WImager() override;
+    WImager(const WImager &) = delete;
+    WImager &operator=(const WImager &) = delete;
+    void reset() override { calls_ = 0; }
+    void calculate() override;
+    void setInput(int inputNo, const ArrayDescriptor &a) override;
+    const ArrayDescriptor &getOutput(int outputNo) const override;
+    long long griddedRecords() const { return gridded_; }  // Statistic:TotalGriddedRecords
+
+private:
+    std::shared_ptr<ConvolutionFunctionGenerator> gen_;
+    Properties conf_;
+    int npol_;
+    ArrayDescriptor in_[5], out_[2];
+    double *norm64_ = nullptr;  // FP64 normaliser the chunks add into
+    int calls_ = 0;
+    long long gridded_ = 0;
+};
+
+// The image finalizer (comp/ImageFinalizer.cpp:31-86): input 0 UVImage
+// [v][u][P] complex_float or complex_double, input 1 Normalizer [P]
+// real_float or real_double; output 0 real_float [P][v][u] Stokes.  conf:
+// the image.* / wproj.* / taper.* keys (the correction's sampling) and
+// PolarizationType (linear | circular).
+class ImageFinalizer : public Module {
+public:
+    explicit ImageFinalizer(const Properties &conf);
+    ~ImageFinalizer() override;
+    ImageFinalizer(const ImageFinalizer &) = delete;
+    ImageFinalizer &operator=(const ImageFinalizer &) = delete;
+    void reset() override {}
+    void calculate() override;
+    void setInput(int inputNo, const ArrayDescriptor &a) override;
+    const ArrayDescriptor &getOutput(int outputNo) const override;
+
+private:
+    Properties conf_;
+    ArrayDescriptor in_[2], out_;
+    double *norm64_ = nullptr;
 };
 
 // The per-device context the operators share (one submit thread per device).

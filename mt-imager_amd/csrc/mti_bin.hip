@@ -291,6 +291,76 @@ int64_t bin_blocks(int64_t nidx, int nb)
     return ceil_div(nidx, (int64_t)BT_ROWS);
 }
 
+// ---- PreProcessUVW with the reference's seven outputs -----------------------
+
+struct PreRefOut {
+    int4 *data;        // {corner_x, corner_y * columns + corner_x, -(corner_x % S), -(corner_y % S)}
+    int *conv_index;
+    int *to_grid;
+    int *to_compress;
+    int *support;
+    float *take_conj;  // -1 where the record is conjugated (w < 0), else 1
+    float *conv_sum;   // [P][N]
+};
+
+struct PreRefRec {
+    int grid, S, conv_idx;
+    int4 loc;
+};
+
+// The reference's per-record result before its compression step: zeros for a
+// record that is not gridded (all correlations flagged or box off the grid).
+__device__ __forceinline__ PreRefRec pre_ref_record(const VisView &v, const BinGeom &g, int64_t rec, int chan,
+                                                    BinOut *bo)
+{
+    const int64_t row = v.row_index ? (int64_t)v.row_index[rec] : rec;
+    const BinOut b = bin_record(v, g, row, chan);
+    if (bo) *bo = b;
+    PreRefRec r;
+    r.grid = b.to_grid;
+    r.S = b.to_grid ? b.S : 0;
+    r.conv_idx = b.to_grid ? b.conv_idx : 0;
+    r.loc = b.to_grid ? make_int4(b.cx, b.cy * g.nx + b.cx, -(b.cx % b.S), -(b.cy % b.S)) : make_int4(0, 0, 0, 0);
+    return r;
+}
+
+// One thread per record.  A gridded record equal in convolution index and
+// upper-corner position to the record before it (whatever that one's own
+// flags) is marked compressed and not gridded: the rule of
+// op/WProjPreProcessUVW.cu:270-285 and its block-boundary pass :287-324,
+// which together compare every record with its predecessor.
+__global__ void k_pre_ref(VisView v, BinGeom g, int chan, int P, const float *sums, PreRefOut o)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= v.nidx) return;
+    BinOut b;
+    const PreRefRec r = pre_ref_record(v, g, i, chan, &b);
+    const int64_t row = v.row_index ? (int64_t)v.row_index[i] : i;
+    int compress = 0;
+    if (r.grid && i > 0) {
+        const PreRefRec q = pre_ref_record(v, g, i - 1, chan, nullptr);
+        compress = (q.conv_idx == r.conv_idx && q.loc.y == r.loc.y) ? 1 : 0;
+    }
+    o.data[i] = r.loc;
+    o.conv_index[i] = r.conv_idx;
+    o.to_grid[i] = r.grid && !compress;
+    o.to_compress[i] = compress;
+    o.support[i] = r.S;
+    // conjugation follows w < 0 for every record with an unflagged
+    // correlation, gridded or not (:160-175)
+    bool any = false;
+    for (int p = 0; p < P; p++)
+        if (!vis_flag(v, row, chan, p)) any = true;
+    const double w = v.uvw_f32 ? (double)((const float *)v.uvw)[row * v.uvw_rs + 2 * v.uvw_cs]
+                               : ((const double *)v.uvw)[row * v.uvw_rs + 2 * v.uvw_cs];
+    o.take_conj[i] = (any && w < 0.0) ? -1.0f : 1.0f;
+    const double cs = r.grid ? (double)sums[(int64_t)b.plane * g.T * g.T + b.oy * g.T + b.ox] : 0.0;
+    for (int p = 0; p < P; p++) {
+        const double wt = (double)v.weight[row * v.w_rs + p];
+        o.conv_sum[(int64_t)p * v.nidx + i] = vis_flag(v, row, chan, p) ? 0.0f : (float)(cs * wt);
+    }
+}
+
 }  // namespace mti
 
 using namespace mti;
@@ -313,6 +383,48 @@ extern "C" mti_status mti_preprocess(mti_ctx *ctx, const mti_kernels *k, const m
     MTI_CHECK(ctx, bins, "null output");
     hipLaunchKernelGGL(k_bin_only, dim3((unsigned)ceil_div(vw.nidx, 256)), dim3(256), 0, to_stream(stream),
                        vw, g, chan, bins);
+    MTI_HIP(ctx, hipGetLastError());
+    return MTI_OK;
+}
+
+extern "C" mti_status mti_preprocess_reference(mti_ctx *ctx, const mti_preprocess_ref_params *rp,
+                                               const int32_t *convinfo, const float *sums, const mti_vis *v,
+                                               int32_t chan, const mti_preprocess_ref_outputs *out, void *stream)
+{
+    if (!ctx) return MTI_ERR_INVALID;
+    MTI_CHECK(ctx, rp && out, "null parameters or outputs");
+    mti_status r = validate_vis(ctx, v);
+    if (r != MTI_OK) return r;
+    MTI_CHECK(ctx, rp->rows >= 2 && rp->columns >= 2, "uvImage.rows/uvImage.columns must be at least 2");
+    MTI_CHECK(ctx, rp->u_increment > 0.0 && rp->v_increment > 0.0, "uvImage.u_increment/v_increment must be positive");
+    MTI_CHECK(ctx, rp->sampling >= 1, "ConvolutionFunction.sampling must be at least 1");
+    MTI_CHECK(ctx, rp->wplanes >= 1, "the convolution-function data needs at least one plane");
+    MTI_CHECK(ctx, chan >= 0 && (v->inv_lambda_per_row || chan < v->nchan), "channel out of range");
+    VisView vw = make_view(v);
+    if (vw.nidx == 0) return MTI_OK;
+    MTI_CHECK(ctx, convinfo && sums, "null convolution-function data");
+    MTI_CHECK(ctx, out->gridding_data && out->conv_index && out->to_grid && out->to_compress && out->support &&
+                       out->take_conjugate && out->conv_sum,
+              "null output");
+    MTI_CHECK(ctx, vw.nidx < (int64_t(1) << 31), "more than 2^31-1 records");
+    MTI_HIP(ctx, hipSetDevice(ctx->device));
+    BinGeom g;
+    g.nx = rp->columns;
+    g.ny = rp->rows;
+    g.du = rp->u_increment;
+    g.dv = rp->v_increment;
+    g.dw = rp->wsquareincrement;
+    g.hx = (double)(rp->columns / 2);
+    g.hy = (double)(rp->rows / 2);
+    g.mode = MTI_MODE_WPROJ;
+    g.nplanes = rp->wplanes;
+    g.s = rp->sampling;
+    g.T = (rp->sampling % 2) ? rp->sampling : rp->sampling + 1;
+    g.convinfo = (const int2 *)convinfo;
+    PreRefOut o{(int4 *)out->gridding_data, out->conv_index, out->to_grid, out->to_compress, out->support,
+                out->take_conjugate, out->conv_sum};
+    hipLaunchKernelGGL(k_pre_ref, dim3((unsigned)ceil_div(vw.nidx, 256)), dim3(256), 0, to_stream(stream), vw, g,
+                       chan, v->npol, sums, o);
     MTI_HIP(ctx, hipGetLastError());
     return MTI_OK;
 }

@@ -548,6 +548,10 @@ extern "C" mti_status mti_kernels_supports(const mti_kernels *k, int32_t *suppor
 
 extern "C" const void *mti_kernels_table(const mti_kernels *k) { return k ? k->d_table : nullptr; }
 extern "C" const float *mti_kernels_sums(const mti_kernels *k) { return k ? k->d_sums : nullptr; }
+extern "C" const int32_t *mti_kernels_position(const mti_kernels *k)
+{
+    return k ? (const int32_t *)k->d_convinfo : nullptr;
+}
 
 extern "C" void mti_kernels_free(mti_kernels *k)
 {

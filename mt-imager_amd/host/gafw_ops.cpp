@@ -278,107 +278,80 @@ private:
     int64_t kept_ = 0;
 };
 
-// PreProcessUVW (op/WProjPreProcessUVW.cu:111-324): WImager inputs -> int [N][8]
-// {to_grid, plane, support, corner_x, corner_y, off_x, off_y, conj}.
+// PreProcessUVW (op/WProjPreProcessUVW.cpp:39-86): inputs 0 convolution
+// position data int [W][2] {support, base}, 1 sums real_float [W][T*T],
+// 2 UVW [3][N], 3 FREQUENCY [N], 4 WEIGHT [N][P], 5 FLAGS [N][P]; outputs
+// 0 GriddingData int [N][4], 1 ConvolutionIndex, 2 ToGrid, 3 ToCompress,
+// 4 Support int [N], 5 TakeConjugate real_float [N], 6 ConvolutionFunctionSum
+// real_float [P][N]; parameters ConvolutionFunction.sampling,
+// ConvolutionFunction.wsquareincrement, uvImage.{u_increment, v_increment,
+// rows, columns}.
 class PreProcessUVWOp : public ArrayOperator {
 public:
     using ArrayOperator::ArrayOperator;
-    void validate(const std::vector<ArrayDescriptor> &in, std::vector<ArrayDescriptor> &out, const Properties &,
+    void validate(const std::vector<ArrayDescriptor> &in, std::vector<ArrayDescriptor> &out, const Properties &p,
                   ValidationData &) override
     {
-        const ChunkShape c = check_chunk_inputs(in, name_, false);
-        expect(conv_ != nullptr, name_, "no convolution function set");
-        out.resize(1);
-        out[0].dim = ArrayDimensions({c.N, 8});
+        expect(in.size() == 6, name_,
+               "This operator expects six inputs which are Convolution Function support data and UVW entries");
+        for (const char *k : {"ConvolutionFunction.sampling", "uvImage.rows", "uvImage.columns"}) p.getIntProperty(k);
+        for (const char *k : {"ConvolutionFunction.wsquareincrement", "uvImage.u_increment", "uvImage.v_increment"})
+            p.getDoubleProperty(k);
+        expect(in[0].dim.getNoOfDimensions() == 2 && in[0].dim.getX() == 2 && in[0].type == real_int, name_,
+               "input 0 (convolution position data) must be int [W][2]");
+        expect(in[1].type == real_float && in[1].dim.getNoOfDimensions() == 2 && in[1].dim.getY() == in[0].dim.getY(),
+               name_, "input 1 (convolution sums) must be real_float [W][T*T]");
+        const ChunkShape c = check_chunk_inputs(reorder(in), name_, false);
+        out.resize(7);
+        out[0].dim = ArrayDimensions({c.N, 4});
         out[0].type = real_int;
-    }
-    void submitToGPU(SubmissionData &d) override
-    {
-        const ChunkShape c{d.inputs[0].dim.getX(), (int)d.inputs[3].dim.getX()};
-        mti_vis v = chunk_vis(d.inputs, c);
-        mti_ctx *ctx = device_context(current_device());
-        throw_status(mti_preprocess(ctx, conv_->kernels(), &conv_->imageParams(), &v, 0, (int32_t *)d.outputs[0].pointer,
-                                    d.stream),
-                     ctx, name_);
-    }
-};
-
-// The WImager gridding DAG (comp/WImager.cpp:167-234).
-// Output 0 Grid complex_float [v][u][P] (or complex_double), output 1
-// NormalizerValue [P] real_float (or real_double).  Properties: psf,
-// initialize_output (default true; false adds to the outputs like the
-// reference's overwrite() chunks), weighting, robust.
-class ConvolutionGridderOp : public ArrayOperator {
-public:
-    using ArrayOperator::ArrayOperator;
-    void validate(const std::vector<ArrayDescriptor> &in, std::vector<ArrayDescriptor> &out, const Properties &p,
-                  ValidationData &vd) override
-    {
-        const ChunkShape c = check_chunk_inputs(in, name_, true);
-        expect(conv_ != nullptr, name_, "no convolution function set");
-        const mti_image_params ip = conv_->imageParams();
-        weighting_of(p);
-        out.resize(2);
-        const StoreType gt = out[0].type == complex_double ? complex_double : complex_float;
-        out[0].dim = ArrayDimensions({ip.ny, ip.nx, c.P});
-        out[0].type = gt;
-        const StoreType nt = out[1].type == real_double ? real_double : real_float;
-        out[1].dim = ArrayDimensions({c.P});
-        out[1].type = nt;
-        vd.bufferMode = BufferMode::Normal;
-        // FP64 normaliser staging + the gridding workspace
-        expect(conv_->kernels() != nullptr, name_, "convolution function not calculated");
-        const mti_grid_opts o = opts(p, gt);
-        const mti_vis v = chunk_sizes(c);
-        size_t ws = 0;
-        const mti_kernel_info ki = info_of(conv_->kernels());
-        vd.bufferSize = NORM_STAGE + query(mti_grid_workspace_size(&ki, &ip, &v, &o, &ws), ws, name_);
-    }
-    void submitToGPU(SubmissionData &d) override
-    {
-        const ChunkShape c{d.inputs[0].dim.getX(), (int)d.inputs[3].dim.getX()};
-        mti_vis v = chunk_vis(d.inputs, c);
-        const mti_grid_opts o = opts(d.params, d.outputs[0].type);
-        hipStream_t st = (hipStream_t)d.stream;
-        mti_ctx *ctx = device_context(current_device());
-        size_t ws = 0;
-        const mti_kernel_info ki = info_of(conv_->kernels());
-        query(mti_grid_workspace_size(&ki, &conv_->imageParams(), &v, &o, &ws), ws, name_);
-        WorkspaceScope wsc(ctx, d, ws, name_);
-        double *norm = (double *)d.outputs[1].pointer;
-        const bool f32_norm = d.outputs[1].type != real_double;
-        if (f32_norm) {
-            norm = (double *)d.bufferGPUPointer;
-            if (o.accumulate)
-                hipLaunchKernelGGL(k_f32_to_f64, dim3(1), dim3(64), 0, st, (const float *)d.outputs[1].pointer, norm,
-                                   c.P);
+        for (int i = 1; i <= 4; i++) {
+            out[i].dim = ArrayDimensions({c.N});
+            out[i].type = real_int;
         }
-        throw_status(mti_grid(ctx, conv_->kernels(), &conv_->imageParams(), &v, &o, d.outputs[0].pointer, norm,
-                              nullptr, d.stream),
+        out[5].dim = ArrayDimensions({c.N});
+        out[5].type = real_float;
+        out[6].dim = ArrayDimensions({(long long)c.P, c.N});
+        out[6].type = real_float;
+    }
+    void submitToGPU(SubmissionData &d) override
+    {
+        const std::vector<ArrayDescriptor> ch = reorder(d.inputs);
+        const ChunkShape c{ch[0].dim.getX(), (int)ch[3].dim.getX()};
+        mti_vis v = chunk_vis(ch, c);
+        mti_preprocess_ref_params rp{};
+        rp.rows = d.params.getIntProperty("uvImage.rows");
+        rp.columns = d.params.getIntProperty("uvImage.columns");
+        rp.u_increment = d.params.getDoubleProperty("uvImage.u_increment");
+        rp.v_increment = d.params.getDoubleProperty("uvImage.v_increment");
+        rp.sampling = d.params.getIntProperty("ConvolutionFunction.sampling");
+        rp.wsquareincrement = d.params.getDoubleProperty("ConvolutionFunction.wsquareincrement");
+        rp.wplanes = (int)d.inputs[0].dim.getY();
+        mti_preprocess_ref_outputs o{(int32_t *)d.outputs[0].pointer, (int32_t *)d.outputs[1].pointer,
+                                     (int32_t *)d.outputs[2].pointer, (int32_t *)d.outputs[3].pointer,
+                                     (int32_t *)d.outputs[4].pointer, (float *)d.outputs[5].pointer,
+                                     (float *)d.outputs[6].pointer};
+        mti_ctx *ctx = device_context(current_device());
+        throw_status(mti_preprocess_reference(ctx, &rp, (const int32_t *)d.inputs[0].pointer,
+                                              (const float *)d.inputs[1].pointer, &v, 0, &o, d.stream),
                      ctx, name_);
-        if (f32_norm)
-            hipLaunchKernelGGL(k_f64_to_f32, dim3(1), dim3(64), 0, st, norm, (float *)d.outputs[1].pointer, c.P);
-        hip_check(hipGetLastError(), "normaliser conversion");
     }
 
 private:
-    static mti_grid_opts opts(const Properties &p, StoreType grid_type)
+    // the WImager chunk order (UVW, VIS, FREQUENCY, WEIGHT, FLAGS) of inputs 2-5
+    static std::vector<ArrayDescriptor> reorder(const std::vector<ArrayDescriptor> &in)
     {
-        mti_grid_opts o{};
-        o.chan0 = 0;
-        o.nchan = 1;
-        o.accumulate = p.getBoolProperty("initialize_output", true) ? 0 : 1;
-        o.psf = p.getBoolProperty("psf", false) ? 1 : 0;
-        o.weighting = weighting_of(p);
-        o.robust = p.getDoubleProperty("robust", 0.0);
-        o.grid_layout = MTI_GRID_VUP;
-        o.grid_precision = grid_type == complex_double ? MTI_GRID_C128 : MTI_GRID_C64;
-        return o;
+        std::vector<ArrayDescriptor> r(5);
+        r[0] = in[2];
+        r[2] = in[3];
+        r[3] = in[4];
+        r[4] = in[5];
+        return r;
     }
 };
 
 // IFFT2D (std/IFFT2D.cpp:86-111): unnormalised inverse C2C over the last two
-// dimensions of a complex_float array; out-of-place calls copy first.
+// dimensions of a complex_float array, any size; out-of-place calls copy first.
 class IFFT2DOp : public ArrayOperator {
 public:
     using ArrayOperator::ArrayOperator;
@@ -389,8 +362,7 @@ public:
         expect(in[0].type == complex_float, name_, "input must be complex_float");
         expect(in[0].dim.getNoOfDimensions() >= 2, name_, "input must be at least 2-D");
         const long long nx = in[0].dim.getX(), ny = in[0].dim.getY();
-        auto pow2 = [](long long n) { return n >= 2 && n <= 8192 && (n & (n - 1)) == 0; };
-        expect(pow2(nx) && pow2(ny), name_, "dimensions must be powers of two in [2, 8192]");
+        expect(nx >= 1 && ny >= 1 && nx <= 32768 && ny <= 32768, name_, "dimensions must be in [1, 32768]");
         out.resize(1);
         out[0].dim = in[0].dim;
         out[0].type = complex_float;
@@ -416,56 +388,182 @@ public:
     }
 };
 
-// ImageFinalizer (comp/ImageFinalizer.cpp:31-44): UVImage [v][u][P] (complex_float or
-// complex_double) + NormalizerValue [P] (real_float or real_double) ->
-// real_float [P][v][u]; property PolarizationType (linear | circular).
-class ImageFinalizerOp : public ArrayOperator {
+void op_status(mti_status s, const std::string &what)
+{
+    if (s == MTI_ERR_INVALID) throw ValidationException(what + ": invalid arguments");
+    if (s != MTI_OK) throw GpuException(what + ": launch failed");
+}
+
+// PreIFFTUVCorrection (op/PreIFFTUVCorrection.cpp:36-62): one 3-D complex input
+// [v][u][P] -> [P][v][u] of the same type.
+class PreIFFTUVCorrectionOp : public ArrayOperator {
 public:
     using ArrayOperator::ArrayOperator;
-    void validate(const std::vector<ArrayDescriptor> &in, std::vector<ArrayDescriptor> &out, const Properties &p,
-                  ValidationData &vd) override
+    void validate(const std::vector<ArrayDescriptor> &in, std::vector<ArrayDescriptor> &out, const Properties &,
+                  ValidationData &) override
     {
-        expect(in.size() == 2, name_, "needs UVImage and NormalizerValue");
-        expect(in[0].type == complex_float || in[0].type == complex_double, name_, "UVImage must be complex");
-        expect(in[0].dim.getNoOfDimensions() == 3, name_, "UVImage must be [v][u][P], got " + in[0].dim.str());
-        const long long P = in[0].dim.getX();
-        expect(P == 1 || P == 2 || P == 4, name_, "1, 2 or 4 polarisations");
-        expect(in[1].dim.getNoOfDimensions() == 1 && in[1].dim.getX() == P, name_, "NormalizerValue must be [P]");
-        expect(in[1].type == real_float || in[1].type == real_double, name_, "NormalizerValue must be real");
-        const std::string pt = p.getStringProperty("PolarizationType", "linear");
-        expect(pt == "linear" || pt == "circular", name_, "PolarizationType must be linear or circular");
+        expect(in.size() == 1, name_, "Only one input is supported");
+        expect(in[0].dim.getNoOfDimensions() == 3, name_, "Expected a 3D Array (lengthxwidthxpolarization");
+        expect(in[0].type == complex_float || in[0].type == complex_double, name_, "input must be complex");
         out.resize(1);
-        out[0].dim = ArrayDimensions({P, in[0].dim.getZ(), in[0].dim.getY()});
-        out[0].type = real_float;
-        mti_image_params ip = conv_ ? conv_->imageParams() : image_params_from(p);
-        ip.ny = (int)in[0].dim.getZ();
-        ip.nx = (int)in[0].dim.getY();
-        size_t ws = 0;
-        vd.bufferSize = NORM_STAGE + query(mti_finalize_workspace_size(&ip, (int)P, 1, &ws), ws, name_);
+        out[0].dim = ArrayDimensions({in[0].dim.getX(), in[0].dim.getZ(), in[0].dim.getY()});
+        out[0].type = in[0].type;
     }
     void submitToGPU(SubmissionData &d) override
     {
-        mti_image_params ip = conv_ ? conv_->imageParams() : image_params_from(d.params);
-        const int P = (int)d.inputs[0].dim.getX();
-        ip.ny = (int)d.inputs[0].dim.getZ();
-        ip.nx = (int)d.inputs[0].dim.getY();
-        mti_ctx *ctx = device_context(current_device());
-        size_t ws = 0;
-        query(mti_finalize_workspace_size(&ip, P, 1, &ws), ws, name_);
-        WorkspaceScope wsc(ctx, d, ws, name_);
-        const double *norm = (const double *)d.inputs[1].pointer;
-        if (d.inputs[1].type == real_float) {
-            hipLaunchKernelGGL(k_f32_to_f64, dim3(1), dim3(64), 0, (hipStream_t)d.stream,
-                               (const float *)d.inputs[1].pointer, (double *)d.bufferGPUPointer, P);
-            hip_check(hipGetLastError(), "normaliser conversion");
-            norm = (const double *)d.bufferGPUPointer;
+        const ArrayDimensions &dim = d.inputs[0].dim;
+        op_status(mti_uv_reorder(d.inputs[0].pointer, d.outputs[0].pointer, (int)dim.getZ(), (int)dim.getY(),
+                                 (int)dim.getX(), d.inputs[0].type == complex_double, d.stream),
+                  name_);
+    }
+};
+
+// ZeroToCentre2DShift (StandardOperators/ZeroToCentre2DShift.cpp:36-62): a
+// complex array of two or more dimensions, every [y][x] plane shifted.
+class ZeroToCentreOp : public ArrayOperator {
+public:
+    using ArrayOperator::ArrayOperator;
+    void validate(const std::vector<ArrayDescriptor> &in, std::vector<ArrayDescriptor> &out, const Properties &,
+                  ValidationData &) override
+    {
+        expect(in.size() == 1, name_, "Only 1 input is supported");
+        expect(in[0].type == complex_float || in[0].type == complex_double, name_,
+               in[0].type == real_float ? "At current FFT2DShift of a real array is not supported"
+                                        : "Unsupported store type of input");
+        expect(in[0].dim.getNoOfDimensions() >= 2, name_, "FFT2DShift can work on arrays of 2 dimensions or above!");
+        out.resize(1);
+        out[0].dim = in[0].dim;
+        out[0].type = in[0].type;
+    }
+    void submitToGPU(SubmissionData &d) override
+    {
+        const ArrayDimensions &dim = d.inputs[0].dim;
+        const long long ny = dim.getY(), nx = dim.getX();
+        op_status(mti_centre_shift(d.inputs[0].pointer, d.outputs[0].pointer, dim.total() / (nx * ny), (int)ny,
+                                   (int)nx, d.inputs[0].type == complex_double, d.stream),
+                  name_);
+    }
+};
+
+// StokesConverter (op/PostIFFTStokesConverter.cpp:37-67): input 0 complex_float
+// [P][v][u], input 1 the real_float normaliser (P elements); parameter
+// PolarizationType 0 (linear) or 1 (circular); output real_float [P][v][u].
+class StokesConverterOp : public ArrayOperator {
+public:
+    using ArrayOperator::ArrayOperator;
+    void validate(const std::vector<ArrayDescriptor> &in, std::vector<ArrayDescriptor> &out, const Properties &p,
+                  ValidationData &) override
+    {
+        expect(in.size() == 2, name_, "Two inputs are expected");
+        const int pt = p.getIntProperty("PolarizationType");
+        expect(pt == 0 || pt == 1, name_,
+               "PolarizationType parameter is expected to be 0 (linear Polarisation) or 1(Circular Polarisation)");
+        expect(in[0].dim.getNoOfDimensions() == 3, name_, "Expected a 3D Array (polarizationxlengthxwidth)");
+        expect(in[0].type == complex_float, name_, "input 0 must be complex_float");
+        expect(in[0].dim.getZ() == in[1].dim.total(), name_,
+               "The No Of polarizations in imager and normalizer are not the same");
+        expect(in[1].type == real_float, name_, "Normaliser input type expected to be real_float");
+        const long long P = in[0].dim.getZ();
+        expect(P == 1 || P == 2 || P == 4, name_, "1, 2 or 4 polarisations");
+        out.resize(1);
+        out[0].dim = in[0].dim;
+        out[0].type = real_float;
+    }
+    void submitToGPU(SubmissionData &d) override
+    {
+        const ArrayDimensions &dim = d.inputs[0].dim;
+        op_status(mti_stokes((const float *)d.inputs[0].pointer, (const float *)d.inputs[1].pointer,
+                             (float *)d.outputs[0].pointer, (int)dim.getZ(), (int)dim.getY(), (int)dim.getX(),
+                             d.params.getIntProperty("PolarizationType") == 1 ? MTI_POL_CIRCULAR : MTI_POL_LINEAR,
+                             d.stream),
+                  name_);
+    }
+};
+
+// casagridsf (op/CasaGridSF.cpp:38-128).  lmPlanePostCorrection = false,
+// generate = true: no input, the taper-only gridding function [dim][dim]
+// (complexOutput, double-precision select the type); lmPlanePostCorrection
+// = true, generate = false: divide the real input by the taper;
+// lmPlanePostCorrection = true, generate = true: the taper grid [ny][nx].
+class CasaGridSFOp : public ArrayOperator {
+public:
+    using ArrayOperator::ArrayOperator;
+    void validate(const std::vector<ArrayDescriptor> &in, std::vector<ArrayDescriptor> &out, const Properties &p,
+                  ValidationData &) override
+    {
+        const int support = p.getIntProperty("support"), sampling = p.getIntProperty("sampling");
+        const bool post = p.getBoolProperty("lmPlanePostCorrection"), gen = p.getBoolProperty("generate");
+        const bool dbl = p.getBoolProperty("double-precision", false);
+        expect(support >= 1, name_, "support requested is less then 1");
+        expect(support % 2 == 1, name_, "Support is expected to be odd");
+        expect(sampling >= 1, name_, "Sampling must be set to at least 1");
+        out.resize(1);
+        if (!post) {
+            expect(gen, name_, "Only generate=true is supported for lmPlanePostCorrection=false");
+            expect(in.empty(), name_, "This operator does not take any inputs but validation found inputs");
+            const bool cplx = p.getBoolProperty("complexOutput", false);
+            const long long dim = (sampling % 2) ? (long long)support * sampling : (long long)support * sampling + 1;
+            out[0].dim = ArrayDimensions({dim, dim});
+            out[0].type = cplx ? (dbl ? complex_double : complex_float) : (dbl ? real_double : real_float);
+        } else if (!gen) {
+            expect(in.size() == 1, name_, "An input is expected as to correct");
+            expect(in[0].dim.getNoOfDimensions() >= 2, name_, "The input is expected to be at least 2 dimensional");
+            expect(in[0].type == real_float || in[0].type == real_double, name_,
+                   "Input is expected to be of real_float or real_double type");
+            out[0].dim = in[0].dim;
+            out[0].type = in[0].type;
+        } else {
+            expect(in.empty(), name_, "No input is expected");
+            const int nx = p.getIntProperty("nx"), ny = p.getIntProperty("ny");
+            expect(nx >= 1 && ny >= 1, name_, "Parameters nx or ny must be at least 1");
+            out[0].dim = ArrayDimensions({ny, nx});
+            out[0].type = dbl ? real_double : real_float;
         }
-        const int pt = d.params.getStringProperty("PolarizationType", "linear") == "circular" ? MTI_POL_CIRCULAR
-                                                                                              : MTI_POL_LINEAR;
-        throw_status(mti_finalize(ctx, &ip, P, pt, 1, MTI_GRID_VUP,
-                                  d.inputs[0].type == complex_double ? MTI_GRID_C128 : MTI_GRID_C64,
-                                  d.inputs[0].pointer, norm, (float *)d.outputs[0].pointer, d.stream),
-                     ctx, name_);
+    }
+    void submitToGPU(SubmissionData &d) override
+    {
+        const bool post = d.params.getBoolProperty("lmPlanePostCorrection"), gen = d.params.getBoolProperty("generate");
+        const ArrayDescriptor &o = d.outputs[0];
+        const bool f64 = o.type == real_double || o.type == complex_double;
+        if (!post) {
+            op_status(mti_taper_kernel(o.pointer, d.params.getIntProperty("support"),
+                                       d.params.getIntProperty("sampling"),
+                                       o.type == complex_float || o.type == complex_double, f64, d.stream),
+                      name_);
+            return;
+        }
+        const long long ny = o.dim.getY(), nx = o.dim.getX();
+        op_status(mti_taper_correct(gen ? nullptr : d.inputs[0].pointer, o.pointer, o.dim.total() / (nx * ny),
+                                    (int)ny, (int)nx, f64, d.stream),
+                  name_);
+    }
+};
+
+// SincCorrector (op/SincCorrector.cpp:37-50): real_float input of 2 or 3
+// dimensions, parameter sampling.
+class SincCorrectorOp : public ArrayOperator {
+public:
+    using ArrayOperator::ArrayOperator;
+    void validate(const std::vector<ArrayDescriptor> &in, std::vector<ArrayDescriptor> &out, const Properties &p,
+                  ValidationData &) override
+    {
+        expect(in.size() == 1, name_, "This operator expects one input");
+        const int nd = in[0].dim.getNoOfDimensions();
+        expect(nd == 2 || nd == 3, name_, "Input 0 is expected to be of 2 or 3  dimensions");
+        expect(in[0].type == real_float, name_, "This operator works only on real_floats");
+        p.getIntProperty("sampling");
+        out.resize(1);
+        out[0].dim = in[0].dim;
+        out[0].type = real_float;
+    }
+    void submitToGPU(SubmissionData &d) override
+    {
+        const ArrayDimensions &dim = d.inputs[0].dim;
+        const long long ny = dim.getY(), nx = dim.getX();
+        op_status(mti_sinc_correct((const float *)d.inputs[0].pointer, (float *)d.outputs[0].pointer,
+                                   dim.total() / (nx * ny), (int)ny, (int)nx, d.params.getIntProperty("sampling"),
+                                   d.stream),
+                  name_);
     }
 };
 
@@ -541,6 +639,10 @@ static const mti_kernels *need(const mti_kernels *k)
 
 const void *ConvolutionFunctionGenerator::getConvFunction() const { return mti_kernels_table(need(k_)); }
 const float *ConvolutionFunctionGenerator::getConvFunctionSumData() const { return mti_kernels_sums(need(k_)); }
+const int32_t *ConvolutionFunctionGenerator::getConvFunctionPositionDevice() const
+{
+    return mti_kernels_position(need(k_));
+}
 
 std::vector<std::pair<int, long long>> ConvolutionFunctionGenerator::getConvFunctionPositionData() const
 {
@@ -574,12 +676,158 @@ std::unique_ptr<ArrayOperator> ImagerFactoryHelper::createOperator(const std::st
                                                                   const std::string &name) const
 {
     (void)device_;
-    if (name == "SortIndex") return std::unique_ptr<ArrayOperator>(new SortIndexOp(nickname, name));
-    if (name == "PreProcessUVW") return std::unique_ptr<ArrayOperator>(new PreProcessUVWOp(nickname, name));
-    if (name == "ConvolutionGridder") return std::unique_ptr<ArrayOperator>(new ConvolutionGridderOp(nickname, name));
-    if (name == "IFFT2D") return std::unique_ptr<ArrayOperator>(new IFFT2DOp(nickname, name));
-    if (name == "ImageFinalizer") return std::unique_ptr<ArrayOperator>(new ImageFinalizerOp(nickname, name));
+    auto make = [&](ArrayOperator *o) { return std::unique_ptr<ArrayOperator>(o); };
+    if (name == "PreProcessUVW") return make(new PreProcessUVWOp(nickname, name));
+    if (name == "PreIFFTUVCorrection") return make(new PreIFFTUVCorrectionOp(nickname, name));
+    if (name == "IFFT2D") return make(new IFFT2DOp(nickname, name));
+    if (name == "ZeroToCentre2DShift") return make(new ZeroToCentreOp(nickname, name));
+    if (name == "StokesConverter") return make(new StokesConverterOp(nickname, name));
+    if (name == "casagridsf") return make(new CasaGridSFOp(nickname, name));
+    if (name == "SincCorrector") return make(new SincCorrectorOp(nickname, name));
+    if (name == "SortIndex") return make(new SortIndexOp(nickname, name));
     return nullptr;
+}
+
+// ---- modules --------------------------------------------------------------------
+
+namespace {
+
+void *device_alloc(size_t bytes, const char *what)
+{
+    void *p = nullptr;
+    hip_check(hipMalloc(&p, std::max<size_t>(bytes, 256)), what);
+    return p;
+}
+
+}  // namespace
+
+WImager::WImager(std::shared_ptr<ConvolutionFunctionGenerator> gen, const Properties &conf, int npol)
+    : gen_(std::move(gen)), conf_(conf), npol_(npol)
+{
+    if (!gen_) throw ValidationException("WImager: no convolution-function generator");
+    if (!(npol == 1 || npol == 2 || npol == 4))
+        throw ValidationException("No of polarisations requested is not supported, it must be 1,2 or 4");
+    const mti_image_params &ip = gen_->imageParams();
+    if (ip.nx < 1 || ip.ny < 1) throw ValidationException("v or u pixels do not make sense");
+    weighting_of(conf_);
+    const std::string gt = conf_.getStringProperty("Grid.type", "complex_float");
+    if (gt != "complex_float" && gt != "complex_double")
+        throw ValidationException("Grid.type must be complex_float or complex_double: " + gt);
+    hip_check(hipSetDevice(gen_->device()), "hipSetDevice");
+    out_[0].dim = ArrayDimensions({ip.ny, ip.nx, npol});
+    out_[0].type = gt == "complex_double" ? complex_double : complex_float;
+    out_[0].pointer = device_alloc((size_t)out_[0].dim.total() * (gt == "complex_double" ? 16 : 8), "WImager Grid");
+    out_[1].dim = ArrayDimensions({npol});
+    out_[1].type = real_float;
+    out_[1].pointer = device_alloc(npol * sizeof(float), "WImager NormalizerValue");
+    norm64_ = (double *)device_alloc(npol * sizeof(double), "WImager normaliser");
+}
+
+WImager::~WImager()
+{
+    for (void *p : {out_[0].pointer, out_[1].pointer, (void *)norm64_})
+        if (p) (void)hipFree(p);
+}
+
+void WImager::setInput(int inputNo, const ArrayDescriptor &a)
+{
+    if (inputNo < 0 || inputNo > 4) throw ValidationException("WImager: inputs are 0-4 (UVW, VISIBILITY, FREQUENCY, WEIGHT, FLAGS)");
+    in_[inputNo] = a;
+}
+
+const ArrayDescriptor &WImager::getOutput(int outputNo) const
+{
+    if (outputNo < 0 || outputNo > 1) throw ValidationException("WImager: outputs are 0 (Grid) and 1 (NormalizerValue)");
+    return out_[outputNo];
+}
+
+void WImager::calculate()
+{
+    if (!gen_->kernels()) throw ValidationException("WImager: calculateConvFunction() has not been called");
+    const std::vector<ArrayDescriptor> in(in_, in_ + 5);
+    const ChunkShape c = check_chunk_inputs(in, "WImager", true);
+    if (c.P != npol_) throw ValidationException("WImager: chunk has " + std::to_string(c.P) + " polarisations, module " +
+                                                std::to_string(npol_));
+    hip_check(hipSetDevice(gen_->device()), "hipSetDevice");
+    mti_vis v = chunk_vis(in, c);
+    mti_grid_opts o{};
+    o.chan0 = 0;
+    o.nchan = 1;
+    o.accumulate = calls_ > 0 ? 1 : 0;
+    o.psf = conf_.getBoolProperty("psf", false) ? 1 : 0;
+    o.weighting = weighting_of(conf_);
+    o.robust = conf_.getDoubleProperty("robust", 0.0);
+    o.grid_layout = MTI_GRID_VUP;
+    o.grid_precision = out_[0].type == complex_double ? MTI_GRID_C128 : MTI_GRID_C64;
+    mti_ctx *ctx = device_context(gen_->device());
+    mti_grid_stats st{};
+    throw_status(mti_grid(ctx, gen_->kernels(), &gen_->imageParams(), &v, &o, out_[0].pointer, norm64_, &st, stream_),
+                 ctx, "WImager");
+    throw_status(mti_normalizer_f32(norm64_, (float *)out_[1].pointer, npol_, stream_) == MTI_OK ? MTI_OK : MTI_ERR_HIP,
+                 ctx, "WImager normaliser");
+    gridded_ += st.gridded_records;
+    calls_++;
+}
+
+ImageFinalizer::ImageFinalizer(const Properties &conf) : conf_(conf)
+{
+    image_params_from(conf_);
+    const std::string pt = conf_.getStringProperty("PolarizationType", "linear");
+    if (pt != "linear" && pt != "circular") throw ValidationException("PolarizationType must be linear or circular");
+    norm64_ = (double *)device_alloc(4 * sizeof(double), "ImageFinalizer normaliser");
+}
+
+ImageFinalizer::~ImageFinalizer()
+{
+    for (void *p : {out_.pointer, (void *)norm64_})
+        if (p) (void)hipFree(p);
+}
+
+void ImageFinalizer::setInput(int inputNo, const ArrayDescriptor &a)
+{
+    if (inputNo < 0 || inputNo > 1) throw ValidationException("ImageFinalizer: inputs are 0 (UVImage) and 1 (Normalizer)");
+    in_[inputNo] = a;
+}
+
+const ArrayDescriptor &ImageFinalizer::getOutput(int outputNo) const
+{
+    if (outputNo != 0) throw ValidationException("ImageFinalizer: one output");
+    return out_;
+}
+
+void ImageFinalizer::calculate()
+{
+    const ArrayDescriptor &g = in_[0], &n = in_[1];
+    expect(g.type == complex_float || g.type == complex_double, "ImageFinalizer", "UVImage must be complex");
+    expect(g.dim.getNoOfDimensions() == 3, "ImageFinalizer", "UVImage must be [v][u][P], got " + g.dim.str());
+    const int P = (int)g.dim.getX();
+    expect(P == 1 || P == 2 || P == 4, "ImageFinalizer", "1, 2 or 4 polarisations");
+    expect(n.dim.total() == P && (n.type == real_float || n.type == real_double), "ImageFinalizer",
+           "Normalizer must be real [P]");
+    mti_image_params ip = image_params_from(conf_);
+    ip.ny = (int)g.dim.getZ();
+    ip.nx = (int)g.dim.getY();
+    const ArrayDimensions od({P, ip.ny, ip.nx});
+    if (!out_.pointer || !(out_.dim == od)) {
+        if (out_.pointer) hip_check(hipFree(out_.pointer), "hipFree");
+        out_.pointer = device_alloc((size_t)od.total() * sizeof(float), "ImageFinalizer image");
+        out_.dim = od;
+        out_.type = real_float;
+    }
+    hipStream_t st = (hipStream_t)stream_;
+    const double *norm = (const double *)n.pointer;
+    if (n.type == real_float) {
+        hipLaunchKernelGGL(k_f32_to_f64, dim3(1), dim3(64), 0, st, (const float *)n.pointer, norm64_, P);
+        hip_check(hipGetLastError(), "normaliser conversion");
+        norm = norm64_;
+    }
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    mti_ctx *ctx = device_context(dev);
+    const int pt = conf_.getStringProperty("PolarizationType", "linear") == "circular" ? MTI_POL_CIRCULAR : MTI_POL_LINEAR;
+    throw_status(mti_finalize(ctx, &ip, P, pt, 1, MTI_GRID_VUP, g.type == complex_double ? MTI_GRID_C128 : MTI_GRID_C64,
+                              g.pointer, norm, (float *)out_.pointer, stream_),
+                 ctx, "ImageFinalizer");
 }
 
 }  // namespace gafw

@@ -76,6 +76,22 @@ class GridStats(ctypes.Structure):
                 ("work_items", ctypes.c_int64), ("flushes", ctypes.c_int64)]
 
 
+class PreprocessRefParams(ctypes.Structure):
+    """PreProcessUVW parameters (op/WProjPreProcessUVW.cpp:49-56)."""
+    _fields_ = [("rows", ctypes.c_int32), ("columns", ctypes.c_int32),
+                ("u_increment", ctypes.c_double), ("v_increment", ctypes.c_double),
+                ("sampling", ctypes.c_int32), ("wsquareincrement", ctypes.c_double),
+                ("wplanes", ctypes.c_int32)]
+
+
+class PreprocessRefOutputs(ctypes.Structure):
+    """PreProcessUVW outputs 0-6 (op/WProjPreProcessUVW.cpp:70-84)."""
+    _fields_ = [("gridding_data", ctypes.c_void_p), ("conv_index", ctypes.c_void_p),
+                ("to_grid", ctypes.c_void_p), ("to_compress", ctypes.c_void_p),
+                ("support", ctypes.c_void_p), ("take_conjugate", ctypes.c_void_p),
+                ("conv_sum", ctypes.c_void_p)]
+
+
 class Tuning(ctypes.Structure):
     _fields_ = [("unit_pairs", ctypes.c_int32), ("whole_columns_ny", ctypes.c_int32), ("xcd_chunk", ctypes.c_int32)]
 
@@ -87,7 +103,7 @@ STAGES = ["sort_rows", "kernels", "bin", "sort_records", "plan", "grid", "fft_ro
 EXPORTS = ["mti_version", "mti_create", "mti_destroy", "mti_last_error",
            "mti_set_profiling", "mti_stage_times",
            "mti_kernels_build", "mti_kernels_info", "mti_kernels_supports",
-           "mti_kernels_table", "mti_kernels_sums", "mti_kernels_free",
+           "mti_kernels_table", "mti_kernels_sums", "mti_kernels_free", "mti_kernels_position",
            "mti_sort_rows", "mti_preprocess", "mti_grid", "mti_fft2d_inverse",
            "mti_finalize", "mti_grid_jobs", "mti_finalize_jobs",
            "mti_grid_out_shape", "mti_finalize_out_shape", "mti_preprocess_out_shape",
@@ -96,7 +112,9 @@ EXPORTS = ["mti_version", "mti_create", "mti_destroy", "mti_last_error",
            "mti_set_workspace", "mti_context_device_bytes", "mti_set_tuning", "mti_get_tuning",
            "mti_graph_create", "mti_graph_launch", "mti_graph_destroy",
            "mti_fits_create", "mti_fits_write_channel", "mti_fits_close", "mti_finalize_sparse",
-           "mti_set_transform_storage", "mti_get_transform_storage"]
+           "mti_set_transform_storage", "mti_get_transform_storage",
+           "mti_preprocess_reference", "mti_uv_reorder", "mti_centre_shift", "mti_stokes", "mti_taper_correct",
+           "mti_taper_kernel", "mti_sinc_correct", "mti_normalizer_f32"]
 
 _lib = None
 
@@ -132,6 +150,8 @@ def lib():
     L.mti_kernels_table.argtypes = [vp]
     L.mti_kernels_sums.restype = vp
     L.mti_kernels_sums.argtypes = [vp]
+    L.mti_kernels_position.restype = vp
+    L.mti_kernels_position.argtypes = [vp]
     L.mti_kernels_free.argtypes = [vp]
     L.mti_sort_rows.restype = ctypes.c_int
     L.mti_sort_rows.argtypes = [vp, vp, vp, ctypes.c_int64, ctypes.c_int32, vp,
@@ -169,7 +189,16 @@ def lib():
             ("mti_get_transform_storage", [vp, P(i32)]),
             ("mti_graph_create", [vp, i32, vp, P(GridOpts), i32, i32, P(vp)]),
             ("mti_graph_launch", [vp, vp]),
-            ("mti_finalize_sparse", [vp, P(ImageParams), i32, i32, i32, i32, i32, vp, vp, vp, vp, vp])]:
+            ("mti_finalize_sparse", [vp, P(ImageParams), i32, i32, i32, i32, i32, vp, vp, vp, vp, vp]),
+            ("mti_preprocess_reference", [vp, P(PreprocessRefParams), vp, vp, P(Vis), i32,
+                                          P(PreprocessRefOutputs), vp]),
+            ("mti_uv_reorder", [vp, vp, i32, i32, i32, i32, vp]),
+            ("mti_centre_shift", [vp, vp, i64, i32, i32, i32, vp]),
+            ("mti_stokes", [vp, vp, vp, i32, i32, i32, i32, vp]),
+            ("mti_taper_correct", [vp, vp, i64, i32, i32, i32, vp]),
+            ("mti_taper_kernel", [vp, i32, i32, i32, i32, vp]),
+            ("mti_sinc_correct", [vp, vp, i64, i32, i32, i32, vp]),
+            ("mti_normalizer_f32", [vp, vp, i64, vp])]:
         f = getattr(L, name)
         f.restype = ctypes.c_int
         f.argtypes = args
@@ -375,6 +404,16 @@ class Context:
                                   int(chan), bins_out.data_ptr(), _stream(stream))
         self.check(rc, "mti_preprocess")
 
+    def preprocess_reference(self, rp, convinfo, sums, vis, chan, outs, stream=None):
+        """PreProcessUVW with the reference's seven outputs.  rp:
+        PreprocessRefParams; convinfo int32 [W][2] {support, base}; sums f32
+        [W][T*T]; outs: (gridding_data int32 [N][4], conv_index, to_grid,
+        to_compress, support int32 [N], take_conjugate f32 [N], conv_sum f32 [P][N])."""
+        o = PreprocessRefOutputs(*[t.data_ptr() for t in outs])
+        rc = lib().mti_preprocess_reference(self.h, ctypes.byref(rp), convinfo.data_ptr(), sums.data_ptr(),
+                                            ctypes.byref(vis), int(chan), ctypes.byref(o), _stream(stream))
+        self.check(rc, "mti_preprocess_reference")
+
     # ---- gridding ---------------------------------------------------------
     def grid(self, kern, params, vis, grids, norm, chan0=0, nchan=1, accumulate=False,
              psf=False, weighting="natural", robust=0.0, layout="vup", precision=None,
@@ -461,6 +500,60 @@ class Context:
         rc = lib().mti_finalize_jobs(self.h, len(jobs), ctypes.cast(arr, ctypes.c_void_p), POLTYPE[poltype],
                                      LAYOUT[layout], _stream(stream))
         self.check(rc, "mti_finalize_jobs")
+
+
+# ---- the reference's operators with their own array contracts ---------------
+def _op(rc, what):
+    if rc != MTI_OK:
+        raise MTIError(f"{what} failed (status {rc})")
+
+
+def _c128(t):
+    return 1 if str(t.dtype) in ("torch.complex128", "torch.float64") else 0
+
+
+def uv_reorder(inp, out, rows, cols, npol, stream=None):
+    """PreIFFTUVCorrection: complex [rows][cols][npol] -> [npol][rows][cols]."""
+    _op(lib().mti_uv_reorder(inp.data_ptr(), out.data_ptr(), int(rows), int(cols), int(npol), _c128(inp),
+                             _stream(stream)), "mti_uv_reorder")
+
+
+def centre_shift(inp, out, planes, rows, cols, stream=None):
+    """ZeroToCentre2DShift of every [rows][cols] complex plane."""
+    _op(lib().mti_centre_shift(inp.data_ptr(), out.data_ptr(), int(planes), int(rows), int(cols), _c128(inp),
+                               _stream(stream)), "mti_centre_shift")
+
+
+def stokes(inp, norm_f32, out, npol, rows, cols, poltype="linear", stream=None):
+    """StokesConverter: complex64 [npol][rows][cols] + f32 normaliser -> f32 Stokes, rows flipped."""
+    _op(lib().mti_stokes(inp.data_ptr(), norm_f32.data_ptr(), out.data_ptr(), int(npol), int(rows), int(cols),
+                         POLTYPE[poltype], _stream(stream)), "mti_stokes")
+
+
+def taper_correct(inp, out, planes, rows, cols, stream=None):
+    """casagridsf lmPlanePostCorrection: divide by the taper (inp None: write the taper)."""
+    f64 = 1 if str(out.dtype) == "torch.float64" else 0
+    _op(lib().mti_taper_correct(inp.data_ptr() if inp is not None else None, out.data_ptr(), int(planes),
+                                int(rows), int(cols), f64, _stream(stream)), "mti_taper_correct")
+
+
+def taper_kernel(out, support, sampling, complex_out=False, stream=None):
+    """casagridsf generate: the oversampled taper-only gridding function."""
+    f64 = 1 if str(out.dtype) in ("torch.float64", "torch.complex128") else 0
+    _op(lib().mti_taper_kernel(out.data_ptr(), int(support), int(sampling), 1 if complex_out else 0, f64,
+                               _stream(stream)), "mti_taper_kernel")
+
+
+def sinc_correct(inp, out, planes, rows, cols, sampling, stream=None):
+    """SincCorrector of f32 planes."""
+    _op(lib().mti_sinc_correct(inp.data_ptr(), out.data_ptr(), int(planes), int(rows), int(cols), int(sampling),
+                               _stream(stream)), "mti_sinc_correct")
+
+
+def normalizer_f32(norm, out, stream=None):
+    """The reference's f32 NormalizerValue from FP64 normalisers."""
+    _op(lib().mti_normalizer_f32(norm.data_ptr(), out.data_ptr(), int(norm.numel()), _stream(stream)),
+        "mti_normalizer_f32")
 
 
 class Graph:

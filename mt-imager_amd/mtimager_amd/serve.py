@@ -28,7 +28,13 @@ import time
 
 from . import Context, MTIError, vis_ms
 
-WORK_BYTES_PER_VIS = 64  # record descriptors, weighted vis, pairs and keys per (row, channel)
+def work_bytes_per_vis(npol):
+    """Scratch of the batched gridding call per (row, channel) record: the
+    16-byte descriptor and 4-byte tag, the weighted visibility (8 bytes per
+    correlation), flag-masked weights and density (4 + 4 per correlation),
+    and the (record, tile) pairs with their sort keys (8 + 8 bytes each, up
+    to four tiles for the supports of these tables)."""
+    return 20 + 16 * int(npol) + 4 * 16
 
 
 def request_bytes(nx, ny, npol, nchan, nvis, grid_bytes=16):
@@ -36,7 +42,7 @@ def request_bytes(nx, ny, npol, nchan, nvis, grid_bytes=16):
     bytes a cell and correlation, complex64: 8), f64 normaliser, f32 image and
     the per-record work space of its wave."""
     pix = int(nx) * int(ny) * int(npol) * int(nchan)
-    return pix * (int(grid_bytes) + 4) + 8 * npol * nchan + WORK_BYTES_PER_VIS * int(nvis)
+    return pix * (int(grid_bytes) + 4) + 8 * npol * nchan + work_bytes_per_vis(npol) * int(nvis)
 
 
 class Request:
@@ -48,6 +54,9 @@ class Request:
 
     After the request is done: image f32 [nchan][npol][ny][nx] and norm f64
     [nchan][npol] (if keep_image), arrival/admitted/done perf_counter stamps,
+    on_done(request) called by the worker once the request's wave has
+    finished on the device (image and norm set for the call, then dropped
+    unless keep_image) -- a result hand-off that keeps no image alive,
     error (an exception, or None), wave (the wave number it ran in) and
     gridded (its gridded (row, channel) count when it ran alone in its wave;
     a batched wave reports one count for all of its requests, summed in
@@ -55,10 +64,10 @@ class Request:
 
     __slots__ = ("rid", "params", "vis_args", "ant1", "ant2", "nant", "chan0", "nchan", "npol",
                  "nvis", "keep_image", "arrival", "admitted", "done", "image", "norm",
-                 "gridded", "error", "nbytes", "wave")
+                 "gridded", "error", "nbytes", "wave", "on_done")
 
     def __init__(self, rid, params, vis_args, ant1, ant2, nant, nchan, npol=1, chan0=0,
-                 nvis=None, keep_image=True):
+                 nvis=None, keep_image=True, on_done=None):
         self.rid = rid
         self.params = params
         self.vis_args = vis_args
@@ -66,6 +75,7 @@ class Request:
         self.chan0, self.nchan, self.npol = int(chan0), int(nchan), int(npol)
         self.nvis = int(nvis) if nvis is not None else int(ant1.numel()) * int(nchan)
         self.keep_image = keep_image
+        self.on_done = on_done
         self.arrival = self.admitted = self.done = None
         self.image = self.norm = None
         self.gridded = 0
@@ -155,10 +165,14 @@ class Server:
         self._queue = collections.deque()
         self._in_use = 0
         self._pending = 0
+        self._running = 0
+        # device bytes each worker's context keeps after its last wave (its
+        # scratch grows to the largest wave it ran and is not returned)
+        self._ctx_bytes = [0] * int(workers)
         self._stop = False
         self._threads = []
-        for _ in range(int(workers)):
-            t = threading.Thread(target=self._worker, daemon=True)
+        for w in range(int(workers)):
+            t = threading.Thread(target=self._worker, args=(w,), daemon=True)
             t.start()
             self._threads.append(t)
 
@@ -199,7 +213,9 @@ class Server:
         return len(dropped)
 
     def close(self):
-        """Stops the workers after their current wave (queued requests are dropped)."""
+        """Stops the workers after their current wave; requests still queued
+        are dropped with an error (and no longer count as pending)."""
+        self.cancel_queued()
         with self._cv:
             self._stop = True
             self._cv.notify_all()
@@ -208,7 +224,14 @@ class Server:
 
     # ---- admission (called with the lock held) --------------------------------
     def _head_fits(self):
-        return bool(self._queue) and self._in_use + self._queue[0].nbytes <= self.budget
+        """The head request fits beside the admitted requests and the scratch
+        the workers' contexts keep; with nothing in flight it is admitted
+        against the budget alone (it was checked against it on submit)."""
+        if not self._queue:
+            return False
+        if self._running == 0 and self._in_use == 0:
+            return True
+        return self._in_use + sum(self._ctx_bytes) + self._queue[0].nbytes <= self.budget
 
     def _take_wave(self):
         """Requests of the next wave: first come first served while they fit
@@ -222,7 +245,7 @@ class Server:
         return wave
 
     # ---- workers ------------------------------------------------------------
-    def _worker(self):
+    def _worker(self, wid_slot):
         torch = self.torch
         torch.cuda.set_device(self.device)
         stream = torch.cuda.Stream(device=self.device)
@@ -238,6 +261,7 @@ class Server:
                     wave = self._take_wave()
                     self.wave_sizes.append(len(wave))
                     wid = len(self.wave_sizes)
+                    self._running += 1
                 t_adm = time.perf_counter()
                 for r in wave:
                     r.admitted, r.wave = t_adm, wid
@@ -248,11 +272,25 @@ class Server:
                         r.error = e
                     self.errors.append(e)
                 t = time.perf_counter()
+                for r in wave:
+                    r.done = t
+                    if r.on_done is not None and r.error is None:
+                        try:
+                            r.on_done(r)
+                        except Exception as e:  # a failing hand-off fails its request only
+                            r.error = e
+                    if not r.keep_image:
+                        r.image = r.norm = None
+                try:
+                    held = ctx.device_bytes()
+                except Exception:
+                    held = 0
                 with self._cv:
                     for r in wave:
-                        r.done = t
                         self._in_use -= r.nbytes
                         self._pending -= 1
+                    self._running -= 1
+                    self._ctx_bytes[wid_slot] = held
                     self._cv.notify_all()
         finally:
             ctx.close()
@@ -319,7 +357,7 @@ class Server:
         if len(owned) == 1:
             owned[0][0].gridded = st.gridded_records
         for r, K, v, grids, norm, image, mask in owned:
-            if r.keep_image:
+            if r.keep_image or r.on_done is not None:
                 r.image, r.norm = image, norm
             if self.cache is None:
                 K.free()

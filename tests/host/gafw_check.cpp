@@ -2,10 +2,13 @@
 // the way mt-imager drives its GAFW operators, and checks it:
 //   gafw_check validate   shape/type validation, factory names, property
 //                         parsing and the exception split (no GPU needed);
-//   gafw_check gpu        one imaging job through SortIndex-free chunk input
-//                         (the reference's WImager chunk layout), the
-//                         ConvolutionGridder and ImageFinalizer operators, in
-//                         two accumulated chunks, against the FP64 oracle.
+//   gafw_check gpu        one imaging job in the reference's WImager chunk
+//                         layout through the WImager module (two accumulated
+//                         chunks) and the ImageFinalizer module, and the same
+//                         grid through the reference's finalizer operators one
+//                         by one (PreIFFTUVCorrection, IFFT2D,
+//                         ZeroToCentre2DShift, StokesConverter, casagridsf,
+//                         SincCorrector), against the FP64 oracle.
 // Test code: it links the oracle as the checker.  Exit status 0 = pass.
 #include <hip/hip_runtime_api.h>
 
@@ -78,10 +81,19 @@ static int run_validate()
 {
     ImagerFactoryHelper f(0);
     CHECK(f.createOperator("x", "NoSuchOperator") == nullptr, "unknown operator name gives nullptr");
-    for (const char *n : {"SortIndex", "PreProcessUVW", "ConvolutionGridder", "IFFT2D", "ImageFinalizer"}) {
+    for (const char *n : {"PreProcessUVW", "PreIFFTUVCorrection", "IFFT2D", "ZeroToCentre2DShift", "StokesConverter",
+                          "casagridsf", "SincCorrector", "SortIndex"}) {
         auto op = f.createOperator(std::string("nick_") + n, n);
         CHECK(op && op->name() == n, (std::string("factory creates ") + n).c_str());
     }
+    // reference names whose contracts consume the gridding plan or the
+    // convolution-function chain: not offered (nullptr, as the reference's
+    // factory gives for names it does not know)
+    for (const char *n : {"ConvolutionGridder", "CreateIndexAndReorder", "NotSameSupportManipulate",
+                          "CreateBlockDataIndex", "CreateCompressionPlan", "CompressVisibilities",
+                          "WProjPreFFTConvFunc", "ConvFunctionReOrganize", "ConvFunctionSupportFind",
+                          "SumConvolutionFunctions", "multiFFT2D", "AccumulatedSum", "Sum"})
+        CHECK(f.createOperator("n", n) == nullptr, (std::string("no operator with another contract under ") + n).c_str());
     Properties p = image_props();
     mti_image_params ip = image_params_from(p);
     CHECK(ip.nx == 256 && ip.wplanes == 16 && ip.sampling == 4 && ip.mode == MTI_MODE_WPROJ,
@@ -91,15 +103,37 @@ static int run_validate()
     bad.setProperty("image.nx", "12x");
     CHECK(throws<ValidationException>([&] { bad.getIntProperty("image.nx"); }), "malformed integer property throws");
     CHECK(throws<ValidationException>([&] { bad.getIntProperty("image.ny"); }), "missing property throws");
-
-    // ConvolutionGridder without convolution functions
-    auto g = f.createOperator("g", "ConvolutionGridder");
-    std::vector<ArrayDescriptor> in = chunk_desc(1000, 4), out;
+    std::vector<ArrayDescriptor> out;
     ValidationData vd;
-    CHECK(throws<ValidationException>([&] { g->validate(in, out, p, vd); }),
-          "gridder without a convolution function is rejected");
 
-    // IFFT2D shapes
+    // PreProcessUVW: six inputs, seven outputs
+    auto pre = f.createOperator("pre", "PreProcessUVW");
+    std::vector<ArrayDescriptor> pin = chunk_desc(1000, 4);
+    pin.erase(pin.begin() + 1);  // no visibilities
+    pin.insert(pin.begin(), ArrayDescriptor());
+    pin.insert(pin.begin(), ArrayDescriptor());
+    pin[0].dim = ArrayDimensions({16, 2});
+    pin[0].type = real_int;
+    pin[1].dim = ArrayDimensions({16, 25});
+    pin[1].type = real_float;
+    Properties pp;
+    pp.setProperty("ConvolutionFunction.sampling", "4");
+    pp.setProperty("ConvolutionFunction.wsquareincrement", 0.01);
+    pp.setProperty("uvImage.u_increment", 50.0);
+    pp.setProperty("uvImage.v_increment", 50.0);
+    pp.setProperty("uvImage.rows", "256");
+    pp.setProperty("uvImage.columns", "256");
+    pre->validate(pin, out, pp, vd);
+    CHECK(out.size() == 7 && out[0].dim == ArrayDimensions({1000, 4}) && out[0].type == real_int &&
+              out[5].type == real_float && out[6].dim == ArrayDimensions({4, 1000}) && out[6].type == real_float,
+          "PreProcessUVW outputs: GriddingData int [N][4] ... ConvolutionFunctionSum real_float [P][N]");
+    std::vector<ArrayDescriptor> five(pin.begin(), pin.begin() + 5);
+    CHECK(throws<ValidationException>([&] { pre->validate(five, out, pp, vd); }), "PreProcessUVW needs six inputs");
+    Properties pq = pp;
+    pq = Properties();
+    CHECK(throws<ValidationException>([&] { pre->validate(pin, out, pq, vd); }), "PreProcessUVW needs its parameters");
+
+    // IFFT2D shapes: any size
     auto fft = f.createOperator("f", "IFFT2D");
     std::vector<ArrayDescriptor> fi(1);
     fi[0].dim = ArrayDimensions({3, 128, 64});
@@ -107,49 +141,76 @@ static int run_validate()
     out.clear();
     fft->validate(fi, out, p, vd);
     CHECK(out.size() == 1 && out[0].dim == fi[0].dim && out[0].type == complex_float, "IFFT2D output shape");
-    fi[0].dim = ArrayDimensions({100, 64});
-    CHECK(throws<ValidationException>([&] { fft->validate(fi, out, p, vd); }), "IFFT2D rejects non powers of two");
-    fi[0].dim = ArrayDimensions({64, 64});
+    fi[0].dim = ArrayDimensions({100, 75});
+    out.clear();
+    fft->validate(fi, out, p, vd);
+    CHECK(out.size() == 1 && out[0].dim == fi[0].dim, "IFFT2D takes sizes that are not powers of two");
     fi[0].type = real_float;
     CHECK(throws<ValidationException>([&] { fft->validate(fi, out, p, vd); }), "IFFT2D rejects real input");
 
-    // ImageFinalizer shapes and buffer request
-    auto fin = f.createOperator("i", "ImageFinalizer");
-    std::vector<ArrayDescriptor> ii(2);
-    ii[0].dim = ArrayDimensions({256, 128, 4});
-    ii[0].type = complex_float;
-    ii[1].dim = ArrayDimensions({4});
-    ii[1].type = real_float;
-    out.clear();
-    vd = ValidationData();
-    fin->validate(ii, out, p, vd);
-    CHECK(out.size() == 1 && out[0].dim == ArrayDimensions({4, 256, 128}) && out[0].type == real_float,
-          "ImageFinalizer output is real_float [P][v][u]");
-    {
-        mti_image_params fp = image_params_from(p);
-        fp.ny = 256;
-        fp.nx = 128;
-        size_t ws = 0;
-        CHECK(mti_finalize_workspace_size(&fp, 4, 1, &ws) == MTI_OK && ws > 0 && vd.bufferSize == 256 + ws,
-              "ImageFinalizer asks for the normaliser staging and the finalize workspace");
-    }
-    ii[1].dim = ArrayDimensions({2});
-    CHECK(throws<ValidationException>([&] { fin->validate(ii, out, p, vd); }), "normaliser length must be P");
-    Properties pc = p;
-    pc.setProperty("PolarizationType", "elliptical");
-    ii[1].dim = ArrayDimensions({4});
-    CHECK(throws<ValidationException>([&] { fin->validate(ii, out, pc, vd); }), "bad PolarizationType rejected");
+    // the finalizer operators
+    auto pre2 = f.createOperator("u", "PreIFFTUVCorrection");
+    std::vector<ArrayDescriptor> g1(1);
+    g1[0].dim = ArrayDimensions({256, 128, 4});
+    g1[0].type = complex_float;
+    pre2->validate(g1, out, p, vd);
+    CHECK(out[0].dim == ArrayDimensions({4, 256, 128}) && out[0].type == complex_float,
+          "PreIFFTUVCorrection: [v][u][P] -> [P][v][u]");
+    auto stk = f.createOperator("s", "StokesConverter");
+    std::vector<ArrayDescriptor> si(2);
+    si[0].dim = ArrayDimensions({4, 256, 128});
+    si[0].type = complex_float;
+    si[1].dim = ArrayDimensions({4});
+    si[1].type = real_float;
+    Properties ps;
+    ps.setProperty("PolarizationType", "1");
+    stk->validate(si, out, ps, vd);
+    CHECK(out[0].dim == si[0].dim && out[0].type == real_float, "StokesConverter output real_float [P][v][u]");
+    si[1].type = real_double;
+    CHECK(throws<ValidationException>([&] { stk->validate(si, out, ps, vd); }), "StokesConverter wants a real_float normaliser");
+    si[1].type = real_float;
+    ps.setProperty("PolarizationType", "2");
+    CHECK(throws<ValidationException>([&] { stk->validate(si, out, ps, vd); }), "bad PolarizationType rejected");
+    auto csf = f.createOperator("c", "casagridsf");
+    Properties pc;
+    pc.setProperty("support", "7");
+    pc.setProperty("sampling", "4");
+    pc.setProperty("lmPlanePostCorrection", "false");
+    pc.setProperty("generate", "true");
+    pc.setProperty("complexOutput", "true");
+    std::vector<ArrayDescriptor> none;
+    csf->validate(none, out, pc, vd);
+    CHECK(out[0].dim == ArrayDimensions({29, 29}) && out[0].type == complex_float,
+          "casagridsf generate: (S s + 1)^2 complex_float for even sampling");
+    pc.setProperty("support", "6");
+    CHECK(throws<ValidationException>([&] { csf->validate(none, out, pc, vd); }), "even support rejected");
+    auto sinc = f.createOperator("n", "SincCorrector");
+    std::vector<ArrayDescriptor> sc(1);
+    sc[0].dim = ArrayDimensions({4, 64, 64});
+    sc[0].type = real_float;
+    Properties pn;
+    pn.setProperty("sampling", "4");
+    sinc->validate(sc, out, pn, vd);
+    CHECK(out[0].dim == sc[0].dim && out[0].type == real_float, "SincCorrector keeps shape");
+    sc[0].type = real_double;
+    CHECK(throws<ValidationException>([&] { sinc->validate(sc, out, pn, vd); }), "SincCorrector works only on real_floats");
 
     // SortIndex shapes
     auto so = f.createOperator("s", "SortIndex");
-    std::vector<ArrayDescriptor> si(2);
-    si[0].dim = si[1].dim = ArrayDimensions({500});
-    si[0].type = si[1].type = real_int;
-    Properties ps;
-    ps.setProperty("nant", "27");
+    std::vector<ArrayDescriptor> sx(2);
+    sx[0].dim = sx[1].dim = ArrayDimensions({500});
+    sx[0].type = sx[1].type = real_int;
+    Properties pa;
+    pa.setProperty("nant", "27");
     out.clear();
-    so->validate(si, out, ps, vd);
-    CHECK(out.size() == 2 && out[0].dim == si[0].dim && out[1].type == real_longint, "SortIndex outputs");
+    so->validate(sx, out, pa, vd);
+    CHECK(out.size() == 2 && out[0].dim == sx[0].dim && out[1].type == real_longint, "SortIndex outputs");
+
+    // modules without a GPU call: configuration checks
+    CHECK(throws<ValidationException>([&] { WImager w(nullptr, p, 4); }), "WImager needs a generator");
+    Properties pf = p;
+    pf.setProperty("PolarizationType", "elliptical");
+    CHECK(throws<ValidationException>([&] { ImageFinalizer fz(pf); }), "ImageFinalizer rejects a bad PolarizationType");
     std::printf("%s\n", failures ? "FAILED" : "PASSED");
     return failures ? 1 : 0;
 }
@@ -227,70 +288,81 @@ static int run_gpu()
     std::printf("convolution functions: max support %d, sampling %d, w increment %.6g\n", conv->getMaxSupport(),
                 conv->getSampling(), conv->getWSquareIncrement());
 
-    ImagerFactoryHelper f(0);
-    auto grid_op = f.createOperator("WImager.gridder", "ConvolutionGridder");
-    auto fin_op = f.createOperator("ImageFinalizer", "ImageFinalizer");
-    grid_op->setConvolutionFunction(conv);
-    fin_op->setConvolutionFunction(conv);
-
-    std::vector<ArrayDescriptor> gout;
-    ValidationData gvd;
-    grid_op->validate(chunk_desc(N, P), gout, p, gvd);
-    CHECK(gout[0].dim == ArrayDimensions({256, 256, P}) && gout[0].type == complex_float, "gridder output shape");
-    float *d_grid = nullptr, *d_norm = nullptr;
-    void *d_buf = nullptr;
-    hipok(hipMalloc(&d_grid, (size_t)gout[0].dim.total() * 8), "grid");
-    hipok(hipMalloc(&d_norm, P * sizeof(float)), "norm");
-    hipok(hipMalloc(&d_buf, gvd.bufferSize), "buffer");
     hipStream_t st;
     hipok(hipStreamCreate(&st), "stream");
-
+    WImager wim(conv, p, P);
+    wim.setStream(st);
     std::vector<void *> owned;
     for (int c = 0; c < 2; c++) {
-        SubmissionData sd;
-        sd.inputs = chunk_desc(N, P);
-        sd.inputs[0].pointer = dev_copy(uvw[c]);
-        sd.inputs[1].pointer = dev_copy(vis[c]);
-        sd.inputs[2].pointer = dev_copy(freq[c]);
-        sd.inputs[3].pointer = dev_copy(wt[c]);
-        sd.inputs[4].pointer = dev_copy(fl[c]);
-        for (auto &d : sd.inputs) owned.push_back(d.pointer);
-        sd.outputs = gout;
-        sd.outputs[0].pointer = d_grid;
-        sd.outputs[1].pointer = d_norm;
-        sd.stream = st;
-        sd.params = p;
-        sd.params.setProperty("initialize_output", c == 0 ? "true" : "false");
-        sd.bufferGPUPointer = d_buf;
-        sd.buffersize = gvd.bufferSize;
-        grid_op->submitToGPU(sd);
+        std::vector<ArrayDescriptor> in = chunk_desc(N, P);
+        in[0].pointer = dev_copy(uvw[c]);
+        in[1].pointer = dev_copy(vis[c]);
+        in[2].pointer = dev_copy(freq[c]);
+        in[3].pointer = dev_copy(wt[c]);
+        in[4].pointer = dev_copy(fl[c]);
+        for (int i = 0; i < 5; i++) {
+            wim.setInput(i, in[i]);
+            owned.push_back(in[i].pointer);
+        }
+        wim.calculate();  // the second chunk adds to the first
     }
-    std::vector<ArrayDescriptor> fin_in(2), fin_out;
-    fin_in[0] = gout[0];
-    fin_in[1] = gout[1];
-    ValidationData fvd;
-    fin_op->validate(fin_in, fin_out, p, fvd);
-    float *d_img = nullptr;
-    void *d_fbuf = nullptr;
-    hipok(hipMalloc(&d_img, (size_t)fin_out[0].dim.total() * 4), "image");
-    hipok(hipMalloc(&d_fbuf, fvd.bufferSize), "fbuf");
-    SubmissionData fd;
-    fd.inputs = fin_in;
-    fd.inputs[0].pointer = d_grid;
-    fd.inputs[1].pointer = d_norm;
-    fd.outputs = fin_out;
-    fd.outputs[0].pointer = d_img;
-    fd.stream = st;
-    fd.params = p;
-    fd.bufferGPUPointer = d_fbuf;
-    fd.buffersize = fvd.bufferSize;
-    fin_op->submitToGPU(fd);
+    const ArrayDescriptor &gout = wim.getOutput(0), &nout = wim.getOutput(1);
+    CHECK(gout.dim == ArrayDimensions({256, 256, P}) && gout.type == complex_float, "WImager Grid complex_float [v][u][P]");
+    CHECK(nout.dim == ArrayDimensions({P}) && nout.type == real_float, "WImager NormalizerValue real_float [P]");
+    ImageFinalizer fin(p);
+    fin.setStream(st);
+    fin.setInput(0, gout);
+    fin.setInput(1, nout);
+    fin.calculate();
+    const ArrayDescriptor &fout = fin.getOutput(0);
+    CHECK(fout.dim == ArrayDimensions({P, 256, 256}) && fout.type == real_float, "ImageFinalizer real_float [P][v][u]");
+
+    // the same grid through the reference's finalizer operators one by one
+    // (comp/ImageFinalizer.cpp:31-44)
+    ImagerFactoryHelper f(0);
+    std::vector<std::unique_ptr<ArrayOperator>> chain;
+    const char *names[] = {"PreIFFTUVCorrection", "IFFT2D", "ZeroToCentre2DShift", "StokesConverter", "casagridsf",
+                           "SincCorrector"};
+    for (const char *n : names) chain.push_back(f.createOperator(std::string("fin.") + n, n));
+    Properties pk = p;
+    pk.setProperty("PolarizationType", "0");
+    pk.setProperty("support", "7");
+    pk.setProperty("sampling", "4");
+    pk.setProperty("lmPlanePostCorrection", "true");
+    pk.setProperty("generate", "false");
+    std::vector<ArrayDescriptor> cur{gout};
+    std::vector<void *> chain_bufs;
+    for (size_t k = 0; k < chain.size(); k++) {
+        std::vector<ArrayDescriptor> in = cur, out;
+        if (k == 3) in.push_back(nout);  // StokesConverter takes the normaliser
+        ValidationData vd;
+        chain[k]->validate(in, out, pk, vd);
+        const size_t esz = out[0].type == complex_float ? 8 : out[0].type == complex_double ? 16 : 4;
+        out[0].pointer = nullptr;
+        hipok(hipMalloc(&out[0].pointer, (size_t)out[0].dim.total() * esz), "chain output");
+        chain_bufs.push_back(out[0].pointer);
+        void *buf = nullptr;
+        if (vd.bufferSize) {
+            hipok(hipMalloc(&buf, vd.bufferSize), "chain buffer");
+            chain_bufs.push_back(buf);
+        }
+        SubmissionData sd;
+        sd.inputs = in;
+        sd.outputs = out;
+        sd.stream = st;
+        sd.params = pk;
+        sd.bufferGPUPointer = buf;
+        sd.buffersize = vd.bufferSize;
+        chain[k]->submitToGPU(sd);
+        cur = out;
+    }
     hipok(hipStreamSynchronize(st), "sync");
 
-    std::vector<float> g_h((size_t)gout[0].dim.total() * 2), n_h(P), img_h((size_t)fin_out[0].dim.total());
-    hipok(hipMemcpy(g_h.data(), d_grid, g_h.size() * 4, hipMemcpyDeviceToHost), "D2H grid");
-    hipok(hipMemcpy(n_h.data(), d_norm, P * 4, hipMemcpyDeviceToHost), "D2H norm");
-    hipok(hipMemcpy(img_h.data(), d_img, img_h.size() * 4, hipMemcpyDeviceToHost), "D2H image");
+    std::vector<float> g_h((size_t)gout.dim.total() * 2), n_h(P), img_h((size_t)fout.dim.total()), chn_h(img_h.size());
+    hipok(hipMemcpy(g_h.data(), gout.pointer, g_h.size() * 4, hipMemcpyDeviceToHost), "D2H grid");
+    hipok(hipMemcpy(n_h.data(), nout.pointer, P * 4, hipMemcpyDeviceToHost), "D2H norm");
+    hipok(hipMemcpy(img_h.data(), fout.pointer, img_h.size() * 4, hipMemcpyDeviceToHost), "D2H image");
+    hipok(hipMemcpy(chn_h.data(), cur[0].pointer, chn_h.size() * 4, hipMemcpyDeviceToHost), "D2H chain image");
 
     // the oracle on the same records (FP64 throughout)
     orc_params op{256, 256, 8.0 * M_PI / 180.0 / 3600.0, 8.0 * M_PI / 180.0 / 3600.0, 1, 16, 4, 7};
@@ -318,16 +390,18 @@ static int run_gpu()
     orc_finalize(&op, P, 0, G.data(), NR.data(), IMG.data());
     orc_kernels_free(&K);
 
-    std::vector<double> gd(g_h.begin(), g_h.end()), id(img_h.begin(), img_h.end());
+    std::vector<double> gd(g_h.begin(), g_h.end()), id(img_h.begin(), img_h.end()), cd(chn_h.begin(), chn_h.end());
     double nerr = 0;
     for (int q = 0; q < P; q++) nerr = std::max(nerr, std::fabs(n_h[q] - NR[q]) / NR[q]);
-    const double ge = rel_rms(gd, G), ie = rel_rms(id, IMG);
-    std::printf("normaliser max rel err %.3e, grid rel RMS %.3e, image rel RMS %.3e\n", nerr, ge, ie);
-    CHECK(nerr < 1e-6, "NormalizerValue (f32 output) matches the oracle");
-    CHECK(ge < 1e-5, "Grid (complex_float [v][u][P], two accumulated chunks) matches the oracle");
-    CHECK(ie < 1e-4, "ImageFinalizer Stokes images match the oracle");
+    const double ge = rel_rms(gd, G), ie = rel_rms(id, IMG), ce = rel_rms(cd, IMG);
+    std::printf("normaliser max rel err %.3e, grid rel RMS %.3e, image rel RMS %.3e, operator chain %.3e\n", nerr, ge,
+                ie, ce);
+    CHECK(nerr < 1e-6, "WImager NormalizerValue (real_float) matches the oracle");
+    CHECK(ge < 1e-5, "WImager Grid (complex_float [v][u][P], two accumulated chunks) matches the oracle");
+    CHECK(ie < 1e-4, "ImageFinalizer module Stokes images match the oracle");
+    CHECK(ce < 1e-4, "the reference's finalizer operators, one by one, match the oracle");
     for (void *q : owned) hipok(hipFree(q), "free");
-    for (void *q : {(void *)d_grid, (void *)d_norm, d_buf, (void *)d_img, d_fbuf}) hipok(hipFree(q), "free");
+    for (void *q : chain_bufs) hipok(hipFree(q), "free");
     hipok(hipStreamDestroy(st), "stream destroy");
     std::printf("%s\n", failures ? "FAILED" : "PASSED");
     return failures ? 1 : 0;

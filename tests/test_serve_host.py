@@ -29,15 +29,17 @@ def _server(budget, max_wave=256):
 
 
 def test_request_bytes():
-    # grids c128 + image f32 per pixel and correlation, norm f64, work per visibility
-    assert serve.request_bytes(256, 128, 2, 3, 1000) == 256 * 128 * 2 * 3 * 20 + 8 * 2 * 3 + 64 * 1000
+    # grids c128 + image f32 per pixel and correlation, norm f64, work per
+    # visibility: descriptor + tag 20, 16 per correlation, pairs and keys 64
+    assert serve.work_bytes_per_vis(1) == 100 and serve.work_bytes_per_vis(4) == 148
+    assert serve.request_bytes(256, 128, 2, 3, 1000) == 256 * 128 * 2 * 3 * 20 + 8 * 2 * 3 + 116 * 1000
     r = _req(0, 512, 4, 100)
     assert r.nvis == 400
     assert r.nbytes == serve.request_bytes(512, 512, 1, 4, 400)
 
 
 def test_complex64_grids_halve_the_grid_bytes():
-    assert serve.request_bytes(256, 128, 2, 3, 1000, grid_bytes=8) == 256 * 128 * 2 * 3 * 12 + 8 * 2 * 3 + 64 * 1000
+    assert serve.request_bytes(256, 128, 2, 3, 1000, grid_bytes=8) == 256 * 128 * 2 * 3 * 12 + 8 * 2 * 3 + 116 * 1000
     r = _req(0, 512, 4, 100)
     s = serve.Server(1 << 40, workers=0, grid_precision="c64")
     s.submit(r)
@@ -150,3 +152,28 @@ def test_cancel_queued_drops_unadmitted_requests():
         assert [r.rid for r in s._take_wave()] == [0]
     assert s.cancel_queued() == 1
     assert b.error is not None and s._pending == 1 and not s._queue
+
+
+def test_close_drops_queued_requests_and_releases_wait():
+    s = _server(1 << 40)
+    a, b = _req(0, 256, 1, 10), _req(1, 256, 1, 10)
+    s.submit(a)
+    s.submit(b)
+    s.close()
+    assert a.error is not None and b.error is not None
+    assert s.wait(timeout=0.1)  # nothing pending any more
+
+
+def test_retained_context_scratch_counts_against_the_budget():
+    a, b = _req(0, 256, 1, 10), _req(1, 256, 1, 10)
+    s = _server(a.nbytes + b.nbytes + 10)
+    s.submit(a)
+    s.submit(b)
+    with s._cv:
+        s._ctx_bytes = [b.nbytes]          # a worker keeps scratch of this size
+        s._running = 1                      # and another wave is in flight
+        w = s._take_wave()
+        assert [r.rid for r in w] == [0]    # b no longer fits beside a and the scratch
+        s._in_use -= a.nbytes
+        s._running = 0                      # nothing in flight: the head is admitted
+        assert [r.rid for r in s._take_wave()] == [1]
