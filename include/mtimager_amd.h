@@ -8,8 +8,10 @@
  *
  * The caller owns every input and output device buffer (as the GAFW engine
  * does for operator I/O, gafw/Implimentation/GPUImplimentation/
- * CUDADeviceManager.cpp:1361-1530); scratch space is held by the context and
- * grows on demand.  All work is enqueued on the stream given to each call;
+ * CUDADeviceManager.cpp:1361-1530).  Scratch space is either a caller's
+ * workspace (mti_set_workspace, sized by the mti_*_workspace_size queries)
+ * or, without one, held by the context and grown on demand.  All work is
+ * enqueued on the stream given to each call;
  * calls on one context must not overlap (one context per host thread, as the
  * reference runs one submit thread per device, CUDADeviceManager.cpp:410-489).
  *
@@ -40,8 +42,11 @@ typedef enum { MTI_POL_LINEAR = 0, MTI_POL_CIRCULAR = 1 } mti_poltype;
  * or planar [P][v][u] (what the reference's PreIFFTUVCorrection reorders it
  * to before the transform, op/PreIFFTUVCorrection.cu:28-90). */
 typedef enum { MTI_GRID_VUP = 0, MTI_GRID_PVU = 1 } mti_grid_layout;
-/* Grid element type.  Gridding always accumulates in FP64; an FP32 grid (the
- * reference's complex<float> Grid) is rounded once at the end of mti_grid. */
+/* Grid element type.  Gridding accumulates in FP64 registers; a complex64
+ * grid (the reference's complex<float> Grid) receives each work unit's FP64
+ * cell sums rounded once -- stored, read-modify-written, or, in a tile split
+ * over several work units, added with float atomics whose order (and so the
+ * last-bit rounding) depends on the order units finish. */
 typedef enum { MTI_GRID_C64 = 0, MTI_GRID_C128 = 1 } mti_grid_precision;
 
 /* Imaging parameters; the fields carry the mt-imager property keys
@@ -106,6 +111,10 @@ typedef struct {
     /* optional row order (baseline-major from mti_sort_rows); NULL = 0..nrows-1 */
     const int32_t *row_index;
     int64_t nindex;           /* entries in row_index (ignored if NULL)       */
+    /* optional f64 [nrows]: a per-row factor on 1/lambda, the TOPO -> LSRK
+     * Doppler factor of the row's time (mti_lsrk_factors); the binning uses
+     * 1/lambda = inv_lambda * row_doppler[row] in FP64.  NULL = 1. */
+    const double *row_doppler;
 } mti_vis;
 
 /* Gridding options for one call (WImager::calculate for one chunk,
@@ -133,8 +142,10 @@ typedef struct {
 typedef struct {
     int64_t gridded_records;  /* Statistic:TotalGriddedRecords                */
     int64_t gridded_points;   /* Statistic:TotalGriddedPoints (sum S^2 * P)   */
-    int64_t work_items;       /* equal-support record runs (<= 512 records)   */
-    int64_t flushes;          /* reserved (0)                                 */
+    int64_t work_items;       /* gridder work units: runs of <= unit_pairs
+                                 (record, tile) pairs of one 32x32-cell tile   */
+    int64_t flushes;          /* (record, tile) pairs: each record counted once
+                                 per tile its support box touches             */
 } mti_grid_stats;
 
 /* Pipeline stages timed with HIP events when profiling is on (the
@@ -191,6 +202,30 @@ const float *mti_kernels_sums(const mti_kernels *k);
 const int32_t *mti_kernels_position(const mti_kernels *k);
 void         mti_kernels_free(mti_kernels *k);
 
+/* ---- TOPO -> LSRK per-record frequency ----------------------------------------
+ * Replaces FrequencyConverter::calculate (vis/FrequencyConverter.cpp:112-141,
+ * casacore MFrequency::Convert TOPO -> LSRK per distinct time).  FP64 on the
+ * device: observatory rotation (mean sidereal time, IAU 1976 precession to
+ * J2000), the Earth's barycentric velocity (JPL Keplerian elements 1800-2050,
+ * lunar offset), solar motion 20 km/s towards 18h03m50.29s +30d00m16.8s
+ * (J2000); factor = sqrt(1 - b.b) / (1 + b.n).  Agrees with casacore to the
+ * ephemeris accuracy (~10 m/s, 3e-8 in frequency), not bit for bit. */
+typedef struct {
+    double obs_itrf[3];   /* observatory geocentric ITRF position, metres     */
+    double ra, dec;       /* phase centre (J2000), radians                    */
+    double tt_minus_utc;  /* TT - UTC, seconds (69.184 from 2017)             */
+} mti_lsrk_frame;
+/* factor_out[r] = f_LSRK / f_TOPO at time_mjd_s[r] (MJD seconds, UTC, the
+ * MeasurementSet TIME column), f64, for mti_vis.row_doppler. */
+mti_status mti_lsrk_factors(mti_ctx *ctx, const mti_lsrk_frame *f, const double *time_mjd_s,
+                            int64_t nrows, double *factor_out, void *stream);
+/* The reference's per-record frequency store: f32 [nchan][nrows] of
+ * f_LSRK / c = freq_hz[c] * factor / c (1/lambda in 1/m), for an mti_vis with
+ * inv_lambda_f32 = inv_lambda_per_row = 1 (one channel per call). */
+mti_status mti_lsrk_inv_lambda(mti_ctx *ctx, const mti_lsrk_frame *f, const double *time_mjd_s,
+                               int64_t nrows, const double *freq_hz, int32_t nchan,
+                               float *inv_lambda_out, void *stream);
+
 /* ---- baseline-major row order --------------------------------------------
  * Replaces SortIndex::sortFuncThread (vis/SortIndex.cpp:46-87): stable order by
  * key Nant^2 - a1*Nant - a2 - 1, autocorrelations dropped.
@@ -220,8 +255,9 @@ mti_status mti_preprocess(mti_ctx *ctx, const mti_kernels *k,
  *           complex64 or complex128 elements (opts->grid_precision)
  *   norm  : f64 [nchan][npol]               (NormalizerValue)
  * Weighting (uniform/Briggs, not in the reference) is applied per channel.
- * Accumulation is FP64 (registers and grid); complex64 grids are rounded
- * once from an FP64 work grid held by the context. */
+ * Accumulation is FP64 in registers; complex128 grids add the sums in FP64,
+ * complex64 grids are written by the gridder directly (each work unit's sum
+ * rounded once; float atomics where a tile is split over work units). */
 mti_status mti_grid(mti_ctx *ctx, const mti_kernels *k, const mti_image_params *p,
                     const mti_vis *v, const mti_grid_opts *opts, void *grids,
                     double *norm, mti_grid_stats *stats, void *stream);

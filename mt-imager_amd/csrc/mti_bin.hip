@@ -11,8 +11,10 @@
 //   offset   = round((loc - pos) s) + s/2, clamped to [0,s-1] for odd s (:206-227);
 //   conv_idx = base_k + oy S^2 T + ox S^2 (:235-239).
 // The reference's per-1024-block "compress if equal to the previous record"
-// flag (:270-324) is not produced: the tile gridder accumulates equal records
-// in LDS, which gives the same sum.
+// flag (:270-324) is not produced by the gridding path: the tile gridder adds
+// equal records one after another into the same FP64 register sums, which
+// gives the same sum (mti_preprocess_reference writes the flag for callers
+// of the reference's operator).
 #include "mti_grid.hpp"
 
 #include <algorithm>
@@ -32,7 +34,8 @@ __device__ __forceinline__ double load_uvw(const VisView &v, int64_t row, int co
 __device__ __forceinline__ double load_invl(const VisView &v, int64_t row, int chan)
 {
     int64_t o = v.invl_per_row ? row : (int64_t)chan;
-    return v.invl_f32 ? (double)((const float *)v.invl)[o] : ((const double *)v.invl)[o];
+    const double f = v.invl_f32 ? (double)((const float *)v.invl)[o] : ((const double *)v.invl)[o];
+    return v.dop ? f * v.dop[row] : f;
 }
 
 // The FP64 binning of one record from its row's (u, v, w) and 1/lambda.
@@ -236,6 +239,7 @@ VisView make_view(const mti_vis *v)
     w.invl = v->inv_lambda;
     w.invl_f32 = v->inv_lambda_f32;
     w.invl_per_row = v->inv_lambda_per_row;
+    w.dop = v->row_doppler;
     w.vis = v->vis;
     w.vis_rs = v->vis_row_stride;
     w.vis_cs = v->vis_chan_stride;

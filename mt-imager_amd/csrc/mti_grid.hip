@@ -834,7 +834,7 @@ static void plan_pass(const std::vector<PassJob> &jobs, int P, bool weighted, Sc
 // grids (zeroed or holding earlier chunks) and accumulates their normalisers.
 // nu_total[0] += work units, [1] += (record, tile) pairs.
 static mti_status grid_pass(mti_ctx *ctx, const std::vector<PassJob> &jobs, const mti_grid_opts *o, int P, int prec,
-                            bool fresh, unsigned long long *dstats, uint32_t *nu_total, hipStream_t st)
+                            bool fresh, unsigned long long *dstats, uint64_t *nu_total, hipStream_t st)
 {
     const int nj = (int)jobs.size();
     if (nj == 0) return MTI_OK;
@@ -1081,7 +1081,7 @@ static mti_status check_job_geometry(mti_ctx *ctx, const mti_kernels *k, const m
     return MTI_OK;
 }
 
-static mti_status stats_out(mti_ctx *ctx, unsigned long long *dstats, const uint32_t *nu, mti_grid_stats *stats,
+static mti_status stats_out(mti_ctx *ctx, unsigned long long *dstats, const uint64_t *nu, mti_grid_stats *stats,
                             hipStream_t st)
 {
     if (!stats) return MTI_OK;
@@ -1096,8 +1096,8 @@ static mti_status stats_out(mti_ctx *ctx, unsigned long long *dstats, const uint
     *stats = mti_grid_stats{};
     stats->gridded_records = (int64_t)s[0];
     stats->gridded_points = (int64_t)s[1];
-    stats->work_items = nu[0];
-    stats->flushes = nu[1];
+    stats->work_items = (int64_t)nu[0];
+    stats->flushes = (int64_t)nu[1];
     return MTI_OK;
 }
 
@@ -1274,7 +1274,7 @@ extern "C" mti_status mti_grid(mti_ctx *ctx, const mti_kernels *k, const mti_ima
     unsigned long long *dstats = (unsigned long long *)sw;
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, st, (uint32_t *)dstats, 4 * BIN_STAT_STRIPES, 0u);
     MTI_HIP(ctx, hipGetLastError());
-    uint32_t nu[2] = {0, 0};
+    uint64_t nu[2] = {0, 0};
 
     MTI_CHECK(ctx, channels_per_pass(base) >= 1, "one channel has too many (record, tile) pairs");
     const int pair_batch = grid_batch(base, nb);
@@ -1391,7 +1391,7 @@ extern "C" mti_status mti_grid_jobs(mti_ctx *ctx, int32_t njobs, const mti_job *
             ip += n;
         }
     }
-    uint32_t nu[2] = {0, 0};
+    uint64_t nu[2] = {0, 0};
     // zero the normalisers; the grids are zeroed by the passes (tiles not stored whole)
     if (!o->accumulate) {
         for (const PassJob &j : all)

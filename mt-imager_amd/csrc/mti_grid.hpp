@@ -12,6 +12,7 @@ struct VisView {
     int64_t uvw_rs, uvw_cs;
     const void *invl;
     int invl_f32, invl_per_row;
+    const double *dop;  // per-row factor on 1/lambda (TOPO -> LSRK), or null
     const float *vis;
     int64_t vis_rs, vis_cs;  // complex elements
     const void *flags;

@@ -51,7 +51,20 @@ class Vis(ctypes.Structure):
                 ("flags", ctypes.c_void_p), ("flags_i32", ctypes.c_int32),
                 ("flags_row_stride", ctypes.c_int64), ("flags_chan_stride", ctypes.c_int64),
                 ("weight", ctypes.c_void_p), ("weight_row_stride", ctypes.c_int64),
-                ("row_index", ctypes.c_void_p), ("nindex", ctypes.c_int64)]
+                ("row_index", ctypes.c_void_p), ("nindex", ctypes.c_int64),
+                ("row_doppler", ctypes.c_void_p)]
+
+
+class LsrkFrame(ctypes.Structure):
+    """mti_lsrk_frame: observatory ITRF position (m), phase centre (J2000,
+    rad), TT - UTC (s)."""
+    _fields_ = [("obs_itrf", ctypes.c_double * 3), ("ra", ctypes.c_double), ("dec", ctypes.c_double),
+                ("tt_minus_utc", ctypes.c_double)]
+
+
+def lsrk_frame(obs_itrf, ra, dec, tt_minus_utc=69.184):
+    return LsrkFrame((ctypes.c_double * 3)(*[float(x) for x in obs_itrf]), float(ra), float(dec),
+                     float(tt_minus_utc))
 
 
 class GridOpts(ctypes.Structure):
@@ -114,7 +127,8 @@ EXPORTS = ["mti_version", "mti_create", "mti_destroy", "mti_last_error",
            "mti_fits_create", "mti_fits_write_channel", "mti_fits_close", "mti_finalize_sparse",
            "mti_set_transform_storage", "mti_get_transform_storage",
            "mti_preprocess_reference", "mti_uv_reorder", "mti_centre_shift", "mti_stokes", "mti_taper_correct",
-           "mti_taper_kernel", "mti_sinc_correct", "mti_normalizer_f32"]
+           "mti_taper_kernel", "mti_sinc_correct", "mti_normalizer_f32",
+           "mti_lsrk_factors", "mti_lsrk_inv_lambda"]
 
 _lib = None
 
@@ -198,7 +212,9 @@ def lib():
             ("mti_taper_correct", [vp, vp, i64, i32, i32, i32, vp]),
             ("mti_taper_kernel", [vp, i32, i32, i32, i32, vp]),
             ("mti_sinc_correct", [vp, vp, i64, i32, i32, i32, vp]),
-            ("mti_normalizer_f32", [vp, vp, i64, vp])]:
+            ("mti_normalizer_f32", [vp, vp, i64, vp]),
+            ("mti_lsrk_factors", [vp, P(LsrkFrame), vp, i64, vp, vp]),
+            ("mti_lsrk_inv_lambda", [vp, P(LsrkFrame), vp, i64, vp, i32, vp, vp])]:
         f = getattr(L, name)
         f.restype = ctypes.c_int
         f.argtypes = args
@@ -397,6 +413,21 @@ class Context:
                                  _stream(stream))
         self.check(rc, "mti_sort_rows")
         return int(kept.value)
+
+    # ---- TOPO -> LSRK -------------------------------------------------------
+    def lsrk_factors(self, frame, time_mjd_s, out, stream=None):
+        """out[r] = f_LSRK / f_TOPO at time_mjd_s[r] (f64 device tensors;
+        MJD seconds, UTC), for vis descriptors' row_doppler."""
+        rc = lib().mti_lsrk_factors(self.h, ctypes.byref(frame), time_mjd_s.data_ptr(), time_mjd_s.numel(),
+                                    out.data_ptr(), _stream(stream))
+        self.check(rc, "mti_lsrk_factors")
+
+    def lsrk_inv_lambda(self, frame, time_mjd_s, freq_hz, out, stream=None):
+        """The reference's per-record frequency store: f32 [nchan][nrows] of
+        f_LSRK / c."""
+        rc = lib().mti_lsrk_inv_lambda(self.h, ctypes.byref(frame), time_mjd_s.data_ptr(), time_mjd_s.numel(),
+                                       freq_hz.data_ptr(), freq_hz.numel(), out.data_ptr(), _stream(stream))
+        self.check(rc, "mti_lsrk_inv_lambda")
 
     # ---- binning ----------------------------------------------------------
     def preprocess(self, kern, params, vis, chan, bins_out, stream=None):
@@ -651,10 +682,11 @@ def _memcpy_d2d(dst, src, nbytes):
         raise MTIError(f"hipMemcpy failed ({rc})")
 
 
-def vis_ms(uvw, inv_lambda, vis, flags, weight, row_index=None):
+def vis_ms(uvw, inv_lambda, vis, flags, weight, row_index=None, row_doppler=None):
     """mti_vis for MeasurementSet-ordered torch tensors on the GPU:
     uvw f64 [nrows][3], inv_lambda f64 [nchan], vis complex64 [nrows][nchan][npol],
-    flags u8 [nrows][nchan][npol], weight f32 [nrows][npol], row_index int32 or None.
+    flags u8 [nrows][nchan][npol], weight f32 [nrows][npol], row_index int32 or None,
+    row_doppler f64 [nrows] (TOPO -> LSRK factors) or None.
     The descriptor keeps references to the tensors it points into."""
     nrows, nchan, npol = flags.shape
     for t in (uvw, inv_lambda, vis, flags, weight):
@@ -671,11 +703,14 @@ def vis_ms(uvw, inv_lambda, vis, flags, weight, row_index=None):
         v.row_index, v.nindex = row_index.data_ptr(), row_index.numel()
     else:
         v.row_index, v.nindex = None, 0
-    v._keep = (uvw, inv_lambda, vis, flags, weight, row_index)
+    if row_doppler is not None:
+        assert row_doppler.is_contiguous() and row_doppler.is_cuda and row_doppler.numel() == nrows
+        v.row_doppler = row_doppler.data_ptr()
+    v._keep = (uvw, inv_lambda, vis, flags, weight, row_index, row_doppler)
     return v
 
 
-def vis_chan_major(uvw, inv_lambda, vis, flags, weight, row_index=None):
+def vis_chan_major(uvw, inv_lambda, vis, flags, weight, row_index=None, row_doppler=None):
     """mti_vis for channel-major torch tensors on the GPU (a block of channels
     uploaded as one contiguous piece): uvw f64 [nrows][3], inv_lambda f64
     [nchan], vis complex64 [nchan][nrows][npol], flags u8 [nchan][nrows][npol],
@@ -695,7 +730,10 @@ def vis_chan_major(uvw, inv_lambda, vis, flags, weight, row_index=None):
         v.row_index, v.nindex = row_index.data_ptr(), row_index.numel()
     else:
         v.row_index, v.nindex = None, 0
-    v._keep = (uvw, inv_lambda, vis, flags, weight, row_index)
+    if row_doppler is not None:
+        assert row_doppler.is_contiguous() and row_doppler.is_cuda and row_doppler.numel() == nrows
+        v.row_doppler = row_doppler.data_ptr()
+    v._keep = (uvw, inv_lambda, vis, flags, weight, row_index, row_doppler)
     return v
 
 

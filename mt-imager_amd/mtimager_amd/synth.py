@@ -18,6 +18,7 @@ import numpy as np
 
 C_LIGHT = 299792458.0
 ARCSEC = np.pi / 180.0 / 3600.0
+MJD0_S = 60000.0 * 86400.0  # 2023-02-25 00:00 UTC in MJD seconds
 
 
 class Observation:
@@ -154,8 +155,13 @@ def make_observation(nant=27, radius_m=3000.0, lat_deg=-30.7, dec_deg=-30.0,
         vis = None
     flags = (rng.random((nrows, nchan)) < flag_frac)[:, :, None].repeat(npol, axis=2).astype(np.uint8)
     weight = np.ones((nrows, npol), np.float32)
+    # TIME column (MJD seconds, UTC): the hour-angle steps in solar seconds from
+    # MJD 60000, one value per time step shared by its rows
+    steps = np.linspace(ha_hours[0], ha_hours[1], ntime) * 3600.0 / 1.00273781191135448
+    time = np.repeat(MJD0_S + steps - steps[0], nrows // ntime)
     return Observation(uvw=uvw, ant1=a1, ant2=a2, freq=freqs_hz, inv_lambda=inv_lambda,
-                       vis=vis, flags=flags, weight=weight, nant=nant, src=src, npol=npol)
+                       vis=vis, flags=flags, weight=weight, nant=nant, src=src, npol=npol,
+                       time=time, lat_deg=lat_deg, dec_deg=dec_deg)
 
 
 def small_array_obs(seed=1, **kw):

@@ -37,6 +37,16 @@ class _Kernels(ctypes.Structure):
                 ("sums", ctypes.POINTER(ctypes.c_double))]
 
 
+class LsrkFrame(ctypes.Structure):
+    _fields_ = [("obs_itrf", ctypes.c_double * 3), ("ra", ctypes.c_double),
+                ("dec", ctypes.c_double), ("tt_minus_utc", ctypes.c_double)]
+
+
+def lsrk_frame(obs_itrf, ra, dec, tt_minus_utc=69.184):
+    return LsrkFrame((ctypes.c_double * 3)(*[float(x) for x in obs_itrf]), float(ra),
+                     float(dec), float(tt_minus_utc))
+
+
 BIN_DTYPE = np.dtype([("to_grid", np.int32), ("plane", np.int32),
                       ("support", np.int32), ("corner_x", np.int32),
                       ("corner_y", np.int32), ("off_x", np.int32),
@@ -86,6 +96,13 @@ def lib():
         L.orc_finalize.restype = ctypes.c_int
         L.orc_finalize.argtypes = [P(Params), ctypes.c_int, ctypes.c_int,
                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_lsrk_velocity.argtypes = [P(LsrkFrame), ctypes.c_double, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_earth_velocity.argtypes = [ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_lsrk_factor.restype = ctypes.c_double
+        L.orc_lsrk_factor.argtypes = [P(LsrkFrame), ctypes.c_double]
+        L.orc_lsrk_factors.argtypes = [P(LsrkFrame), ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
+        L.orc_set_row_doppler.argtypes = [ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -226,3 +243,44 @@ def finalize(params, grid_ri, norm, poltype="linear"):
     lib().orc_finalize(ctypes.byref(params), npol, POLTYPE[poltype], _ptr(grid_ri),
                        _ptr(norm), _ptr(img))
     return img
+
+
+# ---- TOPO -> LSRK (oracle/mti_oracle_lsrk.c) -------------------------------------
+def lsrk_velocity(frame, mjd_utc_s):
+    """(v_rot, v_bary, v_lsr) in m/s, J2000 equatorial axes."""
+    a, b, c = np.zeros(3), np.zeros(3), np.zeros(3)
+    lib().orc_lsrk_velocity(ctypes.byref(frame), float(mjd_utc_s), _ptr(a), _ptr(b), _ptr(c))
+    return a, b, c
+
+
+def earth_velocity(mjd_tt_s):
+    """(Earth relative to the Sun, Sun relative to the barycentre), m/s."""
+    a, b = np.zeros(3), np.zeros(3)
+    lib().orc_earth_velocity(float(mjd_tt_s), _ptr(a), _ptr(b))
+    return a, b
+
+
+def lsrk_factor(frame, mjd_utc_s):
+    return lib().orc_lsrk_factor(ctypes.byref(frame), float(mjd_utc_s))
+
+
+def lsrk_factors(frame, times):
+    t = np.ascontiguousarray(times, dtype=np.float64)
+    out = np.zeros(len(t))
+    lib().orc_lsrk_factors(ctypes.byref(frame), _ptr(t), len(t), _ptr(out))
+    return out
+
+
+class row_doppler:
+    """Context manager: per-record 1/lambda factors for preprocess/weights/grid
+    (rows in the order those calls receive them)."""
+
+    def __init__(self, factors):
+        self.f = np.ascontiguousarray(factors, dtype=np.float64)
+
+    def __enter__(self):
+        lib().orc_set_row_doppler(_ptr(self.f))
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_row_doppler(None)

@@ -393,6 +393,10 @@ int64_t orc_sort_index(const int32_t *ant1, const int32_t *ant2, int64_t nrows,
     return kept;
 }
 
+/* Per-record factor on 1/lambda (TOPO -> LSRK), set by orc_set_row_doppler. */
+static const double *g_row_doppler = NULL;
+void orc_set_row_doppler(const double *factor) { g_row_doppler = factor; }
+
 /* Binning: op/WProjPreProcessUVW.cu:111-286 restated in FP64.
  *  to_grid  = any correlation unflagged (:65-83) and support box inside the
  *             grid (:196-199);
@@ -411,11 +415,13 @@ void orc_preprocess(const orc_params *p, const orc_kernels *K,
     double du = 1.0 / (p->cell_l * (double)p->nx);
     double dv = 1.0 / (p->cell_m * (double)p->ny);
     int s = K->sampling, T = K->T, W = K->nplanes;
-    double f = inv_lambda[chan];
+    const double fc = inv_lambda[chan];
     double hx = (double)(p->nx / 2), hy = (double)(p->ny / 2);
     for (int64_t r = 0; r < nrows; r++) {
         orc_bin b;
         memset(&b, 0, sizeof(b));
+        /* per-record LSRK factor (vis/FrequencyConverter.cpp:112-141) */
+        const double f = g_row_doppler ? fc * g_row_doppler[r] : fc;
         const uint8_t *fl = flags + ((int64_t)r * nchan + chan) * npol;
         int any = 0;
         for (int q = 0; q < npol; q++)

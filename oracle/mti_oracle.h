@@ -105,6 +105,28 @@ void orc_fft1d(double *data, int n, int sign);
 int orc_finalize(const orc_params *p, int npol, int poltype,
                  const double *grid, const double *norm, double *image);
 
+/* TOPO -> LSRK (vis/FrequencyConverter.cpp:112-141; casacore Measures
+ * restated in mti_oracle_lsrk.c). */
+typedef struct {
+    double obs_itrf[3];   /* observatory geocentric ITRF position, metres    */
+    double ra, dec;       /* phase centre, J2000, radians                    */
+    double tt_minus_utc;  /* TT - UTC in seconds (69.184 since 2017)         */
+} orc_lsrk_frame;
+/* Velocities in m/s, J2000 equatorial axes: observatory rotation, Earth
+ * barycentric, solar motion relative to the LSR.  Time: MJD seconds (UTC). */
+void orc_lsrk_velocity(const orc_lsrk_frame *f, double mjd_utc_s, double v_rot[3],
+                       double v_bary[3], double v_lsr[3]);
+/* The Earth's velocity relative to the Sun and the Sun's relative to the
+ * solar-system barycentre, m/s, J2000 equatorial axes; time: MJD seconds (TT). */
+void orc_earth_velocity(double mjd_tt_s, double v_helio[3], double v_sun[3]);
+/* f_LSRK / f_TOPO at one time, and per record (one evaluation per run of
+ * equal times). */
+double orc_lsrk_factor(const orc_lsrk_frame *f, double mjd_utc_s);
+void   orc_lsrk_factors(const orc_lsrk_frame *f, const double *mjd_utc_s, int64_t n, double *out);
+/* Per-record factor on 1/lambda used by orc_preprocess/orc_weights/orc_grid
+ * (f = inv_lambda[chan] * factor[row]); NULL (the default) = no factor. */
+void   orc_set_row_doppler(const double *factor);
+
 #ifdef __cplusplus
 }
 #endif

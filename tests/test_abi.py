@@ -1,5 +1,6 @@
 """The C-ABI library loads without a GPU and exports every symbol that
-include/mtimager_amd.h and include/mtimager_amd_fits.h declare (no compute
+include/mtimager_amd.h, include/mtimager_amd_fits.h and
+include/mtimager_amd_ingest.h declare (no compute
 calls here)."""
 import ctypes
 import os
@@ -10,7 +11,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _declared():
     names = set()
-    for h in ("mtimager_amd.h", "mtimager_amd_fits.h"):
+    for h in ("mtimager_amd.h", "mtimager_amd_fits.h", "mtimager_amd_ingest.h"):
         src = open(os.path.join(ROOT, "include", h)).read()
         src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
         names |= set(re.findall(r"\b(mti_[a-z0-9_]+)\s*\(", src))
@@ -18,7 +19,8 @@ def _declared():
 
 
 def test_header_and_binding_agree(mti):
-    assert _declared() == set(mti.EXPORTS)
+    from mtimager_amd import ingest
+    assert _declared() == set(mti.EXPORTS) | set(ingest.EXPORTS)
 
 
 def test_library_exports_every_declared_symbol(mti):
