@@ -48,6 +48,11 @@ typedef enum { MTI_GRID_VUP = 0, MTI_GRID_PVU = 1 } mti_grid_layout;
  * over several work units, added with float atomics whose order (and so the
  * last-bit rounding) depends on the order units finish. */
 typedef enum { MTI_GRID_C64 = 0, MTI_GRID_C128 = 1 } mti_grid_precision;
+/* Gridder accumulation (a precision form): FP64 registers (default), or FP32
+ * registers per batch of at most 256 records with the batch sums added by
+ * Kahan's compensated summation in FP32 (binning, oversample indices and the
+ * w-plane choice stay FP64 either way). */
+typedef enum { MTI_ACC_F64 = 0, MTI_ACC_F32_KAHAN = 1 } mti_accumulation;
 
 /* Imaging parameters; the fields carry the mt-imager property keys
  * (comp/ConfigurationManager.cpp:33-62, conf/mtimager.properties:1-78). */
@@ -137,6 +142,14 @@ typedef struct {
      * mti_finalize_sparse reads those tiles as zeros.
      * No counterpart in the reference (its grids are always complete). */
     uint8_t *tile_mask;
+    /* 1: multi-frequency synthesis -- the nchan channels are gridded into ONE
+     * grid and ONE normaliser (grids, norm and tile_mask then hold a single
+     * channel), uniform/Briggs densities summed over those channels.  The
+     * reference images one cube plane per channel (app/mtimager.cpp:151-170);
+     * feeding a WImager several channels' per-record frequencies is its MFS.
+     * The channel range must fit one gridding pass (< 2^31 records). */
+    int32_t mfs;
+    int32_t accumulation;     /* mti_accumulation (0: FP64)                      */
 } mti_grid_opts;
 
 typedef struct {

@@ -206,8 +206,9 @@ __global__ __launch_bounds__(BT_THREADS) void k_bin_pass(BinPassArgs a)
                         }
                     }
                 }
-                // entries behind the kept rows (device row order) are not gridded
-                a.cjc[e] = (job << 16) | cl;
+                // entries behind the kept rows (device row order) are not gridded;
+                // the tag names the record's grid (channel 0 for MFS jobs)
+                a.cjc[e] = (job << 16) | (jd.mfs ? 0 : cl);
                 a.desc[e] = d;
             }
         }

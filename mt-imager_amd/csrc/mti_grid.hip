@@ -500,10 +500,20 @@ __device__ __forceinline__ void flush_cell(R *g, int64_t ps, const double (&acc)
 // f32 tables: two 1024-lane workgroups per CU (8 waves per SIMD, at most 64
 // VGPRs) is a requirement, not a hope -- one more register halves the
 // occupancy.  The f16/bf16 forms convert samples and need more registers.
-template <int P, int PREC>
-__global__ __launch_bounds__(TG_THREADS, PREC == MTI_TABLE_F32 ? 8 : 1) void k_grid_tiles(GridArgs a)
+//
+// ACC selects the accumulation: MTI_ACC_F64 (FP64 registers), or
+// MTI_ACC_F32_KAHAN -- each batch's taps summed in FP32 registers (a batch
+// holds at most TG_BATCH records), and the batch sums added with Kahan's
+// compensated summation into an FP32 (sum, compensation) pair per cell
+// component; the cell's value sum - compensation is then added to the grid.
+// Visibilities are staged as FP32 and samples stay FP32 (no conversions).
+template <int P, int PREC, int ACC>
+__global__ __launch_bounds__(TG_THREADS, PREC == MTI_TABLE_F32 || ACC == MTI_ACC_F32_KAHAN ? 8 : 1) void
+k_grid_tiles(GridArgs a)
 {
-    __shared__ __align__(16) double s_vd[TG_BATCH][2 * P];  // weighted visibility
+    constexpr bool F32 = ACC == MTI_ACC_F32_KAHAN;
+    constexpr int VB = F32 ? (int)sizeof(float) : (int)sizeof(double);  // bytes per staged component
+    __shared__ __align__(16) double s_vd[TG_BATCH][2 * P];  // weighted visibility (FP64, or FP32 packed)
     __shared__ ListEntry s_list[TG_REGIONS][TG_BATCH];  // records per region
     __shared__ int s_cnt[TG_REGIONS];
     static_assert(TG_BATCH * 2 * P * sizeof(double) < 65536, "visibility offsets fit 16 bits");
@@ -533,8 +543,12 @@ __global__ __launch_bounds__(TG_THREADS, PREC == MTI_TABLE_F32 ? 8 : 1) void k_g
         const int x0 = ta.tx * TG_T, y0 = ta.ty * TG_T;
         const int gx = x0 + rx, gy = y0 + ry;
         double acc[2 * P];
+        float accf[2 * P], ksum[2 * P], kcmp[2 * P];
 #pragma unroll
-        for (int q = 0; q < 2 * P; q++) acc[q] = 0.0;
+        for (int q = 0; q < 2 * P; q++) {
+            acc[q] = 0.0;
+            accf[q] = ksum[q] = kcmp[q] = 0.0f;
+        }
 
         for (int b0 = 0; b0 < npairs; b0 += TG_BATCH) {
             const int nbat = min(TG_BATCH, npairs - b0);
@@ -549,8 +563,12 @@ __global__ __launch_bounds__(TG_THREADS, PREC == MTI_TABLE_F32 ? 8 : 1) void k_g
                     const uint32_t j = a.pairs[first + b0 + rr] & PAIR_INDEX;
                     if (part < P) {
                         const float2 v = a.wvis[(int64_t)j * P + part];
-                        s_vd[rr][2 * part] = v.x;
-                        s_vd[rr][2 * part + 1] = v.y;
+                        if (F32) {
+                            ((float2 *)&s_vd[0][0])[rr * P + part] = v;
+                        } else {
+                            s_vd[rr][2 * part] = v.x;
+                            s_vd[rr][2 * part + 1] = v.y;
+                        }
                     }
                     if (part == 0) {
                         const int4 d = a.cdesc[j];
@@ -559,7 +577,7 @@ __global__ __launch_bounds__(TG_THREADS, PREC == MTI_TABLE_F32 ? 8 : 1) void k_g
                         ListEntry e;
                         e.xy = d.y;
                         e.koff = d.z;
-                        e.s_vo = S | ((rr * 2 * P * (int)sizeof(double)) << 16);
+                        e.s_vo = S | ((rr * 2 * P * VB) << 16);
                         const int ix0 = max(cx, x0) - x0, ix1 = min(cx + S, x0 + TG_T) - x0;
                         const int iy0 = max(cy, y0) - y0, iy1 = min(cy + S, y0 + TG_T) - y0;
                         for (int gyr = iy0 / TG_RH; gyr <= (iy1 - 1) / TG_RH; gyr++)
@@ -581,15 +599,26 @@ __global__ __launch_bounds__(TG_THREADS, PREC == MTI_TABLE_F32 ? 8 : 1) void k_g
             int next = 4;
             auto visit = [&](Visit &v) {
                 if (v.live) {
-                    const double kx = v.k.x, ky = v.k.y;
-                    // 16-byte aligned pairs: one ds_read_b128 per correlation
-                    const double2 *vis = (const double2 *)((const char *)&s_vd[0][0] + v.vo);
+                    if (F32) {
+                        const float kx = v.k.x, ky = v.k.y;
+                        const float2 *vis = (const float2 *)((const char *)&s_vd[0][0] + v.vo);
 #pragma unroll
-                    for (int p = 0; p < P; p++) {
-                        const double2 x = vis[p];
-                        const double vr = x.x, vi = x.y;
-                        acc[2 * p] = fma(-vi, ky, fma(vr, kx, acc[2 * p]));
-                        acc[2 * p + 1] = fma(vr, ky, fma(vi, kx, acc[2 * p + 1]));
+                        for (int p = 0; p < P; p++) {
+                            const float2 x = vis[p];
+                            accf[2 * p] = fmaf(-x.y, ky, fmaf(x.x, kx, accf[2 * p]));
+                            accf[2 * p + 1] = fmaf(x.x, ky, fmaf(x.y, kx, accf[2 * p + 1]));
+                        }
+                    } else {
+                        const double kx = v.k.x, ky = v.k.y;
+                        // 16-byte aligned pairs: one ds_read_b128 per correlation
+                        const double2 *vis = (const double2 *)((const char *)&s_vd[0][0] + v.vo);
+#pragma unroll
+                        for (int p = 0; p < P; p++) {
+                            const double2 x = vis[p];
+                            const double vr = x.x, vi = x.y;
+                            acc[2 * p] = fma(-vi, ky, fma(vr, kx, acc[2 * p]));
+                            acc[2 * p + 1] = fma(vr, ky, fma(vi, kx, acc[2 * p + 1]));
+                        }
                     }
                 }
                 v = visit_issue<PREC>(next++, n, list, gx, gy, table);
@@ -600,6 +629,21 @@ __global__ __launch_bounds__(TG_THREADS, PREC == MTI_TABLE_F32 ? 8 : 1) void k_g
                 visit(vc);
                 visit(vd);
             }
+            if (F32) {
+                // Kahan: the batch sum into (ksum, kcmp)
+#pragma unroll
+                for (int q = 0; q < 2 * P; q++) {
+                    const float y = accf[q] - kcmp[q];
+                    const float t = ksum[q] + y;
+                    kcmp[q] = (t - ksum[q]) - y;
+                    ksum[q] = t;
+                    accf[q] = 0.0f;
+                }
+            }
+        }
+        if (F32) {
+#pragma unroll
+            for (int q = 0; q < 2 * P; q++) acc[q] = (double)ksum[q] - (double)kcmp[q];
         }
         // add the cell to the grid
         if (gx < jd.g.nx && gy < jd.g.ny) {
@@ -715,21 +759,28 @@ static int persistent_blocks(mti_ctx *ctx, K kern, int threads)
     return ctx->num_cus * per_cu;
 }
 
-template <int P, int PREC>
+template <int P, int PREC, int ACC>
 static mti_status launch_tiles(mti_ctx *ctx, const GridArgs &a, hipStream_t st)
 {
-    auto kern = k_grid_tiles<P, PREC>;
+    auto kern = k_grid_tiles<P, PREC, ACC>;
     hipLaunchKernelGGL(kern, dim3(persistent_blocks(ctx, kern, TG_THREADS)), dim3(TG_THREADS), 0, st, a);
     MTI_HIP(ctx, hipGetLastError());
     return MTI_OK;
 }
 
-template <int P>
-static mti_status dispatch_tiles(mti_ctx *ctx, int prec, const GridArgs &a, hipStream_t st)
+template <int P, int ACC>
+static mti_status dispatch_tiles_acc(mti_ctx *ctx, int prec, const GridArgs &a, hipStream_t st)
 {
-    if (prec == MTI_TABLE_F32) return launch_tiles<P, MTI_TABLE_F32>(ctx, a, st);
-    if (prec == MTI_TABLE_F16) return launch_tiles<P, MTI_TABLE_F16>(ctx, a, st);
-    return launch_tiles<P, MTI_TABLE_BF16>(ctx, a, st);
+    if (prec == MTI_TABLE_F32) return launch_tiles<P, MTI_TABLE_F32, ACC>(ctx, a, st);
+    if (prec == MTI_TABLE_F16) return launch_tiles<P, MTI_TABLE_F16, ACC>(ctx, a, st);
+    return launch_tiles<P, MTI_TABLE_BF16, ACC>(ctx, a, st);
+}
+
+template <int P>
+static mti_status dispatch_tiles(mti_ctx *ctx, int prec, int acc, const GridArgs &a, hipStream_t st)
+{
+    if (acc == MTI_ACC_F32_KAHAN) return dispatch_tiles_acc<P, MTI_ACC_F32_KAHAN>(ctx, prec, a, st);
+    return dispatch_tiles_acc<P, MTI_ACC_F64>(ctx, prec, a, st);
 }
 
 template <int P>
@@ -768,7 +819,11 @@ struct PassJob {
     double *norm;      // [nb][P]
     uint8_t *mask;     // [nb][nty][ntx] tile mask (null: complete grids)
     int c64 = 0;       // complex64 grids
+    int mfs = 0;       // multi-frequency synthesis: the nb channels share one grid
 };
+
+// Grids (and normalisers, tile masks, density channels) a job piece writes.
+static int grids_of(const PassJob &j) { return j.mfs ? 1 : j.nb; }
 
 // Limits of one pass: records < 2^31, (record, tile) pairs < 2^32, keys < 2^31.
 static int64_t job_records(const PassJob &j) { return j.vw.nidx * j.nb; }
@@ -779,7 +834,7 @@ static int64_t job_pairs(const PassJob &j)
 }
 static int64_t job_keys(const PassJob &j)
 {
-    return (int64_t)j.nb * ((j.p.nx + TG_T - 1) / TG_T) * ((j.p.ny + TG_T - 1) / TG_T);
+    return (int64_t)grids_of(j) * ((j.p.nx + TG_T - 1) / TG_T) * ((j.p.ny + TG_T - 1) / TG_T);
 }
 
 static PassJob one_channel(PassJob j)
@@ -802,7 +857,7 @@ static PassSizes pass_sizes(const std::vector<PassJob> &jobs, int P, int chunk)
     for (const PassJob &j : jobs) {
         z.nrec += job_records(j);
         z.nkeys += job_keys(j);
-        z.nch += j.nb;
+        z.nch += grids_of(j);
         z.pair_cap += job_pairs(j);
     }
     z.unit_cap = z.nkeys + z.pair_cap / chunk + 1;
@@ -867,10 +922,11 @@ static mti_status grid_pass(mti_ctx *ctx, const std::vector<PassJob> &jobs, cons
         d.psf = o->psf;
         d.weighting = o->weighting;
         d.tile_mask = j.mask;
+        d.mfs = j.mfs;
         rec0[i + 1] = rec0[i] + job_records(j);
         key0[i + 1] = key0[i] + job_keys(j);
         blk0[i + 1] = blk0[i] + bin_blocks(j.vw.nidx, j.nb);
-        jc0[i + 1] = jc0[i] + j.nb;
+        jc0[i + 1] = jc0[i] + grids_of(j);
         pair_cap += job_pairs(j);
         d.rec0 = rec0[i];
         d.key0 = key0[i];
@@ -887,10 +943,10 @@ static mti_status grid_pass(mti_ctx *ctx, const std::vector<PassJob> &jobs, cons
                     hipLaunchKernelGGL(k_fill_u8, dim3(64), dim3(256), 0, st, j.mask, job_keys(j), (uint8_t)0);
                 else if (j.c64)  // 0.0f is all zero bits
                     hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, st, (uint32_t *)j.grid,
-                                       (int64_t)j.p.nx * j.p.ny * P * 2 * j.nb, 0u);
+                                       (int64_t)j.p.nx * j.p.ny * P * 2 * grids_of(j), 0u);
                 else
                     hipLaunchKernelGGL(k_fill_f64, dim3(1024), dim3(256), 0, st, (double *)j.grid,
-                                       (int64_t)j.p.nx * j.p.ny * P * 2 * j.nb, 0.0);
+                                       (int64_t)j.p.nx * j.p.ny * P * 2 * grids_of(j), 0.0);
             }
         MTI_HIP(ctx, hipGetLastError());
         return MTI_OK;
@@ -1040,9 +1096,9 @@ static mti_status grid_pass(mti_ctx *ctx, const std::vector<PassJob> &jobs, cons
     {
         StageTimer tm(ctx, MTI_STAGE_GRID, st);
         switch (P) {
-        case 1: r = dispatch_tiles<1>(ctx, prec, ga, st); break;
-        case 2: r = dispatch_tiles<2>(ctx, prec, ga, st); break;
-        default: r = dispatch_tiles<4>(ctx, prec, ga, st); break;
+        case 1: r = dispatch_tiles<1>(ctx, prec, o->accumulation, ga, st); break;
+        case 2: r = dispatch_tiles<2>(ctx, prec, o->accumulation, ga, st); break;
+        default: r = dispatch_tiles<4>(ctx, prec, o->accumulation, ga, st); break;
         }
         if (r != MTI_OK) return r;
     }
@@ -1065,6 +1121,9 @@ static mti_status check_opts(mti_ctx *ctx, const mti_grid_opts *o)
     MTI_CHECK(ctx, o->weighting >= MTI_WEIGHT_NATURAL && o->weighting <= MTI_WEIGHT_BRIGGS, "bad weighting");
     MTI_CHECK(ctx, o->grid_layout == MTI_GRID_VUP || o->grid_layout == MTI_GRID_PVU, "bad grid_layout");
     MTI_CHECK(ctx, o->grid_precision == MTI_GRID_C64 || o->grid_precision == MTI_GRID_C128, "bad grid_precision");
+    MTI_CHECK(ctx, o->mfs == 0 || o->mfs == 1, "mfs must be 0 or 1");
+    MTI_CHECK(ctx, o->accumulation == MTI_ACC_F64 || o->accumulation == MTI_ACC_F32_KAHAN,
+              "accumulation must be MTI_ACC_F64 or MTI_ACC_F32_KAHAN");
     return MTI_OK;
 }
 
@@ -1118,7 +1177,9 @@ static mti_status for_each_pass(const std::vector<PassJob> &all, int P, F fn)
     std::vector<PassJob> pass;
     int64_t in_r = 0, in_p = 0, in_k = 0;
     for (const PassJob &j : all) {
-        const int64_t cap = channels_per_pass(j);
+        // an MFS job's channels share one density and grid: never split (the
+        // entry points check that it fits one pass)
+        const int64_t cap = j.mfs ? j.nb : channels_per_pass(j);
         const int64_t gch = (int64_t)j.p.nx * j.p.ny * P;  // complex elements per channel grid
         for (int c0 = 0; c0 < j.nb;) {
             PassJob piece = j;
@@ -1172,6 +1233,7 @@ static void plan_grid_jobs(const std::vector<PassJob> &all, int32_t njobs, const
 // Channels per pass of one mti_grid call (record / pair limits).
 static int grid_batch(const PassJob &base, int nb)
 {
+    if (base.mfs) return nb;
     return (int)std::min<int64_t>(nb, std::max<int64_t>(1, channels_per_pass(base)));
 }
 
@@ -1212,7 +1274,8 @@ extern "C" mti_status mti_grid_workspace_size(const mti_kernel_info *ki, const m
         return MTI_ERR_INVALID;
     mti_kernels kk;
     kk.info = *ki;
-    const PassJob base{&kk, *p, make_view(v), o->chan0, 1, nullptr, o->grid_layout, nullptr};
+    PassJob base{&kk, *p, make_view(v), o->chan0, 1, nullptr, o->grid_layout, nullptr};
+    base.mfs = o->mfs;
     ScratchPlan pl;
     plan_grid(base, o->nchan, v->npol, o, pl);
     *bytes = pl.total();
@@ -1247,6 +1310,12 @@ extern "C" mti_status mti_grid(mti_ctx *ctx, const mti_kernels *k, const mti_ima
     // channel batches keep a pass within the record / pair limits
     PassJob base{k, *p, vw, o->chan0, 1, nullptr, o->grid_layout, nullptr};
     base.c64 = c64;
+    base.mfs = o->mfs;
+    // multi-frequency synthesis: every channel into one grid and normaliser,
+    // one density (the whole channel range in one pass)
+    const int ngrids = o->mfs ? 1 : nb;
+    if (o->mfs)
+        MTI_CHECK(ctx, channels_per_pass(base) >= nb, "MFS channel range above one pass's record/tile limits");
     CallScope cs(ctx);
     if (cs.planning()) {
         ScratchPlan pl;
@@ -1255,16 +1324,19 @@ extern "C" mti_status mti_grid(mti_ctx *ctx, const mti_kernels *k, const mti_ima
     }
 
     if (!o->accumulate) {
-        hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(256), 0, st, norm, (int64_t)nb * P, 0.0);
+        hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(256), 0, st, norm, (int64_t)ngrids * P, 0.0);
         // with rows, the gridding pass zeroes exactly the tiles it does not store whole
         if (vw.nidx == 0) {
             if (o->tile_mask)
                 hipLaunchKernelGGL(k_fill_u8, dim3(64), dim3(256), 0, st, o->tile_mask,
-                                   (int64_t)nb * ((p->nx + TG_T - 1) / TG_T) * ((p->ny + TG_T - 1) / TG_T), (uint8_t)0);
+                                   (int64_t)ngrids * ((p->nx + TG_T - 1) / TG_T) * ((p->ny + TG_T - 1) / TG_T),
+                                   (uint8_t)0);
             else if (c64)  // 0.0f is all zero bits
-                hipLaunchKernelGGL(k_fill_u32, dim3(8192), dim3(256), 0, st, (uint32_t *)grids, chan_reals * nb, 0u);
+                hipLaunchKernelGGL(k_fill_u32, dim3(8192), dim3(256), 0, st, (uint32_t *)grids, chan_reals * ngrids,
+                                   0u);
             else
-                hipLaunchKernelGGL(k_fill_f64, dim3(8192), dim3(256), 0, st, (double *)grids, chan_reals * nb, 0.0);
+                hipLaunchKernelGGL(k_fill_f64, dim3(8192), dim3(256), 0, st, (double *)grids, chan_reals * ngrids,
+                                   0.0);
         }
         MTI_HIP(ctx, hipGetLastError());
     }
@@ -1306,7 +1378,8 @@ extern "C" mti_status mti_grid_jobs_workspace_size(int32_t njobs, const mti_job 
         if (j.vis.npol != P) return MTI_ERR_INVALID;
         all.push_back(PassJob{j.kernels, j.params, make_view(&j.vis), j.chan0, j.nchan, nullptr, o->grid_layout,
                               nullptr});
-        if (channels_per_pass(all.back()) < 1) return MTI_ERR_INVALID;
+        all.back().mfs = o->mfs;
+        if (channels_per_pass(all.back()) < (o->mfs ? j.nchan : 1)) return MTI_ERR_INVALID;
     }
     ScratchPlan pl;
     plan_grid_jobs(all, njobs, jobs, P, o->weighting != MTI_WEIGHT_NATURAL, pl);
@@ -1353,7 +1426,10 @@ extern "C" mti_status mti_grid_jobs(mti_ctx *ctx, int32_t njobs, const mti_job *
                          j.norm, j.tile_mask, j.grids_c64};
         MTI_CHECK(ctx, pj.vw.nidx < (int64_t(1) << 31), "more than 2^31-1 rows in a job");
         MTI_CHECK(ctx, channels_per_pass(pj) >= 1, "one channel has too many (record, tile) pairs");
+        if (o->mfs)
+            MTI_CHECK(ctx, channels_per_pass(pj) >= j.nchan, "MFS job above one pass's record/tile limits");
         all.push_back(pj);
+        all.back().mfs = o->mfs;
     }
     CallScope cs(ctx);
     if (cs.planning()) {
@@ -1395,7 +1471,7 @@ extern "C" mti_status mti_grid_jobs(mti_ctx *ctx, int32_t njobs, const mti_job *
     // zero the normalisers; the grids are zeroed by the passes (tiles not stored whole)
     if (!o->accumulate) {
         for (const PassJob &j : all)
-            hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(64), 0, st, j.norm, (int64_t)j.nb * P, 0.0);
+            hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(64), 0, st, j.norm, (int64_t)grids_of(j) * P, 0.0);
         MTI_HIP(ctx, hipGetLastError());
     }
     r = for_each_pass(all, P, [&](const std::vector<PassJob> &pass) {

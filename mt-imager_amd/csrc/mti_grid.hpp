@@ -57,6 +57,7 @@ struct JobDev {
     double *f2;              // [nb][P] Briggs factor
     int psf, weighting;
     uint8_t *tile_mask;      // [nb][nty][ntx]: 1 where a record reaches the tile; null: complete grids
+    int mfs;                 // 1: every channel grids into grid 0 (one grid, normaliser, density)
 };
 
 // The jobs of a call and prefix tables for locating them.

@@ -17,6 +17,7 @@ WEIGHTING = {"natural": 0, "uniform": 1, "briggs": 2}
 POLTYPE = {"linear": 0, "circular": 1}
 LAYOUT = {"vup": 0, "pvu": 1}
 GRID_PRECISION = {"c64": 0, "c128": 1}
+ACCUMULATION = {"f64": 0, "f32k": 1}
 ARCSEC = 3.141592653589793 / 180.0 / 3600.0
 
 
@@ -72,7 +73,7 @@ class GridOpts(ctypes.Structure):
                 ("accumulate", ctypes.c_int32), ("psf", ctypes.c_int32),
                 ("weighting", ctypes.c_int32), ("robust", ctypes.c_double),
                 ("grid_layout", ctypes.c_int32), ("grid_precision", ctypes.c_int32),
-                ("tile_mask", ctypes.c_void_p)]
+                ("tile_mask", ctypes.c_void_p), ("mfs", ctypes.c_int32), ("accumulation", ctypes.c_int32)]
 
 
 class Job(ctypes.Structure):
@@ -238,10 +239,11 @@ def make_params(nx, ny, cell_l, cell_m=None, mode="wproj", wplanes=1, sampling=4
 
 
 def _grid_opts(chan0=0, nchan=1, accumulate=False, psf=False, weighting="natural", robust=0.0,
-               layout="vup", precision="c128", tile_mask=None):
+               layout="vup", precision="c128", tile_mask=None, mfs=False, accumulation="f64"):
     return GridOpts(int(chan0), int(nchan), 1 if accumulate else 0, 1 if psf else 0,
                     WEIGHTING[weighting], float(robust), LAYOUT[layout], GRID_PRECISION[precision],
-                    tile_mask.data_ptr() if tile_mask is not None else None)
+                    tile_mask.data_ptr() if tile_mask is not None else None, 1 if mfs else 0,
+                    ACCUMULATION[accumulation])
 
 
 def tile_mask_shape(params, nchan):
@@ -448,16 +450,19 @@ class Context:
     # ---- gridding ---------------------------------------------------------
     def grid(self, kern, params, vis, grids, norm, chan0=0, nchan=1, accumulate=False,
              psf=False, weighting="natural", robust=0.0, layout="vup", precision=None,
-             stats=True, stream=None, tile_mask=None):
+             stats=True, stream=None, tile_mask=None, mfs=False, accumulation="f64"):
         """grids: per channel [ny][nx][npol] (layout "vup", the reference's
         Grid) or [npol][ny][nx] ("pvu"); complex64 or complex128 (precision
         "c64"/"c128", taken from the tensor dtype by default).  tile_mask
         (uint8 device tensor of tile_mask_shape): sparse grids -- tiles no
         record reaches are marked 0 and left unwritten (finalize with the
-        same mask)."""
+        same mask).  mfs: the nchan channels go into one grid and normaliser
+        (multi-frequency synthesis).  accumulation: "f64" (default) or "f32k"
+        (FP32 batch sums with Kahan-compensated totals)."""
         if precision is None:
             precision = "c128" if str(grids.dtype) in ("torch.complex128", "torch.float64") else "c64"
-        o = _grid_opts(chan0, nchan, accumulate, psf, weighting, robust, layout, precision, tile_mask)
+        o = _grid_opts(chan0, nchan, accumulate, psf, weighting, robust, layout, precision, tile_mask, mfs,
+                       accumulation)
         st = GridStats()
         rc = lib().mti_grid(self.h, kern.h, ctypes.byref(params), ctypes.byref(vis),
                             ctypes.byref(o), grids.data_ptr(), norm.data_ptr(),
@@ -509,10 +514,11 @@ class Context:
         return j
 
     def grid_jobs(self, jobs, accumulate=False, psf=False, weighting="natural", robust=0.0,
-                  layout="vup", stats=True, stream=None):
+                  layout="vup", stats=True, stream=None, mfs=False, accumulation="f64"):
         arr = (Job * len(jobs))(*jobs)
         o = GridOpts(0, 0, 1 if accumulate else 0, 1 if psf else 0, WEIGHTING[weighting],
-                     float(robust), LAYOUT[layout], GRID_PRECISION["c128"])
+                     float(robust), LAYOUT[layout], GRID_PRECISION["c128"], None, 1 if mfs else 0,
+                     ACCUMULATION[accumulation])
         st = GridStats()
         rc = lib().mti_grid_jobs(self.h, len(jobs), ctypes.cast(arr, ctypes.c_void_p), ctypes.byref(o),
                                  ctypes.byref(st) if stats else None, _stream(stream))

@@ -85,6 +85,12 @@ def lib():
                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                   ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                   ctypes.c_void_p]
+        L.orc_weights_mfs.restype = ctypes.c_int
+        L.orc_weights_mfs.argtypes = [P(Params), P(_Kernels), ctypes.c_int64,
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                      ctypes.c_void_p]
         L.orc_grid.restype = ctypes.c_int
         L.orc_grid.argtypes = [P(Params), P(_Kernels), ctypes.c_int64,
                                ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
@@ -199,6 +205,21 @@ def weights(kern, uvw, inv_lambda, flags, weight, chan, weighting="natural",
                       nchan, npol, _ptr(uvw), _ptr(inv_lambda), _ptr(flags),
                       _ptr(weight), int(chan), WEIGHTING[weighting], float(robust),
                       _ptr(out))
+    return out
+
+
+def weights_mfs(kern, uvw, inv_lambda, flags, weight, chan0, nch, weighting="natural", robust=0.0):
+    """Weights with one density over channels chan0 .. chan0 + nch - 1:
+    [nch][nrows][npol]."""
+    uvw = np.ascontiguousarray(uvw, dtype=np.float64)
+    inv_lambda = np.ascontiguousarray(inv_lambda, dtype=np.float64)
+    flags = np.ascontiguousarray(flags, dtype=np.uint8)
+    weight = np.ascontiguousarray(weight, dtype=np.float32)
+    nrows, nchan, npol = flags.shape
+    out = np.zeros((nch, nrows, npol), dtype=np.float64)
+    lib().orc_weights_mfs(ctypes.byref(kern.params), ctypes.byref(kern._k), nrows, nchan, npol,
+                          _ptr(uvw), _ptr(inv_lambda), _ptr(flags), _ptr(weight), int(chan0), int(nch),
+                          WEIGHTING[weighting], float(robust), _ptr(out))
     return out
 
 

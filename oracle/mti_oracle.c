@@ -483,24 +483,41 @@ int orc_weights(const orc_params *p, const orc_kernels *K,
                 const uint8_t *flags, const float *weight, int chan,
                 int weighting, double robust, double *wout)
 {
-    orc_bin *b = (orc_bin *)malloc(sizeof(orc_bin) * (size_t)(nrows > 0 ? nrows : 1));
-    orc_preprocess(p, K, nrows, nchan, npol, uvw, inv_lambda, flags, chan, b);
-    for (int64_t r = 0; r < nrows; r++)
-        for (int q = 0; q < npol; q++) wout[r * npol + q] = weight[r * npol + q];
+    return orc_weights_mfs(p, K, nrows, nchan, npol, uvw, inv_lambda, flags, weight, chan, 1,
+                           weighting, robust, wout);
+}
+
+/* The same definitions with one density over the records of channels
+ * chan0 .. chan0 + nch - 1 (multi-frequency synthesis: those channels share
+ * one grid).  wout [nch][nrows][npol]. */
+int orc_weights_mfs(const orc_params *p, const orc_kernels *K,
+                    int64_t nrows, int nchan, int npol,
+                    const double *uvw, const double *inv_lambda,
+                    const uint8_t *flags, const float *weight, int chan0, int nch,
+                    int weighting, double robust, double *wout)
+{
+    const int64_t nb = nrows > 0 ? nrows : 1;
+    orc_bin *b = (orc_bin *)malloc(sizeof(orc_bin) * (size_t)nb * (size_t)nch);
+    for (int c = 0; c < nch; c++)
+        orc_preprocess(p, K, nrows, nchan, npol, uvw, inv_lambda, flags, chan0 + c, b + (int64_t)c * nb);
+    for (int c = 0; c < nch; c++)
+        for (int64_t r = 0; r < nrows; r++)
+            for (int q = 0; q < npol; q++) wout[((int64_t)c * nrows + r) * npol + q] = weight[r * npol + q];
     if (weighting == 0) {
         free(b);
         return 0;
     }
     int64_t ncell = (int64_t)p->nx * p->ny;
     double *dens = (double *)calloc((size_t)ncell * npol, sizeof(double));
-    for (int64_t r = 0; r < nrows; r++) {
-        if (!b[r].to_grid) continue;
-        const uint8_t *fl = flags + ((int64_t)r * nchan + chan) * npol;
-        int64_t cell = (int64_t)(b[r].corner_y + b[r].support / 2) * p->nx +
-                       (b[r].corner_x + b[r].support / 2);
-        for (int q = 0; q < npol; q++)
-            dens[q * ncell + cell] += (double)weight[r * npol + q] * (fl[q] ? 0.0 : 1.0);
-    }
+    for (int c = 0; c < nch; c++)
+        for (int64_t r = 0; r < nrows; r++) {
+            const orc_bin *br = b + (int64_t)c * nb + r;
+            if (!br->to_grid) continue;
+            const uint8_t *fl = flags + ((int64_t)r * nchan + chan0 + c) * npol;
+            int64_t cell = (int64_t)(br->corner_y + br->support / 2) * p->nx + (br->corner_x + br->support / 2);
+            for (int q = 0; q < npol; q++)
+                dens[q * ncell + cell] += (double)weight[r * npol + q] * (fl[q] ? 0.0 : 1.0);
+        }
     for (int q = 0; q < npol; q++) {
         double f2 = 0.0;
         if (weighting == 2) {
@@ -512,17 +529,19 @@ int orc_weights(const orc_params *p, const orc_kernels *K,
             double k = 5.0 * pow(10.0, -robust);
             f2 = (sw2 > 0.0) ? (k * k) / (sw2 / sw) : 0.0;
         }
-        for (int64_t r = 0; r < nrows; r++) {
-            if (!b[r].to_grid) continue;
-            int64_t cell = (int64_t)(b[r].corner_y + b[r].support / 2) * p->nx +
-                           (b[r].corner_x + b[r].support / 2);
-            double d = dens[q * ncell + cell];
-            double w = weight[r * npol + q];
-            if (weighting == 1)
-                wout[r * npol + q] = d > 0.0 ? w / d : 0.0;
-            else
-                wout[r * npol + q] = w / (1.0 + f2 * d);
-        }
+        for (int c = 0; c < nch; c++)
+            for (int64_t r = 0; r < nrows; r++) {
+                const orc_bin *br = b + (int64_t)c * nb + r;
+                if (!br->to_grid) continue;
+                int64_t cell = (int64_t)(br->corner_y + br->support / 2) * p->nx + (br->corner_x + br->support / 2);
+                double d = dens[q * ncell + cell];
+                double w = weight[r * npol + q];
+                double *o = &wout[((int64_t)c * nrows + r) * npol + q];
+                if (weighting == 1)
+                    *o = d > 0.0 ? w / d : 0.0;
+                else
+                    *o = w / (1.0 + f2 * d);
+            }
     }
     free(dens);
     free(b);

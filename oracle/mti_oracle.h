@@ -84,6 +84,14 @@ int orc_weights(const orc_params *p, const orc_kernels *K,
                 const uint8_t *flags, const float *weight, int chan,
                 int weighting, double robust, double *wout);
 
+/* Weights with one density over channels chan0 .. chan0 + nch - 1 (MFS:
+ * those channels share one grid); wout [nch][nrows][npol]. */
+int orc_weights_mfs(const orc_params *p, const orc_kernels *K,
+                    int64_t nrows, int nchan, int npol,
+                    const double *uvw, const double *inv_lambda,
+                    const uint8_t *flags, const float *weight, int chan0, int nch,
+                    int weighting, double robust, double *wout);
+
 /* Grid one channel.  vis complex64 [nrows][nchan][npol]; flags u8 same
  * layout; weight f32 [nrows][npol] or wts f64 [nrows][npol] (if non-NULL,
  * overrides weight).  psf!=0 replaces every visibility by 1+0i.

@@ -44,7 +44,8 @@ def gpu_sort(ctx, dev, nant, torch):
 
 
 def gpu_image(mti, ctx, kern, params, dev, torch, chans, index=None, psf=False,
-              weighting="natural", robust=0.0, poltype="linear", layout="vup", precision="c64"):
+              weighting="natural", robust=0.0, poltype="linear", layout="vup", precision="c64",
+              accumulation="f64"):
     v = mti.vis_ms(dev.uvw, dev.invl, dev.vis, dev.flags, dev.weight, index)
     nb, P = len(chans), dev.npol
     assert list(chans) == list(range(chans[0], chans[0] + nb))
@@ -52,7 +53,8 @@ def gpu_image(mti, ctx, kern, params, dev, torch, chans, index=None, psf=False,
     grids = torch.empty((nb, params.ny, params.nx, P, 2), dtype=gdt, device="cuda")
     norm = torch.empty((nb, P), dtype=torch.float64, device="cuda")
     st = ctx.grid(kern, params, v, grids, norm, chan0=chans[0], nchan=nb, psf=psf,
-                  weighting=weighting, robust=robust, layout=layout, precision=precision)
+                  weighting=weighting, robust=robust, layout=layout, precision=precision,
+                  accumulation=accumulation)
     g_host = grids.cpu().numpy().copy()
     if layout == "pvu":  # [nb][P][ny][nx][2] -> [nb][ny][nx][P][2]
         g_host = g_host.reshape(nb, P, params.ny, params.nx, 2).transpose(0, 2, 3, 1, 4).copy()

@@ -54,3 +54,23 @@ def test_briggs_limits(orc):
     np.testing.assert_allclose(hi, nat, rtol=1e-9)
     np.testing.assert_allclose(lo / lo.sum(), uni / uni.sum(), rtol=1e-9)
     assert not np.allclose(uni / uni.sum(), nat / nat.sum(), rtol=1e-2)
+
+
+def test_mfs_density_over_channels(orc):
+    """One density over several channels (MFS): a single channel reproduces
+    orc_weights; two channels with the same 1/lambda double every density, so
+    uniform weights halve while Briggs weights do not change (f^2 halves as
+    sum W^2 / sum w doubles)."""
+    K, uvw, flags, weight = _setup(orc)
+    fl2 = np.repeat(flags, 2, axis=1)
+    for wt in ("uniform", "briggs"):
+        one = orc.weights(K, uvw, np.array([1.0]), flags, weight, 0, wt)
+        m1 = orc.weights_mfs(K, uvw, np.array([1.0]), flags, weight, 0, 1, wt)
+        np.testing.assert_array_equal(m1[0], one)
+        m2 = orc.weights_mfs(K, uvw, np.array([1.0, 1.0]), fl2, weight, 0, 2, wt)
+        np.testing.assert_array_equal(m2[0], m2[1])
+        g = [0, 1, 2, 4]  # gridded, unflagged records
+        if wt == "uniform":
+            np.testing.assert_allclose(m2[0][g], one[g] / 2, rtol=1e-15)
+        else:
+            np.testing.assert_allclose(m2[0][g], one[g], rtol=1e-14)
