@@ -107,72 +107,97 @@ def achievable_peaks():
     return dict(copy_GBps=round(gbps.value, 1), fp64_fma_TFLOPs=round(tflops.value, 2))
 
 
-def end_to_end(torch, mti, ctx, d, p, grids, norm, image, ref_image, group=8, tmask=None):
-    """One imaging job from pinned host memory: the job's inputs (channel-major,
-    so a block of channels is one contiguous copy) go to the device on a copy
-    stream in blocks of `group` channels, double-buffered, while the compute
-    stream grids the previous block (Briggs weights and normalisers are per
-    channel, so blocks are independent); then row sort, kernel tables and the
-    finalize as in a step.  Returns the PCIe-inclusive rate and whether the
-    images equal the HBM-resident run's."""
+def end_to_end(torch, mti, ctx, d, p, grids, norm, image, ref_image, group=8, tmask=None, pinned=True,
+               reader_threads=8):
+    """One imaging job from host memory through the package's chunked loader
+    (mtimager_amd.ingest): the job's columns in the container's layout
+    (DATA/FLAG channel-major) are wrapped as a dataset -- pinned host memory
+    (copied to the device directly) or pageable (copied by the loader's reader
+    threads into its pinned staging buffers first) -- and streamed in blocks
+    of `group` channels on the loader's copy stream while the compute stream
+    grids the previous block (Briggs densities are per channel, so blocks are
+    independent); the row columns are copied, sorted and kept once.  Timed:
+    kernel tables, every block's copy and gridding, the finalize.  Returns the
+    host-inclusive rate and how far the images are from the HBM-resident run's."""
+    from mtimager_amd import ingest
     nrows, nchan, P = d["nrows"], d["nchan"], d["P"]
-    log(f"end-to-end: staging {nchan} channels in pinned host memory")
-    vis_h = torch.empty((nchan, nrows, P), dtype=torch.complex64, pin_memory=True)
-    flg_h = torch.empty((nchan, nrows, P), dtype=torch.uint8, pin_memory=True)
+    kind = "pinned" if pinned else "pageable"
+    log(f"end-to-end ({kind}): staging {nchan} channels in host memory")
+    host = (lambda t: t.pin_memory()) if pinned else (lambda t: t)
+    vis_h = torch.empty((nchan, nrows, P), dtype=torch.complex64, pin_memory=pinned)
+    flg_h = torch.empty((nchan, nrows, P), dtype=torch.uint8, pin_memory=pinned)
     for c0 in range(0, nchan, group):
         vis_h[c0:c0 + group].copy_(d["vis"][:, c0:c0 + group].permute(1, 0, 2).contiguous())
         flg_h[c0:c0 + group].copy_(d["flags"][:, c0:c0 + group].permute(1, 0, 2).contiguous())
-    small_h = {k: d[k].cpu().pin_memory() for k in ("uvw", "invl", "weight", "a1", "a2")}
-    small_d = {k: torch.empty_like(v, device="cuda") for k, v in small_h.items()}
-    vis_d = [torch.empty((group, nrows, P), dtype=torch.complex64, device="cuda") for _ in range(2)]
-    flg_d = [torch.empty((group, nrows, P), dtype=torch.uint8, device="cuda") for _ in range(2)]
-    idx = torch.empty(nrows, dtype=torch.int32, device="cuda")
-    cs, gs = torch.cuda.Stream(), torch.cuda.current_stream()
-    groups = [(c0, min(group, nchan - c0)) for c0 in range(0, nchan, group)]
-    ready, done = [None] * len(groups), [None, None]
-
-    def issue_copy(gi):
-        c0, n = groups[gi]
-        b = gi % 2
-        with torch.cuda.stream(cs):
-            if done[b] is not None:
-                cs.wait_event(done[b])
-            vis_d[b][:n].copy_(vis_h[c0:c0 + n], non_blocking=True)
-            flg_d[b][:n].copy_(flg_h[c0:c0 + n], non_blocking=True)
-            ready[gi] = torch.cuda.Event()
-            ready[gi].record(cs)
-
+    cols = dict(time=host(torch.zeros(nrows, dtype=torch.float64)), a1=host(d["a1"].cpu()),
+                a2=host(d["a2"].cpu()), uvw=host(d["uvw"].cpu()), weight=host(d["weight"].cpu()),
+                freq=host((d["invl"] * 299792458.0).cpu()))
+    ds = ingest.Dataset.wrap(cols["time"], cols["a1"], cols["a2"], cols["uvw"], cols["weight"], cols["freq"],
+                             vis_h, flg_h, d["nant"], pinned=pinned)
+    ld = ingest.Loader(ctx, ds, channel_block=group, chunk_rows=0, weighting=d["weighting"], robust=0.0,
+                       layout="pvu", precision="c64" if grids.dtype == torch.float32 else "c128",
+                       reader_threads=reader_threads)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    with torch.cuda.stream(cs):
-        for k, v in small_h.items():
-            small_d[k].copy_(v, non_blocking=True)
-        ev = torch.cuda.Event()
-        ev.record(cs)
-    issue_copy(0)
-    gs.wait_event(ev)
-    kept = ctx.sort_rows(small_d["a1"], small_d["a2"], d["nant"], idx)
     K = ctx.build_kernels(p)
-    for gi, (c0, n) in enumerate(groups):
-        if gi + 1 < len(groups):
-            issue_copy(gi + 1)  # before this block's gridding call, which waits for its end
-        gs.wait_event(ready[gi])
-        b = gi % 2
-        v = mti.vis_chan_major(small_d["uvw"], small_d["invl"][c0:c0 + n], vis_d[b][:n], flg_d[b][:n],
-                               small_d["weight"], idx[:kept])
-        ctx.grid(K, p, v, grids[c0:c0 + n], norm[c0:c0 + n], chan0=0, nchan=n, weighting=d["weighting"], robust=0.0,
-                 layout="pvu", stats=False, tile_mask=None if tmask is None else tmask[c0:c0 + n])
-        done[b] = torch.cuda.Event()
-        done[b].record(gs)
+    st = ld.grid(K, p, grids, norm, tile_mask=tmask)
     ctx.finalize(p, P, grids, norm, image, nchan=nchan, layout="pvu", tile_mask=tmask)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     K.free()
-    h2d = vis_h.numel() * 8 + flg_h.numel() + sum(v.numel() * v.element_size() for v in small_h.values())
+    ld.close()
+    ds.close()
     diff = float((image - ref_image).abs().max()) / max(float(ref_image.abs().max()), 1e-30)
-    del vis_h, flg_h, vis_d, flg_d
-    return dict(seconds=round(el, 4), h2d_bytes=int(h2d), h2d_share_GBps=round(h2d / el / 1e9, 1),
-                channel_block=group, max_rel_diff_vs_resident=diff)
+    del vis_h, flg_h, cols
+    return dict(host_memory=kind, seconds=round(el, 4), h2d_bytes=int(st["h2d_bytes"]),
+                h2d_share_GBps=round(st["h2d_bytes"] / el / 1e9, 1), channel_block=group, chunks=st["chunks"],
+                reader_seconds=round(st["reader_seconds"], 3), max_rel_diff_vs_resident=diff,
+                path="mtimager_amd.ingest.Loader")
+
+
+def max_rel_rms(image, ref, nchan):
+    """Largest relative RMS over channels and Stokes images, each relative to
+    the RMS of the channel's reference Stokes I."""
+    errs = []
+    for c in range(nchan):
+        dd = image[c].double() - ref[c].double()
+        ri = ref[c, 0].double().pow(2).mean().sqrt()
+        errs.append(float((dd.pow(2).mean(dim=(1, 2)).sqrt() / ri).max()))
+    return max(errs)
+
+
+def precision_forms(torch, mti, ctx, d, p, grids, norm, image, ref, tmask, nchan):
+    """The precision forms on the cube: kernel tables in f16 / bf16 and the
+    gridder's FP32 compensated (Kahan) accumulation, each imaged once with
+    stage timing; gridder milliseconds and the image error against the
+    complex128 FP64 path (ref)."""
+    P = d["P"]
+    out = {}
+    idx = torch.empty(d["nrows"], dtype=torch.int32, device="cuda")
+    for name, table, acc in (("f32 tables, FP64 sums", "f32", "f64"), ("f16 tables, FP64 sums", "f16", "f64"),
+                             ("bf16 tables, FP64 sums", "bf16", "f64"),
+                             ("f32 tables, FP32 Kahan sums", "f32", "f32k"),
+                             ("f16 tables, FP32 Kahan sums", "f16", "f32k"),
+                             ("bf16 tables, FP32 Kahan sums", "bf16", "f32k")):
+        kept = ctx.sort_rows(d["a1"], d["a2"], d["nant"], idx)
+        K = ctx.build_kernels(p, precision=table)
+        v = mti.vis_ms(d["uvw"], d["invl"], d["vis"], d["flags"], d["weight"], idx[:kept])
+        torch.cuda.synchronize()
+        ctx.stage_times()
+        ctx.set_profiling(True)
+        ctx.grid(K, p, v, grids, norm, chan0=0, nchan=nchan, weighting=d["weighting"], robust=0.0, layout="pvu",
+                 stats=False, tile_mask=tmask, accumulation=acc)
+        ctx.finalize(p, P, grids, norm, image, nchan=nchan, layout="pvu", tile_mask=tmask)
+        torch.cuda.synchronize()
+        ctx.set_profiling(False)
+        stg = ctx.stage_times()
+        K.free()
+        err = max_rel_rms(image, ref, nchan)
+        out[name] = dict(grid_ms=round(stg["grid"][0], 3), job_ms=round(sum(v[0] for v in stg.values()), 3),
+                         image_max_rel_rms_vs_fp64=float(f"{err:.3e}"))
+        log(f"precision form {name}: {out[name]}")
+    return dict(forms=out, vs="complex128 grids and transform planes, f32 tables, FP64 sums",
+                over="every channel and Stokes image, relative to the channel's Stokes I RMS")
 
 
 def cpu_model():
@@ -664,6 +689,8 @@ def main():
                     help="element type of the grids the gridder writes")
     ap.add_argument("--no-form-check", action="store_true",
                     help="cube runs: skip imaging the job again on the complex128 path")
+    ap.add_argument("--no-precision-forms", action="store_true",
+                    help="cube runs: skip timing the reduced-precision table / FP32 accumulation forms")
     ap.add_argument("--cpu-rows", type=int, default=60000)
     ap.add_argument("--cpu-threads", type=int, default=16, help="threads of the CPU baseline (one channel each)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -790,6 +817,7 @@ def main():
                     frac_of_achievable=(round(achieved / peaks["fp64_fma_TFLOPs"], 4) if peaks else None))
 
     form_err = None
+    forms = None
     if (args.grid_precision != "c128" or args.transform_storage != "c128") and not args.no_form_check:
         # the last step's job imaged again on the strict FP64 path (complex128
         # grids and transform planes, the form the parity tests hold to the
@@ -811,15 +839,12 @@ def main():
         K.free()
         ctx.set_transform_storage(args.transform_storage)
         del g128, n128, m128
-        errs = []
-        for c in range(nchan):
-            dd = image[c].double() - ref[c].double()
-            ri = ref[c, 0].double().pow(2).mean().sqrt()
-            errs.append(float((dd.pow(2).mean(dim=(1, 2)).sqrt() / ri).max()))
-        form_err = max(errs)
-        del ref, dd
+        form_err = max_rel_rms(image, ref, nchan)
         log(f"precision form (grids {args.grid_precision}, planes {args.transform_storage}): max relative RMS "
             f"vs the complex128 path {form_err:.3e}")
+        if args.config == "c2" and not args.no_precision_forms:
+            forms = precision_forms(torch, mti, ctx, d, p, grids, norm, image, ref, tmask, nchan)
+        del ref
 
     e2e = None
     if not args.no_e2e:
@@ -827,6 +852,10 @@ def main():
         e2e = end_to_end(torch, mti, ctx, d, p, grids, norm, image, ref_image, tmask=tmask)
         e2e["mvis_s"] = round(gridded / e2e["seconds"] / 1e6, 3)
         log(f"end-to-end: {e2e}")
+        pg = end_to_end(torch, mti, ctx, d, p, grids, norm, image, ref_image, tmask=tmask, pinned=False)
+        pg["mvis_s"] = round(gridded / pg["seconds"] / 1e6, 3)
+        log(f"end-to-end from pageable memory: {pg}")
+        e2e["pageable"] = pg
         del ref_image
 
     cpu = None
@@ -926,6 +955,8 @@ def main():
         "end_to_end": e2e,
         "cpu_baseline": cpu,
     }
+    if forms is not None:
+        out["c3_forms"] = forms
     out.update(nested)
     print(json.dumps(out), flush=True)
     ctx.close()

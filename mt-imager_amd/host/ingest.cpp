@@ -362,8 +362,9 @@ const char *data_piece(const mti_dataset *d, int col, int c, int64_t r0)
     return (const char *)d->col[col] + ((size_t)c * (size_t)d->info.nrows + (size_t)r0) * eb;
 }
 
-// Fills pinned slot `dst` with chunk c, the pieces shared by `nthreads` threads.
-void fill_slot(const mti_loader *l, const Chunk &c, char *dst, int nthreads)
+// Fills pinned slot `dst` with chunk c (its row columns too when `rows`),
+// the pieces shared by `nthreads` threads.
+void fill_slot(const mti_loader *l, const Chunk &c, char *dst, int nthreads, bool rows)
 {
     const mti_dataset *d = l->ds;
     const int P = d->info.npol;
@@ -374,7 +375,8 @@ void fill_slot(const mti_loader *l, const Chunk &c, char *dst, int nthreads)
     };
     std::vector<Job> jobs;
     const Pieces p = row_pieces(l, c);
-    for (int k = 0; k < 5; k++) jobs.push_back({dst + p.slot_off[k], p.row[k], p.row_bytes[k]});
+    if (rows)
+        for (int k = 0; k < 5; k++) jobs.push_back({dst + p.slot_off[k], p.row[k], p.row_bytes[k]});
     const size_t vb = (size_t)c.nr * 8 * P, fb = (size_t)c.nr * P;
     for (int k = 0; k < c.nc; k++) {
         jobs.push_back({dst + l->lay.vis + k * vb, data_piece(d, MTI_COL_DATA, c.c0 + k, c.r0), vb});
@@ -516,6 +518,11 @@ extern "C" mti_status mti_loader_grid(mti_loader *l, const mti_kernels *k, const
     double reader_s = 0.0;
     std::thread reader;
     const bool staged = !l->ds->pinned;
+    // one row chunk per channel block: the row columns, their baseline-major
+    // order and LSRK factors are the same for every block -- copied, sorted
+    // and computed once, in device slot 0
+    const bool rows_once = l->rows >= in.nrows;
+    auto rows_of = [&](int j) { return !rows_once || j == 0; };
     if (staged) {
         reader = std::thread([&]() {
             (void)hipSetDevice(l->device);
@@ -532,7 +539,7 @@ extern "C" mti_status mti_loader_grid(mti_loader *l, const mti_kernels *k, const
                     return;
                 }
                 const auto t0 = std::chrono::steady_clock::now();
-                fill_slot(l, chunks[j], l->host[j % 2], o.reader_threads);
+                fill_slot(l, chunks[j], l->host[j % 2], o.reader_threads, rows_of(j));
                 const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
                 std::lock_guard<std::mutex> lk(mu);
                 reader_s += dt;
@@ -564,7 +571,7 @@ extern "C" mti_status mti_loader_grid(mti_loader *l, const mti_kernels *k, const
         if (j >= 2) LHIP(l, hipStreamWaitEvent(l->copy, computed[j % 2], 0));
         const Pieces pc = row_pieces(l, c);
         const char *src = staged ? l->host[j % 2] : nullptr;
-        for (int q = 0; q < 5; q++) {
+        for (int q = 0; q < 5 && rows_of(j); q++) {
             LHIP(l, hipMemcpyAsync(dst + pc.slot_off[q], staged ? src + pc.slot_off[q] : pc.row[q], pc.row_bytes[q],
                                    hipMemcpyHostToDevice, l->copy));
             h2d += (int64_t)pc.row_bytes[q];
@@ -592,31 +599,36 @@ extern "C" mti_status mti_loader_grid(mti_loader *l, const mti_kernels *k, const
     };
 
     mti_loader_stats acc{};
+    int64_t kept_rows = 0;
     mti_status r = nck > 0 ? issue_copy(0) : MTI_OK;
     const size_t grid_chan_bytes = (size_t)p->nx * p->ny * P * (o.grid_precision == MTI_GRID_C64 ? 8 : 16);
     const size_t mask_chan = (size_t)((p->nx + 31) / 32) * ((p->ny + 31) / 32);
     for (int j = 0; j < nck && r == MTI_OK; j++) {
         const Chunk &c = chunks[j];
         if (j + 1 < nck && (r = issue_copy(j + 1)) != MTI_OK) break;
-        char *d = l->dev[j % 2];
+        char *d = l->dev[j % 2];                   // the chunk's visibilities and flags
+        char *dr = rows_once ? l->dev[0] : d;      // its row columns, order and factors
         if (hipStreamWaitEvent(st, copied[j], 0) != hipSuccess) {
             r = lfail(l, MTI_ERR_HIP, "hipStreamWaitEvent failed");
             break;
         }
-        int64_t kept = 0;
-        r = mti_sort_rows(l->ctx, (const int32_t *)(d + l->lay.ant1), (const int32_t *)(d + l->lay.ant2), c.nr,
-                          in.nant, (int32_t *)(d + l->lay.idx), &kept, st);
-        if (r != MTI_OK) {
-            l->err = mti_last_error(l->ctx);
-            break;
+        if (rows_of(j)) {
+            r = mti_sort_rows(l->ctx, (const int32_t *)(dr + l->lay.ant1), (const int32_t *)(dr + l->lay.ant2), c.nr,
+                              in.nant, (int32_t *)(dr + l->lay.idx), &kept_rows, st);
+            if (r != MTI_OK) {
+                l->err = mti_last_error(l->ctx);
+                break;
+            }
         }
-        if (o.lsrk && c.nr > 0) {
+        const int64_t kept = kept_rows;
+        if (o.lsrk && c.nr > 0 && rows_of(j)) {
             mti_lsrk_frame fr{};
             for (int q = 0; q < 3; q++) fr.obs_itrf[q] = in.obs_itrf[q];
             fr.ra = in.ra;
             fr.dec = in.dec;
             fr.tt_minus_utc = o.tt_minus_utc;
-            r = mti_lsrk_factors(l->ctx, &fr, (const double *)(d + l->lay.time), c.nr, (double *)(d + l->lay.dop), st);
+            r = mti_lsrk_factors(l->ctx, &fr, (const double *)(dr + l->lay.time), c.nr, (double *)(dr + l->lay.dop),
+                                 st);
             if (r != MTI_OK) {
                 l->err = mti_last_error(l->ctx);
                 break;
@@ -626,7 +638,7 @@ extern "C" mti_status mti_loader_grid(mti_loader *l, const mti_kernels *k, const
         v.nrows = c.nr;
         v.nchan = c.nc;
         v.npol = P;
-        v.uvw = d + l->lay.uvw;
+        v.uvw = dr + l->lay.uvw;
         v.uvw_row_stride = 3;
         v.uvw_comp_stride = 1;
         v.inv_lambda = l->invl + c.c0;
@@ -636,11 +648,11 @@ extern "C" mti_status mti_loader_grid(mti_loader *l, const mti_kernels *k, const
         v.flags = d + l->lay.flags;
         v.flags_row_stride = P;
         v.flags_chan_stride = c.nr * P;
-        v.weight = (const float *)(d + l->lay.weight);
+        v.weight = (const float *)(dr + l->lay.weight);
         v.weight_row_stride = P;
-        v.row_index = (const int32_t *)(d + l->lay.idx);
+        v.row_index = (const int32_t *)(dr + l->lay.idx);
         v.nindex = kept;
-        v.row_doppler = o.lsrk ? (const double *)(d + l->lay.dop) : nullptr;
+        v.row_doppler = o.lsrk ? (const double *)(dr + l->lay.dop) : nullptr;
         mti_grid_opts go{};
         go.chan0 = 0;
         go.nchan = c.nc;

@@ -77,17 +77,25 @@ def test_row_chunks_equal_resident(tmp_path, mti, ctx, ob, params, precision):
                               chunk_rows=ob.nrows // 4 + 1, reader_threads=4)
     assert st["chunks"] == 3 * 4
     np.testing.assert_allclose(norm, nref, rtol=1e-12)
-    assert _max_rel(img, ref) < (1e-6 if precision == "c64" else 1e-12)
+    if precision == "c128":
+        assert _max_rel(img, ref) < 1e-12
+    else:
+        # complex64 cells are rounded once per row chunk (read, add, write)
+        # instead of once, and the taper correction amplifies cell roundings
+        # up to 6e4-fold in the field corners
+        rr = H.rel_rms(img, ref)
+        print(f"[ingest] c64 streamed vs resident: rel RMS {rr:.2e}, max {_max_rel(img, ref):.2e}")
+        assert rr < 1e-5 and _max_rel(img, ref) < 1e-4
     ds.close()
 
 
 def test_briggs_channel_blocks_equal_resident(tmp_path, mti, ctx, ob, params):
     K = ctx.build_kernels(params)
-    ref, _ = _resident(mti, ctx, K, params, ob, "briggs", "c64")
+    ref, _ = _resident(mti, ctx, K, params, ob, "briggs", "c128")
     ds = _write(tmp_path, ob)
-    img, _, st = _streamed(mti, ctx, K, params, ds, "c64", channel_block=2, chunk_rows=0, weighting="briggs")
+    img, _, st = _streamed(mti, ctx, K, params, ds, "c128", channel_block=2, chunk_rows=0, weighting="briggs")
     assert st["chunks"] == 4
-    assert _max_rel(img, ref) < 1e-6
+    assert _max_rel(img, ref) < 1e-12
     from mtimager_amd import MTIError, ingest
     with pytest.raises(MTIError, match="every row"):
         ingest.Loader(ctx, ds, chunk_rows=100, weighting="briggs")
@@ -112,12 +120,12 @@ def test_lsrk_through_the_loader(tmp_path, mti, ctx, ob, params):
 def test_pinned_wrap_equals_resident(mti, ctx, ob, params):
     from mtimager_amd import ingest
     K = ctx.build_kernels(params)
-    ref, _ = _resident(mti, ctx, K, params, ob, "natural", "c64")
+    ref, _ = _resident(mti, ctx, K, params, ob, "natural", "c128")
     pin = lambda a: torch.from_numpy(np.ascontiguousarray(a)).pin_memory()
     cols = [pin(ob.time), pin(ob.ant1), pin(ob.ant2), pin(ob.uvw), pin(ob.weight), pin(ob.freq),
             pin(ob.vis.transpose(1, 0, 2)), pin(ob.flags.transpose(1, 0, 2))]
     ds = ingest.Dataset.wrap(*cols, ob.nant, pinned=True)
-    img, _, st = _streamed(mti, ctx, K, params, ds, "c64", channel_block=7, chunk_rows=0)
-    assert st["chunks"] == 1 and st["reader_seconds"] == 0.0
-    assert _max_rel(img, ref) < 1e-6
+    img, _, st = _streamed(mti, ctx, K, params, ds, "c128", channel_block=3, chunk_rows=0)
+    assert st["chunks"] == 3 and st["reader_seconds"] == 0.0
+    assert _max_rel(img, ref) < 1e-12
     ds.close()
