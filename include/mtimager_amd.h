@@ -247,6 +247,20 @@ mti_status mti_sort_rows(mti_ctx *ctx, const int32_t *ant1, const int32_t *ant2,
                          int64_t nrows, int32_t nant, int32_t *index_out,
                          int64_t *kept_out, void *stream);
 
+/* Allocation-free mti_sort_rows.  No counterpart in the reference: for
+ * callers that cannot allocate device memory during the call (code under HIP
+ * graph capture, tight serving loops, hosts that forbid allocation).  Its
+ * scratch is a 52 MiB device array of the library (one per GPU, held until
+ * the process ends, looked up by mti_create), so it calls no hipMalloc,
+ * hipFree or allocating routine; nrows up to MTI_SORT_STATIC_MAX_ROWS
+ * (MTI_ERR_INVALID above).  Every call on one GPU shares that array: calls
+ * that overlap on one GPU (several streams, contexts or threads) race.
+ * Results equal mti_sort_rows's. */
+#define MTI_SORT_STATIC_MAX_ROWS 4194304
+mti_status mti_sort_rows_static(mti_ctx *ctx, const int32_t *ant1, const int32_t *ant2,
+                                int64_t nrows, int32_t nant, int32_t *index_out,
+                                int64_t *kept_out, void *stream);
+
 /* ---- binning (PreProcessUVW) ----------------------------------------------
  * Replaces WProjPreProcessUVW (op/WProjPreProcessUVW.cu:111-324) for one
  * channel: per record (row order of vis->row_index) writes

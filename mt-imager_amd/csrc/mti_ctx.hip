@@ -15,6 +15,7 @@ extern "C" mti_status mti_create(int device, mti_ctx **out)
     if (hipSetDevice(device) != hipSuccess) return MTI_ERR_HIP;
     mti_ctx *c = new mti_ctx();
     c->device = device;
+    mti::static_scratch_setup(device);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
     if (c->num_cus <= 0) c->num_cus = 256;

@@ -126,6 +126,10 @@ struct mti_kernels {
 namespace mti {
 
 
+// Device-pointer lookup of the allocation-free sort's static scratch, once
+// per device (mti_sort.hip).
+void static_scratch_setup(int device);
+
 inline mti_status set_error(mti_ctx *ctx, mti_status code, const std::string &m)
 {
     if (ctx) ctx->last_error = m;

@@ -10,7 +10,39 @@
 #include "mti_internal.hpp"
 #include "mti_sort.hpp"
 
+#include <mutex>
+
+// Device scratch of the allocation-free row sort (mti_sort_rows_static):
+// 52 MiB per GPU, enough for the plan of MTI_SORT_STATIC_MAX_ROWS rows (keys
+// and values 12 B per row, radix digit counts and scan words).  In host code
+// `scratch` names a host-side stand-in; kernels get the device address that
+// hipGetSymbolAddress returns for the current device (g_scratch_ptr).
+constexpr size_t kStaticSortBytes = size_t(52) << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticSortBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+bool g_scratch_done[kMaxDevices] = {};
+}  // namespace
+
 namespace mti {
+
+// Looks the static scratch up once per device, with that device current
+// (called by mti_create after hipSetDevice).  A failure leaves the device's
+// pointer null: mti_sort_rows_static then fails with a message.
+void static_scratch_setup(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_done[device]) return;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    g_scratch_ptr[device] = e == hipSuccess ? ptr : nullptr;
+    if (e != hipSuccess) (void)hipGetLastError();
+    g_scratch_done[device] = true;
+}
 
 constexpr int RX_THREADS = 256;
 constexpr int RX_ROUNDS = 16;
@@ -228,4 +260,27 @@ extern "C" mti_status mti_sort_rows(mti_ctx *ctx, const int32_t *ant1, const int
     MTI_HIP(ctx, hipStreamSynchronize(st));
     *kept_out = (int64_t)h;
     return MTI_OK;
+}
+
+extern "C" mti_status mti_sort_rows_static(mti_ctx *ctx, const int32_t *ant1, const int32_t *ant2, int64_t nrows,
+                                           int32_t nant, int32_t *index_out, int64_t *kept_out, void *stream)
+{
+    if (!ctx) return MTI_ERR_INVALID;
+    MTI_CHECK(ctx, nrows >= 0 && nrows <= MTI_SORT_STATIC_MAX_ROWS,
+              "mti_sort_rows_static: nrows outside [0, MTI_SORT_STATIC_MAX_ROWS = 4194304]");
+    MTI_CHECK(ctx, ctx->call_depth == 0, "mti_sort_rows_static inside another call");
+    void *ptr = (ctx->device >= 0 && ctx->device < kMaxDevices) ? g_scratch_ptr[ctx->device] : nullptr;
+    MTI_CHECK(ctx, ptr, "mti_sort_rows_static: the static scratch of this device is unavailable");
+    ScratchPlan pl;
+    plan_sort_rows(nrows, pl);
+    MTI_CHECK(ctx, pl.total() <= kStaticSortBytes, "mti_sort_rows_static: plan above the static scratch");
+    // run mti_sort_rows inside the static array as a caller workspace
+    char *const saved_ptr = ctx->ws_ptr;
+    const size_t saved_bytes = ctx->ws_bytes;
+    ctx->ws_ptr = (char *)ptr;
+    ctx->ws_bytes = kStaticSortBytes;
+    const mti_status r = mti_sort_rows(ctx, ant1, ant2, nrows, nant, index_out, kept_out, stream);
+    ctx->ws_ptr = saved_ptr;
+    ctx->ws_bytes = saved_bytes;
+    return r;
 }

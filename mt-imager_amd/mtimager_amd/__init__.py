@@ -129,7 +129,7 @@ EXPORTS = ["mti_version", "mti_create", "mti_destroy", "mti_last_error",
            "mti_set_transform_storage", "mti_get_transform_storage",
            "mti_preprocess_reference", "mti_uv_reorder", "mti_centre_shift", "mti_stokes", "mti_taper_correct",
            "mti_taper_kernel", "mti_sinc_correct", "mti_normalizer_f32",
-           "mti_lsrk_factors", "mti_lsrk_inv_lambda"]
+           "mti_lsrk_factors", "mti_lsrk_inv_lambda", "mti_sort_rows_static"]
 
 _lib = None
 
@@ -171,6 +171,9 @@ def lib():
     L.mti_sort_rows.restype = ctypes.c_int
     L.mti_sort_rows.argtypes = [vp, vp, vp, ctypes.c_int64, ctypes.c_int32, vp,
                                 P(ctypes.c_int64), vp]
+    L.mti_sort_rows_static.restype = ctypes.c_int
+    L.mti_sort_rows_static.argtypes = [vp, vp, vp, ctypes.c_int64, ctypes.c_int32, vp,
+                                       P(ctypes.c_int64), vp]
     L.mti_preprocess.restype = ctypes.c_int
     L.mti_preprocess.argtypes = [vp, vp, P(ImageParams), P(Vis), ctypes.c_int32, vp, vp]
     L.mti_grid.restype = ctypes.c_int
