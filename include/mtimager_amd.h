@@ -49,9 +49,9 @@ typedef enum { MTI_GRID_VUP = 0, MTI_GRID_PVU = 1 } mti_grid_layout;
  * last-bit rounding) depends on the order units finish. */
 typedef enum { MTI_GRID_C64 = 0, MTI_GRID_C128 = 1 } mti_grid_precision;
 /* Gridder accumulation (a precision form): FP64 registers (default), or FP32
- * registers per batch of at most 256 records with the batch sums added by
- * Kahan's compensated summation in FP32 (binning, oversample indices and the
- * w-plane choice stay FP64 either way). */
+ * registers per four records with those partial sums added by Kahan's
+ * compensated summation in FP32 (binning, oversample indices and the w-plane
+ * choice stay FP64 either way). */
 typedef enum { MTI_ACC_F64 = 0, MTI_ACC_F32_KAHAN = 1 } mti_accumulation;
 
 /* Imaging parameters; the fields carry the mt-imager property keys
@@ -434,6 +434,32 @@ mti_status mti_sinc_correct(const float *in, float *out, int64_t planes, int32_t
 /* The f32 NormalizerValue of the reference's WImager (comp/WImager.cpp:196-197)
  * from the library's FP64 normalisers: n values rounded once. */
 mti_status mti_normalizer_f32(const double *norm, float *out, int64_t n, void *stream);
+
+/* ---- prediction (degridding) ---------------------------------------------------
+ * No counterpart in the reference (it only images): the adjoint of its imaging
+ * path, for the prediction step of a major cycle.
+ * mti_model_grids: Stokes model images f32 [nchan][npol][ny][nx] in the
+ * finalize's layout (m axis flipped; I,Q,U,V / I,Q / I,V / I as mti_finalize
+ * writes them) -> complex64 model grids [nchan][npol][ny][nx] (origin at
+ * (ny/2, nx/2), as the gridder's): correlation images from the Stokes
+ * formulas' inverse (linear XX=I+Q, XY=U+iV, YX=U-iV, YY=I-Q; circular RR=I+V,
+ * RL=Q-iU, LR=Q+iU, LL=I-V), times sinc/taper at each pixel's sky offset (the
+ * inverse of the kernel's attenuation), unnormalised forward FFT in FP64.
+ * Uses context scratch (at most 512 MiB) even with a caller workspace. */
+mti_status mti_model_grids(mti_ctx *ctx, const mti_image_params *p, int32_t npol, int32_t poltype,
+                           int32_t nchan, const float *model, void *grids, void *stream);
+/* mti_degrid: model visibilities of channels chan0 .. chan0 + nchan - 1 of
+ * v's rows (v->row_index order; flags ignored) written as complex64 into
+ * vis_out with v's visibility strides (vis_row_stride, vis_chan_stride):
+ * V_p = sum conj(K) G_p over the record's support box / the sub-kernel sum,
+ * the record binned in FP64 exactly as mti_grid bins it; w < 0 records get
+ * the Hermitian image of the mirrored prediction (XY <-> YX for P = 4).
+ * Records whose box is off the grid get 0; rows outside row_index are not
+ * written.  *predicted (may be NULL; then the call does not block) counts the
+ * predicted records. */
+mti_status mti_degrid(mti_ctx *ctx, const mti_kernels *k, const mti_image_params *p, const mti_vis *v,
+                      int32_t chan0, int32_t nchan, const void *grids, float *vis_out,
+                      int64_t *predicted, void *stream);
 
 /* ---- captured batched imaging (HIP graphs) -----------------------------------
  * No counterpart in the reference (it plans each cuFFT call afresh,

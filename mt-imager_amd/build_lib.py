@@ -21,7 +21,7 @@ COMMON = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-I", os.path.join(HERE, "..", "include")]
 # Bit-exact FP64 binning and kernel generation: no FMA contraction.
-EXACT = {"mti_bin.hip", "mti_wkernel.hip"}
+EXACT = {"mti_bin.hip", "mti_wkernel.hip", "mti_degrid.hip"}
 
 
 def _sources():

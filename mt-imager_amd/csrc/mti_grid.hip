@@ -502,11 +502,11 @@ __device__ __forceinline__ void flush_cell(R *g, int64_t ps, const double (&acc)
 // occupancy.  The f16/bf16 forms convert samples and need more registers.
 //
 // ACC selects the accumulation: MTI_ACC_F64 (FP64 registers), or
-// MTI_ACC_F32_KAHAN -- each batch's taps summed in FP32 registers (a batch
-// holds at most TG_BATCH records), and the batch sums added with Kahan's
-// compensated summation into an FP32 (sum, compensation) pair per cell
-// component; the cell's value sum - compensation is then added to the grid.
-// Visibilities are staged as FP32 and samples stay FP32 (no conversions).
+// MTI_ACC_F32_KAHAN -- every four visits of a wave summed in FP32 registers,
+// and those partial sums added with Kahan's compensated summation into an
+// FP32 (sum, compensation) pair per cell component; the cell's value
+// sum - compensation is then added to the grid.  Visibilities are staged as
+// FP32 and samples stay FP32 (no conversions).
 template <int P, int PREC, int ACC>
 __global__ __launch_bounds__(TG_THREADS, PREC == MTI_TABLE_F32 || ACC == MTI_ACC_F32_KAHAN ? 8 : 1) void
 k_grid_tiles(GridArgs a)
@@ -628,16 +628,16 @@ k_grid_tiles(GridArgs a)
                 visit(vb);
                 visit(vc);
                 visit(vd);
-            }
-            if (F32) {
-                // Kahan: the batch sum into (ksum, kcmp)
+                if (F32) {
+                    // Kahan: the four visits' FP32 sum into (ksum, kcmp)
 #pragma unroll
-                for (int q = 0; q < 2 * P; q++) {
-                    const float y = accf[q] - kcmp[q];
-                    const float t = ksum[q] + y;
-                    kcmp[q] = (t - ksum[q]) - y;
-                    ksum[q] = t;
-                    accf[q] = 0.0f;
+                    for (int q = 0; q < 2 * P; q++) {
+                        const float y = accf[q] - kcmp[q];
+                        const float t = ksum[q] + y;
+                        kcmp[q] = (t - ksum[q]) - y;
+                        ksum[q] = t;
+                        accf[q] = 0.0f;
+                    }
                 }
             }
         }

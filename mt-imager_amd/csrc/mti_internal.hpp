@@ -118,6 +118,7 @@ struct mti_kernels {
     mti_kernel_info info{};
     std::vector<int32_t> support;    // host copy
     std::vector<int64_t> base;       // host copy
+    void *d_mem = nullptr;           // one allocation holding the three arrays below
     int2 *d_convinfo = nullptr;      // {support, base} per plane (base < 2^31)
     void *d_table = nullptr;         // complex samples, precision-dependent
     float *d_sums = nullptr;         // [nplanes][T*T]

@@ -337,6 +337,21 @@ size_t elem_bytes(int prec) { return prec == MTI_TABLE_F32 ? 8 : 4; }
 
 using namespace mti;
 
+// The build's outputs in one device allocation: the sample table (total
+// samples of eb bytes), the {support, base} pairs and the sub-kernel sums.
+static mti_status alloc_tables(mti_ctx *ctx, mti_kernels *K, size_t eb)
+{
+    const int W = K->info.nplanes, T2 = K->info.T * K->info.T;
+    auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+    const size_t tb = up(std::max<size_t>(1, (size_t)K->info.total) * eb), cb = up((size_t)W * sizeof(int2));
+    const size_t sb = up((size_t)W * T2 * sizeof(float));
+    MTI_HIP(ctx, hipMalloc(&K->d_mem, tb + cb + sb));
+    K->d_table = K->d_mem;
+    K->d_convinfo = (int2 *)((char *)K->d_mem + tb);
+    K->d_sums = (float *)((char *)K->d_mem + tb + cb);
+    return MTI_OK;
+}
+
 static mti_status build_impl(mti_ctx *ctx, const mti_image_params *p, int precision,
                              mti_kernels *K, hipStream_t st)
 {
@@ -435,22 +450,20 @@ static mti_status build_impl(mti_ctx *ctx, const mti_image_params *p, int precis
         // from 2^32 - 16 up must lie beyond the table
         if ((uint64_t)tot * (uint64_t)eb > (uint64_t(1) << 32) - 16)
             return set_error(ctx, MTI_ERR_UNSUPPORTED, "convolution table above 4 GiB");
-        MTI_HIP(ctx, hipMalloc(&K->d_table, std::max<size_t>(1, (size_t)tot) * eb));
+        if ((r = alloc_tables(ctx, K, eb)) != MTI_OK) return r;
 
         // windowed DFT as two batched complex products (all planes of a batch)
         const int Wmax = (std::max(smax, 1) - 1) * s + T;
-        int64_t *d_base = nullptr;
-        MTI_HIP(ctx, hipMalloc(&d_base, (size_t)W * sizeof(int64_t)));
-        MTI_HIP(ctx, hipMemcpyAsync(d_base, K->base.data(), (size_t)W * sizeof(int64_t), hipMemcpyHostToDevice, st));
         const size_t a_plane = (size_t)St * Wmax * sizeof(double2), t_plane = (size_t)Wmax * Wmax * sizeof(double2);
         void *aw = nullptr;
-        r = scratch_get(ctx, SLOT_ITEMS, (size_t)nb * (a_plane + t_plane) + 256, &aw);
-        if (r != MTI_OK) {
-            (void)hipFree(d_base);
-            return r;
-        }
+        r = scratch_get(ctx, SLOT_ITEMS, (size_t)nb * (a_plane + t_plane) + (size_t)W * sizeof(int64_t) + 256, &aw);
+        if (r != MTI_OK) return r;
         double2 *d_A = (double2 *)aw;
         double2 *d_T = (double2 *)((char *)aw + (size_t)nb * a_plane);
+        // plane bases, staged through the host vector (the synchronisation at
+        // the end of the build keeps it alive until the copy is done)
+        int64_t *d_base = (int64_t *)((char *)aw + (size_t)nb * (a_plane + t_plane));
+        MTI_HIP(ctx, hipMemcpyAsync(d_base, K->base.data(), (size_t)W * sizeof(int64_t), hipMemcpyHostToDevice, st));
         for (int k0 = 0; k0 < W; k0 += nb) {
             int n = std::min(nb, W - k0);
             dim3 sg((unsigned)ceil_div(St, 16), (unsigned)ceil_div(St, 16), (unsigned)n);
@@ -471,18 +484,18 @@ static mti_status build_impl(mti_ctx *ctx, const mti_image_params *p, int precis
                                    T, inv, K->d_table);
             MTI_HIP(ctx, hipGetLastError());
         }
-        MTI_HIP(ctx, hipStreamSynchronize(st));
-        MTI_HIP(ctx, hipFree(d_base));
     }
 
     const int W = K->info.nplanes;
+    if (!K->d_mem) {
+        mti_status r = alloc_tables(ctx, K, eb);
+        if (r != MTI_OK) return r;
+    }
     std::vector<int2> ci(W);
     for (int k = 0; k < W; k++) ci[k] = make_int2(K->support[k], (int)K->base[k]);
-    MTI_HIP(ctx, hipMalloc(&K->d_convinfo, W * sizeof(int2)));
     MTI_HIP(ctx, hipMemcpyAsync(K->d_convinfo, ci.data(), W * sizeof(int2), hipMemcpyHostToDevice, st));
     if (p->mode == MTI_MODE_NORMAL) {
         const int S = p->taper_support;
-        MTI_HIP(ctx, hipMalloc(&K->d_table, (size_t)K->info.total * eb));
         dim3 gr((unsigned)ceil_div(K->info.total, 256)), bl(256);
         if (precision == MTI_TABLE_F32)
             hipLaunchKernelGGL(k_taper_table<MTI_TABLE_F32>, gr, bl, 0, st, S, s, T, K->d_table);
@@ -493,7 +506,6 @@ static mti_status build_impl(mti_ctx *ctx, const mti_image_params *p, int precis
         MTI_HIP(ctx, hipGetLastError());
     }
     const int T2 = T * T;
-    MTI_HIP(ctx, hipMalloc(&K->d_sums, (size_t)W * T2 * sizeof(float)));
     dim3 gr((unsigned)ceil_div((int64_t)W * T2, 256)), bl(256);
     if (precision == MTI_TABLE_F32)
         hipLaunchKernelGGL(k_sums<MTI_TABLE_F32>, gr, bl, 0, st, K->d_table, K->d_convinfo, W, T2, K->d_sums);
@@ -557,8 +569,6 @@ extern "C" void mti_kernels_free(mti_kernels *k)
 {
     if (!k) return;
     (void)hipSetDevice(k->device);
-    if (k->d_convinfo) (void)hipFree(k->d_convinfo);
-    if (k->d_table) (void)hipFree(k->d_table);
-    if (k->d_sums) (void)hipFree(k->d_sums);
+    if (k->d_mem) (void)hipFree(k->d_mem);
     delete k;
 }

@@ -129,7 +129,8 @@ EXPORTS = ["mti_version", "mti_create", "mti_destroy", "mti_last_error",
            "mti_set_transform_storage", "mti_get_transform_storage",
            "mti_preprocess_reference", "mti_uv_reorder", "mti_centre_shift", "mti_stokes", "mti_taper_correct",
            "mti_taper_kernel", "mti_sinc_correct", "mti_normalizer_f32",
-           "mti_lsrk_factors", "mti_lsrk_inv_lambda", "mti_sort_rows_static"]
+           "mti_lsrk_factors", "mti_lsrk_inv_lambda", "mti_sort_rows_static", "mti_model_grids",
+           "mti_degrid"]
 
 _lib = None
 
@@ -218,6 +219,8 @@ def lib():
             ("mti_sinc_correct", [vp, vp, i64, i32, i32, i32, vp]),
             ("mti_normalizer_f32", [vp, vp, i64, vp]),
             ("mti_lsrk_factors", [vp, P(LsrkFrame), vp, i64, vp, vp]),
+            ("mti_model_grids", [vp, P(ImageParams), i32, i32, i32, vp, vp, vp]),
+            ("mti_degrid", [vp, vp, P(ImageParams), P(Vis), i32, i32, vp, vp, P(i64), vp]),
             ("mti_lsrk_inv_lambda", [vp, P(LsrkFrame), vp, i64, vp, i32, vp, vp])]:
         f = getattr(L, name)
         f.restype = ctypes.c_int
@@ -433,6 +436,25 @@ class Context:
         rc = lib().mti_lsrk_inv_lambda(self.h, ctypes.byref(frame), time_mjd_s.data_ptr(), time_mjd_s.numel(),
                                        freq_hz.data_ptr(), freq_hz.numel(), out.data_ptr(), _stream(stream))
         self.check(rc, "mti_lsrk_inv_lambda")
+
+    # ---- prediction (degridding) -----------------------------------------------
+    def model_grids(self, params, npol, model, grids, nchan=1, poltype="linear", stream=None):
+        """Stokes model images f32 [nchan][npol][ny][nx] (the finalize's layout)
+        -> complex64 model grids [nchan][npol][ny][nx] (float32 [..][2] or
+        complex64 tensor)."""
+        rc = lib().mti_model_grids(self.h, ctypes.byref(params), int(npol), POLTYPE[poltype], int(nchan),
+                                   model.data_ptr(), grids.data_ptr(), _stream(stream))
+        self.check(rc, "mti_model_grids")
+
+    def degrid(self, kern, params, vis, grids, vis_out, chan0=0, nchan=1, stream=None):
+        """Model visibilities of vis's rows (row_index order) for channels
+        chan0 .. chan0 + nchan - 1 into vis_out (complex64, vis's strides);
+        returns the number of predicted records."""
+        n = ctypes.c_int64(0)
+        rc = lib().mti_degrid(self.h, kern.h, ctypes.byref(params), ctypes.byref(vis), int(chan0), int(nchan),
+                              grids.data_ptr(), vis_out.data_ptr(), ctypes.byref(n), _stream(stream))
+        self.check(rc, "mti_degrid")
+        return int(n.value)
 
     # ---- binning ----------------------------------------------------------
     def preprocess(self, kern, params, vis, chan, bins_out, stream=None):

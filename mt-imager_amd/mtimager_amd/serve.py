@@ -299,11 +299,15 @@ class Server:
         """A request's grids, normaliser and image.  They come from the
         allocator's pool of the device's default stream and are recorded on
         the worker's stream, so the workers share one pool of freed blocks
-        (separate per-stream pools would each keep their peak)."""
+        (separate per-stream pools would each keep their peak).  A block of
+        that pool may have been freed by default-stream work that is still
+        queued (a temporary of another thread's result hand-off, say): the
+        worker's stream waits for the default stream before it writes them."""
         torch = self.torch
         p = r.params
         from . import tile_mask_shape
-        with torch.cuda.stream(torch.cuda.default_stream(self.device)):
+        default = torch.cuda.default_stream(self.device)
+        with torch.cuda.stream(default):
             gdt = torch.float32 if self.grid_precision == "c64" else torch.float64
             out = (torch.empty((r.nchan, p.ny, p.nx, r.npol, 2), dtype=gdt, device="cuda"),
                    torch.empty((r.nchan, r.npol), dtype=torch.float64, device="cuda"),
@@ -312,6 +316,7 @@ class Server:
                    torch.empty(tile_mask_shape(p, r.nchan), dtype=torch.uint8, device="cuda"))
         for t in out:
             t.record_stream(stream)
+        stream.wait_stream(default)
         return out
 
     def _run_wave(self, ctx, stream, wave):

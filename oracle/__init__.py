@@ -109,6 +109,14 @@ def lib():
         L.orc_lsrk_factor.argtypes = [P(LsrkFrame), ctypes.c_double]
         L.orc_lsrk_factors.argtypes = [P(LsrkFrame), ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
         L.orc_set_row_doppler.argtypes = [ctypes.c_void_p]
+        L.orc_model_grid.restype = ctypes.c_int
+        L.orc_model_grid.argtypes = [P(Params), ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_model_correction.restype = ctypes.c_double
+        L.orc_model_correction.argtypes = [P(Params), ctypes.c_int, ctypes.c_int]
+        L.orc_degrid.restype = ctypes.c_int
+        L.orc_degrid.argtypes = [P(Params), P(_Kernels), ctypes.c_int64, ctypes.c_int, ctypes.c_void_p,
+                                 ctypes.c_double, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                 P(ctypes.c_int64)]
         _lib = L
     return _lib
 
@@ -305,3 +313,31 @@ class row_doppler:
 
     def __exit__(self, *exc):
         lib().orc_set_row_doppler(None)
+
+
+# ---- prediction (oracle/mti_oracle_degrid.c) ---------------------------------------
+def model_grid(params, image, poltype="linear"):
+    """Stokes model images [npol][ny][nx] (finalize layout) -> model grids
+    [ny][nx][npol] complex128."""
+    image = np.ascontiguousarray(image, dtype=np.float64)
+    npol = image.shape[0]
+    g = np.zeros((params.ny, params.nx, npol, 2))
+    lib().orc_model_grid(ctypes.byref(params), npol, POLTYPE[poltype], _ptr(image), _ptr(g))
+    return g[..., 0] + 1j * g[..., 1]
+
+
+def model_correction(params, jy, jx):
+    return lib().orc_model_correction(ctypes.byref(params), int(jy), int(jx))
+
+
+def degrid(kern, uvw, inv_lambda, grid, normalise=True):
+    """Predicted visibilities [nrows][npol] complex128 of one channel from
+    model grids [ny][nx][npol] complex; returns (vis, predicted count)."""
+    uvw = np.ascontiguousarray(uvw, dtype=np.float64)
+    g = np.ascontiguousarray(np.stack([grid.real, grid.imag], axis=-1), dtype=np.float64)
+    npol = g.shape[2]
+    out = np.zeros((len(uvw), npol, 2))
+    n = ctypes.c_int64(0)
+    lib().orc_degrid(ctypes.byref(kern.params), ctypes.byref(kern._k), len(uvw), npol, _ptr(uvw),
+                     float(inv_lambda), _ptr(g), 1 if normalise else 0, _ptr(out), ctypes.byref(n))
+    return out[..., 0] + 1j * out[..., 1], int(n.value)

@@ -135,6 +135,18 @@ void   orc_lsrk_factors(const orc_lsrk_frame *f, const double *mjd_utc_s, int64_
  * (f = inv_lambda[chan] * factor[row]); NULL (the default) = no factor. */
 void   orc_set_row_doppler(const double *factor);
 
+/* Prediction (degridding; mti_oracle_degrid.c).  Model grids [ny][nx][npol][2]
+ * from Stokes model images [npol][ny][nx] in the finalize's layout: */
+int    orc_model_grid(const orc_params *p, int npol, int poltype, const double *image, double *grid);
+/* inverse kernel attenuation (sinc / taper) at transform pixel (jy, jx) */
+double orc_model_correction(const orc_params *p, int jy, int jx);
+/* Predicted visibilities vis [nrows][npol][2] of one channel (1/lambda
+ * inv_lambda) from model grids; normalise = 1 divides by the sub-kernel sums
+ * (prediction), 0 gives the adjoint of orc_grid's scatter.  Records whose
+ * support box is off the grid get 0; *predicted counts the others. */
+int    orc_degrid(const orc_params *p, const orc_kernels *K, int64_t nrows, int npol, const double *uvw,
+                  double inv_lambda, const double *grid, int normalise, double *vis, int64_t *predicted);
+
 #ifdef __cplusplus
 }
 #endif

@@ -55,7 +55,8 @@ def test_mfs_matches_oracle(orc, mti, ctx, c1, p, weighting):
     K = ctx.build_kernels(p)
     g, n, img = _gpu_mfs(mti, ctx, K, p, dev, idx, 1, 6, weighting)
     rg, rn, rimg = _oracle_mfs(orc, p, c1, idx.cpu().numpy(), 1, 6, weighting)
-    np.testing.assert_allclose(n, rn, rtol=1e-10)
+    # the library sums f32 sub-kernel sums and f32 effective weights in FP64
+    np.testing.assert_allclose(n, rn, rtol=1e-6)
     assert H.rel_rms(g[..., 0] + 1j * g[..., 1], rg[..., 0] + 1j * rg[..., 1]) < 1e-5
     for s in range(4 if weighting == "natural" else 1):
         e = H.image_errors(img[s], rimg[s])
@@ -76,7 +77,7 @@ def test_mfs_equals_sum_of_channels(mti, ctx, c1, p):
     ctx.grid(K, p, v, grids, norm, chan0=0, nchan=8)
     np.testing.assert_allclose(n, norm.sum(0).cpu().numpy(), rtol=1e-12)
     ref = grids.sum(0).cpu().numpy()
-    assert np.abs(g - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert np.abs(g - ref).max() <= 1e-10 * np.abs(ref).max()
 
 
 def test_mfs_batched_jobs(mti, ctx, c1, p):
@@ -94,4 +95,4 @@ def test_mfs_batched_jobs(mti, ctx, c1, p):
         rg, rn, _ = _gpu_mfs(mti, ctx, K, p, dev, idx, 4 * i, 4, "briggs")
         np.testing.assert_allclose(n.cpu().numpy()[0], rn, rtol=1e-12)
         gg = g.cpu().numpy()[0]
-        assert np.abs(gg - rg).max() <= 1e-12 * np.abs(rg).max()
+        assert np.abs(gg - rg).max() <= 1e-10 * np.abs(rg).max()
