@@ -264,6 +264,15 @@ __device__ __forceinline__ int fb_idx(int f, int n)
     return COL ? n * (E / N) + f : fb_pad(f * N + n);
 }
 
+// A loader with issue(f, n) -> token and finish(token) -> element (and a
+// GROUP constant) has the first pass's reads split in two: the reads of GROUP
+// elements are all issued before the first is finished, so they are in flight
+// together instead of one element's reads at a time.
+template <class L, class = void>
+struct HasIssue : std::false_type {};
+template <class L>
+struct HasIssue<L, std::void_t<decltype(std::declval<const L &>().issue(0, 0))>> : std::true_type {};
+
 // One Stockham pass of radix R at stride NS over the block.  load(f, n) gives
 // element n of transform f (FIRST); store(f, k, v) takes output k (LAST).
 template <int E, int TH, int N, bool COL, int R, int NS, bool FIRST, bool LAST, class Load, class Store>
@@ -275,13 +284,33 @@ __device__ __forceinline__ void fb_pass(double2 *lds, const double2 *tw, int NT,
     constexpr int B = E / (R * TH);
     static_assert(B >= 1, "block too small for the radix");
     double2 v[B][R];
+    if constexpr (FIRST && HasIssue<Load>::value) {
+        constexpr int G = Load::GROUP < B * R ? Load::GROUP : B * R;
+        static_assert((B * R) % G == 0, "group");
 #pragma unroll
-    for (int s = 0; s < B; s++) {
-        const int b = threadIdx.x + TH * s;
-        const int f = COL ? b % NF : b / PER, j = COL ? b / NF : b % PER;
+        for (int e0 = 0; e0 < B * R; e0 += G) {
+            typename Load::Token t[G];
 #pragma unroll
-        for (int r = 0; r < R; r++)
-            v[s][r] = FIRST ? load(f, j + PER * r) : lds[fb_idx<E, N, COL>(f, j + PER * r)];
+            for (int e = 0; e < G; e++) {
+                const int s = (e0 + e) / R, r = (e0 + e) % R, b = threadIdx.x + TH * s;
+                const int f = COL ? b % NF : b / PER, j = COL ? b / NF : b % PER;
+                t[e] = load.issue(f, j + PER * r);
+            }
+            __builtin_amdgcn_sched_barrier(0);  // all of the group's reads before the first use
+#pragma unroll
+            for (int e = 0; e < G; e++) v[(e0 + e) / R][(e0 + e) % R] = load.finish(t[e]);
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < B; s++) {
+            const int b = threadIdx.x + TH * s;
+            const int f = COL ? b % NF : b / PER, j = COL ? b / NF : b % PER;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                if constexpr (FIRST) v[s][r] = load(f, j + PER * r);
+                else v[s][r] = lds[fb_idx<E, N, COL>(f, j + PER * r)];
+            }
+        }
     }
     if (!FIRST) __syncthreads();
 #pragma unroll
@@ -325,99 +354,116 @@ __device__ __forceinline__ void fb_ifft(double2 *lds, const double2 *tw, int NT,
     fb_passes<E, TH, N, COL, 1, true>(lds, tw, NT, load, store);
 }
 
-// Row pass of packed planes.  Block size E = max(4096, 2N) elements (256 or
-// 512 threads); a block transforms E / (2N) pairs of rows (y, -y mod ny) of
-// packed plane blockIdx.y, built on the fly from the grid planes with the
-// PreIFFTUVCorrection reorder (transform row y, column x holds grid cell
-// ((y + ny/2) mod ny, (x + N/2) mod N)); the pair makes every conj(S(-y, -x))
-// a read within the block.  Rows 0 and ny/2 pair with themselves.
-constexpr int pk_e(int N) { return 2 * N > FB_E ? 2 * N : FB_E; }
-
-// Grid elements loaded through global-address-space pointers: the grid
-// pointers come from a descriptor in memory, and through a generic pointer
-// a load becomes a FLAT instruction, which also counts against the LDS wait
-// counter -- every LDS read of the tile mask would then wait for all grid
-// loads in flight.  Raw POD types (HIP vector types cannot be copied in that
-// address space by the host compile pass).
-#define MTI_GLOBAL __attribute__((address_space(1)))
-struct Raw128 {
-    unsigned long long lo, hi;
+// Row pass of packed planes.  Blocks of E = 4096 elements on 256 threads
+// (68 KB of LDS: two blocks per CU).  Below N = 4096 a block transforms
+// E / (2N) pairs of rows (y, -y mod ny) of packed plane pk; at N = 4096 it
+// transforms one row, y or its mirror.  Each element is built on the fly from
+// the grid planes with the PreIFFTUVCorrection reorder (transform row y,
+// column x holds grid cell ((y + ny/2) mod ny, (x + N/2) mod N)) from the
+// cells at (y, x) and (-y, -x); rows 0 and ny/2 are their own mirrors and
+// are transformed once.
+constexpr int PK_E = 4096;
+constexpr int PK_TH = 256;
+template <int N>
+struct PkGeom {
+    static constexpr bool ROW = N == PK_E;             // one row a block (y or its mirror)
+    static constexpr int NF = PK_E / N;                // transform rows of a block
+    static constexpr int PB = ROW ? 1 : PK_E / (2 * N);  // row pairs of a block
+    static constexpr int M = ROW ? 2 : 1;              // blocks per row pair
+    static constexpr int NTX = (N + 31) / 32;          // 32-cell tiles of a row
+    static constexpr int MR = ROW ? 2 : NF;            // tile-mask rows staged: own and mirror rows
 };
+
+// Grid elements are read through buffer resources over the grid planes: the
+// byte offset of a cell is one 32-bit register, and the buffer's range check
+// makes an offset past the plane's end (PK_OUTSIDE) read as zero bits without
+// a memory access -- the read of a cell in a tile the mask leaves out, of a
+// row past the last pair, or of an absent plane B needs no branch, so the
+// compiler is free to issue the reads of all of a thread's elements before
+// the arithmetic waits for the first.  Zero bits are +0 in either precision.
+constexpr uint32_t PK_OUTSIDE = 0xfffffff0u;
+constexpr int PK_BUFFER_DWORD3 = 0x00020000;  // raw buffer, gfx9-family dword 3
+typedef unsigned int pk_u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int pk_u32x4 __attribute__((ext_vector_type(4)));
+
 template <typename TI>
-using RawOf = typename std::conditional<sizeof(TI) == 16, Raw128, unsigned long long>::type;
-
-// Element i of a grid of complex128 (Raw128) or complex64 (one 64-bit word).
-__device__ __forceinline__ void raw_load(const void *base, size_t i, Raw128 &t)
-{
-    const MTI_GLOBAL unsigned long long *p = (const MTI_GLOBAL unsigned long long *)base + 2 * i;
-    t.lo = p[0];
-    t.hi = p[1];
-}
-__device__ __forceinline__ void raw_load(const void *base, size_t i, unsigned long long &t)
-{
-    t = ((const MTI_GLOBAL unsigned long long *)base)[i];
-}
-
-__device__ __forceinline__ double2 raw_to_c(Raw128 t)
-{
-    return make_double2(__longlong_as_double((long long)t.lo), __longlong_as_double((long long)t.hi));
-}
-__device__ __forceinline__ double2 raw_to_c(unsigned long long t)
-{
-    return make_double2((double)__uint_as_float((unsigned)t), (double)__uint_as_float((unsigned)(t >> 32)));
-}
-
-template <int N, typename TI, typename XT>
-__global__ __launch_bounds__(pk_e(N) / 16) void k_rows_pk(const PackDesc *pd, XT *X, int ny, int64_t cs,
-                                                         const double2 *tw, int NT)
-{
-    constexpr int E = pk_e(N), TH = E / 16, PB = E / (2 * N), NF = E / N, NTX = (N + 31) / 32;
-    __shared__ double2 lds[E + E / 16];
-    __shared__ uint8_t s_mask[2][NF][NTX];  // tile masks of the block's rows (sparse grids)
-    const PackDesc &d = pd[blockIdx.y];
-    const TI *A = (const TI *)d.a, *B = (const TI *)d.b;
-    const double ia = 1.0 / *d.na, ib = B ? 1.0 / *d.nb : 0.0;
-    const double2 ka0 = make_double2(d.ca[0].x * ia, d.ca[0].y * ia), ka1 = make_double2(d.ca[1].x * ia, d.ca[1].y * ia);
-    const double2 kb0 = make_double2(d.cb[0].x * ib, d.cb[0].y * ib), kb1 = make_double2(d.cb[1].x * ib, d.cb[1].y * ib);
-    XT *dst = X + (size_t)blockIdx.y * N * ny;
-    const int npairs = ny / 2 + 1;
-    // the tile-mask rows of the block's transform rows (all 1 for complete
-    // grids): cells of tiles no record reached are read as zeros, never loaded
-    for (int i = threadIdx.x; i < 2 * NF * NTX; i += TH) {
-        const int ab = i / (NF * NTX), f = (i / NTX) % NF, t = i % NTX;
-        const uint8_t *m = ab ? d.mb : d.ma;
-        const int pair = blockIdx.x * PB + (f >> 1);
-        uint8_t v = 1;
-        if (m && pair < npairs) {
-            const int y = (f & 1) ? ((ny - pair) & (ny - 1)) : pair;
-            v = m[(((y + (ny >> 1)) & (ny - 1)) >> 5) * NTX + t];
-        }
-        s_mask[ab][f][t] = v;
+struct PkRaw;
+template <>
+struct PkRaw<float2> {  // complex64 grid cell
+    typedef pk_u32x2 T;
+    static __device__ __forceinline__ T ld(__amdgpu_buffer_rsrc_t r, uint32_t off)
+    {
+        return __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
     }
-    __syncthreads();
-    // grid loads through global-address-space pointers (see raw_load); an
-    // element comes back as raw bits (all zero bits in a tile the mask leaves
-    // out: +0 in either precision) and is widened only after all four loads
-    // are issued, so no conversion inside the mask branch waits for its load
-    using Raw = RawOf<TI>;
-    auto elem = [&](const TI *P0, int ab, int f, int y, int x) {
-        const int gx = (x + N / 2) & (N - 1);
-        Raw t{};
-        if (s_mask[ab][f][gx >> 5]) raw_load(P0, ((size_t)((y + (ny >> 1)) & (ny - 1)) * N + gx) * cs, t);
-        return t;
+    static __device__ __forceinline__ double2 wide(T t)
+    {
+        return make_double2((double)__uint_as_float(t.x), (double)__uint_as_float(t.y));
+    }
+};
+template <>
+struct PkRaw<double2> {  // complex128 grid cell
+    typedef pk_u32x4 T;
+    static __device__ __forceinline__ T ld(__amdgpu_buffer_rsrc_t r, uint32_t off)
+    {
+        return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    }
+    static __device__ __forceinline__ double2 wide(T t)
+    {
+        return make_double2(__hiloint2double((int)t.y, (int)t.x), __hiloint2double((int)t.w, (int)t.z));
+    }
+};
+
+// Bytes a grid plane spans (cells [ny][N], cs elements apart).
+__host__ __device__ inline uint64_t pk_span_bytes(int N, int ny, int64_t cs, size_t elem)
+{
+    return (((uint64_t)ny * N - 1) * (uint64_t)cs + 1) * elem;
+}
+
+// The row pass's element loader: element x of transform row f of the block
+// is Z = H_0 + i H_1 built from four grid cells (planes A and B at (y, x) and
+// at (-y, -x)).  issue() starts the four reads, finish() widens and packs.
+// GROUP elements (64 reads of complex64 cells, 32 of complex128) are issued
+// before the first is finished, 128 VGPRs of raw cells either way.
+template <int N, typename TI>
+struct PkRowLoad {
+    using G = PkGeom<N>;
+    static constexpr int GROUP = sizeof(TI) == 8 ? 16 : 8;
+    struct Token {
+        typename PkRaw<TI>::T a0, a1, b0, b1;
     };
-    auto load = [&](int f, int x) {
-        const int pair = blockIdx.x * PB + (f >> 1);
-        if (pair >= npairs) return make_double2(0.0, 0.0);
-        const int y = (f & 1) ? ((ny - pair) & (ny - 1)) : pair;
+    __amdgpu_buffer_rsrc_t rA, rB;
+    double2 ka0, ka1, kb0, kb1;
+    const uint8_t (*mask)[G::MR][G::NTX];  // LDS tile-mask rows [2][MR][NTX]
+    int bx, ny, y1;                       // y1: the row of a one-row block
+    uint32_t cb;                          // bytes between cells of a grid row
+    // transform row of block row f, and the mask rows of it and of its mirror
+    __device__ __forceinline__ int row(int f) const
+    {
+        if (G::ROW) return y1;
+        const int pair = bx * G::PB + (f >> 1);
+        return (f & 1) ? ((ny - pair) & (ny - 1)) : pair;
+    }
+    __device__ __forceinline__ uint32_t off(int ab, int mr, int y, int x) const
+    {
+        const int gx = (x + N / 2) & (N - 1);
+        const uint32_t o = ((uint32_t)((y + (ny >> 1)) & (ny - 1)) * N + gx) * cb;
+        return mask[ab][mr][gx >> 5] ? o : PK_OUTSIDE;
+    }
+    __device__ __forceinline__ Token issue(int f, int x) const
+    {
+        const int y = row(f), mr = G::ROW ? 0 : f, mm = G::ROW ? 1 : f ^ 1;
         const int ym = (ny - y) & (ny - 1), xm = (N - x) & (N - 1);
-        const Raw ra0 = elem(A, 0, f, y, x), ra1 = elem(A, 0, f ^ 1, ym, xm);
-        Raw rb0{}, rb1{};
-        if (B) {
-            rb0 = elem(B, 1, f, y, x);
-            rb1 = elem(B, 1, f ^ 1, ym, xm);
-        }
-        const double2 a0 = raw_to_c(ra0), a1 = raw_to_c(ra1), b0 = raw_to_c(rb0), b1 = raw_to_c(rb1);
+        Token t;
+        t.a0 = PkRaw<TI>::ld(rA, off(0, mr, y, x));
+        t.a1 = PkRaw<TI>::ld(rA, off(0, mm, ym, xm));
+        t.b0 = PkRaw<TI>::ld(rB, off(1, mr, y, x));
+        t.b1 = PkRaw<TI>::ld(rB, off(1, mm, ym, xm));
+        return t;
+    }
+    __device__ __forceinline__ double2 finish(const Token &t) const
+    {
+        const double2 a0 = PkRaw<TI>::wide(t.a0), a1 = PkRaw<TI>::wide(t.a1);
+        const double2 b0 = PkRaw<TI>::wide(t.b0), b1 = PkRaw<TI>::wide(t.b1);
         // S_s at (y, x) and at (-y, -x)
         const double2 s0 = c2add(c2mul(ka0, a0), c2mul(kb0, b0)), s0m = c2add(c2mul(ka0, a1), c2mul(kb0, b1));
         const double2 s1 = c2add(c2mul(ka1, a0), c2mul(kb1, b0)), s1m = c2add(c2mul(ka1, a1), c2mul(kb1, b1));
@@ -425,15 +471,106 @@ __global__ __launch_bounds__(pk_e(N) / 16) void k_rows_pk(const PackDesc *pd, XT
         const double h0r = 0.5 * (s0.x + s0m.x), h0i = 0.5 * (s0.y - s0m.y);
         const double h1r = 0.5 * (s1.x + s1m.x), h1i = 0.5 * (s1.y - s1m.y);
         return make_double2(h0r - h1i, h0i + h1r);
-    };
+    }
+};
+
+// Block -> (row block bx, mirror bit mir, pack pk) of the row pass, a
+// bijection over the npk * nbx * M blocks of a 1-D launch.  The two packs 2c,
+// 2c+1 of a full-polarisation channel read the same grid cache lines when the
+// correlations are interleaved per cell (each uses two of the four), and a
+// one-row block reads the lines of its mirror row's block too.  Blocks go
+// round-robin to the 8 XCDs, each with its own L2, so the 2M blocks that
+// share lines are made L, L + 8, ...: they start together on one XCD and the
+// later ones find the lines in L2.  An unpaired last pack takes the tail.
+template <int M>
+__device__ __forceinline__ void pk_block(int nbx, int npk, int &bx, int &mir, int &pk)
+{
+    constexpr int G = 2 * M;
+    const int L = blockIdx.x;
+    const int paired = G * (npk >> 1) * nbx;
+    if (L >= paired) {
+        pk = npk - 1;
+        bx = (L - paired) / M;
+        mir = (L - paired) % M;
+        return;
+    }
+    const int full = paired - paired % (8 * G);
+    int u, k;
+    if (L < full) {
+        u = (L / (8 * G)) * 8 + (L & 7);
+        k = (L >> 3) % G;
+    } else {
+        u = full / G + (L - full) / G;
+        k = (L - full) % G;
+    }
+    pk = 2 * (u / nbx) + (k & 1);
+    bx = u % nbx;
+    mir = k >> 1;
+}
+
+template <int N, typename TI, typename XT>
+__global__ __launch_bounds__(PK_TH) void k_rows_pk(const PackDesc *pd, int npk, XT *X, int ny, int64_t cs,
+                                                  const double2 *tw, int NT)
+{
+    using G = PkGeom<N>;
+    constexpr int E = PK_E, NTX = G::NTX;
+    __shared__ double2 lds[E + E / 16];
+    __shared__ uint8_t s_mask[2][G::MR][NTX];  // tile masks of the block's rows (sparse grids)
+    const int npairs = ny / 2 + 1, nbx = (npairs + G::PB - 1) / G::PB;
+    int bx, mir, pk;
+    pk_block<G::M>(nbx, npk, bx, mir, pk);
+    // a one-row block: row bx, or its mirror; a self-mirrored row has one block
+    const int y1 = mir ? ((ny - bx) & (ny - 1)) : bx;
+    if (G::ROW && mir && y1 == bx) return;
+    const PackDesc &d = pd[pk];
+    const void *B = d.b;
+    const double ia = 1.0 / *d.na, ib = B ? 1.0 / *d.nb : 0.0;
+    XT *dst = X + (size_t)pk * N * ny;
+    PkRowLoad<N, TI> load;
+    load.bx = bx;
+    load.ny = ny;
+    load.y1 = y1;
+    // the tile-mask rows of the block's transform rows and of their mirrors
+    // (all 1 for complete grids): cells of tiles no record reached, and of
+    // rows past the last pair, are read as zeros, never loaded
+    for (int i = threadIdx.x; i < 2 * G::MR * NTX; i += PK_TH) {
+        const int ab = i / (G::MR * NTX), f = (i / NTX) % G::MR, t = i % NTX;
+        const uint8_t *m = ab ? d.mb : d.ma;
+        uint8_t v;
+        int y;
+        if (G::ROW) {
+            v = 1;
+            y = f ? (ny - y1) & (ny - 1) : y1;
+        } else {
+            v = bx * G::PB + (f >> 1) < npairs;
+            y = load.row(f);
+        }
+        if (m && v) v = m[(((y + (ny >> 1)) & (ny - 1)) >> 5) * NTX + t];
+        s_mask[ab][f][t] = v;
+    }
+    __syncthreads();
+    const uint32_t span = (uint32_t)pk_span_bytes(N, ny, cs, sizeof(TI));
+    load.rA = __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(d.a), (short)0, (int)span, PK_BUFFER_DWORD3);
+    load.rB = __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(d.b), (short)0, B ? (int)span : 0,
+                                                PK_BUFFER_DWORD3);
+    load.ka0 = make_double2(d.ca[0].x * ia, d.ca[0].y * ia);
+    load.ka1 = make_double2(d.ca[1].x * ia, d.ca[1].y * ia);
+    load.kb0 = make_double2(d.cb[0].x * ib, d.cb[0].y * ib);
+    load.kb1 = make_double2(d.cb[1].x * ib, d.cb[1].y * ib);
+    load.mask = s_mask;
+    load.cb = (uint32_t)cs * (uint32_t)sizeof(TI);
     auto store = [&](int f, int kx, double2 v) {
-        const int pair = blockIdx.x * PB + (f >> 1);
+        if (G::ROW) {
+            dst[(size_t)y1 * N + kx] = x_st<XT>(v);
+            return;
+        }
+        const int pair = bx * G::PB + (f >> 1);
         if (pair >= npairs) return;
-        const int y = (f & 1) ? ((ny - pair) & (ny - 1)) : pair;
+        const int y = load.row(f);
         if ((f & 1) && y == pair) return;  // a self-paired row is written once
         dst[(size_t)y * N + kx] = x_st<XT>(v);
     };
-    fb_ifft<N, false, E, TH>(lds, tw, NT, load, store);
+    fb_ifft<N, false, E, PK_TH>(lds, tw, NT, load, store);
 }
 
 // Pass B1: length-N1 transforms down the columns (rows N2*n1 + n2) of NF =
@@ -544,12 +681,14 @@ static mti_status pk_rows(mti_ctx *ctx, const FftGeom &g, const PackDesc *pd, in
                           const double2 *tw, hipStream_t st)
 {
     const int npairs = g.ny / 2 + 1;
+    if (pk_span_bytes(g.nx, g.ny, cs, sizeof(TI)) > PK_OUTSIDE)
+        return set_error(ctx, MTI_ERR_UNSUPPORTED, "grid plane spans more than 4 GiB");
     switch (g.nx) {
 #define MTI_PR(NN)                                                                                          \
     case NN: {                                                                                              \
-        constexpr int PB = pk_e(NN) / (2 * NN);                                                             \
-        const dim3 gr((unsigned)ceil_div(npairs, PB), (unsigned)npk);                                       \
-        hipLaunchKernelGGL((k_rows_pk<NN, TI, XT>), gr, dim3(pk_e(NN) / 16), 0, st, pd, X, g.ny, cs, tw, g.NT); \
+        using G = PkGeom<NN>;                                                                               \
+        const dim3 gr((unsigned)(ceil_div(npairs, G::PB) * G::M * npk));                                    \
+        hipLaunchKernelGGL((k_rows_pk<NN, TI, XT>), gr, dim3(PK_TH), 0, st, pd, npk, X, g.ny, cs, tw, g.NT); \
         break;                                                                                              \
     }
         MTI_PR(16) MTI_PR(32) MTI_PR(64) MTI_PR(128) MTI_PR(256) MTI_PR(512) MTI_PR(1024) MTI_PR(2048)
