@@ -419,6 +419,17 @@ __host__ __device__ inline uint64_t pk_span_bytes(int N, int ny, int64_t cs, siz
     return (((uint64_t)ny * N - 1) * (uint64_t)cs + 1) * elem;
 }
 
+// With S_s = ka_s A + kb_s B (ka_s = ca[s] / na, kb_s = cb[s] / nb) and
+// H_s = (S_s + conj(S_s(-y, -x))) / 2, the packed element Z = H_0 + i H_1 is
+//   Z = cA A + cB B + cAm conj(A(-y, -x)) + cBm conj(B(-y, -x)),
+//   cA = (ka_0 + i ka_1) / 2,  cAm = (conj(ka_0) + i conj(ka_1)) / 2,
+// and cB, cBm alike from kb: four complex multiply-adds an element.
+__device__ __forceinline__ void pk_coefs(double2 k0, double2 k1, double2 &c, double2 &cm)
+{
+    c = make_double2(0.5 * (k0.x - k1.y), 0.5 * (k0.y + k1.x));
+    cm = make_double2(0.5 * (k0.x + k1.y), 0.5 * (k1.x - k0.y));
+}
+
 // The row pass's element loader: element x of transform row f of the block
 // is Z = H_0 + i H_1 built from four grid cells (planes A and B at (y, x) and
 // at (-y, -x)).  issue() starts the four reads, finish() widens and packs.
@@ -432,7 +443,7 @@ struct PkRowLoad {
         typename PkRaw<TI>::T a0, a1, b0, b1;
     };
     __amdgpu_buffer_rsrc_t rA, rB;
-    double2 ka0, ka1, kb0, kb1;
+    double2 cA, cB, cAm, cBm;  // packing coefficients (pk_coefs)
     const uint8_t (*mask)[G::MR][G::NTX];  // LDS tile-mask rows [2][MR][NTX]
     int bx, ny, y1;                       // y1: the row of a one-row block
     uint32_t cb;                          // bytes between cells of a grid row
@@ -462,15 +473,26 @@ struct PkRowLoad {
     }
     __device__ __forceinline__ double2 finish(const Token &t) const
     {
-        const double2 a0 = PkRaw<TI>::wide(t.a0), a1 = PkRaw<TI>::wide(t.a1);
-        const double2 b0 = PkRaw<TI>::wide(t.b0), b1 = PkRaw<TI>::wide(t.b1);
-        // S_s at (y, x) and at (-y, -x)
-        const double2 s0 = c2add(c2mul(ka0, a0), c2mul(kb0, b0)), s0m = c2add(c2mul(ka0, a1), c2mul(kb0, b1));
-        const double2 s1 = c2add(c2mul(ka1, a0), c2mul(kb1, b0)), s1m = c2add(c2mul(ka1, a1), c2mul(kb1, b1));
-        // H_s = (S_s + conj(S_s(-y,-x))) / 2;  Z = H_0 + i H_1
-        const double h0r = 0.5 * (s0.x + s0m.x), h0i = 0.5 * (s0.y - s0m.y);
-        const double h1r = 0.5 * (s1.x + s1m.x), h1i = 0.5 * (s1.y - s1m.y);
-        return make_double2(h0r - h1i, h0i + h1r);
+        const double2 a = PkRaw<TI>::wide(t.a0), am = PkRaw<TI>::wide(t.a1);
+        const double2 b = PkRaw<TI>::wide(t.b0), bm = PkRaw<TI>::wide(t.b1);
+        // Z = cA A + cB B + cAm conj(A(-y,-x)) + cBm conj(B(-y,-x))
+        double zr = cA.x * a.x;
+        zr = fma(-cA.y, a.y, zr);
+        zr = fma(cB.x, b.x, zr);
+        zr = fma(-cB.y, b.y, zr);
+        zr = fma(cAm.x, am.x, zr);
+        zr = fma(cAm.y, am.y, zr);
+        zr = fma(cBm.x, bm.x, zr);
+        zr = fma(cBm.y, bm.y, zr);
+        double zi = cA.x * a.y;
+        zi = fma(cA.y, a.x, zi);
+        zi = fma(cB.x, b.y, zi);
+        zi = fma(cB.y, b.x, zi);
+        zi = fma(cAm.y, am.x, zi);
+        zi = fma(-cAm.x, am.y, zi);
+        zi = fma(cBm.y, bm.x, zi);
+        zi = fma(-cBm.x, bm.y, zi);
+        return make_double2(zr, zi);
     }
 };
 
@@ -553,10 +575,10 @@ __global__ __launch_bounds__(PK_TH) void k_rows_pk(const PackDesc *pd, int npk, 
     load.rA = __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(d.a), (short)0, (int)span, PK_BUFFER_DWORD3);
     load.rB = __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(d.b), (short)0, B ? (int)span : 0,
                                                 PK_BUFFER_DWORD3);
-    load.ka0 = make_double2(d.ca[0].x * ia, d.ca[0].y * ia);
-    load.ka1 = make_double2(d.ca[1].x * ia, d.ca[1].y * ia);
-    load.kb0 = make_double2(d.cb[0].x * ib, d.cb[0].y * ib);
-    load.kb1 = make_double2(d.cb[1].x * ib, d.cb[1].y * ib);
+    pk_coefs(make_double2(d.ca[0].x * ia, d.ca[0].y * ia), make_double2(d.ca[1].x * ia, d.ca[1].y * ia), load.cA,
+             load.cAm);
+    pk_coefs(make_double2(d.cb[0].x * ib, d.cb[0].y * ib), make_double2(d.cb[1].x * ib, d.cb[1].y * ib), load.cB,
+             load.cBm);
     load.mask = s_mask;
     load.cb = (uint32_t)cs * (uint32_t)sizeof(TI);
     auto store = [&](int f, int kx, double2 v) {
