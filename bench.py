@@ -796,6 +796,20 @@ def main():
     elapsed = time.perf_counter() - t_all
     ctx.set_profiling(False)
     stages = ctx.stage_times()
+    # the transform's row pass (k_rows_pk): its algorithmic bytes are the grid
+    # cells of marked tiles it reads (every correlation plane of the channel)
+    # and the packed planes it writes
+    rows_pass = None
+    if stages.get("fft_rows", (0, 0))[1] > 0:
+        marked = 1.0 if tmask is None else float(tmask.float().mean().item())
+        gsz = 8 if args.grid_precision == "c64" else 16
+        xsz = 8 if args.transform_storage == "c64" else 16
+        npk = nchan * P / 2.0
+        rd, wr = marked * nchan * P * nx * nx * gsz, npk * nx * nx * xsz
+        rms = stages["fft_rows"][0] / args.steps
+        rows_pass = dict(kernel="k_rows_pk", tiles_marked=round(marked, 4), packs=npk, read_GB=round(rd / 1e9, 3),
+                         write_GB=round(wr / 1e9, 3), ms_per_step=round(rms, 3),
+                         algorithmic_GBps=round((rd + wr) / (rms * 1e-3) / 1e9, 1))
     # the timed job's images: the reference of the end-to-end runs (the
     # precision-form runs below reuse the image buffer)
     resident_image = None if args.no_e2e else image.clone()
@@ -954,6 +968,7 @@ def main():
         "gridding_stage_mvis_s": round(gridded / (sum(stages[k][0] for k in ("bin", "sort_records", "plan", "grid"))
                                                   / args.steps / 1e3) / 1e6, 3),
         "roofline": roofline,
+        "transform_rows": rows_pass,
         # from pinned host memory, host-to-device copies overlapped with gridding (not `value`)
         "end_to_end": e2e,
         "cpu_baseline": cpu,

@@ -284,6 +284,19 @@ __device__ __forceinline__ void fb_pass(double2 *lds, const double2 *tw, int NT,
     constexpr int B = E / (R * TH);
     static_assert(B >= 1, "block too small for the radix");
     double2 v[B][R];
+    // the pass's twiddles are read before its LDS reads and barrier, so their
+    // round trip overlaps the wait for the other waves
+    double2 t[B][R > 1 ? R - 1 : 1];
+    if constexpr (NS > 1) {
+        const int step = NT / (NS * R);
+#pragma unroll
+        for (int s = 0; s < B; s++) {
+            const int b = threadIdx.x + TH * s;
+            const int j = COL ? b / NF : b % PER, k = j % NS;
+#pragma unroll
+            for (int r = 1; r < R; r++) t[s][r - 1] = tw[r * k * step];
+        }
+    }
     if constexpr (FIRST && HasIssue<Load>::value) {
         constexpr int G = Load::GROUP < B * R ? Load::GROUP : B * R;
         static_assert((B * R) % G == 0, "group");
@@ -318,10 +331,9 @@ __device__ __forceinline__ void fb_pass(double2 *lds, const double2 *tw, int NT,
         const int b = threadIdx.x + TH * s;
         const int f = COL ? b % NF : b / PER, j = COL ? b / NF : b % PER;
         const int k = j % NS;
-        if (NS > 1) {
-            const int step = NT / (NS * R);
+        if constexpr (NS > 1) {
 #pragma unroll
-            for (int r = 1; r < R; r++) v[s][r] = c2mul(v[s][r], tw[r * k * step]);
+            for (int r = 1; r < R; r++) v[s][r] = c2mul(v[s][r], t[s][r - 1]);
         }
         idft<R>(v[s]);
         const int base = (j - k) * R + k;
