@@ -208,6 +208,25 @@ __device__ __forceinline__ double2 w16(int m)
     }
 }
 
+// v * exp(+2 pi i M / 16) for a constant M: the multiples of 1/8 turn need
+// no general complex multiply (a product with 0.0 or 1.0 is not folded away
+// by the compiler under IEEE rules).
+template <int M>
+__device__ __forceinline__ double2 c2mul_w16(double2 v)
+{
+    constexpr int m = M & 15;
+    constexpr double H = 0.70710678118654752440;
+    if constexpr (m == 0) return v;
+    else if constexpr (m == 4) return make_double2(-v.y, v.x);
+    else if constexpr (m == 8) return make_double2(-v.x, -v.y);
+    else if constexpr (m == 12) return make_double2(v.y, -v.x);
+    else if constexpr (m == 2) return make_double2(H * (v.x - v.y), H * (v.x + v.y));
+    else if constexpr (m == 6) return make_double2(-H * (v.x + v.y), H * (v.x - v.y));
+    else if constexpr (m == 10) return make_double2(H * (v.y - v.x), -H * (v.x + v.y));
+    else if constexpr (m == 14) return make_double2(H * (v.x + v.y), H * (v.y - v.x));
+    else return c2mul(v, w16(m));
+}
+
 // Inverse DFT of length R in place (natural order in and out).
 template <int R>
 __device__ __forceinline__ void idft(double2 (&v)[R])
@@ -223,11 +242,11 @@ __device__ __forceinline__ void idft(double2 (&v)[R])
         double2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
         idft4(e[0], e[1], e[2], e[3]);
         idft4(o[0], o[1], o[2], o[3]);
+        const double2 t[4] = {o[0], c2mul_w16<2>(o[1]), c2mul_w16<4>(o[2]), c2mul_w16<6>(o[3])};
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            const double2 t = c2mul(o[k], w16(2 * k));
-            v[k] = c2add(e[k], t);
-            v[k + 4] = c2sub(e[k], t);
+            v[k] = c2add(e[k], t[k]);
+            v[k + 4] = c2sub(e[k], t[k]);
         }
     } else {
         static_assert(R == 16, "radix");
@@ -240,10 +259,17 @@ __device__ __forceinline__ void idft(double2 (&v)[R])
             y[n2][2] = v[8 + n2];
             y[n2][3] = v[12 + n2];
             idft4(y[n2][0], y[n2][1], y[n2][2], y[n2][3]);
-#pragma unroll
-            for (int k1 = 1; k1 < 4; k1++)
-                if (n2 > 0) y[n2][k1] = c2mul(y[n2][k1], w16(n2 * k1));
         }
+        // twiddles w16^(n2 k1)
+        y[1][1] = c2mul_w16<1>(y[1][1]);
+        y[1][2] = c2mul_w16<2>(y[1][2]);
+        y[1][3] = c2mul_w16<3>(y[1][3]);
+        y[2][1] = c2mul_w16<2>(y[2][1]);
+        y[2][2] = c2mul_w16<4>(y[2][2]);
+        y[2][3] = c2mul_w16<6>(y[2][3]);
+        y[3][1] = c2mul_w16<3>(y[3][1]);
+        y[3][2] = c2mul_w16<6>(y[3][2]);
+        y[3][3] = c2mul_w16<9>(y[3][3]);
 #pragma unroll
         for (int k1 = 0; k1 < 4; k1++) {
             idft4(y[0][k1], y[1][k1], y[2][k1], y[3][k1]);
@@ -346,24 +372,24 @@ __device__ __forceinline__ void fb_pass(double2 *lds, const double2 *tw, int NT,
     if (!LAST) __syncthreads();
 }
 
-template <int E, int TH, int N, bool COL, int NS, bool FIRST, class Load, class Store>
+template <int E, int TH, int N, bool COL, int NS, bool FIRST, int RMAX, class Load, class Store>
 __device__ __forceinline__ void fb_passes(double2 *lds, const double2 *tw, int NT, const Load &load,
                                           const Store &store)
 {
     constexpr int REM = N / NS;
-    constexpr int R = REM >= 16 ? 16 : REM;
+    constexpr int R = REM >= RMAX ? RMAX : REM;
     fb_pass<E, TH, N, COL, R, NS, FIRST, REM == R>(lds, tw, NT, load, store);
-    if constexpr (REM > R) fb_passes<E, TH, N, COL, NS * R, false>(lds, tw, NT, load, store);
+    if constexpr (REM > R) fb_passes<E, TH, N, COL, NS * R, false, RMAX>(lds, tw, NT, load, store);
 }
 
 // Full inverse transforms of a block of E elements (E / N transforms of
-// length N) on TH threads: load -> passes -> store.
-template <int N, bool COL, int E = FB_E, int TH = FB_THREADS, class Load, class Store>
+// length N) on TH threads, passes of radix up to RMAX: load -> passes -> store.
+template <int N, bool COL, int E = FB_E, int TH = FB_THREADS, int RMAX = 16, class Load, class Store>
 __device__ __forceinline__ void fb_ifft(double2 *lds, const double2 *tw, int NT, const Load &load,
                                         const Store &store)
 {
     static_assert(N >= 16 && N <= E, "length");
-    fb_passes<E, TH, N, COL, 1, true>(lds, tw, NT, load, store);
+    fb_passes<E, TH, N, COL, 1, true, RMAX>(lds, tw, NT, load, store);
 }
 
 // Row pass of packed planes.  Blocks of E = 4096 elements on 256 threads
@@ -376,6 +402,7 @@ __device__ __forceinline__ void fb_ifft(double2 *lds, const double2 *tw, int NT,
 // are transformed once.
 constexpr int PK_E = 4096;
 constexpr int PK_TH = 256;
+constexpr int PK_RMAX = 16;
 template <int N>
 struct PkGeom {
     static constexpr bool ROW = N == PK_E;             // one row a block (y or its mirror)
@@ -450,7 +477,7 @@ __device__ __forceinline__ void pk_coefs(double2 k0, double2 k1, double2 &c, dou
 template <int N, typename TI>
 struct PkRowLoad {
     using G = PkGeom<N>;
-    static constexpr int GROUP = sizeof(TI) == 8 ? 16 : 8;
+    static constexpr int GROUP = 128 / (int)sizeof(TI);
     struct Token {
         typename PkRaw<TI>::T a0, a1, b0, b1;
     };
@@ -517,10 +544,9 @@ struct PkRowLoad {
 // share lines are made L, L + 8, ...: they start together on one XCD and the
 // later ones find the lines in L2.  An unpaired last pack takes the tail.
 template <int M>
-__device__ __forceinline__ void pk_block(int nbx, int npk, int &bx, int &mir, int &pk)
+__device__ __forceinline__ void pk_block(int L, int nbx, int npk, int &bx, int &mir, int &pk)
 {
     constexpr int G = 2 * M;
-    const int L = blockIdx.x;
     const int paired = G * (npk >> 1) * nbx;
     if (L >= paired) {
         pk = npk - 1;
@@ -552,7 +578,7 @@ __global__ __launch_bounds__(PK_TH) void k_rows_pk(const PackDesc *pd, int npk, 
     __shared__ uint8_t s_mask[2][G::MR][NTX];  // tile masks of the block's rows (sparse grids)
     const int npairs = ny / 2 + 1, nbx = (npairs + G::PB - 1) / G::PB;
     int bx, mir, pk;
-    pk_block<G::M>(nbx, npk, bx, mir, pk);
+    pk_block<G::M>(blockIdx.x, nbx, npk, bx, mir, pk);
     // a one-row block: row bx, or its mirror; a self-mirrored row has one block
     const int y1 = mir ? ((ny - bx) & (ny - 1)) : bx;
     if (G::ROW && mir && y1 == bx) return;
@@ -604,7 +630,217 @@ __global__ __launch_bounds__(PK_TH) void k_rows_pk(const PackDesc *pd, int npk, 
         if ((f & 1) && y == pair) return;  // a self-paired row is written once
         dst[(size_t)y * N + kx] = x_st<XT>(v);
     };
-    fb_ifft<N, false, E, PK_TH>(lds, tw, NT, load, store);
+    fb_ifft<N, false, E, PK_TH, PK_RMAX>(lds, tw, NT, load, store);
+}
+
+// Values the same in every lane, moved to scalar registers: in a loop after
+// global stores the compiler cannot prove a descriptor load unclobbered and
+// loads it per lane; a buffer resource built from per-lane registers turns
+// every read through it into a waterfall loop.
+__device__ __forceinline__ const void *pk_uniform_ptr(const void *p)
+{
+    const uint64_t v = (uint64_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (const void *)(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ double pk_uniform_f64(double x)
+{
+    const uint64_t v = (uint64_t)__double_as_longlong(x);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// v * w^r for r = 1..15 from w and w^4 (powers w^2, w^3, w^8, w^12 and
+// w^(4a + b) = w^(4a) w^b): the twiddles of a pass from two table values.
+__device__ __forceinline__ void pk_twiddle16(double2 (&v)[16], double2 w1, double2 w4)
+{
+    const double2 w2 = c2mul(w1, w1), w3 = c2mul(w2, w1), w8 = c2mul(w4, w4), w12 = c2mul(w8, w4);
+    const double2 lo[4] = {make_double2(1.0, 0.0), w1, w2, w3}, hi[4] = {make_double2(1.0, 0.0), w4, w8, w12};
+#pragma unroll
+    for (int r = 1; r < 16; r++) {
+        const int a = r >> 2, b = r & 3;
+        const double2 w = a == 0 ? lo[b] : (b == 0 ? hi[a] : c2mul(hi[a], lo[b]));
+        v[r] = c2mul(v[r], w);
+    }
+}
+
+// One radix-16 pass of a 4096-element row held in LDS (256 threads, stride
+// NS > 1), as fb_pass, with twiddles from the bases w^k, w^4k in LDS.
+template <int NS, bool LAST, class Store>
+__device__ __forceinline__ void pk_row_pass(double2 *lds, const double2 *tb, const Store &store)
+{
+    const int j = threadIdx.x, k = j % NS;
+    double2 v[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) v[r] = lds[fb_pad(j + 256 * r)];
+    __syncthreads();
+    pk_twiddle16(v, tb[2 * k], tb[2 * k + 1]);
+    idft<16>(v);
+    const int base = (j - k) * 16 + k;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        if (LAST) store(base + NS * r, v[r]);
+        else lds[fb_pad(base + NS * r)] = v[r];
+    }
+    if (!LAST) __syncthreads();
+}
+
+// Row pass of 4096-element rows from complex64 grids, software-pipelined:
+// a persistent block walks the row units L = blockIdx.x + k * gridDim.x (the
+// unit order of k_rows_pk, so the blocks that share grid lines still run
+// together on one XCD) and issues the grid reads of its next unit before the
+// second and third radix-16 passes of the current one, so the reads are in
+// flight during those passes instead of after them.  A global load waits for
+// every load issued before it, so no global load follows the grid reads
+// until they are used: the passes' twiddles come from bases w^k, w^4k staged
+// in LDS once per block (the others as products: a few ulp from the table
+// values), and the next unit's tile-mask rows and normalisers are read
+// during the current unit's first pass.
+template <typename XT>
+__global__ __launch_bounds__(PK_TH) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_rows_pk4096_pipe(
+    const PackDesc *__restrict__ pd, int npk, XT *X, int ny, int64_t cs, const double2 *tw, int NT)
+{
+    using TI = float2;
+    using G = PkGeom<4096>;
+    using Load = PkRowLoad<4096, TI>;
+    constexpr int N = 4096, NTX = G::NTX;
+    static_assert(G::ROW && PK_TH == 256 && PK_E == N && NTX == 128, "row geometry");
+    __shared__ double2 lds[N + N / 16];
+    __shared__ double2 s_tw2[2 * 16], s_tw3[2 * 256];  // [k][w^k, w^4k] of the second and third passes
+    __shared__ uint8_t s_mask[2][2][2][NTX];           // [buffer][plane A, B][own row, mirror row][tile]
+    const int npairs = ny / 2 + 1, U = 2 * npk * npairs;
+    const uint32_t span = (uint32_t)pk_span_bytes(N, ny, cs, sizeof(TI));
+    const uint32_t cb = (uint32_t)cs * (uint32_t)sizeof(TI);
+    const int j = threadIdx.x;
+    // unit L -> pack and row; the mirror unit of a self-mirrored row is empty
+    auto unit = [&](int L, int &pk, int &y) {
+        int bx, mir;
+        pk_block<2>(L, npairs, npk, bx, mir, pk);
+        y = mir ? ((ny - bx) & (ny - 1)) : bx;
+        return !(mir && y == bx);
+    };
+    auto next_unit = [&](int L, int &pk, int &y) {
+        while (L < U && !unit(L, pk, y)) L += gridDim.x;
+        return L;
+    };
+    // a unit's tile-mask bytes (thread j: own row for j < 128, mirror row
+    // after; tile j mod 128; planes A and B) and normalisers
+    auto prefetch = [&](int pk, int y, uint8_t (&m)[2], double &na, double &nb) {
+        const PackDesc &d = pd[pk];
+        const int f = j >> 7, t = j & (NTX - 1);
+        const int yy = f ? ((ny - y) & (ny - 1)) : y;
+        const int tr = ((yy + (ny >> 1)) & (ny - 1)) >> 5;
+        // through global-address-space pointers: a FLAT load would also hold
+        // up every LDS wait until it returns
+        typedef const __attribute__((address_space(1))) uint8_t *GU8;
+        typedef const __attribute__((address_space(1))) double *GF64;
+        const uint8_t *ma = (const uint8_t *)pk_uniform_ptr(d.ma), *mb = (const uint8_t *)pk_uniform_ptr(d.mb);
+        m[0] = ma ? ((GU8)ma)[tr * NTX + t] : 1;
+        m[1] = mb ? ((GU8)mb)[tr * NTX + t] : 1;
+        na = *(GF64)pk_uniform_ptr(d.na);
+        nb = d.b ? *(GF64)pk_uniform_ptr(d.nb) : 1.0;
+    };
+    auto stage = [&](const uint8_t (&m)[2], int buf) {
+        const int f = j >> 7, t = j & (NTX - 1);
+        s_mask[buf][0][f][t] = m[0];
+        s_mask[buf][1][f][t] = m[1];
+    };
+    // the reads of a unit need its planes and mask rows only
+    auto reader = [&](int pk, int y, int buf) {
+        const PackDesc &d = pd[pk];
+        const void *a = pk_uniform_ptr(d.a), *b = pk_uniform_ptr(d.b);
+        Load l;
+        l.rA = __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(a), (short)0, (int)span, PK_BUFFER_DWORD3);
+        l.rB = __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(b), (short)0, b ? (int)span : 0,
+                                                 PK_BUFFER_DWORD3);
+        l.mask = s_mask[buf];
+        l.bx = 0;
+        l.ny = ny;
+        l.y1 = y;
+        l.cb = cb;
+        return l;
+    };
+    auto coefs = [&](int pk, double na, double nb, Load &l) {
+        const PackDesc &d = pd[pk];
+        const double ia = 1.0 / pk_uniform_f64(na), ib = pk_uniform_ptr(d.b) ? 1.0 / pk_uniform_f64(nb) : 0.0;
+        pk_coefs(make_double2(d.ca[0].x * ia, d.ca[0].y * ia), make_double2(d.ca[1].x * ia, d.ca[1].y * ia), l.cA,
+                 l.cAm);
+        pk_coefs(make_double2(d.cb[0].x * ib, d.cb[0].y * ib), make_double2(d.cb[1].x * ib, d.cb[1].y * ib), l.cB,
+                 l.cBm);
+    };
+    int pk, y;
+    int L = next_unit(blockIdx.x, pk, y);
+    if (L >= U) return;
+    // twiddle bases: pass 2 (stride 16) w^k = tw[k NT / 256], pass 3 (stride
+    // 256) w^k = tw[k NT / 4096]; and w^4k alike
+    {
+        const int s2 = NT / 256, s3 = NT / 4096;
+        s_tw3[2 * j] = tw[j * s3];
+        s_tw3[2 * j + 1] = tw[4 * j * s3];
+        if (j < 16) {
+            s_tw2[2 * j] = tw[j * s2];
+            s_tw2[2 * j + 1] = tw[4 * j * s2];
+        }
+    }
+    int buf = 0;
+    uint8_t m[2];
+    double na, nb;
+    prefetch(pk, y, m, na, nb);
+    stage(m, buf);
+    __syncthreads();
+    // elements 0..AHEAD-1 of a unit are read during the previous unit's
+    // passes, the rest when its own first pass starts (more than 8 ahead
+    // does not fit two waves per SIMD without spilling)
+    constexpr int AHEAD = 8;
+    typename Load::Token tok[16];
+    {
+        const Load rd = reader(pk, y, buf);
+#pragma unroll
+        for (int r = 0; r < AHEAD; r++) tok[r] = rd.issue(0, j + 256 * r);
+    }
+    while (true) {
+        {
+            const Load rd = reader(pk, y, buf);
+#pragma unroll
+            for (int r = AHEAD; r < 16; r++) tok[r] = rd.issue(0, j + 256 * r);
+        }
+        int pn, yn;
+        const int Ln = next_unit(L + gridDim.x, pn, yn);
+        const bool more = Ln < U;
+        uint8_t mn[2];
+        double nan_ = 1.0, nbn = 1.0;
+        if (more) prefetch(pn, yn, mn, nan_, nbn);
+        // first pass: finish the elements, radix 16 (stride 1), into LDS
+        {
+            Load cur;
+            coefs(pk, na, nb, cur);
+            double2 v[16];
+#pragma unroll
+            for (int r = 0; r < 16; r++) v[r] = cur.finish(tok[r]);
+            idft<16>(v);
+#pragma unroll
+            for (int r = 0; r < 16; r++) lds[fb_pad(16 * j + r)] = v[r];
+        }
+        if (more) stage(mn, buf ^ 1);
+        __syncthreads();
+        XT *dst = X + (size_t)pk * N * ny + (size_t)y * N;
+        if (more) {
+            const Load rd = reader(pn, yn, buf ^ 1);
+#pragma unroll
+            for (int r = 0; r < AHEAD; r++) tok[r] = rd.issue(0, j + 256 * r);
+        }
+        pk_row_pass<16, false>(lds, s_tw2, [](int, double2) {});
+        pk_row_pass<256, true>(lds, s_tw3, [&](int kx, double2 v) { dst[kx] = x_st<XT>(v); });
+        if (!more) break;
+        L = Ln;
+        pk = pn;
+        y = yn;
+        na = nan_;
+        nb = nbn;
+        buf ^= 1;
+    }
 }
 
 // Pass B1: length-N1 transforms down the columns (rows N2*n1 + n2) of NF =
@@ -717,6 +953,18 @@ static mti_status pk_rows(mti_ctx *ctx, const FftGeom &g, const PackDesc *pd, in
     const int npairs = g.ny / 2 + 1;
     if (pk_span_bytes(g.nx, g.ny, cs, sizeof(TI)) > PK_OUTSIDE)
         return set_error(ctx, MTI_ERR_UNSUPPORTED, "grid plane spans more than 4 GiB");
+    if constexpr (std::is_same<TI, float2>::value) {
+        if (g.nx == 4096) {
+            // persistent, pipelined: two blocks per CU (LDS), a multiple of 32
+            // blocks so each keeps the XCD grouping of its units
+            const int64_t units = 2 * (int64_t)npk * npairs;
+            const int blocks = (int)std::min<int64_t>(units, (int64_t)2 * ((ctx->num_cus + 15) / 16) * 16);
+            hipLaunchKernelGGL((k_rows_pk4096_pipe<XT>), dim3(blocks), dim3(PK_TH), 0, st, pd, npk, X, g.ny, cs, tw,
+                               g.NT);
+            MTI_HIP(ctx, hipGetLastError());
+            return MTI_OK;
+        }
+    }
     switch (g.nx) {
 #define MTI_PR(NN)                                                                                          \
     case NN: {                                                                                              \
