@@ -652,191 +652,241 @@ __device__ __forceinline__ double pk_uniform_f64(double x)
     return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
-// v * w^r for r = 1..15 from w and w^4 (powers w^2, w^3, w^8, w^12 and
+// v[r] * w^r for r = 1..R-1 from w and w^4 (powers w^2, w^3, w^8, w^12 and
 // w^(4a + b) = w^(4a) w^b): the twiddles of a pass from two table values.
-__device__ __forceinline__ void pk_twiddle16(double2 (&v)[16], double2 w1, double2 w4)
+template <int R>
+__device__ __forceinline__ void pk_twiddle(double2 (&v)[R], double2 w1, double2 w4)
 {
-    const double2 w2 = c2mul(w1, w1), w3 = c2mul(w2, w1), w8 = c2mul(w4, w4), w12 = c2mul(w8, w4);
-    const double2 lo[4] = {make_double2(1.0, 0.0), w1, w2, w3}, hi[4] = {make_double2(1.0, 0.0), w4, w8, w12};
+    if constexpr (R > 1) {
+        const double2 one = make_double2(1.0, 0.0);
+        const double2 w2 = R > 2 ? c2mul(w1, w1) : one, w3 = R > 3 ? c2mul(w2, w1) : one;
+        const double2 w8 = R > 8 ? c2mul(w4, w4) : one, w12 = R > 8 ? c2mul(w8, w4) : one;
+        const double2 lo[4] = {one, w1, w2, w3}, hi[4] = {one, w4, w8, w12};
 #pragma unroll
-    for (int r = 1; r < 16; r++) {
-        const int a = r >> 2, b = r & 3;
-        const double2 w = a == 0 ? lo[b] : (b == 0 ? hi[a] : c2mul(hi[a], lo[b]));
-        v[r] = c2mul(v[r], w);
+        for (int r = 1; r < R; r++) {
+            const int a = r >> 2, b = r & 3;
+            const double2 w = a == 0 ? lo[b] : (b == 0 ? hi[a] : c2mul(hi[a], lo[b]));
+            v[r] = c2mul(v[r], w);
+        }
     }
 }
 
-// One radix-16 pass of a 4096-element row held in LDS (256 threads, stride
-// NS > 1), as fb_pass, with twiddles from the bases w^k, w^4k in LDS.
-template <int NS, bool LAST, class Store>
-__device__ __forceinline__ void pk_row_pass(double2 *lds, const double2 *tb, const Store &store)
+// One Stockham pass (radix R, stride NS > 1) over the E = 4096 elements of a
+// row-pass block in LDS, as fb_pass, with the twiddles of transform-local
+// index k from the bases tb[2k] = w^k, tb[2k + 1] = w^4k in LDS.
+template <int N, int R, int NS, bool LAST, class Store>
+__device__ __forceinline__ void pk_pipe_pass(double2 *lds, const double2 *tb, const Store &store)
 {
-    const int j = threadIdx.x, k = j % NS;
-    double2 v[16];
+    constexpr int PER = N / R, B = PK_E / (R * PK_TH);
+    double2 v[B][R];
 #pragma unroll
-    for (int r = 0; r < 16; r++) v[r] = lds[fb_pad(j + 256 * r)];
+    for (int s = 0; s < B; s++) {
+        const int b = threadIdx.x + PK_TH * s, f = b / PER, jj = b % PER;
+#pragma unroll
+        for (int r = 0; r < R; r++) v[s][r] = lds[fb_pad(f * N + jj + PER * r)];
+    }
     __syncthreads();
-    pk_twiddle16(v, tb[2 * k], tb[2 * k + 1]);
-    idft<16>(v);
-    const int base = (j - k) * 16 + k;
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
-        if (LAST) store(base + NS * r, v[r]);
-        else lds[fb_pad(base + NS * r)] = v[r];
+    for (int s = 0; s < B; s++) {
+        const int b = threadIdx.x + PK_TH * s, f = b / PER, jj = b % PER, k = jj % NS;
+        pk_twiddle<R>(v[s], tb[2 * k], tb[2 * k + 1]);
+        idft<R>(v[s]);
+        const int base = (jj - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            if (LAST) store(f, base + NS * r, v[s][r]);
+            else lds[fb_pad(f * N + base + NS * r)] = v[s][r];
+        }
     }
     if (!LAST) __syncthreads();
 }
 
-// Row pass of 4096-element rows from complex64 grids, software-pipelined:
-// a persistent block walks the row units L = blockIdx.x + k * gridDim.x (the
-// unit order of k_rows_pk, so the blocks that share grid lines still run
-// together on one XCD) and issues the grid reads of its next unit before the
-// second and third radix-16 passes of the current one, so the reads are in
-// flight during those passes instead of after them.  A global load waits for
-// every load issued before it, so no global load follows the grid reads
-// until they are used: the passes' twiddles come from bases w^k, w^4k staged
-// in LDS once per block (the others as products: a few ulp from the table
-// values), and the next unit's tile-mask rows and normalisers are read
-// during the current unit's first pass.
-template <typename XT>
-__global__ __launch_bounds__(PK_TH) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_rows_pk4096_pipe(
+// The passes after the first (strides 16 and 256 at most, rows <= 4096).
+template <int N, class Store>
+__device__ __forceinline__ void pk_pipe_rest(double2 *lds, const double2 *tb2, const double2 *tb3, const Store &store)
+{
+    constexpr int R2 = N / 16 >= 16 ? 16 : N / 16;
+    if constexpr (N / 16 == R2) {
+        pk_pipe_pass<N, R2, 16, true>(lds, tb2, store);
+    } else {
+        pk_pipe_pass<N, R2, 16, false>(lds, tb2, store);
+        pk_pipe_pass<N, N / 256, 256, true>(lds, tb3, store);
+    }
+}
+
+// Row pass from complex64 grids, software-pipelined: a persistent block
+// walks the units L = blockIdx.x + k * gridDim.x of k_rows_pk (a row pair,
+// or at N = 4096 one row; the same unit order, so the blocks that share grid
+// lines still run together on one XCD) and issues the grid reads of the
+// first AHEAD elements of its next unit before the later passes of the
+// current one, so those reads are in flight during the passes instead of
+// after them.  A global load waits for every load issued before it, so no
+// global load follows the early reads until they are used: the passes'
+// twiddles come from bases w^k, w^4k staged in LDS once per block (the
+// others as products, a few ulp from the table values), and the next unit's
+// tile-mask rows and normalisers are read during the current unit's first
+// pass.
+template <int N, typename XT>
+__global__ __launch_bounds__(PK_TH) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_rows_pk_pipe(
     const PackDesc *__restrict__ pd, int npk, XT *X, int ny, int64_t cs, const double2 *tw, int NT)
 {
     using TI = float2;
-    using G = PkGeom<4096>;
-    using Load = PkRowLoad<4096, TI>;
-    constexpr int N = 4096, NTX = G::NTX;
-    static_assert(G::ROW && PK_TH == 256 && PK_E == N && NTX == 128, "row geometry");
-    __shared__ double2 lds[N + N / 16];
-    __shared__ double2 s_tw2[2 * 16], s_tw3[2 * 256];  // [k][w^k, w^4k] of the second and third passes
-    __shared__ uint8_t s_mask[2][2][2][NTX];           // [buffer][plane A, B][own row, mirror row][tile]
-    const int npairs = ny / 2 + 1, U = 2 * npk * npairs;
+    using G = PkGeom<N>;
+    using Load = PkRowLoad<N, TI>;
+    constexpr int E = PK_E, NTX = G::NTX, PER = N / 16, NQ = 2 * G::MR * NTX / PK_TH;
+    static_assert(N >= 256 && PK_TH == 256 && E / (16 * PK_TH) == 1 && NQ >= 1 && NQ <= 2, "row geometry");
+    __shared__ double2 lds[E + E / 16];
+    __shared__ double2 s_tw2[2 * 16], s_tw3[2 * 256];   // [k][w^k, w^4k] of the strides 16 and 256
+    __shared__ uint8_t s_mask[2][2][G::MR][NTX];         // [buffer][plane A, B][row][tile]
+    const int npairs = ny / 2 + 1, nbx = (npairs + G::PB - 1) / G::PB, U = G::M * npk * nbx;
     const uint32_t span = (uint32_t)pk_span_bytes(N, ny, cs, sizeof(TI));
     const uint32_t cb = (uint32_t)cs * (uint32_t)sizeof(TI);
     const int j = threadIdx.x;
-    // unit L -> pack and row; the mirror unit of a self-mirrored row is empty
-    auto unit = [&](int L, int &pk, int &y) {
-        int bx, mir;
-        pk_block<2>(L, npairs, npk, bx, mir, pk);
-        y = mir ? ((ny - bx) & (ny - 1)) : bx;
-        return !(mir && y == bx);
+    // unit L -> pack, row block and (one-row blocks) row; the mirror unit of
+    // a self-mirrored row is empty
+    struct Unit {
+        int pk, bx, y1;
     };
-    auto next_unit = [&](int L, int &pk, int &y) {
-        while (L < U && !unit(L, pk, y)) L += gridDim.x;
+    auto unit = [&](int L, Unit &u) {
+        int mir;
+        pk_block<G::M>(L, nbx, npk, u.bx, mir, u.pk);
+        u.y1 = mir ? ((ny - u.bx) & (ny - 1)) : u.bx;
+        return !(G::ROW && mir && u.y1 == u.bx);
+    };
+    auto next_unit = [&](int L, Unit &u) {
+        while (L < U && !unit(L, u)) L += gridDim.x;
         return L;
     };
-    // a unit's tile-mask bytes (thread j: own row for j < 128, mirror row
-    // after; tile j mod 128; planes A and B) and normalisers
-    auto prefetch = [&](int pk, int y, uint8_t (&m)[2], double &na, double &nb) {
-        const PackDesc &d = pd[pk];
-        const int f = j >> 7, t = j & (NTX - 1);
-        const int yy = f ? ((ny - y) & (ny - 1)) : y;
-        const int tr = ((yy + (ny >> 1)) & (ny - 1)) >> 5;
-        // through global-address-space pointers: a FLAT load would also hold
-        // up every LDS wait until it returns
-        typedef const __attribute__((address_space(1))) uint8_t *GU8;
-        typedef const __attribute__((address_space(1))) double *GF64;
+    // transform row of block row f
+    auto row_of = [&](const Unit &u, int f) {
+        if (G::ROW) return f ? ((ny - u.y1) & (ny - 1)) : u.y1;
+        const int pair = u.bx * G::PB + (f >> 1);
+        return (f & 1) ? ((ny - pair) & (ny - 1)) : pair;
+    };
+    // a unit's tile-mask bytes (entries j, j + 256 of [plane][row][tile]) and
+    // normalisers, through global-address-space pointers: a FLAT load would
+    // also hold up every LDS wait until it returns
+    typedef const __attribute__((address_space(1))) uint8_t *GU8;
+    typedef const __attribute__((address_space(1))) double *GF64;
+    auto prefetch = [&](const Unit &u, uint8_t (&m)[NQ], double &na, double &nb) {
+        const PackDesc &d = pd[u.pk];
         const uint8_t *ma = (const uint8_t *)pk_uniform_ptr(d.ma), *mb = (const uint8_t *)pk_uniform_ptr(d.mb);
-        m[0] = ma ? ((GU8)ma)[tr * NTX + t] : 1;
-        m[1] = mb ? ((GU8)mb)[tr * NTX + t] : 1;
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            const int i = j + PK_TH * q, ab = i / (G::MR * NTX), f = (i / NTX) % G::MR, t = i % NTX;
+            const uint8_t *mk = ab ? mb : ma;
+            const bool valid = G::ROW || u.bx * G::PB + (f >> 1) < npairs;  // rows past the last pair: zeros
+            const int y = row_of(u, f);
+            m[q] = !valid ? 0 : (mk ? ((GU8)mk)[(((y + (ny >> 1)) & (ny - 1)) >> 5) * NTX + t] : 1);
+        }
         na = *(GF64)pk_uniform_ptr(d.na);
         nb = d.b ? *(GF64)pk_uniform_ptr(d.nb) : 1.0;
     };
-    auto stage = [&](const uint8_t (&m)[2], int buf) {
-        const int f = j >> 7, t = j & (NTX - 1);
-        s_mask[buf][0][f][t] = m[0];
-        s_mask[buf][1][f][t] = m[1];
+    auto stage = [&](const uint8_t (&m)[NQ], int buf) {
+#pragma unroll
+        for (int q = 0; q < NQ; q++) (&s_mask[buf][0][0][0])[j + PK_TH * q] = m[q];
     };
     // the reads of a unit need its planes and mask rows only
-    auto reader = [&](int pk, int y, int buf) {
-        const PackDesc &d = pd[pk];
+    auto reader = [&](const Unit &u, int buf) {
+        const PackDesc &d = pd[u.pk];
         const void *a = pk_uniform_ptr(d.a), *b = pk_uniform_ptr(d.b);
         Load l;
         l.rA = __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(a), (short)0, (int)span, PK_BUFFER_DWORD3);
         l.rB = __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(b), (short)0, b ? (int)span : 0,
                                                  PK_BUFFER_DWORD3);
         l.mask = s_mask[buf];
-        l.bx = 0;
+        l.bx = u.bx;
         l.ny = ny;
-        l.y1 = y;
+        l.y1 = u.y1;
         l.cb = cb;
         return l;
     };
-    auto coefs = [&](int pk, double na, double nb, Load &l) {
-        const PackDesc &d = pd[pk];
+    auto coefs = [&](const Unit &u, double na, double nb, Load &l) {
+        const PackDesc &d = pd[u.pk];
         const double ia = 1.0 / pk_uniform_f64(na), ib = pk_uniform_ptr(d.b) ? 1.0 / pk_uniform_f64(nb) : 0.0;
         pk_coefs(make_double2(d.ca[0].x * ia, d.ca[0].y * ia), make_double2(d.ca[1].x * ia, d.ca[1].y * ia), l.cA,
                  l.cAm);
         pk_coefs(make_double2(d.cb[0].x * ib, d.cb[0].y * ib), make_double2(d.cb[1].x * ib, d.cb[1].y * ib), l.cB,
                  l.cBm);
     };
-    int pk, y;
-    int L = next_unit(blockIdx.x, pk, y);
+    Unit u;
+    int L = next_unit(blockIdx.x, u);
     if (L >= U) return;
-    // twiddle bases: pass 2 (stride 16) w^k = tw[k NT / 256], pass 3 (stride
-    // 256) w^k = tw[k NT / 4096]; and w^4k alike
+    // twiddle bases of the passes of stride 16 (radix R2) and 256 (radix
+    // N / 256): fb_pass's twiddle tw[r k step] with step = NT / (NS R)
     {
-        const int s2 = NT / 256, s3 = NT / 4096;
-        s_tw3[2 * j] = tw[j * s3];
-        s_tw3[2 * j + 1] = tw[4 * j * s3];
+        constexpr int R2 = N / 16 >= 16 ? 16 : N / 16;
+        const int s3 = N > 256 ? NT / N : 0, s2 = NT / (16 * R2);
+        s_tw3[2 * j] = tw[(j * s3) & (NT - 1)];
+        s_tw3[2 * j + 1] = tw[(4 * j * s3) & (NT - 1)];
         if (j < 16) {
-            s_tw2[2 * j] = tw[j * s2];
-            s_tw2[2 * j + 1] = tw[4 * j * s2];
+            s_tw2[2 * j] = tw[(j * s2) & (NT - 1)];
+            s_tw2[2 * j + 1] = tw[(4 * j * s2) & (NT - 1)];
         }
     }
+    // elements 0..AHEAD-1 of a unit are read during the previous unit's
+    // passes, the rest when its own first pass starts (more ahead does not
+    // fit two waves per SIMD without spilling: the radix-8 and radix-4 last
+    // passes of 2048 and 1024 hold more twiddles)
+    constexpr int AHEAD = (N == 1024 || N == 2048) ? 4 : 8;
+    const int f0 = j / PER, jj = j % PER;  // first pass: transform row f0, elements jj + PER r
     int buf = 0;
-    uint8_t m[2];
+    uint8_t m[NQ];
     double na, nb;
-    prefetch(pk, y, m, na, nb);
+    prefetch(u, m, na, nb);
     stage(m, buf);
     __syncthreads();
-    // elements 0..AHEAD-1 of a unit are read during the previous unit's
-    // passes, the rest when its own first pass starts (more than 8 ahead
-    // does not fit two waves per SIMD without spilling)
-    constexpr int AHEAD = 8;
     typename Load::Token tok[16];
     {
-        const Load rd = reader(pk, y, buf);
+        const Load rd = reader(u, buf);
 #pragma unroll
-        for (int r = 0; r < AHEAD; r++) tok[r] = rd.issue(0, j + 256 * r);
+        for (int r = 0; r < AHEAD; r++) tok[r] = rd.issue(f0, jj + PER * r);
     }
     while (true) {
         {
-            const Load rd = reader(pk, y, buf);
+            const Load rd = reader(u, buf);
 #pragma unroll
-            for (int r = AHEAD; r < 16; r++) tok[r] = rd.issue(0, j + 256 * r);
+            for (int r = AHEAD; r < 16; r++) tok[r] = rd.issue(f0, jj + PER * r);
         }
-        int pn, yn;
-        const int Ln = next_unit(L + gridDim.x, pn, yn);
+        Unit un;
+        const int Ln = next_unit(L + gridDim.x, un);
         const bool more = Ln < U;
-        uint8_t mn[2];
+        uint8_t mn[NQ];
         double nan_ = 1.0, nbn = 1.0;
-        if (more) prefetch(pn, yn, mn, nan_, nbn);
+        if (more) prefetch(un, mn, nan_, nbn);
         // first pass: finish the elements, radix 16 (stride 1), into LDS
         {
             Load cur;
-            coefs(pk, na, nb, cur);
+            coefs(u, na, nb, cur);
             double2 v[16];
 #pragma unroll
             for (int r = 0; r < 16; r++) v[r] = cur.finish(tok[r]);
             idft<16>(v);
 #pragma unroll
-            for (int r = 0; r < 16; r++) lds[fb_pad(16 * j + r)] = v[r];
+            for (int r = 0; r < 16; r++) lds[fb_pad(f0 * N + 16 * jj + r)] = v[r];
         }
         if (more) stage(mn, buf ^ 1);
         __syncthreads();
-        XT *dst = X + (size_t)pk * N * ny + (size_t)y * N;
         if (more) {
-            const Load rd = reader(pn, yn, buf ^ 1);
+            const Load rd = reader(un, buf ^ 1);
 #pragma unroll
-            for (int r = 0; r < AHEAD; r++) tok[r] = rd.issue(0, j + 256 * r);
+            for (int r = 0; r < AHEAD; r++) tok[r] = rd.issue(f0, jj + PER * r);
         }
-        pk_row_pass<16, false>(lds, s_tw2, [](int, double2) {});
-        pk_row_pass<256, true>(lds, s_tw3, [&](int kx, double2 v) { dst[kx] = x_st<XT>(v); });
+        XT *dst = X + (size_t)u.pk * N * ny;
+        pk_pipe_rest<N>(lds, s_tw2, s_tw3, [&](int f, int kx, double2 v) {
+            if (G::ROW) {
+                dst[(size_t)u.y1 * N + kx] = x_st<XT>(v);
+                return;
+            }
+            const int pair = u.bx * G::PB + (f >> 1);
+            if (pair >= npairs) return;
+            const int y = row_of(u, f);
+            if ((f & 1) && y == pair) return;  // a self-paired row is written once
+            dst[(size_t)y * N + kx] = x_st<XT>(v);
+        });
         if (!more) break;
         L = Ln;
-        pk = pn;
-        y = yn;
+        u = un;
         na = nan_;
         nb = nbn;
         buf ^= 1;
@@ -954,15 +1004,23 @@ static mti_status pk_rows(mti_ctx *ctx, const FftGeom &g, const PackDesc *pd, in
     if (pk_span_bytes(g.nx, g.ny, cs, sizeof(TI)) > PK_OUTSIDE)
         return set_error(ctx, MTI_ERR_UNSUPPORTED, "grid plane spans more than 4 GiB");
     if constexpr (std::is_same<TI, float2>::value) {
-        if (g.nx == 4096) {
-            // persistent, pipelined: two blocks per CU (LDS), a multiple of 32
-            // blocks so each keeps the XCD grouping of its units
-            const int64_t units = 2 * (int64_t)npk * npairs;
-            const int blocks = (int)std::min<int64_t>(units, (int64_t)2 * ((ctx->num_cus + 15) / 16) * 16);
-            hipLaunchKernelGGL((k_rows_pk4096_pipe<XT>), dim3(blocks), dim3(PK_TH), 0, st, pd, npk, X, g.ny, cs, tw,
-                               g.NT);
-            MTI_HIP(ctx, hipGetLastError());
-            return MTI_OK;
+        // complex64 grids, rows of 256 or more: persistent and pipelined, two
+        // blocks per CU (LDS), a multiple of 32 blocks so each keeps the XCD
+        // grouping of its units
+        const int64_t cap = (int64_t)2 * ((ctx->num_cus + 15) / 16) * 16;
+        switch (g.nx) {
+#define MTI_PP(NN)                                                                                                \
+    case NN: {                                                                                                    \
+        using G = PkGeom<NN>;                                                                                     \
+        const int64_t units = (int64_t)G::M * npk * ceil_div(npairs, G::PB);                                      \
+        hipLaunchKernelGGL((k_rows_pk_pipe<NN, XT>), dim3((unsigned)std::min(units, cap)), dim3(PK_TH), 0, st, pd, \
+                           npk, X, g.ny, cs, tw, g.NT);                                                           \
+        MTI_HIP(ctx, hipGetLastError());                                                                          \
+        return MTI_OK;                                                                                            \
+    }
+            MTI_PP(256) MTI_PP(512) MTI_PP(1024) MTI_PP(2048) MTI_PP(4096)
+#undef MTI_PP
+        default: break;
         }
     }
     switch (g.nx) {
