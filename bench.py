@@ -921,7 +921,11 @@ def main():
         if not args.no_c4:
             r4 = run_c4(args, torch, mti, pool)
             if r4.get("autotune"):
-                r4["autotune"] = dict(r4["autotune"], candidates=len(r4["autotune"].pop("candidates_ms")))
+                cms = r4["autotune"].pop("candidates_ms")
+                # the region shapes the coordinate search timed (others at their defaults)
+                regions = {c["region_width"]: c["ms"] for c in cms
+                           if all(c[k] == v for k, v in cms[0].items() if k not in ("region_width", "ms"))}
+                r4["autotune"] = dict(r4["autotune"], candidates=len(cms), region_width_ms=regions)
             nested["c4"] = r4
         if not args.no_c5:
             a5 = argparse.Namespace(**vars(args))

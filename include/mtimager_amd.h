@@ -484,7 +484,8 @@ void       mti_graph_destroy(mti_graph *g);
  * blocks at <= 440 / <= 110 records (op/common.hcu:26-27, used by
  * NotSameSupportManipulate, op/NotSameSupportManipulate.cu:32-97); results do
  * not depend on these beyond FP64 summation order.  The tile itself is fixed
- * at 32 x 32 cells (one lane per cell of a 1024-lane workgroup). */
+ * at 32 x 32 cells (one lane per cell of a 1024-lane workgroup); how the 16
+ * waves split it into regions is tuned. */
 typedef struct {
     int32_t unit_pairs;       /* (record, tile) pairs per gridder work unit:
                                  power of two in [256, 16384]; 0 = 2048      */
@@ -498,6 +499,9 @@ typedef struct {
                                  XCD's L2 serves their kernel samples); 0 =
                                  plain round-robin, else a power of two in
                                  [1, 4096]; default 0                         */
+    int32_t region_width;     /* gridder: each wave of a tile's workgroup owns
+                                 a region of region_width x (64 / region_width)
+                                 cells: 4, 8 or 16; 0 = 8 (appended field)   */
 } mti_tuning;
 mti_status mti_set_tuning(mti_ctx *ctx, const mti_tuning *t);
 mti_status mti_get_tuning(const mti_ctx *ctx, mti_tuning *t);

@@ -63,9 +63,12 @@ extern "C" mti_status mti_set_tuning(mti_ctx *ctx, const mti_tuning *t)
     const int x = t->xcd_chunk;
     MTI_CHECK(ctx, x == 0 || (x >= 1 && x <= 4096 && (x & (x - 1)) == 0),
               "xcd_chunk must be 0 or a power of two in [1, 4096]");
+    const int r = t->region_width ? t->region_width : 8;
+    MTI_CHECK(ctx, r == 4 || r == 8 || r == 16, "region_width must be 0, 4, 8 or 16");
     ctx->unit_pairs = u;
     ctx->whole_columns_ny = w;
     ctx->xcd_chunk = x;
+    ctx->region_width = r;
     return MTI_OK;
 }
 
@@ -75,6 +78,7 @@ extern "C" mti_status mti_get_tuning(const mti_ctx *ctx, mti_tuning *t)
     t->unit_pairs = ctx->unit_pairs;
     t->whole_columns_ny = ctx->whole_columns_ny;
     t->xcd_chunk = ctx->xcd_chunk;
+    t->region_width = ctx->region_width;
     return MTI_OK;
 }
 

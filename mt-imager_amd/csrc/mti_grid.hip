@@ -49,12 +49,17 @@ constexpr int TG_T = 32;             // tile side in cells
 constexpr int TG_TT = TG_T * TG_T;
 constexpr int TG_THREADS = 1024;     // 16 waves, one 8x8 region each
 constexpr int TG_REGIONS = 16;
-// A wave's region: TG_RW x TG_RH cells (64 lanes).  16 x 4 regions read each
+// A wave's region: RW x (64 / RW) cells (64 lanes), RW = 4, 8 or 16 (the
+// context's tuning, mti_tuning.region_width).  Wide regions read each
 // support-box row of kernel samples as a longer contiguous run, but a box
-// touches more of them: measured slower than 8 x 8 (the 4096^2 full-polarisation
-// cube's gridder 147.5 ms against 141).
-constexpr int TG_RW = 8, TG_RH = 8;
-static_assert(TG_RW * TG_RH == 64 && (TG_T / TG_RW) * (TG_T / TG_RH) == TG_REGIONS, "region shape");
+// touches more of them: 16 x 4 measured slower than 8 x 8 on the 4096^2
+// full-polarisation cube's gridder (147.5 ms against 141), which is why 8 x 8
+// is the default and the others are left to the tuner per workload.
+template <int RW>
+struct Region {
+    static constexpr int W = RW, H = 64 / RW, X = TG_T / RW;  // cells; regions along a tile row
+    static_assert((RW == 4 || RW == 8 || RW == 16) && X * (TG_T / H) == TG_REGIONS, "region shape");
+};
 constexpr int TG_BATCH = 256;        // records staged per round
 static_assert(TG_T == 32 && TG_THREADS == 64 * TG_REGIONS && TG_BATCH <= 256, "region layout");
 constexpr int TG_CHUNK = 2048;       // default (record, tile) pairs per work unit
@@ -507,10 +512,11 @@ __device__ __forceinline__ void flush_cell(R *g, int64_t ps, const double (&acc)
 // FP32 (sum, compensation) pair per cell component; the cell's value
 // sum - compensation is then added to the grid.  Visibilities are staged as
 // FP32 and samples stay FP32 (no conversions).
-template <int P, int PREC, int ACC>
+template <int P, int PREC, int ACC, int RW>
 __global__ __launch_bounds__(TG_THREADS, PREC == MTI_TABLE_F32 || ACC == MTI_ACC_F32_KAHAN ? 8 : 1) void
 k_grid_tiles(GridArgs a)
 {
+    using RG = Region<RW>;
     constexpr bool F32 = ACC == MTI_ACC_F32_KAHAN;
     constexpr int VB = F32 ? (int)sizeof(float) : (int)sizeof(double);  // bytes per staged component
     __shared__ __align__(16) double s_vd[TG_BATCH][2 * P];  // weighted visibility (FP64, or FP32 packed)
@@ -518,8 +524,7 @@ k_grid_tiles(GridArgs a)
     __shared__ int s_cnt[TG_REGIONS];
     static_assert(TG_BATCH * 2 * P * sizeof(double) < 65536, "visibility offsets fit 16 bits");
     const int tid = threadIdx.x, lane = tid & 63, wave = wave_uniform(tid >> 6);
-    constexpr int RX = TG_T / TG_RW;  // regions along a tile row
-    const int rx = (wave % RX) * TG_RW + (lane % TG_RW), ry = (wave / RX) * TG_RH + (lane / TG_RW);  // cell in the tile
+    const int rx = (wave % RG::X) * RG::W + (lane % RG::W), ry = (wave / RG::X) * RG::H + (lane / RG::W);  // cell in the tile
     const uint32_t NU = *a.n_units;
     // XCD-aware order: runs of 2^lg consecutive units (neighbouring tiles of
     // one channel, whose records share w-planes) are dealt to the XCDs in turn
@@ -580,9 +585,9 @@ k_grid_tiles(GridArgs a)
                         e.s_vo = S | ((rr * 2 * P * VB) << 16);
                         const int ix0 = max(cx, x0) - x0, ix1 = min(cx + S, x0 + TG_T) - x0;
                         const int iy0 = max(cy, y0) - y0, iy1 = min(cy + S, y0 + TG_T) - y0;
-                        for (int gyr = iy0 / TG_RH; gyr <= (iy1 - 1) / TG_RH; gyr++)
-                            for (int gxr = ix0 / TG_RW; gxr <= (ix1 - 1) / TG_RW; gxr++) {
-                                const int reg = gyr * (TG_T / TG_RW) + gxr;
+                        for (int gyr = iy0 / RG::H; gyr <= (iy1 - 1) / RG::H; gyr++)
+                            for (int gxr = ix0 / RG::W; gxr <= (ix1 - 1) / RG::W; gxr++) {
+                                const int reg = gyr * RG::X + gxr;
                                 const int pos = atomicAdd(&s_cnt[reg], 1);
                                 s_list[reg][pos] = e;
                             }
@@ -759,13 +764,21 @@ static int persistent_blocks(mti_ctx *ctx, K kern, int threads)
     return ctx->num_cus * per_cu;
 }
 
-template <int P, int PREC, int ACC>
-static mti_status launch_tiles(mti_ctx *ctx, const GridArgs &a, hipStream_t st)
+template <int P, int PREC, int ACC, int RW>
+static mti_status launch_tiles_rw(mti_ctx *ctx, const GridArgs &a, hipStream_t st)
 {
-    auto kern = k_grid_tiles<P, PREC, ACC>;
+    auto kern = k_grid_tiles<P, PREC, ACC, RW>;
     hipLaunchKernelGGL(kern, dim3(persistent_blocks(ctx, kern, TG_THREADS)), dim3(TG_THREADS), 0, st, a);
     MTI_HIP(ctx, hipGetLastError());
     return MTI_OK;
+}
+
+template <int P, int PREC, int ACC>
+static mti_status launch_tiles(mti_ctx *ctx, const GridArgs &a, hipStream_t st)
+{
+    if (ctx->region_width == 4) return launch_tiles_rw<P, PREC, ACC, 4>(ctx, a, st);
+    if (ctx->region_width == 16) return launch_tiles_rw<P, PREC, ACC, 16>(ctx, a, st);
+    return launch_tiles_rw<P, PREC, ACC, 8>(ctx, a, st);
 }
 
 template <int P, int ACC>

@@ -102,6 +102,7 @@ struct mti_ctx {
     int unit_pairs = 2048;       // (record, tile) pairs per gridder work unit
     int whole_columns_ny = 256;  // largest ny transformed by whole columns when the split also applies
     int xcd_chunk = 0;           // gridder units dealt to each XCD in runs of this many (0: round-robin)
+    int region_width = 8;        // gridder wave regions of region_width x (64 / region_width) cells
     // packed planes between the transform passes stored as complex64 (1) or
     // complex128 (0) (mti_set_transform_storage)
     int xform_c64 = 0;

@@ -107,7 +107,8 @@ class PreprocessRefOutputs(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("unit_pairs", ctypes.c_int32), ("whole_columns_ny", ctypes.c_int32), ("xcd_chunk", ctypes.c_int32)]
+    _fields_ = [("unit_pairs", ctypes.c_int32), ("whole_columns_ny", ctypes.c_int32), ("xcd_chunk", ctypes.c_int32),
+                ("region_width", ctypes.c_int32)]
 
 
 # Every symbol include/mtimager_amd.h declares.
@@ -377,21 +378,23 @@ class Context:
             self.check(lib().mti_set_workspace(self.h, buf.data_ptr(), nbytes), "mti_set_workspace")
         self._workspace = buf
 
-    def set_tuning(self, unit_pairs=None, whole_columns_ny=None, xcd_chunk=None):
+    def set_tuning(self, unit_pairs=None, whole_columns_ny=None, xcd_chunk=None, region_width=None):
         """Launch-shape tuning of this context (see mtimager_amd.autotune);
         arguments left None keep their current value."""
         t = self.get_tuning()
-        for k, v in (("unit_pairs", unit_pairs), ("whole_columns_ny", whole_columns_ny), ("xcd_chunk", xcd_chunk)):
+        for k, v in (("unit_pairs", unit_pairs), ("whole_columns_ny", whole_columns_ny), ("xcd_chunk", xcd_chunk),
+                     ("region_width", region_width)):
             if v is not None:
                 t[k] = int(v)
         self.check(lib().mti_set_tuning(self.h, ctypes.byref(Tuning(t["unit_pairs"], t["whole_columns_ny"],
-                                                                    t["xcd_chunk"]))), "mti_set_tuning")
+                                                                    t["xcd_chunk"], t["region_width"]))),
+                   "mti_set_tuning")
 
     def get_tuning(self):
         t = Tuning()
         self.check(lib().mti_get_tuning(self.h, ctypes.byref(t)), "mti_get_tuning")
         return {"unit_pairs": int(t.unit_pairs), "whole_columns_ny": int(t.whole_columns_ny),
-                "xcd_chunk": int(t.xcd_chunk)}
+                "xcd_chunk": int(t.xcd_chunk), "region_width": int(t.region_width)}
 
     def set_transform_storage(self, precision):
         """Element type of the packed planes between the finalize's transform

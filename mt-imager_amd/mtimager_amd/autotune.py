@@ -1,9 +1,12 @@
 """Autotuner for a context's launch-shape choices (mti_tuning): the gridder's
-work-unit size (how many (record, tile) pairs one workgroup takes, which sets
-how a crowded tile splits over workgroups and how many tiles need atomic
-flushes), how work units are dealt to the 8 XCDs (runs of neighbouring tiles
-per XCD, for L2 reuse of kernel samples) and the transform's column strategy
-where two apply.
+region shape (each wave of a tile's workgroup owns 4 x 16, 8 x 8 or 16 x 4
+cells -- a kernel variant each: wider regions read longer runs of a sample
+row, narrower ones waste fewer lanes on small support boxes), its work-unit
+size (how many (record, tile) pairs one workgroup takes, which sets how a
+crowded tile splits over workgroups and how many tiles need atomic flushes),
+how work units are dealt to the 8 XCDs (runs of neighbouring tiles per XCD,
+for L2 reuse of kernel samples) and the transform's column strategy where two
+apply.
 
 Each candidate is set on the context, run once untimed, then timed `repeats`
 times on the caller's representative workload; the fastest stays set.  The
@@ -21,11 +24,12 @@ import time
 
 # the knobs of mti_tuning and the values searched by default
 DEFAULT_SPACE = {
+    "region_width": [8, 16, 4],
     "xcd_chunk": [0, 8, 32, 128],
     "unit_pairs": [512, 1024, 2048, 4096, 8192],
     "whole_columns_ny": [0, 256],
 }
-DEFAULTS = {"unit_pairs": 2048, "whole_columns_ny": 256, "xcd_chunk": 0}
+DEFAULTS = {"unit_pairs": 2048, "whole_columns_ny": 256, "xcd_chunk": 0, "region_width": 8}
 
 
 def grid_candidates(space):

@@ -80,7 +80,8 @@ def test_cache_round_trip(tmp_path):
 
 
 def test_default_space_covers_every_knob():
-    assert set(autotune.DEFAULT_SPACE) == set(autotune.DEFAULTS) == {"unit_pairs", "whole_columns_ny", "xcd_chunk"}
+    assert set(autotune.DEFAULT_SPACE) == set(autotune.DEFAULTS) == {"unit_pairs", "whole_columns_ny", "xcd_chunk",
+                                                                     "region_width"}
     assert all(autotune.DEFAULTS[k] in v for k, v in autotune.DEFAULT_SPACE.items())
 
 
