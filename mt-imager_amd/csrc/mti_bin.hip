@@ -119,7 +119,6 @@ __global__ __launch_bounds__(BT_THREADS) void k_bin_pass(BinPassArgs a)
                 const int64_t e = jd.rec0 + (int64_t)cl * v.nidx + i;
                 int4 d = make_int4(0, 0, 0, 0);
                 if (lane < nr) {
-                    d.x = t.row[lane];
                     bool any = false;
 #pragma unroll
                     for (int p = 0; p < P; p++) any |= !t.flag[c][lane][p];
@@ -127,6 +126,7 @@ __global__ __launch_bounds__(BT_THREADS) void k_bin_pass(BinPassArgs a)
                         const BinOut b = bin_core(t.uvw[lane][0], t.uvw[lane][1], t.uvw[lane][2],
                                                   load_invl(v, (int64_t)t.row[lane], jd.chan0 + cl), jd.g);
                         if (b.to_grid) {
+                            d.x = (b.plane * jd.g.T + b.oy) * jd.g.T + b.ox;  // sub-kernel: sums index
                             d.y = (b.cy << 16) | b.cx;
                             d.z = b.conv_idx;
                             d.w = (b.plane << 16) | (b.conj << 15) | b.S;

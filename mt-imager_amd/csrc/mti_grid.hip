@@ -68,7 +68,9 @@ constexpr int TG_CHUNK_MIN = 256;    // smallest unit a context may be tuned to
 // Arrays of a gridding pass shared by the kernels after the binning pass.
 struct GridArgs {
     JobSet js;
-    const int4 *cdesc;          // record descriptors (support 0: not gridded)
+    const int4 *cdesc;          // record descriptors: x sub-kernel (plane T^2 + oy T + ox, the sums
+                                // index), y corner (cy << 16 | cx), z kernel sample offset, w plane << 16 |
+                                // conj << 15 | support (support 0: not gridded)
     const int32_t *cjc;         // job << 16 | local channel of each record
     const uint32_t *n_rec;      // device scalar: records
     float2 *wvis;               // [M][P] visibilities, weighted in place by k_record_prep
@@ -292,18 +294,63 @@ __global__ __launch_bounds__(256) void k_density_keys(GridArgs a, const uint32_t
         const int x0 = ta.tx * TG_T, y0 = ta.ty * TG_T;
         for (int i = tid; i < P * TG_TT; i += 256) s_d[i] = 0.0f;
         __syncthreads();
-        for (uint32_t i = p0 + tid; i < p1; i += 256) {
+        // pair i of the tile -> its record and the record's nearest cell in the
+        // tile (-1: not the record's home tile, which alone gathers it)
+        auto home_cell = [&](uint32_t i, uint32_t &j) {
             const uint32_t pj = a.pairs[i];
-            if (!(pj & PAIR_HOME)) continue;  // only the record's home tile gathers it
-            const uint32_t j = pj & PAIR_INDEX;
+            j = pj & PAIR_INDEX;
+            if (!(pj & PAIR_HOME)) return -1;
             const int4 d = a.cdesc[j];
             const int S = d.w & 0x7fff;
             const int mx = (d.y & 0xffff) + S / 2 - x0, my = (int)((unsigned)d.y >> 16) + S / 2 - y0;
+            return my * TG_T + mx;
+        };
+        auto gather = [&](uint32_t j, int cell) {
 #pragma unroll
             for (int p = 0; p < P; p++) {
                 const float w = a.wt[(int64_t)j * P + p];
-                if (w != 0.0f) atomicAdd(&s_d[(my * TG_T + mx) * P + p], w);
+                if (w != 0.0f) atomicAdd(&s_d[cell * P + p], w);
             }
+        };
+        // the first DK pairs of each thread keep (record, cell) in registers
+        // for the write-back; pairs past DK * 256 are read again.  Their reads
+        // go in phases (pairs, then the home records' descriptors, then their
+        // weights), each phase's DK reads in flight together
+        constexpr int DK = 4;
+        uint32_t kj[DK];
+        int kc[DK];
+        int4 kd[DK];
+        float kw[DK][P];
+#pragma unroll
+        for (int k = 0; k < DK; k++) {
+            const uint32_t i = p0 + tid + 256u * k;
+            const uint32_t pj = i < p1 ? a.pairs[i] : 0u;  // past the tile: no home bit
+            kj[k] = pj & PAIR_INDEX;
+            kc[k] = (pj & PAIR_HOME) ? 0 : -1;
+        }
+#pragma unroll
+        for (int k = 0; k < DK; k++) kd[k] = kc[k] >= 0 ? a.cdesc[kj[k]] : make_int4(0, 0, 0, 0);
+#pragma unroll
+        for (int k = 0; k < DK; k++) {
+#pragma unroll
+            for (int p = 0; p < P; p++) kw[k][p] = kc[k] >= 0 ? a.wt[(int64_t)kj[k] * P + p] : 0.0f;
+            if (kc[k] >= 0) {
+                const int S = kd[k].w & 0x7fff;
+                const int mx = (kd[k].y & 0xffff) + S / 2 - x0, my = (int)((unsigned)kd[k].y >> 16) + S / 2 - y0;
+                kc[k] = my * TG_T + mx;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < DK; k++)
+            if (kc[k] >= 0) {
+#pragma unroll
+                for (int p = 0; p < P; p++)
+                    if (kw[k][p] != 0.0f) atomicAdd(&s_d[kc[k] * P + p], kw[k][p]);
+            }
+        for (uint32_t i = p0 + tid + 256u * DK; i < p1; i += 256) {
+            uint32_t j;
+            const int cell = home_cell(i, j);
+            if (cell >= 0) gather(j, cell);
         }
         __syncthreads();
         for (int i = tid; i < TG_TT * P; i += 256) {
@@ -311,15 +358,18 @@ __global__ __launch_bounds__(256) void k_density_keys(GridArgs a, const uint32_t
             s1 += v;
             s2 += v * v;
         }
-        for (uint32_t i = p0 + tid; i < p1; i += 256) {
-            const uint32_t pj = a.pairs[i];
-            if (!(pj & PAIR_HOME)) continue;
-            const uint32_t j = pj & PAIR_INDEX;
-            const int4 d = a.cdesc[j];
-            const int S = d.w & 0x7fff;
-            const int mx = (d.y & 0xffff) + S / 2 - x0, my = (int)((unsigned)d.y >> 16) + S / 2 - y0;
 #pragma unroll
-            for (int p = 0; p < P; p++) a.rdens[(int64_t)j * P + p] = s_d[(my * TG_T + mx) * P + p];
+        for (int k = 0; k < DK; k++)
+            if (kc[k] >= 0) {
+#pragma unroll
+                for (int p = 0; p < P; p++) a.rdens[(int64_t)kj[k] * P + p] = s_d[kc[k] * P + p];
+            }
+        for (uint32_t i = p0 + tid + 256u * DK; i < p1; i += 256) {
+            uint32_t j;
+            const int cell = home_cell(i, j);
+            if (cell < 0) continue;
+#pragma unroll
+            for (int p = 0; p < P; p++) a.rdens[(int64_t)j * P + p] = s_d[cell * P + p];
         }
         __syncthreads();  // s_d is cleared for the next tile
     }
@@ -334,6 +384,10 @@ __global__ __launch_bounds__(256) void k_density_keys(GridArgs a, const uint32_t
 template <int P>
 __global__ __launch_bounds__(256) void k_record_prep(GridArgs a)
 {
+    // the block's weighted visibilities, stored from LDS in 16-byte runs: a
+    // record's P samples stored lane by lane (8 bytes at P * 8-byte strides)
+    // left partly written lines that went out to memory once per store
+    __shared__ __align__(16) float2 s_vis[256 * P];
     const uint32_t M = *a.n_rec;
     const int lane = threadIdx.x & 63;
     double np[P];
@@ -364,9 +418,7 @@ __global__ __launch_bounds__(256) void k_record_prep(GridArgs a)
             const int jc = a.cjc[j];
             const JobDev &jd = a.js.jobs[jc >> 16];
             const int c_local = jc & 0xffff;
-            const int cx = d.y & 0xffff, cy = (int)((unsigned)d.y >> 16);
-            const int plane = (int)((unsigned)d.w >> 16);
-            const double csum = (double)jd.sums[plane * jd.T2 + (d.z - jd.g.convinfo[plane].y) / (S * S)];
+            const double csum = (double)jd.sums[d.x];
             float dens[P];
             if (jd.weighting != MTI_WEIGHT_NATURAL) {
 #pragma unroll
@@ -382,11 +434,22 @@ __global__ __launch_bounds__(256) void k_record_prep(GridArgs a)
                 // a zero weight (flagged) gives an exact zero, whatever the
                 // visibility holds (a flagged sample may be NaN)
                 const float2 vv = a.wvis[(int64_t)j * P + p];
-                a.wvis[(int64_t)j * P + p] = w != 0.0f ? make_float2(vv.x * w, vv.y * w) : make_float2(0.0f, 0.0f);
+                s_vis[threadIdx.x * P + p] = w != 0.0f ? make_float2(vv.x * w, vv.y * w) : make_float2(0.0f, 0.0f);
                 if (jc == jc_first) np[p] += csum * (double)w;
                 else unsafeAtomicAdd(&jd.norm[c_local * P + p], csum * (double)w);
             }
+        } else {
+            // not gridded: no kernel reads its visibilities
+#pragma unroll
+            for (int p = 0; p < P; p++) s_vis[threadIdx.x * P + p] = make_float2(0.0f, 0.0f);
         }
+        __syncthreads();
+        const int64_t e0 = (int64_t)j0 * P, ne = (int64_t)min(256u, M - j0) * P;  // the block's samples
+        for (int q = threadIdx.x; 2 * q < ne; q += 256) {
+            if (2 * q + 1 < ne) *(float4 *)&a.wvis[e0 + 2 * q] = ((const float4 *)s_vis)[q];
+            else a.wvis[e0 + 2 * q] = s_vis[2 * q];
+        }
+        __syncthreads();
     }
     if (jc_prev >= 0) flush(jc_prev);
 }

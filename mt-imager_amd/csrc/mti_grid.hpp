@@ -94,7 +94,7 @@ constexpr int BIN_STAT_STRIPES = 32;
 struct BinPassArgs {
     JobSet js;
     int32_t *cjc;               // [nrec] job << 16 | local channel
-    int4 *desc;                 // [nrec]
+    int4 *desc;                 // [nrec] record descriptors (layout at GridArgs::cdesc)
     float2 *wv;                 // [nrec][npol] visibilities of gridded records
     float *wt;                  // [nrec][npol] weights, 0 where flagged
     unsigned long long *stats;  // [2][BIN_STAT_STRIPES]: gridded records, gridded points
