@@ -683,6 +683,8 @@ def main():
                     help="zero-fill and transform every grid tile (default: sparse grids with a tile mask)")
     ap.add_argument("--xcd-chunk", type=int, default=None, help="cube runs: gridder units per XCD run")
     ap.add_argument("--unit-pairs", type=int, default=None, help="cube runs: gridder work-unit size")
+    ap.add_argument("--region-width", type=int, default=None,
+                    help="cube runs: gridder wave regions of W x 64/W cells (4, 8 or 16)")
     ap.add_argument("--transform-storage", choices=["c128", "c64"], default="c64",
                     help="element type of the packed planes between the finalize's transform passes")
     ap.add_argument("--grid-precision", choices=["c128", "c64"], default="c64",
@@ -735,7 +737,7 @@ def main():
     d = make_workload(torch, args.config, args.seed)
     p = mti.make_params(d["nx"], d["nx"], d["cell"], d["cell"], "wproj", d["W"], d["s"])
     ctx = mti.Context(0)
-    ctx.set_tuning(unit_pairs=args.unit_pairs, xcd_chunk=args.xcd_chunk)
+    ctx.set_tuning(unit_pairs=args.unit_pairs, xcd_chunk=args.xcd_chunk, region_width=args.region_width)
     ctx.set_transform_storage(args.transform_storage)
     nrows, nchan, P, nx = d["nrows"], d["nchan"], d["P"], d["nx"]
     grids = torch.empty((nchan, P, nx, nx, 2), dtype=torch.float32 if args.grid_precision == "c64" else torch.float64,

@@ -294,63 +294,64 @@ __global__ __launch_bounds__(256) void k_density_keys(GridArgs a, const uint32_t
         const int x0 = ta.tx * TG_T, y0 = ta.ty * TG_T;
         for (int i = tid; i < P * TG_TT; i += 256) s_d[i] = 0.0f;
         __syncthreads();
-        // pair i of the tile -> its record and the record's nearest cell in the
-        // tile (-1: not the record's home tile, which alone gathers it)
-        auto home_cell = [&](uint32_t i, uint32_t &j) {
-            const uint32_t pj = a.pairs[i];
-            j = pj & PAIR_INDEX;
-            if (!(pj & PAIR_HOME)) return -1;
-            const int4 d = a.cdesc[j];
-            const int S = d.w & 0x7fff;
-            const int mx = (d.y & 0xffff) + S / 2 - x0, my = (int)((unsigned)d.y >> 16) + S / 2 - y0;
-            return my * TG_T + mx;
-        };
-        auto gather = [&](uint32_t j, int cell) {
-#pragma unroll
-            for (int p = 0; p < P; p++) {
-                const float w = a.wt[(int64_t)j * P + p];
-                if (w != 0.0f) atomicAdd(&s_d[cell * P + p], w);
-            }
-        };
-        // the first DK pairs of each thread keep (record, cell) in registers
-        // for the write-back; pairs past DK * 256 are read again.  Their reads
-        // go in phases (pairs, then the home records' descriptors, then their
-        // weights), each phase's DK reads in flight together
+        // DK pairs a thread at a time (pairs i0 + tid + 256 k): their records
+        // and the records' nearest cells in the tile (-1: not the record's home
+        // tile, which alone gathers it), read in phases -- the pairs, then the
+        // home records' descriptors -- so each phase's DK reads are in flight
+        // together (crowded inner-uv tiles hold thousands of pairs)
         constexpr int DK = 4;
-        uint32_t kj[DK];
-        int kc[DK];
-        int4 kd[DK];
-        float kw[DK][P];
+        auto walk = [&](uint32_t i0, uint32_t (&kj)[DK], int (&kc)[DK]) {
+            int4 kd[DK];
 #pragma unroll
-        for (int k = 0; k < DK; k++) {
-            const uint32_t i = p0 + tid + 256u * k;
-            const uint32_t pj = i < p1 ? a.pairs[i] : 0u;  // past the tile: no home bit
-            kj[k] = pj & PAIR_INDEX;
-            kc[k] = (pj & PAIR_HOME) ? 0 : -1;
-        }
-#pragma unroll
-        for (int k = 0; k < DK; k++) kd[k] = kc[k] >= 0 ? a.cdesc[kj[k]] : make_int4(0, 0, 0, 0);
-#pragma unroll
-        for (int k = 0; k < DK; k++) {
-#pragma unroll
-            for (int p = 0; p < P; p++) kw[k][p] = kc[k] >= 0 ? a.wt[(int64_t)kj[k] * P + p] : 0.0f;
-            if (kc[k] >= 0) {
-                const int S = kd[k].w & 0x7fff;
-                const int mx = (kd[k].y & 0xffff) + S / 2 - x0, my = (int)((unsigned)kd[k].y >> 16) + S / 2 - y0;
-                kc[k] = my * TG_T + mx;
+            for (int k = 0; k < DK; k++) {
+                const uint32_t i = i0 + tid + 256u * k;
+                const uint32_t pj = i < p1 ? a.pairs[i] : 0u;  // past the tile: no home bit
+                kj[k] = pj & PAIR_INDEX;
+                kc[k] = (pj & PAIR_HOME) ? 0 : -1;
             }
-        }
 #pragma unroll
-        for (int k = 0; k < DK; k++)
-            if (kc[k] >= 0) {
+            for (int k = 0; k < DK; k++) kd[k] = kc[k] >= 0 ? a.cdesc[kj[k]] : make_int4(0, 0, 0, 0);
 #pragma unroll
-                for (int p = 0; p < P; p++)
-                    if (kw[k][p] != 0.0f) atomicAdd(&s_d[kc[k] * P + p], kw[k][p]);
-            }
-        for (uint32_t i = p0 + tid + 256u * DK; i < p1; i += 256) {
-            uint32_t j;
-            const int cell = home_cell(i, j);
-            if (cell >= 0) gather(j, cell);
+            for (int k = 0; k < DK; k++)
+                if (kc[k] >= 0) {
+                    const int S = kd[k].w & 0x7fff;
+                    const int mx = (kd[k].y & 0xffff) + S / 2 - x0, my = (int)((unsigned)kd[k].y >> 16) + S / 2 - y0;
+                    kc[k] = my * TG_T + mx;
+                }
+        };
+        auto gather = [&](const uint32_t (&kj)[DK], const int (&kc)[DK]) {
+            float kw[DK][P];
+#pragma unroll
+            for (int k = 0; k < DK; k++)
+#pragma unroll
+                for (int p = 0; p < P; p++) kw[k][p] = kc[k] >= 0 ? a.wt[(int64_t)kj[k] * P + p] : 0.0f;
+#pragma unroll
+            for (int k = 0; k < DK; k++)
+                if (kc[k] >= 0) {
+#pragma unroll
+                    for (int p = 0; p < P; p++)
+                        if (kw[k][p] != 0.0f) atomicAdd(&s_d[kc[k] * P + p], kw[k][p]);
+                }
+        };
+        auto hand_back = [&](const uint32_t (&kj)[DK], const int (&kc)[DK]) {
+#pragma unroll
+            for (int k = 0; k < DK; k++)
+                if (kc[k] >= 0) {
+#pragma unroll
+                    for (int p = 0; p < P; p++) a.rdens[(int64_t)kj[k] * P + p] = s_d[kc[k] * P + p];
+                }
+        };
+        // the first chunk's (record, cell) stay in registers for the hand-back;
+        // later chunks are walked again
+        uint32_t cj[DK];
+        int cc[DK];
+        walk(p0, cj, cc);
+        gather(cj, cc);
+        for (uint32_t i0 = p0 + 256u * DK; i0 < p1; i0 += 256u * DK) {
+            uint32_t kj[DK];
+            int kc[DK];
+            walk(i0, kj, kc);
+            gather(kj, kc);
         }
         __syncthreads();
         for (int i = tid; i < TG_TT * P; i += 256) {
@@ -358,18 +359,12 @@ __global__ __launch_bounds__(256) void k_density_keys(GridArgs a, const uint32_t
             s1 += v;
             s2 += v * v;
         }
-#pragma unroll
-        for (int k = 0; k < DK; k++)
-            if (kc[k] >= 0) {
-#pragma unroll
-                for (int p = 0; p < P; p++) a.rdens[(int64_t)kj[k] * P + p] = s_d[kc[k] * P + p];
-            }
-        for (uint32_t i = p0 + tid + 256u * DK; i < p1; i += 256) {
-            uint32_t j;
-            const int cell = home_cell(i, j);
-            if (cell < 0) continue;
-#pragma unroll
-            for (int p = 0; p < P; p++) a.rdens[(int64_t)j * P + p] = s_d[cell * P + p];
+        hand_back(cj, cc);
+        for (uint32_t i0 = p0 + 256u * DK; i0 < p1; i0 += 256u * DK) {
+            uint32_t kj[DK];
+            int kc[DK];
+            walk(i0, kj, kc);
+            hand_back(kj, kc);
         }
         __syncthreads();  // s_d is cleared for the next tile
     }
