@@ -80,31 +80,66 @@ __device__ __forceinline__ int mod_pos(long long a, int m)
     return (int)(r < 0 ? r + m : r);
 }
 
-// One block per plane: support find on the centre row / column.
-constexpr int SUP_THREADS = 256;
-__global__ __launch_bounds__(SUP_THREADS) void k_support(const double2 *scr, ScreenGeom g,
-                                                         const double2 *tw, double inv_div,
-                                                         int k0, int *support_out,
-                                                         double2 *proj /* [nb][2][St] */)
+// Projections of each screen of a batch: rowp[ix] = sum over iy of S[iy][ix]
+// and colp[iy] = sum over ix of S[iy][ix], each summed in ascending index
+// order (the oracle's order, so the support decision sees the same values).
+// Blocks (plane, y): y < PJ_RB(St) takes 64 rows' sums, staging 64 x 64 tiles
+// in LDS so the reads run along rows; the others take 256 columns' sums, a
+// thread a column walking the rows.  Many blocks a plane and reads that run
+// along rows, where one block a plane summing rows thread by thread read
+// 16 bytes a lane at a stride of St x 16 bytes.
+constexpr int PJ_THREADS = 256, PJ_T = 64;
+__host__ __device__ inline int pj_row_blocks(int St) { return (St + PJ_T - 1) / PJ_T; }
+__host__ __device__ inline int pj_col_blocks(int St) { return (St + PJ_THREADS - 1) / PJ_THREADS; }
+__global__ __launch_bounds__(PJ_THREADS) void k_proj(const double2 *scr, int St, double2 *proj /* [nb][2][St] */)
 {
-    const int b = blockIdx.x;
-    const double2 *S = scr + (size_t)b * g.St * g.St;
-    double2 *rowp = proj + (size_t)b * 2 * g.St;
-    double2 *colp = rowp + g.St;
-    for (int i = threadIdx.x; i < g.St; i += blockDim.x) {
-        double2 r = make_double2(0, 0), c = make_double2(0, 0);
-        for (int j = 0; j < g.St; j++) {
-            double2 a = S[(size_t)j * g.St + i];  // column sum over rows j
+    __shared__ double2 tile[PJ_T][PJ_T + 1];
+    const int b = blockIdx.x, rb = pj_row_blocks(St);
+    const double2 *S = scr + (size_t)b * St * St;
+    double2 *rowp = proj + (size_t)b * 2 * St;
+    double2 *colp = rowp + St;
+    if ((int)blockIdx.y >= rb) {
+        const int ix = ((int)blockIdx.y - rb) * PJ_THREADS + threadIdx.x;
+        if (ix >= St) return;
+        double2 r = make_double2(0, 0);
+        for (int iy = 0; iy < St; iy++) {
+            const double2 a = S[(size_t)iy * St + ix];
             r.x += a.x;
             r.y += a.y;
-            double2 d = S[(size_t)i * g.St + j];  // row sum
-            c.x += d.x;
-            c.y += d.y;
         }
-        rowp[i] = r;
-        colp[i] = c;
+        rowp[ix] = r;
+        return;
     }
-    __syncthreads();
+    const int y0 = (int)blockIdx.y * PJ_T, t = threadIdx.x;
+    double2 c = make_double2(0, 0);  // thread t < 64: row y0 + t
+    for (int x0 = 0; x0 < St; x0 += PJ_T) {
+        for (int e = t; e < PJ_T * PJ_T; e += PJ_THREADS) {
+            const int ry = e / PJ_T, rx = e % PJ_T;
+            tile[ry][rx] = (y0 + ry < St && x0 + rx < St) ? S[(size_t)(y0 + ry) * St + x0 + rx] : make_double2(0, 0);
+        }
+        __syncthreads();
+        if (t < PJ_T) {
+            const int n = min(PJ_T, St - x0);
+            for (int rx = 0; rx < n; rx++) {
+                c.x += tile[t][rx].x;
+                c.y += tile[t][rx].y;
+            }
+        }
+        __syncthreads();
+    }
+    if (t < PJ_T && y0 + t < St) colp[y0 + t] = c;
+}
+
+// One block per plane: support find on the centre row / column (the DFTs of
+// the projections k_proj wrote).
+constexpr int SUP_THREADS = 256;
+__global__ __launch_bounds__(SUP_THREADS) void k_support(ScreenGeom g, const double2 *tw, double inv_div,
+                                                         int k0, int *support_out,
+                                                         const double2 *proj /* [nb][2][St] */)
+{
+    const int b = blockIdx.x;
+    const double2 *rowp = proj + (size_t)b * 2 * g.St;
+    const double2 *colp = rowp + g.St;
     __shared__ int found;
     if (threadIdx.x == 0) found = -1;
     __syncthreads();
@@ -116,6 +151,9 @@ __global__ __launch_bounds__(SUP_THREADS) void k_support(const double2 *scr, Scr
             const double2 *pr = pass ? colp : rowp;
             int m = mod_pos(-(long long)g.h * q, g.L);
             double re = 0, im = 0;
+            // unrolled so the twiddle gathers go out ahead of the sums; the
+            // sums stay one chain in ascending c (the oracle's order)
+#pragma unroll 8
             for (int c = 0; c < g.St; c++) {
                 double2 t = tw[m];
                 double2 v = pr[c];
@@ -428,8 +466,11 @@ static mti_status build_impl(mti_ctx *ctx, const mti_image_params *p, int precis
             dim3 sg((unsigned)ceil_div(St, 16), (unsigned)ceil_div(St, 16), (unsigned)n);
             hipLaunchKernelGGL(k_screen, sg, sb, 0, st, d_scr, g, K->info.w_increment, k0);
             MTI_HIP(ctx, hipGetLastError());
-            hipLaunchKernelGGL(k_support, dim3((unsigned)n), dim3(SUP_THREADS), 0, st, d_scr, g,
-                               d_tw, inv, k0, d_sup, d_proj);
+            hipLaunchKernelGGL(k_proj, dim3((unsigned)n, (unsigned)(pj_row_blocks(St) + pj_col_blocks(St))),
+                               dim3(PJ_THREADS), 0, st, d_scr, St, d_proj);
+            MTI_HIP(ctx, hipGetLastError());
+            hipLaunchKernelGGL(k_support, dim3((unsigned)n), dim3(SUP_THREADS), 0, st, g, d_tw, inv, k0, d_sup,
+                               d_proj);
             MTI_HIP(ctx, hipGetLastError());
         }
         K->support.assign(W, 0);
