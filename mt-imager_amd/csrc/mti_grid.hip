@@ -319,19 +319,36 @@ __global__ __launch_bounds__(256) void k_density_keys(GridArgs a, const uint32_t
                     kc[k] = my * TG_T + mx;
                 }
         };
+        // consecutive pairs are consecutive time samples of a baseline and
+        // often share a cell (crowded inner-uv cells): a wave first sums each
+        // run of equal cells in its lanes (segmented shuffle sum down to the
+        // run's head), then issues one LDS atomic per run -- atomics to one
+        // address serialise
         auto gather = [&](const uint32_t (&kj)[DK], const int (&kc)[DK]) {
             float kw[DK][P];
 #pragma unroll
             for (int k = 0; k < DK; k++)
 #pragma unroll
                 for (int p = 0; p < P; p++) kw[k][p] = kc[k] >= 0 ? a.wt[(int64_t)kj[k] * P + p] : 0.0f;
+            const int lane = tid & 63;
 #pragma unroll
-            for (int k = 0; k < DK; k++)
-                if (kc[k] >= 0) {
+            for (int k = 0; k < DK; k++) {
+                const WaveRuns r = wave_runs(kc[k]);
+                if (__ballot(!r.head)) {  // some run is longer than one lane
+                    const int end = lane + r.len;
+                    for (int d = 1; d < 64; d <<= 1)
+#pragma unroll
+                        for (int p = 0; p < P; p++) {
+                            const float o = __shfl_down(kw[k][p], d, 64);
+                            if (lane + d < end) kw[k][p] += o;
+                        }
+                }
+                if (r.head && kc[k] >= 0) {
 #pragma unroll
                     for (int p = 0; p < P; p++)
                         if (kw[k][p] != 0.0f) atomicAdd(&s_d[kc[k] * P + p], kw[k][p]);
                 }
+            }
         };
         auto hand_back = [&](const uint32_t (&kj)[DK], const int (&kc)[DK]) {
 #pragma unroll
