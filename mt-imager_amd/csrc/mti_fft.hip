@@ -422,6 +422,26 @@ struct PkGeom {
 // the arithmetic waits for the first.  Zero bits are +0 in either precision.
 constexpr uint32_t PK_OUTSIDE = 0xfffffff0u;
 constexpr int PK_BUFFER_DWORD3 = 0x00020000;  // raw buffer, gfx9-family dword 3
+
+// Unit queues of the persistent row pass: one counter per XCD, each in its
+// own 128-byte line (PK_QSTRIDE words apart), reset by a one-block kernel
+// before every launch (a fill kernel rather than a memset, so that a
+// captured graph replays it).
+constexpr int PK_QUEUES = 8;
+constexpr int PK_QSTRIDE = 32;
+constexpr size_t PK_QUEUE_BYTES = (size_t)PK_QUEUES * PK_QSTRIDE * sizeof(uint32_t);
+
+__global__ void k_pk_queue_reset(uint32_t *q)
+{
+    if (threadIdx.x < PK_QUEUES) q[threadIdx.x * PK_QSTRIDE] = 0u;
+}
+
+// Bytes of a descriptor array in the descriptor slot, rounded up so that
+// what follows it starts 256-byte aligned.
+static inline size_t desc_region(size_t n, size_t size)
+{
+    return (std::max<size_t>(1, n) * size + 255) / 256 * 256;
+}
 typedef unsigned int pk_u32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned int pk_u32x4 __attribute__((ext_vector_type(4)));
 
@@ -715,12 +735,11 @@ __device__ __forceinline__ void pk_pipe_rest(double2 *lds, const double2 *tb2, c
 }
 
 // Row pass from complex64 grids, software-pipelined: a persistent block
-// walks the units L = blockIdx.x + k * gridDim.x of k_rows_pk (a row pair,
-// or at N = 4096 one row; the same unit order, so the blocks that share grid
-// lines still run together on one XCD) and issues the grid reads of the
-// first AHEAD elements of its next unit before the later passes of the
-// current one, so those reads are in flight during the passes instead of
-// after them.  A global load waits for every load issued before it, so no
+// walks the units of k_rows_pk (a row pair, or at N = 4096 one row; the same
+// unit order, so the blocks that share grid lines run on one XCD) and issues
+// the grid reads of the first AHEAD elements of its next unit before the
+// later passes of the current one, so those reads are in flight during the
+// passes instead of after them.  A global load waits for every load issued before it, so no
 // global load follows the early reads until they are used: the passes'
 // twiddles come from bases w^k, w^4k staged in LDS once per block (the
 // others as products, a few ulp from the table values), and the next unit's
@@ -728,7 +747,8 @@ __device__ __forceinline__ void pk_pipe_rest(double2 *lds, const double2 *tb2, c
 // pass.
 template <int N, typename XT>
 __global__ __launch_bounds__(PK_TH) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_rows_pk_pipe(
-    const PackDesc *__restrict__ pd, int npk, XT *X, int ny, int64_t cs, const double2 *tw, int NT)
+    const PackDesc *__restrict__ pd, int npk, XT *X, int ny, int64_t cs, const double2 *tw, int NT,
+    uint32_t *queues)
 {
     using TI = float2;
     using G = PkGeom<N>;
@@ -753,8 +773,26 @@ __global__ __launch_bounds__(PK_TH) __attribute__((amdgpu_waves_per_eu(2, 2))) v
         u.y1 = mir ? ((ny - u.bx) & (ny - 1)) : u.bx;
         return !(G::ROW && mir && u.y1 == u.bx);
     };
-    auto next_unit = [&](int L, Unit &u) {
+    // Rows of 4096 (one-row units) come from one queue per XCD (workgroups
+    // go round-robin to the 8 XCDs): queue q holds L = q + nq k in order, k
+    // from the queue's atomic counter, so a block that finishes early takes
+    // more units instead of idling while others finish theirs.  Thread 0
+    // takes the unit after the next one while the block works on the current
+    // one; a queue's empty (self-mirrored) units are skipped.  Shorter rows
+    // walk L = blockIdx.x + k * gridDim.x: their launches are small, and the
+    // queue measured slower there.
+    constexpr bool QUEUE = G::ROW;
+    auto next_static = [&](int L, Unit &u) {
         while (L < U && !unit(L, u)) L += gridDim.x;
+        return L;
+    };
+    __shared__ int s_take[2];
+    const int nq = min((int)gridDim.x, PK_QUEUES), q = (int)(blockIdx.x % (unsigned)nq);
+    uint32_t *qc = queues + q * PK_QSTRIDE;
+    auto taken = [&](uint32_t k) {  // thread 0: queue entry k, or the next non-empty one
+        Unit t;
+        int L = q + nq * (int)k;
+        while (L < U && !unit(L, t)) L = q + nq * (int)atomicAdd(qc, 1u);
         return L;
     };
     // transform row of block row f
@@ -810,8 +848,22 @@ __global__ __launch_bounds__(PK_TH) __attribute__((amdgpu_waves_per_eu(2, 2))) v
                  l.cBm);
     };
     Unit u;
-    int L = next_unit(blockIdx.x, u);
-    if (L >= U) return;
+    int L, Ln = 0;  // the current unit; with the queue also the next (U: none)
+    if constexpr (QUEUE) {
+        if (j == 0) {
+            const int L0 = taken(atomicAdd(qc, 1u));
+            s_take[0] = L0;
+            s_take[1] = L0 < U ? taken(atomicAdd(qc, 1u)) : U;
+        }
+        __syncthreads();
+        L = __builtin_amdgcn_readfirstlane(s_take[0]);
+        Ln = __builtin_amdgcn_readfirstlane(s_take[1]);
+        if (L >= U) return;
+        unit(L, u);
+    } else {
+        L = next_static(blockIdx.x, u);
+        if (L >= U) return;
+    }
     // twiddle bases of the passes of stride 16 (radix R2) and 256 (radix
     // N / 256): fb_pass's twiddle tw[r k step] with step = NT / (NS R)
     {
@@ -828,7 +880,7 @@ __global__ __launch_bounds__(PK_TH) __attribute__((amdgpu_waves_per_eu(2, 2))) v
     // passes, the rest when its own first pass starts (more ahead does not
     // fit two waves per SIMD without spilling: the radix-8 and radix-4 last
     // passes of 2048 and 1024 hold more twiddles)
-    constexpr int AHEAD = (N == 1024 || N == 2048) ? 4 : 8;
+    constexpr int AHEAD = (N == 1024 || N == 2048) ? 4 : N == 4096 ? 7 : 8;
     const int f0 = j / PER, jj = j % PER;  // first pass: transform row f0, elements jj + PER r
     int buf = 0;
     uint8_t m[NQ];
@@ -849,7 +901,15 @@ __global__ __launch_bounds__(PK_TH) __attribute__((amdgpu_waves_per_eu(2, 2))) v
             for (int r = AHEAD; r < 16; r++) tok[r] = rd.issue(f0, jj + PER * r);
         }
         Unit un;
-        const int Ln = next_unit(L + gridDim.x, un);
+        uint32_t kt = 0;
+        if constexpr (QUEUE) {
+            if (Ln < U) unit(Ln, un);
+            // the unit after next (wave 0 holds the queue entry in a scalar
+            // register: a vector one would spill at the first pass's peak)
+            if (j < 64) kt = __builtin_amdgcn_readfirstlane(j == 0 && Ln < U ? atomicAdd(qc, 1u) : 0u);
+        } else {
+            Ln = next_static(L + gridDim.x, un);
+        }
         const bool more = Ln < U;
         uint8_t mn[NQ];
         double nan_ = 1.0, nbn = 1.0;
@@ -866,7 +926,10 @@ __global__ __launch_bounds__(PK_TH) __attribute__((amdgpu_waves_per_eu(2, 2))) v
             for (int r = 0; r < 16; r++) lds[fb_pad(f0 * N + 16 * jj + r)] = v[r];
         }
         if (more) stage(mn, buf ^ 1);
+        if constexpr (QUEUE)
+            if (j == 0) s_take[buf] = more ? taken(kt) : U;
         __syncthreads();
+        const int Lnn = QUEUE ? __builtin_amdgcn_readfirstlane(s_take[buf]) : 0;
         if (more) {
             const Load rd = reader(un, buf ^ 1);
 #pragma unroll
@@ -885,7 +948,10 @@ __global__ __launch_bounds__(PK_TH) __attribute__((amdgpu_waves_per_eu(2, 2))) v
             dst[(size_t)y * N + kx] = x_st<XT>(v);
         });
         if (!more) break;
-        L = Ln;
+        if constexpr (QUEUE)
+            Ln = Lnn;
+        else
+            L = Ln;
         u = un;
         na = nan_;
         nb = nbn;
@@ -998,7 +1064,7 @@ static bool whole_columns(const mti_ctx *ctx, const FftGeom &g)
 
 template <typename TI, typename XT>
 static mti_status pk_rows(mti_ctx *ctx, const FftGeom &g, const PackDesc *pd, int npk, XT *X, int64_t cs,
-                          const double2 *tw, hipStream_t st)
+                          const double2 *tw, uint32_t *queues, hipStream_t st)
 {
     const int npairs = g.ny / 2 + 1;
     if (pk_span_bytes(g.nx, g.ny, cs, sizeof(TI)) > PK_OUTSIDE)
@@ -1013,8 +1079,9 @@ static mti_status pk_rows(mti_ctx *ctx, const FftGeom &g, const PackDesc *pd, in
     case NN: {                                                                                                    \
         using G = PkGeom<NN>;                                                                                     \
         const int64_t units = (int64_t)G::M * npk * ceil_div(npairs, G::PB);                                      \
+        if (NN == PK_E) hipLaunchKernelGGL(k_pk_queue_reset, dim3(1), dim3(PK_QUEUES), 0, st, queues);           \
         hipLaunchKernelGGL((k_rows_pk_pipe<NN, XT>), dim3((unsigned)std::min(units, cap)), dim3(PK_TH), 0, st, pd, \
-                           npk, X, g.ny, cs, tw, g.NT);                                                           \
+                           npk, X, g.ny, cs, tw, g.NT, queues);                                                   \
         MTI_HIP(ctx, hipGetLastError());                                                                          \
         return MTI_OK;                                                                                            \
     }
@@ -1159,12 +1226,12 @@ static void make_packs(const std::vector<ChanRef> &ch, int P, int poltype, int l
 // the planes X (complex128 or complex64, XT).
 template <typename TI, typename XT>
 static mti_status finalize_packs(mti_ctx *ctx, const FftGeom &g, const PackDesc *pd, int npk, XT *X, int64_t cs,
-                                 const double2 *tw, hipStream_t st)
+                                 const double2 *tw, uint32_t *queues, hipStream_t st)
 {
     mti_status r;
     {
         StageTimer tm(ctx, MTI_STAGE_FFT_ROWS, st);
-        r = pk_rows<TI, XT>(ctx, g, pd, npk, X, cs, tw, st);
+        r = pk_rows<TI, XT>(ctx, g, pd, npk, X, cs, tw, queues, st);
         if (r != MTI_OK) return r;
     }
     if (whole_columns(ctx, g)) {
@@ -1664,10 +1731,10 @@ static mti_status gen_finalize_packs(mti_ctx *ctx, const GenPlan &g, const doubl
 // Uploads the descriptors of a call (once; the copy completes before any
 // batch is enqueued, so later batches never overwrite what earlier ones read).
 template <typename D>
-static mti_status upload_desc(mti_ctx *ctx, const std::vector<D> &h, D **d, hipStream_t st)
+static mti_status upload_desc(mti_ctx *ctx, const std::vector<D> &h, D **d, hipStream_t st, size_t extra = 0)
 {
     void *w = nullptr;
-    mti_status r = scratch_get(ctx, SLOT_CHANDESC, std::max<size_t>(1, h.size()) * sizeof(D), &w);
+    mti_status r = scratch_get(ctx, SLOT_CHANDESC, desc_region(h.size(), sizeof(D)) + extra, &w);
     if (r != MTI_OK) return r;
     if ((r = upload(ctx, w, h.data(), h.size() * sizeof(D), st)) != MTI_OK) return r;
     if ((r = host_sync(ctx, st)) != MTI_OK) return r;
@@ -1682,9 +1749,11 @@ static mti_status finalize_channels(mti_ctx *ctx, const FftGeom &g, const std::v
     std::vector<PackDesc> h;
     make_packs(ch, P, poltype, layout, g.nx, g.ny, c64 ? sizeof(float2) : sizeof(double2), h);
     PackDesc *pd = nullptr;
-    mti_status r = upload_desc(ctx, h, &pd, st);
+    mti_status r = upload_desc(ctx, h, &pd, st, PK_QUEUE_BYTES);
     if (r != MTI_OK) return r;
     const int npk = (int)h.size();
+    // the row pass's unit queues sit behind the descriptors
+    uint32_t *queues = (uint32_t *)((char *)pd + desc_region(h.size(), sizeof(PackDesc)));
     const int64_t cs = layout == MTI_GRID_VUP ? P : 1;
     if (!pk_ok(g)) {
         // general sizes: complex128 planes, HBM passes
@@ -1715,11 +1784,11 @@ static mti_status finalize_channels(mti_ctx *ctx, const FftGeom &g, const std::v
     for (int k0 = 0; k0 < npk; k0 += batch) {
         const int nk = std::min(batch, npk - k0);
         if (x64)
-            r = c64 ? finalize_packs<float2>(ctx, g, pd + k0, nk, (float2 *)w, cs, tw, st)
-                    : finalize_packs<double2>(ctx, g, pd + k0, nk, (float2 *)w, cs, tw, st);
+            r = c64 ? finalize_packs<float2>(ctx, g, pd + k0, nk, (float2 *)w, cs, tw, queues, st)
+                    : finalize_packs<double2>(ctx, g, pd + k0, nk, (float2 *)w, cs, tw, queues, st);
         else
-            r = c64 ? finalize_packs<float2>(ctx, g, pd + k0, nk, (double2 *)w, cs, tw, st)
-                    : finalize_packs<double2>(ctx, g, pd + k0, nk, (double2 *)w, cs, tw, st);
+            r = c64 ? finalize_packs<float2>(ctx, g, pd + k0, nk, (double2 *)w, cs, tw, queues, st)
+                    : finalize_packs<double2>(ctx, g, pd + k0, nk, (double2 *)w, cs, tw, queues, st);
         if (r != MTI_OK) return r;
     }
     return MTI_OK;
@@ -1736,7 +1805,7 @@ static void plan_gen(const GenPlan &gp, int planes, ScratchPlan &pl)
 
 static void plan_finalize_channels(const FftGeom &g, int npk, ScratchPlan &pl)
 {
-    pl.need(SLOT_CHANDESC, (size_t)std::max(1, npk) * sizeof(PackDesc));
+    pl.need(SLOT_CHANDESC, desc_region((size_t)npk, sizeof(PackDesc)) + PK_QUEUE_BYTES);
     if (!pk_ok(g)) {
         plan_gen(gen_plan(g.nx, g.ny), npk, pl);
         return;
