@@ -58,10 +58,13 @@ __global__ void k_bin_only(VisView v, BinGeom g, int chan, int32_t *out)
 // contiguous), and phase 2 bins the records with the 64 lanes of a wave on 64
 // consecutive index entries of one channel, so every store of descriptors,
 // visibilities and weights is one contiguous run per channel (records are
-// numbered e = rec0 + c_local * nidx + i).
+// numbered e = rec0 + c_local * nidx + i).  Chunks of 8 channels keep the
+// block's LDS at 20 KB for four correlations, so 8 blocks fit a CU: 16-channel
+// chunks (40 KB, 4 blocks) took 8.8-9.1 ms on the 64-channel full-polarisation
+// cube against 7.5-7.6 ms, and 4-channel chunks 7.7-7.9 ms.
 constexpr int BT_THREADS = 256;
 constexpr int BT_ROWS = 64;
-constexpr int BT_CHANS = 16;
+constexpr int BT_CHANS = 8;
 
 template <int P>
 struct BinTile {
