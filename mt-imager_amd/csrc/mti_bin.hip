@@ -115,12 +115,14 @@ __global__ __launch_bounds__(BT_THREADS) void k_bin_pass(BinPassArgs a)
                 t.vis[c][r][p] = *(const float2 *)(v.vis + 2 * (row * v.vis_rs + (int64_t)chan * v.vis_cs + p));
         }
         __syncthreads();
-        // phase 2: wave w takes channels w, w + 4, ...; lane = index entry
-        if (i < v.nidx) {
-            for (int c = wave; c < nc; c += BT_THREADS / 64) {
-                const int cl = c0 + c;
+        // phase 2: wave w takes channels w, w + 4, ...; lane = index entry.
+        // Every lane then adds its record's tiles to the pair counts (all
+        // lanes take part in the count's shuffles; past the index: none).
+        for (int c = wave; c < nc; c += BT_THREADS / 64) {
+            const int cl = c0 + c;
+            int4 d = make_int4(0, 0, 0, 0);
+            if (i < v.nidx) {
                 const int64_t e = jd.rec0 + (int64_t)cl * v.nidx + i;
-                int4 d = make_int4(0, 0, 0, 0);
                 if (lane < nr) {
                     bool any = false;
 #pragma unroll
@@ -150,6 +152,8 @@ __global__ __launch_bounds__(BT_THREADS) void k_bin_pass(BinPassArgs a)
                 a.cjc[e] = (job << 16) | (jd.mfs ? 0 : cl);
                 a.desc[e] = d;
             }
+            if (a.tile_cnt)
+                count_tile_pairs(d, (int)a.js.key0[job] + (jd.mfs ? 0 : cl) * jd.nty * jd.ntx, jd.ntx, a.tile_cnt);
         }
     }
     for (int o = 32; o >= 1; o >>= 1) {

@@ -17,8 +17,8 @@
 //   2. (no compaction: ungridded records are a few percent, and every later
 //      pass reaches records through tile pairs or skips support 0);
 //   3. counting sort of (record, tile) pairs: every record is listed under each
-//      32x32-cell tile its support box touches (wave-aggregated counters, one
-//      scan, scatter); a tile's list is cut into work units of <= unit_pairs
+//      32x32-cell tile its support box touches (wave-aggregated counters in the
+//      binning pass, one scan, scatter); a tile's list is cut into work units of <= unit_pairs
 //      records.  Tile keys are numbered job-major, so the units of all jobs of
 //      a pass form one list and one persistent launch grids them all;
 //   4. uniform/Briggs: one block per tile key gathers the weights of the
@@ -45,7 +45,6 @@
 
 namespace mti {
 
-constexpr int TG_T = 32;             // tile side in cells
 constexpr int TG_TT = TG_T * TG_T;
 constexpr int TG_THREADS = 1024;     // 16 waves, one 8x8 region each
 constexpr int TG_REGIONS = 16;
@@ -129,49 +128,13 @@ __device__ __forceinline__ TileAt tile_at(const JobSet &js, int key)
 
 // ---- tile lists ---------------------------------------------------------------
 
-struct TileRange {
-    int tx0, ty0, ntx, nty;  // tiles touched by a support box
-};
-
-__device__ __forceinline__ TileRange tile_range(int4 d)
-{
-    const int cx = d.y & 0xffff, cy = (int)((unsigned)d.y >> 16), S = d.w & 0x7fff;
-    TileRange t;
-    t.tx0 = cx / TG_T;
-    t.ty0 = cy / TG_T;
-    t.ntx = S > 0 ? (cx + S - 1) / TG_T - t.tx0 + 1 : 0;  // a plane without samples grids nothing
-    t.nty = S > 0 ? (cy + S - 1) / TG_T - t.ty0 + 1 : 0;
-    return t;
-}
-
-// Runs of equal keys in consecutive lanes: head lanes and their run lengths.
-struct WaveRuns {
-    bool head;
-    int len;   // valid on head lanes
-    int hl;    // head lane of this lane's run
-};
-
-__device__ __forceinline__ WaveRuns wave_runs(int key)
-{
-    const int lane = threadIdx.x & 63;
-    const int prev = __shfl_up(key, 1, 64);
-    WaveRuns r;
-    r.head = (lane == 0) || (prev != key);
-    const uint64_t heads = __ballot(r.head);
-    const uint64_t above = (lane == 63) ? 0ull : (heads & ~((2ull << lane) - 1ull));
-    const int end = above ? (__ffsll((unsigned long long)above) - 1) : 64;
-    r.len = end - lane;
-    const uint64_t upto = (lane == 63) ? heads : (heads & ((2ull << lane) - 1ull));
-    r.hl = 63 - __clzll((long long)upto);
-    return r;
-}
-
 // A pair is the record index; bit 31 marks the record's home tile, the one
 // holding its nearest cell (box centre), which the weight density uses.
 constexpr uint32_t PAIR_HOME = 0x80000000u, PAIR_INDEX = 0x7fffffffu;
 
-// Pass 0: count the pairs of every tile.  Pass 1: scatter record indices.
-template <int PASS>
+// Scatters the record indices into the tiles' pair lists (the binning pass
+// counted the pairs of every tile, and start is the exclusive scan of the
+// counts; cnt counts again from zero to place each pair).
 __global__ __launch_bounds__(256) void k_tile_pairs(GridArgs a, int64_t cap, uint32_t *cnt, const uint32_t *start,
                                                     uint32_t *pairs)
 {
@@ -200,15 +163,11 @@ __global__ __launch_bounds__(256) void k_tile_pairs(GridArgs a, int64_t cap, uin
             key = base + (t.ty0 + dy) * ntx + t.tx0 + dx;
         }
         const WaveRuns r = wave_runs(key);
-        if (PASS == 0) {
-            if (r.head && key >= 0) atomicAdd(&cnt[key], (uint32_t)r.len);
-        } else {
-            uint32_t b = 0;
-            if (r.head && key >= 0) b = atomicAdd(&cnt[key], (uint32_t)r.len);
-            b = __shfl(b, r.hl, 64);
-            if (key >= 0)
-                pairs[start[key] + b + (uint32_t)((threadIdx.x & 63) - r.hl)] = (uint32_t)j | (q == home ? PAIR_HOME : 0u);
-        }
+        uint32_t b = 0;
+        if (r.head && key >= 0) b = atomicAdd(&cnt[key], (uint32_t)r.len);
+        b = __shfl(b, r.hl, 64);
+        if (key >= 0)
+            pairs[start[key] + b + (uint32_t)((threadIdx.x & 63) - r.hl)] = (uint32_t)j | (q == home ? PAIR_HOME : 0u);
     }
 }
 
@@ -1106,8 +1065,10 @@ static mti_status grid_pass(mti_ctx *ctx, const std::vector<PassJob> &jobs, cons
     ba.wv = cwv;
     ba.wt = cwt;
     ba.stats = dstats;
+    ba.tile_cnt = start;
     {
         StageTimer tm(ctx, MTI_STAGE_BIN, st);
+        hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, st, start, nkeys, 0u);
         r = launch_bin_pass(ctx, ba, blk0[nj], st);
         if (r != MTI_OK) return r;
     }
@@ -1133,15 +1094,12 @@ static mti_status grid_pass(mti_ctx *ctx, const std::vector<PassJob> &jobs, cons
         // every record stays in place (job, channel, baseline order); records
         // that are not gridded have support 0 and produce no pairs
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, M_d, (int64_t)1, (uint32_t)nrec);
-        // counting sort of (record, tile) pairs by tile
-        hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, st, start, nkeys, 0u);
+        // counting sort of (record, tile) pairs by tile (the binning pass
+        // counted them into start)
         hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, st, fill, nkeys, 0u);
-        hipLaunchKernelGGL(k_tile_pairs<0>, dim3(gblocks), dim3(256), 0, st, ga, nrec, start,
-                           (const uint32_t *)nullptr, (uint32_t *)nullptr);
-        MTI_HIP(ctx, hipGetLastError());
         r = exclusive_scan_u32(ctx, start, nkeys, NP_d, st);
         if (r != MTI_OK) return r;
-        hipLaunchKernelGGL(k_tile_pairs<1>, dim3(gblocks), dim3(256), 0, st, ga, nrec, fill, (const uint32_t *)start,
+        hipLaunchKernelGGL(k_tile_pairs, dim3(gblocks), dim3(256), 0, st, ga, nrec, fill, (const uint32_t *)start,
                            pairs);
         MTI_HIP(ctx, hipGetLastError());
     }

@@ -84,7 +84,8 @@ __device__ __forceinline__ int job_of(const int64_t *prefix, int n, int64_t x)
 }
 
 // Record descriptor written by the binning pass (16 bytes):
-//   x = row, y = corner_y << 16 | corner_x, z = conv_idx,
+//   x = sub-kernel (plane T^2 + oy T + ox: the sums index),
+//   y = corner_y << 16 | corner_x, z = conv_idx,
 //   w = plane << 16 | conj << 15 | support
 // Every (job, channel, row) gets a record; one that is not gridded (all
 // correlations flagged, or its support box off the grid) has support 0 and
@@ -98,9 +99,72 @@ struct BinPassArgs {
     float2 *wv;                 // [nrec][npol] visibilities of gridded records
     float *wt;                  // [nrec][npol] weights, 0 where flagged
     unsigned long long *stats;  // [2][BIN_STAT_STRIPES]: gridded records, gridded points
+    uint32_t *tile_cnt;         // [keys] (record, tile) pairs per tile key, added to (zeroed by the caller)
 };
 
 mti_status launch_bin_pass(mti_ctx *ctx, const BinPassArgs &a, int64_t nblocks, hipStream_t st);
+
+// ---- tiles of the gridder ------------------------------------------------------
+
+constexpr int TG_T = 32;  // tile side in cells
+
+struct TileRange {
+    int tx0, ty0, ntx, nty;  // tiles touched by a support box
+};
+
+__device__ __forceinline__ TileRange tile_range(int4 d)
+{
+    const int cx = d.y & 0xffff, cy = (int)((unsigned)d.y >> 16), S = d.w & 0x7fff;
+    TileRange t;
+    t.tx0 = cx / TG_T;
+    t.ty0 = cy / TG_T;
+    t.ntx = S > 0 ? (cx + S - 1) / TG_T - t.tx0 + 1 : 0;  // a plane without samples grids nothing
+    t.nty = S > 0 ? (cy + S - 1) / TG_T - t.ty0 + 1 : 0;
+    return t;
+}
+
+// Runs of equal keys in consecutive lanes: head lanes and their run lengths.
+struct WaveRuns {
+    bool head;
+    int len;   // valid on head lanes
+    int hl;    // head lane of this lane's run
+};
+
+__device__ __forceinline__ WaveRuns wave_runs(int key)
+{
+    const int lane = threadIdx.x & 63;
+    const int prev = __shfl_up(key, 1, 64);
+    WaveRuns r;
+    r.head = (lane == 0) || (prev != key);
+    const uint64_t heads = __ballot(r.head);
+    const uint64_t above = (lane == 63) ? 0ull : (heads & ~((2ull << lane) - 1ull));
+    const int end = above ? (__ffsll((unsigned long long)above) - 1) : 64;
+    r.len = end - lane;
+    const uint64_t upto = (lane == 63) ? heads : (heads & ((2ull << lane) - 1ull));
+    r.hl = 63 - __clzll((long long)upto);
+    return r;
+}
+
+// Adds the tiles a wave's records touch to the pair counts of their tile
+// keys (base: key of the records' channel grid, ntx: tiles per row); records
+// of support 0 touch none.  Every lane of the wave takes part: consecutive
+// lanes hold consecutive records, so runs of equal keys share one atomic.
+__device__ __forceinline__ void count_tile_pairs(int4 d, int base, int ntx, uint32_t *cnt)
+{
+    const TileRange t = tile_range(d);
+    const int nq = t.ntx * t.nty;
+    int qmax = nq;
+    for (int o = 32; o >= 1; o >>= 1) qmax = max(qmax, __shfl_xor(qmax, o, 64));
+    for (int q = 0; q < qmax; q++) {
+        int key = -1;
+        if (q < nq) {
+            const int dy = q / t.ntx, dx = q - dy * t.ntx;
+            key = base + (t.ty0 + dy) * ntx + t.tx0 + dx;
+        }
+        const WaveRuns r = wave_runs(key);
+        if (r.head && key >= 0) atomicAdd(&cnt[key], (uint32_t)r.len);
+    }
+}
 int64_t bin_blocks(int64_t nidx, int nb);
 VisView make_view(const mti_vis *v);
 BinGeom make_geom(const mti_kernels *k, const mti_image_params *p);
