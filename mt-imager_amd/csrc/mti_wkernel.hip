@@ -246,19 +246,33 @@ __global__ void k_sums(const void *table, const int2 *convinfo, int nplanes, int
     sums[id] = (float)acc;
 }
 
-// The two windowed-DFT stages as tiled complex FP64 matrix products, all
-// planes of a batch in one launch (blockIdx.z = plane).  E[c][j] =
-// tw[((c - h) q_j) mod L] is generated while a tile is staged.
+// The two windowed-DFT stages as tiled complex FP64 matrix products on the
+// FP64 matrix cores, all planes of a batch in one launch (blockIdx.z =
+// plane).  E[c][j] = tw[((c - h) q_j) mod L] is generated while a tile is
+// staged.
 //   stage 1: A[b][iy][j] = sum_cx scr[b][iy][cx] E[cx][j]            (St x Wk)
 //   stage 2: T[b][jy][jx] = sum_iy E[iy][jy] A[b][iy][jx]            (Wk x Wk)
 // Plane b uses the centred window j < Wk(S_b) of the widest window Wmax
 // (offset (Wmax - Wk)/2), qmin = -(S/2) s - s/2.
-constexpr int GT = 32;   // output tile
-constexpr int GK = 16;   // k chunk
+//
+// A block of 4 waves computes a GT x GT output tile, wave (wy, wx) its
+// 32 x 32 quarter as 2 x 2 blocks of 16 x 16, with v_mfma_f64_16x16x4f64:
+// operand a of lane l is X[l % 16][l / 16], operand b is Y[l / 16][l % 16],
+// and result register r of lane l is C[4 r + l / 16][l % 16].  A complex
+// product takes four real ones: Cr += Xr Yr - Xi Yi, Ci += Xr Yi + Xi Yr.
+// The X and Y tiles of a k chunk are staged in LDS as complex FP64, so each
+// lane reads one complex element per operand and k step.
+constexpr int GT = 64;   // output tile
+constexpr int GK = 16;   // k chunk (4 matrix steps of 4)
 
-__device__ __forceinline__ double2 tw_at(const double2 *tw, int c, int h, int q, int L)
+typedef double mti_f64x4 __attribute__((ext_vector_type(4)));
+
+// Twiddle index ((c - h) q) mod L advanced by c += step: idx + (step q mod L),
+// wrapped once (both terms below L).
+__device__ __forceinline__ int tw_step(int idx, int inc, int L)
 {
-    return tw[mod_pos((long long)(c - h) * q, L)];
+    idx += inc;
+    return idx >= L ? idx - L : idx;
 }
 
 template <int STAGE>
@@ -275,44 +289,83 @@ __global__ __launch_bounds__(256) void k_wdft(const double2 *src, ScreenGeom g, 
     const int M = STAGE == 1 ? g.St : Wk, N = Wk, K = g.St;
     const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
     if (m0 >= M || n0 >= N) return;
-    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wy = (wave >> 1) * 32, wx = (wave & 1) * 32;  // the wave's quarter of the tile
+    const int li = lane & 15, lk = lane >> 4;
     const double2 *Sb = src + (size_t)b * g.St * (STAGE == 1 ? g.St : Wmax);
-    double2 acc[2][2];
+    // A thread stages elements t = 0..3 of each k chunk: X[(tid >> 4) + 16 t][tid & 15]
+    // and Y[(tid >> 6) + 4 t][tid & 63].  The generated operand (Y in stage 1,
+    // X in stage 2) comes from the twiddle table through running indices
+    // (one wrapped add per element instead of a modulo).
+    const int xm = tid >> 4, xk = tid & 15, yk = tid >> 6, yn = tid & 63;
+    // stage 1: ti[0] (element 0), toff[t] = 4 t q mod L, tinc[0] = 16 q mod L;
+    // stage 2: ti[t] and tinc[t] = 16 q_t mod L per element t
+    int ti[4], tinc[4], toff[4];
+    if (STAGE == 1) {
+        const int q = qmin + n0 + yn;
+        ti[0] = mod_pos((long long)(yk - g.h) * q, g.L);
+#pragma unroll
+        for (int t = 0; t < 4; t++) toff[t] = mod_pos(4LL * t * q, g.L);
+        tinc[0] = mod_pos(16LL * q, g.L);
+    } else {
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int q = qmin + m0 + xm + 16 * t;
+            ti[t] = mod_pos((long long)(xk - g.h) * q, g.L);
+            tinc[t] = mod_pos(16LL * q, g.L);
+        }
+    }
+    double2 xv[4], yv[4];
+    auto fetch = [&](int k0c) {
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int m = m0 + xm + 16 * t, kx = k0c + xk;
+            const int ky = k0c + yk + 4 * t, n = n0 + yn;
+            if (STAGE == 1) {
+                xv[t] = m < M && kx < K ? Sb[(size_t)m * g.St + kx] : make_double2(0.0, 0.0);
+                yv[t] = n < N && ky < K ? tw[tw_step(ti[0], toff[t], g.L)] : make_double2(0.0, 0.0);
+            } else {
+                xv[t] = m < M && kx < K ? tw[ti[t]] : make_double2(0.0, 0.0);
+                yv[t] = n < N && ky < K ? Sb[(size_t)ky * Wmax + n] : make_double2(0.0, 0.0);
+            }
+        }
+        // the next chunk is 16 further along k
+        if (STAGE == 1) {
+            ti[0] = tw_step(ti[0], tinc[0], g.L);
+        } else {
+#pragma unroll
+            for (int t = 0; t < 4; t++) ti[t] = tw_step(ti[t], tinc[t], g.L);
+        }
+    };
+    mti_f64x4 cr[2][2], ci[2][2];
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++) acc[i][j] = make_double2(0.0, 0.0);
+        for (int j = 0; j < 2; j++) cr[i][j] = ci[i][j] = mti_f64x4{0.0, 0.0, 0.0, 0.0};
+    fetch(0);
     for (int k0c = 0; k0c < K; k0c += GK) {
-        // X tile [GT][GK]: stage 1 screen rows; stage 2 E[k][m] (transposed)
-        for (int e = tid; e < GT * GK; e += 256) {
-            const int mi = e / GK, ki = e - mi * GK;
-            const int m = m0 + mi, k = k0c + ki;
-            double2 v = make_double2(0.0, 0.0);
-            if (m < M && k < K) v = STAGE == 1 ? Sb[(size_t)m * g.St + k] : tw_at(tw, k, g.h, qmin + m, g.L);
-            Xs[mi][ki] = v;
-        }
-        // Y tile [GK][GT]: stage 1 E[k][n]; stage 2 A[k][n]
-        for (int e = tid; e < GK * GT; e += 256) {
-            const int ki = e / GT, ni = e - ki * GT;
-            const int k = k0c + ki, n = n0 + ni;
-            double2 v = make_double2(0.0, 0.0);
-            if (n < N && k < K) v = STAGE == 1 ? tw_at(tw, k, g.h, qmin + n, g.L) : Sb[(size_t)k * Wmax + n];
-            Ys[ki][ni] = v;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            Xs[xm + 16 * t][xk] = xv[t];
+            Ys[yk + 4 * t][yn] = yv[t];
         }
         __syncthreads();
+        if (k0c + GK < K) fetch(k0c + GK);  // in flight during the products
 #pragma unroll
-        for (int ki = 0; ki < GK; ki++) {
+        for (int ks = 0; ks < GK; ks += 4) {
             double2 x[2], y[2];
 #pragma unroll
-            for (int i = 0; i < 2; i++) x[i] = Xs[ty * 2 + i][ki];
+            for (int i = 0; i < 2; i++) x[i] = Xs[wy + 16 * i + li][ks + lk];
 #pragma unroll
-            for (int j = 0; j < 2; j++) y[j] = Ys[ki][tx * 2 + j];
+            for (int j = 0; j < 2; j++) y[j] = Ys[ks + lk][wx + 16 * j + li];
 #pragma unroll
             for (int i = 0; i < 2; i++)
 #pragma unroll
                 for (int j = 0; j < 2; j++) {
-                    acc[i][j].x = fma(x[i].x, y[j].x, fma(-x[i].y, y[j].y, acc[i][j].x));
-                    acc[i][j].y = fma(x[i].x, y[j].y, fma(x[i].y, y[j].x, acc[i][j].y));
+                    cr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[i].x, y[j].x, cr[i][j], 0, 0, 0);
+                    cr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-x[i].y, y[j].y, cr[i][j], 0, 0, 0);
+                    ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[i].x, y[j].y, ci[i][j], 0, 0, 0);
+                    ci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[i].y, y[j].x, ci[i][j], 0, 0, 0);
                 }
         }
         __syncthreads();
@@ -320,10 +373,14 @@ __global__ __launch_bounds__(256) void k_wdft(const double2 *src, ScreenGeom g, 
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++) {
-            const int m = m0 + ty * 2 + i, n = n0 + tx * 2 + j;
-            if (m < M && n < N) dst[((size_t)b * (STAGE == 1 ? g.St : Wmax) + m) * Wmax + n] = acc[i][j];
-        }
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int m = m0 + wy + 16 * i + 4 * r + lk, n = n0 + wx + 16 * j + li;
+                if (m < M && n < N)
+                    dst[((size_t)b * (STAGE == 1 ? g.St : Wmax) + m) * Wmax + n] =
+                        make_double2(cr[i][j][r], ci[i][j][r]);
+            }
 }
 
 // Stage-2 products (T[jy][jx], jy, jx < Wk) scaled by 1/divisor into the
