@@ -257,8 +257,11 @@ __global__ __launch_bounds__(256) void k_density_keys(GridArgs a, const uint32_t
         // and the records' nearest cells in the tile (-1: not the record's home
         // tile, which alone gathers it), read in phases -- the pairs, then the
         // home records' descriptors -- so each phase's DK reads are in flight
-        // together (crowded inner-uv tiles hold thousands of pairs)
-        constexpr int DK = 4;
+        // together.  The pass is bound by the latency of these phases; one
+        // pair a thread keeps the kernel at 8 waves a SIMD, and measured
+        // faster than 2 (7 waves), 3 or 4 a thread (full-polarisation cube:
+        // plan stage 21.7-22.6 ms against 22.7-23.1, 23.6-23.7 and 24.3-24.6).
+        constexpr int DK = 1;
         auto walk = [&](uint32_t i0, uint32_t (&kj)[DK], int (&kc)[DK]) {
             int4 kd[DK];
 #pragma unroll
