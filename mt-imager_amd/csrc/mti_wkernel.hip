@@ -130,58 +130,66 @@ __global__ __launch_bounds__(PJ_THREADS) void k_proj(const double2 *scr, int St,
     if (t < PJ_T && y0 + t < St) colp[y0 + t] = c;
 }
 
-// One block per plane: support find on the centre row / column (the DFTs of
-// the projections k_proj wrote).
+// Support find on the centre row / column (the DFTs of the projections
+// k_proj wrote): the support comes from the largest q whose row or column
+// sample exceeds the threshold, so every q is tried at once -- thread
+// (blockIdx.x, threadIdx.x) of plane blockIdx.y takes q = qmax - 256 x - t
+// -- and found[k0 + plane] keeps the largest hit (filled with -1 before).
+// Each sum runs in ascending c as in the oracle, so the threshold sees the
+// same values.
 constexpr int SUP_THREADS = 256;
 __global__ __launch_bounds__(SUP_THREADS) void k_support(ScreenGeom g, const double2 *tw, double inv_div,
-                                                         int k0, int *support_out,
+                                                         int k0, int *found,
                                                          const double2 *proj /* [nb][2][St] */)
 {
-    const int b = blockIdx.x;
+    const int b = blockIdx.y;
+    const int qmax = g.L - 1 - g.L / 2;
+    const int q = qmax - (int)blockIdx.x * SUP_THREADS - (int)threadIdx.x;
+    if (q < 0) return;
     const double2 *rowp = proj + (size_t)b * 2 * g.St;
     const double2 *colp = rowp + g.St;
-    __shared__ int found;
-    if (threadIdx.x == 0) found = -1;
-    __syncthreads();
-    const int qmax = g.L - 1 - g.L / 2;
-    // Uniform trip count for every thread (the barriers below are block-wide).
-    for (int qb = qmax; qb >= 0; qb -= (int)blockDim.x) {
-        const int q = qb - (int)threadIdx.x;
-        for (int pass = 0; pass < 2 && q >= 0; pass++) {
-            const double2 *pr = pass ? colp : rowp;
-            int m = mod_pos(-(long long)g.h * q, g.L);
-            double re = 0, im = 0;
-            // unrolled so the twiddle gathers go out ahead of the sums; the
-            // sums stay one chain in ascending c (the oracle's order)
+    for (int pass = 0; pass < 2; pass++) {
+        const double2 *pr = pass ? colp : rowp;
+        int m = mod_pos(-(long long)g.h * q, g.L);
+        double re = 0, im = 0;
+        // unrolled so the twiddle gathers go out ahead of the sums; the
+        // sums stay one chain in ascending c (the oracle's order)
 #pragma unroll 8
-            for (int c = 0; c < g.St; c++) {
-                double2 t = tw[m];
-                double2 v = pr[c];
-                re += v.x * t.x - v.y * t.y;
-                im += v.x * t.y + v.y * t.x;
-                m += q;
-                if (m >= g.L) m -= g.L;
-            }
-            re *= inv_div;
-            im *= inv_div;
-            if (re * re + im * im > 1e-6) {
-                atomicMax(&found, q);
-                break;
-            }
+        for (int c = 0; c < g.St; c++) {
+            double2 t = tw[m];
+            double2 v = pr[c];
+            re += v.x * t.x - v.y * t.y;
+            im += v.x * t.y + v.y * t.x;
+            m += q;
+            if (m >= g.L) m -= g.L;
         }
-        __syncthreads();
-        if (found >= 0) break;
-        __syncthreads();
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int S_ = 0;
-        if (found >= 0) {
-            S_ = ((found + (g.s + 1) / 2) / g.s) * 2 + 1;
-            if (S_ > g.St - 2) S_ = g.St - 2;
+        re *= inv_div;
+        im *= inv_div;
+        if (re * re + im * im > 1e-6) {
+            atomicMax(&found[k0 + b], q);
+            return;
         }
-        support_out[k0 + b] = S_;
     }
+}
+
+// found[k] = -1 for the planes of a build, before the support search.
+__global__ void k_support_init(int *found, int W)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < W) found[k] = -1;
+}
+
+// Support of each plane from its largest hit (odd, at most St - 2; 0: none).
+__global__ void k_support_size(ScreenGeom g, const int *found, int W, int *support_out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= W) return;
+    int S_ = 0;
+    if (found[k] >= 0) {
+        S_ = ((found[k] + (g.s + 1) / 2) / g.s) * 2 + 1;
+        if (S_ > g.St - 2) S_ = g.St - 2;
+    }
+    support_out[k] = S_;
 }
 
 template <int PREC>
@@ -501,10 +509,9 @@ static mti_status build_impl(mti_ctx *ctx, const mti_image_params *p, int precis
         const size_t tw_bytes = (size_t)g.L * sizeof(double2);
         const size_t proj_bytes = (size_t)nb * 2 * St * sizeof(double2);
         void *w0 = nullptr;
+        const size_t sup_bytes = ((size_t)W * sizeof(int) + 255) / 256 * 256;
         mti_status r = scratch_get(ctx, SLOT_SORTED,
-                                   tw_bytes + (size_t)nb * plane_bytes + proj_bytes +
-                                       (size_t)W * sizeof(int) + 256,
-                                   &w0);
+                                   tw_bytes + (size_t)nb * plane_bytes + proj_bytes + 2 * sup_bytes + 512, &w0);
         if (r != MTI_OK) return r;
         char *wp = (char *)w0;
         double2 *d_tw = (double2 *)wp;
@@ -513,10 +520,14 @@ static mti_status build_impl(mti_ctx *ctx, const mti_image_params *p, int precis
         wp += (size_t)nb * plane_bytes;
         double2 *d_proj = (double2 *)wp;
         wp += proj_bytes;
+        wp = (char *)w0 + (((size_t)(wp - (char *)w0) + 255) / 256 * 256);
         int *d_sup = (int *)wp;
+        int *d_found = (int *)(wp + sup_bytes);
 
         hipLaunchKernelGGL(k_twiddle, dim3((unsigned)ceil_div(g.L, 256)), dim3(256), 0, st, d_tw, g.L);
+        hipLaunchKernelGGL(k_support_init, dim3((unsigned)ceil_div(W, 256)), dim3(256), 0, st, d_found, W);
         MTI_HIP(ctx, hipGetLastError());
+        const int qchunks = ceil_div(g.L - g.L / 2, SUP_THREADS);  // q = 0 .. L - 1 - L/2
         dim3 sb(16, 16);
         for (int k0 = 0; k0 < W; k0 += nb) {
             int n = std::min(nb, W - k0);
@@ -526,10 +537,12 @@ static mti_status build_impl(mti_ctx *ctx, const mti_image_params *p, int precis
             hipLaunchKernelGGL(k_proj, dim3((unsigned)n, (unsigned)(pj_row_blocks(St) + pj_col_blocks(St))),
                                dim3(PJ_THREADS), 0, st, d_scr, St, d_proj);
             MTI_HIP(ctx, hipGetLastError());
-            hipLaunchKernelGGL(k_support, dim3((unsigned)n), dim3(SUP_THREADS), 0, st, g, d_tw, inv, k0, d_sup,
-                               d_proj);
+            hipLaunchKernelGGL(k_support, dim3((unsigned)qchunks, (unsigned)n), dim3(SUP_THREADS), 0, st, g, d_tw,
+                               inv, k0, d_found, d_proj);
             MTI_HIP(ctx, hipGetLastError());
         }
+        hipLaunchKernelGGL(k_support_size, dim3((unsigned)ceil_div(W, 256)), dim3(256), 0, st, g, d_found, W, d_sup);
+        MTI_HIP(ctx, hipGetLastError());
         K->support.assign(W, 0);
         MTI_HIP(ctx, hipMemcpyAsync(K->support.data(), d_sup, W * sizeof(int),
                                     hipMemcpyDeviceToHost, st));
