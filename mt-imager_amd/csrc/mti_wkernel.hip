@@ -503,9 +503,11 @@ static mti_status build_impl(mti_ctx *ctx, const mti_image_params *p, int precis
         K->info.divisor = div;
         const double inv = 1.0 / div;
 
-        // planes per batch bounded by ~512 MB of FP64 screens
+        // planes per batch bounded by ~2 GB of FP64 screens; when one batch
+        // holds every plane, the DFT stage reuses the screens of the support
+        // search instead of computing them again
         const size_t plane_bytes = (size_t)St * St * sizeof(double2);
-        int nb = (int)std::max<size_t>(1, std::min<size_t>((size_t)W, (512u << 20) / plane_bytes));
+        int nb = (int)std::max<size_t>(1, std::min<size_t>((size_t)W, (size_t(2) << 30) / plane_bytes));
         const size_t tw_bytes = (size_t)g.L * sizeof(double2);
         const size_t proj_bytes = (size_t)nb * 2 * St * sizeof(double2);
         void *w0 = nullptr;
@@ -577,8 +579,10 @@ static mti_status build_impl(mti_ctx *ctx, const mti_image_params *p, int precis
         MTI_HIP(ctx, hipMemcpyAsync(d_base, K->base.data(), (size_t)W * sizeof(int64_t), hipMemcpyHostToDevice, st));
         for (int k0 = 0; k0 < W; k0 += nb) {
             int n = std::min(nb, W - k0);
-            dim3 sg((unsigned)ceil_div(St, 16), (unsigned)ceil_div(St, 16), (unsigned)n);
-            hipLaunchKernelGGL(k_screen, sg, sb, 0, st, d_scr, g, K->info.w_increment, k0);
+            if (nb < W) {
+                dim3 sg((unsigned)ceil_div(St, 16), (unsigned)ceil_div(St, 16), (unsigned)n);
+                hipLaunchKernelGGL(k_screen, sg, sb, 0, st, d_scr, g, K->info.w_increment, k0);
+            }
             dim3 g1((unsigned)ceil_div(Wmax, GT), (unsigned)ceil_div(St, GT), (unsigned)n);
             hipLaunchKernelGGL(k_wdft<1>, g1, dim3(256), 0, st, d_scr, g, d_tw, d_sup, k0, Wmax, T, d_A);
             dim3 g2((unsigned)ceil_div(Wmax, GT), (unsigned)ceil_div(Wmax, GT), (unsigned)n);
