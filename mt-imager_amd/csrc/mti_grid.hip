@@ -209,7 +209,8 @@ __global__ void k_unit_list(const uint32_t *start, const uint32_t *total, const 
 // stored.  acc: [global channel][P][2] FP64 sums; jc0: channel prefix per job.
 template <int P>
 __global__ __launch_bounds__(256) void k_density_keys(GridArgs a, const uint32_t *start, const uint32_t *np_total,
-                                                     int64_t nkeys, const int64_t *jc0, double *acc)
+                                                     int64_t nkeys, const int64_t *jc0, double *acc,
+                                                     uint32_t *queue)
 {
     __shared__ float s_d[TG_TT * P];
     __shared__ double s_r[4][2];
@@ -241,111 +242,126 @@ __global__ __launch_bounds__(256) void k_density_keys(GridArgs a, const uint32_t
         }
         s1 = s2 = 0.0;
     };
-    for (int64_t k = blockIdx.x; k < nkeys; k += gridDim.x) {
-        const uint32_t p0 = start[k], p1 = (k + 1 < nkeys) ? start[k + 1] : NP;
-        if (p0 == p1) continue;  // empty tile (uniform in the block)
-        const TileAt ta = tile_at(a.js, (int)k);
-        const int ch = (int)jc0[ta.job] + ta.c_local;
-        if (ch != ch_cur) {
-            if (ch_cur >= 0) flush();
-            ch_cur = ch;
-        }
-        const int x0 = ta.tx * TG_T, y0 = ta.ty * TG_T;
-        for (int i = tid; i < P * TG_TT; i += 256) s_d[i] = 0.0f;
+    // Blocks take DQ consecutive keys at a time from one atomic counter (zero
+    // before the launch): the crowded inner-uv tiles of every channel hold
+    // most of the pairs, and a fixed stride over the keys left the blocks
+    // whose keys fall on them working long after the others.
+    constexpr uint32_t DQ = 16;
+    __shared__ uint32_t s_next;
+    for (;;) {
+        __syncthreads();  // every thread has read the previous s_next
+        if (tid == 0) s_next = atomicAdd(queue, DQ);
         __syncthreads();
-        // DK pairs a thread at a time (pairs i0 + tid + 256 k): their records
-        // and the records' nearest cells in the tile (-1: not the record's home
-        // tile, which alone gathers it), read in phases -- the pairs, then the
-        // home records' descriptors -- so each phase's DK reads are in flight
-        // together.  The pass is bound by the latency of these phases; one
-        // pair a thread keeps the kernel at 8 waves a SIMD, and measured
-        // faster than 2 (7 waves), 3 or 4 a thread (full-polarisation cube:
-        // plan stage 21.7-22.6 ms against 22.7-23.1, 23.6-23.7 and 24.3-24.6).
-        constexpr int DK = 1;
-        auto walk = [&](uint32_t i0, uint32_t (&kj)[DK], int (&kc)[DK]) {
-            int4 kd[DK];
-#pragma unroll
-            for (int k = 0; k < DK; k++) {
-                const uint32_t i = i0 + tid + 256u * k;
-                const uint32_t pj = i < p1 ? a.pairs[i] : 0u;  // past the tile: no home bit
-                kj[k] = pj & PAIR_INDEX;
-                kc[k] = (pj & PAIR_HOME) ? 0 : -1;
+        const int64_t kq = s_next;
+        if (kq >= nkeys) break;
+        const int64_t kend = min(kq + (int64_t)DQ, nkeys);
+        for (int64_t k = kq; k < kend; k++) {
+            const uint32_t p0 = start[k], p1 = (k + 1 < nkeys) ? start[k + 1] : NP;
+            if (p0 == p1) continue;  // empty tile (uniform in the block)
+            const TileAt ta = tile_at(a.js, (int)k);
+            const int ch = (int)jc0[ta.job] + ta.c_local;
+            if (ch != ch_cur) {
+                if (ch_cur >= 0) flush();
+                ch_cur = ch;
             }
+            const int x0 = ta.tx * TG_T, y0 = ta.ty * TG_T;
+            for (int i = tid; i < P * TG_TT; i += 256) s_d[i] = 0.0f;
+            __syncthreads();
+            // DK pairs a thread at a time (pairs i0 + tid + 256 k): their records
+            // and the records' nearest cells in the tile (-1: not the record's home
+            // tile, which alone gathers it), read in phases -- the pairs, then the
+            // home records' descriptors -- so each phase's DK reads are in flight
+            // together.  The pass is bound by the latency of these phases; one
+            // pair a thread keeps the kernel at 8 waves a SIMD, and measured
+            // faster than 2 (7 waves), 3 or 4 a thread (full-polarisation cube:
+            // plan stage 21.7-22.6 ms against 22.7-23.1, 23.6-23.7 and 24.3-24.6).
+            constexpr int DK = 1;
+            auto walk = [&](uint32_t i0, uint32_t (&kj)[DK], int (&kc)[DK]) {
+                int4 kd[DK];
 #pragma unroll
-            for (int k = 0; k < DK; k++) kd[k] = kc[k] >= 0 ? a.cdesc[kj[k]] : make_int4(0, 0, 0, 0);
-#pragma unroll
-            for (int k = 0; k < DK; k++)
-                if (kc[k] >= 0) {
-                    const int S = kd[k].w & 0x7fff;
-                    const int mx = (kd[k].y & 0xffff) + S / 2 - x0, my = (int)((unsigned)kd[k].y >> 16) + S / 2 - y0;
-                    kc[k] = my * TG_T + mx;
+                for (int k = 0; k < DK; k++) {
+                    const uint32_t i = i0 + tid + 256u * k;
+                    const uint32_t pj = i < p1 ? a.pairs[i] : 0u;  // past the tile: no home bit
+                    kj[k] = pj & PAIR_INDEX;
+                    kc[k] = (pj & PAIR_HOME) ? 0 : -1;
                 }
-        };
-        // consecutive pairs are consecutive time samples of a baseline and
-        // often share a cell (crowded inner-uv cells): a wave first sums each
-        // run of equal cells in its lanes (segmented shuffle sum down to the
-        // run's head), then issues one LDS atomic per run -- atomics to one
-        // address serialise
-        auto gather = [&](const uint32_t (&kj)[DK], const int (&kc)[DK]) {
-            float kw[DK][P];
 #pragma unroll
-            for (int k = 0; k < DK; k++)
+                for (int k = 0; k < DK; k++) kd[k] = kc[k] >= 0 ? a.cdesc[kj[k]] : make_int4(0, 0, 0, 0);
 #pragma unroll
-                for (int p = 0; p < P; p++) kw[k][p] = kc[k] >= 0 ? a.wt[(int64_t)kj[k] * P + p] : 0.0f;
-            const int lane = tid & 63;
+                for (int k = 0; k < DK; k++)
+                    if (kc[k] >= 0) {
+                        const int S = kd[k].w & 0x7fff;
+                        const int mx = (kd[k].y & 0xffff) + S / 2 - x0;
+                        const int my = (int)((unsigned)kd[k].y >> 16) + S / 2 - y0;
+                        kc[k] = my * TG_T + mx;
+                    }
+            };
+            // consecutive pairs are consecutive time samples of a baseline and
+            // often share a cell (crowded inner-uv cells): a wave first sums each
+            // run of equal cells in its lanes (segmented shuffle sum down to the
+            // run's head), then issues one LDS atomic per run -- atomics to one
+            // address serialise
+            auto gather = [&](const uint32_t (&kj)[DK], const int (&kc)[DK]) {
+                float kw[DK][P];
 #pragma unroll
-            for (int k = 0; k < DK; k++) {
-                const WaveRuns r = wave_runs(kc[k]);
-                if (__ballot(!r.head)) {  // some run is longer than one lane
-                    const int end = lane + r.len;
-                    for (int d = 1; d < 64; d <<= 1)
+                for (int k = 0; k < DK; k++)
 #pragma unroll
-                        for (int p = 0; p < P; p++) {
-                            const float o = __shfl_down(kw[k][p], d, 64);
-                            if (lane + d < end) kw[k][p] += o;
-                        }
+                    for (int p = 0; p < P; p++) kw[k][p] = kc[k] >= 0 ? a.wt[(int64_t)kj[k] * P + p] : 0.0f;
+                const int lane = tid & 63;
+#pragma unroll
+                for (int k = 0; k < DK; k++) {
+                    const WaveRuns r = wave_runs(kc[k]);
+                    if (__ballot(!r.head)) {  // some run is longer than one lane
+                        const int end = lane + r.len;
+                        for (int d = 1; d < 64; d <<= 1)
+#pragma unroll
+                            for (int p = 0; p < P; p++) {
+                                const float o = __shfl_down(kw[k][p], d, 64);
+                                if (lane + d < end) kw[k][p] += o;
+                            }
+                    }
+                    if (r.head && kc[k] >= 0) {
+#pragma unroll
+                        for (int p = 0; p < P; p++)
+                            if (kw[k][p] != 0.0f) atomicAdd(&s_d[kc[k] * P + p], kw[k][p]);
+                    }
                 }
-                if (r.head && kc[k] >= 0) {
+            };
+            auto hand_back = [&](const uint32_t (&kj)[DK], const int (&kc)[DK]) {
 #pragma unroll
-                    for (int p = 0; p < P; p++)
-                        if (kw[k][p] != 0.0f) atomicAdd(&s_d[kc[k] * P + p], kw[k][p]);
-                }
+                for (int k = 0; k < DK; k++)
+                    if (kc[k] >= 0) {
+#pragma unroll
+                        for (int p = 0; p < P; p++) a.rdens[(int64_t)kj[k] * P + p] = s_d[kc[k] * P + p];
+                    }
+            };
+            // the first chunk's (record, cell) stay in registers for the hand-back;
+            // later chunks are walked again
+            uint32_t cj[DK];
+            int cc[DK];
+            walk(p0, cj, cc);
+            gather(cj, cc);
+            for (uint32_t i0 = p0 + 256u * DK; i0 < p1; i0 += 256u * DK) {
+                uint32_t kj[DK];
+                int kc[DK];
+                walk(i0, kj, kc);
+                gather(kj, kc);
             }
-        };
-        auto hand_back = [&](const uint32_t (&kj)[DK], const int (&kc)[DK]) {
-#pragma unroll
-            for (int k = 0; k < DK; k++)
-                if (kc[k] >= 0) {
-#pragma unroll
-                    for (int p = 0; p < P; p++) a.rdens[(int64_t)kj[k] * P + p] = s_d[kc[k] * P + p];
-                }
-        };
-        // the first chunk's (record, cell) stay in registers for the hand-back;
-        // later chunks are walked again
-        uint32_t cj[DK];
-        int cc[DK];
-        walk(p0, cj, cc);
-        gather(cj, cc);
-        for (uint32_t i0 = p0 + 256u * DK; i0 < p1; i0 += 256u * DK) {
-            uint32_t kj[DK];
-            int kc[DK];
-            walk(i0, kj, kc);
-            gather(kj, kc);
+            __syncthreads();
+            for (int i = tid; i < TG_TT * P; i += 256) {
+                const double v = s_d[i];
+                s1 += v;
+                s2 += v * v;
+            }
+            hand_back(cj, cc);
+            for (uint32_t i0 = p0 + 256u * DK; i0 < p1; i0 += 256u * DK) {
+                uint32_t kj[DK];
+                int kc[DK];
+                walk(i0, kj, kc);
+                hand_back(kj, kc);
+            }
+            __syncthreads();  // s_d is cleared for the next tile
         }
-        __syncthreads();
-        for (int i = tid; i < TG_TT * P; i += 256) {
-            const double v = s_d[i];
-            s1 += v;
-            s2 += v * v;
-        }
-        hand_back(cj, cc);
-        for (uint32_t i0 = p0 + 256u * DK; i0 < p1; i0 += 256u * DK) {
-            uint32_t kj[DK];
-            int kc[DK];
-            walk(i0, kj, kc);
-            hand_back(kj, kc);
-        }
-        __syncthreads();  // s_d is cleared for the next tile
     }
     if (ch_cur >= 0) flush();
 }
@@ -844,11 +860,12 @@ static mti_status launch_prep(mti_ctx *ctx, const GridArgs &a, hipStream_t st)
 
 template <int P>
 static mti_status launch_density(mti_ctx *ctx, const GridArgs &a, const uint32_t *start, const uint32_t *np_total,
-                                 int64_t nkeys, const int64_t *jc0, double *acc, hipStream_t st)
+                                 int64_t nkeys, const int64_t *jc0, double *acc, uint32_t *queue, hipStream_t st)
 {
     auto kern = k_density_keys<P>;
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, queue, (int64_t)1, 0u);
     hipLaunchKernelGGL(kern, dim3(persistent_blocks(ctx, kern, 256)), dim3(256), 0, st, a, start, np_total, nkeys,
-                       jc0, acc);
+                       jc0, acc, queue);
     MTI_HIP(ctx, hipGetLastError());
     return MTI_OK;
 }
@@ -1019,7 +1036,7 @@ static mti_status grid_pass(mti_ctx *ctx, const std::vector<PassJob> &jobs, cons
     if ((r = scratch_get(ctx, SLOT_NORM, tab, &w7)) != MTI_OK) return r;
     int32_t *cjc = (int32_t *)w1;
     uint32_t *scal = (uint32_t *)((char *)w1 + u32n);
-    uint32_t *M_d = scal, *NP_d = scal + 64, *NU_d = scal + 128;
+    uint32_t *M_d = scal, *NP_d = scal + 64, *NU_d = scal + 128, *DQ_d = scal + 192;  // DQ_d: density queue
     int4 *cdesc = (int4 *)w2;
     uint32_t *start = (uint32_t *)w3;
     uint32_t *fill = (uint32_t *)((char *)w3 + u32k);
@@ -1126,9 +1143,9 @@ static mti_status grid_pass(mti_ctx *ctx, const std::vector<PassJob> &jobs, cons
             hipLaunchKernelGGL(k_fill_f64, dim3(64), dim3(256), 0, st, dacc, (int64_t)nch * P * 2, 0.0);
             MTI_HIP(ctx, hipGetLastError());
             switch (P) {
-            case 1: r = launch_density<1>(ctx, ga, start, NP_d, nkeys, d_jc0, dacc, st); break;
-            case 2: r = launch_density<2>(ctx, ga, start, NP_d, nkeys, d_jc0, dacc, st); break;
-            default: r = launch_density<4>(ctx, ga, start, NP_d, nkeys, d_jc0, dacc, st); break;
+            case 1: r = launch_density<1>(ctx, ga, start, NP_d, nkeys, d_jc0, dacc, DQ_d, st); break;
+            case 2: r = launch_density<2>(ctx, ga, start, NP_d, nkeys, d_jc0, dacc, DQ_d, st); break;
+            default: r = launch_density<4>(ctx, ga, start, NP_d, nkeys, d_jc0, dacc, DQ_d, st); break;
             }
             if (r != MTI_OK) return r;
             hipLaunchKernelGGL(k_briggs_f2, dim3((unsigned)ceil_div(nch * P, 64)), dim3(64), 0, st, js,
