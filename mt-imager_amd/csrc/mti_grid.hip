@@ -81,9 +81,11 @@ struct GridArgs {
     int fresh;                  // grids not pre-filled: a tile's only unit stores its cells,
                                 // every other tile was zeroed by k_zero_tiles
     int xcd_log2;               // gridder units dealt to each XCD in runs of 2^xcd_log2 (-1: round-robin)
+    uint32_t *unit_queue;       // gridder: MTI_XCDS counters, UQ_STRIDE words apart (zero before the launch)
 };
 
 constexpr uint32_t MTI_XCDS = 8;  // XCDs of an MI355X; workgroup b runs on XCD b % 8
+constexpr uint32_t UQ_STRIDE = 16;  // words between the gridder's unit-queue counters
 
 // Global-memory views of pointers read from the device-resident job table.
 // Such pointers are generic to the compiler, and loads through a generic
@@ -584,11 +586,19 @@ k_grid_tiles(GridArgs a)
     // and an XCD's workgroups walk its runs in order, so that XCD's L2 serves
     // the kernel samples the run has in common.  (xcd, i) -> u is a bijection:
     // every unit is gridded once.  lg < 0 (or a grid not a multiple of 8):
-    // round-robin.  All of it is workgroup-uniform (scalar registers).
+    // round-robin.  All of it is workgroup-uniform (scalar registers).  A
+    // workgroup takes its next i from a counter (its XCD's, or one for all),
+    // so one that drew light units takes more of them instead of idling.
     const int lg = (gridDim.x % MTI_XCDS) ? -1 : a.xcd_log2;
     const uint32_t xcd = blockIdx.x % MTI_XCDS;
-    const uint32_t step = lg >= 0 ? gridDim.x / MTI_XCDS : gridDim.x;
-    for (uint32_t i = lg >= 0 ? blockIdx.x / MTI_XCDS : blockIdx.x;; i += step) {
+    uint32_t *queue = a.unit_queue + (lg >= 0 ? xcd : 0u) * UQ_STRIDE;
+    __shared__ uint32_t s_i;
+    for (;;) {
+        // thread 0 writes s_i only after every thread has passed the previous
+        // unit's batch barriers, so after every thread has read it
+        if (tid == 0) s_i = atomicAdd(queue, 1u);
+        __syncthreads();
+        const uint32_t i = s_i;
         const uint32_t u = lg >= 0 ? ((((i >> lg) * MTI_XCDS + xcd) << lg) | (i & ((1u << lg) - 1u))) : i;
         if (u >= NU) break;
         const int4 un = a.units[u];
@@ -821,6 +831,7 @@ template <int P, int PREC, int ACC, int RW>
 static mti_status launch_tiles_rw(mti_ctx *ctx, const GridArgs &a, hipStream_t st)
 {
     auto kern = k_grid_tiles<P, PREC, ACC, RW>;
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, st, a.unit_queue, (int64_t)(MTI_XCDS * UQ_STRIDE), 0u);
     hipLaunchKernelGGL(kern, dim3(persistent_blocks(ctx, kern, TG_THREADS)), dim3(TG_THREADS), 0, st, a);
     MTI_HIP(ctx, hipGetLastError());
     return MTI_OK;
@@ -1037,6 +1048,7 @@ static mti_status grid_pass(mti_ctx *ctx, const std::vector<PassJob> &jobs, cons
     int32_t *cjc = (int32_t *)w1;
     uint32_t *scal = (uint32_t *)((char *)w1 + u32n);
     uint32_t *M_d = scal, *NP_d = scal + 64, *NU_d = scal + 128, *DQ_d = scal + 192;  // DQ_d: density queue
+    uint32_t *UQ_d = scal + 256;  // gridder unit queues [MTI_XCDS][UQ_STRIDE]
     int4 *cdesc = (int4 *)w2;
     uint32_t *start = (uint32_t *)w3;
     uint32_t *fill = (uint32_t *)((char *)w3 + u32k);
@@ -1104,6 +1116,7 @@ static mti_status grid_pass(mti_ctx *ctx, const std::vector<PassJob> &jobs, cons
     ga.rdens = rdens;
     ga.pairs = pairs;
     ga.units = units;
+    ga.unit_queue = UQ_d;
     ga.n_units = NU_d;
     ga.fresh = fresh ? 1 : 0;
     ga.xcd_log2 = -1;
