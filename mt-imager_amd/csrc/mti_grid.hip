@@ -396,36 +396,59 @@ __global__ __launch_bounds__(256) void k_record_prep(GridArgs a)
             np[p] = 0.0;
         }
     };
-    for (uint32_t j0 = blockIdx.x * 256u; j0 < M; j0 += gridDim.x * 256u) {
+    // The next block of records' descriptor, tag, weights, visibilities and
+    // density are read while this block's are weighted (two reads in flight a
+    // thread; the sums and Briggs factors depend on them and are read after).
+    // Reads of records that are not gridded (or past M) are in bounds and
+    // unused.
+    struct In {
+        int4 d;
+        int jc;
+        float w[P], dens[P];
+        float2 v[P];
+    };
+    const bool dense_w = a.rdens != nullptr;
+    auto fetch = [&](uint32_t j, In &x) {
+        x.d = j < M ? a.cdesc[j] : make_int4(0, 0, 0, 0);
+        x.jc = j < M ? a.cjc[j] : 0;
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            x.w[p] = j < M ? a.wt[(int64_t)j * P + p] : 0.0f;
+            x.v[p] = j < M ? a.wvis[(int64_t)j * P + p] : make_float2(0.0f, 0.0f);
+            x.dens[p] = j < M && dense_w ? a.rdens[(int64_t)j * P + p] : 0.0f;
+        }
+    };
+    const uint32_t stride = gridDim.x * 256u;
+    In cur;
+    fetch(blockIdx.x * 256u + threadIdx.x, cur);
+    for (uint32_t j0 = blockIdx.x * 256u; j0 < M; j0 += stride) {
         const uint32_t j = j0 + threadIdx.x;
+        In nxt;
+        fetch(j + stride, nxt);
         // records are ordered by (job, channel): partials are flushed when a
         // block starts on a new one
         const int jc_first = a.cjc[j0];
         if (jc_first != jc_prev && jc_prev >= 0) flush(jc_prev);
         jc_prev = jc_first;
-        const int4 d = j < M ? a.cdesc[j] : make_int4(0, 0, 0, 0);
+        const int4 d = cur.d;
         const int S = d.w & 0x7fff;
         // records that are not gridded (support 0) hold no visibilities or weights
         if (S > 0) {
-            const int jc = a.cjc[j];
+            const int jc = cur.jc;
             const JobDev &jd = a.js.jobs[jc >> 16];
             const int c_local = jc & 0xffff;
             const double csum = (double)jd.sums[d.x];
-            float dens[P];
-            if (jd.weighting != MTI_WEIGHT_NATURAL) {
-#pragma unroll
-                for (int p = 0; p < P; p++) dens[p] = a.rdens[(int64_t)j * P + p];
-            }
 #pragma unroll
             for (int p = 0; p < P; p++) {
-                float w = a.wt[(int64_t)j * P + p];
+                float w = cur.w[p];
+                const float dens = cur.dens[p];
                 if (jd.weighting == MTI_WEIGHT_UNIFORM)
-                    w = dens[p] > 0.0f ? w / dens[p] : 0.0f;
+                    w = dens > 0.0f ? w / dens : 0.0f;
                 else if (jd.weighting == MTI_WEIGHT_BRIGGS)
-                    w = (float)((double)w / (1.0 + jd.f2[c_local * P + p] * (double)dens[p]));
+                    w = (float)((double)w / (1.0 + jd.f2[c_local * P + p] * (double)dens));
                 // a zero weight (flagged) gives an exact zero, whatever the
                 // visibility holds (a flagged sample may be NaN)
-                const float2 vv = a.wvis[(int64_t)j * P + p];
+                const float2 vv = cur.v[p];
                 s_vis[threadIdx.x * P + p] = w != 0.0f ? make_float2(vv.x * w, vv.y * w) : make_float2(0.0f, 0.0f);
                 if (jc == jc_first) np[p] += csum * (double)w;
                 else unsafeAtomicAdd(&jd.norm[c_local * P + p], csum * (double)w);
@@ -442,6 +465,7 @@ __global__ __launch_bounds__(256) void k_record_prep(GridArgs a)
             else a.wvis[e0 + 2 * q] = s_vis[2 * q];
         }
         __syncthreads();
+        cur = nxt;
     }
     if (jc_prev >= 0) flush(jc_prev);
 }
