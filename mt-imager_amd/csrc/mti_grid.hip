@@ -247,7 +247,8 @@ __global__ __launch_bounds__(256) void k_density_keys(GridArgs a, const uint32_t
     // Blocks take DQ consecutive keys at a time from one atomic counter (zero
     // before the launch): the crowded inner-uv tiles of every channel hold
     // most of the pairs, and a fixed stride over the keys left the blocks
-    // whose keys fall on them working long after the others.
+    // whose keys fall on them working long after the others.  16 keys a grab
+    // measured faster than 4 (more atomics) and 64 (coarser balance).
     constexpr uint32_t DQ = 16;
     __shared__ uint32_t s_next;
     for (;;) {
